@@ -1,0 +1,275 @@
+"""Synthetic FEST-3D cases: grids, freestream, boundary setup and seeded
+initial states for the workloads this package is measured on.
+
+All arrays use the reference's Fortran layout (column-major with explicit
+lower bounds, src/update.f90:143-165) stored in C-ordered numpy arrays whose
+last axis is i:  a padded cell array (-2:imx+2,-2:jmx+2,-2:kmx+2,1:n_var) is
+``np.empty((n_var, kmx+5, jmx+5, imx+5))`` and element (i,j,k,l) lives at
+``[l-1, k+2, j+2, i+2]``.  Node arrays (-2:imx+3,...) are
+``(3, kmx+6, jmx+6, imx+6)``.
+
+Random numbers come from a counter-based SplitMix64 stream,
+``u_n = (mix(seed + (n+1)*0x9E3779B97F4A7C15) >> 11) * 2**-53``, so every
+array is reproducible element by element.
+
+Freestream conventions follow src/state.f90:75-133: Reynolds number per unit
+length L = 1 (``mu_ref = rho_inf*|V_inf|/Re``), ``tu_inf`` in percent,
+``k_inf = 1.5 (|V| tu)^2``, ``omega_inf = rho k / (mu_ref * mu_ratio)``.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import List, Optional
+
+import numpy as np
+
+# flux / reconstruction / turbulence ids shared with the C-ABI
+FLUX = {"ausm": 0, "ausmP": 1, "ausmUP": 2, "slau": 3, "ldfss0": 4, "van_leer": 5}
+RECON = {"none": 0, "muscl": 1, "ppm": 2, "weno": 3}
+TURB = {"none": 0, "sst": 1}
+TIME_ACC = {"none": 0, "RK2": 2, "RK4": 4}
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix_uniform(seed: int, n: int, offset: int = 0) -> np.ndarray:
+    """n uniform doubles in [0,1) from draws offset..offset+n-1 of stream `seed`."""
+    with np.errstate(over="ignore"):
+        idx = np.arange(offset + 1, offset + n + 1, dtype=np.uint64)
+        z = np.uint64(seed) + idx * _GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+
+
+@dataclasses.dataclass
+class Case:
+    """Everything one block needs: extent, scheme, flow, boundary values."""
+
+    imx: int
+    jmx: int
+    kmx: int
+    flux: str = "ausmP"
+    recon: str = "muscl"
+    turb: str = "none"
+    time_local: int = 0          # 1: 'l' local steps, 0: 'g' global step
+    time_accuracy: str = "RK4"
+    accur: int = 0
+    lim: tuple = (1, 1, 1)
+    tlim: tuple = (0, 0, 0)
+    mu_sutherland: int = 0
+    bc_id: tuple = (-9, -9, -5, -5, -9, -9)
+    global_time_step: float = -1.0
+    CFL: float = 0.5
+    rho_inf: float = 1.0
+    u_inf: float = 0.0
+    v_inf: float = 0.0
+    w_inf: float = 0.0
+    p_inf: float = 101325.0
+    tk_inf: float = 0.0
+    tw_inf: float = 0.0
+    vel_mag: float = 0.0
+    MInf: float = 0.0
+    gm: float = 1.4
+    R_gas: float = 287.0
+    mu_ref: float = 0.0
+    T_ref: float = 300.0
+    S_ref: float = 110.0
+    Pr: float = 0.72
+    tPr: float = 0.9
+    fix_rho: tuple = (0.0,) * 6
+    fix_u: tuple = (0.0,) * 6
+    fix_v: tuple = (0.0,) * 6
+    fix_w: tuple = (0.0,) * 6
+    fix_p: tuple = (0.0,) * 6
+    fix_tk: tuple = (0.0,) * 6
+    fix_tw: tuple = (0.0,) * 6
+    fix_twall: tuple = (0.0,) * 6
+    name: str = "case"
+
+    @property
+    def n_var(self) -> int:
+        return 7 if self.turb == "sst" else 5
+
+    @property
+    def ncells(self) -> int:
+        return (self.imx - 1) * (self.jmx - 1) * (self.kmx - 1)
+
+    def cell_shape(self, nvar: Optional[int] = None):
+        s = (self.kmx + 5, self.jmx + 5, self.imx + 5)
+        return s if nvar is None else (nvar,) + s
+
+    def node_shape(self):
+        return (3, self.kmx + 6, self.jmx + 6, self.imx + 6)
+
+
+def freestream(case: Case, T_inf=300.0, p_inf=101325.0, mach=0.5, reynolds=1000.0,
+               tu_percent=0.1, mu_ratio=0.01, direction=(1.0, 0.0, 0.0)) -> Case:
+    """Fill the dimensional freestream of `case` (src/state.f90:75-133)."""
+    gm, R = case.gm, case.R_gas
+    rho = p_inf / (R * T_inf)
+    c = math.sqrt(gm * p_inf / rho)
+    V = mach * c
+    d = np.asarray(direction, dtype=float)
+    d = d / np.linalg.norm(d)
+    case.rho_inf, case.p_inf = rho, p_inf
+    case.u_inf, case.v_inf, case.w_inf = (V * d[0], V * d[1], V * d[2])
+    case.vel_mag = math.sqrt(case.u_inf ** 2 + case.v_inf ** 2 + case.w_inf ** 2)
+    case.MInf = case.vel_mag / math.sqrt(gm * p_inf / rho)
+    case.mu_ref = rho * case.vel_mag * 1.0 / reynolds
+    case.T_ref = T_inf
+    if case.turb == "sst":
+        tu = tu_percent / 100
+        case.tk_inf = 1.5 * ((case.vel_mag * tu) ** 2)
+        case.tw_inf = rho * case.tk_inf / (case.mu_ref * mu_ratio)
+    return case
+
+
+def freestream_state(case: Case) -> np.ndarray:
+    q = np.empty(case.cell_shape(case.n_var))
+    vals = [case.rho_inf, case.u_inf, case.v_inf, case.w_inf, case.p_inf, case.tk_inf, case.tw_inf]
+    for l in range(case.n_var):
+        q[l] = vals[l]
+    return q
+
+
+def cartesian_nodes(case: Case, spacing=(1.0, 1.0, 1.0), warp: float = 0.0, seed: int = 0) -> np.ndarray:
+    """Uniform Cartesian nodes 0..(n-1)*h; optional smooth warp for tests."""
+    x = np.zeros(case.node_shape())
+    i = np.arange(1, case.imx + 1)
+    j = np.arange(1, case.jmx + 1)
+    k = np.arange(1, case.kmx + 1)
+    K, J, I = np.meshgrid(k, j, i, indexing="ij")
+    X = (I - 1) * spacing[0]
+    Y = (J - 1) * spacing[1]
+    Z = (K - 1) * spacing[2]
+    if warp:
+        Lx, Ly, Lz = X.max() or 1.0, Y.max() or 1.0, Z.max() or 1.0
+        X = X + warp * Lx * np.sin(2 * np.pi * Y / Ly) * np.sin(2 * np.pi * Z / Lz)
+        Y = Y + 0.5 * warp * Ly * np.sin(2 * np.pi * X / Lx) * np.cos(np.pi * Z / Lz)
+        Z = Z + 0.5 * warp * Lz * np.sin(np.pi * X / Lx) * np.sin(2 * np.pi * Y / Ly)
+    x[0, 3:-3, 3:-3, 3:-3] = X
+    x[1, 3:-3, 3:-3, 3:-3] = Y
+    x[2, 3:-3, 3:-3, 3:-3] = Z
+    return x
+
+
+def tanh_stretch(n_nodes: int, first: float, height: float = 1.0) -> np.ndarray:
+    """Wall-clustered distribution s_0=0..s_{n-1}=height with s_1 = first.
+
+    s(eta) = height*(1 + tanh(beta*(eta-1))/tanh(beta)); beta solved by
+    bisection so that the first spacing equals `first`."""
+    eta = np.linspace(0.0, 1.0, n_nodes)
+
+    def first_of(beta):
+        return height * (1 + math.tanh(beta * (eta[1] - 1)) / math.tanh(beta))
+
+    lo, hi = 1e-6, 50.0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if first_of(mid) > first:
+            lo = mid
+        else:
+            hi = mid
+    beta = 0.5 * (lo + hi)
+    return height * (1 + np.tanh(beta * (eta - 1)) / math.tanh(beta))
+
+
+def bump_nodes(case: Case, length=1.0, height=1.0, span=0.25, bump=0.05, first=1e-5) -> np.ndarray:
+    """Channel over a sinusoidal bump y_w = bump*sin^2(pi(x-0.3)/0.4) on
+    0.3L..0.7L; tanh wall-normal clustering with first spacing `first`,
+    uniform in x and z."""
+    x = np.zeros(case.node_shape())
+    xs = np.linspace(0.0, length, case.imx)
+    yw = np.where((xs >= 0.3 * length) & (xs <= 0.7 * length),
+                  bump * length * np.sin(np.pi * (xs - 0.3 * length) / (0.4 * length)) ** 2, 0.0)
+    s = tanh_stretch(case.jmx, first / height, 1.0)
+    zs = np.linspace(0.0, span, case.kmx)
+    X = np.broadcast_to(xs[None, None, :], (case.kmx, case.jmx, case.imx))
+    Y = yw[None, None, :] + (height - yw[None, None, :]) * s[None, :, None]
+    Z = np.broadcast_to(zs[:, None, None], (case.kmx, case.jmx, case.imx))
+    x[0, 3:-3, 3:-3, 3:-3] = X
+    x[1, 3:-3, 3:-3, 3:-3] = Y
+    x[2, 3:-3, 3:-3, 3:-3] = Z
+    return x
+
+
+def gaussian_bumps(case: Case, q: np.ndarray, seed: int, nbumps=8, width=4.0, amp=0.01):
+    """Adds seeded Gaussian bumps (width in cells) to rho,u,v,w,p of the
+    interior cells of q, amplitude `amp` times rho_inf, |V_inf|, p_inf."""
+    ni, nj, nk = case.imx - 1, case.jmx - 1, case.kmx - 1
+    u = splitmix_uniform(seed, nbumps * 8)
+    ic = np.arange(1, ni + 1) - 0.5
+    jc = np.arange(1, nj + 1) - 0.5
+    kc = np.arange(1, nk + 1) - 0.5
+    K, J, I = np.meshgrid(kc, jc, ic, indexing="ij")
+    scales = [case.rho_inf, case.vel_mag, case.vel_mag, case.vel_mag, case.p_inf]
+    for b in range(nbumps):
+        r = u[8 * b: 8 * b + 8]
+        ci, cj, ck = r[0] * ni, r[1] * nj, r[2] * nk
+        g = np.exp(-((I - ci) ** 2 + (J - cj) ** 2 + (K - ck) ** 2) / (2 * width * width))
+        for l in range(5):
+            q[l, 3:-3, 3:-3, 3:-3] += amp * scales[l] * (2 * r[3 + l] - 1) * g
+    return q
+
+
+def uniform_noise(case: Case, q: np.ndarray, seed: int, amp: float):
+    """Seeded uniform noise of relative amplitude `amp` on every variable of
+    the interior cells (velocities scaled by |V_inf|)."""
+    n = case.ncells
+    inner = (slice(3, -3),) * 3
+    for l in range(case.n_var):
+        r = splitmix_uniform(seed, n, offset=l * n).reshape(case.kmx - 1, case.jmx - 1, case.imx - 1)
+        if l in (1, 2, 3):
+            q[(l,) + inner] += amp * case.vel_mag * (2 * r - 1)
+        else:
+            q[(l,) + inner] *= 1 + amp * (2 * r - 1)
+    return q
+
+
+# ----------------------------------------------------------------- workloads
+def laminar_box(ni=64, nj=64, nk=4, seed=1, flux="ausmP", recon="muscl", time_local=0):
+    """Laminar box: periodic in i and k (BC -9), isothermal no-slip walls at
+    j (Tw = 300 K), M=0.5, Re=1000, Pr=0.72, unit spacing, 8 Gaussian bumps
+    of 1%.  At 64x64x4 cells this is the reference's CPU-runnable case."""
+    c = Case(ni + 1, nj + 1, nk + 1, flux=flux, recon=recon, turb="none", time_local=time_local,
+             accur=0, lim=(1, 1, 1), tlim=(0, 0, 0), bc_id=(-9, -9, -5, -5, -9, -9), CFL=0.5,
+             name=f"laminar_box_{ni}x{nj}x{nk}")
+    freestream(c, mach=0.5, reynolds=1000.0)
+    c.Pr = 0.72
+    c.fix_twall = (0.0, 0.0, 300.0, 300.0, 0.0, 0.0)
+    nodes = cartesian_nodes(c)
+    q = gaussian_bumps(c, freestream_state(c), seed)
+    return c, nodes, q
+
+
+def sst_bump(ni=256, nj=128, nk=128, seed=2, flux="slau", recon="weno", first=1e-5):
+    """RANS SST channel over a 0.05L sinusoidal bump: WENO5 + SLAU, M=0.2,
+    Re=1e6 per L, Tu=0.1%, mu_t/mu=0.01, 0.5% seeded noise.  Boundaries:
+    subsonic inlet (-3), subsonic outlet (-4), adiabatic wall at jmin (-5),
+    far field at jmax (-8), periodic in k (-9)."""
+    c = Case(ni + 1, nj + 1, nk + 1, flux=flux, recon=recon, turb="sst", time_local=1,
+             accur=1, lim=(1, 1, 1), tlim=(1, 1, 1), mu_sutherland=1,
+             bc_id=(-3, -4, -5, -8, -9, -9), CFL=0.5, name=f"sst_bump_{ni}x{nj}x{nk}")
+    freestream(c, mach=0.2, reynolds=1.0e6, tu_percent=0.1, mu_ratio=0.01)
+    c.Pr = 0.72
+    c.tPr = 0.9
+    inf = (c.rho_inf, c.u_inf, c.v_inf, c.w_inf, c.p_inf, c.tk_inf, c.tw_inf)
+    c.fix_rho = (inf[0],) * 6
+    c.fix_u = (inf[1],) * 6
+    c.fix_v = (inf[2],) * 6
+    c.fix_w = (inf[3],) * 6
+    c.fix_p = (inf[4],) * 6
+    c.fix_tk = (inf[5],) * 6
+    c.fix_tw = (inf[6],) * 6
+    c.fix_twall = (0.0,) * 6
+    nodes = bump_nodes(c, first=first)
+    q = uniform_noise(c, freestream_state(c), seed, 0.005)
+    return c, nodes, q
+
+
+WORKLOADS = {"laminar_box": laminar_box, "sst_bump": sst_bump}

@@ -1,0 +1,226 @@
+"""ctypes bindings of the checkers: the C restatement (oracle/liboracle.so)
+and the amdflang build of the reference modules (oracle/_ref/libfestref.so).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libfestref.so")
+
+ARRAY_IDS = ["qp", "temp", "mu", "mut", "f1", "dist", "vol", "cx", "cy", "cz",
+             "iA", "inx", "iny", "inz", "jA", "jnx", "jny", "jnz", "kA", "knx", "kny", "knz",
+             "xl", "xr", "yl", "yr", "zl", "zr", "F", "G", "H", "res", "dt", "ustore", "rstore",
+             "gx", "gy", "gz", "nodes"]
+AID = {n: i for i, n in enumerate(ARRAY_IDS)}
+
+
+class CCase(ctypes.Structure):
+    _fields_ = [("imx", ctypes.c_int32), ("jmx", ctypes.c_int32), ("kmx", ctypes.c_int32),
+                ("n_var", ctypes.c_int32), ("flux", ctypes.c_int32), ("recon", ctypes.c_int32),
+                ("turb", ctypes.c_int32), ("time_local", ctypes.c_int32),
+                ("time_accuracy", ctypes.c_int32), ("accur", ctypes.c_int32),
+                ("lim", ctypes.c_int32 * 3), ("tlim", ctypes.c_int32 * 3),
+                ("mu_sutherland", ctypes.c_int32), ("bc_id", ctypes.c_int32 * 6),
+                ("global_time_step", ctypes.c_double), ("CFL", ctypes.c_double),
+                ("rho_inf", ctypes.c_double), ("u_inf", ctypes.c_double), ("v_inf", ctypes.c_double),
+                ("w_inf", ctypes.c_double), ("p_inf", ctypes.c_double), ("tk_inf", ctypes.c_double),
+                ("tw_inf", ctypes.c_double), ("vel_mag", ctypes.c_double), ("MInf", ctypes.c_double),
+                ("gm", ctypes.c_double), ("R_gas", ctypes.c_double), ("mu_ref", ctypes.c_double),
+                ("T_ref", ctypes.c_double), ("S_ref", ctypes.c_double), ("Pr", ctypes.c_double),
+                ("tPr", ctypes.c_double),
+                ("fix_rho", ctypes.c_double * 6), ("fix_u", ctypes.c_double * 6),
+                ("fix_v", ctypes.c_double * 6), ("fix_w", ctypes.c_double * 6),
+                ("fix_p", ctypes.c_double * 6), ("fix_tk", ctypes.c_double * 6),
+                ("fix_tw", ctypes.c_double * 6), ("fix_twall", ctypes.c_double * 6)]
+
+
+def to_ccase(case) -> CCase:
+    from festgpu.cases import FLUX, RECON, TURB, TIME_ACC
+    c = CCase()
+    c.imx, c.jmx, c.kmx, c.n_var = case.imx, case.jmx, case.kmx, case.n_var
+    c.flux, c.recon, c.turb = FLUX[case.flux], RECON[case.recon], TURB[case.turb]
+    c.time_local, c.time_accuracy, c.accur = case.time_local, TIME_ACC[case.time_accuracy], case.accur
+    c.lim[:] = list(case.lim)
+    c.tlim[:] = list(case.tlim)
+    c.mu_sutherland = case.mu_sutherland
+    c.bc_id[:] = list(case.bc_id)
+    for f in ("global_time_step", "CFL", "rho_inf", "u_inf", "v_inf", "w_inf", "p_inf", "tk_inf",
+              "tw_inf", "vel_mag", "MInf", "gm", "R_gas", "mu_ref", "T_ref", "S_ref", "Pr", "tPr"):
+        setattr(c, f, float(getattr(case, f)))
+    for f in ("fix_rho", "fix_u", "fix_v", "fix_w", "fix_p", "fix_tk", "fix_tw", "fix_twall"):
+        getattr(c, f)[:] = [float(v) for v in getattr(case, f)]
+    return c
+
+
+def array_shape(case, name):
+    """numpy (C-order) shape of each array in the Fortran layout."""
+    imx, jmx, kmx, nv = case.imx, case.jmx, case.kmx, case.n_var
+    ng = 6 if case.turb == "sst" else 4
+    cell = (kmx + 5, jmx + 5, imx + 5)
+    shapes = {
+        "qp": (nv,) + cell, "ustore": (nv,) + cell,
+        "temp": cell, "mu": cell, "mut": cell, "f1": cell, "dist": cell,
+        "vol": cell, "cx": cell, "cy": cell, "cz": cell,
+        "iA": (kmx + 5, jmx + 5, imx + 6), "jA": (kmx + 5, jmx + 6, imx + 5), "kA": (kmx + 6, jmx + 5, imx + 5),
+        "xl": (nv, kmx - 1, jmx - 1, imx + 2), "yl": (nv, kmx - 1, jmx + 2, imx - 1),
+        "zl": (nv, kmx + 2, jmx - 1, imx - 1),
+        "F": (nv, kmx - 1, jmx - 1, imx), "G": (nv, kmx - 1, jmx, imx - 1), "H": (nv, kmx, jmx - 1, imx - 1),
+        "res": (nv, kmx - 1, jmx - 1, imx - 1), "rstore": (nv, kmx - 1, jmx - 1, imx - 1),
+        "dt": (kmx - 1, jmx - 1, imx - 1),
+        "gx": (ng, kmx + 1, jmx + 1, imx + 1),
+        "nodes": (3, kmx + 6, jmx + 6, imx + 6),
+    }
+    for a, b in (("inx", "iA"), ("iny", "iA"), ("inz", "iA"), ("jnx", "jA"), ("jny", "jA"), ("jnz", "jA"),
+                 ("knx", "kA"), ("kny", "kA"), ("knz", "kA"), ("xr", "xl"), ("yr", "yl"), ("zr", "zl"),
+                 ("gy", "gx"), ("gz", "gx")):
+        shapes[a] = shapes[b]
+    return shapes[name]
+
+
+def _load(path):
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is not built (run make -C oracle)")
+    return ctypes.CDLL(path)
+
+
+class Oracle:
+    """One block of the C restatement."""
+
+    _lib = None
+
+    @classmethod
+    def lib(cls):
+        if cls._lib is None:
+            L = _load(ORACLE_SO)
+            L.orc_create.restype = ctypes.c_void_p
+            L.orc_create.argtypes = [ctypes.POINTER(CCase)]
+            L.orc_destroy.argtypes = [ctypes.c_void_p]
+            L.orc_array.restype = ctypes.POINTER(ctypes.c_double)
+            L.orc_array.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]
+            for f in ("orc_geometry", "orc_wall_dist", "orc_time_step"):
+                getattr(L, f).argtypes = [ctypes.c_void_p]
+                getattr(L, f).restype = ctypes.c_int
+            for f in ("orc_residual", "orc_step"):
+                getattr(L, f).argtypes = [ctypes.c_void_p, ctypes.c_int]
+                getattr(L, f).restype = ctypes.c_int
+            L.orc_temp_refresh.argtypes = [ctypes.c_void_p]
+            L.orc_resnorm.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+            cls._lib = L
+        return cls._lib
+
+    def __init__(self, case):
+        self.case = case
+        self._c = to_ccase(case)
+        self.h = self.lib().orc_create(ctypes.byref(self._c))
+        if not self.h:
+            raise MemoryError("orc_create failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib().orc_destroy(self.h)
+            self.h = None
+
+    def arr(self, name) -> np.ndarray:
+        n = ctypes.c_size_t()
+        p = self.lib().orc_array(self.h, AID[name], ctypes.byref(n))
+        a = np.ctypeslib.as_array(p, shape=(n.value,))
+        return a.reshape(array_shape(self.case, name))
+
+    def set_nodes(self, nodes):
+        self.arr("nodes")[...] = nodes
+        st = self.lib().orc_geometry(self.h)
+        if st:
+            raise RuntimeError(f"orc_geometry failed: {st}")
+
+    def wall_dist(self):
+        return self.lib().orc_wall_dist(self.h)
+
+    def set_state(self, q):
+        self.arr("qp")[...] = q
+
+    def temp_refresh(self):
+        self.lib().orc_temp_refresh(self.h)
+
+    def residual(self, it=1):
+        return self.lib().orc_residual(self.h, it)
+
+    def time_step(self):
+        return self.lib().orc_time_step(self.h)
+
+    def step(self, it):
+        return self.lib().orc_step(self.h, it)
+
+    def resnorm(self):
+        out = (ctypes.c_double * 8)()
+        self.lib().orc_resnorm(self.h, out)
+        return np.array(out[: self.case.n_var + 1])
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+_REF_SCRIPT = r"""
+import ctypes, sys, numpy as np, pickle
+sys.path.insert(0, sys.argv[1]); sys.path.insert(0, sys.argv[2])
+from binding import CCase, to_ccase, array_shape, AID, REF_SO
+job = pickle.load(open(sys.argv[3], 'rb'))
+case = job['case']
+L = ctypes.CDLL(REF_SO)
+c = to_ccase(case)
+L.ref_init(ctypes.byref(c))
+dp = ctypes.POINTER(ctypes.c_double)
+def ptr(a): return a.ctypes.data_as(dp)
+nodes = np.ascontiguousarray(job['nodes'], dtype=np.float64)
+L.ref_set_nodes(ptr(nodes))
+if job.get('wall_dist'):
+    path = job['tmpfile'].encode()
+    L.ref_wall_dist(ctypes.c_char_p(path), ctypes.c_int(len(path)))
+elif job.get('dist') is not None:
+    d = np.ascontiguousarray(job['dist'], dtype=np.float64); L.ref_set_dist(ptr(d))
+q = np.ascontiguousarray(job['qp'], dtype=np.float64)
+t = np.ascontiguousarray(job['temp'], dtype=np.float64)
+L.ref_set_state(ptr(q), ptr(t))
+if job.get('stage', 0) > 0:
+    L.ref_stage(ctypes.c_int(job.get('iter', 1)), ctypes.c_int(job['stage']))
+out = {}
+for name in job['want']:
+    shp = array_shape(case, name)
+    a = np.zeros(int(np.prod(shp)))
+    L.ref_get(ctypes.c_int(AID[name]), ptr(a))
+    out[name] = a.reshape(shp)
+pickle.dump(out, open(sys.argv[4], 'wb'))
+"""
+
+
+def run_reference(case, nodes, qp, temp, stage, want, it=1, dist=None, wall_dist=False, tmpdir="/tmp"):
+    """Runs the reference modules in a fresh process (their module state,
+    e.g. the sticky ppm_flag of boundary_state_reconstruction, persists
+    within one process) and returns the requested arrays."""
+    import pickle
+    import tempfile
+    pkg = os.path.join(os.path.dirname(HERE), "fest-3d_amd")
+    with tempfile.TemporaryDirectory(dir=tmpdir) as td:
+        jin, jout = os.path.join(td, "in.pkl"), os.path.join(td, "out.pkl")
+        script = os.path.join(td, "run.py")
+        with open(script, "w") as f:
+            f.write(_REF_SCRIPT)
+        job = dict(case=case, nodes=nodes, qp=qp, temp=temp, stage=stage, want=list(want), iter=it,
+                   dist=dist, wall_dist=wall_dist, tmpfile=os.path.join(td, "surfnode.dat"))
+        with open(jin, "wb") as f:
+            pickle.dump(job, f)
+        r = subprocess.run([sys.executable, script, HERE, pkg, jin, jout], capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference run failed ({r.returncode}):\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+        with open(jout, "rb") as f:
+            return pickle.load(f)

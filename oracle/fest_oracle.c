@@ -1,0 +1,1630 @@
+/*
+ * fest_oracle.c -- ORACLE (test infrastructure only).
+ *
+ * A plain-C, single-threaded restatement of FEST-3D's explicit residual
+ * evaluation and RK stage update.  It follows the reference pass by pass and
+ * in the reference's operation order, so that the GPU path can be checked
+ * against it.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it, and only as the checker.
+ *
+ * Parity pins (see tests/): each pass is compared against the amdflang build
+ * of the reference modules (oracle/_ref, when /root/reference is present),
+ * against the reference's own known-answer tests (tests/test_*.f90), and
+ * against closed forms for the parts the reference build cannot run
+ * (src/time.f90 and src/update.f90 use MPI and do not link here): the RK4
+ * amplification polynomial, the uniform-grid time step and the norms.
+ *
+ * Semantics choices that the reference leaves to the compiler:
+ *   - MAX/MIN with a NaN argument follow amdflang's lowering, `a>b ? a : b`
+ *     and `a<b ? a : b`, folded left to right.
+ *   - SIGN(a,b) with b = -0.0 returns -|a| (copysign).
+ *   - No floating-point contraction (build with -ffp-contract=off).
+ */
+#include "fest_oracle.h"
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+struct orc {
+  orc_case c;
+  int imx, jmx, kmx, nv, ngrad;
+  int pi, pj, pk;
+  size_t ncell, nif, njf, nkf, nnode;
+  size_t nxs, nys, nzs, nF, nG, nH, nR, ngr;
+  double *a[ORC_NARRAYS];
+  size_t n[ORC_NARRAYS];
+  int *mfz, *mgz, *mhz;
+  int ppm_flag;
+  double c1, c2, c3;
+};
+
+/* ---------------------------------------------------------------- indexing */
+#define IMX (s->imx)
+#define JMX (s->jmx)
+#define KMX (s->kmx)
+static inline size_t CI(const orc *s, int i, int j, int k) {
+  return (size_t)(i + 2) + (size_t)s->pi * ((size_t)(j + 2) + (size_t)s->pj * (size_t)(k + 2));
+}
+static inline size_t IFI(const orc *s, int i, int j, int k) {
+  return (size_t)(i + 2) + (size_t)(s->imx + 6) * ((size_t)(j + 2) + (size_t)(s->jmx + 5) * (size_t)(k + 2));
+}
+static inline size_t JFI(const orc *s, int i, int j, int k) {
+  return (size_t)(i + 2) + (size_t)(s->imx + 5) * ((size_t)(j + 2) + (size_t)(s->jmx + 6) * (size_t)(k + 2));
+}
+static inline size_t KFI(const orc *s, int i, int j, int k) {
+  return (size_t)(i + 2) + (size_t)(s->imx + 5) * ((size_t)(j + 2) + (size_t)(s->jmx + 5) * (size_t)(k + 2));
+}
+static inline size_t NDI(const orc *s, int i, int j, int k) {
+  return (size_t)(i + 2) + (size_t)(s->imx + 6) * ((size_t)(j + 2) + (size_t)(s->jmx + 6) * (size_t)(k + 2));
+}
+/* face-state arrays, direction d = 0 (x), 1 (y), 2 (z) */
+static inline size_t XSI(const orc *s, int i, int j, int k) {
+  return (size_t)i + (size_t)(s->imx + 2) * ((size_t)(j - 1) + (size_t)(s->jmx - 1) * (size_t)(k - 1));
+}
+static inline size_t YSI(const orc *s, int i, int j, int k) {
+  return (size_t)(i - 1) + (size_t)(s->imx - 1) * ((size_t)j + (size_t)(s->jmx + 2) * (size_t)(k - 1));
+}
+static inline size_t ZSI(const orc *s, int i, int j, int k) {
+  return (size_t)(i - 1) + (size_t)(s->imx - 1) * ((size_t)(j - 1) + (size_t)(s->jmx - 1) * (size_t)k);
+}
+static inline size_t FI(const orc *s, int i, int j, int k) {
+  return (size_t)(i - 1) + (size_t)s->imx * ((size_t)(j - 1) + (size_t)(s->jmx - 1) * (size_t)(k - 1));
+}
+static inline size_t GI(const orc *s, int i, int j, int k) {
+  return (size_t)(i - 1) + (size_t)(s->imx - 1) * ((size_t)(j - 1) + (size_t)s->jmx * (size_t)(k - 1));
+}
+static inline size_t HI(const orc *s, int i, int j, int k) {
+  return (size_t)(i - 1) + (size_t)(s->imx - 1) * ((size_t)(j - 1) + (size_t)(s->jmx - 1) * (size_t)(k - 1));
+}
+#define RI HI
+static inline size_t GRI(const orc *s, int i, int j, int k) {
+  return (size_t)i + (size_t)(s->imx + 1) * ((size_t)j + (size_t)(s->jmx + 1) * (size_t)k);
+}
+
+#define QP(l, i, j, k) s->a[ORC_QP][(size_t)(l) * s->ncell + CI(s, i, j, k)]
+#define QPL(l) (s->a[ORC_QP] + (size_t)(l) * s->ncell)
+
+/* flang's lowering of MAX/MIN (see header comment) */
+static inline double fmx(double a, double b) { return a > b ? a : b; }
+static inline double fmn(double a, double b) { return a < b ? a : b; }
+static inline double fsign(double a, double b) { return copysign(fabs(a), b); }
+static inline double sq(double x) { return x * x; }
+
+/* SST constants, src/global/global_sst.f90:6-14 */
+static const double sigma_k1 = 0.85, sigma_k2 = 1.0, sigma_w1 = 0.5, sigma_w2 = 0.856;
+static const double beta1 = 0.075, beta2 = 0.0828, bstar = 0.09, kappa_sst = 0.41, a1_sst = 0.31;
+
+/* ----------------------------------------------------------- allocation */
+orc *orc_create(const orc_case *c) {
+  orc *s = (orc *)calloc(1, sizeof(orc));
+  if (!s) return NULL;
+  s->c = *c;
+  s->imx = c->imx; s->jmx = c->jmx; s->kmx = c->kmx; s->nv = c->n_var;
+  s->ngrad = (c->turb == ORC_TURB_SST) ? 6 : 4;
+  s->pi = s->imx + 5; s->pj = s->jmx + 5; s->pk = s->kmx + 5;
+  s->ncell = (size_t)s->pi * s->pj * s->pk;
+  s->nif = (size_t)(s->imx + 6) * (s->jmx + 5) * (s->kmx + 5);
+  s->njf = (size_t)(s->imx + 5) * (s->jmx + 6) * (s->kmx + 5);
+  s->nkf = (size_t)(s->imx + 5) * (s->jmx + 5) * (s->kmx + 6);
+  s->nnode = (size_t)(s->imx + 6) * (s->jmx + 6) * (s->kmx + 6);
+  s->nxs = (size_t)(s->imx + 2) * (s->jmx - 1) * (s->kmx - 1);
+  s->nys = (size_t)(s->imx - 1) * (s->jmx + 2) * (s->kmx - 1);
+  s->nzs = (size_t)(s->imx - 1) * (s->jmx - 1) * (s->kmx + 2);
+  s->nF = (size_t)s->imx * (s->jmx - 1) * (s->kmx - 1);
+  s->nG = (size_t)(s->imx - 1) * s->jmx * (s->kmx - 1);
+  s->nH = (size_t)(s->imx - 1) * (s->jmx - 1) * s->kmx;
+  s->nR = (size_t)(s->imx - 1) * (s->jmx - 1) * (s->kmx - 1);
+  s->ngr = (size_t)(s->imx + 1) * (s->jmx + 1) * (s->kmx + 1);
+  size_t nv = (size_t)s->nv;
+  size_t sz[ORC_NARRAYS] = {0};
+  sz[ORC_QP] = s->ncell * nv; sz[ORC_TEMP] = s->ncell; sz[ORC_MU] = s->ncell;
+  sz[ORC_MUT] = s->ncell; sz[ORC_F1] = s->ncell; sz[ORC_DIST] = s->ncell;
+  sz[ORC_VOL] = sz[ORC_CX] = sz[ORC_CY] = sz[ORC_CZ] = s->ncell;
+  sz[ORC_IA] = sz[ORC_INX] = sz[ORC_INY] = sz[ORC_INZ] = s->nif;
+  sz[ORC_JA] = sz[ORC_JNX] = sz[ORC_JNY] = sz[ORC_JNZ] = s->njf;
+  sz[ORC_KA] = sz[ORC_KNX] = sz[ORC_KNY] = sz[ORC_KNZ] = s->nkf;
+  sz[ORC_XL] = sz[ORC_XR] = s->nxs * nv;
+  sz[ORC_YL] = sz[ORC_YR] = s->nys * nv;
+  sz[ORC_ZL] = sz[ORC_ZR] = s->nzs * nv;
+  sz[ORC_F] = s->nF * nv; sz[ORC_G] = s->nG * nv; sz[ORC_H] = s->nH * nv;
+  sz[ORC_RES] = s->nR * nv; sz[ORC_DT] = s->nR;
+  sz[ORC_USTORE] = s->ncell * nv; sz[ORC_RSTORE] = s->nR * nv;
+  sz[ORC_GX] = sz[ORC_GY] = sz[ORC_GZ] = s->ngr * (size_t)s->ngrad;
+  sz[ORC_NODES] = s->nnode * 3;
+  for (int id = 0; id < ORC_NARRAYS; ++id) {
+    s->n[id] = sz[id];
+    s->a[id] = (double *)calloc(sz[id] ? sz[id] : 1, sizeof(double));
+    if (!s->a[id]) { orc_destroy(s); return NULL; }
+  }
+  /* mu = mu_ref initially (src/viscosity.f90:565-568) */
+  for (size_t n = 0; n < s->ncell; ++n) s->a[ORC_MU][n] = c->mu_ref;
+  /* src/boundary/bc.f90:45-67 */
+  s->c2 = 1 + c->accur;
+  s->c3 = 0.5 * c->accur;
+  s->c1 = s->c2 - s->c3;
+  s->mfz = (int *)malloc(sizeof(int) * (s->imx + 1));
+  s->mgz = (int *)malloc(sizeof(int) * (s->jmx + 1));
+  s->mhz = (int *)malloc(sizeof(int) * (s->kmx + 1));
+  for (int i = 0; i <= s->imx; ++i) s->mfz[i] = 1;
+  for (int j = 0; j <= s->jmx; ++j) s->mgz[j] = 1;
+  for (int k = 0; k <= s->kmx; ++k) s->mhz[k] = 1;
+#define WALLISH(id) ((id) == -5 || (id) == -6 || (id) == -7)
+  if (WALLISH(c->bc_id[0])) s->mfz[1] = 0;
+  if (WALLISH(c->bc_id[2])) s->mgz[1] = 0;
+  if (WALLISH(c->bc_id[4])) s->mhz[1] = 0;
+  if (WALLISH(c->bc_id[1])) s->mfz[s->imx] = 0;
+  if (WALLISH(c->bc_id[3])) s->mgz[s->jmx] = 0;
+  if (WALLISH(c->bc_id[5])) s->mhz[s->kmx] = 0;
+#undef WALLISH
+  /* src/boundary/boundary_state_reconstruction.f90:45-47 */
+  s->ppm_flag = (c->recon == ORC_RECON_PPM || c->recon == ORC_RECON_WENO) ? 1 : 0;
+  for (int f = 0; f < 6; ++f) if (c->bc_id[f] == -7) s->ppm_flag = 1;
+  return s;
+}
+
+void orc_destroy(orc *s) {
+  if (!s) return;
+  for (int id = 0; id < ORC_NARRAYS; ++id) free(s->a[id]);
+  free(s->mfz); free(s->mgz); free(s->mhz);
+  free(s);
+}
+
+double *orc_array(orc *s, int id, size_t *count) {
+  if (id < 0 || id >= ORC_NARRAYS) return NULL;
+  if (count) *count = s->n[id];
+  return s->a[id];
+}
+
+/* ------------------------------------------------------------- geometry */
+/* src/grid.f90:137-236: linear extrapolation of ghost nodes, i then j then k */
+static void ghost_grid(orc *s) {
+  double *X[3] = {s->a[ORC_NODES], s->a[ORC_NODES] + s->nnode, s->a[ORC_NODES] + 2 * s->nnode};
+  for (int c = 0; c < 3; ++c) {
+    double *x = X[c];
+    for (int k = -2; k <= KMX + 3; ++k)
+      for (int j = -2; j <= JMX + 3; ++j) {
+        x[NDI(s, 0, j, k)] = 2 * x[NDI(s, 1, j, k)] - x[NDI(s, 2, j, k)];
+        x[NDI(s, -1, j, k)] = 2 * x[NDI(s, 0, j, k)] - x[NDI(s, 1, j, k)];
+        x[NDI(s, -2, j, k)] = 2 * x[NDI(s, -1, j, k)] - x[NDI(s, 0, j, k)];
+        x[NDI(s, IMX + 1, j, k)] = 2 * x[NDI(s, IMX, j, k)] - x[NDI(s, IMX - 1, j, k)];
+        x[NDI(s, IMX + 2, j, k)] = 2 * x[NDI(s, IMX + 1, j, k)] - x[NDI(s, IMX, j, k)];
+        x[NDI(s, IMX + 3, j, k)] = 2 * x[NDI(s, IMX + 2, j, k)] - x[NDI(s, IMX + 1, j, k)];
+      }
+  }
+  for (int c = 0; c < 3; ++c) {
+    double *x = X[c];
+    for (int k = -2; k <= KMX + 3; ++k)
+      for (int i = -2; i <= IMX + 3; ++i) {
+        x[NDI(s, i, 0, k)] = 2 * x[NDI(s, i, 1, k)] - x[NDI(s, i, 2, k)];
+        x[NDI(s, i, -1, k)] = 2 * x[NDI(s, i, 0, k)] - x[NDI(s, i, 1, k)];
+        x[NDI(s, i, -2, k)] = 2 * x[NDI(s, i, -1, k)] - x[NDI(s, i, 0, k)];
+        x[NDI(s, i, JMX + 1, k)] = 2 * x[NDI(s, i, JMX, k)] - x[NDI(s, i, JMX - 1, k)];
+        x[NDI(s, i, JMX + 2, k)] = 2 * x[NDI(s, i, JMX + 1, k)] - x[NDI(s, i, JMX, k)];
+        x[NDI(s, i, JMX + 3, k)] = 2 * x[NDI(s, i, JMX + 2, k)] - x[NDI(s, i, JMX + 1, k)];
+      }
+  }
+  for (int c = 0; c < 3; ++c) {
+    double *x = X[c];
+    for (int j = -2; j <= JMX + 3; ++j)
+      for (int i = -2; i <= IMX + 3; ++i) {
+        x[NDI(s, i, j, 0)] = 2 * x[NDI(s, i, j, 1)] - x[NDI(s, i, j, 2)];
+        x[NDI(s, i, j, -1)] = 2 * x[NDI(s, i, j, 0)] - x[NDI(s, i, j, 1)];
+        x[NDI(s, i, j, -2)] = 2 * x[NDI(s, i, j, -1)] - x[NDI(s, i, j, 0)];
+        x[NDI(s, i, j, KMX + 1)] = 2 * x[NDI(s, i, j, KMX)] - x[NDI(s, i, j, KMX - 1)];
+        x[NDI(s, i, j, KMX + 2)] = 2 * x[NDI(s, i, j, KMX + 1)] - x[NDI(s, i, j, KMX)];
+        x[NDI(s, i, j, KMX + 3)] = 2 * x[NDI(s, i, j, KMX + 2)] - x[NDI(s, i, j, KMX + 1)];
+      }
+  }
+}
+
+/* src/geometry.f90:342-356 */
+static double vol_tet(const double *p1, const double *p2, const double *p3, const double *p4) {
+  double v = (p4[0] - p1[0]) * ((p2[1] - p1[1]) * (p3[2] - p1[2]) - (p2[2] - p1[2]) * (p3[1] - p1[1]))
+           + (p4[1] - p1[1]) * ((p2[2] - p1[2]) * (p3[0] - p1[0]) - (p2[0] - p1[0]) * (p3[2] - p1[2]))
+           + (p4[2] - p1[2]) * ((p2[0] - p1[0]) * (p3[1] - p1[1]) - (p2[1] - p1[1]) * (p3[0] - p1[0]));
+  return -v / 6.;
+}
+
+/* src/geometry.f90:367-446: max of two five-tetrahedron splits */
+static double vol_hex(double p[8][3]) {
+  double v1 = 0.;
+  v1 = v1 + vol_tet(p[0], p[4], p[7], p[5]);
+  v1 = v1 + vol_tet(p[6], p[7], p[5], p[2]);
+  v1 = v1 + vol_tet(p[7], p[3], p[0], p[2]);
+  v1 = v1 + vol_tet(p[5], p[0], p[2], p[7]);
+  v1 = v1 + vol_tet(p[0], p[1], p[5], p[2]);
+  double v = 0.;
+  v = v + vol_tet(p[1], p[5], p[4], p[6]);
+  v = v + vol_tet(p[7], p[4], p[6], p[3]);
+  v = v + vol_tet(p[4], p[0], p[1], p[3]);
+  v = v + vol_tet(p[6], p[1], p[3], p[4]);
+  v = v + vol_tet(p[1], p[2], p[6], p[3]);
+  return fmx(v, v1);
+}
+
+int orc_geometry(orc *s) {
+  ghost_grid(s);
+  const double *X = s->a[ORC_NODES], *Y = X + s->nnode, *Z = Y + s->nnode;
+#define NX(i, j, k) X[NDI(s, i, j, k)]
+#define NY(i, j, k) Y[NDI(s, i, j, k)]
+#define NZ(i, j, k) Z[NDI(s, i, j, k)]
+  /* area vectors, src/geometry.f90:236-314 */
+  double *A, *nx, *ny, *nz;
+  A = s->a[ORC_IA]; nx = s->a[ORC_INX]; ny = s->a[ORC_INY]; nz = s->a[ORC_INZ];
+  for (int k = -2; k <= KMX + 2; ++k)
+    for (int j = -2; j <= JMX + 2; ++j)
+      for (int i = -2; i <= IMX + 3; ++i) {
+        double d1x = NX(i, j + 1, k + 1) - NX(i, j, k), d1y = NY(i, j + 1, k + 1) - NY(i, j, k),
+               d1z = NZ(i, j + 1, k + 1) - NZ(i, j, k);
+        double d2x = NX(i, j, k + 1) - NX(i, j + 1, k), d2y = NY(i, j, k + 1) - NY(i, j + 1, k),
+               d2z = NZ(i, j, k + 1) - NZ(i, j + 1, k);
+        size_t f = IFI(s, i, j, k);
+        nx[f] = 0.5 * (d1y * d2z - d1z * d2y);
+        ny[f] = 0.5 * (d1z * d2x - d1x * d2z);
+        nz[f] = 0.5 * (d1x * d2y - d1y * d2x);
+      }
+  A = s->a[ORC_JA]; nx = s->a[ORC_JNX]; ny = s->a[ORC_JNY]; nz = s->a[ORC_JNZ];
+  for (int k = -2; k <= KMX + 2; ++k)
+    for (int j = -2; j <= JMX + 3; ++j)
+      for (int i = -2; i <= IMX + 2; ++i) {
+        double d1x = NX(i + 1, j, k + 1) - NX(i, j, k), d1y = NY(i + 1, j, k + 1) - NY(i, j, k),
+               d1z = NZ(i + 1, j, k + 1) - NZ(i, j, k);
+        double d2x = NX(i + 1, j, k) - NX(i, j, k + 1), d2y = NY(i + 1, j, k) - NY(i, j, k + 1),
+               d2z = NZ(i + 1, j, k) - NZ(i, j, k + 1);
+        size_t f = JFI(s, i, j, k);
+        nx[f] = 0.5 * (d1y * d2z - d1z * d2y);
+        ny[f] = 0.5 * (d1z * d2x - d1x * d2z);
+        nz[f] = 0.5 * (d1x * d2y - d1y * d2x);
+      }
+  A = s->a[ORC_KA]; nx = s->a[ORC_KNX]; ny = s->a[ORC_KNY]; nz = s->a[ORC_KNZ];
+  for (int k = -2; k <= KMX + 3; ++k)
+    for (int j = -2; j <= JMX + 2; ++j)
+      for (int i = -2; i <= IMX + 2; ++i) {
+        double d1x = NX(i + 1, j + 1, k) - NX(i, j, k), d1y = NY(i + 1, j + 1, k) - NY(i, j, k),
+               d1z = NZ(i + 1, j + 1, k) - NZ(i, j, k);
+        double d2x = NX(i, j + 1, k) - NX(i + 1, j, k), d2y = NY(i, j + 1, k) - NY(i + 1, j, k),
+               d2z = NZ(i, j + 1, k) - NZ(i + 1, j, k);
+        size_t f = KFI(s, i, j, k);
+        nx[f] = 0.5 * (d1y * d2z - d1z * d2y);
+        ny[f] = 0.5 * (d1z * d2x - d1x * d2z);
+        nz[f] = 0.5 * (d1x * d2y - d1y * d2x);
+      }
+  /* areas and unit normals, src/geometry.f90:50-234 (pole branches: -7) */
+  const int fa[3][4] = {{ORC_IA, ORC_INX, ORC_INY, ORC_INZ},
+                        {ORC_JA, ORC_JNX, ORC_JNY, ORC_JNZ},
+                        {ORC_KA, ORC_KNX, ORC_KNY, ORC_KNZ}};
+  const size_t fn[3] = {s->nif, s->njf, s->nkf};
+  for (int d = 0; d < 3; ++d) {
+    A = s->a[fa[d][0]]; nx = s->a[fa[d][1]]; ny = s->a[fa[d][2]]; nz = s->a[fa[d][3]];
+    for (size_t f = 0; f < fn[d]; ++f) A[f] = sqrt(sq(nx[f]) + sq(ny[f]) + sq(nz[f]));
+    for (size_t f = 0; f < fn[d]; ++f)
+      if (A[f] != 0.) { nx[f] = nx[f] / A[f]; ny[f] = ny[f] / A[f]; nz[f] = nz[f] / A[f]; }
+  }
+  for (int f = 0; f < 6; ++f)
+    if (s->c.bc_id[f] == -7) return ORC_E_CONFIG; /* pole metrics: not restated */
+  /* volumes, src/geometry.f90:448-496 */
+  double *vol = s->a[ORC_VOL];
+  for (size_t n = 0; n < s->ncell; ++n) vol[n] = 1.;
+  for (int k = 0; k <= KMX; ++k)
+    for (int j = 0; j <= JMX; ++j)
+      for (int i = 0; i <= IMX; ++i) {
+        double p[8][3];
+        const int o[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                             {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+        for (int q = 0; q < 8; ++q) {
+          p[q][0] = NX(i + o[q][0], j + o[q][1], k + o[q][2]);
+          p[q][1] = NY(i + o[q][0], j + o[q][1], k + o[q][2]);
+          p[q][2] = NZ(i + o[q][0], j + o[q][1], k + o[q][2]);
+        }
+        vol[CI(s, i, j, k)] = vol_hex(p);
+        if (vol[CI(s, i, j, k)] <= 0.0) {
+          int ghost = (i == 0 || i == IMX || j == 0 || j == JMX || k == 0 || k == KMX);
+          if (!ghost) return ORC_E_NONPHYSICAL;
+        }
+      }
+  for (size_t n = 0; n < s->ncell; ++n)
+    if (vol[n] <= 0.0) return ORC_E_NONPHYSICAL;
+  /* centroids, src/geometry.f90:500-545 */
+  double *cc[3] = {s->a[ORC_CX], s->a[ORC_CY], s->a[ORC_CZ]};
+  const double *XX[3] = {X, Y, Z};
+  for (int c = 0; c < 3; ++c) {
+    const double *x = XX[c];
+    for (int k = -2; k <= KMX + 2; ++k)
+      for (int j = -2; j <= JMX + 2; ++j)
+        for (int i = -2; i <= IMX + 2; ++i)
+          cc[c][CI(s, i, j, k)] = 0.125 * (x[NDI(s, i, j, k)] + x[NDI(s, i + 1, j, k)] +
+                                           x[NDI(s, i + 1, j + 1, k)] + x[NDI(s, i + 1, j + 1, k + 1)] +
+                                           x[NDI(s, i + 1, j, k + 1)] + x[NDI(s, i, j + 1, k)] +
+                                           x[NDI(s, i, j + 1, k + 1)] + x[NDI(s, i, j, k + 1)]);
+  }
+#undef NX
+#undef NY
+#undef NZ
+  return ORC_OK;
+}
+
+/* rounds to the 11 significant digits of an ES18.10E3 edit descriptor */
+static double es18_10(double x) {
+  char buf[64];
+  snprintf(buf, sizeof buf, "%.10E", x);
+  return strtod(buf, NULL);
+}
+
+int orc_wall_dist(orc *s) {
+  const double *X = s->a[ORC_NODES], *Y = X + s->nnode, *Z = Y + s->nnode;
+  size_t nw = 0, cap = 0;
+  double *wx = NULL, *wy = NULL, *wz = NULL;
+  /* src/wall/wall.f90:198-288: node order per no-slip face */
+  for (int f = 0; f < 6; ++f) {
+    if (s->c.bc_id[f] != -5) continue;
+    int im = 1, ix = IMX, jm = 1, jx = JMX, km = 1, kx = KMX;
+    switch (f) {
+      case 0: ix = 1; break;
+      case 1: im = IMX; break;
+      case 2: jx = 1; break;
+      case 3: jm = JMX; break;
+      case 4: kx = 1; break;
+      case 5: km = KMX; break;
+    }
+    for (int k = km; k <= kx; ++k)
+      for (int j = jm; j <= jx; ++j)
+        for (int i = im; i <= ix; ++i) {
+          if (nw == cap) {
+            cap = cap ? 2 * cap : 1024;
+            wx = (double *)realloc(wx, cap * sizeof(double));
+            wy = (double *)realloc(wy, cap * sizeof(double));
+            wz = (double *)realloc(wz, cap * sizeof(double));
+          }
+          wx[nw] = es18_10(X[NDI(s, i, j, k)]);
+          wy[nw] = es18_10(Y[NDI(s, i, j, k)]);
+          wz[nw] = es18_10(Z[NDI(s, i, j, k)]);
+          ++nw;
+        }
+  }
+  double *nd = (double *)malloc(s->nnode * sizeof(double));
+  for (int k = -2; k <= KMX + 3; ++k)
+    for (int j = -2; j <= JMX + 3; ++j)
+      for (int i = -2; i <= IMX + 3; ++i) {
+        size_t q = NDI(s, i, j, k);
+        double best = 1.e+20;
+        for (size_t n = 0; n < nw; ++n) {
+          double cd = sqrt(sq(wx[n] - X[q]) + sq(wy[n] - Y[q]) + sq(wz[n] - Z[q]));
+          best = fmn(best, cd);
+        }
+        nd[q] = best;
+      }
+  double *dist = s->a[ORC_DIST];
+  for (int k = -2; k <= KMX + 2; ++k)
+    for (int j = -2; j <= JMX + 2; ++j)
+      for (int i = -2; i <= IMX + 2; ++i)
+        dist[CI(s, i, j, k)] = 0.125 * (nd[NDI(s, i, j, k)] + nd[NDI(s, i, j + 1, k)] +
+                                        nd[NDI(s, i, j + 1, k + 1)] + nd[NDI(s, i, j, k + 1)] +
+                                        nd[NDI(s, i + 1, j, k + 1)] + nd[NDI(s, i + 1, j, k)] +
+                                        nd[NDI(s, i + 1, j + 1, k)] + nd[NDI(s, i + 1, j + 1, k + 1)]);
+  free(nd); free(wx); free(wy); free(wz);
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------- ghost cells (C2) */
+enum { CP_ANTI, CP_FLAT, CP_SYMM };
+
+/* src/boundary/copy_bc.f90:58-131 */
+static void copy3(orc *s, double *v, int type, int face) {
+  double a1 = 1, a2 = 1, a3 = 0;
+  int i1 = 1, i2 = 2, i3 = 3;
+  if (type == CP_ANTI) { a1 = 1.; i1 = 1; a2 = -1.; i2 = 2; a3 = 0.; i3 = 3; }
+  else if (type == CP_FLAT) { a1 = 1.; i1 = 1; a2 = 1.; i2 = 1; a3 = 0.; i3 = 1; }
+  else { a1 = s->c1; i1 = 1; a2 = s->c2; i2 = 2; a3 = s->c3; i3 = 3; }
+  const int src[3] = {i1, i2, i3};
+  for (int g = 0; g < 3; ++g) {
+    int is = src[g];
+    switch (face) {
+      case 0:
+        for (int k = 1; k <= KMX - 1; ++k) for (int j = 1; j <= JMX - 1; ++j)
+          v[CI(s, -g, j, k)] = (a2 * v[CI(s, is, j, k)] - a3 * v[CI(s, is + 1, j, k)]) / a1;
+        break;
+      case 1:
+        for (int k = 1; k <= KMX - 1; ++k) for (int j = 1; j <= JMX - 1; ++j)
+          v[CI(s, IMX + g, j, k)] = (a2 * v[CI(s, IMX - is, j, k)] - a3 * v[CI(s, IMX - is - 1, j, k)]) / a1;
+        break;
+      case 2:
+        for (int k = 1; k <= KMX - 1; ++k) for (int i = 1; i <= IMX - 1; ++i)
+          v[CI(s, i, -g, k)] = (a2 * v[CI(s, i, is, k)] - a3 * v[CI(s, i, is + 1, k)]) / a1;
+        break;
+      case 3:
+        for (int k = 1; k <= KMX - 1; ++k) for (int i = 1; i <= IMX - 1; ++i)
+          v[CI(s, i, JMX + g, k)] = (a2 * v[CI(s, i, JMX - is, k)] - a3 * v[CI(s, i, JMX - is - 1, k)]) / a1;
+        break;
+      case 4:
+        for (int j = 1; j <= JMX - 1; ++j) for (int i = 1; i <= IMX - 1; ++i)
+          v[CI(s, i, j, -g)] = (a2 * v[CI(s, i, j, is)] - a3 * v[CI(s, i, j, is + 1)]) / a1;
+        break;
+      case 5:
+        for (int j = 1; j <= JMX - 1; ++j) for (int i = 1; i <= IMX - 1; ++i)
+          v[CI(s, i, j, KMX + g)] = (a2 * v[CI(s, i, j, KMX - is)] - a3 * v[CI(s, i, j, KMX - is - 1)]) / a1;
+        break;
+    }
+  }
+}
+
+/* src/boundary/bc_primitive.f90:484-525 */
+static void fixv(orc *s, double *v, double val, int face) {
+  for (int g = 0; g < 3; ++g) {
+    switch (face) {
+      case 0: for (int k = 1; k <= KMX - 1; ++k) for (int j = 1; j <= JMX - 1; ++j) v[CI(s, -g, j, k)] = val; break;
+      case 1: for (int k = 1; k <= KMX - 1; ++k) for (int j = 1; j <= JMX - 1; ++j) v[CI(s, IMX + g, j, k)] = val; break;
+      case 2: for (int k = 1; k <= KMX - 1; ++k) for (int i = 1; i <= IMX - 1; ++i) v[CI(s, i, -g, k)] = val; break;
+      case 3: for (int k = 1; k <= KMX - 1; ++k) for (int i = 1; i <= IMX - 1; ++i) v[CI(s, i, JMX + g, k)] = val; break;
+      case 4: for (int j = 1; j <= JMX - 1; ++j) for (int i = 1; i <= IMX - 1; ++i) v[CI(s, i, j, -g)] = val; break;
+      case 5: for (int j = 1; j <= JMX - 1; ++j) for (int i = 1; i <= IMX - 1; ++i) v[CI(s, i, j, KMX + g)] = val; break;
+    }
+  }
+}
+
+/* Face geometry helper: ghost layer g (0,1,2 -> 0,-1,-2 or imx,imx+1,...),
+ * interior layer m (1,2,3 -> 1,2,3 or imx-1,...), tangential (a,b). */
+static size_t face_cell(const orc *s, int face, int layer_ghost, int a, int b) {
+  /* layer_ghost: >0 interior index counted from the face (1..), <=0 ghost (0,-1,-2) */
+  switch (face) {
+    case 0: return CI(s, layer_ghost, a, b);
+    case 1: return CI(s, IMX - layer_ghost, a, b);
+    case 2: return CI(s, a, layer_ghost, b);
+    case 3: return CI(s, a, JMX - layer_ghost, b);
+    case 4: return CI(s, a, b, layer_ghost);
+    default: return CI(s, a, b, KMX - layer_ghost);
+  }
+}
+static void face_range(const orc *s, int face, int *na, int *nb) {
+  if (face < 2) { *na = JMX - 1; *nb = KMX - 1; }
+  else if (face < 4) { *na = IMX - 1; *nb = KMX - 1; }
+  else { *na = IMX - 1; *nb = JMX - 1; }
+}
+
+/* src/boundary/bc_primitive.f90:1778-1945 */
+static void temp_based_density(orc *s, int face) {
+  double Tw = s->c.fix_twall[face], R = s->c.R_gas, gm = s->c.gm;
+  double *rho = QPL(0), *p = QPL(4);
+  int na, nb; face_range(s, face, &na, &nb);
+  if (Tw < 0.0) {
+    for (int b = 1; b <= nb; ++b) for (int a = 1; a <= na; ++a) {
+      size_t ci = face_cell(s, face, 1, a, b);
+      double stag = (p[ci] / (R * rho[ci])) * (1 + (0.5 * (gm - 1.) * gm * p[ci] / rho[ci]));
+      for (int g = 0; g < 3; ++g) {
+        size_t cg = face_cell(s, face, -g, a, b);
+        rho[cg] = p[cg] / (R * stag);
+      }
+    }
+  } else if (Tw > 1.0) {
+    for (int b = 1; b <= nb; ++b) for (int a = 1; a <= na; ++a)
+      for (int g = 0; g < 3; ++g) {
+        size_t cg = face_cell(s, face, -g, a, b), ci = face_cell(s, face, 1 + g, a, b);
+        rho[cg] = p[cg] / (R * (2 * Tw - (p[ci] / (R * rho[ci]))));
+      }
+  } else {
+    copy3(s, rho, CP_SYMM, face);
+  }
+}
+
+/* src/boundary/bc_primitive.f90:564-643 (Sutherland mu always, Q13) */
+static void set_omega_at_wall(orc *s, int face) {
+  double *rho = QPL(0), *p = QPL(4), *tw = QPL(6), *dist = s->a[ORC_DIST];
+  int na, nb; face_range(s, face, &na, &nb);
+  const orc_case *c = &s->c;
+  for (int l = 1; l <= 3; ++l)
+    for (int b = 1; b <= nb; ++b) for (int a = 1; a <= na; ++a) {
+      size_t c0 = face_cell(s, face, 0, a, b), c1 = face_cell(s, face, 1, a, b);
+      double T_face = 0.5 * ((p[c0] / rho[c0]) + (p[c1] / rho[c1])) / c->R_gas;
+      double mu = c->mu_ref * pow(T_face / c->T_ref, 1.5) * ((c->T_ref + c->S_ref) / (T_face + c->S_ref));
+      double r = 0.5 * (rho[c0] + rho[c1]);
+      double d2 = 2 * dist[c1];
+      tw[face_cell(s, face, 1 - l, a, b)] = 120 * mu / (r * beta1 * (d2 * d2)) - tw[face_cell(s, face, l, a, b)];
+    }
+}
+
+static void turb_copy(orc *s, int type, int face) {
+  if (s->c.turb == ORC_TURB_SST) { copy3(s, QPL(5), type, face); copy3(s, QPL(6), type, face); }
+}
+static void turb_fix(orc *s, int face) {
+  if (s->c.turb == ORC_TURB_SST) {
+    fixv(s, QPL(5), s->c.fix_tk[face], face);
+    fixv(s, QPL(6), s->c.fix_tw[face], face);
+  }
+}
+
+/* face-normal unit vector stored at the boundary face (outward sign applied
+ * by the caller) */
+static void bface_normal(const orc *s, int face, int a, int b, double *nx, double *ny, double *nz) {
+  size_t f;
+  switch (face) {
+    case 0: f = IFI(s, 1, a, b); *nx = s->a[ORC_INX][f]; *ny = s->a[ORC_INY][f]; *nz = s->a[ORC_INZ][f]; break;
+    case 1: f = IFI(s, IMX, a, b); *nx = s->a[ORC_INX][f]; *ny = s->a[ORC_INY][f]; *nz = s->a[ORC_INZ][f]; break;
+    case 2: f = JFI(s, a, 1, b); *nx = s->a[ORC_JNX][f]; *ny = s->a[ORC_JNY][f]; *nz = s->a[ORC_JNZ][f]; break;
+    case 3: f = JFI(s, a, JMX, b); *nx = s->a[ORC_JNX][f]; *ny = s->a[ORC_JNY][f]; *nz = s->a[ORC_JNZ][f]; break;
+    case 4: f = KFI(s, a, b, 1); *nx = s->a[ORC_KNX][f]; *ny = s->a[ORC_KNY][f]; *nz = s->a[ORC_KNZ][f]; break;
+    default: f = KFI(s, a, b, KMX); *nx = s->a[ORC_KNX][f]; *ny = s->a[ORC_KNY][f]; *nz = s->a[ORC_KNZ][f]; break;
+  }
+}
+
+/* src/boundary/bc_primitive.f90:645-1234 */
+static void far_field(orc *s, int face) {
+  const orc_case *c = &s->c;
+  double gm = c->gm;
+  double *rho = QPL(0), *u = QPL(1), *v = QPL(2), *w = QPL(3), *p = QPL(4);
+  int na, nb; face_range(s, face, &na, &nb);
+  double sg = (face % 2 == 0) ? -1.0 : 1.0; /* min faces use -n */
+  int last_positive = 0;
+  for (int b = 1; b <= nb; ++b) for (int a = 1; a <= na; ++a) {
+    size_t ci = face_cell(s, face, 1, a, b), cg = face_cell(s, face, 0, a, b);
+    double fnx, fny, fnz;
+    bface_normal(s, face, a, b, &fnx, &fny, &fnz);
+    double mx = (sg < 0) ? -fnx : fnx, my = (sg < 0) ? -fny : fny, mz = (sg < 0) ? -fnz : fnz;
+    double uu = u[ci], vv = v[ci], ww = w[ci];
+    double uf = c->u_inf, vf = c->v_inf, wf = c->w_inf;
+    double cexp = sqrt(gm * p[ci] / rho[ci]);
+    double cinf = sqrt(gm * c->p_inf / c->rho_inf);
+    double Unexp = uu * mx + vv * my + ww * mz;
+    double Uninf = uf * mx + vf * my + wf * mz;
+    double Rinf = Uninf - 2 * cinf / (gm - 1.);
+    double Rexp = Unexp + 2 * cexp / (gm - 1.);
+    double Unb = 0.5 * (Rexp + Rinf);
+    double Cb = 0.25 * (gm - 1.) * (Rexp - Rinf);
+    if (Unb > 0.) {
+      double vel_diff = Unb - Unexp;
+      u[cg] = u[ci] + vel_diff * mx;
+      v[cg] = v[ci] + vel_diff * my;
+      w[cg] = w[ci] + vel_diff * mz;
+      double sen = p[ci] / pow(rho[ci], gm);
+      rho[cg] = pow(Cb * Cb / (gm * sen), 1. / (gm - 1.));
+      p[cg] = (rho[cg] * Cb * Cb / gm);
+      last_positive = 1;
+    } else {
+      double vel_diff = Unb - Uninf;
+      u[cg] = c->u_inf + vel_diff * mx;
+      v[cg] = c->v_inf + vel_diff * my;
+      w[cg] = c->w_inf + vel_diff * mz;
+      double sen = c->p_inf / pow(c->rho_inf, gm);
+      rho[cg] = pow(Cb * Cb / (gm * sen), 1. / (gm - 1.));
+      p[cg] = (rho[cg] * Cb * Cb / gm);
+      last_positive = 0;
+    }
+  }
+  /* the turbulent ghosts end as set by the last face cell in loop order:
+   * a whole-face flat copy if it was outflow, the fixed values otherwise */
+  if (s->c.turb == ORC_TURB_SST) {
+    if (last_positive) turb_copy(s, CP_FLAT, face); else turb_fix(s, face);
+  }
+  /* qp(-1,:,:,:) = qp(0,:,:,:); qp(-2,:,:,:) = qp(0,:,:,:) on the full plane */
+  for (int l = 0; l < s->nv; ++l) {
+    double *q = QPL(l);
+    for (int g = 1; g <= 2; ++g) {
+      switch (face) {
+        case 0: for (int k = -2; k <= KMX + 2; ++k) for (int j = -2; j <= JMX + 2; ++j) q[CI(s, -g, j, k)] = q[CI(s, 0, j, k)]; break;
+        case 1: for (int k = -2; k <= KMX + 2; ++k) for (int j = -2; j <= JMX + 2; ++j) q[CI(s, IMX + g, j, k)] = q[CI(s, IMX, j, k)]; break;
+        case 2: for (int k = -2; k <= KMX + 2; ++k) for (int i = -2; i <= IMX + 2; ++i) q[CI(s, i, -g, k)] = q[CI(s, i, 0, k)]; break;
+        case 3: for (int k = -2; k <= KMX + 2; ++k) for (int i = -2; i <= IMX + 2; ++i) q[CI(s, i, JMX + g, k)] = q[CI(s, i, JMX, k)]; break;
+        case 4: for (int j = -2; j <= JMX + 2; ++j) for (int i = -2; i <= IMX + 2; ++i) q[CI(s, i, j, -g)] = q[CI(s, i, j, 0)]; break;
+        case 5: for (int j = -2; j <= JMX + 2; ++j) for (int i = -2; i <= IMX + 2; ++i) q[CI(s, i, j, KMX + g)] = q[CI(s, i, j, KMX)]; break;
+      }
+    }
+  }
+}
+
+/* src/boundary/FT_bc.f90:36-106; the j and k faces reflect with the I-face
+ * normal stored at the same index (reference behaviour, kept) */
+static void flow_tangency(orc *s, int face) {
+  double *u = QPL(1), *v = QPL(2), *w = QPL(3);
+  const double *inx = s->a[ORC_INX], *iny = s->a[ORC_INY], *inz = s->a[ORC_INZ];
+  int na, nb; face_range(s, face, &na, &nb);
+  for (int b = 1; b <= nb; ++b) for (int a = 1; a <= na; ++a)
+    for (int l = 1; l <= 3; ++l) {
+      double fnx, fny, fnz;
+      bface_normal(s, face, a, b, &fnx, &fny, &fnz);
+      size_t fi;
+      switch (face) {
+        case 0: fi = IFI(s, 1, a, b); break;
+        case 1: fi = IFI(s, IMX, a, b); break;
+        case 2: fi = IFI(s, a, 1, b); break;
+        case 3: fi = IFI(s, a, JMX, b); break;
+        case 4: fi = IFI(s, a, b, 1); break;
+        default: fi = IFI(s, a, b, KMX); break;
+      }
+      size_t ci = face_cell(s, face, l, a, b), cg = face_cell(s, face, 1 - l, a, b);
+      double dot = u[ci] * fnx + v[ci] * fny + w[ci] * fnz;
+      u[cg] = u[ci] - (2.0 * dot * inx[fi]);
+      v[cg] = v[ci] - (2.0 * dot * iny[fi]);
+      w[cg] = w[ci] - (2.0 * dot * inz[fi]);
+    }
+}
+
+/* src/boundary/bc_primitive.f90:1948-1979 */
+static void periodic_bc(orc *s, int face) {
+  for (int l = 0; l < s->nv; ++l) {
+    double *q = QPL(l);
+    for (int g = 0; g < 3; ++g) {
+      switch (face) {
+        case 0: for (int k = -2; k <= KMX + 2; ++k) for (int j = -2; j <= JMX + 2; ++j) q[CI(s, -2 + g, j, k)] = q[CI(s, IMX - 3 + g, j, k)]; break;
+        case 1: for (int k = -2; k <= KMX + 2; ++k) for (int j = -2; j <= JMX + 2; ++j) q[CI(s, IMX + g, j, k)] = q[CI(s, 1 + g, j, k)]; break;
+        case 2: for (int k = -2; k <= KMX + 2; ++k) for (int i = -2; i <= IMX + 2; ++i) q[CI(s, i, -2 + g, k)] = q[CI(s, i, JMX - 3 + g, k)]; break;
+        case 3: for (int k = -2; k <= KMX + 2; ++k) for (int i = -2; i <= IMX + 2; ++i) q[CI(s, i, JMX + g, k)] = q[CI(s, i, 1 + g, k)]; break;
+        case 4: for (int j = -2; j <= JMX + 2; ++j) for (int i = -2; i <= IMX + 2; ++i) q[CI(s, i, j, -2 + g)] = q[CI(s, i, j, KMX - 3 + g)]; break;
+        case 5: for (int j = -2; j <= JMX + 2; ++j) for (int i = -2; i <= IMX + 2; ++i) q[CI(s, i, j, KMX + g)] = q[CI(s, i, j, 1 + g)]; break;
+      }
+    }
+  }
+}
+
+/* src/boundary/bc_primitive.f90:55-226 */
+static int populate_ghost_primitive(orc *s, int current_iter) {
+  const orc_case *c = &s->c;
+  for (int f = 0; f < 6; ++f) {
+    switch (c->bc_id[f]) {
+      case -1: /* supersonic inlet */
+        if (current_iter <= 2) {
+          fixv(s, QPL(0), c->fix_rho[f], f); fixv(s, QPL(1), c->fix_u[f], f);
+          fixv(s, QPL(2), c->fix_v[f], f); fixv(s, QPL(3), c->fix_w[f], f);
+          fixv(s, QPL(4), c->fix_p[f], f); turb_fix(s, f);
+        }
+        break;
+      case -2: /* supersonic outlet */
+        for (int l = 0; l < 5; ++l) copy3(s, QPL(l), CP_FLAT, f);
+        turb_copy(s, CP_FLAT, f);
+        break;
+      case -3: /* subsonic inlet */
+        if (current_iter <= 2) {
+          fixv(s, QPL(0), c->fix_rho[f], f); fixv(s, QPL(1), c->fix_u[f], f);
+          fixv(s, QPL(2), c->fix_v[f], f); fixv(s, QPL(3), c->fix_w[f], f);
+          turb_fix(s, f);
+        }
+        copy3(s, QPL(4), CP_FLAT, f);
+        break;
+      case -4: /* subsonic outlet */
+        for (int l = 0; l < 4; ++l) copy3(s, QPL(l), CP_FLAT, f);
+        if (current_iter <= 2) fixv(s, QPL(4), c->fix_p[f], f);
+        turb_copy(s, CP_FLAT, f);
+        break;
+      case -5: /* wall */
+        copy3(s, QPL(4), CP_SYMM, f);
+        temp_based_density(s, f);
+        copy3(s, QPL(1), CP_ANTI, f); copy3(s, QPL(2), CP_ANTI, f); copy3(s, QPL(3), CP_ANTI, f);
+        if (c->turb == ORC_TURB_SST) { copy3(s, QPL(5), CP_ANTI, f); set_omega_at_wall(s, f); }
+        break;
+      case -6: /* slip wall */
+        copy3(s, QPL(0), CP_SYMM, f); copy3(s, QPL(4), CP_SYMM, f);
+        turb_copy(s, CP_SYMM, f);
+        flow_tangency(s, f);
+        break;
+      case -7: /* pole */
+        for (int l = 0; l < 5; ++l) copy3(s, QPL(l), CP_FLAT, f);
+        turb_copy(s, CP_FLAT, f);
+        break;
+      case -8: far_field(s, f); break;
+      case -9: periodic_bc(s, f); break;
+      case -11: return ORC_E_CONFIG; /* total-pressure inlet: not restated */
+      default: break; /* interfaces (>=0) and -10 untouched */
+    }
+  }
+  /* edges and corners, src/boundary/bc_primitive.f90:209-224 */
+  for (int l = 0; l < s->nv; ++l) {
+    double *q = QPL(l);
+#define Q(i, j, k) q[CI(s, i, j, k)]
+    for (int i = -2; i <= IMX + 2; ++i) {
+      Q(i, 0, 0) = 0.33 * (Q(i, 1, 1) + Q(i, 0, 1) + Q(i, 1, 0));
+      Q(i, 0, KMX) = 0.33 * (Q(i, 1, KMX - 1) + Q(i, 0, KMX - 1) + Q(i, 1, KMX));
+      Q(i, JMX, 0) = 0.33 * (Q(i, JMX - 1, 1) + Q(i, JMX, 1) + Q(i, JMX - 1, 0));
+      Q(i, JMX, KMX) = 0.33 * (Q(i, JMX - 1, KMX - 1) + Q(i, JMX, KMX - 1) + Q(i, JMX - 1, KMX));
+    }
+    for (int k = -2; k <= KMX + 2; ++k) Q(IMX, 0, k) = 0.33 * (Q(IMX - 1, 1, k) + Q(IMX - 1, 0, k) + Q(IMX, 1, k));
+    for (int k = -2; k <= KMX + 2; ++k) Q(0, 0, k) = 0.33 * (Q(1, 1, k) + Q(1, 0, k) + Q(0, 1, k));
+    for (int k = -2; k <= KMX + 2; ++k) Q(0, JMX, k) = 0.33 * (Q(1, JMX - 1, k) + Q(1, JMX, k) + Q(0, JMX - 1, k));
+    for (int k = -2; k <= KMX + 2; ++k) Q(IMX, JMX, k) = 0.33 * (Q(IMX - 1, JMX - 1, k) + Q(IMX - 1, JMX, k) + Q(IMX, JMX - 1, k));
+    Q(0, 0, 0) = 0.33 * (Q(1, 0, 0) + Q(0, 1, 0) + Q(0, 0, 1));
+    Q(IMX, 0, 0) = 0.33 * (Q(IMX - 1, 0, 0) + Q(IMX, 1, 0) + Q(IMX, 0, 1));
+    Q(0, JMX, 0) = 0.33 * (Q(1, JMX, 0) + Q(0, JMX - 1, 0) + Q(0, JMX, 1));
+    Q(0, 0, KMX) = 0.33 * (Q(1, 0, KMX) + Q(0, 1, KMX) + Q(0, 0, KMX - 1));
+    Q(IMX, JMX, 0) = 0.33 * (Q(IMX - 1, JMX, 0) + Q(IMX, JMX - 1, 0) + Q(IMX, JMX, 1));
+    Q(IMX, 0, KMX) = 0.33 * (Q(IMX - 1, 0, KMX) + Q(IMX, 1, KMX) + Q(IMX, 0, KMX - 1));
+    Q(0, JMX, KMX) = 0.33 * (Q(1, JMX, KMX) + Q(0, JMX - 1, KMX) + Q(0, JMX, KMX - 1));
+    Q(IMX, JMX, KMX) = 0.33 * (Q(IMX - 1, JMX, KMX) + Q(IMX, JMX - 1, KMX) + Q(IMX, JMX, KMX - 1));
+#undef Q
+  }
+  return ORC_OK;
+}
+
+/* ------------------------------------------------ reconstruction (C3, C4) */
+typedef struct { double *L, *R; size_t n; size_t (*idx)(const orc *, int, int, int); } fstates;
+
+static fstates states_of(orc *s, int d) {
+  fstates f;
+  if (d == 0) { f.L = s->a[ORC_XL]; f.R = s->a[ORC_XR]; f.n = s->nxs; f.idx = XSI; }
+  else if (d == 1) { f.L = s->a[ORC_YL]; f.R = s->a[ORC_YR]; f.n = s->nys; f.idx = YSI; }
+  else { f.L = s->a[ORC_ZL]; f.R = s->a[ORC_ZR]; f.n = s->nzs; f.idx = ZSI; }
+  return f;
+}
+
+/* MUSCL, src/face/state/muscl.f90:115-200 */
+static void muscl_dir(orc *s, int d) {
+  fstates fs = states_of(s, d);
+  int ii = d == 0, jj = d == 1, kk = d == 2;
+  double alpha = 2. / 3., phi = 1.0, kappa = 1. / 3., eps = 1e-14;
+  int sw = s->c.lim[d];
+  for (int l = 0; l < s->nv; ++l) {
+    if (l >= 5) sw = s->c.tlim[d];
+    double *q = QPL(l);
+    double *Lp = fs.L + (size_t)l * fs.n, *Rp = fs.R + (size_t)l * fs.n;
+    for (int k = 1 - kk; k <= KMX - 1 + kk; ++k)
+      for (int j = 1 - jj; j <= JMX - 1 + jj; ++j)
+        for (int i = 1 - ii; i <= IMX - 1 + ii; ++i) {
+          double fd = q[CI(s, i + ii, j + jj, k + kk)] - q[CI(s, i, j, k)];
+          double bd = q[CI(s, i, j, k)] - q[CI(s, i - ii, j - jj, k - kk)];
+          double r = fd / (bd + fsign(eps, bd));
+          double psi1 = fmx(0., fmn(fmn(2 * r, alpha * (r - 1.0) + 1.0), 2.));
+          r = bd / (fd + fsign(eps, fd));
+          double psi2 = fmx(0., fmn(fmn(2 * r, alpha * (r - 1.0) + 1.0), 2.));
+          psi1 = (1 - (1 - psi1) * sw);
+          psi2 = (1 - (1 - psi2) * sw);
+          Lp[fs.idx(s, i + ii, j + jj, k + kk)] = q[CI(s, i, j, k)] + 0.25 * phi *
+              (((1. - kappa) * psi1 * bd) + ((1. + kappa) * psi2 * fd));
+          Rp[fs.idx(s, i, j, k)] = q[CI(s, i, j, k)] - 0.25 * phi *
+              (((1. + kappa) * psi1 * bd) + ((1. - kappa) * psi2 * fd));
+        }
+  }
+}
+
+/* WENO5, src/face/state/weno.f90:24-93 */
+static void weno_dir(orc *s, int d) {
+  fstates fs = states_of(s, d);
+  int ii = d == 0, jj = d == 1, kk = d == 2;
+  double g1 = 1.0 / 10.0, g2 = 6.0 / 10.0, g3 = 3.0 / 10.0, eps = 1e-6;
+  for (int l = 0; l < s->nv; ++l) {
+    double *q = QPL(l);
+    double *Lp = fs.L + (size_t)l * fs.n, *Rp = fs.R + (size_t)l * fs.n;
+    for (int k = 1 - kk; k <= KMX - 1 + kk; ++k)
+      for (int j = 1 - jj; j <= JMX - 1 + jj; ++j)
+        for (int i = 1 - ii; i <= IMX - 1 + ii; ++i) {
+          double Um2 = q[CI(s, i - 2 * ii, j - 2 * jj, k - 2 * kk)];
+          double Um1 = q[CI(s, i - ii, j - jj, k - kk)];
+          double U0 = q[CI(s, i, j, k)];
+          double Up1 = q[CI(s, i + ii, j + jj, k + kk)];
+          double Up2 = q[CI(s, i + 2 * ii, j + 2 * jj, k + 2 * kk)];
+          double P1 = (2.0 * Um2 - 7.0 * Um1 + 11.0 * U0) / 6.0;
+          double P2 = (-1.0 * Um1 + 5.0 * U0 + 2.0 * Up1) / 6.0;
+          double P3 = (2.0 * U0 + 5.0 * Up1 - 1.0 * Up2) / 6.0;
+          double B1 = (13.0 / 12.0) * sq(Um2 - 2.0 * Um1 + U0) + (1.0 / 4.0) * sq(Um2 - 4.0 * Um1 + 3.0 * U0);
+          double B2 = (13.0 / 12.0) * sq(Um1 - 2.0 * U0 + Up1) + (1.0 / 4.0) * sq(Um1 - Up1);
+          double B3 = (13.0 / 12.0) * sq(U0 - 2.0 * Up1 + Up2) + (1.0 / 4.0) * sq(3.0 * U0 - 4.0 * Up1 + Up2);
+          double w1 = g1 / sq(eps + B1), w2 = g2 / sq(eps + B2), w3 = g3 / sq(eps + B3);
+          Lp[fs.idx(s, i + ii, j + jj, k + kk)] = (w1 * P1 + w2 * P2 + w3 * P3) / (w1 + w2 + w3);
+          P1 = (2.0 * Up2 - 7.0 * Up1 + 11.0 * U0) / 6.0;
+          P2 = (-1.0 * Up1 + 5.0 * U0 + 2.0 * Um1) / 6.0;
+          P3 = (2.0 * U0 + 5.0 * Um1 - 1.0 * Um2) / 6.0;
+          w1 = g1 / sq(eps + B3); w2 = g2 / sq(eps + B2); w3 = g3 / sq(eps + B1);
+          Rp[fs.idx(s, i, j, k)] = (w1 * P1 + w2 * P2 + w3 * P3) / (w1 + w2 + w3);
+        }
+  }
+}
+
+/* PPM, src/face/state/ppm.f90:21-107 */
+static void ppm_dir(orc *s, int d) {
+  fstates fs = states_of(s, d);
+  int ii = d == 0, jj = d == 1, kk = d == 2;
+  for (int l = 0; l < s->nv; ++l) {
+    double *q = QPL(l);
+    double *Lp = fs.L + (size_t)l * fs.n, *Rp = fs.R + (size_t)l * fs.n;
+    for (int k = 1 - kk; k <= KMX - 1 + 2 * kk; ++k)
+      for (int j = 1 - jj; j <= JMX - 1 + 2 * jj; ++j)
+        for (int i = 1 - ii; i <= IMX - 1 + 2 * ii; ++i) {
+          double v = (7. * (q[CI(s, i, j, k)] + q[CI(s, i - ii, j - jj, k - kk)]) -
+                      (q[CI(s, i + ii, j + jj, k + kk)] + q[CI(s, i - 2 * ii, j - 2 * jj, k - 2 * kk)])) / 12.;
+          Lp[fs.idx(s, i, j, k)] = v;
+          Rp[fs.idx(s, i, j, k)] = v;
+        }
+    if (s->c.lim[d] == 1) {
+      for (int k = 1 - kk; k <= KMX - 1 + kk; ++k)
+        for (int j = 1 - jj; j <= JMX - 1 + jj; ++j)
+          for (int i = 1 - ii; i <= IMX - 1 + ii; ++i) {
+            size_t iL = fs.idx(s, i + ii, j + jj, k + kk), iR = fs.idx(s, i, j, k);
+            double qc = q[CI(s, i, j, k)];
+            if ((Lp[iL] - qc) * (qc - Rp[iR]) <= 0) {
+              Lp[iL] = qc; Rp[iR] = qc;
+            } else {
+              double dqrl = Lp[iL] - Rp[iR];
+              double dq6 = 6. * (qc - 0.5 * (Lp[iL] + Rp[iR]));
+              if (dqrl * dq6 > dqrl * dqrl) Rp[iR] = 3. * qc - 2. * Lp[iL];
+              else if (-dqrl * dqrl > dqrl * dq6) Lp[iL] = 3. * qc - 2. * Rp[iR];
+            }
+          }
+    }
+  }
+}
+
+/* first order, src/face/state/face_interpolant.f90:61-76 */
+static void none_dir(orc *s, int d) {
+  fstates fs = states_of(s, d);
+  int ii = d == 0, jj = d == 1, kk = d == 2;
+  for (int l = 0; l < s->nv; ++l) {
+    double *q = QPL(l);
+    double *Lp = fs.L + (size_t)l * fs.n, *Rp = fs.R + (size_t)l * fs.n;
+    for (int k = 1 - kk; k <= KMX - 1 + 2 * kk; ++k)
+      for (int j = 1 - jj; j <= JMX - 1 + 2 * jj; ++j)
+        for (int i = 1 - ii; i <= IMX - 1 + 2 * ii; ++i) {
+          Lp[fs.idx(s, i, j, k)] = q[CI(s, i - ii, j - jj, k - kk)];
+          Rp[fs.idx(s, i, j, k)] = q[CI(s, i, j, k)];
+        }
+  }
+}
+
+/* src/boundary/boundary_state_reconstruction.f90:78-411 (one face) */
+static void reconstruct_boundary_face(orc *s, int face) {
+  int d = face / 2, hi = face % 2;
+  fstates fs = states_of(s, d);
+  int ii = d == 0, jj = d == 1, kk = d == 2;
+  int n = (d == 0) ? IMX : (d == 1) ? JMX : KMX;
+  double phi = 1.0, kappa = 1. / 3.;
+  int id = s->c.bc_id[face];
+  int na, nb; face_range(s, face, &na, &nb);
+  /* cell index along the sweep: 1 for min, n-1 for max */
+  int cidx = hi ? n - 1 : 1;
+  if (s->ppm_flag) {
+    int sw = s->c.lim[d];
+    for (int l = 0; l < s->nv; ++l) {
+      if (d < 2) { if (l >= 5) sw = s->c.tlim[d]; }
+      else sw = (l >= 5) ? s->c.tlim[d] : s->c.lim[d];
+      double *q = QPL(l);
+      double *Lp = fs.L + (size_t)l * fs.n, *Rp = fs.R + (size_t)l * fs.n;
+      for (int b = 1; b <= nb; ++b) for (int a = 1; a <= na; ++a) {
+        int i, j, k;
+        if (d == 0) { i = cidx; j = a; k = b; } else if (d == 1) { i = a; j = cidx; k = b; } else { i = a; j = b; k = cidx; }
+        if (d == 2 && hi) Rp[fs.idx(s, i, j, k + 1)] = 0.5 * (q[CI(s, i, j, k)] + q[CI(s, i, j, k + 1)]);
+        double fd = q[CI(s, i + ii, j + jj, k + kk)] - q[CI(s, i, j, k)];
+        double bd = q[CI(s, i, j, k)] - q[CI(s, i - ii, j - jj, k - kk)];
+        double r = fd / bd;
+        double psi1 = fmx(0., fmn(fmn(2 * r, (2 + r) / 3.), 2.));
+        psi1 = (1 - (1 - psi1) * sw);
+        r = bd / fd;
+        double psi2 = fmx(0., fmn(fmn(2 * r, (2 + r) / 3.), 2.));
+        psi2 = (1 - (1 - psi2) * sw);
+        Lp[fs.idx(s, i + ii, j + jj, k + kk)] = q[CI(s, i, j, k)] + 0.25 * phi *
+            (((1. - kappa) * psi1 * bd) + ((1. + kappa) * psi2 * fd));
+        Rp[fs.idx(s, i, j, k)] = q[CI(s, i, j, k)] - 0.25 * phi *
+            (((1. + kappa) * psi1 * bd) + ((1. - kappa) * psi2 * fd));
+      }
+    }
+  } else if (d == 2 && hi) {
+    for (int l = 0; l < s->nv; ++l) {
+      double *q = QPL(l);
+      double *Rp = fs.R + (size_t)l * fs.n;
+      for (int b = 1; b <= nb; ++b) for (int a = 1; a <= na; ++a)
+        Rp[fs.idx(s, a, b, KMX)] = 0.5 * (q[CI(s, a, b, KMX - 1)] + q[CI(s, a, b, KMX)]);
+    }
+  }
+  /* face-state overrides at the boundary face */
+  int fidx = hi ? n : 1;     /* face index */
+  int gidx = hi ? n : 0;     /* ghost cell adjacent to the face */
+  int iidx = hi ? n - 1 : 1; /* interior cell adjacent to the face */
+  for (int l = 0; l < s->nv; ++l) {
+    double *q = QPL(l);
+    double *Lp = fs.L + (size_t)l * fs.n, *Rp = fs.R + (size_t)l * fs.n;
+    for (int b = 1; b <= nb; ++b) for (int a = 1; a <= na; ++a) {
+      size_t fi, cg, cin;
+      if (d == 0) { fi = fs.idx(s, fidx, a, b); cg = CI(s, gidx, a, b); cin = CI(s, iidx, a, b); }
+      else if (d == 1) { fi = fs.idx(s, a, fidx, b); cg = CI(s, a, gidx, b); cin = CI(s, a, iidx, b); }
+      else { fi = fs.idx(s, a, b, fidx); cg = CI(s, a, b, gidx); cin = CI(s, a, b, iidx); }
+      if (id == -8 || id == -9) {
+        Lp[fi] = q[cg]; Rp[fi] = q[cg];
+      } else if (!hi) {
+        Lp[fi] = 0.5 * (q[cg] + q[cin]);
+      } else {
+        Rp[fi] = 0.5 * (q[cin] + q[cg]);
+      }
+    }
+  }
+}
+
+static int compute_face_interpolant(orc *s) {
+  for (int d = 0; d < 3; ++d) {
+    switch (s->c.recon) {
+      case ORC_RECON_NONE: none_dir(s, d); break;
+      case ORC_RECON_MUSCL: muscl_dir(s, d); break;
+      case ORC_RECON_PPM: ppm_dir(s, d); break;
+      case ORC_RECON_WENO: weno_dir(s, d); break;
+      default: return ORC_E_CONFIG;
+    }
+  }
+  /* src/boundary/boundary_state_reconstruction.f90:28-75 */
+  if (s->c.recon != ORC_RECON_NONE)
+    for (int f = 0; f < 6; ++f) {
+      int id = s->c.bc_id[f];
+      if (id < 0 && id != -10) reconstruct_boundary_face(s, f);
+    }
+  return ORC_OK;
+}
+
+/* ----------------------------------------------------------- fluxes (C5) */
+typedef struct { double r, u, v, w, p; } prim;
+
+/* convective face flux of one face for the selected scheme; the
+ * turbulent scalars are carried by the split mass flux.  Sources:
+ * src/face/flux/convective/{ausm,ausmP,ausmUP,slau,ldfss0,van_leer}.f90 */
+static void face_flux(const orc *s, const double *qL, const double *qR, double nx, double ny, double nz,
+                      double A, int zero, double *out) {
+  const orc_case *c = &s->c;
+  double gm = c->gm;
+  int nv = s->nv;
+  double rL = qL[0], uL = qL[1], vL = qL[2], wL = qL[3], pL = qL[4];
+  double rR = qR[0], uR = qR[1], vR = qR[2], wR = qR[3], pR = qR[4];
+  double Fp[8], Fm[8];
+  if (c->flux == ORC_FLUX_AUSMP || c->flux == ORC_FLUX_AUSMUP || c->flux == ORC_FLUX_SLAU) {
+    double HL = (0.5 * (uL * uL + vL * vL + wL * wL)) + ((gm / (gm - 1.)) * pL / rL);
+    double HR = (0.5 * (uR * uR + vR * vR + wR * wR)) + ((gm / (gm - 1.)) * pR / rR);
+    double VnL = uL * nx + vL * ny + wL * nz;
+    double VnR = uR * nx + vR * ny + wR * nz;
+    double mass, pbar;
+    if (c->flux == ORC_FLUX_AUSMP) {
+      /* ausmP.f90:105-158 */
+      double cs = sqrt(2.0 * (gm - 1.0) * (0.5 * (HL + HR)) / (gm + 1.0));
+      double cL = cs * cs / (fmx(cs, fabs(VnL)));
+      double cR = cs * cs / (fmx(cs, fabs(VnR)));
+      double C = fmn(cL, cR);
+      double ML = VnL / C, MR = VnR / C;
+      double alphaL = (double)((0 > 1 - (int)floor(fabs(ML))) ? 0 : 1 - (int)floor(fabs(ML)));
+      double alphaR = (double)((0 > 1 - (int)floor(fabs(MR))) ? 0 : 1 - (int)floor(fabs(MR)));
+      double FmL = (0.5 * (1.0 + fsign(1.0, ML)) * (1.0 - alphaL) * ML) + alphaL * 0.25 * sq(1.0 + ML);
+      double betaL = (0.5 * (1.0 + fsign(1.0, ML)) * (1.0 - alphaL)) + alphaL * 0.25 * sq(1.0 + ML) * (2.0 - ML);
+      double FmR = (0.5 * (1.0 - fsign(1.0, MR)) * (1.0 - alphaR) * MR) - alphaR * 0.25 * sq(1.0 - MR);
+      double betaR = (0.5 * (1.0 - fsign(1.0, MR)) * (1.0 - alphaR)) + alphaR * 0.25 * sq(1.0 - MR) * (2.0 + MR);
+      FmL = FmL + alphaL * 0.1250 * sq(ML * ML - 1.0);
+      betaL = betaL + alphaL * 0.1875 * sq(ML * ML - 1.0) * ML;
+      FmR = FmR - alphaR * 0.1250 * sq(MR * MR - 1.0);
+      betaR = betaR - alphaR * 0.1875 * sq(MR * MR - 1.0) * MR;
+      double Mface = FmL + FmR;
+      pbar = betaL * pL + betaR * pR;
+      if (Mface > 0.0) mass = Mface * C * rL; else mass = Mface * C * rR;
+    } else if (c->flux == ORC_FLUX_AUSMUP) {
+      /* ausmUP.f90:113-189 */
+      const double Kp = 0.25, Ku = 0.75, sigma = 1.0;
+      double cs = sqrt(2.0 * (gm - 1.0) * (0.5 * (HL + HR)) / (gm + 1.0));
+      double cL = cs * cs / (fmx(cs, VnL));
+      double cR = cs * cs / (fmx(cs, -VnR));
+      double C = fmn(cL, cR);
+      double ML = VnL / C, MR = VnR / C;
+      double Mb = sqrt(0.5 * ((VnL * VnL) + (VnR * VnR)) / (C * C));
+      double Mo = sqrt(fmn(1.0, fmx(Mb * Mb, c->MInf * c->MInf)));
+      double fna = Mo * (2.0 - Mo);
+      double alfa = 3.0 * (-4.0 + (5.0 * fna * fna)) / 16.0;
+      double alphaL = (double)((0 > 1 - (int)floor(fabs(ML))) ? 0 : 1 - (int)floor(fabs(ML)));
+      double alphaR = (double)((0 > 1 - (int)floor(fabs(MR))) ? 0 : 1 - (int)floor(fabs(MR)));
+      double FmL = (0.5 * (1.0 + fsign(1.0, ML)) * (1.0 - alphaL) * ML) + alphaL * 0.25 * sq(1.0 + ML);
+      double betaL = (0.5 * (1.0 + fsign(1.0, ML)) * (1.0 - alphaL)) + alphaL * 0.25 * sq(1.0 + ML) * (2.0 - ML);
+      double FmR = (0.5 * (1.0 - fsign(1.0, MR)) * (1.0 - alphaR) * MR) - alphaR * 0.25 * sq(1.0 - MR);
+      double betaR = (0.5 * (1.0 - fsign(1.0, MR)) * (1.0 - alphaR)) + alphaR * 0.25 * sq(1.0 - MR) * (2.0 + MR);
+      FmL = FmL + alphaL * 0.125 * sq(ML * ML - 1.0);
+      betaL = betaL + alphaL * alfa * sq(ML * ML - 1.0) * ML;
+      FmR = FmR - alphaR * 0.125 * sq(MR * MR - 1.0);
+      betaR = betaR - alphaR * alfa * sq(MR * MR - 1.0) * MR;
+      double Pu = -Ku * betaL * betaR * (rL + rR) * fna * C * (VnR - VnL);
+      double Mp = -2.0 * Kp * fmx(1.0 - (sigma * Mb * Mb), 0.0) * (pR - pL) / (fna * (rL + rR) * C * C);
+      double Mface = FmL + FmR + Mp;
+      pbar = betaL * pL + betaR * pR + Pu;
+      if (Mface > 0.0) mass = Mface * C * rL; else mass = Mface * C * rR;
+    } else {
+      /* slau.f90:108-161 */
+      double cL = sqrt(gm * pL / rL), cR = sqrt(gm * pR / rR);
+      double C = 0.5 * (cL + cR);
+      double delp = pR - pL;
+      double ML = VnL / C, MR = VnR / C;
+      double alphaL = fmx(0.0, 1.0 - floor(fabs(ML)));
+      double alphaR = fmx(0.0, 1.0 - floor(fabs(MR)));
+      double betaL = (1.0 - alphaL) * 0.5 * (1.0 + fsign(1.0, ML)) + (alphaL) * 0.25 * (2.0 - ML) * sq(ML + 1.0);
+      double betaR = (1.0 - alphaR) * 0.5 * (1.0 - fsign(1.0, MR)) + (alphaR) * 0.25 * (2.0 + MR) * sq(MR - 1.0);
+      double vtface = sqrt(0.5 * ((uL * uL) + (vL * vL) + (wL * wL) + (uR * uR) + (vR * vR) + (wR * wR)));
+      double Mcap = fmn(1.0, vtface / C);
+      double Xi = sq(1.0 - Mcap);
+      double Vnabs = (rL * fabs(VnL) + rR * fabs(VnR)) / (rL + rR);
+      double fnG = -1.0 * fmx(fmn(ML, 0.0), -1.0) * fmn(fmx(MR, 0.0), 1.0);
+      pbar = 0.5 * ((pL + pR) + (betaL - betaR) * (pL - pR) + (1.0 - Xi) * (betaL + betaR - 1.0) * (pL + pR));
+      double VnabsL = (1.0 - fnG) * Vnabs + fnG * fabs(VnL);
+      double VnabsR = (1.0 - fnG) * Vnabs + fnG * fabs(VnR);
+      mass = 0.5 * ((rL * (VnL + VnabsL) + rR * (VnR - VnabsR)) - (Xi * delp / C));
+    }
+    mass = mass * zero;
+    Fp[0] = 0.5 * (mass + fabs(mass));
+    Fp[1] = Fp[0] * uL; Fp[2] = Fp[0] * vL; Fp[3] = Fp[0] * wL; Fp[4] = Fp[0] * HL;
+    Fm[0] = 0.5 * (mass - fabs(mass));
+    Fm[1] = Fm[0] * uR; Fm[2] = Fm[0] * vR; Fm[3] = Fm[0] * wR; Fm[4] = Fm[0] * HR;
+    for (int l = 5; l < nv; ++l) { Fp[l] = Fp[0] * qL[l]; Fm[l] = Fm[0] * qR[l]; }
+    for (int l = 0; l < nv; ++l) out[l] = Fp[l] + Fm[l];
+    out[1] = out[1] + (pbar * nx);
+    out[2] = out[2] + (pbar * ny);
+    out[3] = out[3] + (pbar * nz);
+    for (int l = 0; l < nv; ++l) out[l] = out[l] * A;
+    return;
+  }
+  /* AUSM, LDFSS(0), van Leer: ausm.f90:58-150, ldfss0.f90:60-164, van_leer.f90:56-145 */
+  double ssa = 0.5 * (sqrt(gm * pL / rL) + sqrt(gm * pR / rR));
+  double fns = uL * nx + vL * ny + wL * nz;
+  double MpL = fns / ssa;
+  double alpha_plus = 0.5 * (1.0 + fsign(1.0, MpL));
+  int tb = 1 - (int)floor(fabs(MpL));
+  double beta_left = -(double)(0 > tb ? 0 : tb);
+  double M_plus = 0.25 * sq(1. + MpL);
+  double D_plus = 0.25 * sq(1. + MpL) * (2. - MpL);
+  double c_plus = (alpha_plus * (1.0 + beta_left) * MpL) - beta_left * M_plus;
+  double scrD_plus = (alpha_plus * (1. + beta_left)) - (beta_left * D_plus);
+  fns = uR * nx + vR * ny + wR * nz;
+  double MpR = fns / ssa;
+  double alpha_minus = 0.5 * (1.0 - fsign(1.0, MpR));
+  tb = 1 - (int)floor(fabs(MpR));
+  double beta_right = -(double)(0 > tb ? 0 : tb);
+  double M_minus = -0.25 * sq(1. - MpR);
+  double D_minus = 0.25 * sq(1. - MpR) * (2. + MpR);
+  double c_minus = (alpha_minus * (1.0 + beta_right) * MpR) - beta_right * M_minus;
+  double scrD_minus = (alpha_minus * (1. + beta_right)) - (beta_right * D_minus);
+  if (c->flux == ORC_FLUX_AUSM) {
+    double temp_c = c_plus + c_minus;
+    c_plus = fmx(0., temp_c);
+    c_minus = fmn(0., temp_c);
+  } else if (c->flux == ORC_FLUX_LDFSS0) {
+    double M_ldfss = 0.25 * beta_left * beta_right * sq(sqrt((MpL * MpL + MpR * MpR) * 0.5) - 1);
+    double M_plus_ldfss = M_ldfss * (1 - (pL - pR) / (2 * rL * (ssa * ssa)));
+    double M_minus_ldfss = M_ldfss * (1 - (pL - pR) / (2 * rR * (ssa * ssa)));
+    c_plus = c_plus - M_plus_ldfss;
+    c_minus = c_minus + M_minus_ldfss;
+  }
+  Fp[0] = rL * ssa * c_plus;
+  Fm[0] = rR * ssa * c_minus;
+  Fp[0] = Fp[0] * zero;
+  Fm[0] = Fm[0] * zero;
+  Fp[1] = (Fp[0] * uL) + (scrD_plus * pL * nx);
+  Fp[2] = (Fp[0] * vL) + (scrD_plus * pL * ny);
+  Fp[3] = (Fp[0] * wL) + (scrD_plus * pL * nz);
+  Fp[4] = Fp[0] * ((0.5 * (uL * uL + vL * vL + wL * wL)) + ((gm / (gm - 1.)) * pL / rL));
+  Fm[1] = (Fm[0] * uR) + (scrD_minus * pR * nx);
+  Fm[2] = (Fm[0] * vR) + (scrD_minus * pR * ny);
+  Fm[3] = (Fm[0] * wR) + (scrD_minus * pR * nz);
+  Fm[4] = Fm[0] * ((0.5 * (uR * uR + vR * vR + wR * wR)) + ((gm / (gm - 1.)) * pR / rR));
+  for (int l = 5; l < nv; ++l) { Fp[l] = Fp[0] * qL[l]; Fm[l] = Fm[0] * qR[l]; }
+  for (int l = 0; l < nv; ++l) out[l] = Fp[l] * A + Fm[l] * A;
+}
+
+static int compute_fluxes(orc *s) {
+  int nv = s->nv;
+  double qL[8], qR[8], out[8];
+  for (int d = 0; d < 3; ++d) {
+    fstates fs = states_of(s, d);
+    int ii = d == 0, jj = d == 1, kk = d == 2;
+    double *Fl = d == 0 ? s->a[ORC_F] : d == 1 ? s->a[ORC_G] : s->a[ORC_H];
+    size_t nfl = d == 0 ? s->nF : d == 1 ? s->nG : s->nH;
+    size_t (*fidx)(const orc *, int, int, int) = d == 0 ? FI : d == 1 ? GI : HI;
+    const double *fA, *fx, *fy, *fz;
+    size_t (*gidx)(const orc *, int, int, int) = d == 0 ? IFI : d == 1 ? JFI : KFI;
+    if (d == 0) { fA = s->a[ORC_IA]; fx = s->a[ORC_INX]; fy = s->a[ORC_INY]; fz = s->a[ORC_INZ]; }
+    else if (d == 1) { fA = s->a[ORC_JA]; fx = s->a[ORC_JNX]; fy = s->a[ORC_JNY]; fz = s->a[ORC_JNZ]; }
+    else { fA = s->a[ORC_KA]; fx = s->a[ORC_KNX]; fy = s->a[ORC_KNY]; fz = s->a[ORC_KNZ]; }
+    if (d == 2 && KMX == 2) {
+      for (size_t n = 0; n < nfl * (size_t)nv; ++n) Fl[n] = 0.;
+    } else {
+      for (int k = 1; k <= KMX - 1 + kk; ++k)
+        for (int j = 1; j <= JMX - 1 + jj; ++j)
+          for (int i = 1; i <= IMX - 1 + ii; ++i) {
+            size_t si = fs.idx(s, i, j, k);
+            for (int l = 0; l < nv; ++l) { qL[l] = fs.L[(size_t)l * fs.n + si]; qR[l] = fs.R[(size_t)l * fs.n + si]; }
+            size_t gi = gidx(s, i, j, k);
+            int zero = ii * s->mfz[i] + jj * s->mgz[j] + kk * s->mhz[k];
+            face_flux(s, qL, qR, fx[gi], fy[gi], fz[gi], fA[gi], zero, out);
+            size_t fo = fidx(s, i, j, k);
+            for (int l = 0; l < nv; ++l) Fl[(size_t)l * nfl + fo] = out[l];
+          }
+    }
+    for (size_t n = 0; n < nfl * (size_t)nv; ++n)
+      if (isnan(Fl[n])) return ORC_E_NAN;
+  }
+  return ORC_OK;
+}
+
+/* ----------------------------------------------------- gradients (C6) */
+#define GX(l, i, j, k) s->a[ORC_GX][(size_t)(l) * s->ngr + GRI(s, i, j, k)]
+#define GY(l, i, j, k) s->a[ORC_GY][(size_t)(l) * s->ngr + GRI(s, i, j, k)]
+#define GZ(l, i, j, k) s->a[ORC_GZ][(size_t)(l) * s->ngr + GRI(s, i, j, k)]
+
+/* src/gradients.f90:405-482 */
+static int gradient_G(orc *s, double *grad, const double *var, int dir) {
+  const double *iA = s->a[ORC_IA], *jA = s->a[ORC_JA], *kA = s->a[ORC_KA];
+  const double *in = s->a[ORC_INX + dir], *jn = s->a[ORC_JNX + dir], *kn = s->a[ORC_KNX + dir];
+  const double *vol = s->a[ORC_VOL];
+  for (int k = 0; k <= KMX; ++k)
+    for (int j = 0; j <= JMX; ++j)
+      for (int i = 0; i <= IMX; ++i) {
+        double vc = var[CI(s, i, j, k)];
+        double g = -(var[CI(s, i - 1, j, k)] + vc) * in[IFI(s, i, j, k)] * iA[IFI(s, i, j, k)]
+                   - (var[CI(s, i, j - 1, k)] + vc) * jn[JFI(s, i, j, k)] * jA[JFI(s, i, j, k)]
+                   - (var[CI(s, i, j, k - 1)] + vc) * kn[KFI(s, i, j, k)] * kA[KFI(s, i, j, k)]
+                   + (var[CI(s, i + 1, j, k)] + vc) * in[IFI(s, i + 1, j, k)] * iA[IFI(s, i + 1, j, k)]
+                   + (var[CI(s, i, j + 1, k)] + vc) * jn[JFI(s, i, j + 1, k)] * jA[JFI(s, i, j + 1, k)]
+                   + (var[CI(s, i, j, k + 1)] + vc) * kn[KFI(s, i, j, k + 1)] * kA[KFI(s, i, j, k + 1)];
+        grad[GRI(s, i, j, k)] = g / (2 * vol[CI(s, i, j, k)]);
+      }
+  for (size_t n = 0; n < s->ngr; ++n) if (isnan(grad[n])) return ORC_E_NAN;
+  return ORC_OK;
+}
+
+/* src/gradients.f90:486-676.  apply_gradient_bc_face declares its `faces`
+ * dummy with the I-face extents (-2:imx+3,-2:jmx+2,-2:kmx+2) and is called
+ * with Jfaces and Kfaces too (gradients.f90:559,571,583,595); by sequence
+ * association the J/K metrics are then read at the I-face linear offset.
+ * That is reproduced here: every direction indexes its face arrays with IFI. */
+static void gradient_bc_face(orc *s, int face) {
+  int d = face / 2, hi = face % 2;
+  int id = s->c.bc_id[face];
+  double Tw = s->c.fix_twall[face];
+  int ng = s->ngrad;
+  const double *fA = s->a[d == 0 ? ORC_IA : d == 1 ? ORC_JA : ORC_KA];
+  const double *fx = s->a[d == 0 ? ORC_INX : d == 1 ? ORC_JNX : ORC_KNX];
+  const double *fy = s->a[d == 0 ? ORC_INY : d == 1 ? ORC_JNY : ORC_KNY];
+  const double *fz = s->a[d == 0 ? ORC_INZ : d == 1 ? ORC_JNZ : ORC_KNZ];
+  size_t (*gidx)(const orc *, int, int, int) = IFI;
+  int n = d == 0 ? IMX : d == 1 ? JMX : KMX;
+  int fpos = hi ? n : 1;
+  int sig = hi ? -1 : 1;
+  int na, nb; face_range(s, face, &na, &nb);
+  const double *T = s->a[ORC_TEMP], *vol = s->a[ORC_VOL];
+  for (int b = 1; b <= nb; ++b) for (int a = 1; a <= na; ++a) {
+    int fi_, fj_, fk_;
+    if (d == 0) { fi_ = fpos; fj_ = a; fk_ = b; } else if (d == 1) { fi_ = a; fj_ = fpos; fk_ = b; } else { fi_ = a; fj_ = b; fk_ = fpos; }
+    /* interior I = face - iu, ghost G = face - il */
+    int Ii = fi_, Ij = fj_, Ik = fk_, Gi = fi_, Gj = fj_, Gk = fk_;
+    if (hi) { if (d == 0) Ii--; else if (d == 1) Ij--; else Ik--; }
+    else { if (d == 0) Gi--; else if (d == 1) Gj--; else Gk--; }
+    size_t gf = gidx(s, fi_, fj_, fk_);
+    double nx = fx[gf], ny = fy[gf], nz = fz[gf];
+    double vI = vol[CI(s, Ii, Ij, Ik)];
+    double c_x = fA[gf] * nx / vI, c_y = fA[gf] * ny / vI, c_z = fA[gf] * nz / vI;
+    double T_I = T[CI(s, Ii, Ij, Ik)], T_G = T[CI(s, Gi, Gj, Gk)];
+    for (int l = 0; l < ng; ++l) {
+      double dq = QP(l + 1, Ii, Ij, Ik) - QP(l + 1, Gi, Gj, Gk);
+      GX(l, Gi, Gj, Gk) = sig * dq * c_x;
+      GY(l, Gi, Gj, Gk) = sig * dq * c_y;
+      GZ(l, Gi, Gj, Gk) = sig * dq * c_z;
+    }
+    GX(3, Gi, Gj, Gk) = sig * (T_I - T_G) * c_x;
+    GY(3, Gi, Gj, Gk) = sig * (T_I - T_G) * c_y;
+    GZ(3, Gi, Gj, Gk) = sig * (T_I - T_G) * c_z;
+    if (id == -5 && (Tw < 1. && Tw >= 0.)) {
+      GX(3, Gi, Gj, Gk) = -GX(3, Ii, Ij, Ik);
+      GY(3, Gi, Gj, Gk) = -GY(3, Ii, Ij, Ik);
+      GZ(3, Gi, Gj, Gk) = -GZ(3, Ii, Ij, Ik);
+    }
+    for (int l = 0; l < ng; ++l) {
+      double dot = (GX(l, Ii, Ij, Ik) * nx) + (GY(l, Ii, Ij, Ik) * ny) + (GZ(l, Ii, Ij, Ik) * nz);
+      GX(l, Gi, Gj, Gk) = GX(l, Gi, Gj, Gk) + (GX(l, Ii, Ij, Ik) - dot * nx);
+      GY(l, Gi, Gj, Gk) = GY(l, Gi, Gj, Gk) + (GY(l, Ii, Ij, Ik) - dot * ny);
+      GZ(l, Gi, Gj, Gk) = GZ(l, Gi, Gj, Gk) + (GZ(l, Ii, Ij, Ik) - dot * nz);
+    }
+  }
+}
+
+/* src/gradients.f90:276-402 */
+static int evaluate_all_gradients(orc *s) {
+  double *gxs[3] = {s->a[ORC_GX], s->a[ORC_GY], s->a[ORC_GZ]};
+  const double *vars[6] = {QPL(1), QPL(2), QPL(3), s->a[ORC_TEMP], QPL(5), QPL(6)};
+  int ng = s->ngrad;
+  for (int dir = 0; dir < 3; ++dir) {
+    if (dir == 2 && KMX <= 2) {
+      for (size_t n = 0; n < s->ngr * (size_t)ng; ++n) gxs[2][n] = 0.0;
+      continue;
+    }
+    for (int l = 0; l < ng; ++l) {
+      int st = gradient_G(s, gxs[dir] + (size_t)l * s->ngr, vars[l], dir);
+      if (st) return st;
+    }
+  }
+  for (int f = 0; f < 6; ++f)
+    if (s->c.bc_id[f] < 0) gradient_bc_face(s, f);
+  return ORC_OK;
+}
+
+/* ---------------------------------------------------- viscosity (C7) */
+/* src/viscosity.f90:52-546 (constant / Sutherland mu, SST mu_t and F1) */
+static int calculate_viscosity(orc *s) {
+  const orc_case *c = &s->c;
+  double *mu = s->a[ORC_MU], *mut = s->a[ORC_MUT], *F1 = s->a[ORC_F1], *dist = s->a[ORC_DIST];
+  if (c->mu_sutherland) {
+    for (int k = 0; k <= KMX; ++k) for (int j = 0; j <= JMX; ++j) for (int i = 0; i <= IMX; ++i) {
+      size_t ci = CI(s, i, j, k);
+      double T = QP(4, i, j, k) / (QP(0, i, j, k) * c->R_gas);
+      mu[ci] = c->mu_ref * (pow(T / c->T_ref, 1.5)) * ((c->T_ref + c->S_ref) / (T + c->S_ref));
+    }
+  }
+  if (c->turb == ORC_TURB_SST) {
+    for (int k = 0; k <= KMX; ++k) for (int j = 0; j <= JMX; ++j) for (int i = 0; i <= IMX; ++i) {
+      size_t ci = CI(s, i, j, k);
+      double density = QP(0, i, j, k), tk = QP(5, i, j, k), tw = QP(6, i, j, k);
+      double var1 = sqrt(tk) / (bstar * tw * dist[ci]);
+      double var2 = 500 * (mu[ci] / density) / ((dist[ci] * dist[ci]) * tw);
+      double arg2 = fmx(2 * var1, var2);
+      double F = tanh(arg2 * arg2);
+      double wx = (GY(2, i, j, k) - GZ(1, i, j, k));
+      double wy = (GZ(0, i, j, k) - GX(2, i, j, k));
+      double wz = (GX(1, i, j, k) - GY(0, i, j, k));
+      double wijwij = wx * wx + wy * wy + wz * wz;
+      double vort = sqrt(wijwij);
+      double NUM = density * a1_sst * tk;
+      double DENOM = fmx(fmx((a1_sst * tw), vort * F), 1.e-20);
+      mut[ci] = NUM / DENOM;
+      double CD = fmx(2 * density * sigma_w2 * (GX(4, i, j, k) * GX(5, i, j, k) + GY(4, i, j, k) * GY(5, i, j, k) +
+                                                 GZ(4, i, j, k) * GZ(5, i, j, k)) / tw,
+                      1e-20);
+      double right = 4 * (density * sigma_w2 * tk) / (CD * (dist[ci] * dist[ci]));
+      double left = fmx(var1, var2);
+      double arg1 = fmn(left, right);
+      F1[ci] = tanh(arg1 * arg1 * arg1 * arg1);
+    }
+    for (int f = 0; f < 6; ++f) {
+      int id = c->bc_id[f];
+      if (id >= 0 || id == -10) continue;
+      int anti = (id == -5);
+      if (!(id == -5 || (id <= -1 && id >= -9))) continue;
+      int na, nb; face_range(s, f, &na, &nb);
+      for (int b = 1; b <= nb; ++b) for (int a = 1; a <= na; ++a) {
+        size_t cg = face_cell(s, f, 0, a, b), ci = face_cell(s, f, 1, a, b);
+        mut[cg] = anti ? -mut[ci] : mut[ci];
+        F1[cg] = F1[ci];
+      }
+    }
+  }
+  for (size_t n = 0; n < s->ncell; ++n) if (isnan(mu[n])) return ORC_E_NAN;
+  return ORC_OK;
+}
+
+/* ----------------------------------------------- viscous fluxes (C8) */
+/* src/viscous.f90:144-325 (laminar) and 328-447 (SST) */
+static void viscous_dir(orc *s, int d, int sst) {
+  const orc_case *c = &s->c;
+  int ii = d == 0, jj = d == 1, kk = d == 2;
+  double *Fl = d == 0 ? s->a[ORC_F] : d == 1 ? s->a[ORC_G] : s->a[ORC_H];
+  size_t nfl = d == 0 ? s->nF : d == 1 ? s->nG : s->nH;
+  size_t (*fidx)(const orc *, int, int, int) = d == 0 ? FI : d == 1 ? GI : HI;
+  size_t (*gidx)(const orc *, int, int, int) = d == 0 ? IFI : d == 1 ? JFI : KFI;
+  const double *fA = s->a[d == 0 ? ORC_IA : d == 1 ? ORC_JA : ORC_KA];
+  const double *fx = s->a[d == 0 ? ORC_INX : d == 1 ? ORC_JNX : ORC_KNX];
+  const double *fy = s->a[d == 0 ? ORC_INY : d == 1 ? ORC_JNY : ORC_KNY];
+  const double *fz = s->a[d == 0 ? ORC_INZ : d == 1 ? ORC_JNZ : ORC_KNZ];
+  const double *cx = s->a[ORC_CX], *cy = s->a[ORC_CY], *cz = s->a[ORC_CZ];
+  const double *mu = s->a[ORC_MU], *mut = s->a[ORC_MUT], *F1a = s->a[ORC_F1];
+  int turbulent = (c->turb != ORC_TURB_NONE);
+  for (int k = 1; k <= KMX - 1 + kk; ++k)
+    for (int j = 1; j <= JMX - 1 + jj; ++j)
+      for (int i = 1; i <= IMX - 1 + ii; ++i) {
+        int im = i - ii, jm = j - jj, km = k - kk;
+        size_t cR = CI(s, i, j, k), cL = CI(s, im, jm, km);
+        size_t gf = gidx(s, i, j, k), fo = fidx(s, i, j, k);
+        double delx = cx[cR] - cx[cL], dely = cy[cR] - cy[cL], delz = cz[cR] - cz[cL];
+        double d_LR = sqrt(delx * delx + dely * dely + delz * delz);
+        double nx = fx[gf], ny = fy[gf], nz = fz[gf], area = fA[gf];
+        double mu_f = 0.5 * (mu[cL] + mu[cR]);
+        if (!sst) {
+          double g[4][3];
+          for (int l = 0; l < 4; ++l) {
+            g[l][0] = 0.5 * (GX(l, im, jm, km) + GX(l, i, j, k));
+            g[l][1] = 0.5 * (GY(l, im, jm, km) + GY(l, i, j, k));
+            g[l][2] = 0.5 * (GZ(l, im, jm, km) + GZ(l, i, j, k));
+          }
+          double T_LE = QP(4, im, jm, km) / (QP(0, im, jm, km) * c->R_gas);
+          double T_RE = QP(4, i, j, k) / (QP(0, i, j, k) * c->R_gas);
+          double del[4];
+          del[0] = QP(1, i, j, k) - QP(1, im, jm, km);
+          del[1] = QP(2, i, j, k) - QP(2, im, jm, km);
+          del[2] = QP(3, i, j, k) - QP(3, im, jm, km);
+          del[3] = T_RE - T_LE;
+          for (int l = 0; l < 4; ++l) {
+            double nc = (del[l] - (g[l][0] * delx + g[l][1] * dely + g[l][2] * delz)) / d_LR;
+            g[l][0] = g[l][0] + (nc * delx / d_LR);
+            g[l][1] = g[l][1] + (nc * dely / d_LR);
+            g[l][2] = g[l][2] + (nc * delz / d_LR);
+          }
+          double dudx = g[0][0], dudy = g[0][1], dudz = g[0][2];
+          double dvdx = g[1][0], dvdy = g[1][1], dvdz = g[1][2];
+          double dwdx = g[2][0], dwdy = g[2][1], dwdz = g[2][2];
+          double mut_f = turbulent ? 0.5 * (mut[cL] + mut[cR]) : 0.0;
+          double total_mu = mu_f + mut_f;
+          double Txx = 2. * total_mu * (dudx - ((dudx + dvdy + dwdz) / 3.));
+          double Tyy = 2. * total_mu * (dvdy - ((dudx + dvdy + dwdz) / 3.));
+          double Tzz = 2. * total_mu * (dwdz - ((dudx + dvdy + dwdz) / 3.));
+          double Txy = total_mu * (dvdx + dudy);
+          double Txz = total_mu * (dwdx + dudz);
+          double Tyz = total_mu * (dwdy + dvdz);
+          double K_heat = (mu_f / c->Pr + mut_f / c->tPr) * c->gm * c->R_gas / (c->gm - 1);
+          double Qx = K_heat * g[3][0], Qy = K_heat * g[3][1], Qz = K_heat * g[3][2];
+          double uface = 0.5 * (QP(1, im, jm, km) + QP(1, i, j, k));
+          double vface = 0.5 * (QP(2, im, jm, km) + QP(2, i, j, k));
+          double wface = 0.5 * (QP(3, im, jm, km) + QP(3, i, j, k));
+          Fl[1 * nfl + fo] = Fl[1 * nfl + fo] - ((Txx * nx + Txy * ny + Txz * nz) * area);
+          Fl[2 * nfl + fo] = Fl[2 * nfl + fo] - ((Txy * nx + Tyy * ny + Tyz * nz) * area);
+          Fl[3 * nfl + fo] = Fl[3 * nfl + fo] - ((Txz * nx + Tyz * ny + Tzz * nz) * area);
+          Fl[4 * nfl + fo] = Fl[4 * nfl + fo] - (area * (((Txx * uface + Txy * vface + Txz * wface + Qx) * nx) +
+                                                        ((Txy * uface + Tyy * vface + Tyz * wface + Qy) * ny) +
+                                                        ((Txz * uface + Tyz * vface + Tzz * wface + Qz) * nz)));
+        } else {
+          double g[2][3];
+          for (int l = 0; l < 2; ++l) {
+            g[l][0] = 0.5 * (GX(4 + l, im, jm, km) + GX(4 + l, i, j, k));
+            g[l][1] = 0.5 * (GY(4 + l, im, jm, km) + GY(4 + l, i, j, k));
+            g[l][2] = 0.5 * (GZ(4 + l, im, jm, km) + GZ(4 + l, i, j, k));
+          }
+          double del[2];
+          del[0] = QP(5, i, j, k) - QP(5, im, jm, km);
+          del[1] = QP(6, i, j, k) - QP(6, im, jm, km);
+          for (int l = 0; l < 2; ++l) {
+            double nc = (del[l] - (g[l][0] * delx + g[l][1] * dely + g[l][2] * delz)) / d_LR;
+            g[l][0] = g[l][0] + (nc * delx / d_LR);
+            g[l][1] = g[l][1] + (nc * dely / d_LR);
+            g[l][2] = g[l][2] + (nc * delz / d_LR);
+          }
+          double mut_f = 0.5 * (mut[cL] + mut[cR]);
+          double F1 = 0.5 * (F1a[cL] + F1a[cR]);
+          double sigma_kf = sigma_k1 * F1 + sigma_k2 * (1.0 - F1);
+          double sigma_wf = sigma_w1 * F1 + sigma_w2 * (1.0 - F1);
+          double rhoface = 0.5 * (QP(0, im, jm, km) + QP(0, i, j, k));
+          double tkface = 0.5 * (QP(5, im, jm, km) + QP(5, i, j, k));
+          double Txx = -(2.0 * rhoface * tkface / 3.0);
+          Fl[1 * nfl + fo] = Fl[1 * nfl + fo] - (Txx * nx * area);
+          Fl[2 * nfl + fo] = Fl[2 * nfl + fo] - (Txx * ny * area);
+          Fl[3 * nfl + fo] = Fl[3 * nfl + fo] - (Txx * nz * area);
+          double kflux = area * ((mu_f + sigma_kf * mut_f) * (g[0][0] * nx + g[0][1] * ny + g[0][2] * nz));
+          Fl[4 * nfl + fo] = Fl[4 * nfl + fo] - kflux;
+          Fl[5 * nfl + fo] = Fl[5 * nfl + fo] - kflux;
+          Fl[6 * nfl + fo] = Fl[6 * nfl + fo] - (area * ((mu_f + sigma_wf * mut_f) * (g[1][0] * nx + g[1][1] * ny + g[1][2] * nz)));
+        }
+      }
+}
+
+static int compute_viscous_fluxes(orc *s) {
+  for (int d = 0; d < 3; ++d) viscous_dir(s, d, 0);
+  if (s->c.turb == ORC_TURB_SST) {
+    viscous_dir(s, 0, 1);
+    viscous_dir(s, 1, 1);
+    if (KMX != 2) viscous_dir(s, 2, 1);
+  }
+  size_t nn[3] = {s->nF, s->nG, s->nH};
+  double *arr[3] = {s->a[ORC_F], s->a[ORC_G], s->a[ORC_H]};
+  for (int d = 0; d < 3; ++d)
+    for (size_t n = 0; n < nn[d] * (size_t)s->nv; ++n) if (isnan(arr[d][n])) return ORC_E_NAN;
+  return ORC_OK;
+}
+
+/* ---------------------------------------------- residue + source (C9, C10) */
+/* src/face/flux/convective/scheme.f90:111-141 */
+static void compute_residue(orc *s) {
+  double *F = s->a[ORC_F], *G = s->a[ORC_G], *H = s->a[ORC_H], *R = s->a[ORC_RES];
+  for (int l = 0; l < s->nv; ++l)
+    for (int k = 1; k <= KMX - 1; ++k) for (int j = 1; j <= JMX - 1; ++j) for (int i = 1; i <= IMX - 1; ++i)
+      R[(size_t)l * s->nR + RI(s, i, j, k)] =
+          (F[(size_t)l * s->nF + FI(s, i + 1, j, k)] - F[(size_t)l * s->nF + FI(s, i, j, k)]) +
+          (G[(size_t)l * s->nG + GI(s, i, j + 1, k)] - G[(size_t)l * s->nG + GI(s, i, j, k)]) +
+          (H[(size_t)l * s->nH + HI(s, i, j, k + 1)] - H[(size_t)l * s->nH + HI(s, i, j, k)]);
+}
+
+/* src/source.f90:158-270 (sst; the sst2003 constants are not restated) */
+static void add_sst_source(orc *s) {
+  double *R = s->a[ORC_RES], *mut = s->a[ORC_MUT], *F1a = s->a[ORC_F1], *vol = s->a[ORC_VOL];
+  double gama1 = (beta1 / bstar) - ((sigma_w1 * (kappa_sst * kappa_sst)) / sqrt(bstar));
+  double gama2 = (beta2 / bstar) - ((sigma_w2 * (kappa_sst * kappa_sst)) / sqrt(bstar));
+  int limiter = 20;
+  for (int k = 1; k <= KMX - 1; ++k) for (int j = 1; j <= JMX - 1; ++j) for (int i = 1; i <= IMX - 1; ++i) {
+    size_t ci = CI(s, i, j, k);
+    double density = QP(0, i, j, k), tk = QP(5, i, j, k), tw = QP(6, i, j, k);
+    double vort = sqrt(sq(GY(2, i, j, k) - GZ(1, i, j, k)) + sq(GZ(0, i, j, k) - GX(2, i, j, k)) +
+                       sq(GX(1, i, j, k) - GY(0, i, j, k)));
+    double CD = 2 * density * sigma_w2 * (GX(4, i, j, k) * GX(5, i, j, k) + GY(4, i, j, k) * GY(5, i, j, k) +
+                                          GZ(4, i, j, k) * GZ(5, i, j, k)) / tw;
+    CD = fmx(CD, 1e-20);
+    double F1 = F1a[ci];
+    double gama = gama1 * F1 + gama2 * (1. - F1);
+    double beta = beta1 * F1 + beta2 * (1. - F1);
+    double D_k = bstar * density * tw * tk;
+    double D_w = beta * density * (tw * tw);
+    double divergence = GX(0, i, j, k) + GY(1, i, j, k) + GZ(2, i, j, k);
+    double P_k = mut[ci] * (vort * vort) - ((2.0 / 3.0) * density * tk * divergence);
+    P_k = fmn(P_k, limiter * D_k);
+    double P_w = (density * gama / mut[ci]) * P_k;
+    double lamda = (1. - F1) * CD;
+    double S_k = P_k - D_k;
+    double S_w = P_w - D_w + lamda;
+    S_k = S_k * vol[ci];
+    S_w = S_w * vol[ci];
+    R[5 * s->nR + RI(s, i, j, k)] = R[5 * s->nR + RI(s, i, j, k)] - S_k;
+    R[6 * s->nR + RI(s, i, j, k)] = R[6 * s->nR + RI(s, i, j, k)] - S_w;
+  }
+}
+
+void orc_temp_refresh(orc *s) {
+  double *T = s->a[ORC_TEMP];
+  for (size_t n = 0; n < s->ncell; ++n) T[n] = QPL(4)[n] / (s->c.R_gas * QPL(0)[n]);
+}
+
+int orc_residual(orc *s, int current_iter) {
+  int st;
+  if ((st = populate_ghost_primitive(s, current_iter))) return st;
+  if ((st = compute_face_interpolant(s))) return st;
+  if ((st = compute_fluxes(s))) return st;
+  if (s->c.mu_ref != 0.0) {
+    if ((st = evaluate_all_gradients(s))) return st;
+    if ((st = calculate_viscosity(s))) return st;
+    if ((st = compute_viscous_fluxes(s))) return st;
+  }
+  compute_residue(s);
+  if (s->c.turb == ORC_TURB_SST) add_sst_source(s);
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------- time step (D) */
+static double sspeed(const orc *s, const fstates *fs, size_t si) {
+  double gm = s->c.gm;
+  return 0.5 * (sqrt(gm * fs->L[4 * fs->n + si] / fs->L[si]) + sqrt(gm * fs->R[4 * fs->n + si] / fs->R[si]));
+}
+
+/* src/time.f90:122-246 and 366-531 */
+static void local_time_step(orc *s) {
+  const orc_case *c = &s->c;
+  fstates xs = states_of(s, 0), ys = states_of(s, 1), zs = states_of(s, 2);
+  double *dt = s->a[ORC_DT], *vol = s->a[ORC_VOL];
+  const double *iA = s->a[ORC_IA], *ix = s->a[ORC_INX], *iy = s->a[ORC_INY], *iz = s->a[ORC_INZ];
+  const double *jA = s->a[ORC_JA], *jx = s->a[ORC_JNX], *jy = s->a[ORC_JNY], *jz = s->a[ORC_JNZ];
+  const double *kA = s->a[ORC_KA], *kx = s->a[ORC_KNX], *ky = s->a[ORC_KNY], *kz = s->a[ORC_KNZ];
+  for (int k = 1; k <= KMX - 1; ++k) for (int j = 1; j <= JMX - 1; ++j) for (int i = 1; i <= IMX - 1; ++i) {
+    double xsa = sspeed(s, &xs, XSI(s, i, j, k)), ysa = sspeed(s, &ys, YSI(s, i, j, k)), zsa = sspeed(s, &zs, ZSI(s, i, j, k));
+    size_t f1 = IFI(s, i, j, k), f2 = JFI(s, i, j, k), f3 = KFI(s, i, j, k);
+    size_t f4 = IFI(s, i + 1, j, k), f5 = JFI(s, i, j + 1, k), f6 = KFI(s, i, j, k + 1);
+    double l1 = fabs((QP(1, i, j, k) * ix[f1]) + (QP(2, i, j, k) * iy[f1]) + (QP(3, i, j, k) * iz[f1])) + xsa;
+    double l2 = fabs((QP(1, i, j, k) * jx[f2]) + (QP(2, i, j, k) * jy[f2]) + (QP(3, i, j, k) * jz[f2])) + ysa;
+    double l3 = fabs((QP(1, i, j, k) * kx[f3]) + (QP(2, i, j, k) * ky[f3]) + (QP(3, i, j, k) * kz[f3])) + zsa;
+    xsa = sspeed(s, &xs, XSI(s, i + 1, j, k)); ysa = sspeed(s, &ys, YSI(s, i, j + 1, k)); zsa = sspeed(s, &zs, ZSI(s, i, j, k + 1));
+    double l4 = fabs((QP(1, i + 1, j, k) * ix[f4]) + (QP(2, i + 1, j, k) * iy[f4]) + (QP(3, i + 1, j, k) * iz[f4])) + xsa;
+    double l5 = fabs((QP(1, i, j + 1, k) * jx[f5]) + (QP(2, i, j + 1, k) * jy[f5]) + (QP(3, i, j + 1, k) * jz[f5])) + ysa;
+    double l6 = fabs((QP(1, i, j, k + 1) * kx[f6]) + (QP(2, i, j, k + 1) * ky[f6]) + (QP(3, i, j, k + 1) * kz[f6])) + zsa;
+    double lsum = (iA[f1] * l1) + (jA[f2] * l2) + (kA[f3] * l3) + (iA[f4] * l4) + (jA[f5] * l5) + (kA[f6] * l6);
+    size_t r = RI(s, i, j, k);
+    dt[r] = 1. / lsum;
+    dt[r] = dt[r] * vol[CI(s, i, j, k)] * c->CFL;
+  }
+  if (c->mu_ref == 0.0) return;
+  const double *cx = s->a[ORC_CX], *cy = s->a[ORC_CY], *cz = s->a[ORC_CZ];
+  for (int pass = 0; pass < 2; ++pass) {
+    if (pass == 1 && c->turb == ORC_TURB_NONE) break;
+    const double *m = pass == 0 ? s->a[ORC_MU] : s->a[ORC_MUT];
+    double prn = pass == 0 ? c->Pr : c->tPr;
+    for (int k = 1; k <= KMX - 1; ++k) for (int j = 1; j <= JMX - 1; ++j) for (int i = 1; i <= IMX - 1; ++i) {
+      size_t cc = CI(s, i, j, k);
+      size_t c1 = CI(s, i - 1, j, k), c2 = CI(s, i, j - 1, k), c3 = CI(s, i, j, k - 1);
+      size_t c4 = CI(s, i + 1, j, k), c5 = CI(s, i, j + 1, k), c6 = CI(s, i, j, k + 1);
+      size_t f1 = IFI(s, i, j, k), f2 = JFI(s, i, j, k), f3 = KFI(s, i, j, k);
+      size_t f4 = IFI(s, i + 1, j, k), f5 = JFI(s, i, j + 1, k), f6 = KFI(s, i, j, k + 1);
+      double rho = QPL(0)[cc];
+      double l1 = m[cc] / (rho * fabs(((cx[c1] - cx[cc]) * ix[f1]) + ((cy[c1] - cy[cc]) * iy[f1]) + ((cz[c1] - cz[cc]) * iz[f1])));
+      double l2 = m[cc] / (rho * fabs(((cx[c2] - cx[cc]) * jx[f2]) + ((cy[c2] - cy[cc]) * jy[f2]) + ((cz[c2] - cz[cc]) * jz[f2])));
+      double l3 = m[cc] / (rho * fabs(((cx[c3] - cx[cc]) * kx[f3]) + ((cy[c3] - cy[cc]) * ky[f3]) + ((cz[c3] - cz[cc]) * kz[f3])));
+      double l4 = m[c4] / (QPL(0)[c4] * fabs(((cx[cc] - cx[c4]) * ix[f4]) + ((cy[cc] - cy[c4]) * iy[f4]) + ((cz[cc] - cz[c4]) * iz[f4])));
+      double l5 = m[c5] / (QPL(0)[c5] * fabs(((cx[cc] - cx[c5]) * jx[f5]) + ((cy[cc] - cy[c5]) * jy[f5]) + ((cz[cc] - cz[c5]) * jz[f5])));
+      double l6 = m[c6] / (QPL(0)[c6] * fabs(((cx[cc] - cx[c6]) * kx[f6]) + ((cy[cc] - cy[c6]) * ky[f6]) + ((cz[cc] - cz[c6]) * kz[f6])));
+      double lsum = (iA[f1] * l1) + (jA[f2] * l2) + (kA[f3] * l3) + (iA[f4] * l4) + (jA[f5] * l5) + (kA[f6] * l6);
+      lsum = c->gm * lsum / prn;
+      size_t r = RI(s, i, j, k);
+      lsum = 2. / (lsum + (2. * c->CFL * vol[cc] / dt[r]));
+      dt[r] = c->CFL * (lsum * vol[cc]);
+    }
+  }
+}
+
+int orc_time_step(orc *s) {
+  double *dt = s->a[ORC_DT];
+  if (s->c.time_local) { local_time_step(s); return ORC_OK; }
+  if (s->c.global_time_step > 0) {
+    for (size_t n = 0; n < s->nR; ++n) dt[n] = s->c.global_time_step;
+    return ORC_OK;
+  }
+  local_time_step(s);
+  double m = dt[0];
+  for (size_t n = 1; n < s->nR; ++n) m = fmn(m, dt[n]) ; /* minval */
+  for (size_t n = 0; n < s->nR; ++n) dt[n] = m;
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------ update (E, a17) */
+/* src/update.f90:367-485 with the conservative branch */
+static int update_with(orc *s, double TF, double SF, int TU, int use_store) {
+  const orc_case *c = &s->c;
+  int nv = s->nv;
+  double gm = c->gm;
+  double *dt = s->a[ORC_DT], *vol = s->a[ORC_VOL], *F1a = s->a[ORC_F1];
+  double *Rs = s->a[ORC_RSTORE], *res = s->a[ORC_RES];
+  const double *Qu = use_store ? s->a[ORC_USTORE] : s->a[ORC_QP];
+  int status = ORC_OK;
+  for (int k = 1; k <= KMX - 1; ++k) for (int j = 1; j <= JMX - 1; ++j) for (int i = 1; i <= IMX - 1; ++i) {
+    size_t ci = CI(s, i, j, k), ri = RI(s, i, j, k);
+    double u1[8], u2[8], R[8];
+    u1[0] = Qu[ci];
+    for (int l = 1; l < nv; ++l) u1[l] = Qu[(size_t)l * s->ncell + ci] * u1[0];
+    double KE = 0.;
+    u1[4] = (u1[4] / (gm - 1.) + 0.5 * (u1[1] * u1[1] + u1[2] * u1[2] + u1[3] * u1[3])) / u1[0] + KE;
+    for (int l = 0; l < nv; ++l) R[l] = res[(size_t)l * s->nR + ri];
+    if (c->turb == ORC_TURB_SST) {
+      double beta = beta1 * F1a[ci] + (1. - F1a[ci]) * beta2;
+      R[5] = R[5] / (1 + (beta * QP(6, i, j, k) * dt[ri]));
+      R[6] = R[6] / (1 + (2 * beta * QP(6, i, j, k) * dt[ri]));
+    }
+    if (use_store) {
+      for (int l = 0; l < nv; ++l) Rs[(size_t)l * s->nR + ri] = Rs[(size_t)l * s->nR + ri] + SF * R[l];
+      if (TU) for (int l = 0; l < nv; ++l) R[l] = Rs[(size_t)l * s->nR + ri];
+    }
+    for (int l = 0; l < nv; ++l) u2[l] = u1[l] - R[l] * (TF * dt[ri] / vol[ci]);
+    for (int l = 1; l < nv; ++l) u2[l] = u2[l] / u2[0];
+    u2[4] = (gm - 1.) * u2[0] * (u2[4] - (0.5 * (u2[1] * u2[1] + u2[2] * u2[2] + u2[3] * u2[3])) - KE);
+    int bad = (u2[0] < 0.) || (u2[4] < 0.);
+    for (int l = 0; l < nv; ++l) if (isnan(u2[l])) bad = 1;
+    if (bad) { status = ORC_E_NONPHYSICAL; continue; }
+    for (int l = 0; l < 5; ++l) QP(l, i, j, k) = u2[l];
+    if (c->turb == ORC_TURB_SST) {
+      if (u2[5] >= 0.) QP(5, i, j, k) = u2[5];
+      if (u2[6] >= 0.) QP(6, i, j, k) = u2[6];
+    }
+  }
+  return status;
+}
+
+/* src/update.f90:129-226 (RK4, RK2 and first-order 'none' branches) */
+int orc_step(orc *s, int current_iter) {
+  int st;
+  orc_temp_refresh(s);
+  int ta = s->c.time_accuracy;
+  if (ta == ORC_TA_NONE) {
+    if ((st = orc_residual(s, current_iter))) return st;
+    if ((st = orc_time_step(s))) return st;
+    return update_with(s, 1., 1., 0, 0);
+  }
+  size_t nq = s->ncell * (size_t)s->nv;
+  for (size_t n = 0; n < s->nR * (size_t)s->nv; ++n) s->a[ORC_RSTORE][n] = 0.;
+  memcpy(s->a[ORC_USTORE], s->a[ORC_QP], nq * sizeof(double));
+  if (ta == ORC_TA_RK4) {
+    const double TFs[4] = {0.5, 0.5, 1.0, 1. / 6.};
+    const double SFs[4] = {1., 2., 2., 1.};
+    for (int stage = 0; stage < 4; ++stage) {
+      if ((st = orc_residual(s, current_iter))) return st;
+      if (stage == 0 && (st = orc_time_step(s))) return st;
+      if ((st = update_with(s, TFs[stage], SFs[stage], stage == 3, 1))) return st;
+    }
+    return ORC_OK;
+  }
+  if (ta == ORC_TA_RK2) {
+    if ((st = orc_residual(s, current_iter))) return st;
+    if ((st = orc_time_step(s))) return st;
+    if ((st = update_with(s, 0.5, 1., 0, 1))) return st;
+    if ((st = orc_residual(s, current_iter))) return st;
+    return update_with(s, 0.5, 1., 1, 1);
+  }
+  return ORC_E_CONFIG;
+}
+
+/* src/resnorm.f90:128-225 */
+void orc_resnorm(orc *s, double *out) {
+  const orc_case *c = &s->c;
+  double scale[8];
+  scale[0] = 1.;
+  scale[1] = c->rho_inf * c->vel_mag;
+  scale[2] = scale[3] = scale[4] = c->rho_inf * c->vel_mag * c->vel_mag;
+  scale[5] = (0.5 * c->rho_inf * (c->vel_mag * c->vel_mag * c->vel_mag) + ((c->gm / (c->gm - 1.)) * c->p_inf));
+  scale[6] = c->rho_inf * c->vel_mag * c->tk_inf;
+  scale[7] = c->rho_inf * c->vel_mag * c->tw_inf;
+  double *R = s->a[ORC_RES];
+  for (int l = 0; l < s->nv; ++l) {
+    double sum = 0.;
+    for (size_t n = 0; n < s->nR; ++n) sum += R[(size_t)l * s->nR + n] * R[(size_t)l * s->nR + n];
+    out[l + 1] = sqrt(sum / (scale[l + 1] * scale[l + 1]));
+  }
+  double *F = s->a[ORC_F], *G = s->a[ORC_G], *H = s->a[ORC_H];
+  double sF1 = 0, sFn = 0, sG1 = 0, sGn = 0, sH1 = 0, sHn = 0;
+  for (int k = 1; k <= KMX - 1; ++k) for (int j = 1; j <= JMX - 1; ++j) { sF1 += F[FI(s, 1, j, k)]; sFn += F[FI(s, IMX, j, k)]; }
+  for (int k = 1; k <= KMX - 1; ++k) for (int i = 1; i <= IMX - 1; ++i) { sG1 += G[GI(s, i, 1, k)]; sGn += G[GI(s, i, JMX, k)]; }
+  for (int j = 1; j <= JMX - 1; ++j) for (int i = 1; i <= IMX - 1; ++i) { sH1 += H[HI(s, i, j, 1)]; sHn += H[HI(s, i, j, KMX)]; }
+  double merror = (sF1 - sFn + sG1 - sGn + sH1 - sHn);
+  out[0] = fabs(merror / scale[0]);
+}
