@@ -1,0 +1,111 @@
+"""Benchmark of the explicit residual + RK4 stage update on one MI355X.
+
+Workload: the 256x128x128-cell (4.2M) RANS SST channel over a 0.05L bump,
+WENO5 + SLAU, tanh wall clustering to 1e-5L, M=0.2, Re=1e6/L, seeded 0.5%
+noise (festgpu.cases.sst_bump).  A step is one get_next_solution call: four
+RK stages, each a full residual evaluation plus the stage update, with the
+stage-1 time step.  Metric: cell-RK-stages per second, with the inputs
+resident in HBM before the timed region.
+
+The roofline object prices one RK stage at the algorithmic 406 bytes per cell
+(stored FP64 metrics, n_var=7): q read and write, the RK base, the R
+accumulator, three owned faces x {A,nx,ny,nz}, volume, time step, cell
+centres, stale temperature and wall distance, averaged over the four stages.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fest-3d_amd"))
+
+BYTES_PER_CELL_STAGE = {"sst": 406.0, "none": 326.0}
+HBM_PEAK_GBPS = 8000.0
+
+
+def cpu_baseline(case_fn, full_dist, budget_s=25.0):
+    """The oracle (a single-threaded C restatement of the reference) timed on
+    the host on a bounded sample: RK4 steps of the first 64 i-cells of the
+    same grid, wall distance taken from the device.  Checker code, used here
+    only as the baseline."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from binding import Oracle
+    case, nodes, q = case_fn(64, 128, 128)
+    o = Oracle(case)
+    o.set_nodes(nodes)
+    d = o.arr("dist")
+    d[...] = full_dist[:, :, : d.shape[2]]
+    o.set_state(q)
+    t0 = time.perf_counter()
+    steps = 0
+    while True:
+        assert o.step(steps + 1) == 0
+        steps += 1
+        if time.perf_counter() - t0 > budget_s or steps >= 4:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": case.ncells * 4 * steps / dt, "unit": "cell-RK-stages/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} RK4 step(s) of the 64x128x128-cell slab of the same workload "
+                      f"({case.ncells} cells), oracle/fest_oracle.c single thread, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="sst_bump")
+    ap.add_argument("--size", default="256x128x128")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is a design, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from festgpu import cases as C
+    from festgpu.native import GpuBlock
+
+    ni, nj, nk = (int(v) for v in args.size.split("x"))
+    case_fn = C.WORKLOADS[args.workload]
+    case, nodes, q = case_fn(ni, nj, nk)
+    g = GpuBlock(case)
+    g.set_grid(nodes, wall_dist=(case.turb == "sst"))
+    g.upload_state(q)
+    it = 1
+    for _ in range(args.warmup):
+        g.get_next_solution(it)
+        it += 1
+    g.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g.get_next_solution(it)
+        it += 1
+    g.sync()
+    el = time.perf_counter() - t0
+    stages = 4 * args.steps
+    value = case.ncells * stages / el
+    ms_step = 1e3 * el / args.steps
+    bpcs = BYTES_PER_CELL_STAGE[case.turb]
+    achieved = bpcs * case.ncells / (el / stages) / 1e9
+    out = {
+        "metric": "cell-RK-stages/s", "value": value, "unit": "cell-RK-stages/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": case.name, "cells": case.ncells, "scheme": f"{case.recon}+{case.flux}",
+                   "turbulence": case.turb, "time": "RK4 local" if case.time_local else "RK4 global",
+                   "precision": "fp64"},
+        "roofline": {"bound": "hbm", "kernel": "rk_stage", "achieved": achieved, "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "bytes_per_cell_stage": bpcs},
+    }
+    if not args.no_cpu_baseline:
+        dist = g.download("dist") if case.turb == "sst" else None
+        out["cpu_baseline"] = cpu_baseline(case_fn, dist)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,89 @@
+// blk.h -- the device-resident block: extents, pitches, array pointers and
+// the scheme/flow/boundary constants, passed by value to every kernel.
+//
+// HBM layout (structure of arrays, i fastest, one plane per variable):
+//   padded cell arrays  (-2:imx+2,-2:jmx+2,-2:kmx+2)   pitch pi=imx+5, pj=jmx+5
+//   I / J / K face arrays (-2:imx+3,..), (..,-2:jmx+3,..), (..,-2:kmx+3)
+//   F / G / H            (1:imx,1:jmx-1,1:kmx-1), (1:imx-1,1:jmx,..), (..,1:kmx)
+//   residue, R_store, delta_t (1:imx-1,1:jmx-1,1:kmx-1)
+//   gradients            (0:imx,0:jmx,0:kmx) x n_grad per direction
+// These are the reference's own extents (src/update.f90:143-165), so the
+// host<->device copies are plain memcpy and index mapping is exact.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fest {
+
+struct Blk {
+  int imx, jmx, kmx, nv, ng;
+  int pi, pj, pk;
+  long nc;                    // cells per padded plane
+  int fpi[3], fpj[3], fpk[3]; // face array pitches per direction
+  long nfc[3];                // face array sizes
+  int nfi[3], nfj[3], nfk[3]; // F/G/H extents
+  long nfl[3];                // F/G/H plane sizes
+  long ngr;                   // gradient plane size
+  long nres;                  // residue plane size
+  double *qp, *temp, *mu, *mut, *f1, *dist;
+  double *vol, *cx, *cy, *cz;
+  double *fA[3], *fx[3], *fy[3], *fz[3];
+  double *flux[3];
+  double *csnd[3];            // stage-1 face sound speed (time step)
+  double *res, *dt, *ust, *rst;
+  double *g[3];
+  double *nodes;              // grid nodes (3 planes) when set on device
+  double *red;                // reduction partials
+  double *norms;              // n_var+1 absolute norms
+  double *dtmin;              // global time step
+  unsigned *err;              // error bits
+  // scheme
+  int flux_scheme, recon, turb, ppm_flag, time_local, time_acc, accur, mu_suth;
+  int lim[3], tlim[3];
+  int bc[6];
+  int mz[3][2];               // make_*_flux_zero at the first and last face
+  // flow
+  double gm, R, mu_ref, T_ref, S_ref, Pr, tPr, MInf, vel_mag;
+  double rho_inf, u_inf, v_inf, w_inf, p_inf, tk_inf, tw_inf;
+  double c1, c2, c3;
+  double fix[8][6];           // rho,u,v,w,p,tk,tw,wall temperature
+  double global_dt;
+  double CFLv;                // CFL of the current call
+};
+
+enum { FX_RHO = 0, FX_U, FX_V, FX_W, FX_P, FX_TK, FX_TW, FX_TWALL };
+enum : unsigned { ERR_NONPHYS = 1u, ERR_NAN_FLUX = 2u, ERR_NAN_GRAD = 4u, ERR_NAN_MU = 8u, ERR_CONFIG = 16u };
+
+__host__ __device__ inline long ci(const Blk &b, int i, int j, int k) {
+  return (long)(i + 2) + (long)b.pi * ((long)(j + 2) + (long)b.pj * (long)(k + 2));
+}
+// face array index of direction d
+__host__ __device__ inline long fi(const Blk &b, int d, int i, int j, int k) {
+  return (long)(i + 2) + (long)b.fpi[d] * ((long)(j + 2) + (long)b.fpj[d] * (long)(k + 2));
+}
+// F/G/H index of direction d
+__host__ __device__ inline long fli(const Blk &b, int d, int i, int j, int k) {
+  return (long)(i - 1) + (long)b.nfi[d] * ((long)(j - 1) + (long)b.nfj[d] * (long)(k - 1));
+}
+__host__ __device__ inline long ri(const Blk &b, int i, int j, int k) {
+  return (long)(i - 1) + (long)(b.imx - 1) * ((long)(j - 1) + (long)(b.jmx - 1) * (long)(k - 1));
+}
+__host__ __device__ inline long gri(const Blk &b, int i, int j, int k) {
+  return (long)i + (long)(b.imx + 1) * ((long)j + (long)(b.jmx + 1) * (long)k);
+}
+__host__ __device__ inline long ndi(const Blk &b, int i, int j, int k) {
+  return (long)(i + 2) + (long)(b.imx + 6) * ((long)(j + 2) + (long)(b.jmx + 6) * (long)(k + 2));
+}
+// stride of the padded cell arrays along direction d
+__host__ __device__ inline long cstride(const Blk &b, int d) {
+  return d == 0 ? 1L : d == 1 ? (long)b.pi : (long)b.pi * b.pj;
+}
+
+// amdflang's lowering of MAX/MIN (NaN in the first operand yields the
+// second); kept so that limiter branches agree with the reference build
+__device__ __forceinline__ double fmx(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double fmn(double a, double b) { return a < b ? a : b; }
+__device__ __forceinline__ double fsgn(double a, double b) { return copysign(fabs(a), b); }
+__device__ __forceinline__ double sqd(double x) { return x * x; }
+
+}  // namespace fest

@@ -1,0 +1,519 @@
+// fest_api.hip -- host runtime and C ABI of the MI355X explicit-residual
+// engine (declared in include/fest_hip.h).  One handle owns one block's
+// device-resident arrays and a HIP stream; every kernel of a stage is
+// launched on that stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fest_hip.h"
+#include "blk.h"
+#include "k_bc.h"
+#include "k_flux.h"
+#include "k_geom.h"
+#include "k_grad.h"
+#include "k_update.h"
+
+using namespace fest;
+
+struct fest_handle {
+  Blk b;
+  hipStream_t stream = nullptr;
+  std::string err;
+  fest_extent dims;
+  fest_scheme scheme;
+  fest_flow flow;
+  fest_bc bc;
+  std::vector<void *> allocs;
+  long red_cap = 0;        // doubles in b.red
+  int norm_blocks = 0;
+  bool have_grid = false, have_state = false;
+};
+
+namespace {
+
+int fail(fest_handle *h, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (h) h->err = buf;
+  return code;
+}
+
+#define HIPCHK(h, call)                                                                 \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess) return fail(h, FEST_E_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+#define LAUNCHCHK(h)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = hipGetLastError();                                                  \
+    if (e_ != hipSuccess) return fail(h, FEST_E_HIP, "launch: %s", hipGetErrorString(e_)); \
+  } while (0)
+
+int dalloc(fest_handle *h, double **p, long n) {
+  void *q = nullptr;
+  if (n <= 0) n = 1;
+  hipError_t e = hipMalloc(&q, sizeof(double) * (size_t)n);
+  if (e != hipSuccess) return fail(h, FEST_E_HIP, "hipMalloc(%ld doubles): %s", n, hipGetErrorString(e));
+  h->allocs.push_back(q);
+  *p = (double *)q;
+  return 0;
+}
+
+inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
+
+// grid over an (ni x nj x nk) index box with 64x4 thread blocks along i, j
+inline void box(long ni, long nj, long nk, dim3 &g, dim3 &t) {
+  t = dim3(64, 4, 1);
+  g = dim3(cdiv(ni, 64), cdiv(nj, 4), (unsigned)nk);
+}
+
+int check_err(fest_handle *h) {
+  unsigned e = 0;
+  HIPCHK(h, hipMemcpyAsync(&e, h->b.err, sizeof(unsigned), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (!e) return 0;
+  HIPCHK(h, hipMemsetAsync(h->b.err, 0, sizeof(unsigned), h->stream));
+  if (e & ERR_NONPHYS) return fail(h, FEST_E_NONPHYSICAL, "non-physical state (rho<0, p<0 or NaN) in the update");
+  if (e & ERR_NAN_FLUX) return fail(h, FEST_E_NAN, "NaN in the face fluxes");
+  if (e & ERR_NAN_GRAD) return fail(h, FEST_E_NAN, "NaN in the gradients");
+  if (e & ERR_NAN_MU) return fail(h, FEST_E_NAN, "NaN in the molecular viscosity");
+  return fail(h, FEST_E_CONFIG, "device error bits 0x%x", e);
+}
+
+// C2..C9(+C10 in k_update): ghosts, gradients/viscosity, face fluxes
+int launch_residual_front(fest_handle *h, int current_iter, int store_csnd) {
+  Blk &b = h->b;
+  hipStream_t st = h->stream;
+  for (int f = 0; f < 6; ++f) {
+    int id = b.bc[f];
+    if (id >= 0 || id == -10) continue;
+    int na = face_na(b, f), nb = face_nb(b, f);
+    dim3 t(64, 4, 1), g(cdiv(na + 5, 64), cdiv(nb + 5, 4), 1);
+    hipLaunchKernelGGL(k_bc_face, g, t, 0, st, b, f, current_iter);
+  }
+  hipLaunchKernelGGL(k_bc_edges, dim3(cdiv(4L * (b.imx + 5) * b.nv, 256)), dim3(256), 0, st, b, 0);
+  hipLaunchKernelGGL(k_bc_edges, dim3(cdiv(4L * (b.kmx + 5) * b.nv, 256)), dim3(256), 0, st, b, 1);
+  hipLaunchKernelGGL(k_bc_edges, dim3(1), dim3(64), 0, st, b, 2);
+  LAUNCHCHK(h);
+  dim3 g, t;
+  if (b.mu_ref != 0.0) {
+    box(b.imx + 1, b.jmx + 1, b.kmx + 1, g, t);
+    hipLaunchKernelGGL(k_grad, g, t, 0, st, b);
+    int mx = std::max(std::max(b.imx, b.jmx), b.kmx);
+    dim3 gb(cdiv(mx, 64), cdiv(mx, 4), 6);
+    hipLaunchKernelGGL(k_grad_bc, gb, dim3(64, 4, 1), 0, st, b);
+    hipLaunchKernelGGL(k_visc, g, t, 0, st, b);
+    if (b.turb) hipLaunchKernelGGL(k_visc_ghost, gb, dim3(64, 4, 1), 0, st, b);
+    LAUNCHCHK(h);
+  }
+  box(b.imx, b.jmx - 1, b.kmx - 1, g, t);
+  hipLaunchKernelGGL(k_flux<0>, g, t, 0, st, b, store_csnd);
+  box(b.imx - 1, b.jmx, b.kmx - 1, g, t);
+  hipLaunchKernelGGL(k_flux<1>, g, t, 0, st, b, store_csnd);
+  box(b.imx - 1, b.jmx - 1, b.kmx, g, t);
+  hipLaunchKernelGGL(k_flux<2>, g, t, 0, st, b, store_csnd);
+  LAUNCHCHK(h);
+  return 0;
+}
+
+int launch_time_step(fest_handle *h) {
+  Blk &b = h->b;
+  dim3 g, t;
+  if (!b.time_local && b.global_dt > 0) {
+    hipLaunchKernelGGL(k_dt_fill, dim3(cdiv(b.nres, 256)), dim3(256), 0, h->stream, b, 1);
+    LAUNCHCHK(h);
+    return 0;
+  }
+  box(b.imx - 1, b.jmx - 1, b.kmx - 1, g, t);
+  hipLaunchKernelGGL(k_dt, g, t, 0, h->stream, b);
+  if (!b.time_local) {
+    long nblk = (long)g.x * g.y * g.z;
+    hipLaunchKernelGGL(k_dt_min, dim3(1), dim3(256), 0, h->stream, b, nblk);
+    hipLaunchKernelGGL(k_dt_fill, dim3(cdiv(b.nres, 256)), dim3(256), 0, h->stream, b, 0);
+  }
+  LAUNCHCHK(h);
+  return 0;
+}
+
+int launch_update(fest_handle *h, double TF, double SF, int flags) {
+  Blk &b = h->b;
+  dim3 g, t;
+  box(b.imx - 1, b.jmx - 1, b.kmx - 1, g, t);
+  hipLaunchKernelGGL(k_update, g, t, 0, h->stream, b, TF, SF, flags);
+  LAUNCHCHK(h);
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *scheme, const fest_flow *flow,
+                const fest_bc *bc, int precision) {
+  if (!out || !dims || !scheme || !flow || !bc) return FEST_E_ARG;
+  *out = nullptr;
+  fest_handle *h = new fest_handle();
+  h->dims = *dims; h->scheme = *scheme; h->flow = *flow; h->bc = *bc;
+  auto bad = [&](const char *msg) { h->err = msg; *out = h; return FEST_E_CONFIG; };
+  if (precision != 0) return bad("only the plain FP64 path (precision 0) is built");
+  if (dims->imx < 4 || dims->jmx < 4 || dims->kmx < 4) return bad("each direction needs at least 3 cells");
+  int nv_expect = scheme->turb == FEST_TURB_SST ? 7 : 5;
+  if (scheme->turb != FEST_TURB_NONE && scheme->turb != FEST_TURB_SST) return bad("turbulence model not supported");
+  if (dims->n_var != nv_expect) return bad("n_var does not match the turbulence model");
+  if (scheme->flux < 0 || scheme->flux > 5) return bad("unknown flux scheme");
+  if (scheme->recon < 0 || scheme->recon > 3) return bad("unknown reconstruction");
+  if (scheme->time_accuracy != FEST_TA_RK4 && scheme->time_accuracy != FEST_TA_RK2 &&
+      scheme->time_accuracy != FEST_TA_NONE)
+    return bad("time integration not supported (RK4, RK2, none)");
+  for (int f = 0; f < 6; ++f) {
+    int id = bc->id[f];
+    if (id >= 0 || id == -10 || id == -11) return bad("interfaces, -10 and -11 boundaries are not supported on one block");
+    if (id < -9) return bad("unknown boundary id");
+    if (id == -7) return bad("pole boundaries are not supported");
+  }
+  if (flow->mu_ref == 0.0 && scheme->turb) return bad("turbulence needs mu_ref != 0");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    h->err = "no HIP device";
+    *out = h;
+    return FEST_E_HIP;
+  }
+
+  Blk &b = h->b;
+  memset(&b, 0, sizeof b);
+  b.imx = dims->imx; b.jmx = dims->jmx; b.kmx = dims->kmx; b.nv = dims->n_var;
+  b.ng = scheme->turb ? 6 : 4;
+  b.pi = b.imx + 5; b.pj = b.jmx + 5; b.pk = b.kmx + 5;
+  b.nc = (long)b.pi * b.pj * b.pk;
+  b.fpi[0] = b.imx + 6; b.fpj[0] = b.jmx + 5; b.fpk[0] = b.kmx + 5;
+  b.fpi[1] = b.imx + 5; b.fpj[1] = b.jmx + 6; b.fpk[1] = b.kmx + 5;
+  b.fpi[2] = b.imx + 5; b.fpj[2] = b.jmx + 5; b.fpk[2] = b.kmx + 6;
+  for (int d = 0; d < 3; ++d) b.nfc[d] = (long)b.fpi[d] * b.fpj[d] * b.fpk[d];
+  b.nfi[0] = b.imx; b.nfj[0] = b.jmx - 1; b.nfk[0] = b.kmx - 1;
+  b.nfi[1] = b.imx - 1; b.nfj[1] = b.jmx; b.nfk[1] = b.kmx - 1;
+  b.nfi[2] = b.imx - 1; b.nfj[2] = b.jmx - 1; b.nfk[2] = b.kmx;
+  for (int d = 0; d < 3; ++d) b.nfl[d] = (long)b.nfi[d] * b.nfj[d] * b.nfk[d];
+  b.ngr = (long)(b.imx + 1) * (b.jmx + 1) * (b.kmx + 1);
+  b.nres = (long)(b.imx - 1) * (b.jmx - 1) * (b.kmx - 1);
+  b.flux_scheme = scheme->flux; b.recon = scheme->recon; b.turb = scheme->turb;
+  b.time_local = scheme->time_local; b.time_acc = scheme->time_accuracy; b.accur = scheme->accur;
+  for (int d = 0; d < 3; ++d) { b.lim[d] = scheme->lim[d]; b.tlim[d] = scheme->tlim[d]; }
+  // src/boundary/boundary_state_reconstruction.f90:45-47
+  b.ppm_flag = (scheme->recon == FEST_RECON_PPM || scheme->recon == FEST_RECON_WENO) ? 1 : 0;
+  for (int f = 0; f < 6; ++f) b.bc[f] = bc->id[f];
+  // make_*_flux_zero at the first/last face, src/boundary/bc.f90:60-66
+  for (int d = 0; d < 3; ++d)
+    for (int s = 0; s < 2; ++s) {
+      int id = bc->id[2 * d + s];
+      b.mz[d][s] = (id == -5 || id == -6 || id == -7) ? 0 : 1;
+    }
+  b.mu_suth = flow->mu_sutherland;
+  b.gm = flow->gm; b.R = flow->R_gas; b.mu_ref = flow->mu_ref; b.T_ref = flow->T_ref;
+  b.S_ref = flow->Sutherland_temp; b.Pr = flow->Pr; b.tPr = flow->tPr; b.MInf = flow->MInf;
+  b.vel_mag = flow->vel_mag;
+  b.rho_inf = flow->density_inf; b.u_inf = flow->x_speed_inf; b.v_inf = flow->y_speed_inf;
+  b.w_inf = flow->z_speed_inf; b.p_inf = flow->pressure_inf; b.tk_inf = flow->tk_inf; b.tw_inf = flow->tw_inf;
+  b.c2 = 1 + scheme->accur;
+  b.c3 = 0.5 * scheme->accur;
+  b.c1 = b.c2 - b.c3;
+  for (int f = 0; f < 6; ++f) {
+    b.fix[FX_RHO][f] = bc->fixed_density[f]; b.fix[FX_U][f] = bc->fixed_x_speed[f];
+    b.fix[FX_V][f] = bc->fixed_y_speed[f]; b.fix[FX_W][f] = bc->fixed_z_speed[f];
+    b.fix[FX_P][f] = bc->fixed_pressure[f]; b.fix[FX_TK][f] = bc->fixed_tk[f];
+    b.fix[FX_TW][f] = bc->fixed_tw[f]; b.fix[FX_TWALL][f] = bc->fixed_wall_temperature[f];
+  }
+  b.global_dt = scheme->global_time_step;
+  b.CFLv = 1.0;
+
+  *out = h;
+  int rc = 0;
+  if (hipStreamCreate(&h->stream) != hipSuccess) return fail(h, FEST_E_HIP, "hipStreamCreate failed");
+#define AL(p, n) if ((rc = dalloc(h, &(p), (n)))) return rc
+  AL(b.qp, b.nc * b.nv); AL(b.temp, b.nc); AL(b.mu, b.nc); AL(b.mut, b.nc); AL(b.f1, b.nc); AL(b.dist, b.nc);
+  AL(b.vol, b.nc); AL(b.cx, b.nc); AL(b.cy, b.nc); AL(b.cz, b.nc);
+  for (int d = 0; d < 3; ++d) {
+    AL(b.fA[d], b.nfc[d]); AL(b.fx[d], b.nfc[d]); AL(b.fy[d], b.nfc[d]); AL(b.fz[d], b.nfc[d]);
+    AL(b.flux[d], b.nfl[d] * b.nv); AL(b.csnd[d], b.nfl[d]);
+    AL(b.g[d], b.ngr * b.ng);
+  }
+  AL(b.res, b.nres * b.nv); AL(b.dt, b.nres); AL(b.ust, b.nc * b.nv); AL(b.rst, b.nres * b.nv);
+  AL(b.nodes, 3L * (b.imx + 6) * (b.jmx + 6) * (b.kmx + 6));
+  // reduction partials: the larger of the time-step block count and 8 per norm block
+  long nblk_dt = (long)cdiv(b.imx - 1, 64) * cdiv(b.jmx - 1, 4) * (b.kmx - 1);
+  h->norm_blocks = (int)std::min<long>(1024, std::max<long>(1, cdiv(b.nres, 256)));
+  h->red_cap = std::max<long>(nblk_dt, 8L * h->norm_blocks);
+  AL(b.red, h->red_cap);
+  AL(b.norms, 8); AL(b.dtmin, 1);
+#undef AL
+  void *e = nullptr;
+  if (hipMalloc(&e, sizeof(unsigned)) != hipSuccess) return fail(h, FEST_E_HIP, "hipMalloc(err)");
+  h->allocs.push_back(e);
+  b.err = (unsigned *)e;
+  HIPCHK(h, hipMemsetAsync(b.err, 0, sizeof(unsigned), h->stream));
+  // molecular viscosity starts at mu_ref (src/viscosity.f90:565-568); F1 at 0
+  hipLaunchKernelGGL(k_fill, dim3(cdiv(b.nc, 256)), dim3(256), 0, h->stream, b.mu, b.nc, b.mu_ref);
+  hipLaunchKernelGGL(k_fill, dim3(cdiv(b.nc, 256)), dim3(256), 0, h->stream, b.f1, b.nc, 0.0);
+  hipLaunchKernelGGL(k_fill, dim3(cdiv(b.nc, 256)), dim3(256), 0, h->stream, b.mut, b.nc, 0.0);
+  hipLaunchKernelGGL(k_fill, dim3(cdiv(b.nc, 256)), dim3(256), 0, h->stream, b.dist, b.nc, 1.0);
+  LAUNCHCHK(h);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+void fest_destroy(fest_handle *h) {
+  if (!h) return;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  for (void *p : h->allocs) hipFree(p);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+}
+
+const char *fest_last_error(const fest_handle *h) { return h ? h->err.c_str() : "null handle"; }
+
+int fest_upload_geometry(fest_handle *h, const double *cells4, const double *if4, const double *jf4,
+                         const double *kf4, const double *dist) {
+  if (!h || !cells4 || !if4 || !jf4 || !kf4) return FEST_E_ARG;
+  Blk &b = h->b;
+  size_t nc = sizeof(double) * b.nc;
+  double *cd[4] = {b.vol, b.cx, b.cy, b.cz};
+  for (int a = 0; a < 4; ++a) HIPCHK(h, hipMemcpyAsync(cd[a], cells4 + a * b.nc, nc, hipMemcpyHostToDevice, h->stream));
+  const double *src[3] = {if4, jf4, kf4};
+  for (int d = 0; d < 3; ++d) {
+    size_t n = sizeof(double) * b.nfc[d];
+    double *fd[4] = {b.fA[d], b.fx[d], b.fy[d], b.fz[d]};
+    for (int a = 0; a < 4; ++a)
+      HIPCHK(h, hipMemcpyAsync(fd[a], src[d] + a * b.nfc[d], n, hipMemcpyHostToDevice, h->stream));
+  }
+  if (dist) HIPCHK(h, hipMemcpyAsync(b.dist, dist, nc, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->have_grid = true;
+  return 0;
+}
+
+int fest_set_grid(fest_handle *h, const double *nodes, int with_wall_dist) {
+  if (!h || !nodes) return FEST_E_ARG;
+  Blk &b = h->b;
+  long nn = (long)(b.imx + 6) * (b.jmx + 6) * (b.kmx + 6);
+  HIPCHK(h, hipMemcpyAsync(b.nodes, nodes, sizeof(double) * 3 * nn, hipMemcpyHostToDevice, h->stream));
+  for (int d = 0; d < 3; ++d) {
+    int na = d == 0 ? b.jmx : b.imx, nb = d == 2 ? b.jmx : b.kmx;
+    dim3 g(cdiv(na + 6, 64), cdiv(nb + 6, 4), 3);
+    hipLaunchKernelGGL(k_ghost_nodes, g, dim3(64, 4, 1), 0, h->stream, b, d);
+  }
+  for (int d = 0; d < 3; ++d) {
+    dim3 g(cdiv(b.imx + 5 + (d == 0), 64), cdiv(b.jmx + 5 + (d == 1), 4), (unsigned)(b.kmx + 5 + (d == 2)));
+    hipLaunchKernelGGL(k_face_metrics, g, dim3(64, 4, 1), 0, h->stream, b, d);
+  }
+  {
+    dim3 g(cdiv(b.imx + 5, 64), cdiv(b.jmx + 5, 4), (unsigned)(b.kmx + 5));
+    hipLaunchKernelGGL(k_cells, g, dim3(64, 4, 1), 0, h->stream, b);
+  }
+  LAUNCHCHK(h);
+  if (with_wall_dist) {
+    // surface nodes of the no-slip faces, in the order of src/wall/wall.f90:
+    // 198-288, rounded like the ES18.10E3 text file the reference reads back
+    std::vector<double> hn(3 * nn);
+    HIPCHK(h, hipMemcpyAsync(hn.data(), b.nodes, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    std::vector<double> wall;
+    for (int f = 0; f < 6; ++f) {
+      if (b.bc[f] != -5) continue;
+      int im = 1, ix = b.imx, jm = 1, jx = b.jmx, km = 1, kx = b.kmx;
+      switch (f) {
+        case 0: ix = 1; break;
+        case 1: im = b.imx; break;
+        case 2: jx = 1; break;
+        case 3: jm = b.jmx; break;
+        case 4: kx = 1; break;
+        default: km = b.kmx; break;
+      }
+      for (int k = km; k <= kx; ++k)
+        for (int j = jm; j <= jx; ++j)
+          for (int i = im; i <= ix; ++i)
+            for (int c = 0; c < 3; ++c) {
+              char buf[64];
+              snprintf(buf, sizeof buf, "%.10E", hn[c * nn + ndi(b, i, j, k)]);
+              wall.push_back(strtod(buf, nullptr));
+            }
+    }
+    int nw = (int)(wall.size() / 3);
+    double *dwall = nullptr, *nd = nullptr;
+    HIPCHK(h, hipMalloc(&dwall, sizeof(double) * std::max<size_t>(3, wall.size())));
+    HIPCHK(h, hipMalloc(&nd, sizeof(double) * nn));
+    if (nw) HIPCHK(h, hipMemcpyAsync(dwall, wall.data(), sizeof(double) * wall.size(), hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_node_dist, dim3(cdiv(nn, 256)), dim3(256), 0, h->stream, b, (const double *)dwall, nw, nd);
+    dim3 g(cdiv(b.imx + 5, 64), cdiv(b.jmx + 5, 4), (unsigned)(b.kmx + 5));
+    hipLaunchKernelGGL(k_cell_dist, g, dim3(64, 4, 1), 0, h->stream, b, (const double *)nd);
+    hipError_t le = hipGetLastError();
+    hipStreamSynchronize(h->stream);
+    hipFree(dwall);
+    hipFree(nd);
+    if (le != hipSuccess) return fail(h, FEST_E_HIP, "wall distance: %s", hipGetErrorString(le));
+  }
+  int rc = check_err(h);
+  if (rc) return fail(h, FEST_E_NONPHYSICAL, "non-positive cell volume in the grid");
+  h->have_grid = true;
+  return 0;
+}
+
+int fest_upload_state(fest_handle *h, const double *qp, const double *temp) {
+  if (!h || !qp) return FEST_E_ARG;
+  Blk &b = h->b;
+  HIPCHK(h, hipMemcpyAsync(b.qp, qp, sizeof(double) * b.nc * b.nv, hipMemcpyHostToDevice, h->stream));
+  if (temp) HIPCHK(h, hipMemcpyAsync(b.temp, temp, sizeof(double) * b.nc, hipMemcpyHostToDevice, h->stream));
+  else {
+    hipLaunchKernelGGL(k_step_begin, dim3(cdiv(b.nc, 256)), dim3(256), 0, h->stream, b, 0);
+    LAUNCHCHK(h);
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->have_state = true;
+  return 0;
+}
+
+int fest_residual(fest_handle *h, int current_iter) {
+  if (!h) return FEST_E_ARG;
+  if (!h->have_grid || !h->have_state) return fail(h, FEST_E_ARG, "grid and state must be set first");
+  int rc = launch_residual_front(h, current_iter, 1);
+  if (rc) return rc;
+  if ((rc = launch_update(h, 0.0, 0.0, UPD_WRITE_RES))) return rc;
+  return check_err(h);
+}
+
+int fest_time_step(fest_handle *h, double cfl) {
+  if (!h) return FEST_E_ARG;
+  h->b.CFLv = cfl;
+  int rc = launch_time_step(h);
+  if (rc) return rc;
+  return check_err(h);
+}
+
+int fest_get_next_solution(fest_handle *h, int current_iter, double cfl) {
+  if (!h) return FEST_E_ARG;
+  if (!h->have_grid || !h->have_state) return fail(h, FEST_E_ARG, "grid and state must be set first");
+  Blk &b = h->b;
+  b.CFLv = cfl;
+  int rc;
+  const int ta = b.time_acc;
+  hipLaunchKernelGGL(k_step_begin, dim3(cdiv(b.nc, 256)), dim3(256), 0, h->stream, b, ta != FEST_TA_NONE);
+  LAUNCHCHK(h);
+  if (ta == FEST_TA_NONE) {
+    if ((rc = launch_residual_front(h, current_iter, 1))) return rc;
+    if ((rc = launch_update(h, 0.0, 0.0, UPD_WRITE_RES))) return rc;
+    if ((rc = launch_time_step(h))) return rc;
+    if ((rc = launch_update(h, 1.0, 1.0, UPD_APPLY))) return rc;
+    return check_err(h);
+  }
+  // src/update.f90:176-195
+  const double TF4[4] = {0.5, 0.5, 1.0, 1. / 6.}, SF4[4] = {1., 2., 2., 1.};
+  const double TF2[2] = {0.5, 0.5}, SF2[2] = {1., 1.};
+  int nst = ta == FEST_TA_RK4 ? 4 : 2;
+  const double *TF = ta == FEST_TA_RK4 ? TF4 : TF2;
+  const double *SF = ta == FEST_TA_RK4 ? SF4 : SF2;
+  for (int s = 0; s < nst; ++s) {
+    if ((rc = launch_residual_front(h, current_iter, s == 0))) return rc;
+    if (s == 0 && (rc = launch_time_step(h))) return rc;
+    int flags = UPD_APPLY | UPD_STORE;
+    if (s == 0) flags |= UPD_FIRST;
+    if (s == nst - 1) flags |= UPD_LAST | UPD_WRITE_RES;
+    if ((rc = launch_update(h, TF[s], SF[s], flags))) return rc;
+  }
+  return check_err(h);
+}
+
+int fest_download(fest_handle *h, uint32_t what, double *qp, double *temp, double *residue, double *F, double *G,
+                  double *H, double *delta_t, double *mu, double *mu_t, double *grads, double *sst_f1, double *dist,
+                  double *cells4, double *faces) {
+  if (!h) return FEST_E_ARG;
+  Blk &b = h->b;
+  hipStream_t st = h->stream;
+  auto cp = [&](void *dst, const void *src, size_t n) -> int {
+    if (!dst) return FEST_E_ARG;
+    hipError_t e = hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, st);
+    return e == hipSuccess ? 0 : fail(h, FEST_E_HIP, "download: %s", hipGetErrorString(e));
+  };
+  int rc = 0;
+#define DL(bit, ptr, src, n) if ((what & (bit)) && (rc = cp(ptr, src, (size_t)(n)))) return rc
+  DL(FEST_DL_QP, qp, b.qp, b.nc * b.nv);
+  DL(FEST_DL_TEMP, temp, b.temp, b.nc);
+  DL(FEST_DL_RESIDUE, residue, b.res, b.nres * b.nv);
+  DL(FEST_DL_F, F, b.flux[0], b.nfl[0] * b.nv);
+  DL(FEST_DL_G, G, b.flux[1], b.nfl[1] * b.nv);
+  DL(FEST_DL_H, H, b.flux[2], b.nfl[2] * b.nv);
+  DL(FEST_DL_DT, delta_t, b.dt, b.nres);
+  DL(FEST_DL_MU, mu, b.mu, b.nc);
+  DL(FEST_DL_MUT, mu_t, b.mut, b.nc);
+  DL(FEST_DL_F1, sst_f1, b.f1, b.nc);
+  DL(FEST_DL_DIST, dist, b.dist, b.nc);
+#undef DL
+  if (what & FEST_DL_GRADS) {
+    if (!grads) return FEST_E_ARG;
+    for (int d = 0; d < 3; ++d)
+      if ((rc = cp(grads + (size_t)d * b.ngr * b.ng, b.g[d], (size_t)(b.ngr * b.ng)))) return rc;
+  }
+  if (what & FEST_DL_CELLS) {
+    if (!cells4) return FEST_E_ARG;
+    double *src[4] = {b.vol, b.cx, b.cy, b.cz};
+    for (int a = 0; a < 4; ++a)
+      if ((rc = cp(cells4 + (size_t)a * b.nc, src[a], (size_t)b.nc))) return rc;
+  }
+  if (what & FEST_DL_FACES) {
+    if (!faces) return FEST_E_ARG;
+    size_t off = 0;
+    for (int d = 0; d < 3; ++d) {
+      double *src[4] = {b.fA[d], b.fx[d], b.fy[d], b.fz[d]};
+      for (int a = 0; a < 4; ++a) {
+        if ((rc = cp(faces + off, src[a], (size_t)b.nfc[d]))) return rc;
+        off += (size_t)b.nfc[d];
+      }
+    }
+  }
+  HIPCHK(h, hipStreamSynchronize(st));
+  return 0;
+}
+
+int fest_resnorm(fest_handle *h, double *res_abs) {
+  if (!h || !res_abs) return FEST_E_ARG;
+  Blk &b = h->b;
+  hipLaunchKernelGGL(k_norm_partial, dim3(h->norm_blocks), dim3(256), 0, h->stream, b);
+  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(64), 0, h->stream, b, h->norm_blocks);
+  LAUNCHCHK(h);
+  HIPCHK(h, hipMemcpyAsync(res_abs, b.norms, sizeof(double) * (b.nv + 1), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int fest_device_ptr(fest_handle *h, const char *name, void **ptr, size_t *count) {
+  if (!h || !name || !ptr) return FEST_E_ARG;
+  Blk &b = h->b;
+  struct { const char *n; void *p; long c; } t[] = {
+      {"qp", b.qp, b.nc * b.nv}, {"temp", b.temp, b.nc}, {"residue", b.res, b.nres * b.nv},
+      {"F", b.flux[0], b.nfl[0] * b.nv}, {"G", b.flux[1], b.nfl[1] * b.nv}, {"H", b.flux[2], b.nfl[2] * b.nv},
+      {"dt", b.dt, b.nres}};
+  for (auto &e : t)
+    if (!strcmp(e.n, name)) {
+      *ptr = e.p;
+      if (count) *count = (size_t)e.c;
+      return 0;
+    }
+  return fail(h, FEST_E_ARG, "unknown array '%s'", name);
+}
+
+int fest_sync(fest_handle *h) {
+  if (!h) return FEST_E_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,322 @@
+// k_bc.h -- ghost-cell boundary conditions (one launch per block face, in
+// the reference's face order imin, imax, jmin, jmax, kmin, kmax), then the
+// edge/corner averages.  src/boundary/bc_primitive.f90:55-227 and the
+// routines it calls; every face kernel gives one thread a column of three
+// ghost cells, which only reads its own column, so the per-face passes keep
+// the reference's sequential semantics.
+#pragma once
+#include "blk.h"
+
+namespace fest {
+
+// cell of layer L (L >= 1 interior counted from the face, L <= 0 ghost) at
+// tangential position (a, bb) of a face
+__device__ __forceinline__ long fcell(const Blk &b, int face, int L, int a, int bb) {
+  switch (face) {
+    case 0: return ci(b, L, a, bb);
+    case 1: return ci(b, b.imx - L, a, bb);
+    case 2: return ci(b, a, L, bb);
+    case 3: return ci(b, a, b.jmx - L, bb);
+    case 4: return ci(b, a, bb, L);
+    default: return ci(b, a, bb, b.kmx - L);
+  }
+}
+__host__ __device__ inline int face_na(const Blk &b, int face) { return face < 2 ? b.jmx : b.imx; }
+__host__ __device__ inline int face_nb(const Blk &b, int face) { return face < 4 ? b.kmx : b.jmx; }
+
+enum { CP_ANTI = 0, CP_FLAT = 1, CP_SYMM = 2 };
+
+// src/boundary/copy_bc.f90:58-131 on one column
+__device__ void copy3col(const Blk &b, double *v, int type, int face, int a, int bb) {
+  double a1, a2, a3;
+  int src[3];
+  if (type == CP_ANTI) { a1 = 1.; a2 = -1.; a3 = 0.; src[0] = 1; src[1] = 2; src[2] = 3; }
+  else if (type == CP_FLAT) { a1 = 1.; a2 = 1.; a3 = 0.; src[0] = 1; src[1] = 1; src[2] = 1; }
+  else { a1 = b.c1; a2 = b.c2; a3 = b.c3; src[0] = 1; src[1] = 2; src[2] = 3; }
+  for (int g = 0; g < 3; ++g)
+    v[fcell(b, face, -g, a, bb)] = (a2 * v[fcell(b, face, src[g], a, bb)] - a3 * v[fcell(b, face, src[g] + 1, a, bb)]) / a1;
+}
+
+// src/boundary/bc_primitive.f90:484-525 on one column
+__device__ void fixcol(const Blk &b, double *v, double val, int face, int a, int bb) {
+  for (int g = 0; g < 3; ++g) v[fcell(b, face, -g, a, bb)] = val;
+}
+
+__device__ void turb_copycol(const Blk &b, int type, int face, int a, int bb) {
+  if (b.turb) {
+    copy3col(b, b.qp + 5 * b.nc, type, face, a, bb);
+    copy3col(b, b.qp + 6 * b.nc, type, face, a, bb);
+  }
+}
+__device__ void turb_fixcol(const Blk &b, int face, int a, int bb) {
+  if (b.turb) {
+    fixcol(b, b.qp + 5 * b.nc, b.fix[FX_TK][face], face, a, bb);
+    fixcol(b, b.qp + 6 * b.nc, b.fix[FX_TW][face], face, a, bb);
+  }
+}
+
+// unit normal of the boundary face at (a, bb)
+__device__ __forceinline__ void bnormal(const Blk &b, int face, int a, int bb, double &nx, double &ny, double &nz) {
+  int d = face >> 1;
+  long f;
+  switch (face) {
+    case 0: f = fi(b, 0, 1, a, bb); break;
+    case 1: f = fi(b, 0, b.imx, a, bb); break;
+    case 2: f = fi(b, 1, a, 1, bb); break;
+    case 3: f = fi(b, 1, a, b.jmx, bb); break;
+    case 4: f = fi(b, 2, a, bb, 1); break;
+    default: f = fi(b, 2, a, bb, b.kmx); break;
+  }
+  nx = b.fx[d][f]; ny = b.fy[d][f]; nz = b.fz[d][f];
+}
+
+// src/boundary/bc_primitive.f90:1778-1945
+__device__ void temp_based_density_col(const Blk &b, int face, int a, int bb) {
+  double Tw = b.fix[FX_TWALL][face], R = b.R, gm = b.gm;
+  double *rho = b.qp, *p = b.qp + 4 * b.nc;
+  if (Tw < 0.0) {
+    long c1 = fcell(b, face, 1, a, bb);
+    double stag = (p[c1] / (R * rho[c1])) * (1 + (0.5 * (gm - 1.) * gm * p[c1] / rho[c1]));
+    for (int g = 0; g < 3; ++g) {
+      long cg = fcell(b, face, -g, a, bb);
+      rho[cg] = p[cg] / (R * stag);
+    }
+  } else if (Tw > 1.0) {
+    for (int g = 0; g < 3; ++g) {
+      long cg = fcell(b, face, -g, a, bb), cn = fcell(b, face, 1 + g, a, bb);
+      rho[cg] = p[cg] / (R * (2 * Tw - (p[cn] / (R * rho[cn]))));
+    }
+  } else {
+    copy3col(b, rho, CP_SYMM, face, a, bb);
+  }
+}
+
+// src/boundary/bc_primitive.f90:564-643 (Sutherland mu whatever mu_variation)
+__device__ void omega_wall_col(const Blk &b, int face, int a, int bb) {
+  double *rho = b.qp, *p = b.qp + 4 * b.nc, *tw = b.qp + 6 * b.nc;
+  long c0 = fcell(b, face, 0, a, bb), c1 = fcell(b, face, 1, a, bb);
+  for (int l = 1; l <= 3; ++l) {
+    double T_face = 0.5 * ((p[c0] / rho[c0]) + (p[c1] / rho[c1])) / b.R;
+    double mu = b.mu_ref * pow(T_face / b.T_ref, 1.5) * ((b.T_ref + b.S_ref) / (T_face + b.S_ref));
+    double r = 0.5 * (rho[c0] + rho[c1]);
+    double d2 = 2 * b.dist[c1];
+    tw[fcell(b, face, 1 - l, a, bb)] = 120 * mu / (r * 0.075 * (d2 * d2)) - tw[fcell(b, face, l, a, bb)];
+  }
+}
+
+// far-field Riemann state of one column (src/boundary/bc_primitive.f90:645-1234);
+// returns Unb and, when write != 0, stores layer 0 of rho,u,v,w,p
+__device__ double farfield_col(const Blk &b, int face, int a, int bb, int write) {
+  double gm = b.gm;
+  double *rho = b.qp, *u = b.qp + b.nc, *v = b.qp + 2 * b.nc, *w = b.qp + 3 * b.nc, *p = b.qp + 4 * b.nc;
+  double nx, ny, nz;
+  bnormal(b, face, a, bb, nx, ny, nz);
+  if ((face & 1) == 0) { nx = -nx; ny = -ny; nz = -nz; }
+  long c1 = fcell(b, face, 1, a, bb), cg = fcell(b, face, 0, a, bb);
+  double cexp = sqrt(gm * p[c1] / rho[c1]);
+  double cinf = sqrt(gm * b.p_inf / b.rho_inf);
+  double Unexp = u[c1] * nx + v[c1] * ny + w[c1] * nz;
+  double Uninf = b.u_inf * nx + b.v_inf * ny + b.w_inf * nz;
+  double Rinf = Uninf - 2 * cinf / (gm - 1.);
+  double Rexp = Unexp + 2 * cexp / (gm - 1.);
+  double Unb = 0.5 * (Rexp + Rinf);
+  double Cb = 0.25 * (gm - 1.) * (Rexp - Rinf);
+  if (!write) return Unb;
+  if (Unb > 0.) {
+    double vd = Unb - Unexp;
+    u[cg] = u[c1] + vd * nx;
+    v[cg] = v[c1] + vd * ny;
+    w[cg] = w[c1] + vd * nz;
+    double sen = p[c1] / pow(rho[c1], gm);
+    rho[cg] = pow(Cb * Cb / (gm * sen), 1. / (gm - 1.));
+    p[cg] = (rho[cg] * Cb * Cb / gm);
+  } else {
+    double vd = Unb - Uninf;
+    u[cg] = b.u_inf + vd * nx;
+    v[cg] = b.v_inf + vd * ny;
+    w[cg] = b.w_inf + vd * nz;
+    double sen = b.p_inf / pow(b.rho_inf, gm);
+    rho[cg] = pow(Cb * Cb / (gm * sen), 1. / (gm - 1.));
+    p[cg] = (rho[cg] * Cb * Cb / gm);
+  }
+  return Unb;
+}
+
+// src/boundary/FT_bc.f90:36-106; j and k faces reflect with the I-face
+// normal stored at the same index, as the reference does
+__device__ void flow_tangency_col(const Blk &b, int face, int a, int bb) {
+  double *u = b.qp + b.nc, *v = b.qp + 2 * b.nc, *w = b.qp + 3 * b.nc;
+  double nx, ny, nz;
+  bnormal(b, face, a, bb, nx, ny, nz);
+  long f;
+  switch (face) {
+    case 0: f = fi(b, 0, 1, a, bb); break;
+    case 1: f = fi(b, 0, b.imx, a, bb); break;
+    case 2: f = fi(b, 0, a, 1, bb); break;
+    case 3: f = fi(b, 0, a, b.jmx, bb); break;
+    case 4: f = fi(b, 0, a, bb, 1); break;
+    default: f = fi(b, 0, a, bb, b.kmx); break;
+  }
+  double rx = b.fx[0][f], ry = b.fy[0][f], rz = b.fz[0][f];
+  for (int l = 1; l <= 3; ++l) {
+    long c = fcell(b, face, l, a, bb), g = fcell(b, face, 1 - l, a, bb);
+    double dot = u[c] * nx + v[c] * ny + w[c] * nz;
+    u[g] = u[c] - (2.0 * dot * rx);
+    v[g] = v[c] - (2.0 * dot * ry);
+    w[g] = w[c] - (2.0 * dot * rz);
+  }
+}
+
+// One face of populate_ghost_primitive.  The grid spans the padded
+// tangential plane (-2..na+2) x (-2..nb+2); most conditions act on the face
+// interior 1..na-1 x 1..nb-1, the periodic and far-field plane copies on the
+// whole plane.
+__global__ void k_bc_face(Blk b, int face, int current_iter) {
+  int a = (int)(blockIdx.x * blockDim.x + threadIdx.x) - 2;
+  int bb = (int)(blockIdx.y * blockDim.y + threadIdx.y) - 2;
+  int na = face_na(b, face), nb = face_nb(b, face);
+  if (a > na + 2 || bb > nb + 2) return;
+  int id = b.bc[face];
+  bool inner = (a >= 1 && a <= na - 1 && bb >= 1 && bb <= nb - 1);
+  double *q = b.qp;
+  const long nc = b.nc;
+  switch (id) {
+    case -1:  // supersonic inlet
+      if (inner && current_iter <= 2) {
+        for (int l = 0; l < 5; ++l) fixcol(b, q + l * nc, b.fix[l][face], face, a, bb);
+        turb_fixcol(b, face, a, bb);
+      }
+      break;
+    case -2:  // supersonic outlet
+      if (inner) {
+        for (int l = 0; l < 5; ++l) copy3col(b, q + l * nc, CP_FLAT, face, a, bb);
+        turb_copycol(b, CP_FLAT, face, a, bb);
+      }
+      break;
+    case -3:  // subsonic inlet
+      if (inner) {
+        if (current_iter <= 2) {
+          for (int l = 0; l < 4; ++l) fixcol(b, q + l * nc, b.fix[l][face], face, a, bb);
+          turb_fixcol(b, face, a, bb);
+        }
+        copy3col(b, q + 4 * nc, CP_FLAT, face, a, bb);
+      }
+      break;
+    case -4:  // subsonic outlet
+      if (inner) {
+        for (int l = 0; l < 4; ++l) copy3col(b, q + l * nc, CP_FLAT, face, a, bb);
+        if (current_iter <= 2) fixcol(b, q + 4 * nc, b.fix[FX_P][face], face, a, bb);
+        turb_copycol(b, CP_FLAT, face, a, bb);
+      }
+      break;
+    case -5:  // no-slip wall
+      if (inner) {
+        copy3col(b, q + 4 * nc, CP_SYMM, face, a, bb);
+        temp_based_density_col(b, face, a, bb);
+        copy3col(b, q + 1 * nc, CP_ANTI, face, a, bb);
+        copy3col(b, q + 2 * nc, CP_ANTI, face, a, bb);
+        copy3col(b, q + 3 * nc, CP_ANTI, face, a, bb);
+        if (b.turb) {
+          copy3col(b, q + 5 * nc, CP_ANTI, face, a, bb);
+          omega_wall_col(b, face, a, bb);
+        }
+      }
+      break;
+    case -6:  // slip wall
+      if (inner) {
+        copy3col(b, q, CP_SYMM, face, a, bb);
+        copy3col(b, q + 4 * nc, CP_SYMM, face, a, bb);
+        turb_copycol(b, CP_SYMM, face, a, bb);
+        flow_tangency_col(b, face, a, bb);
+      }
+      break;
+    case -7:  // pole
+      if (inner) {
+        for (int l = 0; l < 5; ++l) copy3col(b, q + l * nc, CP_FLAT, face, a, bb);
+        turb_copycol(b, CP_FLAT, face, a, bb);
+      }
+      break;
+    case -8: {  // far field
+      if (inner) {
+        farfield_col(b, face, a, bb, 1);
+        if (b.turb) {
+          // the whole-face turbulent copy/fix of the reference ends as set by
+          // the last face cell in its loop order
+          double last = farfield_col(b, face, na - 1, nb - 1, 0);
+          if (last > 0.) turb_copycol(b, CP_FLAT, face, a, bb);
+          else turb_fixcol(b, face, a, bb);
+        }
+      }
+      for (int l = 0; l < b.nv; ++l) {
+        double v0 = q[l * nc + fcell(b, face, 0, a, bb)];
+        q[l * nc + fcell(b, face, -1, a, bb)] = v0;
+        q[l * nc + fcell(b, face, -2, a, bb)] = v0;
+      }
+      break;
+    }
+    case -9: {  // single-block periodic: whole plane, three layers
+      for (int l = 0; l < b.nv; ++l) {
+        double *v = q + l * nc;
+        for (int g = 0; g < 3; ++g) {
+          long dst, src;
+          switch (face) {
+            case 0: dst = ci(b, -2 + g, a, bb); src = ci(b, b.imx - 3 + g, a, bb); break;
+            case 1: dst = ci(b, b.imx + g, a, bb); src = ci(b, 1 + g, a, bb); break;
+            case 2: dst = ci(b, a, -2 + g, bb); src = ci(b, a, b.jmx - 3 + g, bb); break;
+            case 3: dst = ci(b, a, b.jmx + g, bb); src = ci(b, a, 1 + g, bb); break;
+            case 4: dst = ci(b, a, bb, -2 + g); src = ci(b, a, bb, b.kmx - 3 + g); break;
+            default: dst = ci(b, a, bb, b.kmx + g); src = ci(b, a, bb, 1 + g); break;
+          }
+          v[dst] = v[src];
+        }
+      }
+      break;
+    }
+    default:
+      break;  // interfaces (>=0) and -10 are filled by the interface exchange
+  }
+}
+
+// edges and corners, src/boundary/bc_primitive.f90:209-224: phase 0 the
+// four lines along i, phase 1 the four lines along k, phase 2 the corners
+__global__ void k_bc_edges(Blk b, int phase) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int I = b.imx, J = b.jmx, K = b.kmx;
+#define Q(i, j, k) v[ci(b, i, j, k)]
+  if (phase == 0) {
+    int n = I + 5;
+    if (t >= 4 * n * b.nv) return;
+    int l = t / (4 * n), r = t % (4 * n), s = r / n, i = r % n - 2;
+    double *v = b.qp + (long)l * b.nc;
+    if (s == 0) Q(i, 0, 0) = 0.33 * (Q(i, 1, 1) + Q(i, 0, 1) + Q(i, 1, 0));
+    else if (s == 1) Q(i, 0, K) = 0.33 * (Q(i, 1, K - 1) + Q(i, 0, K - 1) + Q(i, 1, K));
+    else if (s == 2) Q(i, J, 0) = 0.33 * (Q(i, J - 1, 1) + Q(i, J, 1) + Q(i, J - 1, 0));
+    else Q(i, J, K) = 0.33 * (Q(i, J - 1, K - 1) + Q(i, J, K - 1) + Q(i, J - 1, K));
+  } else if (phase == 1) {
+    int n = K + 5;
+    if (t >= 4 * n * b.nv) return;
+    int l = t / (4 * n), r = t % (4 * n), s = r / n, k = r % n - 2;
+    double *v = b.qp + (long)l * b.nc;
+    if (s == 0) Q(I, 0, k) = 0.33 * (Q(I - 1, 1, k) + Q(I - 1, 0, k) + Q(I, 1, k));
+    else if (s == 1) Q(0, 0, k) = 0.33 * (Q(1, 1, k) + Q(1, 0, k) + Q(0, 1, k));
+    else if (s == 2) Q(0, J, k) = 0.33 * (Q(1, J - 1, k) + Q(1, J, k) + Q(0, J - 1, k));
+    else Q(I, J, k) = 0.33 * (Q(I - 1, J - 1, k) + Q(I - 1, J, k) + Q(I, J - 1, k));
+  } else {
+    if (t >= 8 * b.nv) return;
+    int l = t / 8, s = t % 8;
+    double *v = b.qp + (long)l * b.nc;
+    switch (s) {
+      case 0: Q(0, 0, 0) = 0.33 * (Q(1, 0, 0) + Q(0, 1, 0) + Q(0, 0, 1)); break;
+      case 1: Q(I, 0, 0) = 0.33 * (Q(I - 1, 0, 0) + Q(I, 1, 0) + Q(I, 0, 1)); break;
+      case 2: Q(0, J, 0) = 0.33 * (Q(1, J, 0) + Q(0, J - 1, 0) + Q(0, J, 1)); break;
+      case 3: Q(0, 0, K) = 0.33 * (Q(1, 0, K) + Q(0, 1, K) + Q(0, 0, K - 1)); break;
+      case 4: Q(I, J, 0) = 0.33 * (Q(I - 1, J, 0) + Q(I, J - 1, 0) + Q(I, J, 1)); break;
+      case 5: Q(I, 0, K) = 0.33 * (Q(I - 1, 0, K) + Q(I, 1, K) + Q(I, 0, K - 1)); break;
+      case 6: Q(0, J, K) = 0.33 * (Q(1, J, K) + Q(0, J - 1, K) + Q(0, J, K - 1)); break;
+      default: Q(I, J, K) = 0.33 * (Q(I - 1, J, K) + Q(I, J - 1, K) + Q(I, J, K - 1)); break;
+    }
+  }
+#undef Q
+}
+
+}  // namespace fest

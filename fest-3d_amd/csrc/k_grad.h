@@ -1,0 +1,170 @@
+// k_grad.h -- Green-Gauss cell gradients (src/gradients.f90:276-482) and the
+// ghost-cell gradient rule (src/gradients.f90:486-676), molecular and SST
+// eddy viscosity with the F1 blending function (src/viscosity.f90:52-546).
+//
+// One thread per cell computes all 3*n_grad gradients (the reference makes
+// 3*n_grad separate passes): the 6 face metrics and the volume are read once
+// and reused for every variable.
+#pragma once
+#include "blk.h"
+#include "k_bc.h"
+
+namespace fest {
+
+// gradient slots: u, v, w, T, k, omega
+__device__ __forceinline__ const double *grad_var(const Blk &b, int l) {
+  return l < 3 ? b.qp + (long)(l + 1) * b.nc : l == 3 ? b.temp : b.qp + (long)(l + 1) * b.nc;
+}
+
+__global__ void __launch_bounds__(256) k_grad(Blk b) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int k = blockIdx.z;
+  if (i > b.imx || j > b.jmx || k > b.kmx) return;
+  long c = ci(b, i, j, k);
+  long f0 = fi(b, 0, i, j, k), f1 = fi(b, 0, i + 1, j, k);
+  long g0 = fi(b, 1, i, j, k), g1 = fi(b, 1, i, j + 1, k);
+  long h0 = fi(b, 2, i, j, k), h1 = fi(b, 2, i, j, k + 1);
+  double Ai0 = b.fA[0][f0], Ai1 = b.fA[0][f1], Aj0 = b.fA[1][g0], Aj1 = b.fA[1][g1];
+  double Ak0 = b.fA[2][h0], Ak1 = b.fA[2][h1];
+  double n[3][6];  // [component][face]
+  n[0][0] = b.fx[0][f0]; n[1][0] = b.fy[0][f0]; n[2][0] = b.fz[0][f0];
+  n[0][1] = b.fx[1][g0]; n[1][1] = b.fy[1][g0]; n[2][1] = b.fz[1][g0];
+  n[0][2] = b.fx[2][h0]; n[1][2] = b.fy[2][h0]; n[2][2] = b.fz[2][h0];
+  n[0][3] = b.fx[0][f1]; n[1][3] = b.fy[0][f1]; n[2][3] = b.fz[0][f1];
+  n[0][4] = b.fx[1][g1]; n[1][4] = b.fy[1][g1]; n[2][4] = b.fz[1][g1];
+  n[0][5] = b.fx[2][h1]; n[1][5] = b.fy[2][h1]; n[2][5] = b.fz[2][h1];
+  double vol2 = 2 * b.vol[c];
+  const long si = 1, sj = b.pi, sk = (long)b.pi * b.pj;
+  long gidx = gri(b, i, j, k);
+  bool bad = false;
+  int ndir = (b.kmx > 2) ? 3 : 2;
+  for (int l = 0; l < b.ng; ++l) {
+    const double *v = grad_var(b, l);
+    double vc = v[c];
+    double sm_i = v[c - si] + vc, sm_j = v[c - sj] + vc, sm_k = v[c - sk] + vc;
+    double sp_i = v[c + si] + vc, sp_j = v[c + sj] + vc, sp_k = v[c + sk] + vc;
+    for (int d = 0; d < 3; ++d) {
+      double g;
+      if (d < ndir) {
+        g = -(sm_i * n[d][0] * Ai0);
+        g = g - sm_j * n[d][1] * Aj0;
+        g = g - sm_k * n[d][2] * Ak0;
+        g = g + sp_i * n[d][3] * Ai1;
+        g = g + sp_j * n[d][4] * Aj1;
+        g = g + sp_k * n[d][5] * Ak1;
+        g = g / vol2;
+      } else {
+        g = 0.0;
+      }
+      bad |= isnan(g);
+      b.g[d][(long)l * b.ngr + gidx] = g;
+    }
+  }
+  if (bad) atomicOr(b.err, ERR_NAN_GRAD);
+}
+
+// ghost-cell gradients on the face interiors of every face with a
+// boundary id < 0.  The reference passes Jfaces/Kfaces to a dummy declared
+// with the I-face extents, so the J/K metrics are read at the I-face linear
+// offset (src/gradients.f90:523-588 calling :591); that is reproduced.
+__global__ void k_grad_bc(Blk b) {
+  int face = blockIdx.z;
+  int a = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int bb = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  int na = face_na(b, face), nb = face_nb(b, face);
+  if (a > na - 1 || bb > nb - 1) return;
+  int id = b.bc[face];
+  if (id >= 0) return;
+  int d = face >> 1, hi = face & 1;
+  double Tw = b.fix[FX_TWALL][face];
+  // face position and the interior (I) / ghost (G) cells
+  int fi_, fj_, fk_;
+  int N = d == 0 ? b.imx : d == 1 ? b.jmx : b.kmx;
+  int fp = hi ? N : 1;
+  if (d == 0) { fi_ = fp; fj_ = a; fk_ = bb; } else if (d == 1) { fi_ = a; fj_ = fp; fk_ = bb; } else { fi_ = a; fj_ = bb; fk_ = fp; }
+  int Ii = fi_, Ij = fj_, Ik = fk_, Gi = fi_, Gj = fj_, Gk = fk_;
+  if (hi) { if (d == 0) Ii--; else if (d == 1) Ij--; else Ik--; }
+  else { if (d == 0) Gi--; else if (d == 1) Gj--; else Gk--; }
+  long fo = fi(b, 0, fi_, fj_, fk_);  // I-face offset for every direction
+  double nx = b.fx[d][fo], ny = b.fy[d][fo], nz = b.fz[d][fo], A = b.fA[d][fo];
+  long cI = ci(b, Ii, Ij, Ik), cG = ci(b, Gi, Gj, Gk);
+  double vI = b.vol[cI];
+  double c_x = A * nx / vI, c_y = A * ny / vI, c_z = A * nz / vI;
+  double sig = hi ? -1.0 : 1.0;
+  long gI = gri(b, Ii, Ij, Ik), gG = gri(b, Gi, Gj, Gk);
+  double *gx = b.g[0], *gy = b.g[1], *gz = b.g[2];
+  for (int l = 0; l < b.ng; ++l) {
+    double dq = b.qp[(long)(l + 1) * b.nc + cI] - b.qp[(long)(l + 1) * b.nc + cG];
+    if (l == 3) dq = b.temp[cI] - b.temp[cG];
+    double ox = sig * dq * c_x, oy = sig * dq * c_y, oz = sig * dq * c_z;
+    long o = (long)l * b.ngr;
+    if (l == 3 && id == -5 && (Tw < 1. && Tw >= 0.)) {
+      ox = -gx[o + gI]; oy = -gy[o + gI]; oz = -gz[o + gI];
+    }
+    double ix = gx[o + gI], iy = gy[o + gI], iz = gz[o + gI];
+    double dot = (ix * nx) + (iy * ny) + (iz * nz);
+    gx[o + gG] = ox + (ix - dot * nx);
+    gy[o + gG] = oy + (iy - dot * ny);
+    gz[o + gG] = oz + (iz - dot * nz);
+  }
+}
+
+// mu (Sutherland) and SST mu_t / F1 on cells 0..imx (src/viscosity.f90:52-388)
+__global__ void __launch_bounds__(256) k_visc(Blk b) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int k = blockIdx.z;
+  if (i > b.imx || j > b.jmx || k > b.kmx) return;
+  long c = ci(b, i, j, k);
+  double rho = b.qp[c];
+  double mu;
+  if (b.mu_suth) {
+    double T = b.qp[4 * b.nc + c] / (rho * b.R);
+    mu = b.mu_ref * (pow(T / b.T_ref, 1.5)) * ((b.T_ref + b.S_ref) / (T + b.S_ref));
+    b.mu[c] = mu;
+    if (isnan(mu)) atomicOr(b.err, ERR_NAN_MU);
+  } else {
+    mu = b.mu[c];
+  }
+  if (!b.turb) return;
+  const double bstar = 0.09, a1 = 0.31, sigma_w2 = 0.856;
+  double tk = b.qp[5 * b.nc + c], tw = b.qp[6 * b.nc + c], dist = b.dist[c];
+  long g = gri(b, i, j, k);
+  const double *gx = b.g[0], *gy = b.g[1], *gz = b.g[2];
+  long ngr = b.ngr;
+  double var1 = sqrt(tk) / (bstar * tw * dist);
+  double var2 = 500 * (mu / rho) / ((dist * dist) * tw);
+  double arg2 = fmx(2 * var1, var2);
+  double F = tanh(arg2 * arg2);
+  double wx = (gy[2 * ngr + g] - gz[1 * ngr + g]);
+  double wy = (gz[0 * ngr + g] - gx[2 * ngr + g]);
+  double wz = (gx[1 * ngr + g] - gy[0 * ngr + g]);
+  double vort = sqrt(wx * wx + wy * wy + wz * wz);
+  double NUM = rho * a1 * tk;
+  double DENOM = fmx(fmx((a1 * tw), vort * F), 1.e-20);
+  b.mut[c] = NUM / DENOM;
+  double CD = fmx(2 * rho * sigma_w2 * (gx[4 * ngr + g] * gx[5 * ngr + g] + gy[4 * ngr + g] * gy[5 * ngr + g] +
+                                        gz[4 * ngr + g] * gz[5 * ngr + g]) / tw,
+                  1e-20);
+  double right = 4 * (rho * sigma_w2 * tk) / (CD * (dist * dist));
+  double left = fmx(var1, var2);
+  double arg1 = fmn(left, right);
+  b.f1[c] = tanh(arg1 * arg1 * arg1 * arg1);
+}
+
+// mu_t / F1 ghost rules (src/viscosity.f90:407-465)
+__global__ void k_visc_ghost(Blk b) {
+  int face = blockIdx.z;
+  int a = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int bb = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  int na = face_na(b, face), nb = face_nb(b, face);
+  if (a > na - 1 || bb > nb - 1) return;
+  int id = b.bc[face];
+  if (!(id == -5 || (id <= -1 && id >= -9))) return;
+  long cg = fcell(b, face, 0, a, bb), cn = fcell(b, face, 1, a, bb);
+  b.mut[cg] = (id == -5) ? -b.mut[cn] : b.mut[cn];
+  b.f1[cg] = b.f1[cn];
+}
+
+}  // namespace fest

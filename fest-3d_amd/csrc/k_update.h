@@ -1,0 +1,257 @@
+// k_update.h -- residue (flux divergence, src/face/flux/convective/
+// scheme.f90:111-141), SST source (src/source.f90:158-270), the
+// conservative RK stage update (src/update.f90:367-485), the local/global
+// time step (src/time.f90:122-531) and the residual norms
+// (src/resnorm.f90:171-225).
+#pragma once
+#include "blk.h"
+
+namespace fest {
+
+// Temp = p/(R rho) on the whole padded array (src/update.f90:170) and the RK
+// base U_store = qp (src/update.f90:178)
+__global__ void k_step_begin(Blk b, int store_u) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= b.nc) return;
+  double rho = b.qp[t];
+  b.temp[t] = b.qp[4 * b.nc + t] / (b.R * rho);
+  if (store_u)
+    for (int l = 0; l < b.nv; ++l) b.ust[(long)l * b.nc + t] = b.qp[(long)l * b.nc + t];
+}
+
+__device__ __forceinline__ double sspd(const Blk &b, int d, long o) { return b.csnd[d][o]; }
+
+// local time step of one cell (src/time.f90:122-246, 366-531)
+__global__ void __launch_bounds__(256) k_dt(Blk b) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  int k = blockIdx.z + 1;
+  bool active = (i <= b.imx - 1 && j <= b.jmx - 1 && k <= b.kmx - 1);
+  double dtv = 1e300;
+  if (active) {
+    const long nc = b.nc;
+    const double *qp = b.qp;
+    long c = ci(b, i, j, k);
+    long c4 = c + 1, c5 = c + b.pi, c6 = c + (long)b.pi * b.pj;
+    long f1 = fi(b, 0, i, j, k), f2 = fi(b, 1, i, j, k), f3 = fi(b, 2, i, j, k);
+    long f4 = fi(b, 0, i + 1, j, k), f5 = fi(b, 1, i, j + 1, k), f6 = fi(b, 2, i, j, k + 1);
+    double u = qp[nc + c], v = qp[2 * nc + c], w = qp[3 * nc + c];
+    double l1 = fabs((u * b.fx[0][f1]) + (v * b.fy[0][f1]) + (w * b.fz[0][f1])) + sspd(b, 0, fli(b, 0, i, j, k));
+    double l2 = fabs((u * b.fx[1][f2]) + (v * b.fy[1][f2]) + (w * b.fz[1][f2])) + sspd(b, 1, fli(b, 1, i, j, k));
+    double l3 = fabs((u * b.fx[2][f3]) + (v * b.fy[2][f3]) + (w * b.fz[2][f3])) + sspd(b, 2, fli(b, 2, i, j, k));
+    double l4 = fabs((qp[nc + c4] * b.fx[0][f4]) + (qp[2 * nc + c4] * b.fy[0][f4]) + (qp[3 * nc + c4] * b.fz[0][f4])) +
+                sspd(b, 0, fli(b, 0, i + 1, j, k));
+    double l5 = fabs((qp[nc + c5] * b.fx[1][f5]) + (qp[2 * nc + c5] * b.fy[1][f5]) + (qp[3 * nc + c5] * b.fz[1][f5])) +
+                sspd(b, 1, fli(b, 1, i, j + 1, k));
+    double l6 = fabs((qp[nc + c6] * b.fx[2][f6]) + (qp[2 * nc + c6] * b.fy[2][f6]) + (qp[3 * nc + c6] * b.fz[2][f6])) +
+                sspd(b, 2, fli(b, 2, i, j, k + 1));
+    double A1 = b.fA[0][f1], A2 = b.fA[1][f2], A3 = b.fA[2][f3], A4 = b.fA[0][f4], A5 = b.fA[1][f5], A6 = b.fA[2][f6];
+    double lsum = (A1 * l1) + (A2 * l2) + (A3 * l3) + (A4 * l4) + (A5 * l5) + (A6 * l6);
+    double vol = b.vol[c];
+    dtv = 1. / lsum;
+    dtv = dtv * vol * b.CFLv;
+    if (b.mu_ref != 0.0) {
+      long c1 = c - 1, c2 = c - b.pi, c3 = c - (long)b.pi * b.pj;
+      const double *cx = b.cx, *cy = b.cy, *cz = b.cz;
+      double rho = qp[c];
+      for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1 && !b.turb) break;
+        const double *m = pass == 0 ? b.mu : b.mut;
+        double prn = pass == 0 ? b.Pr : b.tPr;
+        double m1 = m[c] / (rho * fabs(((cx[c1] - cx[c]) * b.fx[0][f1]) + ((cy[c1] - cy[c]) * b.fy[0][f1]) + ((cz[c1] - cz[c]) * b.fz[0][f1])));
+        double m2 = m[c] / (rho * fabs(((cx[c2] - cx[c]) * b.fx[1][f2]) + ((cy[c2] - cy[c]) * b.fy[1][f2]) + ((cz[c2] - cz[c]) * b.fz[1][f2])));
+        double m3 = m[c] / (rho * fabs(((cx[c3] - cx[c]) * b.fx[2][f3]) + ((cy[c3] - cy[c]) * b.fy[2][f3]) + ((cz[c3] - cz[c]) * b.fz[2][f3])));
+        double m4 = m[c4] / (qp[c4] * fabs(((cx[c] - cx[c4]) * b.fx[0][f4]) + ((cy[c] - cy[c4]) * b.fy[0][f4]) + ((cz[c] - cz[c4]) * b.fz[0][f4])));
+        double m5 = m[c5] / (qp[c5] * fabs(((cx[c] - cx[c5]) * b.fx[1][f5]) + ((cy[c] - cy[c5]) * b.fy[1][f5]) + ((cz[c] - cz[c5]) * b.fz[1][f5])));
+        double m6 = m[c6] / (qp[c6] * fabs(((cx[c] - cx[c6]) * b.fx[2][f6]) + ((cy[c] - cy[c6]) * b.fy[2][f6]) + ((cz[c] - cz[c6]) * b.fz[2][f6])));
+        double ls = (A1 * m1) + (A2 * m2) + (A3 * m3) + (A4 * m4) + (A5 * m5) + (A6 * m6);
+        ls = b.gm * ls / prn;
+        ls = 2. / (ls + (2. * b.CFLv * vol / dtv));
+        dtv = b.CFLv * (ls * vol);
+      }
+    }
+    b.dt[ri(b, i, j, k)] = dtv;
+  }
+  if (!b.time_local) {
+    // block minimum for the global step (src/time.f90:248-289)
+    __shared__ double smin[256];
+    int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    smin[tid] = dtv;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+      if (tid < st) smin[tid] = fmn(smin[tid], smin[tid + st]);
+      __syncthreads();
+    }
+    if (tid == 0) {
+      long blk = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+      b.red[blk] = smin[0];
+    }
+  }
+}
+
+// global step: minimum of the block minima, broadcast into delta_t
+__global__ void k_dt_min(Blk b, long nblk) {
+  __shared__ double s[256];
+  double m = 1e300;
+  for (long t = threadIdx.x; t < nblk; t += blockDim.x) m = fmn(m, b.red[t]);
+  s[threadIdx.x] = m;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) s[threadIdx.x] = fmn(s[threadIdx.x], s[threadIdx.x + st]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) b.dtmin[0] = s[0];
+}
+__global__ void k_dt_fill(Blk b, int use_fixed) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= b.nres) return;
+  b.dt[t] = use_fixed ? b.global_dt : b.dtmin[0];
+}
+
+enum { UPD_FIRST = 1, UPD_LAST = 2, UPD_WRITE_RES = 4, UPD_APPLY = 8, UPD_STORE = 16 };
+
+// residue + SST source, then (UPD_APPLY) the conservative update of the cell
+__global__ void __launch_bounds__(256) k_update(Blk b, double TF, double SF, int flags) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  int k = blockIdx.z + 1;
+  if (i > b.imx - 1 || j > b.jmx - 1 || k > b.kmx - 1) return;
+  const int nv = b.nv;
+  const long nc = b.nc;
+  long c = ci(b, i, j, k);
+  long r = ri(b, i, j, k);
+  long F0 = fli(b, 0, i, j, k), G0 = fli(b, 1, i, j, k), H0 = fli(b, 2, i, j, k);
+  long sG = b.nfi[1], sH = (long)b.nfi[2] * b.nfj[2];
+  double R[7];
+  for (int l = 0; l < nv; ++l) {
+    const double *F = b.flux[0] + (long)l * b.nfl[0];
+    const double *G = b.flux[1] + (long)l * b.nfl[1];
+    const double *H = b.flux[2] + (long)l * b.nfl[2];
+    R[l] = (F[F0 + 1] - F[F0]) + (G[G0 + sG] - G[G0]) + (H[H0 + sH] - H[H0]);
+  }
+  double rho = b.qp[c];
+  if (b.turb) {
+    // src/source.f90:158-270
+    const double sigma_w1 = 0.5, sigma_w2 = 0.856, beta1 = 0.075, beta2 = 0.0828, bstar = 0.09, kap = 0.41;
+    const double gama1 = (beta1 / bstar) - ((sigma_w1 * (kap * kap)) / sqrt(bstar));
+    const double gama2 = (beta2 / bstar) - ((sigma_w2 * (kap * kap)) / sqrt(bstar));
+    long g = gri(b, i, j, k);
+    const long ngr = b.ngr;
+    const double *gx = b.g[0], *gy = b.g[1], *gz = b.g[2];
+    double tk = b.qp[5 * nc + c], tw = b.qp[6 * nc + c];
+    double vort = sqrt(sqd(gy[2 * ngr + g] - gz[1 * ngr + g]) + sqd(gz[0 * ngr + g] - gx[2 * ngr + g]) +
+                       sqd(gx[1 * ngr + g] - gy[0 * ngr + g]));
+    double CD = 2 * rho * sigma_w2 * (gx[4 * ngr + g] * gx[5 * ngr + g] + gy[4 * ngr + g] * gy[5 * ngr + g] +
+                                      gz[4 * ngr + g] * gz[5 * ngr + g]) / tw;
+    CD = fmx(CD, 1e-20);
+    double F1 = b.f1[c];
+    double gama = gama1 * F1 + gama2 * (1. - F1);
+    double beta = beta1 * F1 + beta2 * (1. - F1);
+    double D_k = bstar * rho * tw * tk;
+    double D_w = beta * rho * (tw * tw);
+    double divergence = gx[g] + gy[ngr + g] + gz[2 * ngr + g];
+    double mut = b.mut[c];
+    double P_k = mut * (vort * vort) - ((2.0 / 3.0) * rho * tk * divergence);
+    P_k = fmn(P_k, 20 * D_k);
+    double P_w = (rho * gama / mut) * P_k;
+    double lamda = (1. - F1) * CD;
+    double S_k = (P_k - D_k) * b.vol[c];
+    double S_w = (P_w - D_w + lamda) * b.vol[c];
+    R[5] = R[5] - S_k;
+    R[6] = R[6] - S_w;
+  }
+  if (flags & UPD_WRITE_RES)
+    for (int l = 0; l < nv; ++l) b.res[(long)l * b.nres + r] = R[l];
+  if (!(flags & UPD_APPLY)) return;
+  // src/update.f90:367-485
+  const double gm = b.gm;
+  const double *Qu = (flags & UPD_STORE) ? b.ust : b.qp;
+  double u1[7], u2[7];
+  u1[0] = Qu[c];
+  for (int l = 1; l < nv; ++l) u1[l] = Qu[(long)l * nc + c] * u1[0];
+  u1[4] = (u1[4] / (gm - 1.) + 0.5 * (u1[1] * u1[1] + u1[2] * u1[2] + u1[3] * u1[3])) / u1[0] + 0.0;
+  double dt = b.dt[r];
+  if (b.turb) {
+    double F1 = b.f1[c];
+    double beta = 0.075 * F1 + (1. - F1) * 0.0828;
+    double w7 = b.qp[6 * nc + c];
+    R[5] = R[5] / (1 + (beta * w7 * dt));
+    R[6] = R[6] / (1 + (2 * beta * w7 * dt));
+  }
+  if (flags & UPD_STORE) {
+    for (int l = 0; l < nv; ++l) {
+      double rn = (flags & UPD_FIRST) ? 0.0 + SF * R[l] : b.rst[(long)l * b.nres + r] + SF * R[l];
+      if (flags & UPD_LAST) R[l] = rn;
+      else b.rst[(long)l * b.nres + r] = rn;
+    }
+  }
+  double fac = TF * dt / b.vol[c];
+  for (int l = 0; l < nv; ++l) u2[l] = u1[l] - R[l] * fac;
+  for (int l = 1; l < nv; ++l) u2[l] = u2[l] / u2[0];
+  u2[4] = (gm - 1.) * u2[0] * (u2[4] - (0.5 * (u2[1] * u2[1] + u2[2] * u2[2] + u2[3] * u2[3])) - 0.0);
+  bool bad = (u2[0] < 0.) || (u2[4] < 0.);
+  for (int l = 0; l < nv; ++l) bad |= isnan(u2[l]);
+  if (bad) {
+    atomicOr(b.err, ERR_NONPHYS);
+    return;
+  }
+  for (int l = 0; l < 5; ++l) b.qp[(long)l * nc + c] = u2[l];
+  if (b.turb) {
+    if (u2[5] >= 0.) b.qp[5 * nc + c] = u2[5];
+    if (u2[6] >= 0.) b.qp[6 * nc + c] = u2[6];
+  }
+}
+
+// per-block sums of residue^2 and of the boundary mass fluxes
+__global__ void k_norm_partial(Blk b) {
+  __shared__ double s[256][8];
+  const int nv = b.nv;
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  long n = b.nres;
+  long stride = (long)gridDim.x * blockDim.x;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
+    for (int l = 0; l < nv; ++l) {
+      double r = b.res[(long)l * n + t];
+      acc[1 + l] += r * r;
+    }
+    int i = (int)(t % (b.imx - 1)) + 1;
+    long q = t / (b.imx - 1);
+    int j = (int)(q % (b.jmx - 1)) + 1;
+    int k = (int)(q / (b.jmx - 1)) + 1;
+    if (i == 1) acc[0] += b.flux[0][fli(b, 0, 1, j, k)];
+    if (i == b.imx - 1) acc[0] -= b.flux[0][fli(b, 0, b.imx, j, k)];
+    if (j == 1) acc[0] += b.flux[1][fli(b, 1, i, 1, k)];
+    if (j == b.jmx - 1) acc[0] -= b.flux[1][fli(b, 1, i, b.jmx, k)];
+    if (k == 1) acc[0] += b.flux[2][fli(b, 2, i, j, 1)];
+    if (k == b.kmx - 1) acc[0] -= b.flux[2][fli(b, 2, i, j, b.kmx)];
+  }
+  for (int l = 0; l <= nv; ++l) s[threadIdx.x][l] = acc[l];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st)
+      for (int l = 0; l <= nv; ++l) s[threadIdx.x][l] += s[threadIdx.x + st][l];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int l = 0; l <= nv; ++l) b.red[(long)blockIdx.x * 8 + l] = s[0][l];
+}
+
+// final norms, src/resnorm.f90:128-225
+__global__ void k_norm_final(Blk b, int nblk) {
+  if (threadIdx.x != 0) return;
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int t = 0; t < nblk; ++t)
+    for (int l = 0; l <= b.nv; ++l) acc[l] += b.red[(long)t * 8 + l];
+  double v = b.vel_mag;
+  double scale[8];
+  scale[0] = 1.;
+  scale[1] = b.rho_inf * v;
+  scale[2] = scale[3] = scale[4] = b.rho_inf * v * v;
+  scale[5] = (0.5 * b.rho_inf * (v * v * v) + ((b.gm / (b.gm - 1.)) * b.p_inf));
+  scale[6] = b.rho_inf * v * b.tk_inf;
+  scale[7] = b.rho_inf * v * b.tw_inf;
+  b.norms[0] = fabs(acc[0] / scale[0]);
+  for (int l = 1; l <= b.nv; ++l) b.norms[l] = sqrt(acc[l] / (scale[l] * scale[l]));
+}
+
+}  // namespace fest

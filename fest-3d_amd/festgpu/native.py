@@ -1,0 +1,232 @@
+"""ctypes binding of libfest_hip.so, the C ABI declared in include/fest_hip.h.
+
+A `GpuBlock` mirrors the reference's `update` module surface for one block:
+``set_grid``/``upload_geometry`` (setup_geometry), ``upload_state``,
+``get_next_solution`` (src/update.f90:129-226), ``residual``
+(get_total_conservative_Residue, src/update.f90:495-547), ``time_step``
+(compute_time_step, src/time.f90:291-334) and ``resnorm`` (find_resnorm's
+absolute norms, src/resnorm.f90:171-225).  Errors raise `FestError`
+carrying the C ABI code, the counterpart of the reference's Fatal_error.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .cases import FLUX, RECON, TURB, TIME_ACC, Case
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libfest_hip.so")
+
+FEST_E = {-1: "FEST_E_NONPHYSICAL", -2: "FEST_E_NAN", -3: "FEST_E_CONFIG", -4: "FEST_E_HIP", -5: "FEST_E_ARG"}
+
+DL = {"qp": 1 << 0, "temp": 1 << 1, "residue": 1 << 2, "F": 1 << 3, "G": 1 << 4, "H": 1 << 5, "dt": 1 << 6,
+      "mu": 1 << 7, "mut": 1 << 8, "grads": 1 << 9, "f1": 1 << 10, "dist": 1 << 11, "cells": 1 << 12,
+      "faces": 1 << 13}
+
+
+class FestError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{FEST_E.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Extent(ctypes.Structure):
+    _fields_ = [("imx", ctypes.c_int32), ("jmx", ctypes.c_int32), ("kmx", ctypes.c_int32), ("n_var", ctypes.c_int32)]
+
+
+class Scheme(ctypes.Structure):
+    _fields_ = [("flux", ctypes.c_int32), ("recon", ctypes.c_int32), ("turb", ctypes.c_int32),
+                ("time_local", ctypes.c_int32), ("time_accuracy", ctypes.c_int32), ("accur", ctypes.c_int32),
+                ("lim", ctypes.c_int32 * 3), ("tlim", ctypes.c_int32 * 3), ("global_time_step", ctypes.c_double)]
+
+
+class Flow(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in
+                ("density_inf", "x_speed_inf", "y_speed_inf", "z_speed_inf", "pressure_inf", "tk_inf", "tw_inf",
+                 "vel_mag", "MInf", "gm", "R_gas", "mu_ref", "T_ref", "Sutherland_temp", "Pr", "tPr")] + \
+               [("mu_sutherland", ctypes.c_int32)]
+
+
+class BC(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_int32 * 6)] + [(n, ctypes.c_double * 6) for n in
+                ("fixed_density", "fixed_x_speed", "fixed_y_speed", "fixed_z_speed", "fixed_pressure",
+                 "fixed_tk", "fixed_tw", "fixed_wall_temperature")]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libfest_hip.so; there is no fallback path."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} is not built: run __graft_entry__.build() or make -C fest-3d_amd")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int
+        L.fest_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(Extent), ctypes.POINTER(Scheme),
+                                  ctypes.POINTER(Flow), ctypes.POINTER(BC), ip]
+        L.fest_destroy.argtypes = [vp]
+        L.fest_last_error.argtypes = [vp]
+        L.fest_last_error.restype = ctypes.c_char_p
+        L.fest_upload_geometry.argtypes = [vp, dp, dp, dp, dp, dp]
+        L.fest_set_grid.argtypes = [vp, dp, ip]
+        L.fest_upload_state.argtypes = [vp, dp, dp]
+        L.fest_get_next_solution.argtypes = [vp, ip, ctypes.c_double]
+        L.fest_residual.argtypes = [vp, ip]
+        L.fest_time_step.argtypes = [vp, ctypes.c_double]
+        L.fest_download.argtypes = [vp, ctypes.c_uint32] + [dp] * 14
+        L.fest_resnorm.argtypes = [vp, dp]
+        L.fest_device_ptr.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_size_t)]
+        L.fest_sync.argtypes = [vp]
+        for f in ("fest_create", "fest_upload_geometry", "fest_set_grid", "fest_upload_state",
+                  "fest_get_next_solution", "fest_residual", "fest_time_step", "fest_download", "fest_resnorm",
+                  "fest_device_ptr", "fest_sync"):
+            getattr(L, f).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def c_structs(case: Case):
+    e = Extent(case.imx, case.jmx, case.kmx, case.n_var)
+    s = Scheme()
+    s.flux, s.recon, s.turb = FLUX[case.flux], RECON[case.recon], TURB[case.turb]
+    s.time_local, s.time_accuracy, s.accur = case.time_local, TIME_ACC[case.time_accuracy], case.accur
+    s.lim[:] = list(case.lim)
+    s.tlim[:] = list(case.tlim)
+    s.global_time_step = case.global_time_step
+    f = Flow()
+    f.density_inf, f.x_speed_inf, f.y_speed_inf, f.z_speed_inf = case.rho_inf, case.u_inf, case.v_inf, case.w_inf
+    f.pressure_inf, f.tk_inf, f.tw_inf, f.vel_mag, f.MInf = case.p_inf, case.tk_inf, case.tw_inf, case.vel_mag, case.MInf
+    f.gm, f.R_gas, f.mu_ref, f.T_ref, f.Sutherland_temp = case.gm, case.R_gas, case.mu_ref, case.T_ref, case.S_ref
+    f.Pr, f.tPr, f.mu_sutherland = case.Pr, case.tPr, case.mu_sutherland
+    b = BC()
+    b.id[:] = list(case.bc_id)
+    b.fixed_density[:] = list(case.fix_rho)
+    b.fixed_x_speed[:] = list(case.fix_u)
+    b.fixed_y_speed[:] = list(case.fix_v)
+    b.fixed_z_speed[:] = list(case.fix_w)
+    b.fixed_pressure[:] = list(case.fix_p)
+    b.fixed_tk[:] = list(case.fix_tk)
+    b.fixed_tw[:] = list(case.fix_tw)
+    b.fixed_wall_temperature[:] = list(case.fix_twall)
+    return e, s, f, b
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if a is not None else None
+
+
+class GpuBlock:
+    """One device-resident block."""
+
+    def __init__(self, case: Case, precision: int = 0):
+        self.case = case
+        self.h = ctypes.c_void_p()
+        self._structs = c_structs(case)
+        e, s, f, b = self._structs
+        rc = lib().fest_create(ctypes.byref(self.h), ctypes.byref(e), ctypes.byref(s), ctypes.byref(f),
+                               ctypes.byref(b), precision)
+        self._check(rc)
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = lib().fest_last_error(self.h).decode() if self.h else ""
+            raise FestError(rc, msg)
+
+    def close(self):
+        if self.h:
+            lib().fest_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- setup
+    def set_grid(self, nodes: np.ndarray, wall_dist: bool = False):
+        nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+        assert nodes.shape == self.case.node_shape()
+        self._check(lib().fest_set_grid(self.h, _ptr(nodes), int(bool(wall_dist))))
+
+    def upload_geometry(self, cells4, if4, jf4, kf4, dist=None):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (cells4, if4, jf4, kf4)]
+        d = None if dist is None else np.ascontiguousarray(dist, dtype=np.float64)
+        self._check(lib().fest_upload_geometry(self.h, *[_ptr(a) for a in arrs], _ptr(d)))
+
+    def upload_state(self, qp: np.ndarray, temp: np.ndarray = None):
+        qp = np.ascontiguousarray(qp, dtype=np.float64)
+        assert qp.shape == self.case.cell_shape(self.case.n_var)
+        t = None if temp is None else np.ascontiguousarray(temp, dtype=np.float64)
+        self._check(lib().fest_upload_state(self.h, _ptr(qp), _ptr(t)))
+
+    # -- hot path
+    def get_next_solution(self, current_iter: int, cfl: float = None):
+        cfl = self.case.CFL if cfl is None else cfl
+        self._check(lib().fest_get_next_solution(self.h, int(current_iter), float(cfl)))
+
+    def residual(self, current_iter: int = 1):
+        self._check(lib().fest_residual(self.h, int(current_iter)))
+
+    def time_step(self, cfl: float = None):
+        cfl = self.case.CFL if cfl is None else cfl
+        self._check(lib().fest_time_step(self.h, float(cfl)))
+
+    def resnorm(self) -> np.ndarray:
+        out = np.zeros(8)
+        self._check(lib().fest_resnorm(self.h, _ptr(out)))
+        return out[: self.case.n_var + 1]
+
+    def sync(self):
+        self._check(lib().fest_sync(self.h))
+
+    # -- downloads in the reference layout
+    def shape_of(self, name):
+        c = self.case
+        nv, ng = c.n_var, (6 if c.turb == "sst" else 4)
+        cell = c.cell_shape()
+        return {
+            "qp": (nv,) + cell, "temp": cell, "mu": cell, "mut": cell, "f1": cell, "dist": cell,
+            "residue": (nv, c.kmx - 1, c.jmx - 1, c.imx - 1), "dt": (c.kmx - 1, c.jmx - 1, c.imx - 1),
+            "F": (nv, c.kmx - 1, c.jmx - 1, c.imx), "G": (nv, c.kmx - 1, c.jmx, c.imx - 1),
+            "H": (nv, c.kmx, c.jmx - 1, c.imx - 1), "grads": (3, ng, c.kmx + 1, c.jmx + 1, c.imx + 1),
+            "cells": (4,) + cell,
+        }[name]
+
+    def download(self, *names):
+        order = ["qp", "temp", "residue", "F", "G", "H", "dt", "mu", "mut", "grads", "f1", "dist", "cells", "faces"]
+        bufs = {}
+        mask = 0
+        for n in names:
+            if n == "faces":
+                c = self.case
+                nf = [(c.kmx + 5) * (c.jmx + 5) * (c.imx + 6), (c.kmx + 5) * (c.jmx + 6) * (c.imx + 5),
+                      (c.kmx + 6) * (c.jmx + 5) * (c.imx + 5)]
+                bufs[n] = np.zeros(4 * sum(nf))
+            else:
+                bufs[n] = np.zeros(self.shape_of(n))
+            mask |= DL[n]
+        args = [_ptr(bufs.get(n)) for n in order]
+        self._check(lib().fest_download(self.h, mask, *args))
+        if "faces" in bufs:
+            c = self.case
+            flat, out, off = bufs["faces"], {}, 0
+            shapes = [(c.kmx + 5, c.jmx + 5, c.imx + 6), (c.kmx + 5, c.jmx + 6, c.imx + 5),
+                      (c.kmx + 6, c.jmx + 5, c.imx + 5)]
+            for d, shp in enumerate(shapes):
+                n = int(np.prod(shp))
+                out[d] = flat[off: off + 4 * n].reshape((4,) + shp)
+                off += 4 * n
+            bufs["faces"] = out
+        return bufs if len(names) > 1 else bufs[names[0]]
+
+    def device_ptr(self, name):
+        p = ctypes.c_void_p()
+        n = ctypes.c_size_t()
+        self._check(lib().fest_device_ptr(self.h, name.encode(), ctypes.byref(p), ctypes.byref(n)))
+        return p.value, n.value

@@ -1,0 +1,163 @@
+/*
+ * fest_hip.h -- C ABI of the MI355X explicit-residual engine for FEST-3D.
+ *
+ * The entry points replace the body of the reference's time-integration
+ * module and the residual passes it calls; a Fortran host binds them with
+ * ISO_C_BINDING and keeps the reference's call surface:
+ *
+ *   setup_update(control, scheme, flow, dims)          src/update.f90:85-126
+ *   get_next_solution(qp, Temp, residue, delta_t, cells, F, G, H,
+ *                     Ifaces, Jfaces, Kfaces, control, scheme, flow, bc, dims)
+ *                                                      src/update.f90:129-226
+ *   get_total_conservative_Residue(...)                src/update.f90:495-547
+ *   compute_time_step(...)                             src/time.f90:291-334
+ *   find_resnorm(...) (absolute norms)                 src/resnorm.f90:62-91,171-225
+ *
+ * Conventions
+ *   - Host arrays are plain double arrays in the reference's Fortran layout
+ *     (column-major, explicit lower bounds); derived-type arrays are passed
+ *     component by component (the Fortran side extracts cells%volume,
+ *     Ifaces%nx, ...).  Nothing is retained after a call returns.
+ *   - State and geometry stay resident on the device between calls.
+ *   - Every function returns 0 on success or a negative FEST_E_* code;
+ *     fest_last_error() gives the message.  The Fortran wrapper turns a
+ *     nonzero code into the print + STOP of the reference's Fatal_error
+ *     (src/error.h:17).
+ *   - One host thread per handle; calls on one handle are not re-entrant.
+ */
+#ifndef FEST_HIP_H
+#define FEST_HIP_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define FEST_OK 0
+#define FEST_E_NONPHYSICAL (-1) /* rho<0, p<0 or NaN in the update, src/update.f90:448-452 */
+#define FEST_E_NAN (-2)         /* NaN in fluxes, gradients or mu (the reference's isnan passes) */
+#define FEST_E_CONFIG (-3)      /* unknown/unsupported scheme, BC or extent */
+#define FEST_E_HIP (-4)         /* HIP runtime error or no device */
+#define FEST_E_ARG (-5)         /* bad argument (null pointer, bad mask) */
+
+/* scheme ids; the Fortran side maps schemetype strings onto them */
+enum { FEST_FLUX_AUSM = 0, FEST_FLUX_AUSMP = 1, FEST_FLUX_AUSMUP = 2,
+       FEST_FLUX_SLAU = 3, FEST_FLUX_LDFSS0 = 4, FEST_FLUX_VANLEER = 5 };
+enum { FEST_RECON_NONE = 0, FEST_RECON_MUSCL = 1, FEST_RECON_PPM = 2,
+       FEST_RECON_WENO = 3 };
+enum { FEST_TURB_NONE = 0, FEST_TURB_SST = 1 };
+enum { FEST_TA_NONE = 0, FEST_TA_RK2 = 2, FEST_TA_RK4 = 4 };
+
+/* type(extent), src/vartypes.f90:21-26 */
+typedef struct { int32_t imx, jmx, kmx, n_var; } fest_extent;
+
+/* the hot-path fields of type(schemetype), src/vartypes.f90:195-236 */
+typedef struct {
+  int32_t flux;            /* scheme_name  -> FEST_FLUX_*                */
+  int32_t recon;           /* interpolant  -> FEST_RECON_*               */
+  int32_t turb;            /* turbulence   -> FEST_TURB_*                */
+  int32_t time_local;      /* time_stepping_method: 1 'l', 0 'g'         */
+  int32_t time_accuracy;   /* time_step_accuracy -> FEST_TA_*            */
+  int32_t accur;           /* accur (higher-order symmetry copy)         */
+  int32_t lim[3];          /* i/j/klimiter_switch                        */
+  int32_t tlim[3];         /* i/j/ktlimiter_switch                       */
+  double global_time_step; /* global_time_step (used when > 0, 'g')      */
+} fest_scheme;
+
+/* the hot-path fields of type(flowtype), src/vartypes.f90:239-290 */
+typedef struct {
+  double density_inf, x_speed_inf, y_speed_inf, z_speed_inf, pressure_inf;
+  double tk_inf, tw_inf, vel_mag, MInf;
+  double gm, R_gas, mu_ref, T_ref, Sutherland_temp, Pr, tPr;
+  int32_t mu_sutherland;   /* mu_variation: 0 'constant', 1 'sutherland_law' */
+} fest_flow;
+
+/* the hot-path fields of type(boundarytype), src/vartypes.f90:293-365 */
+typedef struct {
+  int32_t id[6];           /* imin,imax,jmin,jmax,kmin,kmax ids          */
+  double fixed_density[6], fixed_x_speed[6], fixed_y_speed[6], fixed_z_speed[6];
+  double fixed_pressure[6], fixed_tk[6], fixed_tw[6], fixed_wall_temperature[6];
+} fest_bc;
+
+typedef struct fest_handle fest_handle;
+
+/* Allocates the device-resident block.  precision: 0 = plain FP64.
+ * Replaces the allocations of setup_update/setup_scheme/setup_gradients/
+ * setup_viscosity (src/update.f90:85-126, src/face/flux/convective/
+ * scheme.f90:29-65, src/gradients.f90:102-200, src/viscosity.f90:548-592). */
+int fest_create(fest_handle **h, const fest_extent *dims, const fest_scheme *scheme,
+                const fest_flow *flow, const fest_bc *bc, int precision);
+void fest_destroy(fest_handle *h);
+const char *fest_last_error(const fest_handle *h);
+
+/* Geometry, option 1: upload the reference's metrics as the solver holds
+ * them after setup_geometry (src/geometry.f90:548-580): cells4 = four
+ * padded cell planes [volume, centerx, centery, centerz]; if4/jf4/kf4 = four
+ * face planes [A, nx, ny, nz] each; dist = wall distance (nullable, padded
+ * cell layout, src/wall/wall_dist.f90:22). */
+int fest_upload_geometry(fest_handle *h, const double *cells4, const double *if4,
+                         const double *jf4, const double *kf4, const double *dist);
+
+/* Geometry, option 2: compute it on the device from the grid nodes, three
+ * planes over (-2:imx+3,-2:jmx+3,-2:kmx+3) with the interior 1..imx filled:
+ * ghost nodes (src/grid.f90:137-236), face areas/normals, volumes and
+ * centroids (src/geometry.f90:50-545) and, when with_wall_dist != 0, the
+ * brute-force wall distance to the no-slip nodes (src/wall/wall.f90:198-288,
+ * src/wall/wall_dist.f90:84-131). */
+int fest_set_grid(fest_handle *h, const double *nodes, int with_wall_dist);
+
+/* Upload the primitive state qp (-2:imx+2,-2:jmx+2,-2:kmx+2,1:n_var) and,
+ * optionally, Temp (nullable). */
+int fest_upload_state(fest_handle *h, const double *qp, const double *temp);
+
+/* One full time step of get_next_solution (src/update.f90:129-226),
+ * including the Temp refresh at the start (src/update.f90:170) and the
+ * stage-1 time step.  current_iter feeds the inflow 'fix' rule
+ * (src/boundary/bc_primitive.f90:236,318,366). */
+int fest_get_next_solution(fest_handle *h, int current_iter, double cfl);
+
+/* One residual evaluation, get_total_conservative_Residue
+ * (src/update.f90:495-547), on the resident state and Temp. */
+int fest_residual(fest_handle *h, int current_iter);
+
+/* compute_time_step (src/time.f90:291-334) from the face states of the
+ * last residual evaluation. */
+int fest_time_step(fest_handle *h, double cfl);
+
+/* Download selected arrays in the reference layout.  Each pointer may be
+ * NULL when its bit is not set in `what`. */
+#define FEST_DL_QP      (1u << 0)
+#define FEST_DL_TEMP    (1u << 1)
+#define FEST_DL_RESIDUE (1u << 2)
+#define FEST_DL_F       (1u << 3)
+#define FEST_DL_G       (1u << 4)
+#define FEST_DL_H       (1u << 5)
+#define FEST_DL_DT      (1u << 6)
+#define FEST_DL_MU      (1u << 7)
+#define FEST_DL_MUT     (1u << 8)
+#define FEST_DL_GRADS   (1u << 9)  /* gradqp_x,y,z: 3*n_grad planes (0:imx,0:jmx,0:kmx) */
+#define FEST_DL_F1      (1u << 10)
+#define FEST_DL_DIST    (1u << 11)
+#define FEST_DL_CELLS   (1u << 12) /* cells4 as in fest_upload_geometry */
+#define FEST_DL_FACES   (1u << 13) /* if4, jf4, kf4 concatenated */
+int fest_download(fest_handle *h, uint32_t what, double *qp, double *temp, double *residue,
+                  double *F, double *G, double *H, double *delta_t, double *mu, double *mu_t,
+                  double *grads, double *sst_f1, double *dist, double *cells4, double *faces);
+
+/* Absolute residual norms Res_abs(0:n_var) of the last residue
+ * (src/resnorm.f90:171-225): [|mass imbalance|, sqrt(sum R_l^2)/scale_l]. */
+int fest_resnorm(fest_handle *h, double *res_abs);
+
+/* Device pointers of the resident arrays for zero-copy consumers (tests,
+ * the autograd harness).  name: "qp","temp","residue","F","G","H","dt". */
+int fest_device_ptr(fest_handle *h, const char *name, void **ptr, size_t *count);
+
+/* Synchronise the handle's stream. */
+int fest_sync(fest_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

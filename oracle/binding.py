@@ -113,6 +113,10 @@ class Oracle:
             for f in ("orc_residual", "orc_step"):
                 getattr(L, f).argtypes = [ctypes.c_void_p, ctypes.c_int]
                 getattr(L, f).restype = ctypes.c_int
+            L.orc_stage.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+            L.orc_stage.restype = ctypes.c_int
+            dp = ctypes.POINTER(ctypes.c_double)
+            L.orc_face_flux.argtypes = [ctypes.c_void_p, dp, dp] + [ctypes.c_double] * 4 + [ctypes.c_int, dp]
             L.orc_temp_refresh.argtypes = [ctypes.c_void_p]
             L.orc_resnorm.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
             cls._lib = L
@@ -156,6 +160,19 @@ class Oracle:
 
     def time_step(self):
         return self.lib().orc_time_step(self.h)
+
+    def stage(self, it, mask):
+        return self.lib().orc_stage(self.h, it, mask)
+
+    def face_flux(self, qL, qR, n, A=1.0, zero=1):
+        nv = self.case.n_var
+        a = np.ascontiguousarray(qL, dtype=np.float64)
+        b = np.ascontiguousarray(qR, dtype=np.float64)
+        out = np.zeros(nv)
+        dp = ctypes.POINTER(ctypes.c_double)
+        self.lib().orc_face_flux(self.h, a.ctypes.data_as(dp), b.ctypes.data_as(dp), float(n[0]), float(n[1]),
+                                 float(n[2]), float(A), int(zero), out.ctypes.data_as(dp))
+        return out
 
     def step(self, it):
         return self.lib().orc_step(self.h, it)

@@ -1442,6 +1442,28 @@ void orc_temp_refresh(orc *s) {
   for (size_t n = 0; n < s->ncell; ++n) T[n] = QPL(4)[n] / (s->c.R_gas * QPL(0)[n]);
 }
 
+int orc_stage(orc *s, int current_iter, int mask) {
+  int st;
+  if ((mask & 1) && (st = populate_ghost_primitive(s, current_iter))) return st;
+  if ((mask & 2) && (st = compute_face_interpolant(s))) return st;
+  if ((mask & 4) && (st = compute_fluxes(s))) return st;
+  if ((mask & 8) && s->c.mu_ref != 0.0) {
+    if ((st = evaluate_all_gradients(s))) return st;
+    if ((st = calculate_viscosity(s))) return st;
+    if ((st = compute_viscous_fluxes(s))) return st;
+  }
+  if (mask & 16) {
+    compute_residue(s);
+    if (s->c.turb == ORC_TURB_SST) add_sst_source(s);
+  }
+  return ORC_OK;
+}
+
+void orc_face_flux(orc *s, const double *qL, const double *qR, double nx, double ny, double nz, double A,
+                   int zero, double *out) {
+  face_flux(s, qL, qR, nx, ny, nz, A, zero, out);
+}
+
 int orc_residual(orc *s, int current_iter) {
   int st;
   if ((st = populate_ghost_primitive(s, current_iter))) return st;

@@ -78,6 +78,13 @@ int orc_wall_dist(orc *s);
 
 /* Temp = p/(R rho) over the padded array (src/update.f90:170). */
 void orc_temp_refresh(orc *s);
+/* Selected passes of one residual evaluation: mask bit 0 ghost cells,
+ * 1 face states (+ boundary states), 2 convective fluxes, 3 gradients,
+ * viscosity and viscous fluxes, 4 residue and turbulence source. */
+int orc_stage(orc *s, int current_iter, int mask);
+/* One convective face flux of the selected scheme (n_var values). */
+void orc_face_flux(orc *s, const double *qL, const double *qR, double nx, double ny, double nz, double A,
+                   int zero, double *out);
 /* One residual evaluation C2..C10 (src/update.f90:495-547). */
 int orc_residual(orc *s, int current_iter);
 /* Local / global time step (src/time.f90:122-334). */

@@ -1,0 +1,62 @@
+"""The C ABI library without a GPU: it loads, it exports every entry point
+declared in include/fest_hip.h, and it rejects bad configurations with
+FEST_E_CONFIG before touching a device (the reference's Fatal_error on an
+unknown scheme or boundary, src/error.h:17)."""
+import ctypes
+import dataclasses
+import os
+import re
+
+import pytest
+
+from festgpu import cases as C
+from festgpu import native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "fest_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fest_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = native.lib()
+    names = declared_functions()
+    assert len(names) >= 12
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_struct_sizes_match_header():
+    # fest_extent is four int32; fest_scheme 12 int32 + one double
+    assert ctypes.sizeof(native.Extent) == 16
+    assert ctypes.sizeof(native.Scheme) == 56
+    assert ctypes.sizeof(native.BC) == 24 + 8 * 48
+
+
+@pytest.mark.parametrize("change,code", [
+    (dict(flux="roe"), None),
+    (dict(bc_id=(-7, -9, -5, -5, -9, -9)), -3),
+    (dict(bc_id=(-11, -9, -5, -5, -9, -9)), -3),
+    (dict(bc_id=(2, -9, -5, -5, -9, -9)), -3),
+    (dict(time_accuracy="RK2", bc_id=(-9, -9, -5, -5, -9, -9), imx=3), -3),
+])
+def test_config_errors(change, code):
+    case, _, _ = C.laminar_box(8, 6, 5)
+    if "flux" in change:
+        with pytest.raises(KeyError):
+            native.c_structs(dataclasses.replace(case, **change))
+        return
+    with pytest.raises(native.FestError) as e:
+        native.GpuBlock(dataclasses.replace(case, **change))
+    assert e.value.code == code
+
+
+def test_precision_modes():
+    case, _, _ = C.laminar_box(8, 6, 5)
+    with pytest.raises(native.FestError) as e:
+        native.GpuBlock(case, precision=7)
+    assert e.value.code == -3

@@ -1,0 +1,201 @@
+"""Parity of the HIP path (called through the C ABI, libfest_hip.so) with the
+oracle on the same seeded inputs.
+
+Tolerances: the plain FP64 path must match the oracle to 1e-12 relative
+(max |gpu - oracle| / max |oracle| per array); ghost cells, geometry, index
+mapping and the first residual evaluation are expected to agree bit for bit
+and are checked with a tolerance of 0 where the arithmetic is identical.
+"""
+import dataclasses
+import sys
+
+import numpy as np
+import pytest
+
+from binding import Oracle
+from festgpu import cases as C
+from festgpu.native import FestError, GpuBlock
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def relerr(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    fa, fb = np.isfinite(a), np.isfinite(b)
+    assert (fa == fb).all(), "non-finite entries differ"
+    s = np.abs(b[fb]).max() if fb.any() else 0.0
+    d = np.abs(a[fa] - b[fb]).max() if fa.any() else 0.0
+    return d / s if s > 0 else d
+
+
+def pair(case, nodes, q, wall=False, warp=None):
+    if warp is not None:
+        nodes = C.cartesian_nodes(case, warp=warp)
+    o = Oracle(case)
+    o.set_nodes(nodes)
+    if wall:
+        o.wall_dist()
+    o.set_state(q)
+    g = GpuBlock(case)
+    g.set_grid(nodes, wall_dist=wall)
+    g.upload_state(q)
+    return g, o
+
+
+def test_geometry_and_wall_distance_bit_exact():
+    case, nodes, q = C.sst_bump(24, 16, 8)
+    say("geometry sst_bump 24x16x8")
+    g, o = pair(case, nodes, q, wall=True)
+    cells = g.download("cells")
+    for a, n in enumerate(("vol", "cx", "cy", "cz")):
+        assert np.array_equal(cells[a], o.arr(n)), n
+    faces = g.download("faces")
+    for d, p in enumerate("ijk"):
+        for a, n in enumerate(("A", "nx", "ny", "nz")):
+            assert np.array_equal(faces[d][a], o.arr(p + n)), p + n
+    assert np.array_equal(g.download("dist"), o.arr("dist"))
+
+
+FLUXES = ["ausm", "ausmP", "ausmUP", "slau", "ldfss0", "van_leer"]
+RECONS = ["none", "muscl", "ppm", "weno"]
+
+
+@pytest.mark.parametrize("flux", FLUXES)
+@pytest.mark.parametrize("recon", RECONS)
+def test_residual_sst_all_schemes(flux, recon):
+    case, nodes, q = C.sst_bump(20, 12, 8)
+    case = dataclasses.replace(case, flux=flux, recon=recon)
+    say("residual sst", flux, recon)
+    g, o = pair(case, nodes, q, wall=True)
+    o.temp_refresh()
+    assert o.residual(3) == 0
+    g.residual(3)
+    out = g.download("qp", "F", "G", "H", "residue", "mu", "mut", "f1", "grads")
+    assert relerr(out["qp"], o.arr("qp")) == 0.0
+    for n, on in (("F", "F"), ("G", "G"), ("H", "H"), ("residue", "res"), ("mu", "mu")):
+        assert relerr(out[n], o.arr(on)) <= TOL, n
+    ng = 6
+    for d, n in enumerate(("gx", "gy", "gz")):
+        assert relerr(out["grads"][d], o.arr(n)) <= TOL, n
+    inner = (slice(2, -2),) * 3
+    assert relerr(out["mut"][inner], o.arr("mut")[inner]) <= TOL
+    assert relerr(out["f1"][inner], o.arr("f1")[inner]) <= TOL
+
+
+@pytest.mark.parametrize("bcs,tw", [
+    ((-9, -9, -5, -5, -9, -9), (0, 0, 300, 300, 0, 0)),
+    ((-3, -4, -6, -6, -8, -8), (0,) * 6),
+    ((-1, -2, -5, -5, -6, -6), (0, 0, -1, 0.5, 0, 0)),
+    ((-8, -8, -5, -8, -9, -9), (0,) * 6),
+])
+def test_residual_laminar_boundaries(bcs, tw):
+    case, _, q = C.laminar_box(12, 10, 8)
+    case = dataclasses.replace(case, bc_id=bcs, fix_twall=tw, fix_rho=(case.rho_inf * 1.01,) * 6,
+                               fix_u=(case.u_inf,) * 6, fix_v=(3.0,) * 6, fix_w=(0.0,) * 6,
+                               fix_p=(case.p_inf * 0.99,) * 6, mu_sutherland=1, accur=1)
+    say("residual laminar", bcs)
+    g, o = pair(case, None, q, warp=0.05)
+    o.temp_refresh()
+    for it in (1, 3):
+        assert o.residual(it) == 0
+        g.residual(it)
+        out = g.download("qp", "F", "G", "H", "residue")
+        assert relerr(out["qp"], o.arr("qp")) == 0.0
+        for n, on in (("F", "F"), ("G", "G"), ("H", "H"), ("residue", "res")):
+            assert relerr(out[n], o.arr(on)) <= TOL, (n, it)
+
+
+@pytest.mark.parametrize("time_local", [0, 1])
+def test_rk4_steps_laminar(time_local):
+    case, nodes, q = C.laminar_box(16, 12, 6)
+    case = dataclasses.replace(case, time_local=time_local)
+    say("rk4 laminar", time_local)
+    g, o = pair(case, nodes, q, warp=0.02)
+    for it in range(1, 6):
+        assert o.step(it) == 0
+        g.get_next_solution(it)
+    out = g.download("qp", "dt", "residue", "temp")
+    assert relerr(out["qp"], o.arr("qp")) <= TOL
+    assert relerr(out["dt"], o.arr("dt")) <= TOL
+    assert relerr(out["residue"], o.arr("res")) <= 1e-10
+    np.testing.assert_allclose(g.resnorm(), o.resnorm(), rtol=1e-10)
+
+
+def test_rk4_steps_sst_bump():
+    case, nodes, q = C.sst_bump(24, 16, 8)
+    say("rk4 sst bump 24x16x8")
+    g, o = pair(case, nodes, q, wall=True)
+    for it in range(1, 4):
+        assert o.step(it) == 0
+        g.get_next_solution(it)
+    out = g.download("qp", "dt")
+    assert relerr(out["qp"], o.arr("qp")) <= TOL
+    assert relerr(out["dt"], o.arr("dt")) <= TOL
+
+
+def test_laminar_box_64x64x4_100_steps():
+    """The reference's CPU-runnable case: 100 RK4 steps (400 stages)."""
+    case, nodes, q = C.laminar_box(64, 64, 4)
+    say("laminar box 64x64x4, 100 steps")
+    g, o = pair(case, nodes, q)
+    for it in range(1, 101):
+        assert o.step(it) == 0
+        g.get_next_solution(it)
+    out = g.download("qp", "dt")
+    assert relerr(out["qp"], o.arr("qp")) <= TOL
+    assert relerr(out["dt"], o.arr("dt")) <= TOL
+    np.testing.assert_allclose(g.resnorm(), o.resnorm(), rtol=1e-9)
+
+
+def test_full_size_sst_bump_residual():
+    """One residual evaluation on the 256x128x128 SST bump grid (4.2M cells)
+    against the oracle; the wall distance comes from the device (its parity
+    is pinned on smaller grids)."""
+    case, nodes, q = C.sst_bump()
+    say("full-size sst bump residual")
+    g = GpuBlock(case)
+    g.set_grid(nodes, wall_dist=True)
+    g.upload_state(q)
+    g.residual(1)
+    out = g.download("F", "residue", "dist")
+    o = Oracle(case)
+    o.set_nodes(nodes)
+    o.arr("dist")[...] = out["dist"]
+    o.set_state(q)
+    o.temp_refresh()
+    assert o.residual(1) == 0
+    assert relerr(out["F"], o.arr("F")) <= TOL
+    assert relerr(out["residue"], o.arr("res")) <= TOL
+
+
+def test_full_size_k_invariance():
+    """A state that does not vary along the periodic k direction must stay
+    k-invariant, plane for plane bit for bit, over RK4 steps at full size."""
+    case, nodes, q = C.sst_bump()
+    say("full-size k invariance")
+    q[:, :, :, :] = q[:, 64:65, :, :]
+    g = GpuBlock(case)
+    g.set_grid(nodes, wall_dist=True)
+    g.upload_state(q)
+    for it in range(1, 4):
+        g.get_next_solution(it)
+    qp = g.download("qp")[:, 3:-3, 3:-3, 3:-3]
+    assert np.isfinite(qp).all()
+    assert (qp == qp[:, :1]).all()
+
+
+def test_nonphysical_state_is_reported():
+    case, nodes, q = C.laminar_box(12, 10, 6)
+    q[0, 6, 6, 6] = -1.0
+    g = GpuBlock(case)
+    g.set_grid(nodes)
+    g.upload_state(q)
+    with pytest.raises(FestError) as e:
+        g.get_next_solution(1)
+    assert e.value.code in (-1, -2)
