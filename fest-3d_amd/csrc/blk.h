@@ -31,6 +31,7 @@ struct Blk {
   double *flux[3];
   double *csnd[3];            // stage-1 face sound speed (time step)
   double *res, *dt, *ust, *rst;
+  double *src;                // SST source S_k, S_w of the interior cells (x volume)
   double *g[3];
   double *nodes;              // grid nodes (3 planes) when set on device
   double *red;                // reduction partials

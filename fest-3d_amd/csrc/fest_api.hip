@@ -96,12 +96,22 @@ int check_err(fest_handle *h) {
 int launch_residual_front(fest_handle *h, int current_iter, int store_csnd) {
   Blk &b = h->b;
   hipStream_t st = h->stream;
-  for (int f = 0; f < 6; ++f) {
-    int id = b.bc[f];
-    if (id >= 0 || id == -10) continue;
-    int na = face_na(b, f), nb = face_nb(b, f);
-    dim3 t(64, 4, 1), g(cdiv(na + 5, 64), cdiv(nb + 5, 4), 1);
-    hipLaunchKernelGGL(k_bc_face, g, t, 0, st, b, f, current_iter);
+  // faces in the reference's order imin, imax, jmin, ...; the two faces of
+  // one direction share a launch when the direction has at least 8 nodes:
+  // each side then reads only its own interior layers 1..4 and ghost layers,
+  // so neither sees the other's writes
+  for (int d = 0; d < 3; ++d) {
+    int N = d == 0 ? b.imx : d == 1 ? b.jmx : b.kmx;
+    int na = face_na(b, 2 * d), nb = face_nb(b, 2 * d);
+    dim3 t(64, 4, 1);
+    if (N >= 8) {
+      dim3 g(cdiv(na + 5, 64), cdiv(nb + 5, 4), 2);
+      hipLaunchKernelGGL(k_bc_face, g, t, 0, st, b, 2 * d, current_iter);
+    } else {
+      dim3 g(cdiv(na + 5, 64), cdiv(nb + 5, 4), 1);
+      hipLaunchKernelGGL(k_bc_face, g, t, 0, st, b, 2 * d, current_iter);
+      hipLaunchKernelGGL(k_bc_face, g, t, 0, st, b, 2 * d + 1, current_iter);
+    }
   }
   hipLaunchKernelGGL(k_bc_edges, dim3(cdiv(4L * (b.imx + 5) * b.nv, 256)), dim3(256), 0, st, b, 0);
   hipLaunchKernelGGL(k_bc_edges, dim3(cdiv(4L * (b.kmx + 5) * b.nv, 256)), dim3(256), 0, st, b, 1);
@@ -114,8 +124,6 @@ int launch_residual_front(fest_handle *h, int current_iter, int store_csnd) {
     int mx = std::max(std::max(b.imx, b.jmx), b.kmx);
     dim3 gb(cdiv(mx, 64), cdiv(mx, 4), 6);
     hipLaunchKernelGGL(k_grad_bc, gb, dim3(64, 4, 1), 0, st, b);
-    hipLaunchKernelGGL(k_visc, g, t, 0, st, b);
-    if (b.turb) hipLaunchKernelGGL(k_visc_ghost, gb, dim3(64, 4, 1), 0, st, b);
     LAUNCHCHK(h);
   }
   box(b.imx, b.jmx - 1, b.kmx - 1, g, t);
@@ -248,7 +256,7 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
     AL(b.flux[d], b.nfl[d] * b.nv); AL(b.csnd[d], b.nfl[d]);
     AL(b.g[d], b.ngr * b.ng);
   }
-  AL(b.res, b.nres * b.nv); AL(b.dt, b.nres); AL(b.ust, b.nc * b.nv); AL(b.rst, b.nres * b.nv);
+  AL(b.res, b.nres * b.nv); AL(b.dt, b.nres); AL(b.src, 2 * b.nres); AL(b.ust, b.nc * b.nv); AL(b.rst, b.nres * b.nv);
   AL(b.nodes, 3L * (b.imx + 6) * (b.jmx + 6) * (b.kmx + 6));
   // reduction partials: the larger of the time-step block count and 8 per norm block
   long nblk_dt = (long)cdiv(b.imx - 1, 64) * cdiv(b.jmx - 1, 4) * (b.kmx - 1);

@@ -171,7 +171,10 @@ __device__ void flow_tangency_col(const Blk &b, int face, int a, int bb) {
 // tangential plane (-2..na+2) x (-2..nb+2); most conditions act on the face
 // interior 1..na-1 x 1..nb-1, the periodic and far-field plane copies on the
 // whole plane.
-__global__ void k_bc_face(Blk b, int face, int current_iter) {
+// blockIdx.z selects face0 + z: the min and max faces of one direction run
+// in one launch when they cannot touch each other's cells (see the caller).
+__global__ void k_bc_face(Blk b, int face0, int current_iter) {
+  const int face = face0 + (int)blockIdx.z;
   int a = (int)(blockIdx.x * blockDim.x + threadIdx.x) - 2;
   int bb = (int)(blockIdx.y * blockDim.y + threadIdx.y) - 2;
   int na = face_na(b, face), nb = face_nb(b, face);

@@ -44,29 +44,36 @@ __device__ __forceinline__ void bmuscl_cell(const double *u, long s, int sw, dou
   lo = u[0] - 0.25 * phi * (((1. + kappa) * psi1 * bd) + ((1. - kappa) * psi2 * fd));
 }
 
-// WENO5 upper-face state of the cell (weno.f90:59-74)
-__device__ __forceinline__ double weno_up(double Um2, double Um1, double U0, double Up1, double Up2) {
-  const double g1 = 1.0 / 10.0, g2 = 6.0 / 10.0, g3 = 3.0 / 10.0, eps = 1e-6;
-  double P1 = (2.0 * Um2 - 7.0 * Um1 + 11.0 * U0) / 6.0;
-  double P2 = (-1.0 * Um1 + 5.0 * U0 + 2.0 * Up1) / 6.0;
-  double P3 = (2.0 * U0 + 5.0 * Up1 - 1.0 * Up2) / 6.0;
-  double B1 = (13.0 / 12.0) * sqd(Um2 - 2.0 * Um1 + U0) + (1.0 / 4.0) * sqd(Um2 - 4.0 * Um1 + 3.0 * U0);
-  double B2 = (13.0 / 12.0) * sqd(Um1 - 2.0 * U0 + Up1) + (1.0 / 4.0) * sqd(Um1 - Up1);
-  double B3 = (13.0 / 12.0) * sqd(U0 - 2.0 * Up1 + Up2) + (1.0 / 4.0) * sqd(3.0 * U0 - 4.0 * Up1 + Up2);
-  double w1 = g1 / sqd(eps + B1), w2 = g2 / sqd(eps + B2), w3 = g3 / sqd(eps + B3);
-  return (w1 * P1 + w2 * P2 + w3 * P3) / (w1 + w2 + w3);
+// WENO5 upper- and lower-face states of a cell (weno.f90:59-87), Jiang-Shu
+// smoothness indicators and linear weights (0.1, 0.6, 0.3), eps = 1e-6.
+// The normalised weights w_i/sum(w) with w_i = g_i/D_i, D_i = (eps+B_i)^2 are
+// formed as g_i E_i / sum(g_j E_j) with E_i the product of the other two
+// D's, and the 1/6 of the candidate polynomials is folded into the same
+// single division; this is the reference's quotient up to rounding.
+__device__ __forceinline__ double weno_combine(double D1, double D2, double D3, double Q1, double Q2, double Q3) {
+  const double g1 = 1.0 / 10.0, g2 = 6.0 / 10.0, g3 = 3.0 / 10.0;
+  double e1 = g1 * (D2 * D3), e2 = g2 * (D1 * D3), e3 = g3 * (D1 * D2);
+  return (e1 * Q1 + e2 * Q2 + e3 * Q3) / (6.0 * (e1 + e2 + e3));
 }
-// WENO5 lower-face state of the cell (weno.f90:76-87)
-__device__ __forceinline__ double weno_lo(double Um2, double Um1, double U0, double Up1, double Up2) {
-  const double g1 = 1.0 / 10.0, g2 = 6.0 / 10.0, g3 = 3.0 / 10.0, eps = 1e-6;
+__device__ __forceinline__ void weno_betas(double Um2, double Um1, double U0, double Up1, double Up2, double &D1,
+                                           double &D2, double &D3) {
+  const double eps = 1e-6;
   double B1 = (13.0 / 12.0) * sqd(Um2 - 2.0 * Um1 + U0) + (1.0 / 4.0) * sqd(Um2 - 4.0 * Um1 + 3.0 * U0);
   double B2 = (13.0 / 12.0) * sqd(Um1 - 2.0 * U0 + Up1) + (1.0 / 4.0) * sqd(Um1 - Up1);
   double B3 = (13.0 / 12.0) * sqd(U0 - 2.0 * Up1 + Up2) + (1.0 / 4.0) * sqd(3.0 * U0 - 4.0 * Up1 + Up2);
-  double P1 = (2.0 * Up2 - 7.0 * Up1 + 11.0 * U0) / 6.0;
-  double P2 = (-1.0 * Up1 + 5.0 * U0 + 2.0 * Um1) / 6.0;
-  double P3 = (2.0 * U0 + 5.0 * Um1 - 1.0 * Um2) / 6.0;
-  double w1 = g1 / sqd(eps + B3), w2 = g2 / sqd(eps + B2), w3 = g3 / sqd(eps + B1);
-  return (w1 * P1 + w2 * P2 + w3 * P3) / (w1 + w2 + w3);
+  D1 = sqd(eps + B1); D2 = sqd(eps + B2); D3 = sqd(eps + B3);
+}
+__device__ __forceinline__ double weno_up(double Um2, double Um1, double U0, double Up1, double Up2) {
+  double D1, D2, D3;
+  weno_betas(Um2, Um1, U0, Up1, Up2, D1, D2, D3);
+  return weno_combine(D1, D2, D3, 2.0 * Um2 - 7.0 * Um1 + 11.0 * U0, -1.0 * Um1 + 5.0 * U0 + 2.0 * Up1,
+                      2.0 * U0 + 5.0 * Up1 - 1.0 * Up2);
+}
+__device__ __forceinline__ double weno_lo(double Um2, double Um1, double U0, double Up1, double Up2) {
+  double D1, D2, D3;
+  weno_betas(Um2, Um1, U0, Up1, Up2, D1, D2, D3);
+  return weno_combine(D3, D2, D1, 2.0 * Up2 - 7.0 * Up1 + 11.0 * U0, -1.0 * Up1 + 5.0 * U0 + 2.0 * Um1,
+                      2.0 * U0 + 5.0 * Um1 - 1.0 * Um2);
 }
 
 // PPM face estimate at the face between cells m-1 and m; u points at cell m
@@ -302,7 +309,8 @@ __global__ void __launch_bounds__(256) k_flux(Blk b, int store_csnd) {
     const double *qp = b.qp;
     const long nc = b.nc;
     double delx = b.cx[c] - b.cx[cm], dely = b.cy[c] - b.cy[cm], delz = b.cz[c] - b.cz[cm];
-    double d_LR = sqrt(delx * delx + dely * dely + delz * delz);
+    double d2 = delx * delx + dely * dely + delz * delz;
+    double id2 = 1.0 / d2;  // normal_comp*del/d_LR = (del_phi - grad.del)*del/d_LR^2
     const long gL = gri(b, i - ii, j - jj, k - kk), gR = gri(b, i, j, k);
     const double *gx = b.g[0], *gy = b.g[1], *gz = b.g[2];
     const long ngr = b.ngr;
@@ -317,10 +325,10 @@ __global__ void __launch_bounds__(256) k_flux(Blk b, int store_csnd) {
     double del[4] = {qp[nc + c] - qp[nc + cm], qp[2 * nc + c] - qp[2 * nc + cm], qp[3 * nc + c] - qp[3 * nc + cm],
                      T_RE - T_LE};
     for (int l = 0; l < 4; ++l) {
-      double ncmp = (del[l] - (gr[l][0] * delx + gr[l][1] * dely + gr[l][2] * delz)) / d_LR;
-      gr[l][0] = gr[l][0] + (ncmp * delx / d_LR);
-      gr[l][1] = gr[l][1] + (ncmp * dely / d_LR);
-      gr[l][2] = gr[l][2] + (ncmp * delz / d_LR);
+      double ncmp = (del[l] - (gr[l][0] * delx + gr[l][1] * dely + gr[l][2] * delz)) * id2;
+      gr[l][0] = gr[l][0] + ncmp * delx;
+      gr[l][1] = gr[l][1] + ncmp * dely;
+      gr[l][2] = gr[l][2] + ncmp * delz;
     }
     double mu_f = 0.5 * (b.mu[cm] + b.mu[c]);
     double mut_f = b.turb ? 0.5 * (b.mut[cm] + b.mut[c]) : 0.0;
@@ -328,9 +336,10 @@ __global__ void __launch_bounds__(256) k_flux(Blk b, int store_csnd) {
     double dudx = gr[0][0], dudy = gr[0][1], dudz = gr[0][2];
     double dvdx = gr[1][0], dvdy = gr[1][1], dvdz = gr[1][2];
     double dwdx = gr[2][0], dwdy = gr[2][1], dwdz = gr[2][2];
-    double Txx = 2. * tm * (dudx - ((dudx + dvdy + dwdz) / 3.));
-    double Tyy = 2. * tm * (dvdy - ((dudx + dvdy + dwdz) / 3.));
-    double Tzz = 2. * tm * (dwdz - ((dudx + dvdy + dwdz) / 3.));
+    double div3 = (dudx + dvdy + dwdz) / 3.;
+    double Txx = 2. * tm * (dudx - div3);
+    double Tyy = 2. * tm * (dvdy - div3);
+    double Tzz = 2. * tm * (dwdz - div3);
     double Txy = tm * (dvdx + dudy);
     double Txz = tm * (dwdx + dudz);
     double Tyz = tm * (dwdy + dvdz);
@@ -354,10 +363,10 @@ __global__ void __launch_bounds__(256) k_flux(Blk b, int store_csnd) {
       }
       double dt2[2] = {qp[5 * nc + c] - qp[5 * nc + cm], qp[6 * nc + c] - qp[6 * nc + cm]};
       for (int l = 0; l < 2; ++l) {
-        double ncmp = (dt2[l] - (gt[l][0] * delx + gt[l][1] * dely + gt[l][2] * delz)) / d_LR;
-        gt[l][0] = gt[l][0] + (ncmp * delx / d_LR);
-        gt[l][1] = gt[l][1] + (ncmp * dely / d_LR);
-        gt[l][2] = gt[l][2] + (ncmp * delz / d_LR);
+        double ncmp = (dt2[l] - (gt[l][0] * delx + gt[l][1] * dely + gt[l][2] * delz)) * id2;
+        gt[l][0] = gt[l][0] + ncmp * delx;
+        gt[l][1] = gt[l][1] + ncmp * dely;
+        gt[l][2] = gt[l][2] + ncmp * delz;
       }
       double F1 = 0.5 * (b.f1[cm] + b.f1[c]);
       double sigma_kf = 0.85 * F1 + 1.0 * (1.0 - F1);

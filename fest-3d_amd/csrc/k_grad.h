@@ -16,6 +16,55 @@ __device__ __forceinline__ const double *grad_var(const Blk &b, int l) {
   return l < 3 ? b.qp + (long)(l + 1) * b.nc : l == 3 ? b.temp : b.qp + (long)(l + 1) * b.nc;
 }
 
+// SST eddy viscosity and F1 of one cell from its gradients (src/viscosity.f90:343-388)
+__device__ __forceinline__ void sst_mut_f1(double rho, double tk, double tw, double dist, double mu,
+                                           const double (&g)[6][3], double &mut, double &f1) {
+  const double bstar = 0.09, a1 = 0.31, sigma_w2 = 0.856;
+  double var1 = sqrt(tk) / (bstar * tw * dist);
+  double var2 = 500 * (mu / rho) / ((dist * dist) * tw);
+  double arg2 = fmx(2 * var1, var2);
+  double F = tanh(arg2 * arg2);
+  double wx = (g[2][1] - g[1][2]);
+  double wy = (g[0][2] - g[2][0]);
+  double wz = (g[1][0] - g[0][1]);
+  double vort = sqrt(wx * wx + wy * wy + wz * wz);
+  double NUM = rho * a1 * tk;
+  double DENOM = fmx(fmx((a1 * tw), vort * F), 1.e-20);
+  mut = NUM / DENOM;
+  double CD = fmx(2 * rho * sigma_w2 * (g[4][0] * g[5][0] + g[4][1] * g[5][1] + g[4][2] * g[5][2]) / tw, 1e-20);
+  double right = 4 * (rho * sigma_w2 * tk) / (CD * (dist * dist));
+  double left = fmx(var1, var2);
+  double arg1 = fmn(left, right);
+  f1 = tanh(arg1 * arg1 * arg1 * arg1);
+}
+
+// SST source of an interior cell, times its volume (src/source.f90:158-270)
+__device__ __forceinline__ void sst_source(double rho, double tk, double tw, double mut, double F1, double vol,
+                                           const double (&g)[6][3], double &S_k, double &S_w) {
+  const double sigma_w1 = 0.5, sigma_w2 = 0.856, beta1 = 0.075, beta2 = 0.0828, bstar = 0.09, kap = 0.41;
+  const double gama1 = (beta1 / bstar) - ((sigma_w1 * (kap * kap)) / sqrt(bstar));
+  const double gama2 = (beta2 / bstar) - ((sigma_w2 * (kap * kap)) / sqrt(bstar));
+  double vort = sqrt(sqd(g[2][1] - g[1][2]) + sqd(g[0][2] - g[2][0]) + sqd(g[1][0] - g[0][1]));
+  double CD = 2 * rho * sigma_w2 * (g[4][0] * g[5][0] + g[4][1] * g[5][1] + g[4][2] * g[5][2]) / tw;
+  CD = fmx(CD, 1e-20);
+  double gama = gama1 * F1 + gama2 * (1. - F1);
+  double beta = beta1 * F1 + beta2 * (1. - F1);
+  double D_k = bstar * rho * tw * tk;
+  double D_w = beta * rho * (tw * tw);
+  double divergence = g[0][0] + g[1][1] + g[2][2];
+  double P_k = mut * (vort * vort) - ((2.0 / 3.0) * rho * tk * divergence);
+  P_k = fmn(P_k, 20 * D_k);
+  double P_w = (rho * gama / mut) * P_k;
+  double lamda = (1. - F1) * CD;
+  S_k = (P_k - D_k) * vol;
+  S_w = (P_w - D_w + lamda) * vol;
+}
+
+// One thread per cell 0..imx x 0..jmx x 0..kmx: all Green-Gauss gradients,
+// then (same cell, gradients in registers) the molecular viscosity, the SST
+// mu_t and F1 (src/viscosity.f90:52-388) and, for interior cells, the SST
+// source.  Interior gradients are final here: the ghost-gradient rule only
+// writes ghost cells.
 __global__ void __launch_bounds__(256) k_grad(Blk b) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -34,11 +83,13 @@ __global__ void __launch_bounds__(256) k_grad(Blk b) {
   n[0][3] = b.fx[0][f1]; n[1][3] = b.fy[0][f1]; n[2][3] = b.fz[0][f1];
   n[0][4] = b.fx[1][g1]; n[1][4] = b.fy[1][g1]; n[2][4] = b.fz[1][g1];
   n[0][5] = b.fx[2][h1]; n[1][5] = b.fy[2][h1]; n[2][5] = b.fz[2][h1];
-  double vol2 = 2 * b.vol[c];
+  double vol = b.vol[c];
+  double vol2 = 2 * vol;
   const long si = 1, sj = b.pi, sk = (long)b.pi * b.pj;
   long gidx = gri(b, i, j, k);
   bool bad = false;
   int ndir = (b.kmx > 2) ? 3 : 2;
+  double gg[6][3];
   for (int l = 0; l < b.ng; ++l) {
     const double *v = grad_var(b, l);
     double vc = v[c];
@@ -58,11 +109,45 @@ __global__ void __launch_bounds__(256) k_grad(Blk b) {
         g = 0.0;
       }
       bad |= isnan(g);
+      gg[l][d] = g;
       b.g[d][(long)l * b.ngr + gidx] = g;
     }
   }
   if (bad) atomicOr(b.err, ERR_NAN_GRAD);
+  double rho = b.qp[c];
+  double mu;
+  if (b.mu_suth) {
+    double T = b.qp[4 * b.nc + c] / (rho * b.R);
+    mu = b.mu_ref * (pow(T / b.T_ref, 1.5)) * ((b.T_ref + b.S_ref) / (T + b.S_ref));
+    b.mu[c] = mu;
+    if (isnan(mu)) atomicOr(b.err, ERR_NAN_MU);
+  } else {
+    mu = b.mu[c];
+  }
+  if (!b.turb) return;
+  double tk = b.qp[5 * b.nc + c], tw = b.qp[6 * b.nc + c];
+  double mut, F1;
+  sst_mut_f1(rho, tk, tw, b.dist[c], mu, gg, mut, F1);
+  b.mut[c] = mut;
+  b.f1[c] = F1;
+  if (i >= 1 && i <= b.imx - 1 && j >= 1 && j <= b.jmx - 1 && k >= 1 && k <= b.kmx - 1) {
+    double S_k, S_w;
+    sst_source(rho, tk, tw, mut, F1, vol, gg, S_k, S_w);
+    long r = ri(b, i, j, k);
+    b.src[r] = S_k;
+    b.src[b.nres + r] = S_w;
+  }
 }
+
+// mu_t / F1 ghost rules (src/viscosity.f90:407-465), face interiors
+__device__ __forceinline__ void visc_ghost(const Blk &b, int face, int a, int bb) {
+  int id = b.bc[face];
+  if (!(id == -5 || (id <= -1 && id >= -9))) return;
+  long cg = fcell(b, face, 0, a, bb), cn = fcell(b, face, 1, a, bb);
+  b.mut[cg] = (id == -5) ? -b.mut[cn] : b.mut[cn];
+  b.f1[cg] = b.f1[cn];
+}
+
 
 // ghost-cell gradients on the face interiors of every face with a
 // boundary id < 0.  The reference passes Jfaces/Kfaces to a dummy declared
@@ -76,6 +161,7 @@ __global__ void k_grad_bc(Blk b) {
   if (a > na - 1 || bb > nb - 1) return;
   int id = b.bc[face];
   if (id >= 0) return;
+  if (b.turb) visc_ghost(b, face, a, bb);
   int d = face >> 1, hi = face & 1;
   double Tw = b.fix[FX_TWALL][face];
   // face position and the interior (I) / ghost (G) cells
@@ -108,63 +194,6 @@ __global__ void k_grad_bc(Blk b) {
     gy[o + gG] = oy + (iy - dot * ny);
     gz[o + gG] = oz + (iz - dot * nz);
   }
-}
-
-// mu (Sutherland) and SST mu_t / F1 on cells 0..imx (src/viscosity.f90:52-388)
-__global__ void __launch_bounds__(256) k_visc(Blk b) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int j = blockIdx.y * blockDim.y + threadIdx.y;
-  int k = blockIdx.z;
-  if (i > b.imx || j > b.jmx || k > b.kmx) return;
-  long c = ci(b, i, j, k);
-  double rho = b.qp[c];
-  double mu;
-  if (b.mu_suth) {
-    double T = b.qp[4 * b.nc + c] / (rho * b.R);
-    mu = b.mu_ref * (pow(T / b.T_ref, 1.5)) * ((b.T_ref + b.S_ref) / (T + b.S_ref));
-    b.mu[c] = mu;
-    if (isnan(mu)) atomicOr(b.err, ERR_NAN_MU);
-  } else {
-    mu = b.mu[c];
-  }
-  if (!b.turb) return;
-  const double bstar = 0.09, a1 = 0.31, sigma_w2 = 0.856;
-  double tk = b.qp[5 * b.nc + c], tw = b.qp[6 * b.nc + c], dist = b.dist[c];
-  long g = gri(b, i, j, k);
-  const double *gx = b.g[0], *gy = b.g[1], *gz = b.g[2];
-  long ngr = b.ngr;
-  double var1 = sqrt(tk) / (bstar * tw * dist);
-  double var2 = 500 * (mu / rho) / ((dist * dist) * tw);
-  double arg2 = fmx(2 * var1, var2);
-  double F = tanh(arg2 * arg2);
-  double wx = (gy[2 * ngr + g] - gz[1 * ngr + g]);
-  double wy = (gz[0 * ngr + g] - gx[2 * ngr + g]);
-  double wz = (gx[1 * ngr + g] - gy[0 * ngr + g]);
-  double vort = sqrt(wx * wx + wy * wy + wz * wz);
-  double NUM = rho * a1 * tk;
-  double DENOM = fmx(fmx((a1 * tw), vort * F), 1.e-20);
-  b.mut[c] = NUM / DENOM;
-  double CD = fmx(2 * rho * sigma_w2 * (gx[4 * ngr + g] * gx[5 * ngr + g] + gy[4 * ngr + g] * gy[5 * ngr + g] +
-                                        gz[4 * ngr + g] * gz[5 * ngr + g]) / tw,
-                  1e-20);
-  double right = 4 * (rho * sigma_w2 * tk) / (CD * (dist * dist));
-  double left = fmx(var1, var2);
-  double arg1 = fmn(left, right);
-  b.f1[c] = tanh(arg1 * arg1 * arg1 * arg1);
-}
-
-// mu_t / F1 ghost rules (src/viscosity.f90:407-465)
-__global__ void k_visc_ghost(Blk b) {
-  int face = blockIdx.z;
-  int a = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  int bb = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  int na = face_na(b, face), nb = face_nb(b, face);
-  if (a > na - 1 || bb > nb - 1) return;
-  int id = b.bc[face];
-  if (!(id == -5 || (id <= -1 && id >= -9))) return;
-  long cg = fcell(b, face, 0, a, bb), cn = fcell(b, face, 1, a, bb);
-  b.mut[cg] = (id == -5) ? -b.mut[cn] : b.mut[cn];
-  b.f1[cg] = b.f1[cn];
 }
 
 }  // namespace fest

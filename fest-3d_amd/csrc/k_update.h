@@ -131,34 +131,9 @@ __global__ void __launch_bounds__(256) k_update(Blk b, double TF, double SF, int
   }
   double rho = b.qp[c];
   if (b.turb) {
-    // src/source.f90:158-270
-    const double sigma_w1 = 0.5, sigma_w2 = 0.856, beta1 = 0.075, beta2 = 0.0828, bstar = 0.09, kap = 0.41;
-    const double gama1 = (beta1 / bstar) - ((sigma_w1 * (kap * kap)) / sqrt(bstar));
-    const double gama2 = (beta2 / bstar) - ((sigma_w2 * (kap * kap)) / sqrt(bstar));
-    long g = gri(b, i, j, k);
-    const long ngr = b.ngr;
-    const double *gx = b.g[0], *gy = b.g[1], *gz = b.g[2];
-    double tk = b.qp[5 * nc + c], tw = b.qp[6 * nc + c];
-    double vort = sqrt(sqd(gy[2 * ngr + g] - gz[1 * ngr + g]) + sqd(gz[0 * ngr + g] - gx[2 * ngr + g]) +
-                       sqd(gx[1 * ngr + g] - gy[0 * ngr + g]));
-    double CD = 2 * rho * sigma_w2 * (gx[4 * ngr + g] * gx[5 * ngr + g] + gy[4 * ngr + g] * gy[5 * ngr + g] +
-                                      gz[4 * ngr + g] * gz[5 * ngr + g]) / tw;
-    CD = fmx(CD, 1e-20);
-    double F1 = b.f1[c];
-    double gama = gama1 * F1 + gama2 * (1. - F1);
-    double beta = beta1 * F1 + beta2 * (1. - F1);
-    double D_k = bstar * rho * tw * tk;
-    double D_w = beta * rho * (tw * tw);
-    double divergence = gx[g] + gy[ngr + g] + gz[2 * ngr + g];
-    double mut = b.mut[c];
-    double P_k = mut * (vort * vort) - ((2.0 / 3.0) * rho * tk * divergence);
-    P_k = fmn(P_k, 20 * D_k);
-    double P_w = (rho * gama / mut) * P_k;
-    double lamda = (1. - F1) * CD;
-    double S_k = (P_k - D_k) * b.vol[c];
-    double S_w = (P_w - D_w + lamda) * b.vol[c];
-    R[5] = R[5] - S_k;
-    R[6] = R[6] - S_w;
+    // SST source (src/source.f90:158-270), computed with the gradients
+    R[5] = R[5] - b.src[r];
+    R[6] = R[6] - b.src[b.nres + r];
   }
   if (flags & UPD_WRITE_RES)
     for (int l = 0; l < nv; ++l) b.res[(long)l * b.nres + r] = R[l];

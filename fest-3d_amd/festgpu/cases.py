@@ -117,7 +117,7 @@ def freestream(case: Case, T_inf=300.0, p_inf=101325.0, mach=0.5, reynolds=1000.
     d = np.asarray(direction, dtype=float)
     d = d / np.linalg.norm(d)
     case.rho_inf, case.p_inf = rho, p_inf
-    case.u_inf, case.v_inf, case.w_inf = (V * d[0], V * d[1], V * d[2])
+    case.u_inf, case.v_inf, case.w_inf = (float(V * d[0]), float(V * d[1]), float(V * d[2]))
     case.vel_mag = math.sqrt(case.u_inf ** 2 + case.v_inf ** 2 + case.w_inf ** 2)
     case.MInf = case.vel_mag / math.sqrt(gm * p_inf / rho)
     case.mu_ref = rho * case.vel_mag * 1.0 / reynolds
@@ -127,6 +127,28 @@ def freestream(case: Case, T_inf=300.0, p_inf=101325.0, mach=0.5, reynolds=1000.
         case.tk_inf = 1.5 * ((case.vel_mag * tu) ** 2)
         case.tw_inf = rho * case.tk_inf / (case.mu_ref * mu_ratio)
     return case
+
+
+def case_to_json(case: Case) -> str:
+    """Lossless JSON of a case (floats round-trip through repr)."""
+    import json
+
+    def conv(v):
+        if isinstance(v, (tuple, list)):
+            return [conv(x) for x in v]
+        if isinstance(v, (np.floating, float)):
+            return float(v)
+        if isinstance(v, (np.integer, int)):
+            return int(v)
+        return v
+
+    return json.dumps({k: conv(v) for k, v in dataclasses.asdict(case).items()})
+
+
+def case_from_json(text: str) -> Case:
+    import json
+    d = json.loads(text)
+    return Case(**{k: tuple(v) if isinstance(v, list) else v for k, v in d.items()})
 
 
 def freestream_state(case: Case) -> np.ndarray:

@@ -1,0 +1,60 @@
+"""Writes tests/golden/*.npz: inputs and outputs of the reference's own
+Fortran modules (the amdflang build in oracle/_ref, see oracle/Makefile) on
+small seeded cases, so that the oracle stays pinned where the reference
+sources are absent.  Each fixture holds the full input (ghost-filled nodes,
+state, stale temperature, wall distance) and the reference's face fluxes,
+residue and ghost-filled state after one residual evaluation.
+
+Run from the repository root:  python tests/golden/make_golden.py
+"""
+import dataclasses
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "fest-3d_amd"))
+
+from binding import Oracle, run_reference  # noqa: E402
+from festgpu import cases as C  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def cases():
+    c, _, q = C.laminar_box(10, 8, 6)
+    yield "laminar_muscl_ausmP", c, C.cartesian_nodes(c, warp=0.03), q, 1, False
+    c, n, q = C.sst_bump(12, 10, 6)
+    yield "sst_weno_slau", c, n, q, 3, True
+    c2 = dataclasses.replace(c, flux="ausmUP", recon="ppm")
+    yield "sst_ppm_ausmUP", c2, n, q, 1, True
+    c, _, q = C.laminar_box(10, 8, 6)
+    c3 = dataclasses.replace(c, flux="ldfss0", recon="muscl", bc_id=(-3, -4, -6, -5, -8, -8), mu_sutherland=1,
+                             accur=1, fix_rho=(c.rho_inf,) * 6, fix_u=(c.u_inf,) * 6, fix_p=(c.p_inf,) * 6,
+                             fix_twall=(0, 0, 0, 320.0, 0, 0))
+    yield "laminar_bcmix_ldfss0", c3, C.cartesian_nodes(c3, warp=0.05), q, 2, False
+
+
+def main():
+    for name, case, nodes, q, it, wall in cases():
+        o = Oracle(case)
+        o.set_nodes(nodes)
+        if wall:
+            o.wall_dist()
+        o.set_state(q)
+        o.temp_refresh()
+        inputs = dict(nodes=o.arr("nodes").copy(), qp=o.arr("qp").copy(), temp=o.arr("temp").copy(),
+                      dist=o.arr("dist").copy())
+        ref = run_reference(case, inputs["nodes"], inputs["qp"], inputs["temp"], 5,
+                            ["qp", "F", "G", "H", "res"], it=it, dist=inputs["dist"])
+        meta = np.array(C.case_to_json(case))
+        np.savez_compressed(os.path.join(OUT, name + ".npz"), meta=meta, iter=np.array(it),
+                            **{"in_" + k: v for k, v in inputs.items()},
+                            **{"out_" + k: v for k, v in ref.items()})
+        print("wrote", name)
+
+
+if __name__ == "__main__":
+    main()

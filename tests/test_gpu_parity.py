@@ -2,9 +2,11 @@
 oracle on the same seeded inputs.
 
 Tolerances: the plain FP64 path must match the oracle to 1e-12 relative
-(max |gpu - oracle| / max |oracle| per array); ghost cells, geometry, index
-mapping and the first residual evaluation are expected to agree bit for bit
-and are checked with a tolerance of 0 where the arithmetic is identical.
+(max |gpu - oracle| / max |oracle| per array).  Geometry and wall distance
+use only correctly rounded operations and must agree bit for bit; pow, tanh
+and exp (Sutherland viscosity, the SST wall omega, the far-field Riemann
+state, the blending functions) come from different libraries on the host and
+the device and may differ in the last ulp, which the 1e-12 bound absorbs.
 """
 import dataclasses
 import sys
@@ -77,7 +79,7 @@ def test_residual_sst_all_schemes(flux, recon):
     assert o.residual(3) == 0
     g.residual(3)
     out = g.download("qp", "F", "G", "H", "residue", "mu", "mut", "f1", "grads")
-    assert relerr(out["qp"], o.arr("qp")) == 0.0
+    assert relerr(out["qp"], o.arr("qp")) <= TOL
     for n, on in (("F", "F"), ("G", "G"), ("H", "H"), ("residue", "res"), ("mu", "mu")):
         assert relerr(out[n], o.arr(on)) <= TOL, n
     ng = 6
@@ -106,7 +108,7 @@ def test_residual_laminar_boundaries(bcs, tw):
         assert o.residual(it) == 0
         g.residual(it)
         out = g.download("qp", "F", "G", "H", "residue")
-        assert relerr(out["qp"], o.arr("qp")) == 0.0
+        assert relerr(out["qp"], o.arr("qp")) <= TOL
         for n, on in (("F", "F"), ("G", "G"), ("H", "H"), ("residue", "res")):
             assert relerr(out[n], o.arr(on)) <= TOL, (n, it)
 
@@ -174,9 +176,30 @@ def test_full_size_sst_bump_residual():
     assert relerr(out["residue"], o.arr("res")) <= TOL
 
 
+def test_full_size_sst_bump_rk4_step():
+    """One full RK4 step (four stages, stage-1 local time step) of the 4.2M-cell
+    workload against the oracle."""
+    case, nodes, q = C.sst_bump()
+    say("full-size sst bump, one RK4 step")
+    g = GpuBlock(case)
+    g.set_grid(nodes, wall_dist=True)
+    g.upload_state(q)
+    g.get_next_solution(1)
+    out = g.download("qp", "dt", "dist")
+    o = Oracle(case)
+    o.set_nodes(nodes)
+    o.arr("dist")[...] = out["dist"]
+    o.set_state(q)
+    assert o.step(1) == 0
+    assert relerr(out["qp"], o.arr("qp")) <= TOL
+    assert relerr(out["dt"], o.arr("dt")) <= TOL
+
+
 def test_full_size_k_invariance():
-    """A state that does not vary along the periodic k direction must stay
-    k-invariant, plane for plane bit for bit, over RK4 steps at full size."""
+    """A state that does not vary along the periodic k direction stays
+    k-invariant over RK4 steps at full size, up to rounding (the z face
+    states of different planes come from different formulas: ghost value,
+    boundary MUSCL, WENO, so planes agree to the last ulps only)."""
     case, nodes, q = C.sst_bump()
     say("full-size k invariance")
     q[:, :, :, :] = q[:, 64:65, :, :]
@@ -187,7 +210,9 @@ def test_full_size_k_invariance():
         g.get_next_solution(it)
     qp = g.download("qp")[:, 3:-3, 3:-3, 3:-3]
     assert np.isfinite(qp).all()
-    assert (qp == qp[:, :1]).all()
+    for l in range(case.n_var):
+        spread = np.abs(qp[l] - qp[l, :1]).max() / np.abs(qp[l]).max()
+        assert spread <= 1e-12, (l, spread)
 
 
 def test_nonphysical_state_is_reported():
