@@ -7,10 +7,15 @@ RK stages, each a full residual evaluation plus the stage update, with the
 stage-1 time step.  Metric: cell-RK-stages per second, with the inputs
 resident in HBM before the timed region.
 
-The roofline object prices one RK stage at the algorithmic 406 bytes per cell
-(stored FP64 metrics, n_var=7): q read and write, the RK base, the R
-accumulator, three owned faces x {A,nx,ny,nz}, volume, time step, cell
-centres, stale temperature and wall distance, averaged over the four stages.
+The roofline object is for the dominant kernel: during the warmup steps
+every kernel group is bracketed with HIP events on the engine's stream and
+the group with the largest total is chosen; in the timed steps only that
+group is bracketed, and its average launch duration prices the algorithmic
+bytes of one launch (festgpu.roofline).  `traffic` is the measured HBM bytes
+per launch of that kernel (FETCH_SIZE + WRITE_SIZE, scaled by a copy of
+known size) from the rocprofv3 counter summary committed for this workload,
+or null when there is none.  `stage_roofline` prices a whole RK stage at the
+fused minimum of 406 (SST) / 326 (laminar) bytes per cell.
 """
 import argparse
 import json
@@ -21,8 +26,24 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "fest-3d_amd"))
 
-BYTES_PER_CELL_STAGE = {"sst": 406.0, "none": 326.0}
-HBM_PEAK_GBPS = 8000.0
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def measured_traffic(case, group):
+    """HBM bytes per launch of `group` from the committed counter summary
+    profiles/hbm_traffic_<workload>_<size>.md (its table rows read
+    `| group | kernel | fetch B | write B | traffic B |`), else None."""
+    path = os.path.join(PROFILES, f"hbm_traffic_{case.name}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0] == group:
+            try:
+                return float(cells[4])
+            except ValueError:
+                return None
+    return None
 
 
 def cpu_baseline(case_fn, full_dist, budget_s=25.0):
@@ -62,10 +83,11 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding is a design, see DESIGN.md)", file=sys.stderr)
+              "(one process drives one GPU; the block is not sharded across GPUs)", file=sys.stderr)
         sys.exit(2)
 
     from festgpu import cases as C
+    from festgpu import roofline as R
     from festgpu.native import GpuBlock
 
     ni, nj, nk = (int(v) for v in args.size.split("x"))
@@ -75,21 +97,31 @@ def main():
     g.set_grid(nodes, wall_dist=(case.turb == "sst"))
     g.upload_state(q)
     it = 1
+    g.timing("*")
     for _ in range(args.warmup):
         g.get_next_solution(it)
         it += 1
     g.sync()
+    breakdown = {grp: g.kernel_time(grp)[0] for grp in g.KERNEL_GROUPS}
+    modelled = [grp for grp in breakdown if R.bytes_per_launch(case, grp) is not None]
+    dominant = max(modelled, key=lambda grp: breakdown[grp]) if args.warmup > 0 else "flux_i"
+    g.timing(dominant)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         g.get_next_solution(it)
         it += 1
     g.sync()
     el = time.perf_counter() - t0
+    k_ms, k_launches = g.kernel_time(dominant)
+    g.timing(None)
     stages = 4 * args.steps
     value = case.ncells * stages / el
     ms_step = 1e3 * el / args.steps
-    bpcs = BYTES_PER_CELL_STAGE[case.turb]
-    achieved = bpcs * case.ncells / (el / stages) / 1e9
+    k_avg_s = 1e-3 * k_ms / max(k_launches, 1)
+    k_bytes = R.bytes_per_launch(case, dominant)
+    achieved = k_bytes / k_avg_s / 1e9
+    bpcs = R.BYTES_PER_CELL_STAGE[case.turb]
+    stage_gbps = bpcs * case.ncells / (el / stages) / 1e9
     out = {
         "metric": "cell-RK-stages/s", "value": value, "unit": "cell-RK-stages/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
@@ -97,9 +129,14 @@ def main():
         "config": {"workload": case.name, "cells": case.ncells, "scheme": f"{case.recon}+{case.flux}",
                    "turbulence": case.turb, "time": "RK4 local" if case.time_local else "RK4 global",
                    "precision": "fp64"},
-        "roofline": {"bound": "hbm", "kernel": "rk_stage", "achieved": achieved, "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
-                     "bytes_per_cell_stage": bpcs},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": R.HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": achieved / R.HBM_PEAK_GBPS,
+                     "traffic": measured_traffic(case, dominant),
+                     "bytes_per_launch": k_bytes, "launches": k_launches, "avg_launch_us": 1e6 * k_avg_s},
+        "stage_roofline": {"bytes_per_cell_stage": bpcs, "achieved": stage_gbps, "peak": R.HBM_PEAK_GBPS,
+                           "unit": "GB/s", "frac": stage_gbps / R.HBM_PEAK_GBPS},
+        "kernel_ms_per_step": ({grp: breakdown[grp] / args.warmup for grp in breakdown}
+                               if args.warmup > 0 else None),
     }
     if not args.no_cpu_baseline:
         dist = g.download("dist") if case.turb == "sst" else None

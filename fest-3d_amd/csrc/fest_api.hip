@@ -23,6 +23,18 @@
 
 using namespace fest;
 
+// kernel groups timed by fest_timing (one or a few launches each)
+enum { KG_STEP_BEGIN = 0, KG_BC_FACE, KG_BC_EDGES, KG_GRAD, KG_GRAD_BC, KG_FLUX_I, KG_FLUX_J, KG_FLUX_K,
+       KG_TIME_STEP, KG_UPDATE, KG_COUNT };
+static const char *const kg_names[KG_COUNT] = {"step_begin", "bc_face", "bc_edges", "grad", "grad_bc",
+                                               "flux_i", "flux_j", "flux_k", "time_step", "update"};
+
+struct KgTimer {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  double ms = 0.0;
+  long launches = 0;
+};
+
 struct fest_handle {
   Blk b;
   hipStream_t stream = nullptr;
@@ -35,6 +47,11 @@ struct fest_handle {
   long red_cap = 0;        // doubles in b.red
   int norm_blocks = 0;
   bool have_grid = false, have_state = false;
+  // opt-in event timing: groups whose bit is set record a start and a stop
+  // event on the stream around their launches
+  unsigned timed = 0;
+  KgTimer timers[KG_COUNT];
+  std::vector<hipEvent_t> ev_free;
 };
 
 namespace {
@@ -71,6 +88,60 @@ int dalloc(fest_handle *h, double **p, long n) {
   return 0;
 }
 
+hipEvent_t take_event(fest_handle *h) {
+  if (!h->ev_free.empty()) {
+    hipEvent_t e = h->ev_free.back();
+    h->ev_free.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// brackets the launches of one kernel group with events when it is timed
+struct KgScope {
+  fest_handle *h;
+  int g;
+  hipEvent_t e0 = nullptr;
+  KgScope(fest_handle *h_, int g_) : h(h_), g(g_) {
+    if (h->timed & (1u << g)) {
+      e0 = take_event(h);
+      if (e0 && hipEventRecord(e0, h->stream) != hipSuccess) {
+        h->ev_free.push_back(e0);
+        e0 = nullptr;
+      }
+    }
+  }
+  ~KgScope() {
+    if (!e0) return;
+    hipEvent_t e1 = take_event(h);
+    if (!e1 || hipEventRecord(e1, h->stream) != hipSuccess) {
+      h->ev_free.push_back(e0);
+      if (e1) h->ev_free.push_back(e1);
+      return;
+    }
+    h->timers[g].pending.emplace_back(e0, e1);
+  }
+};
+
+// waits for the stream and folds the recorded event pairs into the totals
+int settle_timers(fest_handle *h) {
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (auto &t : h->timers) {
+    for (auto &pr : t.pending) {
+      float ms = 0.f;
+      HIPCHK(h, hipEventElapsedTime(&ms, pr.first, pr.second));
+      t.ms += ms;
+      t.launches += 1;
+      h->ev_free.push_back(pr.first);
+      h->ev_free.push_back(pr.second);
+    }
+    t.pending.clear();
+  }
+  return 0;
+}
+
 inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
 // grid over an (ni x nj x nk) index box with 64x4 thread blocks along i, j
@@ -100,6 +171,8 @@ int launch_residual_front(fest_handle *h, int current_iter, int store_csnd) {
   // one direction share a launch when the direction has at least 8 nodes:
   // each side then reads only its own interior layers 1..4 and ghost layers,
   // so neither sees the other's writes
+  {
+  KgScope ts(h, KG_BC_FACE);
   for (int d = 0; d < 3; ++d) {
     int N = d == 0 ? b.imx : d == 1 ? b.jmx : b.kmx;
     int na = face_na(b, 2 * d), nb = face_nb(b, 2 * d);
@@ -113,25 +186,45 @@ int launch_residual_front(fest_handle *h, int current_iter, int store_csnd) {
       hipLaunchKernelGGL(k_bc_face, g, t, 0, st, b, 2 * d + 1, current_iter);
     }
   }
-  hipLaunchKernelGGL(k_bc_edges, dim3(cdiv(4L * (b.imx + 5) * b.nv, 256)), dim3(256), 0, st, b, 0);
-  hipLaunchKernelGGL(k_bc_edges, dim3(cdiv(4L * (b.kmx + 5) * b.nv, 256)), dim3(256), 0, st, b, 1);
-  hipLaunchKernelGGL(k_bc_edges, dim3(1), dim3(64), 0, st, b, 2);
+  }
+  {
+    KgScope ts(h, KG_BC_EDGES);
+    hipLaunchKernelGGL(k_bc_edges, dim3(cdiv(4L * (b.imx + 5) * b.nv, 256)), dim3(256), 0, st, b, 0);
+    hipLaunchKernelGGL(k_bc_edges, dim3(cdiv(4L * (b.kmx + 5) * b.nv, 256)), dim3(256), 0, st, b, 1);
+    hipLaunchKernelGGL(k_bc_edges, dim3(1), dim3(64), 0, st, b, 2);
+  }
   LAUNCHCHK(h);
   dim3 g, t;
   if (b.mu_ref != 0.0) {
     box(b.imx + 1, b.jmx + 1, b.kmx + 1, g, t);
-    hipLaunchKernelGGL(k_grad, g, t, 0, st, b);
+    {
+      KgScope ts(h, KG_GRAD);
+      if (b.ng == 6) hipLaunchKernelGGL(k_grad<6>, g, t, 0, st, b);
+      else hipLaunchKernelGGL(k_grad<4>, g, t, 0, st, b);
+    }
     int mx = std::max(std::max(b.imx, b.jmx), b.kmx);
     dim3 gb(cdiv(mx, 64), cdiv(mx, 4), 6);
-    hipLaunchKernelGGL(k_grad_bc, gb, dim3(64, 4, 1), 0, st, b);
+    {
+      KgScope ts(h, KG_GRAD_BC);
+      hipLaunchKernelGGL(k_grad_bc, gb, dim3(64, 4, 1), 0, st, b);
+    }
     LAUNCHCHK(h);
   }
   box(b.imx, b.jmx - 1, b.kmx - 1, g, t);
-  hipLaunchKernelGGL(k_flux<0>, g, t, 0, st, b, store_csnd);
+  {
+    KgScope ts(h, KG_FLUX_I);
+    hipLaunchKernelGGL(k_flux<0>, g, t, 0, st, b, store_csnd);
+  }
   box(b.imx - 1, b.jmx, b.kmx - 1, g, t);
-  hipLaunchKernelGGL(k_flux<1>, g, t, 0, st, b, store_csnd);
+  {
+    KgScope ts(h, KG_FLUX_J);
+    hipLaunchKernelGGL(k_flux<1>, g, t, 0, st, b, store_csnd);
+  }
   box(b.imx - 1, b.jmx - 1, b.kmx, g, t);
-  hipLaunchKernelGGL(k_flux<2>, g, t, 0, st, b, store_csnd);
+  {
+    KgScope ts(h, KG_FLUX_K);
+    hipLaunchKernelGGL(k_flux<2>, g, t, 0, st, b, store_csnd);
+  }
   LAUNCHCHK(h);
   return 0;
 }
@@ -139,6 +232,7 @@ int launch_residual_front(fest_handle *h, int current_iter, int store_csnd) {
 int launch_time_step(fest_handle *h) {
   Blk &b = h->b;
   dim3 g, t;
+  KgScope ts(h, KG_TIME_STEP);
   if (!b.time_local && b.global_dt > 0) {
     hipLaunchKernelGGL(k_dt_fill, dim3(cdiv(b.nres, 256)), dim3(256), 0, h->stream, b, 1);
     LAUNCHCHK(h);
@@ -159,6 +253,7 @@ int launch_update(fest_handle *h, double TF, double SF, int flags) {
   Blk &b = h->b;
   dim3 g, t;
   box(b.imx - 1, b.jmx - 1, b.kmx - 1, g, t);
+  KgScope ts(h, KG_UPDATE);
   hipLaunchKernelGGL(k_update, g, t, 0, h->stream, b, TF, SF, flags);
   LAUNCHCHK(h);
   return 0;
@@ -282,9 +377,15 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
 
 void fest_destroy(fest_handle *h) {
   if (!h) return;
-  if (h->stream) hipStreamSynchronize(h->stream);
-  for (void *p : h->allocs) hipFree(p);
-  if (h->stream) hipStreamDestroy(h->stream);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (auto &t : h->timers)
+    for (auto &pr : t.pending) {
+      (void)hipEventDestroy(pr.first);
+      (void)hipEventDestroy(pr.second);
+    }
+  for (hipEvent_t e : h->ev_free) (void)hipEventDestroy(e);
+  for (void *p : h->allocs) (void)hipFree(p);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
 
@@ -358,16 +459,20 @@ int fest_set_grid(fest_handle *h, const double *nodes, int with_wall_dist) {
     }
     int nw = (int)(wall.size() / 3);
     double *dwall = nullptr, *nd = nullptr;
-    HIPCHK(h, hipMalloc(&dwall, sizeof(double) * std::max<size_t>(3, wall.size())));
-    HIPCHK(h, hipMalloc(&nd, sizeof(double) * nn));
-    if (nw) HIPCHK(h, hipMemcpyAsync(dwall, wall.data(), sizeof(double) * wall.size(), hipMemcpyHostToDevice, h->stream));
-    hipLaunchKernelGGL(k_node_dist, dim3(cdiv(nn, 256)), dim3(256), 0, h->stream, b, (const double *)dwall, nw, nd);
-    dim3 g(cdiv(b.imx + 5, 64), cdiv(b.jmx + 5, 4), (unsigned)(b.kmx + 5));
-    hipLaunchKernelGGL(k_cell_dist, g, dim3(64, 4, 1), 0, h->stream, b, (const double *)nd);
-    hipError_t le = hipGetLastError();
-    hipStreamSynchronize(h->stream);
-    hipFree(dwall);
-    hipFree(nd);
+    hipError_t le = hipMalloc(&dwall, sizeof(double) * std::max<size_t>(3, wall.size()));
+    if (le == hipSuccess) le = hipMalloc(&nd, sizeof(double) * nn);
+    if (le == hipSuccess && nw)
+      le = hipMemcpyAsync(dwall, wall.data(), sizeof(double) * wall.size(), hipMemcpyHostToDevice, h->stream);
+    if (le == hipSuccess) {
+      hipLaunchKernelGGL(k_node_dist, dim3(cdiv(nn, 256)), dim3(256), 0, h->stream, b, (const double *)dwall, nw, nd);
+      dim3 g(cdiv(b.imx + 5, 64), cdiv(b.jmx + 5, 4), (unsigned)(b.kmx + 5));
+      hipLaunchKernelGGL(k_cell_dist, g, dim3(64, 4, 1), 0, h->stream, b, (const double *)nd);
+      le = hipGetLastError();
+    }
+    hipError_t se = hipStreamSynchronize(h->stream);
+    if (le == hipSuccess) le = se;
+    if (dwall) (void)hipFree(dwall);
+    if (nd) (void)hipFree(nd);
     if (le != hipSuccess) return fail(h, FEST_E_HIP, "wall distance: %s", hipGetErrorString(le));
   }
   int rc = check_err(h);
@@ -414,7 +519,10 @@ int fest_get_next_solution(fest_handle *h, int current_iter, double cfl) {
   b.CFLv = cfl;
   int rc;
   const int ta = b.time_acc;
-  hipLaunchKernelGGL(k_step_begin, dim3(cdiv(b.nc, 256)), dim3(256), 0, h->stream, b, ta != FEST_TA_NONE);
+  {
+    KgScope ts(h, KG_STEP_BEGIN);
+    hipLaunchKernelGGL(k_step_begin, dim3(cdiv(b.nc, 256)), dim3(256), 0, h->stream, b, ta != FEST_TA_NONE);
+  }
   LAUNCHCHK(h);
   if (ta == FEST_TA_NONE) {
     if ((rc = launch_residual_front(h, current_iter, 1))) return rc;
@@ -516,6 +624,51 @@ int fest_device_ptr(fest_handle *h, const char *name, void **ptr, size_t *count)
       return 0;
     }
   return fail(h, FEST_E_ARG, "unknown array '%s'", name);
+}
+
+int fest_timing(fest_handle *h, const char *groups) {
+  if (!h) return FEST_E_ARG;
+  int rc = settle_timers(h);
+  if (rc) return rc;
+  for (auto &t : h->timers) {
+    t.ms = 0.0;
+    t.launches = 0;
+  }
+  h->timed = 0;
+  if (!groups || !*groups) return 0;
+  if (!strcmp(groups, "*")) {
+    h->timed = (1u << KG_COUNT) - 1;
+    return 0;
+  }
+  std::string all(groups);
+  size_t pos = 0;
+  while (pos <= all.size()) {
+    size_t e = all.find(',', pos);
+    if (e == std::string::npos) e = all.size();
+    std::string name = all.substr(pos, e - pos);
+    int g = 0;
+    while (g < KG_COUNT && name != kg_names[g]) ++g;
+    if (g == KG_COUNT) {
+      h->timed = 0;
+      return fail(h, FEST_E_ARG, "unknown kernel group '%s'", name.c_str());
+    }
+    h->timed |= 1u << g;
+    pos = e + 1;
+  }
+  return 0;
+}
+
+int fest_kernel_time(fest_handle *h, const char *group, double *total_ms, long *launches) {
+  if (!h || !group || !total_ms) return FEST_E_ARG;
+  int rc = settle_timers(h);
+  if (rc) return rc;
+  for (int g = 0; g < KG_COUNT; ++g)
+    if (!strcmp(group, kg_names[g])) {
+      *total_ms = h->timers[g].ms;
+      if (launches) *launches = h->timers[g].launches;
+      return 0;
+    }
+  return fail(h, FEST_E_ARG, "unknown kernel group '%s'", group);
 }
 
 int fest_sync(fest_handle *h) {

@@ -26,6 +26,34 @@ __host__ __device__ inline int face_nb(const Blk &b, int face) { return face < 4
 
 enum { CP_ANTI = 0, CP_FLAT = 1, CP_SYMM = 2 };
 
+// boundary id and fixed values of one face, gathered with compile-time
+// indices: indexing the by-value kernel argument with a run-time face would
+// make the compiler copy the whole argument block to private memory
+struct FaceBC {
+  int id;
+  double fix[8];
+};
+__device__ __forceinline__ FaceBC face_params(const Blk &b, int face) {
+  FaceBC p;
+  p.id = b.bc[0];
+#pragma unroll
+  for (int l = 0; l < 8; ++l) p.fix[l] = b.fix[l][0];
+#pragma unroll
+  for (int f = 1; f < 6; ++f)
+    if (face == f) {
+      p.id = b.bc[f];
+#pragma unroll
+      for (int l = 0; l < 8; ++l) p.fix[l] = b.fix[l][f];
+    }
+  return p;
+}
+
+// face-metric planes of direction d (0 i, 1 j, 2 k) without indexing the
+// pointer arrays at run time
+__device__ __forceinline__ const double *dsel(const double *const (&arr)[3], int d) {
+  return d == 0 ? arr[0] : (d == 1 ? arr[1] : arr[2]);
+}
+
 // src/boundary/copy_bc.f90:58-131 on one column
 __device__ void copy3col(const Blk &b, double *v, int type, int face, int a, int bb) {
   double a1, a2, a3;
@@ -48,10 +76,10 @@ __device__ void turb_copycol(const Blk &b, int type, int face, int a, int bb) {
     copy3col(b, b.qp + 6 * b.nc, type, face, a, bb);
   }
 }
-__device__ void turb_fixcol(const Blk &b, int face, int a, int bb) {
+__device__ void turb_fixcol(const Blk &b, const FaceBC &p, int face, int a, int bb) {
   if (b.turb) {
-    fixcol(b, b.qp + 5 * b.nc, b.fix[FX_TK][face], face, a, bb);
-    fixcol(b, b.qp + 6 * b.nc, b.fix[FX_TW][face], face, a, bb);
+    fixcol(b, b.qp + 5 * b.nc, p.fix[FX_TK], face, a, bb);
+    fixcol(b, b.qp + 6 * b.nc, p.fix[FX_TW], face, a, bb);
   }
 }
 
@@ -67,12 +95,12 @@ __device__ __forceinline__ void bnormal(const Blk &b, int face, int a, int bb, d
     case 4: f = fi(b, 2, a, bb, 1); break;
     default: f = fi(b, 2, a, bb, b.kmx); break;
   }
-  nx = b.fx[d][f]; ny = b.fy[d][f]; nz = b.fz[d][f];
+  nx = dsel(b.fx, d)[f]; ny = dsel(b.fy, d)[f]; nz = dsel(b.fz, d)[f];
 }
 
 // src/boundary/bc_primitive.f90:1778-1945
-__device__ void temp_based_density_col(const Blk &b, int face, int a, int bb) {
-  double Tw = b.fix[FX_TWALL][face], R = b.R, gm = b.gm;
+__device__ void temp_based_density_col(const Blk &b, const FaceBC &p_, int face, int a, int bb) {
+  double Tw = p_.fix[FX_TWALL], R = b.R, gm = b.gm;
   double *rho = b.qp, *p = b.qp + 4 * b.nc;
   if (Tw < 0.0) {
     long c1 = fcell(b, face, 1, a, bb);
@@ -179,15 +207,16 @@ __global__ void k_bc_face(Blk b, int face0, int current_iter) {
   int bb = (int)(blockIdx.y * blockDim.y + threadIdx.y) - 2;
   int na = face_na(b, face), nb = face_nb(b, face);
   if (a > na + 2 || bb > nb + 2) return;
-  int id = b.bc[face];
+  const FaceBC p = face_params(b, face);
+  const int id = p.id;
   bool inner = (a >= 1 && a <= na - 1 && bb >= 1 && bb <= nb - 1);
   double *q = b.qp;
   const long nc = b.nc;
   switch (id) {
     case -1:  // supersonic inlet
       if (inner && current_iter <= 2) {
-        for (int l = 0; l < 5; ++l) fixcol(b, q + l * nc, b.fix[l][face], face, a, bb);
-        turb_fixcol(b, face, a, bb);
+        for (int l = 0; l < 5; ++l) fixcol(b, q + l * nc, p.fix[l], face, a, bb);
+        turb_fixcol(b, p, face, a, bb);
       }
       break;
     case -2:  // supersonic outlet
@@ -199,8 +228,8 @@ __global__ void k_bc_face(Blk b, int face0, int current_iter) {
     case -3:  // subsonic inlet
       if (inner) {
         if (current_iter <= 2) {
-          for (int l = 0; l < 4; ++l) fixcol(b, q + l * nc, b.fix[l][face], face, a, bb);
-          turb_fixcol(b, face, a, bb);
+          for (int l = 0; l < 4; ++l) fixcol(b, q + l * nc, p.fix[l], face, a, bb);
+          turb_fixcol(b, p, face, a, bb);
         }
         copy3col(b, q + 4 * nc, CP_FLAT, face, a, bb);
       }
@@ -208,14 +237,14 @@ __global__ void k_bc_face(Blk b, int face0, int current_iter) {
     case -4:  // subsonic outlet
       if (inner) {
         for (int l = 0; l < 4; ++l) copy3col(b, q + l * nc, CP_FLAT, face, a, bb);
-        if (current_iter <= 2) fixcol(b, q + 4 * nc, b.fix[FX_P][face], face, a, bb);
+        if (current_iter <= 2) fixcol(b, q + 4 * nc, p.fix[FX_P], face, a, bb);
         turb_copycol(b, CP_FLAT, face, a, bb);
       }
       break;
     case -5:  // no-slip wall
       if (inner) {
         copy3col(b, q + 4 * nc, CP_SYMM, face, a, bb);
-        temp_based_density_col(b, face, a, bb);
+        temp_based_density_col(b, p, face, a, bb);
         copy3col(b, q + 1 * nc, CP_ANTI, face, a, bb);
         copy3col(b, q + 2 * nc, CP_ANTI, face, a, bb);
         copy3col(b, q + 3 * nc, CP_ANTI, face, a, bb);
@@ -247,7 +276,7 @@ __global__ void k_bc_face(Blk b, int face0, int current_iter) {
           // the last face cell in its loop order
           double last = farfield_col(b, face, na - 1, nb - 1, 0);
           if (last > 0.) turb_copycol(b, CP_FLAT, face, a, bb);
-          else turb_fixcol(b, face, a, bb);
+          else turb_fixcol(b, p, face, a, bb);
         }
       }
       for (int l = 0; l < b.nv; ++l) {

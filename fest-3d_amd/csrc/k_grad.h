@@ -64,7 +64,9 @@ __device__ __forceinline__ void sst_source(double rho, double tk, double tw, dou
 // then (same cell, gradients in registers) the molecular viscosity, the SST
 // mu_t and F1 (src/viscosity.f90:52-388) and, for interior cells, the SST
 // source.  Interior gradients are final here: the ghost-gradient rule only
-// writes ghost cells.
+// writes ghost cells.  NG is the number of gradient slots (4 laminar, 6
+// SST); a compile-time count keeps all 3*NG gradients in registers.
+template <int NG>
 __global__ void __launch_bounds__(256) k_grad(Blk b) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -89,12 +91,14 @@ __global__ void __launch_bounds__(256) k_grad(Blk b) {
   long gidx = gri(b, i, j, k);
   bool bad = false;
   int ndir = (b.kmx > 2) ? 3 : 2;
-  double gg[6][3];
-  for (int l = 0; l < b.ng; ++l) {
+  double gg[NG][3];
+#pragma unroll
+  for (int l = 0; l < NG; ++l) {
     const double *v = grad_var(b, l);
     double vc = v[c];
     double sm_i = v[c - si] + vc, sm_j = v[c - sj] + vc, sm_k = v[c - sk] + vc;
     double sp_i = v[c + si] + vc, sp_j = v[c + sj] + vc, sp_k = v[c + sk] + vc;
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
       double g;
       if (d < ndir) {
@@ -124,7 +128,7 @@ __global__ void __launch_bounds__(256) k_grad(Blk b) {
   } else {
     mu = b.mu[c];
   }
-  if (!b.turb) return;
+  if constexpr (NG == 6) {
   double tk = b.qp[5 * b.nc + c], tw = b.qp[6 * b.nc + c];
   double mut, F1;
   sst_mut_f1(rho, tk, tw, b.dist[c], mu, gg, mut, F1);
@@ -136,6 +140,7 @@ __global__ void __launch_bounds__(256) k_grad(Blk b) {
     long r = ri(b, i, j, k);
     b.src[r] = S_k;
     b.src[b.nres + r] = S_w;
+  }
   }
 }
 

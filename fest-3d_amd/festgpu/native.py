@@ -82,9 +82,11 @@ def lib():
         L.fest_resnorm.argtypes = [vp, dp]
         L.fest_device_ptr.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_size_t)]
         L.fest_sync.argtypes = [vp]
+        L.fest_timing.argtypes = [vp, ctypes.c_char_p]
+        L.fest_kernel_time.argtypes = [vp, ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_long)]
         for f in ("fest_create", "fest_upload_geometry", "fest_set_grid", "fest_upload_state",
                   "fest_get_next_solution", "fest_residual", "fest_time_step", "fest_download", "fest_resnorm",
-                  "fest_device_ptr", "fest_sync"):
+                  "fest_device_ptr", "fest_sync", "fest_timing", "fest_kernel_time"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -184,6 +186,23 @@ class GpuBlock:
 
     def sync(self):
         self._check(lib().fest_sync(self.h))
+
+    KERNEL_GROUPS = ("step_begin", "bc_face", "bc_edges", "grad", "grad_bc", "flux_i", "flux_j", "flux_k",
+                     "time_step", "update")
+
+    def timing(self, groups=None):
+        """Resets the event totals and brackets the launches of the named
+        kernel groups with HIP events from now on ("*" all, None none)."""
+        if groups is not None and not isinstance(groups, str):
+            groups = ",".join(groups)
+        self._check(lib().fest_timing(self.h, groups.encode() if groups else None))
+
+    def kernel_time(self, group):
+        """(summed event milliseconds, bracketed launches) of one group."""
+        ms = ctypes.c_double()
+        n = ctypes.c_long()
+        self._check(lib().fest_kernel_time(self.h, group.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
 
     # -- downloads in the reference layout
     def shape_of(self, name):

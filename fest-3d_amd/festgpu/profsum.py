@@ -1,0 +1,132 @@
+"""Summaries of rocprofv3 CSV output for the engine's kernels.
+
+  python -m festgpu.profsum stats DIR
+      kernel table from the *_kernel_stats.csv under DIR (--kernel-trace
+      --stats --output-format csv): calls, average and total duration.
+  python -m festgpu.profsum traffic --fetch DIR --write DIR
+                 --probe-fetch DIR --probe-write DIR --case NAME
+      HBM bytes per launch of each kernel from two counter runs of the
+      engine (--pmc FETCH_SIZE, --pmc WRITE_SIZE; the counters are KiB per
+      dispatch) and the same two on copy_probe, whose known traffic
+      (512 MiB read, 512 MiB written per launch) gives the scale factors
+      applied to the engine's counters.  The table rows are
+      `| group | kernel | fetch B | write B | traffic B |`.
+
+Run from fest-3d_amd/ (or with it on sys.path).
+"""
+import argparse
+import csv
+import glob
+import os
+import sys
+from collections import defaultdict
+
+from . import cases as C
+from . import roofline as R
+
+PROBE_BYTES = 512 * 2 ** 20  # copy_probe: read and written per launch
+
+GROUPS = [  # (substring of the demangled kernel name, kernel group)
+    ("k_flux<0>", "flux_i"), ("k_flux<1>", "flux_j"), ("k_flux<2>", "flux_k"),
+    ("k_grad<", "grad"), ("k_grad_bc", "grad_bc"), ("k_update", "update"),
+    ("k_bc_face", "bc_face"), ("k_bc_edges", "bc_edges"), ("k_step_begin", "step_begin"),
+    ("k_dt_min", "time_step"), ("k_dt_fill", "time_step"), ("k_dt", "time_step"),
+    ("k_norm", "resnorm"), ("k_node_dist", "setup"), ("k_cell_dist", "setup"), ("k_cells", "setup"),
+    ("k_face_metrics", "setup"), ("k_ghost_nodes", "setup"), ("k_fill", "setup"),
+    ("k_copy_probe", "copy_probe"),
+]
+
+
+def group_of(name):
+    for key, grp in GROUPS:
+        if key in name:
+            return grp
+    return "-"
+
+
+def short(name):
+    """fest::k_flux<0>(fest::Blk, int) -> k_flux<0>"""
+    n = name.split("(")[0]
+    for pre in ("void ", "fest::"):
+        n = n.replace(pre, "")
+    return n.strip()
+
+
+def _csvs(d, suffix):
+    out = sorted(glob.glob(os.path.join(d, "**", "*" + suffix), recursive=True))
+    if not out:
+        sys.exit(f"no *{suffix} under {d}")
+    return out
+
+
+def stats(d):
+    rows = []
+    for path in _csvs(d, "kernel_stats.csv"):
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                rows.append((short(r["Name"]), int(r["Calls"]), float(r["TotalDurationNs"]),
+                             float(r["AverageNs"]), float(r["Percentage"])))
+    rows.sort(key=lambda r: -r[2])
+    lines = ["| kernel | group | calls | avg us | total ms | % |", "|---|---|---|---|---|---|"]
+    for n, c, tot, avg, pct in rows:
+        lines.append(f"| {n} | {group_of(n)} | {c} | {avg / 1e3:.1f} | {tot / 1e6:.3f} | {pct:.1f} |")
+    return "\n".join(lines)
+
+
+def counters(d, counter):
+    """Average counter value per dispatch, by kernel."""
+    acc = defaultdict(list)
+    for path in _csvs(d, "counter_collection.csv"):
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if r.get("Counter_Name") != counter:
+                    continue
+                acc[short(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def traffic(fetch_dir, write_dir, probe_fetch, probe_write, case_name):
+    fe, wr = counters(fetch_dir, "FETCH_SIZE"), counters(write_dir, "WRITE_SIZE")
+    pf, pw = counters(probe_fetch, "FETCH_SIZE"), counters(probe_write, "WRITE_SIZE")
+    kf = PROBE_BYTES / (1024.0 * pf["k_copy_probe"])
+    kw = PROBE_BYTES / (1024.0 * pw["k_copy_probe"])
+    workload, size = case_name.rsplit("_", 1)
+    case = C.WORKLOADS[workload](*(int(v) for v in size.split("x")))[0]
+    lines = [f"copy_probe: FETCH_SIZE {pf['k_copy_probe']:.0f} KiB and WRITE_SIZE {pw['k_copy_probe']:.0f} KiB "
+             f"per launch for {PROBE_BYTES} B read and written: scale factors fetch {kf:.4f}, write {kw:.4f}",
+             "",
+             "| group | kernel | fetch B | write B | traffic B |", "|---|---|---|---|---|"]
+    extra = ["", "| group | algorithmic B | traffic / algorithmic |", "|---|---|---|"]
+    for k in sorted(fe, key=lambda k: -fe[k]):
+        grp = group_of(k)
+        if grp in ("copy_probe", "-"):
+            continue
+        f_b = fe[k] * 1024.0 * kf
+        w_b = wr.get(k, 0.0) * 1024.0 * kw
+        lines.append(f"| {grp} | {k} | {f_b:.4g} | {w_b:.4g} | {f_b + w_b:.6g} |")
+        alg = R.bytes_per_launch(case, grp) if case is not None else None
+        if alg:
+            extra.append(f"| {grp} | {alg:.6g} | {(f_b + w_b) / alg:.3f} |")
+    return "\n".join(lines + extra)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    s = sub.add_parser("stats")
+    s.add_argument("dir")
+    t = sub.add_parser("traffic")
+    t.add_argument("--fetch", required=True)
+    t.add_argument("--write", required=True)
+    t.add_argument("--probe-fetch", required=True)
+    t.add_argument("--probe-write", required=True)
+    t.add_argument("--case", required=True)
+    a = ap.parse_args(argv)
+    if a.cmd == "stats":
+        print(stats(a.dir))
+    else:
+        print(traffic(a.fetch, a.write, a.probe_fetch, a.probe_write, a.case))
+
+
+if __name__ == "__main__":
+    main()

@@ -3,7 +3,7 @@
  *
  * The entry points replace the body of the reference's time-integration
  * module and the residual passes it calls; a Fortran host binds them with
- * ISO_C_BINDING (see INTEGRATION.md) and keeps the reference's call surface:
+ * ISO_C_BINDING and keeps the reference's call surface:
  *
  *   setup_update(control, scheme, flow, dims)          src/update.f90:85-126
  *   get_next_solution(qp, Temp, residue, delta_t, cells, F, G, H,
@@ -150,9 +150,20 @@ int fest_download(fest_handle *h, uint32_t what, double *qp, double *temp, doubl
  * (src/resnorm.f90:171-225): [|mass imbalance|, sqrt(sum R_l^2)/scale_l]. */
 int fest_resnorm(fest_handle *h, double *res_abs);
 
-/* Device pointers of the resident arrays for zero-copy consumers (tests,
- * the autograd harness).  name: "qp","temp","residue","F","G","H","dt". */
+/* Device pointers of the resident arrays for callers that keep the state
+ * on the device.  name: "qp","temp","residue","F","G","H","dt". */
 int fest_device_ptr(fest_handle *h, const char *name, void **ptr, size_t *count);
+
+/* Kernel timing (no counterpart in the reference; it is for benchmarks
+ * and profiling): fest_timing resets the totals and selects the kernel
+ * groups whose launches are bracketed with HIP events on the handle's
+ * stream -- NULL or "" none, "*" all, else a comma-separated list of
+ * "step_begin", "bc_face", "bc_edges", "grad", "grad_bc", "flux_i",
+ * "flux_j", "flux_k", "time_step", "update".  fest_kernel_time waits for
+ * the stream and returns one group's summed event time and the number of
+ * bracketed launches since the last reset. */
+int fest_timing(fest_handle *h, const char *groups);
+int fest_kernel_time(fest_handle *h, const char *group, double *total_ms, long *launches);
 
 /* Synchronise the handle's stream. */
 int fest_sync(fest_handle *h);

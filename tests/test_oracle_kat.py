@@ -19,6 +19,7 @@ import pytest
 
 from binding import Oracle
 from festgpu import cases as C
+from props import boundary_flux_mismatch
 
 PROFILE = np.array([-2.0, -1.0, 0.0, 1.0, 2.0, 3.0, 5.0, 7.0, 7.0, 7.0])  # qp(-2..7,1,1,:)
 
@@ -163,6 +164,26 @@ def test_residue_kat():
     assert r[0] == 1.0 and r[1] == 0.0 and r[2] == 6.0 and 0.09 < r[4] < 1.1
 
 
+def test_residue_telescopes_to_boundary_flux():
+    """The oracle's mean-flow residues on the SST bump sum to the net
+    boundary flux (the property the GPU checks at full size); a residue
+    scattered with a shifted face index breaks it."""
+    case, nodes, q = C.sst_bump(24, 12, 8)
+    o = Oracle(case)
+    o.set_nodes(nodes)
+    o.wall_dist()
+    o.set_state(q)
+    o.temp_refresh()
+    assert o.residual(1) == 0
+    F, G, H, res = (o.arr(n).copy() for n in ("F", "G", "H", "res"))
+    for l in range(5):
+        assert boundary_flux_mismatch(F, G, H, res, l) <= 1e-13, l
+    # planted mistake: the residue built from G shifted by one j face
+    bad = res.copy()
+    bad[:, :, :-1, :] += G[:, :, 1:-1, :] - G[:, :, 2:, :]
+    assert max(boundary_flux_mismatch(F, G, H, bad, l) for l in range(5)) > 1e-6
+
+
 def test_gradient_kat():
     case = tiny_case(2, 2, 2, bc_id=(3,) * 6, mu_ref=1.0)  # tests/test_gradient.f90:35-62
     o = Oracle(case)
@@ -289,6 +310,25 @@ def test_freestream_values():
     assert c.rho_inf * c.vel_mag / c.mu_ref == pytest.approx(1e6, rel=1e-12)
 
 
+def test_ghost_nodes_affine_exact():
+    """Ghost nodes are linear extrapolations (src/grid.f90:137-236): on a grid
+    whose coordinates are an integer-valued affine map of the node indices
+    every operation is exact, so all three ghost layers of every face, edge
+    and corner must equal the map itself bit for bit."""
+    case = tiny_case(7, 6, 5)
+    o = Oracle(case)
+    n = np.zeros(case.node_shape())
+    k, j, i = np.meshgrid(np.arange(-2, case.kmx + 4), np.arange(-2, case.jmx + 4), np.arange(-2, case.imx + 4),
+                          indexing="ij")
+    exact = np.stack([2.0 * i + j - k + 5.0, 1.0 * j + 3.0 * k, 1.0 * i + 2.0 * k + 0.5 * j])
+    n[:, 3:-3, 3:-3, 3:-3] = exact[:, 3:-3, 3:-3, 3:-3]
+    o.set_nodes(n)
+    assert np.array_equal(o.arr("nodes"), exact)
+    # unit-cube cells of the map have the map's determinant as volume
+    det = np.linalg.det(np.array([[2.0, 1.0, -1.0], [0.0, 1.0, 3.0], [1.0, 0.5, 2.0]]))
+    np.testing.assert_allclose(o.arr("vol")[3:-3, 3:-3, 3:-3], det, rtol=1e-14)
+
+
 def test_rk4_composition():
     """The step equals the RK4 combination of four residual evaluations taken
     with the same stale Temp and the stage-1 time step (src/update.f90:176-187)."""
@@ -336,6 +376,92 @@ def test_rk4_composition():
     o2.set_state(q0)
     assert o2.step(1) == 0
     np.testing.assert_allclose(o2.arr("qp")[inner], to_prim(U4), rtol=1e-12)
+
+
+def _to_cons(q, gm):
+    u = q.copy()
+    u[1:] = q[1:] * q[0]
+    u[4] = (u[4] / (gm - 1) + 0.5 * (u[1] ** 2 + u[2] ** 2 + u[3] ** 2)) / u[0]
+    return u
+
+
+def _to_prim(u, gm):
+    q = u.copy()
+    q[1:] = u[1:] / u[0]
+    q[4] = (gm - 1) * q[0] * (q[4] - 0.5 * (q[1] ** 2 + q[2] ** 2 + q[3] ** 2))
+    return q
+
+
+STEP_RULES = {  # src/update.f90:162-187: per stage (TF, SF); None = no R accumulator
+    "RK4": [(0.5, 1.0), (0.5, 2.0), (1.0, 2.0), (1.0 / 6.0, 1.0)],
+    "RK2": [(0.5, 1.0), (0.5, 1.0)],
+    "none": [(1.0, None)],
+}
+
+
+@pytest.mark.parametrize("rule,turb,time_local", [
+    ("RK4", "sst", 1), ("RK4", "sst", 0), ("RK2", "none", 1), ("RK2", "sst", 1), ("none", "none", 0),
+    ("none", "sst", 1)])
+def test_step_composition(rule, turb, time_local):
+    """A step equals its stages composed from separately evaluated residuals
+    (src/update.f90:162-195, 367-485): stale Temp, the stage-1 time step,
+    R_store = sum SF R with R replaced by the sum at the last stage, the
+    stage state U0 - TF R dt/V.  With SST the k and omega residues are first
+    divided by 1 + beta omega dt and 1 + 2 beta omega dt, beta blended by the
+    stage's F1 and omega the stage's current value."""
+    if turb == "sst":
+        case, nodes, q0 = C.sst_bump(14, 10, 6)
+        wall = True
+    else:
+        case, _, q0 = C.laminar_box(10, 8, 6)
+        nodes = C.cartesian_nodes(case, warp=0.03)
+        wall = False
+    case = dataclasses.replace(case, time_accuracy=rule, time_local=time_local)
+    gm = case.gm
+    inner = (slice(None), slice(3, -3), slice(3, -3), slice(3, -3))
+    o = Oracle(case)
+    o.set_nodes(nodes)
+    if wall:
+        o.wall_dist()
+    o.set_state(q0)
+    o.temp_refresh()
+    temp = o.arr("temp").copy()
+    vol = o.arr("vol")[3:-3, 3:-3, 3:-3].copy()
+    U0 = _to_cons(o.arr("qp").copy()[inner], gm)
+    acc = None
+    stages = STEP_RULES[rule]
+    for s, (TF, SF) in enumerate(stages):
+        assert o.residual(1) == 0
+        if s == 0:
+            o.time_step()
+            dt = o.arr("dt").copy()
+        q = o.arr("qp").copy()
+        R = o.arr("res").copy()
+        if turb == "sst":
+            F1 = o.arr("f1")[3:-3, 3:-3, 3:-3]
+            w = q[6][3:-3, 3:-3, 3:-3]
+            beta = 0.075 * F1 + (1.0 - F1) * 0.0828
+            R[5] = R[5] / (1 + beta * w * dt)
+            R[6] = R[6] / (1 + 2 * beta * w * dt)
+        base = U0 if SF is not None else _to_cons(q[inner], gm)
+        if SF is not None:
+            acc = SF * R if acc is None else acc + SF * R
+            if s == len(stages) - 1:
+                R = acc
+        new = _to_prim(base - R * (TF * dt / vol), gm)
+        if turb == "sst":
+            assert (new[5] >= 0).all() and (new[6] >= 0).all()
+        q[inner] = new
+        o.set_state(q)
+        o.arr("temp")[...] = temp
+    expect = o.arr("qp").copy()[inner]
+    o2 = Oracle(case)
+    o2.set_nodes(nodes)
+    if wall:
+        o2.wall_dist()
+    o2.set_state(q0)
+    assert o2.step(1) == 0
+    np.testing.assert_allclose(o2.arr("qp")[inner], expect, rtol=1e-12, atol=0)
 
 
 def test_resnorm_closed_form():
