@@ -40,6 +40,7 @@ struct Blk {
   unsigned *err;              // error bits
   // scheme
   int flux_scheme, recon, turb, ppm_flag, time_local, time_acc, accur, mu_suth;
+  int xcd_remap;  // XCD-aware block order of the sweep kernels (plane_block)
   int lim[3], tlim[3];
   int bc[6];
   int mz[3][2];               // make_*_flux_zero at the first and last face
@@ -75,6 +76,43 @@ __host__ __device__ inline long gri(const Blk &b, int i, int j, int k) {
 __host__ __device__ inline long ndi(const Blk &b, int i, int j, int k) {
   return (long)(i + 2) + (long)(b.imx + 6) * ((long)(j + 2) + (long)(b.jmx + 6) * (long)(k + 2));
 }
+// Launch geometry of the cell and face sweeps: 256-thread blocks cover the
+// i-fastest (i, j) plane of one k as a flat index and blockIdx.y is k, so an
+// i extent that is not a multiple of 64 leaves no idle lanes per row.
+constexpr int FLAT_BLOCK = 256;
+
+// Block of the (i, j) plane this work-group handles.  Work-groups go to the
+// 8 XCDs round-robin by linear id.  With remap the grid carries a multiple of
+// 8 slots per plane (blockIdx.x) and XCD e takes the contiguous range of
+// blocks e*S/8 .. (e+1)*S/8-1 of every plane, so stencil neighbours along j
+// (adjacent rows) and along k (the same range of the next plane) are read
+// through one L2.  Blocks past the plane's end find no cells.
+__device__ __forceinline__ int plane_block(int remap) {
+  const int s = (int)blockIdx.x;
+  if (!remap) return s;
+  const int S = (int)gridDim.x;
+  return (s & 7) * (S >> 3) + (s >> 3);
+}
+
+__device__ __forceinline__ bool flat_ij(int ni, int nj, int &i, int &j, int blk) {
+  int t = blk * FLAT_BLOCK + (int)threadIdx.x;
+  j = t / ni;
+  i = t - j * ni;
+  return j < nj;
+}
+
+// Tiled variant: each 256-thread block is a 64 (i) x 4 (j) tile, tiles
+// numbered i-fastest in blockIdx.x, blockIdx.y = k.  Sweeps whose stencil
+// runs along j or k keep four rows of one i-range in one block.
+constexpr int TILE_I = 64, TILE_J = 4;
+__device__ __forceinline__ bool tile_ij(int ni, int nj, int &i, int &j, int blk) {
+  const int ntx = (ni + TILE_I - 1) / TILE_I;
+  const int tx = blk % ntx, ty = blk / ntx;
+  i = tx * TILE_I + ((int)threadIdx.x & (TILE_I - 1));
+  j = ty * TILE_J + ((int)threadIdx.x / TILE_I);
+  return i < ni && j < nj;
+}
+
 // stride of the padded cell arrays along direction d
 __host__ __device__ inline long cstride(const Blk &b, int d) {
   return d == 0 ? 1L : d == 1 ? (long)b.pi : (long)b.pi * b.pj;

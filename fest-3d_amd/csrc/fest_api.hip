@@ -144,10 +144,14 @@ int settle_timers(fest_handle *h) {
 
 inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
-// grid over an (ni x nj x nk) index box with 64x4 thread blocks along i, j
-inline void box(long ni, long nj, long nk, dim3 &g, dim3 &t) {
-  t = dim3(64, 4, 1);
-  g = dim3(cdiv(ni, 64), cdiv(nj, 4), (unsigned)nk);
+// blocks per plane, rounded up to a multiple of 8 for the XCD-aware order
+inline unsigned plane_slots(long nblk, int pad8) { return pad8 ? (unsigned)((nblk + 7) / 8 * 8) : (unsigned)nblk; }
+
+// grid over an (ni x nj x nk) index box: flat 256-thread blocks over the
+// (i, j) plane, one grid row per k (the kernels use flat_ij)
+inline void box(long ni, long nj, long nk, dim3 &g, dim3 &t, int pad8 = 0) {
+  t = dim3(FLAT_BLOCK, 1, 1);
+  g = dim3(plane_slots(cdiv(ni * nj, FLAT_BLOCK), pad8), (unsigned)nk, 1);
 }
 
 int check_err(fest_handle *h) {
@@ -161,6 +165,29 @@ int check_err(fest_handle *h) {
   if (e & ERR_NAN_GRAD) return fail(h, FEST_E_NAN, "NaN in the gradients");
   if (e & ERR_NAN_MU) return fail(h, FEST_E_NAN, "NaN in the molecular viscosity");
   return fail(h, FEST_E_CONFIG, "device error bits 0x%x", e);
+}
+
+// the flux kernel of sweep D, specialised on the variable count and the
+// reconstruction
+template <int D, int NV>
+void launch_flux_nv(const Blk &b, dim3 g, dim3 t, hipStream_t st, int store_csnd) {
+  switch (b.recon) {
+    case 0: hipLaunchKernelGGL((k_flux<D, NV, 0>), g, t, 0, st, b, store_csnd); break;
+    case 1: hipLaunchKernelGGL((k_flux<D, NV, 1>), g, t, 0, st, b, store_csnd); break;
+    case 2: hipLaunchKernelGGL((k_flux<D, NV, 2>), g, t, 0, st, b, store_csnd); break;
+    default: hipLaunchKernelGGL((k_flux<D, NV, 3>), g, t, 0, st, b, store_csnd); break;
+  }
+}
+template <int D>
+void launch_flux(const Blk &b, dim3 g, dim3 t, hipStream_t st, int store_csnd) {
+  if (b.nv == 7) launch_flux_nv<D, 7>(b, g, t, st, store_csnd);
+  else launch_flux_nv<D, 5>(b, g, t, st, store_csnd);
+}
+
+// grid of 64 x 4 tiles over an (ni x nj x nk) box (the kernels use tile_ij)
+inline void tiles(long ni, long nj, long nk, dim3 &g, dim3 &t, int pad8) {
+  t = dim3(TILE_I * TILE_J, 1, 1);
+  g = dim3(plane_slots((long)cdiv(ni, TILE_I) * cdiv(nj, TILE_J), pad8), (unsigned)nk, 1);
 }
 
 // C2..C9(+C10 in k_update): ghosts, gradients/viscosity, face fluxes
@@ -196,7 +223,7 @@ int launch_residual_front(fest_handle *h, int current_iter, int store_csnd) {
   LAUNCHCHK(h);
   dim3 g, t;
   if (b.mu_ref != 0.0) {
-    box(b.imx + 1, b.jmx + 1, b.kmx + 1, g, t);
+    box(b.imx + 1, b.jmx + 1, b.kmx + 1, g, t, b.xcd_remap);
     {
       KgScope ts(h, KG_GRAD);
       if (b.ng == 6) hipLaunchKernelGGL(k_grad<6>, g, t, 0, st, b);
@@ -210,20 +237,20 @@ int launch_residual_front(fest_handle *h, int current_iter, int store_csnd) {
     }
     LAUNCHCHK(h);
   }
-  box(b.imx, b.jmx - 1, b.kmx - 1, g, t);
+  box(b.imx, b.jmx - 1, b.kmx - 1, g, t, b.xcd_remap);
   {
     KgScope ts(h, KG_FLUX_I);
-    hipLaunchKernelGGL(k_flux<0>, g, t, 0, st, b, store_csnd);
+    launch_flux<0>(b, g, t, st, store_csnd);
   }
-  box(b.imx - 1, b.jmx, b.kmx - 1, g, t);
+  tiles(b.imx - 1, b.jmx, b.kmx - 1, g, t, b.xcd_remap);
   {
     KgScope ts(h, KG_FLUX_J);
-    hipLaunchKernelGGL(k_flux<1>, g, t, 0, st, b, store_csnd);
+    launch_flux<1>(b, g, t, st, store_csnd);
   }
-  box(b.imx - 1, b.jmx - 1, b.kmx, g, t);
+  tiles(b.imx - 1, b.jmx - 1, b.kmx, g, t, b.xcd_remap);
   {
     KgScope ts(h, KG_FLUX_K);
-    hipLaunchKernelGGL(k_flux<2>, g, t, 0, st, b, store_csnd);
+    launch_flux<2>(b, g, t, st, store_csnd);
   }
   LAUNCHCHK(h);
   return 0;
@@ -252,7 +279,7 @@ int launch_time_step(fest_handle *h) {
 int launch_update(fest_handle *h, double TF, double SF, int flags) {
   Blk &b = h->b;
   dim3 g, t;
-  box(b.imx - 1, b.jmx - 1, b.kmx - 1, g, t);
+  box(b.imx - 1, b.jmx - 1, b.kmx - 1, g, t, b.xcd_remap);
   KgScope ts(h, KG_UPDATE);
   hipLaunchKernelGGL(k_update, g, t, 0, h->stream, b, TF, SF, flags);
   LAUNCHCHK(h);
@@ -315,6 +342,12 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
   for (int d = 0; d < 3; ++d) { b.lim[d] = scheme->lim[d]; b.tlim[d] = scheme->tlim[d]; }
   // src/boundary/boundary_state_reconstruction.f90:45-47
   b.ppm_flag = (scheme->recon == FEST_RECON_PPM || scheme->recon == FEST_RECON_WENO) ? 1 : 0;
+  {
+    // XCD-aware block order of the sweep kernels; FEST_XCD_REMAP=0 switches
+    // it off (a tuning switch: results do not depend on it)
+    const char *env = getenv("FEST_XCD_REMAP");
+    b.xcd_remap = (env && env[0] == '0') ? 0 : 1;
+  }
   for (int f = 0; f < 6; ++f) b.bc[f] = bc->id[f];
   // make_*_flux_zero at the first/last face, src/boundary/bc.f90:60-66
   for (int d = 0; d < 3; ++d)
@@ -354,7 +387,7 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
   AL(b.res, b.nres * b.nv); AL(b.dt, b.nres); AL(b.src, 2 * b.nres); AL(b.ust, b.nc * b.nv); AL(b.rst, b.nres * b.nv);
   AL(b.nodes, 3L * (b.imx + 6) * (b.jmx + 6) * (b.kmx + 6));
   // reduction partials: the larger of the time-step block count and 8 per norm block
-  long nblk_dt = (long)cdiv(b.imx - 1, 64) * cdiv(b.jmx - 1, 4) * (b.kmx - 1);
+  long nblk_dt = (long)cdiv((long)(b.imx - 1) * (b.jmx - 1), FLAT_BLOCK) * (b.kmx - 1);
   h->norm_blocks = (int)std::min<long>(1024, std::max<long>(1, cdiv(b.nres, 256)));
   h->red_cap = std::max<long>(nblk_dt, 8L * h->norm_blocks);
   AL(b.red, h->red_cap);

@@ -5,6 +5,8 @@
 // ghost cells, which only reads its own column, so the per-face passes keep
 // the reference's sequential semantics.
 #pragma once
+// ghost cells and geometry keep the reference's unfused operation order
+#pragma clang fp contract(off)
 #include "blk.h"
 
 namespace fest {

@@ -10,6 +10,10 @@
 // (first/last interior cell MUSCL when ppm_flag, ghost or averaged states on
 // the boundary face).
 #pragma once
+// the residual kernels let multiply-adds fuse into FMA: results differ from
+// the unfused reference order by rounding only, far inside the 1e-12 parity
+// bound, and the FP64 instruction count, which bounds these kernels, drops
+#pragma clang fp contract(fast)
 #include "blk.h"
 
 namespace fest {
@@ -93,10 +97,12 @@ __device__ __forceinline__ void ppm_extrema(double qc, double &Lc, double &Rc) {
   }
 }
 
-// left (L) and right (R) states of face n (u points at cell n, stride s)
+// left (L) and right (R) states of face n (u points at cell n, stride s);
+// RECON is the reconstruction (0 none, 1 MUSCL, 2 PPM, 3 WENO5)
+template <int RECON>
 __device__ __forceinline__ void face_states(const Blk &b, int d, int n, int N, int l, const double *u, long s,
                                             double &L, double &R) {
-  const int recon = b.recon;
+  constexpr int recon = RECON;
   const int sw = (l >= 5) ? b.tlim[d] : b.lim[d];
   const int minbc = b.bc[2 * d], maxbc = b.bc[2 * d + 1];
   const bool on = recon != 0;
@@ -114,16 +120,16 @@ __device__ __forceinline__ void face_states(const Blk &b, int d, int n, int N, i
     bmuscl_cell(u - s, s, sw, L, dummy);
   } else {
     const double *c = u - s;
-    switch (recon) {
-      case 0: L = c[0]; break;
-      case 1: muscl_cell(c, s, sw, L, dummy); break;
-      case 2: {
-        double Lc = ppm_face(u, s), Rc = ppm_face(c, s);
-        if (b.lim[d] == 1) ppm_extrema(c[0], Lc, Rc);
-        L = Lc;
-        break;
-      }
-      default: L = weno_up(c[-2 * s], c[-s], c[0], c[s], c[2 * s]); break;
+    if constexpr (recon == 0) {
+      L = c[0];
+    } else if constexpr (recon == 1) {
+      muscl_cell(c, s, sw, L, dummy);
+    } else if constexpr (recon == 2) {
+      double Lc = ppm_face(u, s), Rc = ppm_face(c, s);
+      if (b.lim[d] == 1) ppm_extrema(c[0], Lc, Rc);
+      L = Lc;
+    } else {
+      L = weno_up(c[-2 * s], c[-s], c[0], c[s], c[2 * s]);
     }
   }
   // right state: lower face of cell n
@@ -134,16 +140,16 @@ __device__ __forceinline__ void face_states(const Blk &b, int d, int n, int N, i
   } else if ((bmin && pf && n == 1) || (bmax && pf && n == N - 1)) {
     bmuscl_cell(u, s, sw, dummy, R);
   } else {
-    switch (recon) {
-      case 0: R = u[0]; break;
-      case 1: muscl_cell(u, s, sw, dummy, R); break;
-      case 2: {
-        double Lc = ppm_face(u + s, s), Rc = ppm_face(u, s);
-        if (b.lim[d] == 1) ppm_extrema(u[0], Lc, Rc);
-        R = Rc;
-        break;
-      }
-      default: R = weno_lo(u[-2 * s], u[-s], u[0], u[s], u[2 * s]); break;
+    if constexpr (recon == 0) {
+      R = u[0];
+    } else if constexpr (recon == 1) {
+      muscl_cell(u, s, sw, dummy, R);
+    } else if constexpr (recon == 2) {
+      double Lc = ppm_face(u + s, s), Rc = ppm_face(u, s);
+      if (b.lim[d] == 1) ppm_extrema(u[0], Lc, Rc);
+      R = Rc;
+    } else {
+      R = weno_lo(u[-2 * s], u[-s], u[0], u[s], u[2 * s]);
     }
   }
 }
@@ -151,10 +157,11 @@ __device__ __forceinline__ void face_states(const Blk &b, int d, int n, int N, i
 // ---- inviscid flux of one face (src/face/flux/convective/*.f90)
 __device__ __forceinline__ double alpha_sub(double M) { return fabs(M) < 1.0 ? 1.0 : 0.0; }
 
-__device__ void inviscid_flux(const Blk &b, const double *qL, const double *qR, double nx, double ny, double nz,
-                              double A, double zero, double *out) {
+template <int NV>
+__device__ __forceinline__ void inviscid_flux(const Blk &b, const double (&qL)[NV], const double (&qR)[NV], double nx,
+                                              double ny, double nz, double A, double zero, double (&out)[NV]) {
   const double gm = b.gm;
-  const int nv = b.nv;
+  constexpr int nv = NV;
   double rL = qL[0], uL = qL[1], vL = qL[2], wL = qL[3], pL = qL[4];
   double rR = qR[0], uR = qR[1], vR = qR[2], wR = qR[3], pR = qR[4];
   const int sch = b.flux_scheme;
@@ -232,6 +239,7 @@ __device__ void inviscid_flux(const Blk &b, const double *qL, const double *qR, 
     out[2] = ((Fp0 * vL + Fm0 * vR) + (pbar * ny)) * A;
     out[3] = ((Fp0 * wL + Fm0 * wR) + (pbar * nz)) * A;
     out[4] = (Fp0 * HL + Fm0 * HR) * A;
+#pragma unroll
     for (int l = 5; l < nv; ++l) out[l] = (Fp0 * qL[l] + Fm0 * qR[l]) * A;
     return;
   }
@@ -275,36 +283,48 @@ __device__ void inviscid_flux(const Blk &b, const double *qL, const double *qR, 
   Fm[2] = (Fm0 * vR) + (sDm * pR * ny);
   Fm[3] = (Fm0 * wR) + (sDm * pR * nz);
   Fm[4] = Fm0 * ((0.5 * (uR * uR + vR * vR + wR * wR)) + ((gm / (gm - 1.)) * pR / rR));
+#pragma unroll
   for (int l = 0; l < 5; ++l) out[l] = Fp[l] * A + Fm[l] * A;
+#pragma unroll
   for (int l = 5; l < nv; ++l) out[l] = (Fp0 * qL[l]) * A + (Fm0 * qR[l]) * A;
 }
 
 // ---- the face kernel
-// D: sweep direction.  Faces: n = 1..N along D, 1..N-1 across.
-template <int D>
-__global__ void __launch_bounds__(256) k_flux(Blk b, int store_csnd) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  int k = blockIdx.z + 1;
+// D: sweep direction.  Faces: n = 1..N along D, 1..N-1 across.  NV: 5
+// (laminar) or 7 (SST) variables; RECON: the reconstruction.  Compile-time
+// counts keep the per-variable arrays in registers.
+template <int D, int NV, int RECON>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_flux(Blk b, int store_csnd) {
   const int ii = D == 0, jj = D == 1, kk = D == 2;
-  if (i > b.imx - 1 + ii || j > b.jmx - 1 + jj || k > b.kmx - 1 + kk) return;
+  int i, j;
+  // i sweep: flat rows (imx faces per row); j, k sweeps: 64 x 4 tiles
+  const int blk = plane_block(b.xcd_remap);
+  const bool in = (D == 0) ? flat_ij(b.imx - 1 + ii, b.jmx - 1 + jj, i, j, blk)
+                           : tile_ij(b.imx - 1 + ii, b.jmx - 1 + jj, i, j, blk);
+  if (!in) return;
+  i += 1;
+  j += 1;
+  const int k = blockIdx.y + 1;
+  if (k > b.kmx - 1 + kk) return;
   const int N = D == 0 ? b.imx : D == 1 ? b.jmx : b.kmx;
   const int n = D == 0 ? i : D == 1 ? j : k;
   const long s = cstride(b, D);
   const long c = ci(b, i, j, k);
   const long cm = c - s;
-  const int nv = b.nv;
-  double qL[7], qR[7], out[7];
+  constexpr int nv = NV;
+  double qL[NV], qR[NV], out[NV];
+#pragma unroll
   for (int l = 0; l < nv; ++l) {
     const double *u = b.qp + (long)l * b.nc + c;
-    face_states(b, D, n, N, l, u, s, qL[l], qR[l]);
+    face_states<RECON>(b, D, n, N, l, u, s, qL[l], qR[l]);
   }
+  const double csf = store_csnd ? 0.5 * (sqrt(b.gm * qL[4] / qL[0]) + sqrt(b.gm * qR[4] / qR[0])) : 0.0;
   const long fo = fi(b, D, i, j, k);
   const double nx = b.fx[D][fo], ny = b.fy[D][fo], nz = b.fz[D][fo], A = b.fA[D][fo];
   const double zero = (n == 1) ? (double)b.mz[D][0] : (n == N) ? (double)b.mz[D][1] : 1.0;
   inviscid_flux(b, qL, qR, nx, ny, nz, A, zero, out);
-
-  if (b.mu_ref != 0.0) {
+  const bool viscous = b.mu_ref != 0.0;
+  if (viscous) {
     // laminar viscous flux, src/viscous.f90:144-325
     const double *qp = b.qp;
     const long nc = b.nc;
@@ -315,6 +335,7 @@ __global__ void __launch_bounds__(256) k_flux(Blk b, int store_csnd) {
     const double *gx = b.g[0], *gy = b.g[1], *gz = b.g[2];
     const long ngr = b.ngr;
     double gr[4][3];
+#pragma unroll
     for (int l = 0; l < 4; ++l) {
       gr[l][0] = 0.5 * (gx[l * ngr + gL] + gx[l * ngr + gR]);
       gr[l][1] = 0.5 * (gy[l * ngr + gL] + gy[l * ngr + gR]);
@@ -324,6 +345,7 @@ __global__ void __launch_bounds__(256) k_flux(Blk b, int store_csnd) {
     double T_RE = qp[4 * nc + c] / (qp[c] * b.R);
     double del[4] = {qp[nc + c] - qp[nc + cm], qp[2 * nc + c] - qp[2 * nc + cm], qp[3 * nc + c] - qp[3 * nc + cm],
                      T_RE - T_LE};
+#pragma unroll
     for (int l = 0; l < 4; ++l) {
       double ncmp = (del[l] - (gr[l][0] * delx + gr[l][1] * dely + gr[l][2] * delz)) * id2;
       gr[l][0] = gr[l][0] + ncmp * delx;
@@ -331,7 +353,7 @@ __global__ void __launch_bounds__(256) k_flux(Blk b, int store_csnd) {
       gr[l][2] = gr[l][2] + ncmp * delz;
     }
     double mu_f = 0.5 * (b.mu[cm] + b.mu[c]);
-    double mut_f = b.turb ? 0.5 * (b.mut[cm] + b.mut[c]) : 0.0;
+    double mut_f = (NV == 7) ? 0.5 * (b.mut[cm] + b.mut[c]) : 0.0;
     double tm = mu_f + mut_f;
     double dudx = gr[0][0], dudy = gr[0][1], dudz = gr[0][2];
     double dvdx = gr[1][0], dvdy = gr[1][1], dvdz = gr[1][2];
@@ -353,15 +375,17 @@ __global__ void __launch_bounds__(256) k_flux(Blk b, int store_csnd) {
     out[3] = out[3] - ((Txz * nx + Tyz * ny + Tzz * nz) * A);
     out[4] = out[4] - (A * (((Txx * uf + Txy * vf + Txz * wf + Qx) * nx) + ((Txy * uf + Tyy * vf + Tyz * wf + Qy) * ny) +
                             ((Txz * uf + Tyz * vf + Tzz * wf + Qz) * nz)));
-    if (b.turb) {
+    if constexpr (NV == 7) {
       // SST k / omega diffusion and the -2/3 rho k stress, src/viscous.f90:328-447
       double gt[2][3];
+#pragma unroll
       for (int l = 0; l < 2; ++l) {
         gt[l][0] = 0.5 * (gx[(4 + l) * ngr + gL] + gx[(4 + l) * ngr + gR]);
         gt[l][1] = 0.5 * (gy[(4 + l) * ngr + gL] + gy[(4 + l) * ngr + gR]);
         gt[l][2] = 0.5 * (gz[(4 + l) * ngr + gL] + gz[(4 + l) * ngr + gR]);
       }
       double dt2[2] = {qp[5 * nc + c] - qp[5 * nc + cm], qp[6 * nc + c] - qp[6 * nc + cm]};
+#pragma unroll
       for (int l = 0; l < 2; ++l) {
         double ncmp = (dt2[l] - (gt[l][0] * delx + gt[l][1] * dely + gt[l][2] * delz)) * id2;
         gt[l][0] = gt[l][0] + ncmp * delx;
@@ -383,15 +407,16 @@ __global__ void __launch_bounds__(256) k_flux(Blk b, int store_csnd) {
       out[6] = out[6] - (A * ((mu_f + sigma_wf * mut_f) * (gt[1][0] * nx + gt[1][1] * ny + gt[1][2] * nz)));
     }
   }
+
   const long o = fli(b, D, i, j, k);
   bool bad = false;
+#pragma unroll
   for (int l = 0; l < nv; ++l) {
     bad |= isnan(out[l]);
     b.flux[D][(long)l * b.nfl[D] + o] = out[l];
   }
   if (bad) atomicOr(b.err, ERR_NAN_FLUX);
-  if (store_csnd)
-    b.csnd[D][o] = 0.5 * (sqrt(b.gm * qL[4] / qL[0]) + sqrt(b.gm * qR[4] / qR[0]));
+  if (store_csnd) b.csnd[D][o] = csf;
 }
 
 }  // namespace fest

@@ -4,6 +4,8 @@
 // 342-496), centroids (src/geometry.f90:500-545) and the brute-force wall
 // distance (src/wall/wall_dist.f90:84-131).
 #pragma once
+// ghost cells and geometry keep the reference's unfused operation order
+#pragma clang fp contract(off)
 #include "blk.h"
 
 namespace fest {

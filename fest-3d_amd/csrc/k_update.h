@@ -4,6 +4,10 @@
 // time step (src/time.f90:122-531) and the residual norms
 // (src/resnorm.f90:171-225).
 #pragma once
+// the residual kernels let multiply-adds fuse into FMA: results differ from
+// the unfused reference order by rounding only, far inside the 1e-12 parity
+// bound, and the FP64 instruction count, which bounds these kernels, drops
+#pragma clang fp contract(fast)
 #include "blk.h"
 
 namespace fest {
@@ -23,10 +27,12 @@ __device__ __forceinline__ double sspd(const Blk &b, int d, long o) { return b.c
 
 // local time step of one cell (src/time.f90:122-246, 366-531)
 __global__ void __launch_bounds__(256) k_dt(Blk b) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  int k = blockIdx.z + 1;
-  bool active = (i <= b.imx - 1 && j <= b.jmx - 1 && k <= b.kmx - 1);
+  int i, j;
+  bool active = flat_ij(b.imx - 1, b.jmx - 1, i, j, (int)blockIdx.x);
+  i += 1;
+  j += 1;
+  const int k = blockIdx.y + 1;
+  active = active && k <= b.kmx - 1;
   double dtv = 1e300;
   if (active) {
     const long nc = b.nc;
@@ -75,7 +81,7 @@ __global__ void __launch_bounds__(256) k_dt(Blk b) {
   if (!b.time_local) {
     // block minimum for the global step (src/time.f90:248-289)
     __shared__ double smin[256];
-    int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    int tid = threadIdx.x;
     smin[tid] = dtv;
     __syncthreads();
     for (int st = 128; st > 0; st >>= 1) {
@@ -83,7 +89,7 @@ __global__ void __launch_bounds__(256) k_dt(Blk b) {
       __syncthreads();
     }
     if (tid == 0) {
-      long blk = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+      long blk = blockIdx.x + (long)gridDim.x * blockIdx.y;
       b.red[blk] = smin[0];
     }
   }
@@ -112,10 +118,12 @@ enum { UPD_FIRST = 1, UPD_LAST = 2, UPD_WRITE_RES = 4, UPD_APPLY = 8, UPD_STORE 
 
 // residue + SST source, then (UPD_APPLY) the conservative update of the cell
 __global__ void __launch_bounds__(256) k_update(Blk b, double TF, double SF, int flags) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  int k = blockIdx.z + 1;
-  if (i > b.imx - 1 || j > b.jmx - 1 || k > b.kmx - 1) return;
+  int i, j;
+  if (!flat_ij(b.imx - 1, b.jmx - 1, i, j, plane_block(b.xcd_remap))) return;
+  i += 1;
+  j += 1;
+  const int k = blockIdx.y + 1;
+  if (k > b.kmx - 1) return;
   const int nv = b.nv;
   const long nc = b.nc;
   long c = ci(b, i, j, k);
