@@ -19,6 +19,7 @@
 #include "k_flux.h"
 #include "k_geom.h"
 #include "k_grad.h"
+#include "k_iface.h"
 #include "k_update.h"
 
 using namespace fest;
@@ -37,7 +38,9 @@ struct KgTimer {
 
 struct fest_handle {
   Blk b;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;      // the stream every launch goes to
+  hipStream_t own_stream = nullptr;  // the handle's own; a group may lend another
+  struct fest_group *group = nullptr;
   std::string err;
   fest_extent dims;
   fest_scheme scheme;
@@ -52,6 +55,18 @@ struct fest_handle {
   unsigned timed = 0;
   KgTimer timers[KG_COUNT];
   std::vector<hipEvent_t> ev_free;
+};
+
+// a group's view of its blocks; a block destroyed while in a group detaches
+// itself and leaves the group unusable (its links point into the block)
+struct fest_group {
+  std::vector<fest_handle *> blocks;
+  std::vector<fest_link> links;
+  LinkDev *d_links = nullptr;
+  int max_layer = 0;  // largest cells-per-layer of any link
+  hipStream_t stream = nullptr;
+  bool broken = false;
+  std::string err;
 };
 
 namespace {
@@ -309,8 +324,10 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
     return bad("time integration not supported (RK4, RK2, none)");
   for (int f = 0; f < 6; ++f) {
     int id = bc->id[f];
-    if (id >= 0 || id == -10 || id == -11) return bad("interfaces, -10 and -11 boundaries are not supported on one block");
-    if (id < -9) return bad("unknown boundary id");
+    // ids >= 0 (interfaces) and -10 (cross-block periodic) take their ghost
+    // layers from a group's interface exchange (fest_group_create)
+    if (id == -11) return bad("total-pressure inlets (-11) are not supported");
+    if (id < -10) return bad("unknown boundary id");
     if (id == -7) return bad("pole boundaries are not supported");
   }
   if (flow->mu_ref == 0.0 && scheme->turb) return bad("turbulence needs mu_ref != 0");
@@ -375,7 +392,8 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
 
   *out = h;
   int rc = 0;
-  if (hipStreamCreate(&h->stream) != hipSuccess) return fail(h, FEST_E_HIP, "hipStreamCreate failed");
+  if (hipStreamCreate(&h->own_stream) != hipSuccess) return fail(h, FEST_E_HIP, "hipStreamCreate failed");
+  h->stream = h->own_stream;
 #define AL(p, n) if ((rc = dalloc(h, &(p), (n)))) return rc
   AL(b.qp, b.nc * b.nv); AL(b.temp, b.nc); AL(b.mu, b.nc); AL(b.mut, b.nc); AL(b.f1, b.nc); AL(b.dist, b.nc);
   AL(b.vol, b.nc); AL(b.cx, b.nc); AL(b.cy, b.nc); AL(b.cz, b.nc);
@@ -411,6 +429,12 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
 void fest_destroy(fest_handle *h) {
   if (!h) return;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->group) {
+    auto &v = h->group->blocks;
+    v.erase(std::remove(v.begin(), v.end(), h), v.end());
+    h->group->broken = true;
+    h->group->err = "a block of the group was destroyed";
+  }
   for (auto &t : h->timers)
     for (auto &pr : t.pending) {
       (void)hipEventDestroy(pr.first);
@@ -418,7 +442,7 @@ void fest_destroy(fest_handle *h) {
     }
   for (hipEvent_t e : h->ev_free) (void)hipEventDestroy(e);
   for (void *p : h->allocs) (void)hipFree(p);
-  if (h->stream) (void)hipStreamDestroy(h->stream);
+  if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
   delete h;
 }
 
@@ -702,6 +726,185 @@ int fest_kernel_time(fest_handle *h, const char *group, double *total_ms, long *
       return 0;
     }
   return fail(h, FEST_E_ARG, "unknown kernel group '%s'", group);
+}
+
+// ---------------------------------------------------------------- groups
+namespace {
+
+int gfail(fest_group *g, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (g) g->err = buf;
+  return code;
+}
+
+// cells of the first and second tangential direction of face f (1..6)
+void face_cells(const Blk &b, int f, int &n1, int &n2) {
+  if (f <= 2) { n1 = b.jmx - 1; n2 = b.kmx - 1; }
+  else if (f <= 4) { n1 = b.imx - 1; n2 = b.kmx - 1; }
+  else { n1 = b.imx - 1; n2 = b.jmx - 1; }
+}
+
+// map node range -> receiving loop start and step, as
+// change_map_to_particular_range (src/mapping.f90:186-258)
+void loop_range(int lo, int hi, int &start, int &end, int &dir) {
+  start = lo; end = hi; dir = 1;
+  if (lo == 1) start = 1;
+  if (hi == 1) { end = 1; dir = -1; }
+  if (lo > 1) { start = lo - 1; dir = -1; }
+  if (hi > 1) end = hi - 1;
+}
+
+int group_exchange(fest_group *g) {
+  if (g->links.empty()) return 0;
+  dim3 grid(cdiv(3L * g->max_layer, 256), (unsigned)g->links.size(), 1);
+  hipLaunchKernelGGL(k_exchange, grid, dim3(256), 0, g->stream, (const LinkDev *)g->d_links);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : gfail(g, FEST_E_HIP, "exchange launch: %s", hipGetErrorString(e));
+}
+
+int group_check(fest_group *g) {
+  for (size_t n = 0; n < g->blocks.size(); ++n) {
+    int rc = check_err(g->blocks[n]);
+    if (rc) return gfail(g, rc, "block %zu: %s", n, g->blocks[n]->err.c_str());
+  }
+  return 0;
+}
+
+}  // namespace
+
+int fest_group_create(fest_group **out, fest_handle *const *blocks, int nblocks, const fest_link *links,
+                      int nlinks) {
+  if (!out || !blocks || nblocks < 1 || (nlinks > 0 && !links) || nlinks < 0) return FEST_E_ARG;
+  fest_group *g = new fest_group();
+  *out = g;
+  for (int n = 0; n < nblocks; ++n) {
+    if (!blocks[n]) return gfail(g, FEST_E_ARG, "block %d is null", n);
+    if (blocks[n]->group) return gfail(g, FEST_E_ARG, "block %d is already in a group", n);
+    if (blocks[n]->b.nv != blocks[0]->b.nv || blocks[n]->b.time_acc != blocks[0]->b.time_acc)
+      return gfail(g, FEST_E_CONFIG, "blocks of a group need the same n_var and time integration");
+    for (int m = 0; m < n; ++m)
+      if (blocks[m] == blocks[n]) return gfail(g, FEST_E_ARG, "block %d appears twice", n);
+  }
+  std::vector<LinkDev> dl;
+  for (int e = 0; e < nlinks; ++e) {
+    const fest_link &L = links[e];
+    if (L.block < 0 || L.block >= nblocks || L.from_block < 0 || L.from_block >= nblocks || L.face < 1 ||
+        L.face > 6 || L.from_face < 1 || L.from_face > 6)
+      return gfail(g, FEST_E_ARG, "link %d: block or face out of range", e);
+    const Blk &r = blocks[L.block]->b, &s = blocks[L.from_block]->b;
+    int rn1, rn2, sn1, sn2;
+    face_cells(r, L.face, rn1, rn2);
+    face_cells(s, L.from_face, sn1, sn2);
+    int lo1, hi1, d1, lo2, hi2, d2;
+    loop_range(L.lo1, L.hi1, lo1, hi1, d1);
+    loop_range(L.lo2, L.hi2, lo2, hi2, d2);
+    // the receiving loops must cover the face cells exactly, and the sender
+    // must supply as many cells per layer as the receiver takes
+    if ((hi1 - lo1) * d1 + 1 != rn1 || (hi2 - lo2) * d2 + 1 != rn2 || std::min(lo1, hi1) != 1 ||
+        std::min(lo2, hi2) != 1 || (long)sn1 * sn2 != (long)rn1 * rn2)
+      return gfail(g, FEST_E_CONFIG, "link %d: map ranges do not match the faces (%d x %d <- %d x %d)", e, rn1, rn2,
+                   sn1, sn2);
+    LinkDev d;
+    memset(&d, 0, sizeof d);
+    d.src = s.qp; d.dst = r.qp;
+    d.src_nc = s.nc; d.dst_nc = r.nc;
+    d.src_pi = s.pi; d.src_pj = s.pj; d.dst_pi = r.pi; d.dst_pj = r.pj;
+    d.src_N[0] = s.imx; d.src_N[1] = s.jmx; d.src_N[2] = s.kmx;
+    d.dst_N[0] = r.imx; d.dst_N[1] = r.jmx; d.dst_N[2] = r.kmx;
+    d.src_face = L.from_face; d.dst_face = L.face;
+    d.s_n1 = sn1; d.per_layer = sn1 * sn2;
+    d.lo1 = lo1; d.d1 = d1; d.lo2 = lo2; d.d2 = d2;
+    d.dir_switch = L.dir_switch ? 1 : 0;
+    d.r_inner = d.dir_switch ? rn2 : rn1;
+    d.nv = r.nv;
+    dl.push_back(d);
+    g->max_layer = std::max(g->max_layer, d.per_layer);
+  }
+  g->links.assign(links, links + nlinks);
+  if (!dl.empty()) {
+    if (hipMalloc(&g->d_links, sizeof(LinkDev) * dl.size()) != hipSuccess)
+      return gfail(g, FEST_E_HIP, "hipMalloc of the link table failed");
+    if (hipMemcpy(g->d_links, dl.data(), sizeof(LinkDev) * dl.size(), hipMemcpyHostToDevice) != hipSuccess)
+      return gfail(g, FEST_E_HIP, "link table upload failed");
+  }
+  if (hipStreamCreate(&g->stream) != hipSuccess) return gfail(g, FEST_E_HIP, "hipStreamCreate failed");
+  for (int n = 0; n < nblocks; ++n) {
+    fest_handle *h = blocks[n];
+    (void)hipStreamSynchronize(h->own_stream);
+    h->stream = g->stream;
+    h->group = g;
+    g->blocks.push_back(h);
+  }
+  return 0;
+}
+
+void fest_group_destroy(fest_group *g) {
+  if (!g) return;
+  if (g->stream) (void)hipStreamSynchronize(g->stream);
+  for (fest_handle *h : g->blocks) {
+    h->stream = h->own_stream;
+    h->group = nullptr;
+  }
+  if (g->d_links) (void)hipFree(g->d_links);
+  if (g->stream) (void)hipStreamDestroy(g->stream);
+  delete g;
+}
+
+const char *fest_group_last_error(const fest_group *g) { return g ? g->err.c_str() : "null group"; }
+
+int fest_group_exchange(fest_group *g) {
+  if (!g) return FEST_E_ARG;
+  if (g->broken) return FEST_E_ARG;
+  int rc = group_exchange(g);
+  if (rc) return rc;
+  if (hipStreamSynchronize(g->stream) != hipSuccess) return gfail(g, FEST_E_HIP, "exchange failed");
+  return 0;
+}
+
+int fest_group_get_next_solution(fest_group *g, int current_iter, double cfl) {
+  if (!g) return FEST_E_ARG;
+  if (g->broken) return FEST_E_ARG;
+  for (fest_handle *h : g->blocks)
+    if (!h->have_grid || !h->have_state) return gfail(g, FEST_E_ARG, "grid and state must be set on every block");
+  int rc;
+  const int ta = g->blocks[0]->b.time_acc;
+  for (fest_handle *h : g->blocks) {
+    h->b.CFLv = cfl;
+    KgScope ts(h, KG_STEP_BEGIN);
+    hipLaunchKernelGGL(k_step_begin, dim3(cdiv(h->b.nc, 256)), dim3(256), 0, h->stream, h->b, ta != FEST_TA_NONE);
+  }
+  if (ta == FEST_TA_NONE) {
+    if ((rc = group_exchange(g))) return rc;
+    for (fest_handle *h : g->blocks) {
+      if ((rc = launch_residual_front(h, current_iter, 1))) return gfail(g, rc, "%s", h->err.c_str());
+      if ((rc = launch_update(h, 0.0, 0.0, UPD_WRITE_RES))) return gfail(g, rc, "%s", h->err.c_str());
+      if ((rc = launch_time_step(h))) return gfail(g, rc, "%s", h->err.c_str());
+      if ((rc = launch_update(h, 1.0, 1.0, UPD_APPLY))) return gfail(g, rc, "%s", h->err.c_str());
+    }
+    return group_check(g);
+  }
+  // src/update.f90:176-195, every stage opened by the exchange (:534)
+  const double TF4[4] = {0.5, 0.5, 1.0, 1. / 6.}, SF4[4] = {1., 2., 2., 1.};
+  const double TF2[2] = {0.5, 0.5}, SF2[2] = {1., 1.};
+  const int nst = ta == FEST_TA_RK4 ? 4 : 2;
+  const double *TF = ta == FEST_TA_RK4 ? TF4 : TF2;
+  const double *SF = ta == FEST_TA_RK4 ? SF4 : SF2;
+  for (int st = 0; st < nst; ++st) {
+    if ((rc = group_exchange(g))) return rc;
+    for (fest_handle *h : g->blocks) {
+      if ((rc = launch_residual_front(h, current_iter, st == 0))) return gfail(g, rc, "%s", h->err.c_str());
+      if (st == 0 && (rc = launch_time_step(h))) return gfail(g, rc, "%s", h->err.c_str());
+      int flags = UPD_APPLY | UPD_STORE;
+      if (st == 0) flags |= UPD_FIRST;
+      if (st == nst - 1) flags |= UPD_LAST | UPD_WRITE_RES;
+      if ((rc = launch_update(h, TF[st], SF[st], flags))) return gfail(g, rc, "%s", h->err.c_str());
+    }
+  }
+  return group_check(g);
 }
 
 int fest_sync(fest_handle *h) {

@@ -312,17 +312,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   const long c = ci(b, i, j, k);
   const long cm = c - s;
   constexpr int nv = NV;
-  double qL[NV], qR[NV], out[NV];
-#pragma unroll
-  for (int l = 0; l < nv; ++l) {
-    const double *u = b.qp + (long)l * b.nc + c;
-    face_states<RECON>(b, D, n, N, l, u, s, qL[l], qR[l]);
-  }
-  const double csf = store_csnd ? 0.5 * (sqrt(b.gm * qL[4] / qL[0]) + sqrt(b.gm * qR[4] / qR[0])) : 0.0;
+  const long o = fli(b, D, i, j, k);
+  double out[NV];
+  double csf = 0.0;
   const long fo = fi(b, D, i, j, k);
   const double nx = b.fx[D][fo], ny = b.fy[D][fo], nz = b.fz[D][fo], A = b.fA[D][fo];
-  const double zero = (n == 1) ? (double)b.mz[D][0] : (n == N) ? (double)b.mz[D][1] : 1.0;
-  inviscid_flux(b, qL, qR, nx, ny, nz, A, zero, out);
+  {
+    double qL[NV], qR[NV];
+#pragma unroll
+    for (int l = 0; l < nv; ++l) {
+      const double *u = b.qp + (long)l * b.nc + c;
+      face_states<RECON>(b, D, n, N, l, u, s, qL[l], qR[l]);
+    }
+    if (store_csnd) csf = 0.5 * (sqrt(b.gm * qL[4] / qL[0]) + sqrt(b.gm * qR[4] / qR[0]));
+    const double zero = (n == 1) ? (double)b.mz[D][0] : (n == N) ? (double)b.mz[D][1] : 1.0;
+    inviscid_flux(b, qL, qR, nx, ny, nz, A, zero, out);
+  }
   const bool viscous = b.mu_ref != 0.0;
   if (viscous) {
     // laminar viscous flux, src/viscous.f90:144-325
@@ -408,7 +413,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     }
   }
 
-  const long o = fli(b, D, i, j, k);
   bool bad = false;
 #pragma unroll
   for (int l = 0; l < nv; ++l) {

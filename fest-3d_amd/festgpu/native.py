@@ -82,14 +82,27 @@ def lib():
         L.fest_resnorm.argtypes = [vp, dp]
         L.fest_device_ptr.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_size_t)]
         L.fest_sync.argtypes = [vp]
+        L.fest_group_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp), ip, ctypes.POINTER(FestLink), ip]
+        L.fest_group_destroy.argtypes = [vp]
+        L.fest_group_last_error.argtypes = [vp]
+        L.fest_group_last_error.restype = ctypes.c_char_p
+        L.fest_group_exchange.argtypes = [vp]
+        L.fest_group_get_next_solution.argtypes = [vp, ip, ctypes.c_double]
         L.fest_timing.argtypes = [vp, ctypes.c_char_p]
         L.fest_kernel_time.argtypes = [vp, ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_long)]
         for f in ("fest_create", "fest_upload_geometry", "fest_set_grid", "fest_upload_state",
                   "fest_get_next_solution", "fest_residual", "fest_time_step", "fest_download", "fest_resnorm",
-                  "fest_device_ptr", "fest_sync", "fest_timing", "fest_kernel_time"):
+                  "fest_device_ptr", "fest_sync", "fest_timing", "fest_kernel_time", "fest_group_create",
+                  "fest_group_exchange", "fest_group_get_next_solution"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
+
+
+class FestLink(ctypes.Structure):
+    """fest_link (include/fest_hip.h)."""
+    _fields_ = [(n, ctypes.c_int32) for n in
+                ("block", "face", "from_block", "from_face", "lo1", "hi1", "lo2", "hi2", "dir_switch")]
 
 
 def c_structs(case: Case):
@@ -249,3 +262,43 @@ class GpuBlock:
         n = ctypes.c_size_t()
         self._check(lib().fest_device_ptr(self.h, name.encode(), ctypes.byref(p), ctypes.byref(n)))
         return p.value, n.value
+
+
+class GpuGroup:
+    """A multiblock grid over GpuBlocks joined by interface links
+    (festgpu.multiblock.Link tuples); fest_group_* of the C ABI."""
+
+    def __init__(self, blocks, links):
+        self.blocks = list(blocks)
+        arr = (FestLink * max(len(links), 1))()
+        for n, L in enumerate(links):
+            arr[n] = FestLink(*[int(v) for v in L])
+        hs = (ctypes.c_void_p * len(self.blocks))(*[b.h for b in self.blocks])
+        self.g = ctypes.c_void_p()
+        rc = lib().fest_group_create(ctypes.byref(self.g), hs, len(self.blocks), arr, len(links))
+        if rc != 0:
+            msg = lib().fest_group_last_error(self.g).decode() if self.g else ""
+            self.close()
+            raise FestError(rc, msg)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise FestError(rc, lib().fest_group_last_error(self.g).decode())
+
+    def exchange(self):
+        self._check(lib().fest_group_exchange(self.g))
+
+    def get_next_solution(self, current_iter: int, cfl: float = None):
+        c = self.blocks[0].case.CFL if cfl is None else cfl
+        self._check(lib().fest_group_get_next_solution(self.g, int(current_iter), float(c)))
+
+    def close(self):
+        if getattr(self, "g", None):
+            lib().fest_group_destroy(self.g)
+            self.g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

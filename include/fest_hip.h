@@ -168,6 +168,44 @@ int fest_kernel_time(fest_handle *h, const char *group, double *total_ms, long *
 /* Synchronise the handle's stream. */
 int fest_sync(fest_handle *h);
 
+/* ---- multiblock grids
+ * One interface link (the reference's interface map, src/mapping.f90:64-258):
+ * the three ghost layers of face `face` (1..6 = imin, imax, jmin, jmax, kmin,
+ * kmax) of block `block` receive the three interior layers next to face
+ * `from_face` of block `from_block` (block = index into the group's array).
+ * The sender packs them in its natural loop order, first tangential
+ * direction fastest (j on i-faces, i on j- and k-faces); the receiver walks
+ * its first / second tangential direction through the node ranges
+ * lo1..hi1 / lo2..hi2 (the map's S1 E1 S2 E2 of the receiving side; a range
+ * with hi < lo runs backwards) with the second direction as the outer loop,
+ * or the first when dir_switch is set (src/interface1.f90:96-493).  A
+ * cross-block periodic pair (bc id -10, src/boundary/bc.f90:27-32,
+ * src/interface1.f90:496-705) is a link with aligned ranges. */
+typedef struct {
+  int32_t block, face, from_block, from_face;
+  int32_t lo1, hi1, lo2, hi2;
+  int32_t dir_switch;
+} fest_link;
+
+typedef struct fest_group fest_group;
+
+/* A multiblock grid over existing blocks (each with its grid and state set
+ * or to be set): every block's boundary ids >= 0 and -10 are filled by the
+ * links.  The blocks' launches move to the group's stream while the group
+ * exists; destroy the group before its blocks.  Replaces the per-rank
+ * setup_interface (src/interface1.f90:60-92) of the reference's MPI run,
+ * with all blocks resident on one device. */
+int fest_group_create(fest_group **g, fest_handle *const *blocks, int nblocks, const fest_link *links,
+                      int nlinks);
+void fest_group_destroy(fest_group *g);
+const char *fest_group_last_error(const fest_group *g);
+/* apply_interface (src/interface1.f90:96-493) on every link, device to device. */
+int fest_group_exchange(fest_group *g);
+/* get_next_solution (src/update.f90:129-226) on every block, stage by stage,
+ * the interface exchange opening every residual evaluation
+ * (src/update.f90:534). */
+int fest_group_get_next_solution(fest_group *g, int current_iter, double cfl);
+
 #ifdef __cplusplus
 }
 #endif

@@ -93,6 +93,31 @@ def _load(path):
     return ctypes.CDLL(path)
 
 
+class CLink(ctypes.Structure):
+    """orc_link (oracle/fest_oracle.h), one interface link."""
+    _fields_ = [(n, ctypes.c_int32) for n in
+                ("block", "face", "from_block", "from_face", "lo1", "hi1", "lo2", "hi2", "dir_switch")]
+
+
+def _links(links):
+    arr = (CLink * max(len(links), 1))()
+    for n, L in enumerate(links):
+        arr[n] = CLink(*[int(v) for v in L])
+    return arr
+
+
+def group_exchange(blocks, links):
+    """The interface exchange over Oracle blocks (src/interface1.f90)."""
+    hs = (ctypes.c_void_p * len(blocks))(*[o.h for o in blocks])
+    return Oracle.lib().orc_exchange(hs, len(blocks), _links(links), len(links))
+
+
+def group_step(blocks, links, current_iter):
+    """One time step of every block, the exchange opening each residual."""
+    hs = (ctypes.c_void_p * len(blocks))(*[o.h for o in blocks])
+    return Oracle.lib().orc_group_step(hs, len(blocks), _links(links), len(links), int(current_iter))
+
+
 class Oracle:
     """One block of the C restatement."""
 
@@ -119,6 +144,12 @@ class Oracle:
             L.orc_face_flux.argtypes = [ctypes.c_void_p, dp, dp] + [ctypes.c_double] * 4 + [ctypes.c_int, dp]
             L.orc_temp_refresh.argtypes = [ctypes.c_void_p]
             L.orc_resnorm.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+            L.orc_exchange.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.POINTER(CLink),
+                                       ctypes.c_int]
+            L.orc_exchange.restype = ctypes.c_int
+            L.orc_group_step.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.POINTER(CLink),
+                                         ctypes.c_int, ctypes.c_int]
+            L.orc_group_step.restype = ctypes.c_int
             cls._lib = L
         return cls._lib
 

@@ -1563,7 +1563,7 @@ static int update_with(orc *s, double TF, double SF, int TU, int use_store) {
   int status = ORC_OK;
   for (int k = 1; k <= KMX - 1; ++k) for (int j = 1; j <= JMX - 1; ++j) for (int i = 1; i <= IMX - 1; ++i) {
     size_t ci = CI(s, i, j, k), ri = RI(s, i, j, k);
-    double u1[8], u2[8], R[8];
+    double u1[8] = {0}, u2[8] = {0}, R[8] = {0};
     u1[0] = Qu[ci];
     for (int l = 1; l < nv; ++l) u1[l] = Qu[(size_t)l * s->ncell + ci] * u1[0];
     double KE = 0.;
@@ -1649,4 +1649,138 @@ void orc_resnorm(orc *s, double *out) {
   for (int j = 1; j <= JMX - 1; ++j) for (int i = 1; i <= IMX - 1; ++i) { sH1 += H[HI(s, i, j, 1)]; sHn += H[HI(s, i, j, KMX)]; }
   double merror = (sF1 - sFn + sG1 - sGn + sH1 - sHn);
   out[0] = fabs(merror / scale[0]);
+}
+
+/* ------------------------------------------------------------ multiblock */
+/* ghost cell of layer l (1..3) at tangential cell (a, b) of face f (1..6),
+ * src/interface1.f90 receive loops: qp(1-l,..), qp(imx+l-1,..), ... */
+static size_t ghost_cell(const orc *s, int f, int l, int a, int b) {
+  switch (f) {
+    case 1: return CI(s, 1 - l, a, b);
+    case 2: return CI(s, IMX + l - 1, a, b);
+    case 3: return CI(s, a, 1 - l, b);
+    case 4: return CI(s, a, JMX + l - 1, b);
+    case 5: return CI(s, a, b, 1 - l);
+    default: return CI(s, a, b, KMX + l - 1);
+  }
+}
+/* interior cell of layer l (1..3) next to face f, src/interface1.f90 pack
+ * loops: qp(l,..), qp(imx-l,..), ... */
+static size_t layer_cell(const orc *s, int f, int l, int a, int b) {
+  switch (f) {
+    case 1: return CI(s, l, a, b);
+    case 2: return CI(s, IMX - l, a, b);
+    case 3: return CI(s, a, l, b);
+    case 4: return CI(s, a, JMX - l, b);
+    case 5: return CI(s, a, b, l);
+    default: return CI(s, a, b, KMX - l);
+  }
+}
+/* cell extents (node count - 1) of the first and second tangential
+ * direction of face f */
+static void face_extents(const orc *s, int f, int *n1, int *n2) {
+  if (f <= 2) { *n1 = JMX - 1; *n2 = KMX - 1; }
+  else if (f <= 4) { *n1 = IMX - 1; *n2 = KMX - 1; }
+  else { *n1 = IMX - 1; *n2 = JMX - 1; }
+}
+/* node range of the map -> cell loop start/end/step,
+ * change_map_to_particular_range (src/mapping.f90:186-258) */
+static void cell_range(int lo, int hi, int *P_lo, int *P_hi, int *dir) {
+  *P_lo = lo; *P_hi = hi; *dir = 1;
+  if (lo == 1) *P_lo = 1;
+  if (hi == 1) { *P_hi = 1; *dir = -1; }
+  if (lo > 1) { *P_lo = lo - 1; *dir = -1; }
+  if (hi > 1) *P_hi = hi - 1;
+}
+
+int orc_exchange(orc **blocks, int nblocks, const orc_link *links, int nlinks) {
+  /* pack every sender first (the reference's SENDRECV reads interior layers
+   * only and writes ghost layers only, so the order of links is free) */
+  double **buf = (double **)calloc((size_t)(nlinks > 0 ? nlinks : 1), sizeof(double *));
+  if (!buf) return ORC_E_CONFIG;
+  int st = ORC_OK;
+  for (int e = 0; e < nlinks && st == ORC_OK; ++e) {
+    const orc_link *L = &links[e];
+    if (L->block < 0 || L->block >= nblocks || L->from_block < 0 || L->from_block >= nblocks ||
+        L->face < 1 || L->face > 6 || L->from_face < 1 || L->from_face > 6) { st = ORC_E_CONFIG; break; }
+    const orc *src = blocks[L->from_block];
+    int n1, n2;
+    face_extents(src, L->from_face, &n1, &n2);
+    buf[e] = (double *)malloc(sizeof(double) * (size_t)src->nv * 3 * (size_t)n1 * (size_t)n2);
+    if (!buf[e]) { st = ORC_E_CONFIG; break; }
+    size_t count = 0;
+    for (int n = 0; n < src->nv; ++n)
+      for (int l = 1; l <= 3; ++l)
+        for (int b = 1; b <= n2; ++b)
+          for (int a = 1; a <= n1; ++a)
+            buf[e][count++] = src->a[ORC_QP][(size_t)n * src->ncell + layer_cell(src, L->from_face, l, a, b)];
+  }
+  for (int e = 0; e < nlinks && st == ORC_OK; ++e) {
+    const orc_link *L = &links[e];
+    orc *dst = blocks[L->block];
+    const orc *src = blocks[L->from_block];
+    int sn1, sn2, dn1, dn2;
+    face_extents(src, L->from_face, &sn1, &sn2);
+    face_extents(dst, L->face, &dn1, &dn2);
+    if ((long)sn1 * sn2 != (long)dn1 * dn2 || src->nv != dst->nv) { st = ORC_E_CONFIG; break; }
+    int lo1, hi1, d1, lo2, hi2, d2;
+    cell_range(L->lo1, L->hi1, &lo1, &hi1, &d1);
+    cell_range(L->lo2, L->hi2, &lo2, &hi2, &d2);
+    if ((hi1 - lo1) * d1 + 1 != dn1 || (hi2 - lo2) * d2 + 1 != dn2) { st = ORC_E_CONFIG; break; }
+    size_t count = 0;
+    for (int n = 0; n < dst->nv; ++n) {
+      double *q = dst->a[ORC_QP] + (size_t)n * dst->ncell;
+      for (int l = 1; l <= 3; ++l) {
+        if (L->dir_switch == 0) {
+          for (int b = lo2; d2 > 0 ? b <= hi2 : b >= hi2; b += d2)
+            for (int a = lo1; d1 > 0 ? a <= hi1 : a >= hi1; a += d1) q[ghost_cell(dst, L->face, l, a, b)] = buf[e][count++];
+        } else {
+          for (int a = lo1; d1 > 0 ? a <= hi1 : a >= hi1; a += d1)
+            for (int b = lo2; d2 > 0 ? b <= hi2 : b >= hi2; b += d2) q[ghost_cell(dst, L->face, l, a, b)] = buf[e][count++];
+        }
+      }
+    }
+  }
+  for (int e = 0; e < nlinks; ++e) free(buf[e]);
+  free(buf);
+  return st;
+}
+
+int orc_group_step(orc **blocks, int nb, const orc_link *links, int nl, int current_iter) {
+  int st;
+  for (int b = 0; b < nb; ++b) {
+    if (blocks[b]->c.time_accuracy != blocks[0]->c.time_accuracy) return ORC_E_CONFIG;
+    orc_temp_refresh(blocks[b]);
+  }
+  int ta = blocks[0]->c.time_accuracy;
+  if (ta == ORC_TA_NONE) {
+    if ((st = orc_exchange(blocks, nb, links, nl))) return st;
+    for (int b = 0; b < nb; ++b) {
+      if ((st = orc_residual(blocks[b], current_iter))) return st;
+      if ((st = orc_time_step(blocks[b]))) return st;
+      if ((st = update_with(blocks[b], 1., 1., 0, 0))) return st;
+    }
+    return ORC_OK;
+  }
+  for (int b = 0; b < nb; ++b) {
+    orc *s = blocks[b];
+    size_t nq = s->ncell * (size_t)s->nv;
+    for (size_t n = 0; n < s->nR * (size_t)s->nv; ++n) s->a[ORC_RSTORE][n] = 0.;
+    memcpy(s->a[ORC_USTORE], s->a[ORC_QP], nq * sizeof(double));
+  }
+  const double TF4[4] = {0.5, 0.5, 1.0, 1. / 6.}, SF4[4] = {1., 2., 2., 1.};
+  const double TF2[2] = {0.5, 0.5}, SF2[2] = {1., 1.};
+  int nst = ta == ORC_TA_RK4 ? 4 : ta == ORC_TA_RK2 ? 2 : 0;
+  if (!nst) return ORC_E_CONFIG;
+  for (int stage = 0; stage < nst; ++stage) {
+    if ((st = orc_exchange(blocks, nb, links, nl))) return st;
+    for (int b = 0; b < nb; ++b) {
+      orc *s = blocks[b];
+      if ((st = orc_residual(s, current_iter))) return st;
+      if (stage == 0 && (st = orc_time_step(s))) return st;
+      double TF = nst == 4 ? TF4[stage] : TF2[stage], SF = nst == 4 ? SF4[stage] : SF2[stage];
+      if ((st = update_with(s, TF, SF, stage == nst - 1, 1))) return st;
+    }
+  }
+  return ORC_OK;
 }

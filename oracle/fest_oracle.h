@@ -94,6 +94,25 @@ int orc_step(orc *s, int current_iter);
 /* Absolute residual norms Res_abs(0:n_var) (src/resnorm.f90:171-225). */
 void orc_resnorm(orc *s, double *out);
 
+/* One interface link of a multiblock grid: the three ghost layers of face
+ * `face` (1..6: imin..kmax) of block `block` receive the three interior
+ * layers next to face `from_face` of block `from_block`.  lo1/hi1, lo2/hi2
+ * are the map's node ranges for the receiving face's first and second
+ * tangential direction (j,k on i-faces; i,k on j-faces; i,j on k-faces),
+ * dir_switch the map's direction swap (src/mapping.f90:64-258). */
+typedef struct {
+  int32_t block, face, from_block, from_face;
+  int32_t lo1, hi1, lo2, hi2, dir_switch;
+} orc_link;
+/* The interface exchange of src/interface1.f90:96-705 over all links: every
+ * sender packs its layers in its own loop order, every receiver unpacks them
+ * in the order its map ranges give. */
+int orc_exchange(orc **blocks, int nblocks, const orc_link *links, int nlinks);
+/* One time step of get_next_solution on every block of a multiblock grid,
+ * the interface exchange opening every residual evaluation
+ * (src/update.f90:129-226, 534). */
+int orc_group_step(orc **blocks, int nblocks, const orc_link *links, int nlinks, int current_iter);
+
 #ifdef __cplusplus
 }
 #endif

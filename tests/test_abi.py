@@ -41,7 +41,7 @@ def test_struct_sizes_match_header():
     (dict(flux="roe"), None),
     (dict(bc_id=(-7, -9, -5, -5, -9, -9)), -3),
     (dict(bc_id=(-11, -9, -5, -5, -9, -9)), -3),
-    (dict(bc_id=(2, -9, -5, -5, -9, -9)), -3),
+    (dict(bc_id=(-12, -9, -5, -5, -9, -9)), -3),
     (dict(time_accuracy="RK2", bc_id=(-9, -9, -5, -5, -9, -9), imx=3), -3),
 ])
 def test_config_errors(change, code):
