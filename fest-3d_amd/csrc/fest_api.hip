@@ -301,6 +301,69 @@ int launch_update(fest_handle *h, double TF, double SF, int flags) {
   return 0;
 }
 
+// One explicit time step of get_next_solution (src/update.f90:129-226) as a
+// sequence of operations: the step start (Temp refresh, U_store), residual
+// evaluations, the stage-1 time step, stage updates and the TVD blends.
+enum { OP_BEGIN, OP_RESID, OP_DT, OP_UPDATE, OP_BLEND };
+struct StepOp {
+  int kind;
+  double a, c;  // TF, SF of an update; the two weights of a blend
+  int flags;    // update flags; for OP_BEGIN: store U_store; for OP_RESID: store the face sound speed
+};
+
+std::vector<StepOp> step_plan(int ta) {
+  std::vector<StepOp> p;
+  const int S = UPD_APPLY | UPD_STORE;
+  switch (ta) {
+    case FEST_TA_RK4:  // :176-187
+      p = {{OP_BEGIN, 0, 0, 1}, {OP_RESID, 0, 0, 1}, {OP_DT, 0, 0, 0}, {OP_UPDATE, 0.5, 1., S | UPD_FIRST},
+           {OP_RESID, 0, 0, 0}, {OP_UPDATE, 0.5, 2., S}, {OP_RESID, 0, 0, 0}, {OP_UPDATE, 1.0, 2., S},
+           {OP_RESID, 0, 0, 0}, {OP_UPDATE, 1. / 6., 1., S | UPD_LAST | UPD_WRITE_RES}};
+      break;
+    case FEST_TA_RK2:  // :188-195
+      p = {{OP_BEGIN, 0, 0, 1}, {OP_RESID, 0, 0, 1}, {OP_DT, 0, 0, 0}, {OP_UPDATE, 0.5, 1., S | UPD_FIRST},
+           {OP_RESID, 0, 0, 0}, {OP_UPDATE, 0.5, 1., S | UPD_LAST | UPD_WRITE_RES}};
+      break;
+    case FEST_TA_TVDRK3:  // :196-206
+      p = {{OP_BEGIN, 0, 0, 1}, {OP_RESID, 0, 0, 1}, {OP_DT, 0, 0, 0}, {OP_UPDATE, 1., 1., UPD_APPLY},
+           {OP_RESID, 0, 0, 0}, {OP_UPDATE, 1., 1., UPD_APPLY}, {OP_BLEND, 0.75, 0.25, 0},
+           {OP_RESID, 0, 0, 0}, {OP_UPDATE, 1., 1., UPD_APPLY | UPD_WRITE_RES}, {OP_BLEND, 1. / 3., 2. / 3., 0}};
+      break;
+    case FEST_TA_TVDRK2:  // :207-214
+      p = {{OP_BEGIN, 0, 0, 1}, {OP_RESID, 0, 0, 1}, {OP_DT, 0, 0, 0}, {OP_UPDATE, 1., 1., UPD_APPLY},
+           {OP_RESID, 0, 0, 0}, {OP_UPDATE, 1., 1., UPD_APPLY | UPD_WRITE_RES}, {OP_BLEND, 0.5, 0.5, 0}};
+      break;
+    default:  // "none", :171-174
+      p = {{OP_BEGIN, 0, 0, 0}, {OP_RESID, 0, 0, 1}, {OP_DT, 0, 0, 0},
+           {OP_UPDATE, 1., 1., UPD_APPLY | UPD_WRITE_RES}};
+      break;
+  }
+  return p;
+}
+
+// one operation on one block; residual evaluations of a group are opened by
+// the group's exchange (run_group_op)
+int run_op(fest_handle *h, const StepOp &op, int current_iter) {
+  Blk &b = h->b;
+  switch (op.kind) {
+    case OP_BEGIN: {
+      KgScope ts(h, KG_STEP_BEGIN);
+      hipLaunchKernelGGL(k_step_begin, dim3(cdiv(b.nc, 256)), dim3(256), 0, h->stream, b, op.flags);
+      LAUNCHCHK(h);
+      return 0;
+    }
+    case OP_RESID: return launch_residual_front(h, current_iter, op.flags);
+    case OP_DT: return launch_time_step(h);
+    case OP_UPDATE: return launch_update(h, op.a, op.c, op.flags);
+    default: {
+      KgScope ts(h, KG_UPDATE);
+      hipLaunchKernelGGL(k_blend, dim3(cdiv(b.nc * b.nv, 256)), dim3(256), 0, h->stream, b, op.a, op.c);
+      LAUNCHCHK(h);
+      return 0;
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -320,8 +383,9 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
   if (scheme->flux < 0 || scheme->flux > 5) return bad("unknown flux scheme");
   if (scheme->recon < 0 || scheme->recon > 3) return bad("unknown reconstruction");
   if (scheme->time_accuracy != FEST_TA_RK4 && scheme->time_accuracy != FEST_TA_RK2 &&
-      scheme->time_accuracy != FEST_TA_NONE)
-    return bad("time integration not supported (RK4, RK2, none)");
+      scheme->time_accuracy != FEST_TA_NONE && scheme->time_accuracy != FEST_TA_TVDRK2 &&
+      scheme->time_accuracy != FEST_TA_TVDRK3)
+    return bad("time integration not supported (RK4, RK2, TVDRK2, TVDRK3, none)");
   for (int f = 0; f < 6; ++f) {
     int id = bc->id[f];
     // ids >= 0 (interfaces) and -10 (cross-block periodic) take their ghost
@@ -572,36 +636,10 @@ int fest_time_step(fest_handle *h, double cfl) {
 int fest_get_next_solution(fest_handle *h, int current_iter, double cfl) {
   if (!h) return FEST_E_ARG;
   if (!h->have_grid || !h->have_state) return fail(h, FEST_E_ARG, "grid and state must be set first");
-  Blk &b = h->b;
-  b.CFLv = cfl;
+  h->b.CFLv = cfl;
   int rc;
-  const int ta = b.time_acc;
-  {
-    KgScope ts(h, KG_STEP_BEGIN);
-    hipLaunchKernelGGL(k_step_begin, dim3(cdiv(b.nc, 256)), dim3(256), 0, h->stream, b, ta != FEST_TA_NONE);
-  }
-  LAUNCHCHK(h);
-  if (ta == FEST_TA_NONE) {
-    if ((rc = launch_residual_front(h, current_iter, 1))) return rc;
-    if ((rc = launch_update(h, 0.0, 0.0, UPD_WRITE_RES))) return rc;
-    if ((rc = launch_time_step(h))) return rc;
-    if ((rc = launch_update(h, 1.0, 1.0, UPD_APPLY))) return rc;
-    return check_err(h);
-  }
-  // src/update.f90:176-195
-  const double TF4[4] = {0.5, 0.5, 1.0, 1. / 6.}, SF4[4] = {1., 2., 2., 1.};
-  const double TF2[2] = {0.5, 0.5}, SF2[2] = {1., 1.};
-  int nst = ta == FEST_TA_RK4 ? 4 : 2;
-  const double *TF = ta == FEST_TA_RK4 ? TF4 : TF2;
-  const double *SF = ta == FEST_TA_RK4 ? SF4 : SF2;
-  for (int s = 0; s < nst; ++s) {
-    if ((rc = launch_residual_front(h, current_iter, s == 0))) return rc;
-    if (s == 0 && (rc = launch_time_step(h))) return rc;
-    int flags = UPD_APPLY | UPD_STORE;
-    if (s == 0) flags |= UPD_FIRST;
-    if (s == nst - 1) flags |= UPD_LAST | UPD_WRITE_RES;
-    if ((rc = launch_update(h, TF[s], SF[s], flags))) return rc;
-  }
+  for (const StepOp &op : step_plan(h->b.time_acc))
+    if ((rc = run_op(h, op, current_iter))) return rc;
   return check_err(h);
 }
 
@@ -730,6 +768,8 @@ int fest_kernel_time(fest_handle *h, const char *group, double *total_ms, long *
 
 // ---------------------------------------------------------------- groups
 namespace {
+
+int run_group_op(fest_group *g, const StepOp &op, int current_iter);
 
 int gfail(fest_group *g, int code, const char *fmt, ...) {
   char buf[512];
@@ -870,42 +910,24 @@ int fest_group_get_next_solution(fest_group *g, int current_iter, double cfl) {
   if (g->broken) return FEST_E_ARG;
   for (fest_handle *h : g->blocks)
     if (!h->have_grid || !h->have_state) return gfail(g, FEST_E_ARG, "grid and state must be set on every block");
+  for (fest_handle *h : g->blocks) h->b.CFLv = cfl;
   int rc;
-  const int ta = g->blocks[0]->b.time_acc;
-  for (fest_handle *h : g->blocks) {
-    h->b.CFLv = cfl;
-    KgScope ts(h, KG_STEP_BEGIN);
-    hipLaunchKernelGGL(k_step_begin, dim3(cdiv(h->b.nc, 256)), dim3(256), 0, h->stream, h->b, ta != FEST_TA_NONE);
-  }
-  if (ta == FEST_TA_NONE) {
-    if ((rc = group_exchange(g))) return rc;
-    for (fest_handle *h : g->blocks) {
-      if ((rc = launch_residual_front(h, current_iter, 1))) return gfail(g, rc, "%s", h->err.c_str());
-      if ((rc = launch_update(h, 0.0, 0.0, UPD_WRITE_RES))) return gfail(g, rc, "%s", h->err.c_str());
-      if ((rc = launch_time_step(h))) return gfail(g, rc, "%s", h->err.c_str());
-      if ((rc = launch_update(h, 1.0, 1.0, UPD_APPLY))) return gfail(g, rc, "%s", h->err.c_str());
-    }
-    return group_check(g);
-  }
-  // src/update.f90:176-195, every stage opened by the exchange (:534)
-  const double TF4[4] = {0.5, 0.5, 1.0, 1. / 6.}, SF4[4] = {1., 2., 2., 1.};
-  const double TF2[2] = {0.5, 0.5}, SF2[2] = {1., 1.};
-  const int nst = ta == FEST_TA_RK4 ? 4 : 2;
-  const double *TF = ta == FEST_TA_RK4 ? TF4 : TF2;
-  const double *SF = ta == FEST_TA_RK4 ? SF4 : SF2;
-  for (int st = 0; st < nst; ++st) {
-    if ((rc = group_exchange(g))) return rc;
-    for (fest_handle *h : g->blocks) {
-      if ((rc = launch_residual_front(h, current_iter, st == 0))) return gfail(g, rc, "%s", h->err.c_str());
-      if (st == 0 && (rc = launch_time_step(h))) return gfail(g, rc, "%s", h->err.c_str());
-      int flags = UPD_APPLY | UPD_STORE;
-      if (st == 0) flags |= UPD_FIRST;
-      if (st == nst - 1) flags |= UPD_LAST | UPD_WRITE_RES;
-      if ((rc = launch_update(h, TF[st], SF[st], flags))) return gfail(g, rc, "%s", h->err.c_str());
-    }
-  }
+  for (const StepOp &op : step_plan(g->blocks[0]->b.time_acc))
+    if ((rc = run_group_op(g, op, current_iter))) return rc;
   return group_check(g);
 }
+
+namespace {
+// an operation on every block of a group; each residual evaluation opens
+// with the interface exchange (src/update.f90:534)
+int run_group_op(fest_group *g, const StepOp &op, int current_iter) {
+  int rc;
+  if (op.kind == OP_RESID && (rc = group_exchange(g))) return rc;
+  for (fest_handle *h : g->blocks)
+    if ((rc = run_op(h, op, current_iter))) return gfail(g, rc, "%s", h->err.c_str());
+  return 0;
+}
+}  // namespace
 
 int fest_sync(fest_handle *h) {
   if (!h) return FEST_E_ARG;

@@ -289,7 +289,97 @@ __device__ __forceinline__ void inviscid_flux(const Blk &b, const double (&qL)[N
   for (int l = 5; l < nv; ++l) out[l] = (Fp0 * qL[l]) * A + (Fm0 * qR[l]) * A;
 }
 
-// ---- the face kernel
+// ---- viscous face terms (src/viscous.f90:144-447): v[l], l = 1..NV-1, is
+// what the reference subtracts from the inviscid flux of a face between
+// cells cm and c (gradient indices gL, gR) with unit normal n and area A;
+// the laminar and SST parts of one component are summed before the
+// subtraction
+template <int NV>
+__device__ __forceinline__ void visc_face(const Blk &b, long c, long cm, long gL, long gR, double nx, double ny,
+                                          double nz, double A, double (&v)[NV]) {
+  const double *qp = b.qp;
+  const long nc = b.nc;
+  double delx = b.cx[c] - b.cx[cm], dely = b.cy[c] - b.cy[cm], delz = b.cz[c] - b.cz[cm];
+  double d2 = delx * delx + dely * dely + delz * delz;
+  double id2 = 1.0 / d2;  // normal_comp*del/d_LR = (del_phi - grad.del)*del/d_LR^2
+  const double *gx = b.g[0], *gy = b.g[1], *gz = b.g[2];
+  const long ngr = b.ngr;
+  double gr[4][3];
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    gr[l][0] = 0.5 * (gx[l * ngr + gL] + gx[l * ngr + gR]);
+    gr[l][1] = 0.5 * (gy[l * ngr + gL] + gy[l * ngr + gR]);
+    gr[l][2] = 0.5 * (gz[l * ngr + gL] + gz[l * ngr + gR]);
+  }
+  double T_LE = qp[4 * nc + cm] / (qp[cm] * b.R);
+  double T_RE = qp[4 * nc + c] / (qp[c] * b.R);
+  double del[4] = {qp[nc + c] - qp[nc + cm], qp[2 * nc + c] - qp[2 * nc + cm], qp[3 * nc + c] - qp[3 * nc + cm],
+                   T_RE - T_LE};
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    double ncmp = (del[l] - (gr[l][0] * delx + gr[l][1] * dely + gr[l][2] * delz)) * id2;
+    gr[l][0] = gr[l][0] + ncmp * delx;
+    gr[l][1] = gr[l][1] + ncmp * dely;
+    gr[l][2] = gr[l][2] + ncmp * delz;
+  }
+  double mu_f = 0.5 * (b.mu[cm] + b.mu[c]);
+  double mut_f = (NV == 7) ? 0.5 * (b.mut[cm] + b.mut[c]) : 0.0;
+  double tm = mu_f + mut_f;
+  double dudx = gr[0][0], dudy = gr[0][1], dudz = gr[0][2];
+  double dvdx = gr[1][0], dvdy = gr[1][1], dvdz = gr[1][2];
+  double dwdx = gr[2][0], dwdy = gr[2][1], dwdz = gr[2][2];
+  double div3 = (dudx + dvdy + dwdz) / 3.;
+  double Txx = 2. * tm * (dudx - div3);
+  double Tyy = 2. * tm * (dvdy - div3);
+  double Tzz = 2. * tm * (dwdz - div3);
+  double Txy = tm * (dvdx + dudy);
+  double Txz = tm * (dwdx + dudz);
+  double Tyz = tm * (dwdy + dvdz);
+  double K_heat = (mu_f / b.Pr + mut_f / b.tPr) * b.gm * b.R / (b.gm - 1);
+  double Qx = K_heat * gr[3][0], Qy = K_heat * gr[3][1], Qz = K_heat * gr[3][2];
+  double uf = 0.5 * (qp[nc + cm] + qp[nc + c]);
+  double vf = 0.5 * (qp[2 * nc + cm] + qp[2 * nc + c]);
+  double wf = 0.5 * (qp[3 * nc + cm] + qp[3 * nc + c]);
+  v[0] = 0.0;
+  v[1] = (Txx * nx + Txy * ny + Txz * nz) * A;
+  v[2] = (Txy * nx + Tyy * ny + Tyz * nz) * A;
+  v[3] = (Txz * nx + Tyz * ny + Tzz * nz) * A;
+  v[4] = A * (((Txx * uf + Txy * vf + Txz * wf + Qx) * nx) + ((Txy * uf + Tyy * vf + Tyz * wf + Qy) * ny) +
+              ((Txz * uf + Tyz * vf + Tzz * wf + Qz) * nz));
+  if constexpr (NV == 7) {
+    // SST k / omega diffusion and the -2/3 rho k stress, src/viscous.f90:328-447
+    double gt[2][3];
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+      gt[l][0] = 0.5 * (gx[(4 + l) * ngr + gL] + gx[(4 + l) * ngr + gR]);
+      gt[l][1] = 0.5 * (gy[(4 + l) * ngr + gL] + gy[(4 + l) * ngr + gR]);
+      gt[l][2] = 0.5 * (gz[(4 + l) * ngr + gL] + gz[(4 + l) * ngr + gR]);
+    }
+    double dt2[2] = {qp[5 * nc + c] - qp[5 * nc + cm], qp[6 * nc + c] - qp[6 * nc + cm]};
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+      double ncmp = (dt2[l] - (gt[l][0] * delx + gt[l][1] * dely + gt[l][2] * delz)) * id2;
+      gt[l][0] = gt[l][0] + ncmp * delx;
+      gt[l][1] = gt[l][1] + ncmp * dely;
+      gt[l][2] = gt[l][2] + ncmp * delz;
+    }
+    double F1 = 0.5 * (b.f1[cm] + b.f1[c]);
+    double sigma_kf = 0.85 * F1 + 1.0 * (1.0 - F1);
+    double sigma_wf = 0.5 * F1 + 0.856 * (1.0 - F1);
+    double rhof = 0.5 * (qp[cm] + qp[c]);
+    double tkf = 0.5 * (qp[5 * nc + cm] + qp[5 * nc + c]);
+    double Tkk = -(2.0 * rhof * tkf / 3.0);
+    v[1] = v[1] + Tkk * nx * A;
+    v[2] = v[2] + Tkk * ny * A;
+    v[3] = v[3] + Tkk * nz * A;
+    double kfl = A * ((mu_f + sigma_kf * mut_f) * (gt[0][0] * nx + gt[0][1] * ny + gt[0][2] * nz));
+    v[4] = v[4] + kfl;
+    v[5] = kfl;
+    v[6] = A * ((mu_f + sigma_wf * mut_f) * (gt[1][0] * nx + gt[1][1] * ny + gt[1][2] * nz));
+  }
+}
+
+// ---- the face kernel// ---- the face kernel
 // D: sweep direction.  Faces: n = 1..N along D, 1..N-1 across.  NV: 5
 // (laminar) or 7 (SST) variables; RECON: the reconstruction.  Compile-time
 // counts keep the per-variable arrays in registers.
@@ -328,89 +418,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     const double zero = (n == 1) ? (double)b.mz[D][0] : (n == N) ? (double)b.mz[D][1] : 1.0;
     inviscid_flux(b, qL, qR, nx, ny, nz, A, zero, out);
   }
-  const bool viscous = b.mu_ref != 0.0;
-  if (viscous) {
-    // laminar viscous flux, src/viscous.f90:144-325
-    const double *qp = b.qp;
-    const long nc = b.nc;
-    double delx = b.cx[c] - b.cx[cm], dely = b.cy[c] - b.cy[cm], delz = b.cz[c] - b.cz[cm];
-    double d2 = delx * delx + dely * dely + delz * delz;
-    double id2 = 1.0 / d2;  // normal_comp*del/d_LR = (del_phi - grad.del)*del/d_LR^2
-    const long gL = gri(b, i - ii, j - jj, k - kk), gR = gri(b, i, j, k);
-    const double *gx = b.g[0], *gy = b.g[1], *gz = b.g[2];
-    const long ngr = b.ngr;
-    double gr[4][3];
+  if (b.mu_ref != 0.0) {
+    double v[NV];
+    visc_face<NV>(b, c, cm, gri(b, i - ii, j - jj, k - kk), gri(b, i, j, k), nx, ny, nz, A, v);
 #pragma unroll
-    for (int l = 0; l < 4; ++l) {
-      gr[l][0] = 0.5 * (gx[l * ngr + gL] + gx[l * ngr + gR]);
-      gr[l][1] = 0.5 * (gy[l * ngr + gL] + gy[l * ngr + gR]);
-      gr[l][2] = 0.5 * (gz[l * ngr + gL] + gz[l * ngr + gR]);
-    }
-    double T_LE = qp[4 * nc + cm] / (qp[cm] * b.R);
-    double T_RE = qp[4 * nc + c] / (qp[c] * b.R);
-    double del[4] = {qp[nc + c] - qp[nc + cm], qp[2 * nc + c] - qp[2 * nc + cm], qp[3 * nc + c] - qp[3 * nc + cm],
-                     T_RE - T_LE};
-#pragma unroll
-    for (int l = 0; l < 4; ++l) {
-      double ncmp = (del[l] - (gr[l][0] * delx + gr[l][1] * dely + gr[l][2] * delz)) * id2;
-      gr[l][0] = gr[l][0] + ncmp * delx;
-      gr[l][1] = gr[l][1] + ncmp * dely;
-      gr[l][2] = gr[l][2] + ncmp * delz;
-    }
-    double mu_f = 0.5 * (b.mu[cm] + b.mu[c]);
-    double mut_f = (NV == 7) ? 0.5 * (b.mut[cm] + b.mut[c]) : 0.0;
-    double tm = mu_f + mut_f;
-    double dudx = gr[0][0], dudy = gr[0][1], dudz = gr[0][2];
-    double dvdx = gr[1][0], dvdy = gr[1][1], dvdz = gr[1][2];
-    double dwdx = gr[2][0], dwdy = gr[2][1], dwdz = gr[2][2];
-    double div3 = (dudx + dvdy + dwdz) / 3.;
-    double Txx = 2. * tm * (dudx - div3);
-    double Tyy = 2. * tm * (dvdy - div3);
-    double Tzz = 2. * tm * (dwdz - div3);
-    double Txy = tm * (dvdx + dudy);
-    double Txz = tm * (dwdx + dudz);
-    double Tyz = tm * (dwdy + dvdz);
-    double K_heat = (mu_f / b.Pr + mut_f / b.tPr) * b.gm * b.R / (b.gm - 1);
-    double Qx = K_heat * gr[3][0], Qy = K_heat * gr[3][1], Qz = K_heat * gr[3][2];
-    double uf = 0.5 * (qp[nc + cm] + qp[nc + c]);
-    double vf = 0.5 * (qp[2 * nc + cm] + qp[2 * nc + c]);
-    double wf = 0.5 * (qp[3 * nc + cm] + qp[3 * nc + c]);
-    out[1] = out[1] - ((Txx * nx + Txy * ny + Txz * nz) * A);
-    out[2] = out[2] - ((Txy * nx + Tyy * ny + Tyz * nz) * A);
-    out[3] = out[3] - ((Txz * nx + Tyz * ny + Tzz * nz) * A);
-    out[4] = out[4] - (A * (((Txx * uf + Txy * vf + Txz * wf + Qx) * nx) + ((Txy * uf + Tyy * vf + Tyz * wf + Qy) * ny) +
-                            ((Txz * uf + Tyz * vf + Tzz * wf + Qz) * nz)));
-    if constexpr (NV == 7) {
-      // SST k / omega diffusion and the -2/3 rho k stress, src/viscous.f90:328-447
-      double gt[2][3];
-#pragma unroll
-      for (int l = 0; l < 2; ++l) {
-        gt[l][0] = 0.5 * (gx[(4 + l) * ngr + gL] + gx[(4 + l) * ngr + gR]);
-        gt[l][1] = 0.5 * (gy[(4 + l) * ngr + gL] + gy[(4 + l) * ngr + gR]);
-        gt[l][2] = 0.5 * (gz[(4 + l) * ngr + gL] + gz[(4 + l) * ngr + gR]);
-      }
-      double dt2[2] = {qp[5 * nc + c] - qp[5 * nc + cm], qp[6 * nc + c] - qp[6 * nc + cm]};
-#pragma unroll
-      for (int l = 0; l < 2; ++l) {
-        double ncmp = (dt2[l] - (gt[l][0] * delx + gt[l][1] * dely + gt[l][2] * delz)) * id2;
-        gt[l][0] = gt[l][0] + ncmp * delx;
-        gt[l][1] = gt[l][1] + ncmp * dely;
-        gt[l][2] = gt[l][2] + ncmp * delz;
-      }
-      double F1 = 0.5 * (b.f1[cm] + b.f1[c]);
-      double sigma_kf = 0.85 * F1 + 1.0 * (1.0 - F1);
-      double sigma_wf = 0.5 * F1 + 0.856 * (1.0 - F1);
-      double rhof = 0.5 * (qp[cm] + qp[c]);
-      double tkf = 0.5 * (qp[5 * nc + cm] + qp[5 * nc + c]);
-      double Tkk = -(2.0 * rhof * tkf / 3.0);
-      out[1] = out[1] - (Tkk * nx * A);
-      out[2] = out[2] - (Tkk * ny * A);
-      out[3] = out[3] - (Tkk * nz * A);
-      double kfl = A * ((mu_f + sigma_kf * mut_f) * (gt[0][0] * nx + gt[0][1] * ny + gt[0][2] * nz));
-      out[4] = out[4] - kfl;
-      out[5] = out[5] - kfl;
-      out[6] = out[6] - (A * ((mu_f + sigma_wf * mut_f) * (gt[1][0] * nx + gt[1][1] * ny + gt[1][2] * nz)));
-    }
+    for (int l = 1; l < nv; ++l) out[l] = out[l] - v[l];
   }
 
   bool bad = false;

@@ -23,6 +23,14 @@ __global__ void k_step_begin(Blk b, int store_u) {
     for (int l = 0; l < b.nv; ++l) b.ust[(long)l * b.nc + t] = b.qp[(long)l * b.nc + t];
 }
 
+// q = a U_store + c q over the whole padded primitive array, the TVD-RK
+// blends of src/update.f90:204,206,213
+__global__ void k_blend(Blk b, double a, double c) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= b.nc * b.nv) return;
+  b.qp[t] = a * b.ust[t] + c * b.qp[t];
+}
+
 __device__ __forceinline__ double sspd(const Blk &b, int d, long o) { return b.csnd[d][o]; }
 
 // local time step of one cell (src/time.f90:122-246, 366-531)

@@ -23,7 +23,8 @@ module fest_gpu
   integer(c_int32_t), parameter, public :: FEST_RECON_NONE = 0, FEST_RECON_MUSCL = 1, &
       FEST_RECON_PPM = 2, FEST_RECON_WENO = 3
   integer(c_int32_t), parameter, public :: FEST_TURB_NONE = 0, FEST_TURB_SST = 1
-  integer(c_int32_t), parameter, public :: FEST_TA_NONE = 0, FEST_TA_RK2 = 2, FEST_TA_RK4 = 4
+  integer(c_int32_t), parameter, public :: FEST_TA_NONE = 0, FEST_TA_RK2 = 2, FEST_TA_RK4 = 4, &
+      FEST_TA_TVDRK2 = 12, FEST_TA_TVDRK3 = 13
 
   integer(c_int32_t), parameter, public :: FEST_DL_QP = 1, FEST_DL_TEMP = 2, &
       FEST_DL_RESIDUE = 4, FEST_DL_F = 8, FEST_DL_G = 16, FEST_DL_H = 32, FEST_DL_DT = 64, &
@@ -57,9 +58,20 @@ module fest_gpu
     real(c_double) :: fixed_pressure(6), fixed_tk(6), fixed_tw(6), fixed_wall_temperature(6)
   end type fest_bc
 
+  !< one interface link of a multiblock grid (the reference's map line,
+  !< src/mapping.f90:64-258): the ghost layers of `face` of `block` take the
+  !< interior layers next to `from_face` of `from_block`, walked through the
+  !< receiving side's node ranges lo1..hi1, lo2..hi2 (and dir_switch)
+  type, bind(c), public :: fest_link
+    integer(c_int32_t) :: block, face, from_block, from_face
+    integer(c_int32_t) :: lo1, hi1, lo2, hi2
+    integer(c_int32_t) :: dir_switch
+  end type fest_link
+
   public :: fest_create, fest_destroy, fest_last_error, fest_upload_geometry, fest_set_grid, &
             fest_upload_state, fest_get_next_solution, fest_residual, fest_time_step, &
-            fest_download, fest_resnorm, fest_sync, fest_check
+            fest_download, fest_resnorm, fest_sync, fest_check, &
+            fest_group_create, fest_group_destroy, fest_group_exchange, fest_group_get_next_solution
 
   interface
     integer(c_int) function fest_create(h, dims, scheme, flow, bc, precision) bind(c, name='fest_create')
@@ -145,6 +157,34 @@ module fest_gpu
       import :: c_int, c_ptr
       type(c_ptr), value :: h
     end function fest_sync
+
+    !< blocks: the handles of the group's blocks (type(c_ptr) array)
+    integer(c_int) function fest_group_create(g, blocks, nblocks, links, nlinks) bind(c, name='fest_group_create')
+      import :: c_int, c_ptr, fest_link
+      type(c_ptr), intent(out) :: g
+      type(c_ptr), intent(in) :: blocks(*)
+      integer(c_int), value :: nblocks
+      type(fest_link), intent(in) :: links(*)
+      integer(c_int), value :: nlinks
+    end function fest_group_create
+
+    subroutine fest_group_destroy(g) bind(c, name='fest_group_destroy')
+      import :: c_ptr
+      type(c_ptr), value :: g
+    end subroutine fest_group_destroy
+
+    integer(c_int) function fest_group_exchange(g) bind(c, name='fest_group_exchange')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: g
+    end function fest_group_exchange
+
+    integer(c_int) function fest_group_get_next_solution(g, current_iter, cfl) &
+        bind(c, name='fest_group_get_next_solution')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: g
+      integer(c_int), value :: current_iter
+      real(c_double), value :: cfl
+    end function fest_group_get_next_solution
   end interface
 
 contains

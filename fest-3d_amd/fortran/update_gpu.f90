@@ -13,7 +13,8 @@
 !< qp, runs one whole time step on the device (Temp refresh, all RK stages,
 !< the stage-1 time step) and downloads qp, Temp, residue, delta_t and the
 !< face fluxes F, G, H that find_resnorm and the writers read next.
-!< Time integrators other than none / RK2 / RK4 stay on the host path.
+!< Time integrators other than none / RK2 / RK4 / TVDRK2 / TVDRK3 (the
+!< implicit LU-SGS ones) stay on the host path.
 module update
   use iso_c_binding
   use vartypes
@@ -36,7 +37,7 @@ contains
     type(flowtype), intent(in) :: flow
     type(extent), intent(in) :: dims
     select case (trim(scheme%time_step_accuracy))
-      case ("none", "RK2", "RK4")
+      case ("none", "RK2", "RK4", "TVDRK2", "TVDRK3")
         continue
       case default
         print *, "update: time_step_accuracy '", trim(scheme%time_step_accuracy), &
@@ -97,6 +98,8 @@ contains
     select case (trim(scheme%time_step_accuracy))
       case ("RK4"); s%time_accuracy = FEST_TA_RK4
       case ("RK2"); s%time_accuracy = FEST_TA_RK2
+      case ("TVDRK2"); s%time_accuracy = FEST_TA_TVDRK2
+      case ("TVDRK3"); s%time_accuracy = FEST_TA_TVDRK3
       case default; s%time_accuracy = FEST_TA_NONE
     end select
     s%accur = scheme%accur

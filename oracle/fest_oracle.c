@@ -1604,6 +1604,26 @@ int orc_step(orc *s, int current_iter) {
     return update_with(s, 1., 1., 0, 0);
   }
   size_t nq = s->ncell * (size_t)s->nv;
+  if (ta == ORC_TA_TVDRK2 || ta == ORC_TA_TVDRK3) {
+    /* src/update.f90:196-214: plain stage updates, then blends of the
+     * primitive state with U_store over the whole padded array */
+    double *q = s->a[ORC_QP], *u = s->a[ORC_USTORE];
+    memcpy(u, q, nq * sizeof(double));
+    if ((st = orc_residual(s, current_iter))) return st;
+    if ((st = orc_time_step(s))) return st;
+    if ((st = update_with(s, 1., 1., 0, 0))) return st;
+    if ((st = orc_residual(s, current_iter))) return st;
+    if ((st = update_with(s, 1., 1., 0, 0))) return st;
+    if (ta == ORC_TA_TVDRK2) {
+      for (size_t n = 0; n < nq; ++n) q[n] = 0.5 * u[n] + 0.5 * q[n];
+      return ORC_OK;
+    }
+    for (size_t n = 0; n < nq; ++n) q[n] = 0.75 * u[n] + 0.25 * q[n];
+    if ((st = orc_residual(s, current_iter))) return st;
+    if ((st = update_with(s, 1., 1., 0, 0))) return st;
+    for (size_t n = 0; n < nq; ++n) q[n] = (1. / 3.) * u[n] + (2. / 3.) * q[n];
+    return ORC_OK;
+  }
   for (size_t n = 0; n < s->nR * (size_t)s->nv; ++n) s->a[ORC_RSTORE][n] = 0.;
   memcpy(s->a[ORC_USTORE], s->a[ORC_QP], nq * sizeof(double));
   if (ta == ORC_TA_RK4) {

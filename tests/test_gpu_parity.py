@@ -145,6 +145,28 @@ def test_rk4_steps_sst_bump():
     assert relerr(out["dt"], o.arr("dt")) <= TOL
 
 
+@pytest.mark.parametrize("rule", ["none", "RK2", "TVDRK2", "TVDRK3"])
+@pytest.mark.parametrize("turb", ["none", "sst"])
+def test_time_integrators(rule, turb):
+    """Every explicit integrator of get_next_solution (src/update.f90:162-214)
+    over two steps."""
+    if turb == "sst":
+        case, nodes, q = C.sst_bump(20, 12, 6)
+        wall = True
+    else:
+        case, nodes, q = C.laminar_box(16, 12, 6)
+        wall = False
+    case = dataclasses.replace(case, time_accuracy=rule)
+    say("integrator", rule, turb)
+    g, o = pair(case, nodes, q, wall=wall)
+    for it in range(1, 3):
+        assert o.step(it) == 0
+        g.get_next_solution(it)
+    out = g.download("qp", "residue")
+    assert relerr(out["qp"], o.arr("qp")) <= TOL
+    assert relerr(out["residue"], o.arr("res")) <= 1e-10
+
+
 def test_laminar_box_64x64x4_100_steps():
     """The reference's CPU-runnable case: 100 RK4 steps (400 stages).  The
     residual kernels fuse multiply-adds, so each stage differs from the

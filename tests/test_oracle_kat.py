@@ -464,6 +464,78 @@ def test_step_composition(rule, turb, time_local):
     np.testing.assert_allclose(o2.arr("qp")[inner], expect, rtol=1e-12, atol=0)
 
 
+@pytest.mark.parametrize("rule", ["TVDRK2", "TVDRK3"])
+@pytest.mark.parametrize("turb", ["none", "sst"])
+def test_tvd_step_composition(rule, turb):
+    """TVD-RK steps (src/update.f90:196-214): plain stage updates
+    U = U(q) - R dt/V from the current state (the stage-1 time step, the SST
+    point-implicit scaling), and the blends q = 1/2 U0 + 1/2 q (TVDRK2) or
+    q = 3/4 U0 + 1/4 q and q = 1/3 U0 + 2/3 q (TVDRK3) of the primitive state
+    with the stored base."""
+    if turb == "sst":
+        case, nodes, q0 = C.sst_bump(14, 10, 6)
+        wall = True
+    else:
+        case, _, q0 = C.laminar_box(10, 8, 6)
+        nodes = C.cartesian_nodes(case, warp=0.03)
+        wall = False
+    case = dataclasses.replace(case, time_accuracy=rule, time_local=1)
+    gm = case.gm
+    inner = (slice(None), slice(3, -3), slice(3, -3), slice(3, -3))
+    o = Oracle(case)
+    o.set_nodes(nodes)
+    if wall:
+        o.wall_dist()
+    o.set_state(q0)
+    o.temp_refresh()
+    temp = o.arr("temp").copy()
+    vol = o.arr("vol")[3:-3, 3:-3, 3:-3].copy()
+    U0 = o.arr("qp").copy()
+    dt = None
+
+    def stage():
+        nonlocal dt
+        assert o.residual(1) == 0
+        if dt is None:
+            o.time_step()
+            dt = o.arr("dt").copy()
+        q = o.arr("qp").copy()
+        R = o.arr("res").copy()
+        if turb == "sst":
+            F1 = o.arr("f1")[3:-3, 3:-3, 3:-3]
+            w = q[6][3:-3, 3:-3, 3:-3]
+            beta = 0.075 * F1 + (1.0 - F1) * 0.0828
+            R[5] = R[5] / (1 + beta * w * dt)
+            R[6] = R[6] / (1 + 2 * beta * w * dt)
+        new = _to_prim(_to_cons(q[inner], gm) - R * (1.0 * dt / vol), gm)
+        if turb == "sst":
+            assert (new[5] >= 0).all() and (new[6] >= 0).all()
+        q[inner] = new
+        o.set_state(q)
+        o.arr("temp")[...] = temp
+
+    def blend(a, b):
+        o.set_state(a * U0 + b * o.arr("qp").copy())
+        o.arr("temp")[...] = temp
+
+    stage()
+    stage()
+    if rule == "TVDRK2":
+        blend(0.5, 0.5)
+    else:
+        blend(0.75, 0.25)
+        stage()
+        blend(1.0 / 3.0, 2.0 / 3.0)
+    expect = o.arr("qp").copy()[inner]
+    o2 = Oracle(case)
+    o2.set_nodes(nodes)
+    if wall:
+        o2.wall_dist()
+    o2.set_state(q0)
+    assert o2.step(1) == 0
+    np.testing.assert_allclose(o2.arr("qp")[inner], expect, rtol=1e-12, atol=0)
+
+
 def test_resnorm_closed_form():
     case, nodes, q = C.laminar_box(6, 5, 4)
     o = Oracle(case)
