@@ -46,6 +46,7 @@ struct Blk {
   int mz[3][2];               // make_*_flux_zero at the first and last face
   // flow
   double gm, R, mu_ref, T_ref, S_ref, Pr, tPr, MInf, vel_mag;
+  double iPr, itPr, cpR;  // 1/Pr, 1/Pr_t, gamma R/(gamma-1): the heat-flux factors
   double rho_inf, u_inf, v_inf, w_inf, p_inf, tk_inf, tw_inf;
   double c1, c2, c3;
   double fix[8][6];           // rho,u,v,w,p,tk,tw,wall temperature

@@ -439,6 +439,7 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
   b.mu_suth = flow->mu_sutherland;
   b.gm = flow->gm; b.R = flow->R_gas; b.mu_ref = flow->mu_ref; b.T_ref = flow->T_ref;
   b.S_ref = flow->Sutherland_temp; b.Pr = flow->Pr; b.tPr = flow->tPr; b.MInf = flow->MInf;
+  b.iPr = 1.0 / flow->Pr; b.itPr = 1.0 / flow->tPr; b.cpR = flow->gm * flow->R_gas / (flow->gm - 1.0);
   b.vel_mag = flow->vel_mag;
   b.rho_inf = flow->density_inf; b.u_inf = flow->x_speed_inf; b.v_inf = flow->y_speed_inf;
   b.w_inf = flow->z_speed_inf; b.p_inf = flow->pressure_inf; b.tk_inf = flow->tk_inf; b.tw_inf = flow->tw_inf;

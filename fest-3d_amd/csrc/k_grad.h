@@ -128,7 +128,8 @@ __global__ void __launch_bounds__(256) k_grad(Blk b) {
   double mu;
   if (b.mu_suth) {
     double T = b.qp[4 * b.nc + c] / (rho * b.R);
-    mu = b.mu_ref * (pow(T / b.T_ref, 1.5)) * ((b.T_ref + b.S_ref) / (T + b.S_ref));
+    const double x = T / b.T_ref;  // x**1.5 as x*sqrt(x): within an ulp of pow, without its log/exp
+    mu = b.mu_ref * (x * sqrt(x)) * ((b.T_ref + b.S_ref) / (T + b.S_ref));
     b.mu[c] = mu;
     if (isnan(mu)) atomicOr(b.err, ERR_NAN_MU);
   } else {

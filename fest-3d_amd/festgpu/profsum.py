@@ -3,6 +3,10 @@
   python -m festgpu.profsum stats DIR
       kernel table from the *_kernel_stats.csv under DIR (--kernel-trace
       --stats --output-format csv): calls, average and total duration.
+  python -m festgpu.profsum sq DIR [DIR ...]
+      per-kernel VALU and wait figures from --pmc runs with SQ_WAVES,
+      SQ_INSTS_VALU, SQ_WAVE_CYCLES, SQ_WAIT_INST_ANY, SQ_INSTS_VMEM_RD,
+      GRBM_GUI_ACTIVE (and SQC_ICACHE_REQ/HITS).
   python -m festgpu.profsum traffic --fetch DIR --write DIR
                  --probe-fetch DIR --probe-write DIR --case NAME
       HBM bytes per launch of each kernel from two counter runs of the
@@ -27,7 +31,7 @@ from . import roofline as R
 PROBE_BYTES = 512 * 2 ** 20  # copy_probe: read and written per launch
 
 GROUPS = [  # (substring of the demangled kernel name, kernel group)
-    ("k_flux<0>", "flux_i"), ("k_flux<1>", "flux_j"), ("k_flux<2>", "flux_k"),
+    ("k_flux<0", "flux_i"), ("k_flux<1", "flux_j"), ("k_flux<2", "flux_k"),
     ("k_grad<", "grad"), ("k_grad_bc", "grad_bc"), ("k_update", "update"),
     ("k_bc_face", "bc_face"), ("k_bc_edges", "bc_edges"), ("k_step_begin", "step_begin"),
     ("k_dt_min", "time_step"), ("k_dt_fill", "time_step"), ("k_dt", "time_step"),
@@ -110,11 +114,47 @@ def traffic(fetch_dir, write_dir, probe_fetch, probe_write, case_name):
     return "\n".join(lines + extra)
 
 
+def sq(d):
+    """Per kernel: waves, VALU instructions per wave, the fraction of SIMD
+    cycles issuing VALU (SQ_INSTS_VALU x 4 cycles over 1024 SIMDs x the
+    kernel's cycles; GRBM_GUI_ACTIVE sums the 8 XCDs), vector-memory reads
+    per wave, the fraction of wave cycles waiting on dependencies, and the
+    instruction-cache hit rate when those counters were collected."""
+    acc = defaultdict(lambda: defaultdict(list))
+    for path in _csvs(d, "counter_collection.csv"):
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                acc[short(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    lines = ["| kernel | group | waves | VALU/wave | VALU busy | VMEM rd/wave | dep. wait | I-cache hit |",
+             "|---|---|---|---|---|---|---|---|"]
+    for k in sorted(acc):
+        grp = group_of(k)
+        if grp in ("setup", "-", "copy_probe", "resnorm"):
+            continue
+        m = {c: sum(v) / len(v) for c, v in acc[k].items()}
+        waves = m.get("SQ_WAVES", 0.0)
+
+        def fmt(x, f):
+            return f.format(x) if x is not None else "-"
+        valu = m["SQ_INSTS_VALU"] / waves if waves and "SQ_INSTS_VALU" in m else None
+        busy = (m["SQ_INSTS_VALU"] * 4.0 / (1024.0 * m["GRBM_GUI_ACTIVE"] / 8.0)
+                if "SQ_INSTS_VALU" in m and m.get("GRBM_GUI_ACTIVE") else None)
+        vmem = m["SQ_INSTS_VMEM_RD"] / waves if waves and "SQ_INSTS_VMEM_RD" in m else None
+        wait = (m["SQ_WAIT_INST_ANY"] / m["SQ_WAVE_CYCLES"]
+                if "SQ_WAIT_INST_ANY" in m and m.get("SQ_WAVE_CYCLES") else None)
+        ic = (m["SQC_ICACHE_HITS"] / m["SQC_ICACHE_REQ"] if m.get("SQC_ICACHE_REQ") else None)
+        lines.append(f"| {k} | {grp} | {waves:.0f} | {fmt(valu, '{:.0f}')} | {fmt(busy, '{:.2f}')} | "
+                     f"{fmt(vmem, '{:.0f}')} | {fmt(wait, '{:.2f}')} | {fmt(ic, '{:.3f}')} |")
+    return "\n".join(lines)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     sub = ap.add_subparsers(dest="cmd", required=True)
     s = sub.add_parser("stats")
     s.add_argument("dir")
+    q = sub.add_parser("sq")
+    q.add_argument("dir", nargs="+")
     t = sub.add_parser("traffic")
     t.add_argument("--fetch", required=True)
     t.add_argument("--write", required=True)
@@ -124,6 +164,8 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.cmd == "stats":
         print(stats(a.dir))
+    elif a.cmd == "sq":
+        print("\n\n".join(sq(d) for d in a.dir))
     else:
         print(traffic(a.fetch, a.write, a.probe_fetch, a.probe_write, a.case))
 
