@@ -20,15 +20,17 @@
 #include "k_geom.h"
 #include "k_grad.h"
 #include "k_iface.h"
+#include "k_lusgs.h"
 #include "k_update.h"
 
 using namespace fest;
 
 // kernel groups timed by fest_timing (one or a few launches each)
 enum { KG_STEP_BEGIN = 0, KG_BC_FACE, KG_BC_EDGES, KG_GRAD, KG_GRAD_BC, KG_FLUX_I, KG_FLUX_J, KG_FLUX_K,
-       KG_TIME_STEP, KG_UPDATE, KG_COUNT };
+       KG_TIME_STEP, KG_UPDATE, KG_LU_DIAG, KG_LU_SWEEP, KG_LU_APPLY, KG_COUNT };
 static const char *const kg_names[KG_COUNT] = {"step_begin", "bc_face", "bc_edges", "grad", "grad_bc",
-                                               "flux_i", "flux_j", "flux_k", "time_step", "update"};
+                                               "flux_i", "flux_j", "flux_k", "time_step", "update",
+                                               "lusgs_diag", "lusgs_sweep", "lusgs_apply"};
 
 struct KgTimer {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
@@ -301,10 +303,41 @@ int launch_update(fest_handle *h, double TF, double SF, int flags) {
   return 0;
 }
 
-// One explicit time step of get_next_solution (src/update.f90:129-226) as a
-// sequence of operations: the step start (Temp refresh, U_store), residual
-// evaluations, the stage-1 time step, stage updates and the TVD blends.
-enum { OP_BEGIN, OP_RESID, OP_DT, OP_UPDATE, OP_BLEND };
+// LU-SGS update (src/lusgs.f90:134-183): the diagonal, one launch per
+// hyperplane i + j + k = s forward and backward, then qp += delQ
+template <int NV>
+int launch_lusgs_nv(fest_handle *h) {
+  Blk &b = h->b;
+  hipStream_t st = h->stream;
+  dim3 g, t;
+  box(b.imx - 1, b.jmx - 1, b.kmx - 1, g, t);
+  {
+    KgScope ts(h, KG_LU_DIAG);
+    hipLaunchKernelGGL(k_lusgs_diag<NV>, g, t, 0, st, b);
+  }
+  LAUNCHCHK(h);
+  const int smax = (b.imx - 1) + (b.jmx - 1) + (b.kmx - 1);
+  const dim3 gs(cdiv((long)(b.jmx - 1) * (b.kmx - 1), 256)), ts256(256);
+  {
+    KgScope ts(h, KG_LU_SWEEP);
+    for (int s = 3; s <= smax; ++s) hipLaunchKernelGGL((k_lusgs_sweep<NV, 0>), gs, ts256, 0, st, b, s);
+    for (int s = smax; s >= 3; --s) hipLaunchKernelGGL((k_lusgs_sweep<NV, 1>), gs, ts256, 0, st, b, s);
+  }
+  LAUNCHCHK(h);
+  {
+    KgScope ts(h, KG_LU_APPLY);
+    hipLaunchKernelGGL(k_lusgs_apply<NV>, g, t, 0, st, b);
+  }
+  LAUNCHCHK(h);
+  return 0;
+}
+int launch_lusgs(fest_handle *h) { return h->b.nv == 7 ? launch_lusgs_nv<7>(h) : launch_lusgs_nv<5>(h); }
+
+// One time step of get_next_solution (src/update.f90:129-226) as a sequence
+// of operations: the step start (Temp refresh, U_store), residual
+// evaluations, the stage-1 time step, stage updates, the TVD blends and the
+// LU-SGS update.
+enum { OP_BEGIN, OP_RESID, OP_DT, OP_UPDATE, OP_BLEND, OP_LUSGS };
 struct StepOp {
   int kind;
   double a, c;  // TF, SF of an update; the two weights of a blend
@@ -333,6 +366,10 @@ std::vector<StepOp> step_plan(int ta) {
       p = {{OP_BEGIN, 0, 0, 1}, {OP_RESID, 0, 0, 1}, {OP_DT, 0, 0, 0}, {OP_UPDATE, 1., 1., UPD_APPLY},
            {OP_RESID, 0, 0, 0}, {OP_UPDATE, 1., 1., UPD_APPLY | UPD_WRITE_RES}, {OP_BLEND, 0.5, 0.5, 0}};
       break;
+    case FEST_TA_IMPLICIT:  // :215-218; the update pass writes the residue only
+      p = {{OP_BEGIN, 0, 0, 0}, {OP_RESID, 0, 0, 1}, {OP_DT, 0, 0, 0}, {OP_UPDATE, 1., 1., UPD_WRITE_RES},
+           {OP_LUSGS, 0, 0, 0}};
+      break;
     default:  // "none", :171-174
       p = {{OP_BEGIN, 0, 0, 0}, {OP_RESID, 0, 0, 1}, {OP_DT, 0, 0, 0},
            {OP_UPDATE, 1., 1., UPD_APPLY | UPD_WRITE_RES}};
@@ -355,6 +392,7 @@ int run_op(fest_handle *h, const StepOp &op, int current_iter) {
     case OP_RESID: return launch_residual_front(h, current_iter, op.flags);
     case OP_DT: return launch_time_step(h);
     case OP_UPDATE: return launch_update(h, op.a, op.c, op.flags);
+    case OP_LUSGS: return launch_lusgs(h);
     default: {
       KgScope ts(h, KG_UPDATE);
       hipLaunchKernelGGL(k_blend, dim3(cdiv(b.nc * b.nv, 256)), dim3(256), 0, h->stream, b, op.a, op.c);
@@ -384,8 +422,8 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
   if (scheme->recon < 0 || scheme->recon > 3) return bad("unknown reconstruction");
   if (scheme->time_accuracy != FEST_TA_RK4 && scheme->time_accuracy != FEST_TA_RK2 &&
       scheme->time_accuracy != FEST_TA_NONE && scheme->time_accuracy != FEST_TA_TVDRK2 &&
-      scheme->time_accuracy != FEST_TA_TVDRK3)
-    return bad("time integration not supported (RK4, RK2, TVDRK2, TVDRK3, none)");
+      scheme->time_accuracy != FEST_TA_TVDRK3 && scheme->time_accuracy != FEST_TA_IMPLICIT)
+    return bad("time integration not supported (RK4, RK2, TVDRK2, TVDRK3, none, implicit)");
   for (int f = 0; f < 6; ++f) {
     int id = bc->id[f];
     // ids >= 0 (interfaces) and -10 (cross-block periodic) take their ghost
@@ -475,6 +513,11 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
   h->red_cap = std::max<long>(nblk_dt, 8L * h->norm_blocks);
   AL(b.red, h->red_cap);
   AL(b.norms, 8); AL(b.dtmin, 1);
+  if (b.time_acc == FEST_TA_IMPLICIT) {
+    // LU-SGS sweep array (padded cells, zero outside the interior), radii, diagonal
+    AL(b.dq, b.nc * b.nv); AL(b.lam, 6 * b.nres); AL(b.ldg, b.nres);
+    HIPCHK(h, hipMemsetAsync(b.dq, 0, sizeof(double) * b.nc * b.nv, h->stream));
+  }
 #undef AL
   void *e = nullptr;
   if (hipMalloc(&e, sizeof(unsigned)) != hipSuccess) return fail(h, FEST_E_HIP, "hipMalloc(err)");

@@ -24,7 +24,7 @@ module fest_gpu
       FEST_RECON_PPM = 2, FEST_RECON_WENO = 3
   integer(c_int32_t), parameter, public :: FEST_TURB_NONE = 0, FEST_TURB_SST = 1
   integer(c_int32_t), parameter, public :: FEST_TA_NONE = 0, FEST_TA_RK2 = 2, FEST_TA_RK4 = 4, &
-      FEST_TA_TVDRK2 = 12, FEST_TA_TVDRK3 = 13
+      FEST_TA_TVDRK2 = 12, FEST_TA_TVDRK3 = 13, FEST_TA_IMPLICIT = 20
 
   integer(c_int32_t), parameter, public :: FEST_DL_QP = 1, FEST_DL_TEMP = 2, &
       FEST_DL_RESIDUE = 4, FEST_DL_F = 8, FEST_DL_G = 16, FEST_DL_H = 32, FEST_DL_DT = 64, &

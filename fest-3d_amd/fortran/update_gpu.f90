@@ -10,11 +10,11 @@
 !< and is compiled into the reference build in place of src/update.f90.  The
 !< first call creates the device block from the solver's derived types and
 !< uploads the geometry (cells, faces, wall distance); every call uploads
-!< qp, runs one whole time step on the device (Temp refresh, all RK stages,
-!< the stage-1 time step) and downloads qp, Temp, residue, delta_t and the
-!< face fluxes F, G, H that find_resnorm and the writers read next.
-!< Time integrators other than none / RK2 / RK4 / TVDRK2 / TVDRK3 (the
-!< implicit LU-SGS ones) stay on the host path.
+!< qp, runs one whole time step on the device (Temp refresh, all RK stages
+!< or the LU-SGS update, the stage-1 time step) and downloads qp, Temp,
+!< residue, delta_t and the face fluxes F, G, H that find_resnorm and the
+!< writers read next.  The parallel LU-SGS variant 'plusgs' stays on the
+!< host path.
 module update
   use iso_c_binding
   use vartypes
@@ -37,7 +37,7 @@ contains
     type(flowtype), intent(in) :: flow
     type(extent), intent(in) :: dims
     select case (trim(scheme%time_step_accuracy))
-      case ("none", "RK2", "RK4", "TVDRK2", "TVDRK3")
+      case ("none", "RK2", "RK4", "TVDRK2", "TVDRK3", "implicit")
         continue
       case default
         print *, "update: time_step_accuracy '", trim(scheme%time_step_accuracy), &
@@ -100,6 +100,7 @@ contains
       case ("RK2"); s%time_accuracy = FEST_TA_RK2
       case ("TVDRK2"); s%time_accuracy = FEST_TA_TVDRK2
       case ("TVDRK3"); s%time_accuracy = FEST_TA_TVDRK3
+      case ("implicit"); s%time_accuracy = FEST_TA_IMPLICIT
       case default; s%time_accuracy = FEST_TA_NONE
     end select
     s%accur = scheme%accur

@@ -48,7 +48,8 @@ enum { FEST_FLUX_AUSM = 0, FEST_FLUX_AUSMP = 1, FEST_FLUX_AUSMUP = 2,
 enum { FEST_RECON_NONE = 0, FEST_RECON_MUSCL = 1, FEST_RECON_PPM = 2,
        FEST_RECON_WENO = 3 };
 enum { FEST_TURB_NONE = 0, FEST_TURB_SST = 1 };
-enum { FEST_TA_NONE = 0, FEST_TA_RK2 = 2, FEST_TA_RK4 = 4, FEST_TA_TVDRK2 = 12, FEST_TA_TVDRK3 = 13 };
+enum { FEST_TA_NONE = 0, FEST_TA_RK2 = 2, FEST_TA_RK4 = 4, FEST_TA_TVDRK2 = 12, FEST_TA_TVDRK3 = 13,
+       FEST_TA_IMPLICIT = 20 /* LU-SGS, src/lusgs.f90 */ };
 
 /* type(extent), src/vartypes.f90:21-26 */
 typedef struct { int32_t imx, jmx, kmx, n_var; } fest_extent;

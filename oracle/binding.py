@@ -241,6 +241,8 @@ t = np.ascontiguousarray(job['temp'], dtype=np.float64)
 L.ref_set_state(ptr(q), ptr(t))
 if job.get('stage', 0) > 0:
     L.ref_stage(ctypes.c_int(job.get('iter', 1)), ctypes.c_int(job['stage']))
+if job.get('lusgs_dt') is not None:
+    d = np.ascontiguousarray(job['lusgs_dt'], dtype=np.float64); L.ref_lusgs(ptr(d))
 out = {}
 for name in job['want']:
     shp = array_shape(case, name)
@@ -251,7 +253,8 @@ pickle.dump(out, open(sys.argv[4], 'wb'))
 """
 
 
-def run_reference(case, nodes, qp, temp, stage, want, it=1, dist=None, wall_dist=False, tmpdir="/tmp"):
+def run_reference(case, nodes, qp, temp, stage, want, it=1, dist=None, wall_dist=False, tmpdir="/tmp",
+                  lusgs_dt=None):
     """Runs the reference modules in a fresh process (their module state,
     e.g. the sticky ppm_flag of boundary_state_reconstruction, persists
     within one process) and returns the requested arrays."""
@@ -264,7 +267,8 @@ def run_reference(case, nodes, qp, temp, stage, want, it=1, dist=None, wall_dist
         with open(script, "w") as f:
             f.write(_REF_SCRIPT)
         job = dict(case=case, nodes=nodes, qp=qp, temp=temp, stage=stage, want=list(want), iter=it,
-                   dist=dist, wall_dist=wall_dist, tmpfile=os.path.join(td, "surfnode.dat"))
+                   dist=dist, wall_dist=wall_dist, tmpfile=os.path.join(td, "surfnode.dat"),
+                   lusgs_dt=lusgs_dt)
         with open(jin, "wb") as f:
             pickle.dump(job, f)
         r = subprocess.run([sys.executable, script, HERE, pkg, jin, jout], capture_output=True, text=True)

@@ -1593,11 +1593,244 @@ static int update_with(orc *s, double TF, double SF, int TU, int use_store) {
   return status;
 }
 
-/* src/update.f90:129-226 (RK4, RK2 and first-order 'none' branches) */
+/* ------------------------------------------------------- LU-SGS (f1) */
+/* One face of a cell as the sweeps see it: the Flist arrays of
+ * src/lusgs.f90:260-312 (area, outward normal of the cell, the averages of
+ * volume, mu, mu_t and, for SST, F1 over the two cells). */
+typedef struct { double A, nx, ny, nz, vol, mm, tm, F1; } lu_face;
+
+/* Flux / SSTFlux, src/lusgs.f90:490-609 and 1027-1196: the face flux with
+ * the state of neighbour ql moved by du (conservative), against the cell's
+ * own state qr for the face averages and differences. */
+static void lu_flux(const orc *s, int nv, const double *ql, const double *qr, const double *du, const lu_face *f,
+                    double *out) {
+  const orc_case *c = &s->c;
+  double gm = c->gm, R_gas = c->R_gas;
+  double U[7], W[7];
+  const double *P = qr;
+  U[0] = ql[0];
+  U[1] = ql[0] * ql[1];
+  U[2] = ql[0] * ql[2];
+  U[3] = ql[0] * ql[3];
+  U[4] = (ql[4] / (gm - 1.0)) + (0.5 * ql[0] * (ql[1] * ql[1] + ql[2] * ql[2] + ql[3] * ql[3]));
+  if (nv == 7) { U[5] = ql[0] * ql[5]; U[6] = ql[0] * ql[6]; }
+  for (int l = 0; l < nv; ++l) U[l] = U[l] + du[l];
+  W[0] = U[0];
+  W[1] = U[1] / U[0];
+  W[2] = U[2] / U[0];
+  W[3] = U[3] / U[0];
+  W[4] = (gm - 1.0) * (U[4] - (0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / U[0]));
+  if (nv == 7) {
+    W[5] = U[5] / U[0];
+    W[6] = U[6] / U[0];
+    /* a negative k or omega falls back to the neighbour's old value */
+    W[5] = W[5] + 0.5 * (1. - copysign(1., W[5])) * (ql[5] - W[5]);
+    W[6] = W[6] + 0.5 * (1. - copysign(1., W[6])) * (ql[6] - W[6]);
+  }
+  double nx = f->nx, ny = f->ny, nz = f->nz, Area = f->A, Volume = f->vol, mmu = f->mm, tmu = f->tm;
+  double Vn = (W[1] * nx) + (W[2] * ny) + (W[3] * nz);
+  double uface = 0.5 * (W[1] + P[1]), vface = 0.5 * (W[2] + P[2]), wface = 0.5 * (W[3] + P[3]);
+  double Fx[7];
+  Fx[0] = W[0] * Vn;
+  Fx[1] = (W[1] * Fx[0]) + (W[4] * nx);
+  Fx[2] = (W[2] * Fx[0]) + (W[4] * ny);
+  Fx[3] = (W[3] * Fx[0]) + (W[4] * nz);
+  double HalfRhoUsquare = 0.5 * W[0] * (W[1] * W[1] + W[2] * W[2] + W[3] * W[3]);
+  double RhoHt = ((gm / (gm - 1.0)) * W[4]) + HalfRhoUsquare;
+  Fx[4] = RhoHt * Vn;
+  if (nv == 7) { Fx[5] = W[5] * Fx[0]; Fx[6] = W[6] * Fx[0]; }
+  double mu = mmu + tmu;
+  double T1 = W[4] / (W[0] * R_gas), T2 = P[4] / (P[0] * R_gas);
+  double dTdx = (T2 - T1) * nx * Area / Volume, dTdy = (T2 - T1) * ny * Area / Volume,
+         dTdz = (T2 - T1) * nz * Area / Volume;
+  double dudx = (P[1] - W[1]) * nx * Area / Volume, dudy = (P[1] - W[1]) * ny * Area / Volume,
+         dudz = (P[1] - W[1]) * nz * Area / Volume;
+  double dvdx = (P[2] - W[2]) * nx * Area / Volume, dvdy = (P[2] - W[2]) * ny * Area / Volume,
+         dvdz = (P[2] - W[2]) * nz * Area / Volume;
+  double dwdx = (P[3] - W[3]) * nx * Area / Volume, dwdy = (P[3] - W[3]) * ny * Area / Volume,
+         dwdz = (P[3] - W[3]) * nz * Area / Volume;
+  double trace = dudx + dvdy + dwdz;
+  double Tauxx = 2. * mu * (dudx - trace / 3.0), Tauyy = 2. * mu * (dvdy - trace / 3.0),
+         Tauzz = 2. * mu * (dwdz - trace / 3.0);
+  double Tauxy = mu * (dvdx + dudy), Tauxz = mu * (dwdx + dudz), Tauyz = mu * (dwdy + dvdz);
+  double K_heat = (mmu / c->Pr + tmu / c->tPr) * gm * R_gas / (gm - 1.0);
+  double Qx = K_heat * dTdx, Qy = K_heat * dTdy, Qz = K_heat * dTdz;
+  Fx[1] = Fx[1] - (Tauxx * nx + Tauxy * ny + Tauxz * nz);
+  Fx[2] = Fx[2] - (Tauxy * nx + Tauyy * ny + Tauyz * nz);
+  Fx[3] = Fx[3] - (Tauxz * nx + Tauyz * ny + Tauzz * nz);
+  Fx[4] = Fx[4] - (Tauxx * uface + Tauxy * vface + Tauxz * wface + Qx) * nx;
+  Fx[4] = Fx[4] - (Tauxy * uface + Tauyy * vface + Tauyz * wface + Qy) * ny;
+  Fx[4] = Fx[4] - (Tauxz * uface + Tauyz * vface + Tauzz * wface + Qz) * nz;
+  if (nv == 7) {
+    double F1 = f->F1;
+    double dtkdx = (P[5] - W[5]) * nx * Area / Volume, dtkdy = (P[5] - W[5]) * ny * Area / Volume,
+           dtkdz = (P[5] - W[5]) * nz * Area / Volume;
+    double dtwdx = (P[6] - W[6]) * nx * Area / Volume, dtwdy = (P[6] - W[6]) * ny * Area / Volume,
+           dtwdz = (P[6] - W[6]) * nz * Area / Volume;
+    double sk = sigma_k1 * F1 + sigma_k2 * (1.0 - F1), sw = sigma_w1 * F1 + sigma_w2 * (1.0 - F1);
+    Fx[5] = Fx[5] + (mmu + sk * tmu) * (dtkdx * nx + dtkdy * ny + dtkdz * nz);
+    Fx[6] = Fx[6] + (mmu + sw * tmu) * (dtwdx * nx + dtwdy * ny + dtwdz * nz);
+  }
+  for (int l = 0; l < nv; ++l) out[l] = Fx[l] * Area;
+}
+
+/* SpectralRadius, src/lusgs.f90:612-660: convective plus viscous spectral
+ * radius of a face times its area; c1, c2 the two cell centres */
+static double lu_radius(const orc *s, const double *ql, const double *qr, const lu_face *f, const double *c1,
+                        const double *c2) {
+  const orc_case *c = &s->c;
+  double gm = c->gm;
+  double Vn = 0.5 * (((ql[1] + qr[1]) * f->nx) + ((ql[2] + qr[2]) * f->ny) + ((ql[3] + qr[3]) * f->nz));
+  Vn = fabs(Vn);
+  double a = 0.5 * (sqrt(gm * ql[4] / ql[0]) + sqrt(gm * qr[4] / qr[0]));
+  double rho = 0.5 * (ql[0] + qr[0]);
+  double dist = sqrt((c1[0] - c2[0]) * (c1[0] - c2[0]) + (c1[1] - c2[1]) * (c1[1] - c2[1]) +
+                     (c1[2] - c2[2]) * (c1[2] - c2[2]));
+  double vis = gm * (f->mm / c->Pr + f->tm / c->tPr) / (rho * dist);
+  return (Vn + a + vis) * f->A;
+}
+
+/* index into the sweep arrays delQstar/delQ over (0:imx,0:jmx,0:kmx) */
+static inline size_t DQI(const orc *s, int i, int j, int k) {
+  return (size_t)i + (size_t)(s->imx + 1) * ((size_t)j + (size_t)(s->jmx + 1) * (size_t)k);
+}
+
+/* The six faces, neighbour states and centres of cell (i,j,k), in the
+ * reference's numbering 1..6 = i-, j-, k-, i+, j+, k+ (src/lusgs.f90:246-312). */
+static void lu_cell(const orc *s, int i, int j, int k, double Q[7][7], double C[7][3], lu_face F[6]) {
+  int nv = s->nv, sst = s->c.turb == ORC_TURB_SST;
+  const int di[7] = {0, -1, 0, 0, 1, 0, 0}, dj[7] = {0, 0, -1, 0, 0, 1, 0}, dk[7] = {0, 0, 0, -1, 0, 0, 1};
+  const double *vol = s->a[ORC_VOL], *mu = s->a[ORC_MU], *mut = s->a[ORC_MUT], *F1 = s->a[ORC_F1];
+  int lam_mu = s->c.mu_ref != 0.0;
+  size_t c0 = CI(s, i, j, k);
+  for (int n = 0; n < 7; ++n) {
+    size_t cn = CI(s, i + di[n], j + dj[n], k + dk[n]);
+    for (int l = 0; l < nv; ++l) Q[n][l] = s->a[ORC_QP][(size_t)l * s->ncell + cn];
+    C[n][0] = s->a[ORC_CX][cn]; C[n][1] = s->a[ORC_CY][cn]; C[n][2] = s->a[ORC_CZ][cn];
+  }
+  for (int n = 1; n <= 6; ++n) {
+    int d = (n - 1) % 3, up = n > 3;
+    int fi_ = i + (d == 0 && up), fj = j + (d == 1 && up), fk = k + (d == 2 && up);
+    size_t g = d == 0 ? IFI(s, fi_, fj, fk) : d == 1 ? JFI(s, fi_, fj, fk) : KFI(s, fi_, fj, fk);
+    const double *A = s->a[d == 0 ? ORC_IA : d == 1 ? ORC_JA : ORC_KA];
+    const double *nx = s->a[d == 0 ? ORC_INX : d == 1 ? ORC_JNX : ORC_KNX];
+    const double *ny = s->a[d == 0 ? ORC_INY : d == 1 ? ORC_JNY : ORC_KNY];
+    const double *nz = s->a[d == 0 ? ORC_INZ : d == 1 ? ORC_JNZ : ORC_KNZ];
+    size_t cn = CI(s, i + di[n], j + dj[n], k + dk[n]);
+    lu_face *f = &F[n - 1];
+    f->A = A[g];
+    f->nx = up ? nx[g] : -nx[g];
+    f->ny = up ? ny[g] : -ny[g];
+    f->nz = up ? nz[g] : -nz[g];
+    f->vol = 0.5 * (vol[cn] + vol[c0]);
+    /* mmu points at zeros for inviscid flow and tmu for laminar flow
+     * (src/lusgs.f90:115-130) */
+    f->mm = lam_mu ? 0.5 * (mu[cn] + mu[c0]) : 0.0;
+    f->tm = sst ? 0.5 * (mut[cn] + mut[c0]) : 0.0;
+    f->F1 = sst ? 0.5 * (F1[cn] + F1[c0]) : 0.0;
+  }
+}
+
+/* update_laminar_variables / update_SST_variables, src/lusgs.f90:186-488 and
+ * 686-1024: forward sweep for delQstar, backward sweep for delQ, then the
+ * conservative update of the interior cells. */
+static int lusgs(orc *s) {
+  const orc_case *c = &s->c;
+  int nv = s->nv, sst = c->turb == ORC_TURB_SST;
+  double gm = c->gm;
+  size_t ndq = (size_t)(IMX + 1) * (JMX + 1) * (KMX + 1);
+  double *dqs = calloc(ndq * (size_t)nv, sizeof(double));
+  double *dq = calloc(ndq * (size_t)nv, sizeof(double));
+  if (!dqs || !dq) { free(dqs); free(dq); return ORC_E_CONFIG; }
+  const double *res = s->a[ORC_RES], *dt = s->a[ORC_DT], *vol = s->a[ORC_VOL], *F1a = s->a[ORC_F1];
+  const double zero[7] = {0, 0, 0, 0, 0, 0, 0};
+  double Q[7][7], C[7][3], L[6], D[7], New[3][7], Old[3][7];
+  lu_face F[6];
+#define DQS(l, i, j, k) dqs[(size_t)(l) * ndq + DQI(s, i, j, k)]
+#define DQ(l, i, j, k) dq[(size_t)(l) * ndq + DQI(s, i, j, k)]
+  for (int pass = 0; pass < 2; ++pass) {
+    int ni = IMX - 1, nj = JMX - 1, nk = KMX - 1;
+    /* forward: k, j, i ascending; backward: i, j, k descending */
+    for (int a = 0; a < (pass ? ni : nk); ++a)
+      for (int b = 0; b < nj; ++b)
+        for (int e = 0; e < (pass ? nk : ni); ++e) {
+          int i = pass ? ni - a : e + 1, j = pass ? nj - b : b + 1, k = pass ? nk - e : a + 1;
+          lu_cell(s, i, j, k, Q, C, F);
+          const int di[3] = {1, 0, 0}, dj[3] = {0, 1, 0}, dk[3] = {0, 0, 1};
+          for (int d = 0; d < 3; ++d) {
+            int n = pass ? d + 4 : d + 1; /* neighbour 1..6 */
+            int si = pass ? 1 : -1;
+            double dn[7];
+            for (int l = 0; l < nv; ++l)
+              dn[l] = pass ? DQ(l, i + si * di[d], j + si * dj[d], k + si * dk[d])
+                           : DQS(l, i + si * di[d], j + si * dj[d], k + si * dk[d]);
+            lu_flux(s, nv, Q[n], Q[0], dn, &F[n - 1], New[d]);
+            lu_flux(s, nv, Q[n], Q[0], zero, &F[n - 1], Old[d]);
+          }
+          for (int n = 1; n <= 6; ++n) L[n - 1] = lu_radius(s, Q[n], Q[0], &F[n - 1], C[n], C[0]);
+          size_t c0 = CI(s, i, j, k), r = RI(s, i, j, k);
+          double sumL = L[0] + L[1] + L[2] + L[3] + L[4] + L[5];
+          for (int l = 0; l < nv; ++l) D[l] = (vol[c0] / dt[r]) + 0.5 * sumL;
+          if (sst) {
+            double beta = F1a[c0] * beta1 + (1.0 - F1a[c0]) * beta2;
+            D[5] = (D[5] + (bstar * QP(6, i, j, k)) * vol[c0]);
+            D[6] = (D[6] + 2.0 * beta * QP(6, i, j, k) * vol[c0]);
+          }
+          for (int l = 0; l < nv; ++l) {
+            double dI = New[0][l] - Old[0][l], dJ = New[1][l] - Old[1][l], dK = New[2][l] - Old[2][l];
+            if (!pass) {
+              double du = -res[(size_t)l * s->nR + r] -
+                          0.5 * ((dI - L[0] * DQS(l, i - 1, j, k)) + (dJ - L[1] * DQS(l, i, j - 1, k)) +
+                                 (dK - L[2] * DQS(l, i, j, k - 1)));
+              DQS(l, i, j, k) = du / D[l];
+            } else {
+              DQ(l, i, j, k) = DQS(l, i, j, k) - 0.5 *
+                                                     ((dI - L[3] * DQ(l, i + 1, j, k)) + (dJ - L[4] * DQ(l, i, j + 1, k)) +
+                                                      (dK - L[5] * DQ(l, i, j, k + 1))) /
+                                                     D[l];
+            }
+          }
+        }
+  }
+  for (int k = 1; k <= KMX - 1; ++k) for (int j = 1; j <= JMX - 1; ++j) for (int i = 1; i <= IMX - 1; ++i) {
+    double U[7];
+    U[0] = QP(0, i, j, k);
+    U[1] = QP(0, i, j, k) * QP(1, i, j, k);
+    U[2] = QP(0, i, j, k) * QP(2, i, j, k);
+    U[3] = QP(0, i, j, k) * QP(3, i, j, k);
+    U[4] = (QP(4, i, j, k) / (gm - 1.0)) +
+           (0.5 * QP(0, i, j, k) * (QP(1, i, j, k) * QP(1, i, j, k) + QP(2, i, j, k) * QP(2, i, j, k) +
+                                    QP(3, i, j, k) * QP(3, i, j, k)));
+    if (sst) { U[5] = QP(0, i, j, k) * QP(5, i, j, k); U[6] = QP(0, i, j, k) * QP(6, i, j, k); }
+    for (int l = 0; l < nv; ++l) U[l] = U[l] + DQ(l, i, j, k);
+    QP(0, i, j, k) = U[0];
+    QP(1, i, j, k) = U[1] / U[0];
+    QP(2, i, j, k) = U[2] / U[0];
+    QP(3, i, j, k) = U[3] / U[0];
+    QP(4, i, j, k) = (gm - 1.0) * (U[4] - (0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / U[0]));
+    if (sst) {
+      if (U[5] > 0) QP(5, i, j, k) = U[5] / U[0];
+      if (U[6] > 0) QP(6, i, j, k) = U[6] / U[0];
+    }
+  }
+#undef DQS
+#undef DQ
+  free(dqs);
+  free(dq);
+  return ORC_OK;
+}
+
+/* src/update.f90:129-226 (RK4, RK2, TVD-RK, first-order 'none' and the
+ * LU-SGS 'implicit' branches) */
 int orc_step(orc *s, int current_iter) {
   int st;
   orc_temp_refresh(s);
   int ta = s->c.time_accuracy;
+  if (ta == ORC_TA_IMPLICIT) {
+    if ((st = orc_residual(s, current_iter))) return st;
+    if ((st = orc_time_step(s))) return st;
+    return lusgs(s);
+  }
   if (ta == ORC_TA_NONE) {
     if ((st = orc_residual(s, current_iter))) return st;
     if ((st = orc_time_step(s))) return st;
@@ -1773,12 +2006,14 @@ int orc_group_step(orc **blocks, int nb, const orc_link *links, int nl, int curr
     orc_temp_refresh(blocks[b]);
   }
   int ta = blocks[0]->c.time_accuracy;
-  if (ta == ORC_TA_NONE) {
+  if (ta == ORC_TA_NONE || ta == ORC_TA_IMPLICIT) {
+    /* LU-SGS runs on each block by itself: its sweep arrays are zero in
+     * the ghost layers, interfaces included (src/lusgs.f90:251,361) */
     if ((st = orc_exchange(blocks, nb, links, nl))) return st;
     for (int b = 0; b < nb; ++b) {
       if ((st = orc_residual(blocks[b], current_iter))) return st;
       if ((st = orc_time_step(blocks[b]))) return st;
-      if ((st = update_with(blocks[b], 1., 1., 0, 0))) return st;
+      if ((st = ta == ORC_TA_NONE ? update_with(blocks[b], 1., 1., 0, 0) : lusgs(blocks[b]))) return st;
     }
     return ORC_OK;
   }

@@ -29,7 +29,8 @@ enum { ORC_FLUX_AUSM = 0, ORC_FLUX_AUSMP = 1, ORC_FLUX_AUSMUP = 2,
 enum { ORC_RECON_NONE = 0, ORC_RECON_MUSCL = 1, ORC_RECON_PPM = 2,
        ORC_RECON_WENO = 3 };
 enum { ORC_TURB_NONE = 0, ORC_TURB_SST = 1 };
-enum { ORC_TA_NONE = 0, ORC_TA_RK2 = 2, ORC_TA_RK4 = 4, ORC_TA_TVDRK2 = 12, ORC_TA_TVDRK3 = 13 };
+enum { ORC_TA_NONE = 0, ORC_TA_RK2 = 2, ORC_TA_RK4 = 4, ORC_TA_TVDRK2 = 12, ORC_TA_TVDRK3 = 13,
+       ORC_TA_IMPLICIT = 20 };
 
 /* Status codes: same meaning as the product library's. */
 enum { ORC_OK = 0, ORC_E_NONPHYSICAL = -1, ORC_E_NAN = -2, ORC_E_CONFIG = -3 };

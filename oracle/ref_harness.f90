@@ -26,6 +26,7 @@ module ref_harness
   use source,                        only: add_source_term_residue
   use global_sst,                    only: sst_F1
   use wall_dist,                     only: setup_wall_dist, find_wall_dist, dist
+  use lusgs,                         only: setup_lusgs, update_with_lusgs
   implicit none
   private
 
@@ -57,7 +58,7 @@ module ref_harness
   integer :: imx, jmx, kmx, nv
 
   public :: ref_init, ref_set_nodes, ref_set_state, ref_set_dist, ref_residual, ref_get, &
-            ref_wall_dist, ref_stage
+            ref_wall_dist, ref_stage, ref_lusgs
 
 contains
 
@@ -241,6 +242,19 @@ contains
     integer(c_int), value :: current_iter
     call ref_stage(current_iter, 5)
   end subroutine ref_residual
+
+  ! one LU-SGS update (src/lusgs.f90:134-183) of the current state with the
+  ! residue of the last ref_stage(.., 5) and the time step dt; the module
+  ! allocates its sweep arrays once, so one call per process
+  subroutine ref_lusgs(dt) bind(c, name='ref_lusgs')
+    real(c_double), intent(in) :: dt(1:imx-1, 1:jmx-1, 1:kmx-1)
+    real(wp), allocatable :: delta_t(:,:,:)
+    schm%time_step_accuracy = 'implicit'
+    call setup_lusgs(control, schm, flow, dims)
+    allocate(delta_t(1:imx-1, 1:jmx-1, 1:kmx-1))
+    delta_t = dt
+    call update_with_lusgs(qp, residue, delta_t, cells, Ifaces, Jfaces, Kfaces, schm, dims)
+  end subroutine ref_lusgs
 
   ! copies one array out in the oracle's layout; ids follow orc_array()
   subroutine ref_get(id, out) bind(c, name='ref_get')

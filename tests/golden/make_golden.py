@@ -5,6 +5,11 @@ sources are absent.  Each fixture holds the full input (ghost-filled nodes,
 state, stale temperature, wall distance) and the reference's face fluxes,
 residue and ghost-filled state after one residual evaluation.
 
+The lusgs_*.npz fixtures hold one 'implicit' time step instead: the
+reference's residual evaluation followed by its LU-SGS update
+(src/lusgs.f90:134-183) with the time step delta_t stored as in_dt (the
+reference's time.f90 needs MPI and is not built), and the state after it.
+
 Run from the repository root:  python tests/golden/make_golden.py
 """
 import dataclasses
@@ -37,7 +42,43 @@ def cases():
     yield "laminar_bcmix_ldfss0", c3, C.cartesian_nodes(c3, warp=0.05), q, 2, False
 
 
+def lusgs_cases():
+    c, _, q = C.laminar_box(10, 8, 6)
+    c = dataclasses.replace(c, time_accuracy="implicit", mu_sutherland=1, bc_id=(-3, -4, -5, -5, -9, -9),
+                            fix_rho=(c.rho_inf,) * 6, fix_u=(c.u_inf,) * 6, fix_p=(c.p_inf,) * 6,
+                            fix_twall=(0, 0, 310.0, 0, 0, 0))
+    yield "lusgs_laminar_walls", c, C.cartesian_nodes(c, warp=0.04), q, 1, False
+    c, _, q = C.laminar_box(9, 7, 6)
+    c = dataclasses.replace(c, time_accuracy="implicit", mu_ref=0.0, flux="slau", recon="weno")
+    yield "lusgs_inviscid_slau", c, C.cartesian_nodes(c, warp=0.03), q, 1, False
+    c, n, q = C.sst_bump(12, 10, 6)
+    c = dataclasses.replace(c, time_accuracy="implicit")
+    yield "lusgs_sst_bump", c, n, q, 2, True
+
+
+def write_lusgs():
+    for name, case, nodes, q, it, wall in lusgs_cases():
+        o = Oracle(case)
+        o.set_nodes(nodes)
+        if wall:
+            o.wall_dist()
+        o.set_state(q)
+        o.temp_refresh()
+        inputs = dict(nodes=o.arr("nodes").copy(), qp=o.arr("qp").copy(), temp=o.arr("temp").copy(),
+                      dist=o.arr("dist").copy())
+        assert o.step(it) == 0
+        inputs["dt"] = o.arr("dt").copy()
+        ref = run_reference(case, inputs["nodes"], inputs["qp"], inputs["temp"], 5, ["qp", "res"], it=it,
+                            dist=inputs["dist"], lusgs_dt=inputs["dt"])
+        meta = np.array(C.case_to_json(case))
+        np.savez_compressed(os.path.join(OUT, name + ".npz"), meta=meta, iter=np.array(it),
+                            **{"in_" + k: v for k, v in inputs.items()},
+                            **{"out_" + k: v for k, v in ref.items()})
+        print("wrote", name)
+
+
 def main():
+    write_lusgs()
     for name, case, nodes, q, it, wall in cases():
         o = Oracle(case)
         o.set_nodes(nodes)

@@ -72,3 +72,44 @@ def test_laminar_boundaries(bcs, tw, it):
 def test_laminar_box_warped():
     case, _, q = C.laminar_box(12, 10, 6)
     compare(case, C.cartesian_nodes(case, warp=0.03), q)
+
+
+def compare_lusgs(case, nodes, q, it=1, wall_dist=False):
+    """One 'implicit' time step: the oracle's residual, time step and LU-SGS
+    update against the reference's residual and update_with_lusgs
+    (src/lusgs.f90:134-183) fed the oracle's time step (time.f90 needs MPI
+    and is not built)."""
+    case = dataclasses.replace(case, time_accuracy="implicit")
+    o = Oracle(case)
+    o.set_nodes(nodes)
+    if wall_dist:
+        o.wall_dist()
+    o.set_state(q)
+    o.temp_refresh()
+    temp, qin = o.arr("temp").copy(), o.arr("qp").copy()
+    fn, dist = o.arr("nodes").copy(), o.arr("dist").copy()
+    assert o.step(it) == 0
+    ref = run_reference(case, fn, qin, temp, 5, ["qp", "res"], it=it, dist=None if wall_dist else dist,
+                        wall_dist=wall_dist, lusgs_dt=o.arr("dt").copy())
+    assert np.array_equal(o.arr("res"), ref["res"])
+    got, want = o.arr("qp")[:, 3:-3, 3:-3, 3:-3], ref["qp"][:, 3:-3, 3:-3, 3:-3]
+    assert not np.array_equal(got, qin[:, 3:-3, 3:-3, 3:-3])
+    assert np.array_equal(got, want), float(np.abs(got - want).max())
+
+
+@pytest.mark.parametrize("mu_mode", ["constant", "sutherland", "inviscid"])
+@pytest.mark.parametrize("bcs", [(-9, -9, -5, -5, -9, -9), (-3, -4, -6, -5, -8, -8), (-1, -2, -5, -6, -6, -6)])
+def test_lusgs_laminar(mu_mode, bcs):
+    case, _, q = C.laminar_box(10, 8, 6)
+    case = dataclasses.replace(case, bc_id=bcs, fix_rho=(case.rho_inf,) * 6, fix_u=(case.u_inf,) * 6,
+                               fix_v=(2.0,) * 6, fix_w=(0.0,) * 6, fix_p=(case.p_inf,) * 6,
+                               fix_twall=(0, 0, 320.0, 0, 0, 0), mu_sutherland=int(mu_mode == "sutherland"),
+                               mu_ref=0.0 if mu_mode == "inviscid" else case.mu_ref)
+    compare_lusgs(case, C.cartesian_nodes(case, warp=0.05), q)
+
+
+@pytest.mark.parametrize("flux,recon", [("ausm", "muscl"), ("slau", "weno"), ("van_leer", "none")])
+def test_lusgs_sst(flux, recon):
+    case, nodes, q = C.sst_bump(14, 10, 6)
+    case = dataclasses.replace(case, flux=flux, recon=recon)
+    compare_lusgs(case, nodes, q, it=2, wall_dist=True)
