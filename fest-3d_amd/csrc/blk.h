@@ -6,7 +6,7 @@
 //   I / J / K face arrays (-2:imx+3,..), (..,-2:jmx+3,..), (..,-2:kmx+3)
 //   F / G / H            (1:imx,1:jmx-1,1:kmx-1), (1:imx-1,1:jmx,..), (..,1:kmx)
 //   residue, R_store, delta_t (1:imx-1,1:jmx-1,1:kmx-1)
-//   LU-SGS delQ (padded cell planes), radii and diagonal (1:imx-1,..)
+//   LU-SGS arrays      hyperplane-major slots (i+j+k, k, j), see k_lusgs.h
 //   gradients            (0:imx,0:jmx,0:kmx) x n_grad per direction
 // These are the reference's own extents (src/update.f90:143-165), so the
 // host<->device copies are plain memcpy and index mapping is exact.
@@ -33,8 +33,10 @@ struct Blk {
   double *csnd[3];            // stage-1 face sound speed (time step)
   double *res, *dt, *ust, *rst;
   double *src;                // SST source S_k, S_w of the interior cells (x volume)
-  double *dq;                 // LU-SGS delQstar/delQ, padded cells (zero outside the interior)
-  double *lam, *ldg;          // LU-SGS radii x area (6 planes) and diagonal, interior cells
+  // LU-SGS inputs and sweep values in hyperplane-major storage (k_lusgs.h)
+  int sk_j;                   // jmx + 1: positions per k row of a hyperplane
+  long sk_p, sk_n;            // positions per hyperplane; hyperplanes x sk_p
+  double *sk;                 // 3 nv + 23 components x sk_n
   double *g[3];
   double *nodes;              // grid nodes (3 planes) when set on device
   double *red;                // reduction partials

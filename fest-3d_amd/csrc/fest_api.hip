@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <initializer_list>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -52,6 +53,7 @@ struct fest_handle {
   long red_cap = 0;        // doubles in b.red
   int norm_blocks = 0;
   bool have_grid = false, have_state = false;
+  bool sk_geometry = false;  // the LU-SGS storage holds the current grid's geometry
   // opt-in event timing: groups whose bit is set record a start and a stop
   // event on the stream around their launches
   unsigned timed = 0;
@@ -303,21 +305,59 @@ int launch_update(fest_handle *h, double TF, double SF, int flags) {
   return 0;
 }
 
-// LU-SGS update (src/lusgs.f90:134-183): the diagonal, one launch per
-// hyperplane i + j + k = s forward and backward, then qp += delQ
+// LU-SGS update (src/lusgs.f90:134-183): the hyperplane-major copies and
+// the diagonal, one launch per hyperplane i + j + k = s forward and
+// backward, then qp += delQ
+// copies up to 8 planes into hyperplane-major components c0.. (k_to_skew)
+void to_skew(fest_handle *h, std::initializer_list<const double *> planes, int c0, int layout, int lo, int hi) {
+  Blk &b = h->b;
+  SkewIn a{};
+  a.n = 0;
+  for (const double *p : planes) a.p[a.n++] = p;
+  a.c0 = c0;
+  a.layout = layout;
+  a.lo = lo;
+  a.hi = hi;
+  const int ni = b.imx - 1 + hi - lo + 1, nj = b.jmx - 1 + hi - lo + 1, nk = b.kmx - 1 + hi - lo + 1;
+  const dim3 g((unsigned)(cdiv(ni, 32) * cdiv(nj, 32)), (unsigned)nk);
+  hipLaunchKernelGGL(k_to_skew, g, dim3(256), 0, h->stream, b, a);
+}
+
 template <int NV>
 int launch_lusgs_nv(fest_handle *h) {
+  using C = Sk<NV>;
   Blk &b = h->b;
   hipStream_t st = h->stream;
   dim3 g, t;
   box(b.imx - 1, b.jmx - 1, b.kmx - 1, g, t);
   {
     KgScope ts(h, KG_LU_DIAG);
-    hipLaunchKernelGGL(k_lusgs_diag<NV>, g, t, 0, st, b);
+    if (!h->sk_geometry) {
+      // volumes and faces: once per grid
+      to_skew(h, {b.vol}, C::VOL, 0, 0, 1);
+      for (int d = 0; d < 3; ++d) to_skew(h, {b.fA[d], b.fx[d], b.fy[d], b.fz[d]}, C::FACE + 4 * d, 2 + d, 1, 1);
+      h->sk_geometry = true;
+    }
+    const long nc = b.nc, nr = b.nres;
+    if (NV == 7)
+      to_skew(h, {b.qp, b.qp + nc, b.qp + 2 * nc, b.qp + 3 * nc, b.qp + 4 * nc, b.qp + 5 * nc, b.qp + 6 * nc},
+              C::Q, 0, 0, 1);
+    else
+      to_skew(h, {b.qp, b.qp + nc, b.qp + 2 * nc, b.qp + 3 * nc, b.qp + 4 * nc}, C::Q, 0, 0, 1);
+    to_skew(h, {b.mu, b.mut, b.f1}, C::MU, 0, 0, 1);
+    if (NV == 7)
+      to_skew(h, {b.res, b.res + nr, b.res + 2 * nr, b.res + 3 * nr, b.res + 4 * nr, b.res + 5 * nr, b.res + 6 * nr},
+              C::RES, 1, 1, 0);
+    else
+      to_skew(h, {b.res, b.res + nr, b.res + 2 * nr, b.res + 3 * nr, b.res + 4 * nr}, C::RES, 1, 1, 0);
+    // radii, Old fluxes and diagonal over the cells 1..n+1
+    const dim3 gp((unsigned)(cdiv(b.imx, 16) * cdiv(b.jmx, 16)), (unsigned)b.kmx);
+    hipLaunchKernelGGL(k_lusgs_prep<NV>, gp, dim3(256), 0, st, b);
   }
   LAUNCHCHK(h);
   const int smax = (b.imx - 1) + (b.jmx - 1) + (b.kmx - 1);
-  const dim3 gs(cdiv((long)(b.jmx - 1) * (b.kmx - 1), 256)), ts256(256);
+  // four lanes per (j, k) pair
+  const dim3 gs(cdiv(4L * (b.jmx - 1) * (b.kmx - 1), 256)), ts256(256);
   {
     KgScope ts(h, KG_LU_SWEEP);
     for (int s = 3; s <= smax; ++s) hipLaunchKernelGGL((k_lusgs_sweep<NV, 0>), gs, ts256, 0, st, b, s);
@@ -514,9 +554,14 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
   AL(b.red, h->red_cap);
   AL(b.norms, 8); AL(b.dtmin, 1);
   if (b.time_acc == FEST_TA_IMPLICIT) {
-    // LU-SGS sweep array (padded cells, zero outside the interior), radii, diagonal
-    AL(b.dq, b.nc * b.nv); AL(b.lam, 6 * b.nres); AL(b.ldg, b.nres);
-    HIPCHK(h, hipMemsetAsync(b.dq, 0, sizeof(double) * b.nc * b.nv, h->stream));
+    // LU-SGS storage over the hyperplanes of (0:imx, 0:jmx, 0:kmx); zero
+    // where nothing is written (the sweep values outside the interior)
+    b.sk_j = b.jmx + 1;
+    b.sk_p = (long)b.sk_j * (b.kmx + 1);
+    b.sk_n = (long)(b.imx + b.jmx + b.kmx + 1) * b.sk_p;
+    const long nsk = sk_ncomp(b.nv) * b.sk_n;
+    AL(b.sk, nsk);
+    HIPCHK(h, hipMemsetAsync(b.sk, 0, sizeof(double) * nsk, h->stream));
   }
 #undef AL
   void *e = nullptr;
@@ -573,6 +618,7 @@ int fest_upload_geometry(fest_handle *h, const double *cells4, const double *if4
   if (dist) HIPCHK(h, hipMemcpyAsync(b.dist, dist, nc, hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->have_grid = true;
+  h->sk_geometry = false;
   return 0;
 }
 
@@ -643,6 +689,7 @@ int fest_set_grid(fest_handle *h, const double *nodes, int with_wall_dist) {
   int rc = check_err(h);
   if (rc) return fail(h, FEST_E_NONPHYSICAL, "non-positive cell volume in the grid");
   h->have_grid = true;
+  h->sk_geometry = false;
   return 0;
 }
 

@@ -13,7 +13,8 @@
 // delQstar and delQ share one array: a cell's backward value overwrites its
 // own forward value, which nothing else reads afterwards; the layer of cells
 // next to the interior (index 0 and imx, jmx, kmx) stays zero, as the
-// reference's arrays are zero there.
+// reference's arrays are zero there.  The sweeps read only hyperplane-major
+// copies (Sk), so qp is updated after the backward sweep.
 #pragma once
 // unfused, as the reference: the sweep arithmetic matches it bit for bit
 #pragma clang fp contract(off)
@@ -109,7 +110,8 @@ __device__ __forceinline__ void lu_flux(const Blk &b, const double (&ql)[NV], co
 }
 
 // face n (0..5 = i-, j-, k-, i+, j+, k+) of cell c0 = (i, j, k) with
-// neighbour cn: the Flist arrays of src/lusgs.f90:260-312
+// neighbour cn, from the natural-layout arrays: the Flist arrays of
+// src/lusgs.f90:260-312
 __device__ __forceinline__ LuFace lu_face(const Blk &b, int n, int i, int j, int k, long c0, long cn) {
   const int d = n % 3;
   const bool up = n >= 3;
@@ -128,16 +130,18 @@ __device__ __forceinline__ LuFace lu_face(const Blk &b, int n, int i, int j, int
   return f;
 }
 
-// SpectralRadius (src/lusgs.f90:612-660) of face f times its area
+// SpectralRadius (src/lusgs.f90:612-660) of face f times its area; c1 the
+// neighbour's centre, c2 the cell's
 template <int NV>
 __device__ __forceinline__ double lu_radius(const Blk &b, const double (&ql)[NV], const double (&qr)[NV],
-                                            const LuFace &f, long c1, long c2) {
+                                            const LuFace &f, double c1x, double c1y, double c1z, double c2x,
+                                            double c2y, double c2z) {
   const double gm = b.gm;
   double Vn = 0.5 * (((ql[1] + qr[1]) * f.nx) + ((ql[2] + qr[2]) * f.ny) + ((ql[3] + qr[3]) * f.nz));
   Vn = fabs(Vn);
   const double a = 0.5 * (sqrt(gm * ql[4] / ql[0]) + sqrt(gm * qr[4] / qr[0]));
   const double rho = 0.5 * (ql[0] + qr[0]);
-  const double dx = b.cx[c1] - b.cx[c2], dy = b.cy[c1] - b.cy[c2], dz = b.cz[c1] - b.cz[c2];
+  const double dx = c1x - c2x, dy = c1y - c2y, dz = c1z - c2z;
   const double dist = sqrt(dx * dx + dy * dy + dz * dz);
   const double vis = gm * (f.mm / b.Pr + f.tm / b.tPr) / (rho * dist);
   return (Vn + a + vis) * f.A;
@@ -149,93 +153,261 @@ __device__ __forceinline__ void lu_load(const Blk &b, long c, double (&q)[NV]) {
   for (int l = 0; l < NV; ++l) q[l] = b.qp[(long)l * b.nc + c];
 }
 
-// Diagonal of every interior cell: the six spectral radii times area
-// (lam, 6 planes) and vol/dt + 0.5 sum(lam) (ldg); src/lusgs.f90:321-344.
-// Both sweeps use the same values.
+// Hyperplane-major ("skewed") storage of everything the sweeps read.  Cell
+// (i, j, k), 0 <= i <= imx, 0 <= j <= jmx, 0 <= k <= kmx, lies on hyperplane
+// s = i + j + k at position r = k sk_j + j; component c of it is stored at
+// (s NC + c) sk_p + r.  One hyperplane's components are one contiguous block
+// (few TLB pages per launch), its cells are contiguous per component, and
+// the i-, j-, k- neighbours of a cell sit on hyperplane s - 1 at r, r - 1,
+// r - sk_j: every sweep load reads whole cache lines.
 template <int NV>
-__global__ void __launch_bounds__(256) k_lusgs_diag(Blk b) {
-  int i, j;
-  if (!flat_ij(b.imx - 1, b.jmx - 1, i, j, (int)blockIdx.x)) return;
-  i += 1;
-  j += 1;
-  const int k = blockIdx.y + 1;
-  const long c0 = ci(b, i, j, k), r = ri(b, i, j, k);
-  double Q0[NV], Qn[NV];
-  lu_load<NV>(b, c0, Q0);
-  double L[6];
-#pragma unroll
-  for (int n = 0; n < 6; ++n) {
-    const long cn = c0 + (n < 3 ? -cstride(b, n) : cstride(b, n - 3));
-    lu_load<NV>(b, cn, Qn);
-    const LuFace f = lu_face(b, n, i, j, k, c0, cn);
-    L[n] = lu_radius<NV>(b, Qn, Q0, f, cn, c0);
-    b.lam[(long)n * b.nres + r] = L[n];
-  }
-  const double sumL = L[0] + L[1] + L[2] + L[3] + L[4] + L[5];
-  b.ldg[r] = (b.vol[c0] / b.dt[r]) + 0.5 * sumL;
+struct Sk {
+  static constexpr int Q = 0;              // state (cells 0..n+1)
+  static constexpr int MU = NV;            // mu, mu_t, F1 (cells 0..n+1)
+  static constexpr int MUT = NV + 1;
+  static constexpr int F1 = NV + 2;
+  static constexpr int RES = NV + 3;       // residue (interior)
+  static constexpr int VOL = 2 * NV + 3;   // volume (cells 0..n+1)
+  static constexpr int FACE = 2 * NV + 4;  // lower face d: A, nx, ny, nz at FACE + 4 d (cells 1..n+1)
+  static constexpr int DQ = 2 * NV + 16;   // delQstar / delQ (zero outside the interior)
+  static constexpr int DG = 3 * NV + 16;   // diagonal (interior; SST: 3 values)
+  static constexpr int LAM = 3 * NV + 19;  // radius x area of the lower face d (cells 1..n+1)
+  static constexpr int OLD = 3 * NV + 22;  // Old flux of face n = 0..5, NV each (interior)
+  static constexpr int NC = 9 * NV + 22;
+};
+__host__ __device__ inline int sk_ncomp(int nv) { return 9 * nv + 22; }
+
+__device__ __forceinline__ const double *sk_at(const Blk &b, int NC, int s, int c) {
+  return b.sk + ((long)s * NC + c) * b.sk_p;
 }
 
-// One hyperplane i + j + k = s of the forward (BACK = 0, delQstar) or
-// backward (BACK = 1, delQ) sweep.  Threads run over the (j, k) pairs;
-// a pair whose i falls outside 1..imx-1 has no cell on this hyperplane.
-template <int NV, int BACK>
-__global__ void __launch_bounds__(256) k_lusgs_sweep(Blk b, int s) {
-  const int nj = b.jmx - 1, nk = b.kmx - 1;
-  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (t >= nj * nk) return;
-  const int k = t / nj + 1, j = t - (k - 1) * nj + 1;
-  const int i = s - j - k;
-  if (i < 1 || i > b.imx - 1) return;
-  const long c0 = ci(b, i, j, k), r = ri(b, i, j, k);
-  const long nc = b.nc;
-  double Q0[NV], Qn[NV], dn[NV], zero[NV], New[NV], Old[NV], S[NV];
+// Copies n natural-layout planes into skewed components c0.. for the cells
+// lo <= i, j, k <= imx-1+hi (layout 0: padded cell arrays, 1: interior
+// arrays of the residue's extent, 2 + d: face arrays of direction d, the
+// lower face of each cell).  32 x 32 (i, j) tiles of one k go through LDS:
+// rows are read whole, and each diagonal i + j = const of the tile, a run
+// of consecutive positions on one hyperplane, is written whole.
+struct SkewIn {
+  const double *p[8];
+  int n, c0, layout, lo, hi;
+};
+__global__ void __launch_bounds__(256) k_to_skew(Blk b, SkewIn a) {
+  __shared__ double tile[32][33];
+  const int NC = sk_ncomp(b.nv);
+  const int ilo = a.lo, ihi = b.imx - 1 + a.hi, jhi = b.jmx - 1 + a.hi;
+  const int ntx = (ihi - ilo + 32) / 32;
+  const int i0 = ilo + ((int)blockIdx.x % ntx) * 32, j0 = ilo + ((int)blockIdx.x / ntx) * 32;
+  const int k = ilo + (int)blockIdx.y;
+  const int tid = threadIdx.x;
+  for (int c = 0; c < a.n; ++c) {
+    const double *src = a.p[c];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int ii = tid & 31, jj = (tid >> 5) + 8 * m, i = i0 + ii, j = j0 + jj;
+      if (i <= ihi && j <= jhi) {
+        long o = a.layout == 0 ? ci(b, i, j, k) : a.layout == 1 ? ri(b, i, j, k) : fi(b, a.layout - 2, i, j, k);
+        tile[jj][ii] = src[o];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      // position e of the tile in diagonal order: diagonals 0..31 first,
+      // then 32..62 as the mirror image of 30..0
+      const int e = tid + 256 * m, em = e < 528 ? e : 1023 - e;
+      int sg = (int)((sqrtf((float)(8 * em + 1)) - 1.0f) * 0.5f);
+      // exact whatever sqrtf rounds to: sg (sg + 1) / 2 <= em < (sg + 1) (sg + 2) / 2
+      if ((sg + 1) * (sg + 2) / 2 <= em) ++sg;
+      if (sg * (sg + 1) / 2 > em) --sg;
+      const int pos = em - sg * (sg + 1) / 2;
+      int jj = pos, ii = sg - pos;
+      if (e >= 528) { jj = 31 - jj; ii = 31 - ii; }
+      const int i = i0 + ii, j = j0 + jj;
+      if (i <= ihi && j <= jhi) {
+        double *dst = b.sk + ((long)(i + j + k) * NC + a.c0 + c) * b.sk_p + (long)k * b.sk_j + j;
+        *dst = tile[jj][ii];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Everything of a cell the sweeps use that does not depend on them, written
+// into hyperplane-major storage: the radius x area of the lower faces
+// (src/lusgs.f90:612-660; a face's radius is the same seen from either of
+// its cells -- the normal flips sign, every other operand pairs up
+// commutatively -- so the backward sweep reads a cell's upper faces from the
+// neighbours), the six "Old" fluxes of the neighbours' unmoved states
+// (src/lusgs.f90:314-319, 424-429) and the diagonal vol/dt + 0.5 sum of the
+// six radii (src/lusgs.f90:329-345).  16 x 16 (i, j) tiles of one k over
+// the cells 1..n+1 read the stencil whole; each value goes through LDS and
+// out along the tile's diagonals, runs of consecutive positions of one
+// hyperplane.  Values of cells outside the interior are never read.
+template <int NV>
+__global__ void __launch_bounds__(256) k_lusgs_prep(Blk b) {
+  using C = Sk<NV>;
+  __shared__ double tile[16][17];
+  const int ntx = (b.imx + 15) / 16;
+  const int i0 = 1 + ((int)blockIdx.x % ntx) * 16, j0 = 1 + ((int)blockIdx.x / ntx) * 16;
+  const int k = 1 + (int)blockIdx.y;
+  const int tid = threadIdx.x;
+  const int ii = tid & 15, jj = tid >> 4;
+  const int i = min(i0 + ii, b.imx), j = min(j0 + jj, b.jmx);  // clamped lanes compute, never write
+  const bool inner = i <= b.imx - 1 && j <= b.jmx - 1 && k <= b.kmx - 1;
+  // where the tile's diagonal order puts this thread on the way out
+  int oi, oj;
+  {
+    const int e = tid, em = e < 136 ? e : 255 - e;
+    int sg = (int)((sqrtf((float)(8 * em + 1)) - 1.0f) * 0.5f);
+    if ((sg + 1) * (sg + 2) / 2 <= em) ++sg;
+    if (sg * (sg + 1) / 2 > em) --sg;
+    oj = em - sg * (sg + 1) / 2;
+    oi = sg - oj;
+    if (e >= 136) {
+      oj = 15 - oj;
+      oi = 15 - oi;
+    }
+  }
+  const bool out_ok = i0 + oi <= b.imx && j0 + oj <= b.jmx;
+  double *const out_base =
+      b.sk + (long)(i0 + oi + j0 + oj + k) * C::NC * b.sk_p + (long)k * b.sk_j + (j0 + oj);
+  auto emit = [&](int c, double v) {
+    tile[jj][ii] = v;
+    __syncthreads();
+    if (out_ok) out_base[(long)c * b.sk_p] = tile[oj][oi];
+    __syncthreads();
+  };
+  const long c0 = ci(b, i, j, k);
+  double Q0[NV], Qn[NV], zero[NV], Old[NV], L[6];
   lu_load<NV>(b, c0, Q0);
 #pragma unroll
   for (int l = 0; l < NV; ++l) zero[l] = 0.0;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const int n = BACK ? d + 3 : d;
-    const long cn = BACK ? c0 + cstride(b, d) : c0 - cstride(b, d);
+#pragma unroll 1
+  for (int n = 0; n < 6; ++n) {
+    const int d = n % 3;
+    const long cn = n < 3 ? c0 - cstride(b, d) : c0 + cstride(b, d);
     lu_load<NV>(b, cn, Qn);
-#pragma unroll
-    for (int l = 0; l < NV; ++l) dn[l] = b.dq[(long)l * nc + cn];
     const LuFace f = lu_face(b, n, i, j, k, c0, cn);
-    lu_flux<NV>(b, Qn, Q0, dn, f, New);
+    L[n] = lu_radius<NV>(b, Qn, Q0, f, b.cx[cn], b.cy[cn], b.cz[cn], b.cx[c0], b.cy[c0], b.cz[c0]);
+    if (n < 3) emit(C::LAM + n, L[n]);
     lu_flux<NV>(b, Qn, Q0, zero, f, Old);
-    const double lam = b.lam[(long)n * b.nres + r];
-#pragma unroll
-    for (int l = 0; l < NV; ++l) {
-      const double term = (New[l] - Old[l]) - lam * dn[l];
-      S[l] = d == 0 ? term : S[l] + term;
+#pragma unroll 1
+    for (int l = 0; l < NV; ++l) emit(C::OLD + n * NV + l, Old[l]);
+  }
+  double dg = 0.0, d5 = 0.0, d6 = 0.0;
+  if (inner) {
+    const double sumL = L[0] + L[1] + L[2] + L[3] + L[4] + L[5];
+    const double vol = b.vol[c0];
+    dg = (vol / b.dt[ri(b, i, j, k)]) + 0.5 * sumL;
+    if constexpr (NV == 7) {
+      const double F1 = b.f1[c0], w = Q0[6];
+      const double beta = F1 * 0.075 + (1.0 - F1) * 0.0828;
+      d5 = (dg + (0.09 * w) * vol);
+      d6 = (dg + 2.0 * beta * w * vol);
     }
   }
-  double D[NV];
-  const double dg = b.ldg[r];
-#pragma unroll
-  for (int l = 0; l < NV; ++l) D[l] = dg;
+  emit(C::DG, dg);
   if constexpr (NV == 7) {
-    const double F1 = b.f1[c0], vol = b.vol[c0];
-    const double beta = F1 * 0.075 + (1.0 - F1) * 0.0828;
-    D[5] = (D[5] + (0.09 * Q0[6]) * vol);
-    D[6] = (D[6] + 2.0 * beta * Q0[6] * vol);
+    emit(C::DG + 1, d5);
+    emit(C::DG + 2, d6);
   }
+}
+
+// One hyperplane i + j + k = s of the forward (BACK = 0, delQstar) or
+// backward (BACK = 1, delQ) sweep, four lanes per cell: lane d = 0, 1, 2
+// takes face d (the cell's lower face forward, its upper face backward):
+// the flux of the neighbour's state moved by its sweep value (the "New"
+// flux of src/lusgs.f90:308-310, 418-420) minus the Old flux and the radius
+// term; lane 3 takes the diagonal and the right-hand side, sums the three
+// face terms in the reference's order and writes the cell.  Cells run over
+// the (j, k) pairs; a pair whose i falls outside 1..imx-1 has no cell on
+// this hyperplane.
+template <int NV, int BACK>
+__global__ void __launch_bounds__(256) k_lusgs_sweep(Blk b, int s) {
+  using C = Sk<NV>;
+  constexpr int NC = C::NC;
+  const int nj = b.jmx - 1, nk = b.kmx - 1;
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int cell = t >> 2, q = t & 3;
+  // the four lanes of a cell leave together
+  if (cell >= nj * nk) return;
+  const int k = cell / nj + 1, j = cell - (k - 1) * nj + 1;
+  const int i = s - j - k;
+  if (i < 1 || i > b.imx - 1) return;
+  const long P = b.sk_p;
+  const int r = k * b.sk_j + j;
+  double *const W0 = b.sk + (long)s * NC * P;  // this hyperplane
+  const double *const S0 = W0;
+  double v[NV];  // lanes 0..2: face term; lane 3: right-hand side
+  double D[NV];
+  if (q < 3) {
+    const int d = q;
+    const int dr = d == 0 ? 0 : d == 1 ? 1 : b.sk_j;
+    const int rn = BACK ? r + dr : r - dr;
+    const double *const S1 = sk_at(b, NC, BACK ? s + 1 : s - 1, 0);
+    // the lower face d of the cell forward, of the neighbour backward
+    const double *const SF = BACK ? S1 : S0;
+    const int rf = BACK ? rn : r;
+    double Q0[NV], Qn[NV], dn[NV], New[NV];
+#pragma unroll
+    for (int l = 0; l < NV; ++l) {
+      Q0[l] = S0[(C::Q + l) * P + r];
+      Qn[l] = S1[(C::Q + l) * P + rn];
+      dn[l] = S1[(C::DQ + l) * P + rn];
+    }
+    LuFace f;
+    f.A = SF[(C::FACE + 4 * d) * P + rf];
+    // outward normal of this cell: the lower face's own normal flipped forward
+    const double nx = SF[(C::FACE + 4 * d + 1) * P + rf], ny = SF[(C::FACE + 4 * d + 2) * P + rf],
+                 nz = SF[(C::FACE + 4 * d + 3) * P + rf];
+    f.nx = BACK ? nx : -nx;
+    f.ny = BACK ? ny : -ny;
+    f.nz = BACK ? nz : -nz;
+    f.vol = 0.5 * (S1[C::VOL * P + rn] + S0[C::VOL * P + r]);
+    f.mm = 0.5 * (S1[C::MU * P + rn] + S0[C::MU * P + r]);
+    f.tm = 0.5 * (S1[C::MUT * P + rn] + S0[C::MUT * P + r]);
+    f.F1 = 0.5 * (S1[C::F1 * P + rn] + S0[C::F1 * P + r]);
+    lu_flux<NV>(b, Qn, Q0, dn, f, New);
+    const double lam = SF[(C::LAM + d) * P + rf];
+#pragma unroll
+    for (int l = 0; l < NV; ++l) v[l] = (New[l] - S0[(C::OLD + (BACK * 3 + d) * NV + l) * P + r]) - lam * dn[l];
+  } else {
+    const double dg = S0[C::DG * P + r];
+#pragma unroll
+    for (int l = 0; l < NV; ++l) D[l] = dg;
+    if constexpr (NV == 7) {
+      D[5] = S0[(C::DG + 1) * P + r];
+      D[6] = S0[(C::DG + 2) * P + r];
+    }
+#pragma unroll
+    for (int l = 0; l < NV; ++l) v[l] = BACK ? S0[(C::DQ + l) * P + r] : -S0[(C::RES + l) * P + r];
+  }
+  double T0[NV], T1[NV], T2[NV];
 #pragma unroll
   for (int l = 0; l < NV; ++l) {
-    double *p = b.dq + (long)l * nc + c0;
-    if (!BACK) *p = (-b.res[(long)l * b.nres + r] - 0.5 * S[l]) / D[l];
-    else *p = *p - 0.5 * S[l] / D[l];
+    T0[l] = __shfl(v[l], 0, 4);
+    T1[l] = __shfl(v[l], 1, 4);
+    T2[l] = __shfl(v[l], 2, 4);
+  }
+  if (q != 3) return;
+#pragma unroll
+  for (int l = 0; l < NV; ++l) {
+    const double S = (T0[l] + T1[l]) + T2[l];
+    W0[(C::DQ + l) * P + r] = BACK ? v[l] - 0.5 * S / D[l] : (v[l] - 0.5 * S) / D[l];
   }
 }
 
 // qp += delQ in conservative variables (src/lusgs.f90:456-486, 996-1022)
 template <int NV>
 __global__ void __launch_bounds__(256) k_lusgs_apply(Blk b) {
+  using C = Sk<NV>;
   int i, j;
   if (!flat_ij(b.imx - 1, b.jmx - 1, i, j, (int)blockIdx.x)) return;
   i += 1;
   j += 1;
   const int k = blockIdx.y + 1;
   const long c = ci(b, i, j, k), nc = b.nc;
+  const double *dq = sk_at(b, C::NC, i + j + k, C::DQ) + (long)k * b.sk_j + j;
   const double gm = b.gm;
   double q[NV], U[NV];
   lu_load<NV>(b, c, q);
@@ -249,7 +421,7 @@ __global__ void __launch_bounds__(256) k_lusgs_apply(Blk b) {
     U[6] = q[0] * q[6];
   }
 #pragma unroll
-  for (int l = 0; l < NV; ++l) U[l] = U[l] + b.dq[(long)l * nc + c];
+  for (int l = 0; l < NV; ++l) U[l] = U[l] + dq[(long)l * b.sk_p];
   b.qp[c] = U[0];
   b.qp[nc + c] = U[1] / U[0];
   b.qp[2 * nc + c] = U[2] / U[0];
