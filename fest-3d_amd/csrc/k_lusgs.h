@@ -246,7 +246,8 @@ __global__ void __launch_bounds__(256) k_to_skew(Blk b, SkewIn a) {
 template <int NV>
 __global__ void __launch_bounds__(256) k_lusgs_prep(Blk b) {
   using C = Sk<NV>;
-  __shared__ double tile[16][17];
+  constexpr int NT = NV + 1;  // values staged per round: a face's Old flux and radius
+  __shared__ double tile[NT][16][17];
   const int ntx = (b.imx + 15) / 16;
   const int i0 = 1 + ((int)blockIdx.x % ntx) * 16, j0 = 1 + ((int)blockIdx.x / ntx) * 16;
   const int k = 1 + (int)blockIdx.y;
@@ -271,12 +272,6 @@ __global__ void __launch_bounds__(256) k_lusgs_prep(Blk b) {
   const bool out_ok = i0 + oi <= b.imx && j0 + oj <= b.jmx;
   double *const out_base =
       b.sk + (long)(i0 + oi + j0 + oj + k) * C::NC * b.sk_p + (long)k * b.sk_j + (j0 + oj);
-  auto emit = [&](int c, double v) {
-    tile[jj][ii] = v;
-    __syncthreads();
-    if (out_ok) out_base[(long)c * b.sk_p] = tile[oj][oi];
-    __syncthreads();
-  };
   const long c0 = ci(b, i, j, k);
   double Q0[NV], Qn[NV], zero[NV], Old[NV], L[6];
   lu_load<NV>(b, c0, Q0);
@@ -289,10 +284,17 @@ __global__ void __launch_bounds__(256) k_lusgs_prep(Blk b) {
     lu_load<NV>(b, cn, Qn);
     const LuFace f = lu_face(b, n, i, j, k, c0, cn);
     L[n] = lu_radius<NV>(b, Qn, Q0, f, b.cx[cn], b.cy[cn], b.cz[cn], b.cx[c0], b.cy[c0], b.cz[c0]);
-    if (n < 3) emit(C::LAM + n, L[n]);
     lu_flux<NV>(b, Qn, Q0, zero, f, Old);
-#pragma unroll 1
-    for (int l = 0; l < NV; ++l) emit(C::OLD + n * NV + l, Old[l]);
+#pragma unroll
+    for (int l = 0; l < NV; ++l) tile[l][jj][ii] = Old[l];
+    tile[NV][jj][ii] = L[n];
+    __syncthreads();
+    if (out_ok) {
+#pragma unroll
+      for (int l = 0; l < NV; ++l) out_base[(long)(C::OLD + n * NV + l) * b.sk_p] = tile[l][oj][oi];
+      if (n < 3) out_base[(long)(C::LAM + n) * b.sk_p] = tile[NV][oj][oi];
+    }
+    __syncthreads();
   }
   double dg = 0.0, d5 = 0.0, d6 = 0.0;
   if (inner) {
@@ -306,10 +308,16 @@ __global__ void __launch_bounds__(256) k_lusgs_prep(Blk b) {
       d6 = (dg + 2.0 * beta * w * vol);
     }
   }
-  emit(C::DG, dg);
-  if constexpr (NV == 7) {
-    emit(C::DG + 1, d5);
-    emit(C::DG + 2, d6);
+  tile[0][jj][ii] = dg;
+  tile[1][jj][ii] = d5;
+  tile[2][jj][ii] = d6;
+  __syncthreads();
+  if (out_ok) {
+    out_base[(long)C::DG * b.sk_p] = tile[0][oj][oi];
+    if constexpr (NV == 7) {
+      out_base[(long)(C::DG + 1) * b.sk_p] = tile[1][oj][oi];
+      out_base[(long)(C::DG + 2) * b.sk_p] = tile[2][oj][oi];
+    }
   }
 }
 
