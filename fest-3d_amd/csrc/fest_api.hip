@@ -454,7 +454,9 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
   h->dims = *dims; h->scheme = *scheme; h->flow = *flow; h->bc = *bc;
   auto bad = [&](const char *msg) { h->err = msg; *out = h; return FEST_E_CONFIG; };
   if (precision != 0) return bad("only the plain FP64 path (precision 0) is built");
-  if (dims->imx < 4 || dims->jmx < 4 || dims->kmx < 4) return bad("each direction needs at least 3 cells");
+  // a 2-D grid has one cell in k (kmx == 2), src/face/flux/convective/*.f90
+  if (dims->imx < 4 || dims->jmx < 4 || (dims->kmx < 4 && dims->kmx != 2))
+    return bad("i and j need at least 3 cells, k at least 3 or exactly 1");
   int nv_expect = scheme->turb == FEST_TURB_SST ? 7 : 5;
   if (scheme->turb != FEST_TURB_NONE && scheme->turb != FEST_TURB_SST) return bad("turbulence model not supported");
   if (dims->n_var != nv_expect) return bad("n_var does not match the turbulence model");

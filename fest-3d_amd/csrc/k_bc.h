@@ -289,20 +289,26 @@ __global__ void k_bc_face(Blk b, int face0, int current_iter) {
       break;
     }
     case -9: {  // single-block periodic: whole plane, three layers
+      // the reference's array assignment reads its source layers whole
+      // before writing; with one or two cells in the direction they overlap
+      // the destination layers
       for (int l = 0; l < b.nv; ++l) {
         double *v = q + l * nc;
+        long dst[3];
+        double val[3];
         for (int g = 0; g < 3; ++g) {
-          long dst, src;
+          long src;
           switch (face) {
-            case 0: dst = ci(b, -2 + g, a, bb); src = ci(b, b.imx - 3 + g, a, bb); break;
-            case 1: dst = ci(b, b.imx + g, a, bb); src = ci(b, 1 + g, a, bb); break;
-            case 2: dst = ci(b, a, -2 + g, bb); src = ci(b, a, b.jmx - 3 + g, bb); break;
-            case 3: dst = ci(b, a, b.jmx + g, bb); src = ci(b, a, 1 + g, bb); break;
-            case 4: dst = ci(b, a, bb, -2 + g); src = ci(b, a, bb, b.kmx - 3 + g); break;
-            default: dst = ci(b, a, bb, b.kmx + g); src = ci(b, a, bb, 1 + g); break;
+            case 0: dst[g] = ci(b, -2 + g, a, bb); src = ci(b, b.imx - 3 + g, a, bb); break;
+            case 1: dst[g] = ci(b, b.imx + g, a, bb); src = ci(b, 1 + g, a, bb); break;
+            case 2: dst[g] = ci(b, a, -2 + g, bb); src = ci(b, a, b.jmx - 3 + g, bb); break;
+            case 3: dst[g] = ci(b, a, b.jmx + g, bb); src = ci(b, a, 1 + g, bb); break;
+            case 4: dst[g] = ci(b, a, bb, -2 + g); src = ci(b, a, bb, b.kmx - 3 + g); break;
+            default: dst[g] = ci(b, a, bb, b.kmx + g); src = ci(b, a, bb, 1 + g); break;
           }
-          v[dst] = v[src];
+          val[g] = v[src];
         }
+        for (int g = 0; g < 3; ++g) v[dst[g]] = val[g];
       }
       break;
     }

@@ -296,10 +296,12 @@ __device__ __forceinline__ void inviscid_flux(const Blk &b, const double (&qL)[N
 // what the reference subtracts from the inviscid flux of a face between
 // cells cm and c (gradient indices gL, gR) with unit normal n and area A;
 // the laminar and SST parts of one component are summed before the
-// subtraction
+// subtraction.  laminar_only keeps the laminar part alone: the k faces of a
+// 2-D grid, where the reference skips the SST viscous flux
+// (src/viscous.f90:101-105)
 template <int NV>
 __device__ __forceinline__ void visc_face(const Blk &b, long c, long cm, long gL, long gR, double nx, double ny,
-                                          double nz, double A, double (&v)[NV]) {
+                                          double nz, double A, bool laminar_only, double (&v)[NV]) {
   const double *qp = b.qp;
   const long nc = b.nc;
   double delx = b.cx[c] - b.cx[cm], dely = b.cy[c] - b.cy[cm], delz = b.cz[c] - b.cz[cm];
@@ -326,7 +328,7 @@ __device__ __forceinline__ void visc_face(const Blk &b, long c, long cm, long gL
     gr[l][2] = gr[l][2] + ncmp * delz;
   }
   double mu_f = 0.5 * (b.mu[cm] + b.mu[c]);
-  double mut_f = (NV == 7) ? 0.5 * (b.mut[cm] + b.mut[c]) : 0.0;
+  double mut_f = (NV == 7 && !laminar_only) ? 0.5 * (b.mut[cm] + b.mut[c]) : 0.0;
   double tm = mu_f + mut_f;
   double dudx = gr[0][0], dudy = gr[0][1], dudz = gr[0][2];
   double dvdx = gr[1][0], dvdy = gr[1][1], dvdz = gr[1][2];
@@ -350,6 +352,10 @@ __device__ __forceinline__ void visc_face(const Blk &b, long c, long cm, long gL
   v[4] = A * (((Txx * uf + Txy * vf + Txz * wf + Qx) * nx) + ((Txy * uf + Tyy * vf + Tyz * wf + Qy) * ny) +
               ((Txz * uf + Tyz * vf + Tzz * wf + Qz) * nz));
   if constexpr (NV == 7) {
+    v[5] = 0.0;
+    v[6] = 0.0;
+  }
+  if (NV == 7 && !laminar_only) {
     // SST k / omega diffusion and the -2/3 rho k stress, src/viscous.f90:328-447
     double gt[2][3];
 #pragma unroll
@@ -377,8 +383,8 @@ __device__ __forceinline__ void visc_face(const Blk &b, long c, long cm, long gL
     v[3] = v[3] + Tkk * nz * A;
     double kfl = A * ((mu_f + sigma_kf * mut_f) * (gt[0][0] * nx + gt[0][1] * ny + gt[0][2] * nz));
     v[4] = v[4] + kfl;
-    v[5] = kfl;
-    v[6] = A * ((mu_f + sigma_wf * mut_f) * (gt[1][0] * nx + gt[1][1] * ny + gt[1][2] * nz));
+    v[NV == 7 ? 5 : 0] = kfl;
+    v[NV == 7 ? 6 : 0] = A * ((mu_f + sigma_wf * mut_f) * (gt[1][0] * nx + gt[1][1] * ny + gt[1][2] * nz));
   }
 }
 
@@ -408,6 +414,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   const long cm = c - s;
   constexpr int nv = NV;
   const long o = fli(b, D, i, j, k);
+  // k faces of a 2-D grid (one cell in k): no convective flux
+  // (src/face/flux/convective/ausm.f90:205-207 and alike in every scheme)
+  const bool flat = D == 2 && b.kmx == 2;
   double out[NV];
   double csf = 0.0;
   const long fo = fi(b, D, i, j, k);
@@ -421,11 +430,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     }
     if (store_csnd) csf = 0.5 * (sqrt(b.gm * qL[4] / qL[0]) + sqrt(b.gm * qR[4] / qR[0]));
     const double zero = (n == 1) ? (double)b.mz[D][0] : (n == N) ? (double)b.mz[D][1] : 1.0;
-    inviscid_flux(b, qL, qR, nx, ny, nz, A, zero, out);
+    if (flat) {
+#pragma unroll
+      for (int l = 0; l < nv; ++l) out[l] = 0.0;
+    } else {
+      inviscid_flux(b, qL, qR, nx, ny, nz, A, zero, out);
+    }
   }
   if (b.mu_ref != 0.0) {
     double v[NV];
-    visc_face<NV>(b, c, cm, gri(b, i - ii, j - jj, k - kk), gri(b, i, j, k), nx, ny, nz, A, v);
+    visc_face<NV>(b, c, cm, gri(b, i - ii, j - jj, k - kk), gri(b, i, j, k), nx, ny, nz, A, flat, v);
 #pragma unroll
     for (int l = 1; l < nv; ++l) out[l] = out[l] - v[l];
   }

@@ -1,0 +1,207 @@
+"""Readers for a FEST-3D case directory, so that the reference's own cases
+run on the engine as they ship:
+
+  system/control.md, fvscheme.md, flow.md   src/read_write/read/read.f90
+  system/mesh/layout/layout.md             src/layout.f90 (one block per line)
+  system/mesh/gridfiles/grid_XX.txt        src/grid.f90:60-140
+  system/mesh/layout/mapping.txt           src/mapping.f90:64-258
+  system/mesh/bc/bc_XX.md                  src/boundary/read_bc.f90:16-190
+  time_directories/NNNN/process_XX.dat     src/read_write/write/write_output_tec.f90:244-305
+
+The .md files are read by their "## title" lines, the value on the next
+non-empty line; the reference reads them in a fixed order, and the titles
+are what that order means.  Fixed boundary values default to the free
+stream, as read_bc.f90's set_value does when a line carries no number.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import os
+from typing import Dict, List, Tuple
+
+import numpy as np
+
+from .cases import Case
+from .multiblock import Link
+
+_FACES = ("imn", "imx", "jmn", "jmx", "kmn", "kmx")
+
+
+def read_md(path: str) -> Dict[str, List[str]]:
+    """{title: [value lines]} of a system/*.md file."""
+    out: Dict[str, List[str]] = {}
+    key = None
+    with open(path) as f:
+        for line in f:
+            s = line.strip()
+            if s.startswith("##"):
+                key = s[2:].strip()
+                out[key] = []
+            elif s and key is not None and not s.startswith("="):
+                out[key].append(s)
+    return out
+
+
+def _first(md: Dict[str, List[str]], title: str) -> str:
+    return md[title][0].split()[0]
+
+
+def read_layout(path: str) -> List[Tuple[str, str, Tuple[int, ...]]]:
+    """[(grid file, bc file, (imin, imax, jmin, jmax, kmin, kmax) ids)]"""
+    blocks = []
+    with open(path) as f:
+        for line in f:
+            s = line.split()
+            if len(s) == 9 and not line.lstrip().startswith("#") and s[1].startswith("grid"):
+                blocks.append((s[1], s[2], tuple(int(x) for x in s[3:9])))
+    return blocks
+
+
+def read_grid(path: str):
+    """(imx, jmx, kmx, nodes (3, kmx, jmx, imx)); the file holds a header
+    line "imx jmx kmx" and then x y z per node, i fastest."""
+    with open(path) as f:
+        imx, jmx, kmx = (int(x) for x in f.readline().split()[:3])
+        a = np.loadtxt(f, dtype=np.float64, max_rows=imx * jmx * kmx)
+    return imx, jmx, kmx, a.T.reshape(3, kmx, jmx, imx)
+
+
+def padded_nodes(imx: int, jmx: int, kmx: int, nodes: np.ndarray) -> np.ndarray:
+    """Interior nodes placed in the (-2:imx+3,...) node array; the ghost
+    nodes are left for the engine's geometry setup (src/grid.f90:137-236)."""
+    out = np.zeros((3, kmx + 6, jmx + 6, imx + 6))
+    out[:, 3:kmx + 3, 3:jmx + 3, 3:imx + 3] = nodes
+    return out
+
+
+def read_mapping(path: str) -> List[Link]:
+    """One link per map line: block b1's face f1 receives from block b2's face
+    f2 over b2's node ranges with the direction switch (mapping.f90:121-171)."""
+    links = []
+    with open(path) as f:
+        for line in f:
+            s = line.split()
+            if not s or s[0].startswith("#"):
+                continue
+            v = [int(x) for x in s[:14]]
+            b1, f1, _, _, _, _, b2, f2, s21, e21, s22, e22, switch = v[:13]
+            links.append(Link(b1, f1, b2, f2, s21, e21, s22, e22, switch))
+    return links
+
+
+def read_bc(path: str, case: Case) -> Case:
+    """Fixed values of the six faces (read_bc.f90:16-190)."""
+    vals = {k: list(getattr(case, k)) for k in ("fix_rho", "fix_u", "fix_v", "fix_w", "fix_p", "fix_tk",
+                                                  "fix_tw", "fix_twall")}
+    inf = dict(fix_rho=case.rho_inf, fix_u=case.u_inf, fix_v=case.v_inf, fix_w=case.w_inf, fix_p=case.p_inf,
+               fix_tk=case.tk_inf, fix_tw=case.tw_inf, fix_twall=0.0)
+    for k in vals:  # fill_fixed_values: the free stream everywhere
+        vals[k] = [inf[k]] * 6
+    names = {"FIX_DENSITY": "fix_rho", "FIX_X_SPEED": "fix_u", "FIX_Y_SPEED": "fix_v", "FIX_Z_SPEED": "fix_w",
+             "FIX_PRESSURE": "fix_p", "WALL_TEMPERATURE": "fix_twall", "FIX_tk": "fix_tk", "FIX_tw": "fix_tw"}
+    face = -1
+    with open(path) as f:
+        for line in f:
+            s = line.strip()
+            if s.startswith("#"):
+                face += 1
+            elif s.startswith("- ") and 0 <= face < 6:
+                parts = s[2:].split()
+                key = names.get(parts[0])
+                if key and len(parts) > 1:
+                    try:
+                        vals[key][face] = float(parts[1])
+                    except ValueError:
+                        pass
+    return dataclasses.replace(case, **{k: tuple(v) for k, v in vals.items()})
+
+
+def read_case(root: str):
+    """The blocks of a case directory: [(Case, padded nodes)], links, and the
+    control values {CFL, max_iter, ...}."""
+    sysd = os.path.join(root, "system")
+    control = read_md(os.path.join(sysd, "control.md"))
+    scheme = read_md(os.path.join(sysd, "fvscheme.md"))
+    flow = read_md(os.path.join(sysd, "flow.md"))
+    lim = [int(x) for x in " ".join(scheme["Switch: Limiter - Pressure based switch"]).split()]
+    tlim = [int(x) for x in " ".join(scheme["Turbulence Limiter Switch"]).split()]
+    turb = _first(scheme, "Turbulence model")
+    ts = scheme["Time Step"][0].split()
+    rho, u, v, w, p = (float(_first(flow, t)) for t in ("Free Stream Density", "Free Stream X-Speed",
+                                                         "Free Stream Y-Speed", "Free Stream Z-Speed",
+                                                         "Free Stream Pressure"))
+    gm, R = float(_first(flow, "Specific heat ratio")), float(_first(flow, "Gas Constant"))
+    prs = flow["Prandtl numbers"][0].split()
+    # src/state.f90:75-133
+    vel = math.sqrt(u ** 2 + v ** 2 + w ** 2)
+    mu_ref = float(_first(flow, "Reference Viscosity"))
+    base = Case(imx=2, jmx=2, kmx=2, flux=_first(scheme, "Inviscid Flux Scheme"),
+                recon=_first(scheme, "Higher Order Method"), turb=turb,
+                time_local=1 if ts[0] == "l" else 0,
+                global_time_step=float(ts[1]) if len(ts) > 1 else -1.0,
+                time_accuracy=_first(scheme, "Time Integration Method"),
+                accur=int(_first(scheme, "Higher Order Boundary Conditions")), lim=tuple(lim[:3]),
+                tlim=tuple(tlim[:3]), mu_sutherland=int(_first(flow, "Viscosity law") == "sutherland_law"),
+                CFL=float(_first(control, "CFL")), rho_inf=rho, u_inf=u, v_inf=v, w_inf=w, p_inf=p,
+                vel_mag=vel, MInf=vel / math.sqrt(gm * p / rho), gm=gm, R_gas=R, mu_ref=mu_ref,
+                T_ref=float(_first(flow, "Reference Temperature")),
+                S_ref=float(_first(flow, "Sutherland Temperature")), Pr=float(prs[0]), tPr=float(prs[1]))
+    if turb == "sst":
+        tu = float(_first(flow, "Free Stream Turbulence Intensity")) / 100
+        ratio = float(_first(flow, "Free Stream Viscosity Ratio"))
+        tk = 1.5 * ((vel * tu) ** 2)
+        base = dataclasses.replace(base, tk_inf=tk, tw_inf=rho * tk / (mu_ref * ratio))
+    mesh = os.path.join(sysd, "mesh")
+    blocks = []
+    for b, (gfile, bcfile, ids) in enumerate(read_layout(os.path.join(mesh, "layout", "layout.md"))):
+        imx, jmx, kmx, nodes = read_grid(os.path.join(mesh, "gridfiles", gfile))
+        c = dataclasses.replace(base, imx=imx, jmx=jmx, kmx=kmx, bc_id=ids, name=f"{os.path.basename(root)}_{b}")
+        c = read_bc(os.path.join(mesh, "bc", bcfile), c)
+        blocks.append((c, padded_nodes(imx, jmx, kmx, nodes)))
+    links = read_mapping(os.path.join(mesh, "layout", "mapping.txt"))
+    ctl = dict(CFL=float(_first(control, "CFL")), max_iter=int(_first(control, "Maximum Iterations")))
+    return blocks, links, ctl
+
+
+def read_tecplot(path: str, case: Case):
+    """(nodes (3, kmx, jmx, imx), qp) of a block-packed Tecplot dump of
+    x y z (nodal) and u v w Density Pressure [tk tw] (cell centred); qp is
+    the padded primitive array with the dumped interior cells and zero
+    ghosts."""
+    with open(path) as f:
+        lines = f.read().split("\n")
+    start = 0
+    for n, line in enumerate(lines):
+        tok = line.split()
+        if tok:
+            try:
+                float(tok[0])
+                start = n
+                break
+            except ValueError:
+                continue
+    a = np.array(" ".join(lines[start:]).split(), dtype=np.float64)
+    imx, jmx, kmx = case.imx, case.jmx, case.kmx
+    nn, ncl = imx * jmx * kmx, (imx - 1) * (jmx - 1) * (kmx - 1)
+    nodes = a[:3 * nn].reshape(3, kmx, jmx, imx)
+    cells = a[3 * nn:3 * nn + case.n_var * ncl].reshape(case.n_var, kmx - 1, jmx - 1, imx - 1)
+    qp = np.zeros(case.cell_shape(case.n_var))
+    order = [3, 0, 1, 2, 4, 5, 6][:case.n_var]  # rho, u, v, w, p, tk, tw from u v w rho p tk tw
+    for l, src in enumerate(order):
+        qp[l, 3:kmx + 2, 3:jmx + 2, 3:imx + 2] = cells[src]
+    return nodes, qp
+
+
+def entropy_error(states, volumes, case: Case) -> float:
+    """Relative change of entropy over the grid, the measure of the smooth
+    bump test (tests/SmoothBump/pp/entropy.py:5-30):
+    sqrt(sum_blocks |(s - s_inf) V / s_inf|^2 / sum V), s = p / rho^gamma."""
+    s_inf = case.p_inf / case.rho_inf ** case.gm
+    err, vol = [], []
+    for q, v in zip(states, volumes):
+        rho, p = q[0, 3:-3, 3:-3, 3:-3], q[4, 3:-3, 3:-3, 3:-3]
+        s = p / np.power(rho, case.gm)
+        err.append(np.sqrt(np.sum(np.power((s - s_inf) * v / s_inf, 2))))
+        vol.append(np.sum(v))
+    return float(np.sqrt(sum(np.power(np.array(err), 2)) / sum(vol)))

@@ -637,11 +637,16 @@ static void flow_tangency(orc *s, int face) {
     }
 }
 
-/* src/boundary/bc_primitive.f90:1948-1979 */
+/* src/boundary/bc_primitive.f90:1948-1979.  The array assignments read
+ * their right-hand side whole before writing; with one or two cells in a
+ * direction the ranges overlap, so the min faces copy layers in ascending
+ * order and the max faces in descending order, which reads every source
+ * layer before it is overwritten. */
 static void periodic_bc(orc *s, int face) {
   for (int l = 0; l < s->nv; ++l) {
     double *q = QPL(l);
-    for (int g = 0; g < 3; ++g) {
+    for (int gg = 0; gg < 3; ++gg) {
+      int g = (face & 1) ? 2 - gg : gg;
       switch (face) {
         case 0: for (int k = -2; k <= KMX + 2; ++k) for (int j = -2; j <= JMX + 2; ++j) q[CI(s, -2 + g, j, k)] = q[CI(s, IMX - 3 + g, j, k)]; break;
         case 1: for (int k = -2; k <= KMX + 2; ++k) for (int j = -2; j <= JMX + 2; ++j) q[CI(s, IMX + g, j, k)] = q[CI(s, 1 + g, j, k)]; break;

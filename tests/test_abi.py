@@ -43,6 +43,7 @@ def test_struct_sizes_match_header():
     (dict(bc_id=(-11, -9, -5, -5, -9, -9)), -3),
     (dict(bc_id=(-12, -9, -5, -5, -9, -9)), -3),
     (dict(time_accuracy="RK2", bc_id=(-9, -9, -5, -5, -9, -9), imx=3), -3),
+    (dict(kmx=3), -3),  # two cells in k: neither 3-D nor the reference's 2-D
 ])
 def test_config_errors(change, code):
     case, _, _ = C.laminar_box(8, 6, 5)
@@ -60,3 +61,13 @@ def test_precision_modes():
     with pytest.raises(native.FestError) as e:
         native.GpuBlock(case, precision=7)
     assert e.value.code == -3
+
+
+def test_two_dimensional_extent_passes_the_configuration_check():
+    """One cell in k (kmx == 2) is the reference's 2-D mode; without a GPU
+    the handle fails on the device, after the configuration check."""
+    case, _, _ = C.laminar_box(8, 6, 1)
+    try:
+        native.GpuBlock(case).close()
+    except native.FestError as e:
+        assert e.code == -4, e

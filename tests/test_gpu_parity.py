@@ -254,3 +254,29 @@ def test_nonphysical_state_is_reported():
     with pytest.raises(FestError) as e:
         g.get_next_solution(1)
     assert e.value.code in (-1, -2)
+
+
+@pytest.mark.parametrize("rule", ["RK4", "implicit"])
+@pytest.mark.parametrize("kind", ["inviscid", "laminar", "sst"])
+def test_two_dimensional_grids(rule, kind):
+    """One cell in k (kmx == 2, the reference's 2-D mode): no convective k
+    flux, no z gradients, no SST viscous k flux; two steps."""
+    if kind == "sst":
+        case, nodes, q = C.sst_bump(20, 12, 1)
+        wall = True
+    else:
+        case, _, q = C.laminar_box(16, 12, 1)
+        case = dataclasses.replace(case, bc_id=(-8, -4, -6, -6, -6, -6), fix_p=(case.p_inf,) * 6,
+                                   mu_ref=0.0 if kind == "inviscid" else case.mu_ref)
+        nodes = C.cartesian_nodes(case, warp=0.04)
+        wall = False
+    case = dataclasses.replace(case, time_accuracy=rule)
+    say("2-D", rule, kind)
+    g, o = pair(case, nodes, q, wall=wall)
+    for it in range(1, 3):
+        assert o.step(it) == 0
+        g.get_next_solution(it)
+    out = g.download("qp", "residue", "H")
+    assert relerr(out["qp"][:, 3:-3, 3:-3, 3:-3], o.arr("qp")[:, 3:-3, 3:-3, 3:-3]) <= TOL
+    assert relerr(out["residue"], o.arr("res")) <= 1e-10
+    assert not out["H"][0].any()
