@@ -442,6 +442,63 @@ int run_op(fest_handle *h, const StepOp &op, int current_iter) {
   }
 }
 
+// surface nodes of the no-slip faces of one block, in the order of
+// src/wall/wall.f90:198-288, rounded like the ES18.10E3 surface-node file
+// the reference writes and reads back (src/wall/wall.f90:101)
+int wall_nodes(fest_handle *h, std::vector<double> &wall) {
+  Blk &b = h->b;
+  long nn = (long)(b.imx + 6) * (b.jmx + 6) * (b.kmx + 6);
+  std::vector<double> hn(3 * nn);
+  HIPCHK(h, hipMemcpyAsync(hn.data(), b.nodes, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (int f = 0; f < 6; ++f) {
+    if (b.bc[f] != -5) continue;
+    int im = 1, ix = b.imx, jm = 1, jx = b.jmx, km = 1, kx = b.kmx;
+    switch (f) {
+      case 0: ix = 1; break;
+      case 1: im = b.imx; break;
+      case 2: jx = 1; break;
+      case 3: jm = b.jmx; break;
+      case 4: kx = 1; break;
+      default: km = b.kmx; break;
+    }
+    for (int k = km; k <= kx; ++k)
+      for (int j = jm; j <= jx; ++j)
+        for (int i = im; i <= ix; ++i)
+          for (int c = 0; c < 3; ++c) {
+            char buf[64];
+            snprintf(buf, sizeof buf, "%.10E", hn[c * nn + ndi(b, i, j, k)]);
+            wall.push_back(strtod(buf, nullptr));
+          }
+  }
+  return 0;
+}
+
+// cell distances of one block to the surface nodes `wall`
+// (src/wall/wall_dist.f90:84-131)
+int dist_from_walls(fest_handle *h, const std::vector<double> &wall) {
+  Blk &b = h->b;
+  long nn = (long)(b.imx + 6) * (b.jmx + 6) * (b.kmx + 6);
+  int nw = (int)(wall.size() / 3);
+  double *dwall = nullptr, *nd = nullptr;
+  hipError_t le = hipMalloc(&dwall, sizeof(double) * std::max<size_t>(3, wall.size()));
+  if (le == hipSuccess) le = hipMalloc(&nd, sizeof(double) * nn);
+  if (le == hipSuccess && nw)
+    le = hipMemcpyAsync(dwall, wall.data(), sizeof(double) * wall.size(), hipMemcpyHostToDevice, h->stream);
+  if (le == hipSuccess) {
+    hipLaunchKernelGGL(k_node_dist, dim3(cdiv(nn, 256)), dim3(256), 0, h->stream, b, (const double *)dwall, nw, nd);
+    dim3 g(cdiv(b.imx + 5, 64), cdiv(b.jmx + 5, 4), (unsigned)(b.kmx + 5));
+    hipLaunchKernelGGL(k_cell_dist, g, dim3(64, 4, 1), 0, h->stream, b, (const double *)nd);
+    le = hipGetLastError();
+  }
+  hipError_t se = hipStreamSynchronize(h->stream);
+  if (le == hipSuccess) le = se;
+  if (dwall) (void)hipFree(dwall);
+  if (nd) (void)hipFree(nd);
+  if (le != hipSuccess) return fail(h, FEST_E_HIP, "wall distance: %s", hipGetErrorString(le));
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -644,49 +701,9 @@ int fest_set_grid(fest_handle *h, const double *nodes, int with_wall_dist) {
   }
   LAUNCHCHK(h);
   if (with_wall_dist) {
-    // surface nodes of the no-slip faces, in the order of src/wall/wall.f90:
-    // 198-288, rounded like the ES18.10E3 text file the reference reads back
-    std::vector<double> hn(3 * nn);
-    HIPCHK(h, hipMemcpyAsync(hn.data(), b.nodes, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
     std::vector<double> wall;
-    for (int f = 0; f < 6; ++f) {
-      if (b.bc[f] != -5) continue;
-      int im = 1, ix = b.imx, jm = 1, jx = b.jmx, km = 1, kx = b.kmx;
-      switch (f) {
-        case 0: ix = 1; break;
-        case 1: im = b.imx; break;
-        case 2: jx = 1; break;
-        case 3: jm = b.jmx; break;
-        case 4: kx = 1; break;
-        default: km = b.kmx; break;
-      }
-      for (int k = km; k <= kx; ++k)
-        for (int j = jm; j <= jx; ++j)
-          for (int i = im; i <= ix; ++i)
-            for (int c = 0; c < 3; ++c) {
-              char buf[64];
-              snprintf(buf, sizeof buf, "%.10E", hn[c * nn + ndi(b, i, j, k)]);
-              wall.push_back(strtod(buf, nullptr));
-            }
-    }
-    int nw = (int)(wall.size() / 3);
-    double *dwall = nullptr, *nd = nullptr;
-    hipError_t le = hipMalloc(&dwall, sizeof(double) * std::max<size_t>(3, wall.size()));
-    if (le == hipSuccess) le = hipMalloc(&nd, sizeof(double) * nn);
-    if (le == hipSuccess && nw)
-      le = hipMemcpyAsync(dwall, wall.data(), sizeof(double) * wall.size(), hipMemcpyHostToDevice, h->stream);
-    if (le == hipSuccess) {
-      hipLaunchKernelGGL(k_node_dist, dim3(cdiv(nn, 256)), dim3(256), 0, h->stream, b, (const double *)dwall, nw, nd);
-      dim3 g(cdiv(b.imx + 5, 64), cdiv(b.jmx + 5, 4), (unsigned)(b.kmx + 5));
-      hipLaunchKernelGGL(k_cell_dist, g, dim3(64, 4, 1), 0, h->stream, b, (const double *)nd);
-      le = hipGetLastError();
-    }
-    hipError_t se = hipStreamSynchronize(h->stream);
-    if (le == hipSuccess) le = se;
-    if (dwall) (void)hipFree(dwall);
-    if (nd) (void)hipFree(nd);
-    if (le != hipSuccess) return fail(h, FEST_E_HIP, "wall distance: %s", hipGetErrorString(le));
+    int rc = wall_nodes(h, wall);
+    if (rc || (rc = dist_from_walls(h, wall))) return rc;
   }
   int rc = check_err(h);
   if (rc) return fail(h, FEST_E_NONPHYSICAL, "non-positive cell volume in the grid");
@@ -995,6 +1012,20 @@ int fest_group_exchange(fest_group *g) {
   int rc = group_exchange(g);
   if (rc) return rc;
   if (hipStreamSynchronize(g->stream) != hipSuccess) return gfail(g, FEST_E_HIP, "exchange failed");
+  return 0;
+}
+
+int fest_group_wall_dist(fest_group *g) {
+  if (!g) return FEST_E_ARG;
+  if (g->broken) return FEST_E_ARG;
+  for (fest_handle *h : g->blocks)
+    if (!h->have_grid) return gfail(g, FEST_E_ARG, "the grid must be set on every block");
+  std::vector<double> wall;
+  int rc;
+  for (fest_handle *h : g->blocks)
+    if ((rc = wall_nodes(h, wall))) return gfail(g, rc, "%s", h->err.c_str());
+  for (fest_handle *h : g->blocks)
+    if ((rc = dist_from_walls(h, wall))) return gfail(g, rc, "%s", h->err.c_str());
   return 0;
 }
 

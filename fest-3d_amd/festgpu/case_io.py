@@ -22,7 +22,7 @@ from typing import Dict, List, Tuple
 
 import numpy as np
 
-from .cases import Case
+from .cases import Case, freestream_state
 from .multiblock import Link
 
 _FACES = ("imn", "imx", "jmn", "jmx", "kmn", "kmx")
@@ -160,37 +160,87 @@ def read_case(root: str):
         c = read_bc(os.path.join(mesh, "bc", bcfile), c)
         blocks.append((c, padded_nodes(imx, jmx, kmx, nodes)))
     links = read_mapping(os.path.join(mesh, "layout", "mapping.txt"))
-    ctl = dict(CFL=float(_first(control, "CFL")), max_iter=int(_first(control, "Maximum Iterations")))
+    ctl = dict(CFL=float(_first(control, "CFL")), max_iter=int(_first(control, "Maximum Iterations")),
+               restart=int(_first(control, "Restart level")))
     return blocks, links, ctl
 
 
 def read_tecplot(path: str, case: Case):
-    """(nodes (3, kmx, jmx, imx), qp) of a block-packed Tecplot dump of
-    x y z (nodal) and u v w Density Pressure [tk tw] (cell centred); qp is
-    the padded primitive array with the dumped interior cells and zero
-    ghosts."""
+    """(nodes (3, kmx, jmx, imx), {name: cell array (kmx-1, jmx-1, imx-1)})
+    of a block-packed Tecplot dump: x y z at the nodes, then the cell-centred
+    variables named in the header (u v w Density Pressure ...)."""
     with open(path) as f:
         lines = f.read().split("\n")
-    start = 0
+    names, start = [], 0
     for n, line in enumerate(lines):
         tok = line.split()
-        if tok:
-            try:
-                float(tok[0])
-                start = n
-                break
-            except ValueError:
-                continue
+        if not tok:
+            continue
+        try:
+            float(tok[0])
+            start = n
+            break
+        except ValueError:
+            pass
+        if tok[0].lower().startswith("zone") or "=" in tok[0] and not tok[0].startswith("variables"):
+            continue
+        if tok[0] == "variables":
+            tok = tok[2:]
+        if not any("=" in t for t in tok):
+            names += tok
     a = np.array(" ".join(lines[start:]).split(), dtype=np.float64)
     imx, jmx, kmx = case.imx, case.jmx, case.kmx
     nn, ncl = imx * jmx * kmx, (imx - 1) * (jmx - 1) * (kmx - 1)
     nodes = a[:3 * nn].reshape(3, kmx, jmx, imx)
-    cells = a[3 * nn:3 * nn + case.n_var * ncl].reshape(case.n_var, kmx - 1, jmx - 1, imx - 1)
-    qp = np.zeros(case.cell_shape(case.n_var))
-    order = [3, 0, 1, 2, 4, 5, 6][:case.n_var]  # rho, u, v, w, p, tk, tw from u v w rho p tk tw
-    for l, src in enumerate(order):
-        qp[l, 3:kmx + 2, 3:jmx + 2, 3:imx + 2] = cells[src]
-    return nodes, qp
+    cells = {}
+    for v, name in enumerate(names[3:]):
+        cells[name] = a[3 * nn + v * ncl:3 * nn + (v + 1) * ncl].reshape(kmx - 1, jmx - 1, imx - 1)
+    return nodes, cells
+
+
+def read_restart_list(path: str) -> List[str]:
+    """The restart variables of system/output_control.md (its second
+    {...} list)."""
+    lists, cur = [], None
+    with open(path) as f:
+        for line in f:
+            s = line.strip()
+            if s.startswith("{"):
+                cur = []
+            elif s.startswith("}"):
+                lists.append(cur)
+                cur = None
+            elif cur is not None and s:
+                cur.append(s)
+    return lists[1] if len(lists) > 1 else []
+
+
+def restart_state(case: Case, path: str, restart_list: List[str]) -> np.ndarray:
+    """The padded state a restart starts from: the free stream everywhere,
+    then the interior cells of the variables in the restart list read from
+    the dump (src/state.f90:178-188, src/read_write/read/read_output_tec.f90:
+    100-136; other variables keep the free stream)."""
+    qp = freestream_state(case)
+    _, cells = read_tecplot(path, case)
+    kmx, jmx, imx = case.kmx, case.jmx, case.imx
+    inner = (slice(3, kmx + 2), slice(3, jmx + 2), slice(3, imx + 2))
+    take = {"Velocity": [(1, "u"), (2, "v"), (3, "w")], "Density": [(0, "Density")], "Pressure": [(4, "Pressure")],
+            "TKE": [(5, "TKE")], "Omega": [(6, "Omega")]}
+    for var in restart_list:
+        for l, name in take.get(var, []):
+            if l < case.n_var:
+                qp[(l,) + inner] = cells[name]
+    return qp
+
+
+def initial_states(root: str, blocks, ctl) -> List[np.ndarray]:
+    """The state each block starts from: the free stream, or the restart
+    dump time_directories/NNNN/process_XX.dat (control.md "Restart level")."""
+    if ctl["restart"] == 0:
+        return [freestream_state(c) for c, _ in blocks]
+    rl = read_restart_list(os.path.join(root, "system", "output_control.md"))
+    return [restart_state(c, os.path.join(root, "time_directories", f"{ctl['restart']:04d}", f"process_{b:02d}.dat"),
+                          rl) for b, (c, _) in enumerate(blocks)]
 
 
 def entropy_error(states, volumes, case: Case) -> float:
@@ -205,3 +255,57 @@ def entropy_error(states, volumes, case: Case) -> float:
         err.append(np.sqrt(np.sum(np.power((s - s_inf) * v / s_inf, 2))))
         vol.append(np.sum(v))
     return float(np.sqrt(sum(np.power(np.array(err), 2)) / sum(vol)))
+
+
+def drag_coefficient(cases, nodes, qps, grads, mus, length: float) -> float:
+    """Drag coefficient of the no-slip j faces, as the flat-plate tests
+    compute it from a solution dump (tests/Lfp/pp/Surface.py:85-160): the
+    viscous stress of the first cell row, with the molecular viscosity and
+    the cell gradients, projected on the face's unit normal, plus the
+    pressure force, over 0.5 rho |V|^2 and the plate length.
+
+    nodes: padded node arrays; qps, mus: padded cell arrays; grads: the
+    gradient arrays (3 directions, n_grad, 0:kmx, 0:jmx, 0:imx), variables
+    u, v, w first."""
+    c0 = cases[0]
+    vel = math.sqrt(c0.u_inf ** 2 + c0.v_inf ** 2 + c0.w_inf ** 2)
+    dyn = 0.5 * c0.rho_inf * vel * vel
+    cd = []
+    for case, nd, qp, g, mu in zip(cases, nodes, qps, grads, mus):
+        imx, jmx, kmx = case.imx, case.jmx, case.kmx
+        X = nd[:, 3:kmx + 3, 3:jmx + 3, 3:imx + 3]  # (3, k, j, i)
+        for face, bc in enumerate(case.bc_id):
+            if bc != -5:
+                continue
+            if face not in (2, 3):
+                raise ValueError("the flat-plate measure covers no-slip j faces only")
+            jf = 0 if face == 2 else jmx - 1        # face node row (0-based)
+            jc = 1 if face == 2 else jmx - 1        # first cell row (cell index)
+            # j-face area vectors from the nodes, tests/geometry.f90:44-73
+            d1 = X[:, 1:, jf, 1:] - X[:, :-1, jf, :-1]
+            d2 = X[:, :-1, jf, 1:] - X[:, 1:, jf, :-1]
+            yn = 0.5 * np.stack([d1[1] * d2[2] - d1[2] * d2[1], d1[2] * d2[0] - d1[0] * d2[2],
+                                 d1[0] * d2[1] - d1[1] * d2[0]])
+            area = np.linalg.norm(yn, axis=0)
+            n = yn / area
+            if face == 3:
+                n = -n
+            nx, ny, nz = n
+            cells = (slice(1, kmx), jc, slice(1, imx))
+            dudx, dudy, dudz = (g[d][0][cells] for d in range(3))
+            dvdx, dvdy, dvdz = (g[d][1][cells] for d in range(3))
+            dwdx, dwdy, dwdz = (g[d][2][cells] for d in range(3))
+            pc = (slice(3, kmx + 2), jc + 2, slice(3, imx + 2))
+            m, p = mu[pc], qp[4][pc]
+            delv = dudx + dvdy + dwdz
+            txx = m * ((dudx + dudx) - 2.0 * delv / 3.0)
+            tyy = m * ((dvdy + dvdy) - 2.0 * delv / 3.0)
+            tzz = m * ((dwdz + dwdz) - 2.0 * delv / 3.0)
+            txy = m * (dudy + dvdx)
+            tyz = m * (dwdy + dvdz)
+            txz = m * (dudz + dwdx)
+            Fx = txx * nx + txy * ny + txz * nz
+            cp = (p - c0.p_inf) / dyn
+            cd.append(np.sum(Fx * area / dyn))
+            cd.append(np.sum(-cp * nx * area))
+    return float(sum(cd) / length)

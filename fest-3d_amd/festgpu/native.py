@@ -87,13 +87,14 @@ def lib():
         L.fest_group_last_error.argtypes = [vp]
         L.fest_group_last_error.restype = ctypes.c_char_p
         L.fest_group_exchange.argtypes = [vp]
+        L.fest_group_wall_dist.argtypes = [vp]
         L.fest_group_get_next_solution.argtypes = [vp, ip, ctypes.c_double]
         L.fest_timing.argtypes = [vp, ctypes.c_char_p]
         L.fest_kernel_time.argtypes = [vp, ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_long)]
         for f in ("fest_create", "fest_upload_geometry", "fest_set_grid", "fest_upload_state",
                   "fest_get_next_solution", "fest_residual", "fest_time_step", "fest_download", "fest_resnorm",
                   "fest_device_ptr", "fest_sync", "fest_timing", "fest_kernel_time", "fest_group_create",
-                  "fest_group_exchange", "fest_group_get_next_solution"):
+                  "fest_group_exchange", "fest_group_wall_dist", "fest_group_get_next_solution"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -284,6 +285,10 @@ class GpuGroup:
     def _check(self, rc):
         if rc != 0:
             raise FestError(rc, lib().fest_group_last_error(self.g).decode())
+
+    def wall_dist(self):
+        """Wall distance of every block to the walls of all blocks."""
+        self._check(lib().fest_group_wall_dist(self.g))
 
     def exchange(self):
         self._check(lib().fest_group_exchange(self.g))

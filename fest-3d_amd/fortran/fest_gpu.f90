@@ -71,7 +71,8 @@ module fest_gpu
   public :: fest_create, fest_destroy, fest_last_error, fest_upload_geometry, fest_set_grid, &
             fest_upload_state, fest_get_next_solution, fest_residual, fest_time_step, &
             fest_download, fest_resnorm, fest_sync, fest_check, &
-            fest_group_create, fest_group_destroy, fest_group_exchange, fest_group_get_next_solution
+            fest_group_create, fest_group_destroy, fest_group_exchange, fest_group_wall_dist, &
+            fest_group_get_next_solution
 
   interface
     integer(c_int) function fest_create(h, dims, scheme, flow, bc, precision) bind(c, name='fest_create')
@@ -177,6 +178,12 @@ module fest_gpu
       import :: c_int, c_ptr
       type(c_ptr), value :: g
     end function fest_group_exchange
+
+    ! wall distance of every block to the no-slip walls of all blocks
+    integer(c_int) function fest_group_wall_dist(g) bind(c, name='fest_group_wall_dist')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: g
+    end function fest_group_wall_dist
 
     integer(c_int) function fest_group_get_next_solution(g, current_iter, cfl) &
         bind(c, name='fest_group_get_next_solution')

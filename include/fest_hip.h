@@ -202,6 +202,12 @@ void fest_group_destroy(fest_group *g);
 const char *fest_group_last_error(const fest_group *g);
 /* apply_interface (src/interface1.f90:96-493) on every link, device to device. */
 int fest_group_exchange(fest_group *g);
+/* Wall distance of every block to the no-slip surface nodes of all blocks
+ * of the group, as the reference's processes all read one shared
+ * surface-node file: write_surfnode + find_wall_dist
+ * (src/wall/wall.f90:55-111, src/wall/wall_dist.f90:84-131).  Call after
+ * fest_set_grid on every block; replaces their single-block distances. */
+int fest_group_wall_dist(fest_group *g);
 /* get_next_solution (src/update.f90:129-226) on every block, stage by stage,
  * the interface exchange opening every residual evaluation
  * (src/update.f90:534). */

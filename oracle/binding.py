@@ -112,6 +112,13 @@ def group_exchange(blocks, links):
     return Oracle.lib().orc_exchange(hs, len(blocks), _links(links), len(links))
 
 
+def group_wall_dist(blocks):
+    """The wall distance of every Oracle block to the no-slip surface nodes
+    of all blocks (src/wall/wall.f90:55-111, src/wall/wall_dist.f90)."""
+    arr = (ctypes.c_void_p * len(blocks))(*[o.h for o in blocks])
+    return Oracle.lib().orc_group_wall_dist(arr, len(blocks))
+
+
 def group_step(blocks, links, current_iter):
     """One time step of every block, the exchange opening each residual."""
     hs = (ctypes.c_void_p * len(blocks))(*[o.h for o in blocks])

@@ -351,11 +351,13 @@ static double es18_10(double x) {
   return strtod(buf, NULL);
 }
 
-int orc_wall_dist(orc *s) {
+typedef struct { size_t n, cap; double *x, *y, *z; } wall_set;
+
+/* src/wall/wall.f90:198-288: the nodes of every no-slip face in face and
+ * loop order, rounded like the ES18.10E3 surface-node file
+ * (src/wall/wall.f90:101) that every process reads back */
+static void wall_nodes(orc *s, wall_set *w) {
   const double *X = s->a[ORC_NODES], *Y = X + s->nnode, *Z = Y + s->nnode;
-  size_t nw = 0, cap = 0;
-  double *wx = NULL, *wy = NULL, *wz = NULL;
-  /* src/wall/wall.f90:198-288: node order per no-slip face */
   for (int f = 0; f < 6; ++f) {
     if (s->c.bc_id[f] != -5) continue;
     int im = 1, ix = IMX, jm = 1, jx = JMX, km = 1, kx = KMX;
@@ -370,26 +372,32 @@ int orc_wall_dist(orc *s) {
     for (int k = km; k <= kx; ++k)
       for (int j = jm; j <= jx; ++j)
         for (int i = im; i <= ix; ++i) {
-          if (nw == cap) {
-            cap = cap ? 2 * cap : 1024;
-            wx = (double *)realloc(wx, cap * sizeof(double));
-            wy = (double *)realloc(wy, cap * sizeof(double));
-            wz = (double *)realloc(wz, cap * sizeof(double));
+          if (w->n == w->cap) {
+            w->cap = w->cap ? 2 * w->cap : 1024;
+            w->x = (double *)realloc(w->x, w->cap * sizeof(double));
+            w->y = (double *)realloc(w->y, w->cap * sizeof(double));
+            w->z = (double *)realloc(w->z, w->cap * sizeof(double));
           }
-          wx[nw] = es18_10(X[NDI(s, i, j, k)]);
-          wy[nw] = es18_10(Y[NDI(s, i, j, k)]);
-          wz[nw] = es18_10(Z[NDI(s, i, j, k)]);
-          ++nw;
+          w->x[w->n] = es18_10(X[NDI(s, i, j, k)]);
+          w->y[w->n] = es18_10(Y[NDI(s, i, j, k)]);
+          w->z[w->n] = es18_10(Z[NDI(s, i, j, k)]);
+          ++w->n;
         }
   }
+}
+
+/* src/wall/wall_dist.f90:84-131: node distance to the nearest surface node,
+ * cell distance the mean of its eight nodes' */
+static void dist_from_walls(orc *s, const wall_set *w) {
+  const double *X = s->a[ORC_NODES], *Y = X + s->nnode, *Z = Y + s->nnode;
   double *nd = (double *)malloc(s->nnode * sizeof(double));
   for (int k = -2; k <= KMX + 3; ++k)
     for (int j = -2; j <= JMX + 3; ++j)
       for (int i = -2; i <= IMX + 3; ++i) {
         size_t q = NDI(s, i, j, k);
         double best = 1.e+20;
-        for (size_t n = 0; n < nw; ++n) {
-          double cd = sqrt(sq(wx[n] - X[q]) + sq(wy[n] - Y[q]) + sq(wz[n] - Z[q]));
+        for (size_t n = 0; n < w->n; ++n) {
+          double cd = sqrt(sq(w->x[n] - X[q]) + sq(w->y[n] - Y[q]) + sq(w->z[n] - Z[q]));
           best = fmn(best, cd);
         }
         nd[q] = best;
@@ -402,7 +410,22 @@ int orc_wall_dist(orc *s) {
                                         nd[NDI(s, i, j + 1, k + 1)] + nd[NDI(s, i, j, k + 1)] +
                                         nd[NDI(s, i + 1, j, k + 1)] + nd[NDI(s, i + 1, j, k)] +
                                         nd[NDI(s, i + 1, j + 1, k)] + nd[NDI(s, i + 1, j + 1, k + 1)]);
-  free(nd); free(wx); free(wy); free(wz);
+  free(nd);
+}
+
+int orc_wall_dist(orc *s) {
+  wall_set w = {0, 0, NULL, NULL, NULL};
+  wall_nodes(s, &w);
+  dist_from_walls(s, &w);
+  free(w.x); free(w.y); free(w.z);
+  return ORC_OK;
+}
+
+int orc_group_wall_dist(orc **blocks, int nblocks) {
+  wall_set w = {0, 0, NULL, NULL, NULL};
+  for (int b = 0; b < nblocks; ++b) wall_nodes(blocks[b], &w);
+  for (int b = 0; b < nblocks; ++b) dist_from_walls(blocks[b], &w);
+  free(w.x); free(w.y); free(w.z);
   return ORC_OK;
 }
 

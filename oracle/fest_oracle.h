@@ -76,6 +76,10 @@ int orc_geometry(orc *s);
  * src/wall/wall_dist.f90:84-131), surface nodes rounded like the
  * ES18.10E3 text round trip of src/wall/wall.f90:101. */
 int orc_wall_dist(orc *s);
+/* The same over a multiblock grid: every block's distance to the no-slip
+ * surface nodes of all blocks, the one surface-node file every process
+ * reads (src/wall/wall.f90:55-111). */
+int orc_group_wall_dist(orc **blocks, int nblocks);
 
 /* Temp = p/(R rho) over the padded array (src/update.f90:170). */
 void orc_temp_refresh(orc *s);
