@@ -1,4 +1,5 @@
-"""Benchmark of the explicit residual + RK4 stage update on one MI355X.
+"""Benchmark of the explicit residual + RK4 stage update on one MI355X
+(and, with --time-accuracy implicit, of the LU-SGS 'implicit' step).
 
 Workload: the 256x128x128-cell (4.2M) RANS SST channel over a 0.05L bump,
 WENO5 + SLAU, tanh wall clustering to 1e-5L, M=0.2, Re=1e6/L, seeded 0.5%
@@ -16,6 +17,12 @@ per launch of that kernel (FETCH_SIZE + WRITE_SIZE, scaled by a copy of
 known size) from the rocprofv3 counter summary committed for this workload,
 or null when there is none.  `stage_roofline` prices a whole RK stage at the
 fused minimum of 406 (SST) / 326 (laminar) bytes per cell.
+
+--time-accuracy implicit times the reference's LU-SGS integrator on the same
+workload instead: a step is one residual evaluation, the time step and the
+LU-SGS update (two sweeps of one launch per hyperplane i+j+k); the metric
+is cell-implicit-steps per second and the CPU baseline times the oracle's
+implicit step.
 """
 import argparse
 import json
@@ -46,14 +53,17 @@ def measured_traffic(case, group):
     return None
 
 
-def cpu_baseline(case_fn, full_dist, budget_s=25.0):
+def cpu_baseline(case_fn, full_dist, budget_s=25.0, implicit=False):
     """The oracle (a single-threaded C restatement of the reference) timed on
     the host on a bounded sample: RK4 steps of the first 64 i-cells of the
     same grid, wall distance taken from the device.  Checker code, used here
     only as the baseline."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from binding import Oracle
+    import dataclasses
     case, nodes, q = case_fn(64, 128, 128)
+    if implicit:
+        case = dataclasses.replace(case, time_accuracy="implicit")
     o = Oracle(case)
     o.set_nodes(nodes)
     d = o.arr("dist")
@@ -67,8 +77,10 @@ def cpu_baseline(case_fn, full_dist, budget_s=25.0):
         if time.perf_counter() - t0 > budget_s or steps >= 4:
             break
     dt = time.perf_counter() - t0
-    return {"value": case.ncells * 4 * steps / dt, "unit": "cell-RK-stages/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} RK4 step(s) of the 64x128x128-cell slab of the same workload "
+    per, unit, what = (1, "cell-implicit-steps/s", "implicit (LU-SGS)") if implicit else \
+        (4, "cell-RK-stages/s", "RK4")
+    return {"value": case.ncells * per * steps / dt, "unit": unit, "cores": 1, "kind": "port",
+            "sample": f"{steps} {what} step(s) of the 64x128x128-cell slab of the same workload "
                       f"({case.ncells} cells), oracle/fest_oracle.c single thread, {dt:.1f} s"}
 
 
@@ -80,6 +92,7 @@ def main():
     ap.add_argument("--workload", default="sst_bump")
     ap.add_argument("--size", default="256x128x128")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--time-accuracy", default="RK4", choices=["RK4", "implicit"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -93,6 +106,10 @@ def main():
     ni, nj, nk = (int(v) for v in args.size.split("x"))
     case_fn = C.WORKLOADS[args.workload]
     case, nodes, q = case_fn(ni, nj, nk)
+    implicit = args.time_accuracy == "implicit"
+    if implicit:
+        import dataclasses
+        case = dataclasses.replace(case, time_accuracy="implicit")
     g = GpuBlock(case)
     g.set_grid(nodes, wall_dist=(case.turb == "sst"))
     g.upload_state(q)
@@ -114,7 +131,7 @@ def main():
     el = time.perf_counter() - t0
     k_ms, k_launches = g.kernel_time(dominant)
     g.timing(None)
-    stages = 4 * args.steps
+    stages = (1 if implicit else 4) * args.steps
     value = case.ncells * stages / el
     ms_step = 1e3 * el / args.steps
     k_avg_s = 1e-3 * k_ms / max(k_launches, 1)
@@ -122,25 +139,28 @@ def main():
     achieved = k_bytes / k_avg_s / 1e9
     bpcs = R.BYTES_PER_CELL_STAGE[case.turb]
     stage_gbps = bpcs * case.ncells / (el / stages) / 1e9
+    unit = "cell-implicit-steps/s" if implicit else "cell-RK-stages/s"
     out = {
-        "metric": "cell-RK-stages/s", "value": value, "unit": "cell-RK-stages/s", "n_gpus": 1,
+        "metric": unit, "value": value, "unit": unit, "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": case.name, "cells": case.ncells, "scheme": f"{case.recon}+{case.flux}",
-                   "turbulence": case.turb, "time": "RK4 local" if case.time_local else "RK4 global",
+                   "turbulence": case.turb,
+                   "time": ("implicit (LU-SGS)" if implicit else "RK4") + (" local" if case.time_local else " global"),
                    "precision": "fp64"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": R.HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": achieved / R.HBM_PEAK_GBPS,
                      "traffic": measured_traffic(case, dominant),
                      "bytes_per_launch": k_bytes, "launches": k_launches, "avg_launch_us": 1e6 * k_avg_s},
-        "stage_roofline": {"bytes_per_cell_stage": bpcs, "achieved": stage_gbps, "peak": R.HBM_PEAK_GBPS,
-                           "unit": "GB/s", "frac": stage_gbps / R.HBM_PEAK_GBPS},
+        "stage_roofline": None if implicit else {"bytes_per_cell_stage": bpcs, "achieved": stage_gbps,
+                                                 "peak": R.HBM_PEAK_GBPS, "unit": "GB/s",
+                                                 "frac": stage_gbps / R.HBM_PEAK_GBPS},
         "kernel_ms_per_step": ({grp: breakdown[grp] / args.warmup for grp in breakdown}
                                if args.warmup > 0 else None),
     }
     if not args.no_cpu_baseline:
         dist = g.download("dist") if case.turb == "sst" else None
-        out["cpu_baseline"] = cpu_baseline(case_fn, dist)
+        out["cpu_baseline"] = cpu_baseline(case_fn, dist, implicit=implicit)
     print(json.dumps(out))
 
 

@@ -188,19 +188,26 @@ int check_err(fest_handle *h) {
 
 // the flux kernel of sweep D, specialised on the variable count and the
 // reconstruction
-template <int D, int NV>
+template <int D, int NV, bool FLAT>
 void launch_flux_nv(const Blk &b, dim3 g, dim3 t, hipStream_t st, int store_csnd) {
   switch (b.recon) {
-    case 0: hipLaunchKernelGGL((k_flux<D, NV, 0>), g, t, 0, st, b, store_csnd); break;
-    case 1: hipLaunchKernelGGL((k_flux<D, NV, 1>), g, t, 0, st, b, store_csnd); break;
-    case 2: hipLaunchKernelGGL((k_flux<D, NV, 2>), g, t, 0, st, b, store_csnd); break;
-    default: hipLaunchKernelGGL((k_flux<D, NV, 3>), g, t, 0, st, b, store_csnd); break;
+    case 0: hipLaunchKernelGGL((k_flux<D, NV, 0, FLAT>), g, t, 0, st, b, store_csnd); break;
+    case 1: hipLaunchKernelGGL((k_flux<D, NV, 1, FLAT>), g, t, 0, st, b, store_csnd); break;
+    case 2: hipLaunchKernelGGL((k_flux<D, NV, 2, FLAT>), g, t, 0, st, b, store_csnd); break;
+    default: hipLaunchKernelGGL((k_flux<D, NV, 3, FLAT>), g, t, 0, st, b, store_csnd); break;
   }
 }
+// the k faces of a 2-D grid (one cell in k) carry no convective flux
 template <int D>
 void launch_flux(const Blk &b, dim3 g, dim3 t, hipStream_t st, int store_csnd) {
-  if (b.nv == 7) launch_flux_nv<D, 7>(b, g, t, st, store_csnd);
-  else launch_flux_nv<D, 5>(b, g, t, st, store_csnd);
+  const bool flat = D == 2 && b.kmx == 2;
+  if (b.nv == 7) {
+    if (flat) launch_flux_nv<D, 7, true>(b, g, t, st, store_csnd);
+    else launch_flux_nv<D, 7, false>(b, g, t, st, store_csnd);
+  } else {
+    if (flat) launch_flux_nv<D, 5, true>(b, g, t, st, store_csnd);
+    else launch_flux_nv<D, 5, false>(b, g, t, st, store_csnd);
+  }
 }
 
 // grid of 64 x 4 tiles over an (ni x nj x nk) box (the kernels use tile_ij)

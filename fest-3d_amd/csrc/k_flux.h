@@ -394,7 +394,7 @@ __device__ __forceinline__ void visc_face(const Blk &b, long c, long cm, long gL
 // counts keep the per-variable arrays in registers.
 // Registers are fitted to 4 waves per SIMD: 5 spills the SST/WENO variants
 // and measured slower even where it does not spill.
-template <int D, int NV, int RECON>
+template <int D, int NV, int RECON, bool FLAT = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_flux(Blk b, int store_csnd) {
   const int ii = D == 0, jj = D == 1, kk = D == 2;
   int i, j;
@@ -414,9 +414,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   const long cm = c - s;
   constexpr int nv = NV;
   const long o = fli(b, D, i, j, k);
-  // k faces of a 2-D grid (one cell in k): no convective flux
+  // FLAT: the k faces of a 2-D grid (one cell in k), no convective flux
   // (src/face/flux/convective/ausm.f90:205-207 and alike in every scheme)
-  const bool flat = D == 2 && b.kmx == 2;
+  constexpr bool flat = FLAT;
   double out[NV];
   double csf = 0.0;
   const long fo = fi(b, D, i, j, k);
@@ -430,9 +430,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     }
     if (store_csnd) csf = 0.5 * (sqrt(b.gm * qL[4] / qL[0]) + sqrt(b.gm * qR[4] / qR[0]));
     const double zero = (n == 1) ? (double)b.mz[D][0] : (n == N) ? (double)b.mz[D][1] : 1.0;
-    if (flat) {
+    if constexpr (flat) {
 #pragma unroll
       for (int l = 0; l < nv; ++l) out[l] = 0.0;
+      (void)zero;
     } else {
       inviscid_flux(b, qL, qR, nx, ny, nz, A, zero, out);
     }

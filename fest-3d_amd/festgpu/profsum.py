@@ -37,7 +37,8 @@ GROUPS = [  # (substring of the demangled kernel name, kernel group)
     ("k_dt_min", "time_step"), ("k_dt_fill", "time_step"), ("k_dt", "time_step"),
     ("k_norm", "resnorm"), ("k_node_dist", "setup"), ("k_cell_dist", "setup"), ("k_cells", "setup"),
     ("k_face_metrics", "setup"), ("k_ghost_nodes", "setup"), ("k_fill", "setup"),
-    ("k_copy_probe", "copy_probe"),
+    ("k_copy_probe", "copy_probe"), ("k_lusgs_sweep", "lusgs_sweep"), ("k_lusgs_prep", "lusgs_diag"),
+    ("k_to_skew", "lusgs_diag"), ("k_lusgs_apply", "lusgs_apply"), ("k_exchange", "exchange"),
 ]
 
 

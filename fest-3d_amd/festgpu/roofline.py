@@ -24,6 +24,23 @@ fest-3d_amd/csrc:
           cell), the RK base state (n_var), dt, volume, the R accumulator
           (stages 2-4), the SST source (2), F1 and omega (SST)
     write q (n_var), the R accumulator (stages 1-3), the residue (stage 4)
+  lusgs_sweep (k_lusgs_sweep, the LU-SGS update of the 'implicit' step: one
+    launch per hyperplane i+j+k = s, 2 (ni+nj+nk-2) launches a step, which
+    the engine's event timer brackets as one; priced per bracket, i.e. both
+    sweeps over every interior cell), per interior cell and sweep, from the
+    hyperplane-major storage:
+    read  q (n_var), three faces x {A,nx,ny,nz}, volume, mu, mu_t, F1, three
+          radii, three Old fluxes (3 x n_var), the diagonal (1, or 3 with
+          SST), the right-hand side (residue or own value, n_var), the
+          neighbours' sweep values (n_var, each value once)
+    write the sweep value (n_var)
+  lusgs_diag (k_to_skew + k_lusgs_prep, once a step), per interior cell:
+    the copies of q, mu, mu_t, F1 and the residue into hyperplane-major
+    storage (read + write, 2 x (2 n_var + 3)); the prep's stencil reads (q,
+    six faces x 4 metrics counted once a face, volume, mu, mu_t, F1, centres,
+    dt: n_var + 12 + 8) and writes (three radii, six Old fluxes, diagonal:
+    3 + 6 n_var + 1 or 3)
+  lusgs_apply (k_lusgs_apply): read q and the sweep value, write q (3 n_var)
 """
 
 W = 8  # bytes per FP64 value
@@ -33,15 +50,25 @@ def _ngrad(case):
     return 6 if case.turb == "sst" else 4
 
 
+def lusgs_launches(case):
+    """Hyperplane launches of the two LU-SGS sweeps of one step."""
+    return 2 * ((case.imx - 1) + (case.jmx - 1) + (case.kmx - 1) - 2)
+
+
 def launch_items(case, group):
-    """Number of faces or cells one launch of `group` processes."""
+    """Number of faces or cells one timed launch of `group` processes (for
+    the sweep, cell-sweeps of both sweeps of a step)."""
     I, J, K = case.imx, case.jmx, case.kmx
+    if group == "lusgs_sweep":
+        return 2 * (I - 1) * (J - 1) * (K - 1)
     return {
         "flux_i": I * (J - 1) * (K - 1),
         "flux_j": (I - 1) * J * (K - 1),
         "flux_k": (I - 1) * (J - 1) * K,
         "grad": (I + 1) * (J + 1) * (K + 1),
         "update": (I - 1) * (J - 1) * (K - 1),
+        "lusgs_diag": (I - 1) * (J - 1) * (K - 1),
+        "lusgs_apply": (I - 1) * (J - 1) * (K - 1),
     }.get(group)
 
 
@@ -66,6 +93,15 @@ def bytes_per_item(case, group):
             base += 2 + 2  # source, F1 + omega
         acc = (nv + 2 * nv + 2 * nv + 2 * nv) / 4.0  # R: write; read+write x2; read + residue write
         return W * (base + acc)
+    ndg = 3 if sst else 1
+    if group == "lusgs_sweep":
+        reads = nv + 12 + 4 + 3 + 3 * nv + ndg + nv + nv
+        return W * (reads + nv)
+    if group == "lusgs_diag":
+        copies = 2 * (2 * nv + 3)
+        return W * (copies + (nv + 12 + 8) + (3 + 6 * nv + ndg))
+    if group == "lusgs_apply":
+        return W * 3 * nv
     return None
 
 

@@ -166,7 +166,7 @@ def test_gpu_lusgs_group():
 
 @pytest.mark.gpu
 def test_gpu_lusgs_full_size_sst_bump():
-    """One implicit step on the 256x128x128 SST bump grid (4.2M cells, 507
+    """One implicit step on the 256x128x128 SST bump grid (4.2M cells, 510
     hyperplanes each way) against the oracle; the wall distance comes from
     the device (its parity is pinned on smaller grids)."""
     from festgpu.native import GpuBlock
