@@ -111,7 +111,7 @@ def main():
         import dataclasses
         case = dataclasses.replace(case, time_accuracy="implicit")
     g = GpuBlock(case)
-    g.set_grid(nodes, wall_dist=(case.turb == "sst"))
+    g.set_grid(nodes, wall_dist=case.sst_family)
     g.upload_state(q)
     it = 1
     g.timing("*")
@@ -137,7 +137,7 @@ def main():
     k_avg_s = 1e-3 * k_ms / max(k_launches, 1)
     k_bytes = R.bytes_per_launch(case, dominant)
     achieved = k_bytes / k_avg_s / 1e9
-    bpcs = R.BYTES_PER_CELL_STAGE[case.turb]
+    bpcs = R.BYTES_PER_CELL_STAGE["sst" if case.sst_family else "none"]
     stage_gbps = bpcs * case.ncells / (el / stages) / 1e9
     unit = "cell-implicit-steps/s" if implicit else "cell-RK-stages/s"
     out = {
@@ -159,7 +159,7 @@ def main():
                                if args.warmup > 0 else None),
     }
     if not args.no_cpu_baseline:
-        dist = g.download("dist") if case.turb == "sst" else None
+        dist = g.download("dist") if case.sst_family else None
         out["cpu_baseline"] = cpu_baseline(case_fn, dist, implicit=implicit)
     print(json.dumps(out))
 

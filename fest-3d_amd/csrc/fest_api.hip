@@ -252,8 +252,9 @@ int launch_residual_front(fest_handle *h, int current_iter, int store_csnd) {
     box(b.imx + 1, b.jmx + 1, b.kmx + 1, g, t, b.xcd_remap);
     {
       KgScope ts(h, KG_GRAD);
-      if (b.ng == 6) hipLaunchKernelGGL(k_grad<6>, g, t, 0, st, b);
-      else hipLaunchKernelGGL(k_grad<4>, g, t, 0, st, b);
+      if (b.turb == FEST_TURB_SST2003) hipLaunchKernelGGL((k_grad<6, true>), g, t, 0, st, b);
+      else if (b.ng == 6) hipLaunchKernelGGL((k_grad<6, false>), g, t, 0, st, b);
+      else hipLaunchKernelGGL((k_grad<4, false>), g, t, 0, st, b);
     }
     int mx = std::max(std::max(b.imx, b.jmx), b.kmx);
     dim3 gb(cdiv(mx, 64), cdiv(mx, 4), 6);
@@ -521,8 +522,9 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
   // a 2-D grid has one cell in k (kmx == 2), src/face/flux/convective/*.f90
   if (dims->imx < 4 || dims->jmx < 4 || (dims->kmx < 4 && dims->kmx != 2))
     return bad("i and j need at least 3 cells, k at least 3 or exactly 1");
-  int nv_expect = scheme->turb == FEST_TURB_SST ? 7 : 5;
-  if (scheme->turb != FEST_TURB_NONE && scheme->turb != FEST_TURB_SST) return bad("turbulence model not supported");
+  if (scheme->turb != FEST_TURB_NONE && scheme->turb != FEST_TURB_SST && scheme->turb != FEST_TURB_SST2003)
+    return bad("turbulence model not supported");
+  int nv_expect = scheme->turb != FEST_TURB_NONE ? 7 : 5;
   if (dims->n_var != nv_expect) return bad("n_var does not match the turbulence model");
   if (scheme->flux < 0 || scheme->flux > 5) return bad("unknown flux scheme");
   if (scheme->recon < 0 || scheme->recon > 3) return bad("unknown reconstruction");

@@ -20,7 +20,10 @@ __device__ __forceinline__ const double *grad_var(const Blk &b, int l) {
   return l < 3 ? b.qp + (long)(l + 1) * b.nc : l == 3 ? b.temp : b.qp + (long)(l + 1) * b.nc;
 }
 
-// SST eddy viscosity and F1 of one cell from its gradients (src/viscosity.f90:343-388)
+// SST eddy viscosity and F1 of one cell from its gradients
+// (src/viscosity.f90:343-388); S2003: the 2003 form, with the strain rate in
+// place of the vorticity and 1e-10 floors (src/viscosity.f90:215-275)
+template <bool S2003>
 __device__ __forceinline__ void sst_mut_f1(double rho, double tk, double tw, double dist, double mu,
                                            const double (&g)[6][3], double &mut, double &f1) {
   const double bstar = 0.09, a1 = 0.31, sigma_w2 = 0.856;
@@ -28,36 +31,47 @@ __device__ __forceinline__ void sst_mut_f1(double rho, double tk, double tw, dou
   double var2 = 500 * (mu / rho) / ((dist * dist) * tw);
   double arg2 = fmx(2 * var1, var2);
   double F = tanh(arg2 * arg2);
-  double wx = (g[2][1] - g[1][2]);
-  double wy = (g[0][2] - g[2][0]);
-  double wz = (g[1][0] - g[0][1]);
-  double vort = sqrt(wx * wx + wy * wy + wz * wz);
-  double NUM = rho * a1 * tk;
-  double DENOM = fmx(fmx((a1 * tw), vort * F), 1.e-20);
+  double NUM = rho * a1 * tk, DENOM;
+  if constexpr (!S2003) {
+    double wx = (g[2][1] - g[1][2]);
+    double wy = (g[0][2] - g[2][0]);
+    double wz = (g[1][0] - g[0][1]);
+    double vort = sqrt(wx * wx + wy * wy + wz * wz);
+    DENOM = fmx(fmx((a1 * tw), vort * F), 1.e-20);
+  } else {
+    double sxx = g[0][0], syy = g[1][1], szz = g[2][2];
+    double syz = g[2][1] + g[1][2], szx = g[0][2] + g[2][0], sxy = g[1][0] + g[0][1];
+    double SijSij = (2.0 * (sxx * sxx)) + (2.0 * (syy * syy)) + (2.0 * (szz * szz)) + syz * syz + szx * szx + sxy * sxy;
+    DENOM = fmx(fmx((a1 * tw), sqrt(SijSij) * F), 1.0e-10);
+  }
   mut = NUM / DENOM;
-  double CD = fmx(2 * rho * sigma_w2 * (g[4][0] * g[5][0] + g[4][1] * g[5][1] + g[4][2] * g[5][2]) / tw, 1e-20);
+  double CD = fmx(2 * rho * sigma_w2 * (g[4][0] * g[5][0] + g[4][1] * g[5][1] + g[4][2] * g[5][2]) / tw,
+                  S2003 ? 1.0e-10 : 1e-20);
   double right = 4 * (rho * sigma_w2 * tk) / (CD * (dist * dist));
   double left = fmx(var1, var2);
   double arg1 = fmn(left, right);
   f1 = tanh(arg1 * arg1 * arg1 * arg1);
 }
 
-// SST source of an interior cell, times its volume (src/source.f90:158-270)
+// SST source of an interior cell, times its volume (src/source.f90:158-270);
+// S2003: production limited to 10 D_k, CD floored at 1e-10, gama1 = 5/9,
+// gama2 = 0.44 (src/source.f90:205-211)
+template <bool S2003>
 __device__ __forceinline__ void sst_source(double rho, double tk, double tw, double mut, double F1, double vol,
                                            const double (&g)[6][3], double &S_k, double &S_w) {
   const double sigma_w1 = 0.5, sigma_w2 = 0.856, beta1 = 0.075, beta2 = 0.0828, bstar = 0.09, kap = 0.41;
-  const double gama1 = (beta1 / bstar) - ((sigma_w1 * (kap * kap)) / sqrt(bstar));
-  const double gama2 = (beta2 / bstar) - ((sigma_w2 * (kap * kap)) / sqrt(bstar));
+  const double gama1 = S2003 ? 5.0 / 9.0 : (beta1 / bstar) - ((sigma_w1 * (kap * kap)) / sqrt(bstar));
+  const double gama2 = S2003 ? 0.44 : (beta2 / bstar) - ((sigma_w2 * (kap * kap)) / sqrt(bstar));
   double vort = sqrt(sqd(g[2][1] - g[1][2]) + sqd(g[0][2] - g[2][0]) + sqd(g[1][0] - g[0][1]));
   double CD = 2 * rho * sigma_w2 * (g[4][0] * g[5][0] + g[4][1] * g[5][1] + g[4][2] * g[5][2]) / tw;
-  CD = fmx(CD, 1e-20);
+  CD = fmx(CD, S2003 ? 1e-10 : 1e-20);
   double gama = gama1 * F1 + gama2 * (1. - F1);
   double beta = beta1 * F1 + beta2 * (1. - F1);
   double D_k = bstar * rho * tw * tk;
   double D_w = beta * rho * (tw * tw);
   double divergence = g[0][0] + g[1][1] + g[2][2];
   double P_k = mut * (vort * vort) - ((2.0 / 3.0) * rho * tk * divergence);
-  P_k = fmn(P_k, 20 * D_k);
+  P_k = fmn(P_k, (S2003 ? 10 : 20) * D_k);
   double P_w = (rho * gama / mut) * P_k;
   double lamda = (1. - F1) * CD;
   S_k = (P_k - D_k) * vol;
@@ -70,7 +84,7 @@ __device__ __forceinline__ void sst_source(double rho, double tk, double tw, dou
 // source.  Interior gradients are final here: the ghost-gradient rule only
 // writes ghost cells.  NG is the number of gradient slots (4 laminar, 6
 // SST); a compile-time count keeps all 3*NG gradients in registers.
-template <int NG>
+template <int NG, bool S2003>
 __global__ void __launch_bounds__(256) k_grad(Blk b) {
   int i, j;
   if (!flat_ij(b.imx + 1, b.jmx + 1, i, j, plane_block(b.xcd_remap))) return;
@@ -138,12 +152,12 @@ __global__ void __launch_bounds__(256) k_grad(Blk b) {
   if constexpr (NG == 6) {
   double tk = b.qp[5 * b.nc + c], tw = b.qp[6 * b.nc + c];
   double mut, F1;
-  sst_mut_f1(rho, tk, tw, b.dist[c], mu, gg, mut, F1);
+  sst_mut_f1<S2003>(rho, tk, tw, b.dist[c], mu, gg, mut, F1);
   b.mut[c] = mut;
   b.f1[c] = F1;
   if (i >= 1 && i <= b.imx - 1 && j >= 1 && j <= b.jmx - 1 && k >= 1 && k <= b.kmx - 1) {
     double S_k, S_w;
-    sst_source(rho, tk, tw, mut, F1, vol, gg, S_k, S_w);
+    sst_source<S2003>(rho, tk, tw, mut, F1, vol, gg, S_k, S_w);
     long r = ri(b, i, j, k);
     b.src[r] = S_k;
     b.src[b.nres + r] = S_w;

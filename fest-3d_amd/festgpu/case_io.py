@@ -147,7 +147,7 @@ def read_case(root: str):
                 vel_mag=vel, MInf=vel / math.sqrt(gm * p / rho), gm=gm, R_gas=R, mu_ref=mu_ref,
                 T_ref=float(_first(flow, "Reference Temperature")),
                 S_ref=float(_first(flow, "Sutherland Temperature")), Pr=float(prs[0]), tPr=float(prs[1]))
-    if turb == "sst":
+    if turb in ("sst", "sst2003"):
         tu = float(_first(flow, "Free Stream Turbulence Intensity")) / 100
         ratio = float(_first(flow, "Free Stream Viscosity Ratio"))
         tk = 1.5 * ((vel * tu) ** 2)

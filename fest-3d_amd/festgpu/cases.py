@@ -27,7 +27,7 @@ import numpy as np
 # flux / reconstruction / turbulence ids shared with the C-ABI
 FLUX = {"ausm": 0, "ausmP": 1, "ausmUP": 2, "slau": 3, "ldfss0": 4, "van_leer": 5}
 RECON = {"none": 0, "muscl": 1, "ppm": 2, "weno": 3}
-TURB = {"none": 0, "sst": 1}
+TURB = {"none": 0, "sst": 1, "sst2003": 2}
 TIME_ACC = {"none": 0, "RK2": 2, "RK4": 4, "TVDRK2": 12, "TVDRK3": 13, "implicit": 20}
 
 _GOLDEN = np.uint64(0x9E3779B97F4A7C15)
@@ -92,8 +92,14 @@ class Case:
     name: str = "case"
 
     @property
+    def sst_family(self) -> bool:
+        """SST k-omega, in its original or its 2003 form: k and omega are
+        variables 6 and 7."""
+        return self.turb in ("sst", "sst2003")
+
+    @property
     def n_var(self) -> int:
-        return 7 if self.turb == "sst" else 5
+        return 7 if self.sst_family else 5
 
     @property
     def ncells(self) -> int:
@@ -122,7 +128,7 @@ def freestream(case: Case, T_inf=300.0, p_inf=101325.0, mach=0.5, reynolds=1000.
     case.MInf = case.vel_mag / math.sqrt(gm * p_inf / rho)
     case.mu_ref = rho * case.vel_mag * 1.0 / reynolds
     case.T_ref = T_inf
-    if case.turb == "sst":
+    if case.sst_family:
         tu = tu_percent / 100
         case.tk_inf = 1.5 * ((case.vel_mag * tu) ** 2)
         case.tw_inf = rho * case.tk_inf / (case.mu_ref * mu_ratio)

@@ -47,7 +47,7 @@ W = 8  # bytes per FP64 value
 
 
 def _ngrad(case):
-    return 6 if case.turb == "sst" else 4
+    return 6 if case.sst_family else 4
 
 
 def lusgs_launches(case):
@@ -74,7 +74,7 @@ def launch_items(case, group):
 
 def bytes_per_item(case, group):
     """Algorithmic bytes per face or cell of one launch of `group`."""
-    nv, ng, sst = case.n_var, _ngrad(case), case.turb == "sst"
+    nv, ng, sst = case.n_var, _ngrad(case), case.sst_family
     if group in ("flux_i", "flux_j", "flux_k"):
         reads = nv + 4 + 3 + 3 * ng + (3 if sst else 1)
         writes = nv + 0.25

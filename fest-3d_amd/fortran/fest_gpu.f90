@@ -22,7 +22,7 @@ module fest_gpu
       FEST_FLUX_AUSMUP = 2, FEST_FLUX_SLAU = 3, FEST_FLUX_LDFSS0 = 4, FEST_FLUX_VANLEER = 5
   integer(c_int32_t), parameter, public :: FEST_RECON_NONE = 0, FEST_RECON_MUSCL = 1, &
       FEST_RECON_PPM = 2, FEST_RECON_WENO = 3
-  integer(c_int32_t), parameter, public :: FEST_TURB_NONE = 0, FEST_TURB_SST = 1
+  integer(c_int32_t), parameter, public :: FEST_TURB_NONE = 0, FEST_TURB_SST = 1, FEST_TURB_SST2003 = 2
   integer(c_int32_t), parameter, public :: FEST_TA_NONE = 0, FEST_TA_RK2 = 2, FEST_TA_RK4 = 4, &
       FEST_TA_TVDRK2 = 12, FEST_TA_TVDRK3 = 13, FEST_TA_IMPLICIT = 20
 

@@ -90,6 +90,7 @@ contains
     select case (trim(scheme%turbulence))
       case ("none"); s%turb = FEST_TURB_NONE
       case ("sst");  s%turb = FEST_TURB_SST
+      case ("sst2003");  s%turb = FEST_TURB_SST2003
       case default
         print *, "update: turbulence model '", trim(scheme%turbulence), "' is not on the device path"
         stop 1

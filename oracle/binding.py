@@ -65,7 +65,7 @@ def to_ccase(case) -> CCase:
 def array_shape(case, name):
     """numpy (C-order) shape of each array in the Fortran layout."""
     imx, jmx, kmx, nv = case.imx, case.jmx, case.kmx, case.n_var
-    ng = 6 if case.turb == "sst" else 4
+    ng = 6 if case.sst_family else 4
     cell = (kmx + 5, jmx + 5, imx + 5)
     shapes = {
         "qp": (nv,) + cell, "ustore": (nv,) + cell,

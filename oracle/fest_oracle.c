@@ -101,7 +101,7 @@ orc *orc_create(const orc_case *c) {
   if (!s) return NULL;
   s->c = *c;
   s->imx = c->imx; s->jmx = c->jmx; s->kmx = c->kmx; s->nv = c->n_var;
-  s->ngrad = (c->turb == ORC_TURB_SST) ? 6 : 4;
+  s->ngrad = (c->turb != ORC_TURB_NONE) ? 6 : 4;
   s->pi = s->imx + 5; s->pj = s->jmx + 5; s->pk = s->kmx + 5;
   s->ncell = (size_t)s->pi * s->pj * s->pk;
   s->nif = (size_t)(s->imx + 6) * (s->jmx + 5) * (s->kmx + 5);
@@ -546,10 +546,10 @@ static void set_omega_at_wall(orc *s, int face) {
 }
 
 static void turb_copy(orc *s, int type, int face) {
-  if (s->c.turb == ORC_TURB_SST) { copy3(s, QPL(5), type, face); copy3(s, QPL(6), type, face); }
+  if (s->c.turb != ORC_TURB_NONE) { copy3(s, QPL(5), type, face); copy3(s, QPL(6), type, face); }
 }
 static void turb_fix(orc *s, int face) {
-  if (s->c.turb == ORC_TURB_SST) {
+  if (s->c.turb != ORC_TURB_NONE) {
     fixv(s, QPL(5), s->c.fix_tk[face], face);
     fixv(s, QPL(6), s->c.fix_tw[face], face);
   }
@@ -614,7 +614,7 @@ static void far_field(orc *s, int face) {
   }
   /* the turbulent ghosts end as set by the last face cell in loop order:
    * a whole-face flat copy if it was outflow, the fixed values otherwise */
-  if (s->c.turb == ORC_TURB_SST) {
+  if (s->c.turb != ORC_TURB_NONE) {
     if (last_positive) turb_copy(s, CP_FLAT, face); else turb_fix(s, face);
   }
   /* qp(-1,:,:,:) = qp(0,:,:,:); qp(-2,:,:,:) = qp(0,:,:,:) on the full plane */
@@ -715,7 +715,7 @@ static int populate_ghost_primitive(orc *s, int current_iter) {
         copy3(s, QPL(4), CP_SYMM, f);
         temp_based_density(s, f);
         copy3(s, QPL(1), CP_ANTI, f); copy3(s, QPL(2), CP_ANTI, f); copy3(s, QPL(3), CP_ANTI, f);
-        if (c->turb == ORC_TURB_SST) { copy3(s, QPL(5), CP_ANTI, f); set_omega_at_wall(s, f); }
+        if (c->turb != ORC_TURB_NONE) { copy3(s, QPL(5), CP_ANTI, f); set_omega_at_wall(s, f); }
         break;
       case -6: /* slip wall */
         copy3(s, QPL(0), CP_SYMM, f); copy3(s, QPL(4), CP_SYMM, f);
@@ -1256,6 +1256,7 @@ static int evaluate_all_gradients(orc *s) {
 static int calculate_viscosity(orc *s) {
   const orc_case *c = &s->c;
   double *mu = s->a[ORC_MU], *mut = s->a[ORC_MUT], *F1 = s->a[ORC_F1], *dist = s->a[ORC_DIST];
+  int s2003 = c->turb == ORC_TURB_SST2003;
   if (c->mu_sutherland) {
     for (int k = 0; k <= KMX; ++k) for (int j = 0; j <= JMX; ++j) for (int i = 0; i <= IMX; ++i) {
       size_t ci = CI(s, i, j, k);
@@ -1263,7 +1264,7 @@ static int calculate_viscosity(orc *s) {
       mu[ci] = c->mu_ref * (pow(T / c->T_ref, 1.5)) * ((c->T_ref + c->S_ref) / (T + c->S_ref));
     }
   }
-  if (c->turb == ORC_TURB_SST) {
+  if (c->turb != ORC_TURB_NONE) {
     for (int k = 0; k <= KMX; ++k) for (int j = 0; j <= JMX; ++j) for (int i = 0; i <= IMX; ++i) {
       size_t ci = CI(s, i, j, k);
       double density = QP(0, i, j, k), tk = QP(5, i, j, k), tw = QP(6, i, j, k);
@@ -1271,17 +1272,28 @@ static int calculate_viscosity(orc *s) {
       double var2 = 500 * (mu[ci] / density) / ((dist[ci] * dist[ci]) * tw);
       double arg2 = fmx(2 * var1, var2);
       double F = tanh(arg2 * arg2);
-      double wx = (GY(2, i, j, k) - GZ(1, i, j, k));
-      double wy = (GZ(0, i, j, k) - GX(2, i, j, k));
-      double wz = (GX(1, i, j, k) - GY(0, i, j, k));
-      double wijwij = wx * wx + wy * wy + wz * wz;
-      double vort = sqrt(wijwij);
-      double NUM = density * a1_sst * tk;
-      double DENOM = fmx(fmx((a1_sst * tw), vort * F), 1.e-20);
+      double NUM = density * a1_sst * tk, DENOM;
+      if (!s2003) {
+        double wx = (GY(2, i, j, k) - GZ(1, i, j, k));
+        double wy = (GZ(0, i, j, k) - GX(2, i, j, k));
+        double wz = (GX(1, i, j, k) - GY(0, i, j, k));
+        double wijwij = wx * wx + wy * wy + wz * wz;
+        double vort = sqrt(wijwij);
+        DENOM = fmx(fmx((a1_sst * tw), vort * F), 1.e-20);
+      } else {
+        /* src/viscosity.f90:215-275: the strain rate, limits 1e-10 */
+        double sxx = GX(0, i, j, k), syy = GY(1, i, j, k), szz = GZ(2, i, j, k);
+        double syz = GY(2, i, j, k) + GZ(1, i, j, k), szx = GZ(0, i, j, k) + GX(2, i, j, k);
+        double sxy = GX(1, i, j, k) + GY(0, i, j, k);
+        double SijSij = (2.0 * (sxx * sxx)) + (2.0 * (syy * syy)) + (2.0 * (szz * szz)) + syz * syz + szx * szx +
+                        sxy * sxy;
+        double strain = sqrt(SijSij);
+        DENOM = fmx(fmx((a1_sst * tw), strain * F), 1.0e-10);
+      }
       mut[ci] = NUM / DENOM;
       double CD = fmx(2 * density * sigma_w2 * (GX(4, i, j, k) * GX(5, i, j, k) + GY(4, i, j, k) * GY(5, i, j, k) +
                                                  GZ(4, i, j, k) * GZ(5, i, j, k)) / tw,
-                      1e-20);
+                      s2003 ? 1.0e-10 : 1e-20);
       double right = 4 * (density * sigma_w2 * tk) / (CD * (dist[ci] * dist[ci]));
       double left = fmx(var1, var2);
       double arg1 = fmn(left, right);
@@ -1408,7 +1420,7 @@ static void viscous_dir(orc *s, int d, int sst) {
 
 static int compute_viscous_fluxes(orc *s) {
   for (int d = 0; d < 3; ++d) viscous_dir(s, d, 0);
-  if (s->c.turb == ORC_TURB_SST) {
+  if (s->c.turb != ORC_TURB_NONE) {
     viscous_dir(s, 0, 1);
     viscous_dir(s, 1, 1);
     if (KMX != 2) viscous_dir(s, 2, 1);
@@ -1432,12 +1444,15 @@ static void compute_residue(orc *s) {
           (H[(size_t)l * s->nH + HI(s, i, j, k + 1)] - H[(size_t)l * s->nH + HI(s, i, j, k)]);
 }
 
-/* src/source.f90:158-270 (sst; the sst2003 constants are not restated) */
+/* src/source.f90:158-270; sst2003 limits the production to 10 D_k, floors
+ * CD at 1e-10 and sets gama1 = 5/9, gama2 = 0.44 (source.f90:205-211) */
 static void add_sst_source(orc *s) {
   double *R = s->a[ORC_RES], *mut = s->a[ORC_MUT], *F1a = s->a[ORC_F1], *vol = s->a[ORC_VOL];
-  double gama1 = (beta1 / bstar) - ((sigma_w1 * (kappa_sst * kappa_sst)) / sqrt(bstar));
-  double gama2 = (beta2 / bstar) - ((sigma_w2 * (kappa_sst * kappa_sst)) / sqrt(bstar));
-  int limiter = 20;
+  int s2003 = s->c.turb == ORC_TURB_SST2003;
+  double gama1 = s2003 ? 5.0 / 9.0 : (beta1 / bstar) - ((sigma_w1 * (kappa_sst * kappa_sst)) / sqrt(bstar));
+  double gama2 = s2003 ? 0.44 : (beta2 / bstar) - ((sigma_w2 * (kappa_sst * kappa_sst)) / sqrt(bstar));
+  int limiter = s2003 ? 10 : 20;
+  double cd_floor = s2003 ? 1e-10 : 1e-20;
   for (int k = 1; k <= KMX - 1; ++k) for (int j = 1; j <= JMX - 1; ++j) for (int i = 1; i <= IMX - 1; ++i) {
     size_t ci = CI(s, i, j, k);
     double density = QP(0, i, j, k), tk = QP(5, i, j, k), tw = QP(6, i, j, k);
@@ -1445,7 +1460,7 @@ static void add_sst_source(orc *s) {
                        sq(GX(1, i, j, k) - GY(0, i, j, k)));
     double CD = 2 * density * sigma_w2 * (GX(4, i, j, k) * GX(5, i, j, k) + GY(4, i, j, k) * GY(5, i, j, k) +
                                           GZ(4, i, j, k) * GZ(5, i, j, k)) / tw;
-    CD = fmx(CD, 1e-20);
+    CD = fmx(CD, cd_floor);
     double F1 = F1a[ci];
     double gama = gama1 * F1 + gama2 * (1. - F1);
     double beta = beta1 * F1 + beta2 * (1. - F1);
@@ -1482,7 +1497,7 @@ int orc_stage(orc *s, int current_iter, int mask) {
   }
   if (mask & 16) {
     compute_residue(s);
-    if (s->c.turb == ORC_TURB_SST) add_sst_source(s);
+    if (s->c.turb != ORC_TURB_NONE) add_sst_source(s);
   }
   return ORC_OK;
 }
@@ -1503,7 +1518,7 @@ int orc_residual(orc *s, int current_iter) {
     if ((st = compute_viscous_fluxes(s))) return st;
   }
   compute_residue(s);
-  if (s->c.turb == ORC_TURB_SST) add_sst_source(s);
+  if (s->c.turb != ORC_TURB_NONE) add_sst_source(s);
   return ORC_OK;
 }
 
@@ -1597,7 +1612,7 @@ static int update_with(orc *s, double TF, double SF, int TU, int use_store) {
     double KE = 0.;
     u1[4] = (u1[4] / (gm - 1.) + 0.5 * (u1[1] * u1[1] + u1[2] * u1[2] + u1[3] * u1[3])) / u1[0] + KE;
     for (int l = 0; l < nv; ++l) R[l] = res[(size_t)l * s->nR + ri];
-    if (c->turb == ORC_TURB_SST) {
+    if (c->turb != ORC_TURB_NONE) {
       double beta = beta1 * F1a[ci] + (1. - F1a[ci]) * beta2;
       R[5] = R[5] / (1 + (beta * QP(6, i, j, k) * dt[ri]));
       R[6] = R[6] / (1 + (2 * beta * QP(6, i, j, k) * dt[ri]));
@@ -1613,7 +1628,7 @@ static int update_with(orc *s, double TF, double SF, int TU, int use_store) {
     for (int l = 0; l < nv; ++l) if (isnan(u2[l])) bad = 1;
     if (bad) { status = ORC_E_NONPHYSICAL; continue; }
     for (int l = 0; l < 5; ++l) QP(l, i, j, k) = u2[l];
-    if (c->turb == ORC_TURB_SST) {
+    if (c->turb != ORC_TURB_NONE) {
       if (u2[5] >= 0.) QP(5, i, j, k) = u2[5];
       if (u2[6] >= 0.) QP(6, i, j, k) = u2[6];
     }
@@ -1726,7 +1741,7 @@ static inline size_t DQI(const orc *s, int i, int j, int k) {
 /* The six faces, neighbour states and centres of cell (i,j,k), in the
  * reference's numbering 1..6 = i-, j-, k-, i+, j+, k+ (src/lusgs.f90:246-312). */
 static void lu_cell(const orc *s, int i, int j, int k, double Q[7][7], double C[7][3], lu_face F[6]) {
-  int nv = s->nv, sst = s->c.turb == ORC_TURB_SST;
+  int nv = s->nv, sst = s->c.turb != ORC_TURB_NONE;
   const int di[7] = {0, -1, 0, 0, 1, 0, 0}, dj[7] = {0, 0, -1, 0, 0, 1, 0}, dk[7] = {0, 0, 0, -1, 0, 0, 1};
   const double *vol = s->a[ORC_VOL], *mu = s->a[ORC_MU], *mut = s->a[ORC_MUT], *F1 = s->a[ORC_F1];
   int lam_mu = s->c.mu_ref != 0.0;
@@ -1764,7 +1779,7 @@ static void lu_cell(const orc *s, int i, int j, int k, double Q[7][7], double C[
  * conservative update of the interior cells. */
 static int lusgs(orc *s) {
   const orc_case *c = &s->c;
-  int nv = s->nv, sst = c->turb == ORC_TURB_SST;
+  int nv = s->nv, sst = c->turb != ORC_TURB_NONE;
   double gm = c->gm;
   size_t ndq = (size_t)(IMX + 1) * (JMX + 1) * (KMX + 1);
   double *dqs = calloc(ndq * (size_t)nv, sizeof(double));

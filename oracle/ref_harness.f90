@@ -76,6 +76,8 @@ contains
     schm%interpolant = trim(recons(hc%recon))
     if (hc%turb == 1) then
       schm%turbulence = 'sst'
+    else if (hc%turb == 2) then
+      schm%turbulence = 'sst2003'
     else
       schm%turbulence = 'none'
     end if

@@ -280,3 +280,20 @@ def test_two_dimensional_grids(rule, kind):
     assert relerr(out["qp"][:, 3:-3, 3:-3, 3:-3], o.arr("qp")[:, 3:-3, 3:-3, 3:-3]) <= TOL
     assert relerr(out["residue"], o.arr("res")) <= 1e-10
     assert not out["H"][0].any()
+
+
+@pytest.mark.parametrize("rule", ["RK4", "implicit"])
+def test_sst2003_steps(rule):
+    """The 2003 form of SST (strain-rate eddy viscosity, production limit
+    10, fixed gammas): two steps against the oracle."""
+    case, nodes, q = C.sst_bump(20, 12, 6)
+    case = dataclasses.replace(case, turb="sst2003", time_accuracy=rule)
+    say("sst2003", rule)
+    g, o = pair(case, nodes, q, wall=True)
+    for it in range(1, 3):
+        assert o.step(it) == 0
+        g.get_next_solution(it)
+    out = g.download("qp", "mut", "residue")
+    assert relerr(out["qp"], o.arr("qp")) <= TOL
+    assert relerr(out["mut"], o.arr("mut")) <= TOL
+    assert relerr(out["residue"], o.arr("res")) <= 1e-10

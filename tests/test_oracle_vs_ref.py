@@ -161,3 +161,32 @@ def test_two_dimensional_sst_periodic_k():
     case, nodes, q = C.sst_bump(14, 10, 1)
     compare_2d(case, nodes, q, it=2, wall_dist=True, extra=("mut", "f1"))
     compare_lusgs(case, nodes, q, it=2, wall_dist=True)
+
+
+# ------------------------------------------------------------ SST-2003
+@pytest.mark.parametrize("flux,recon", [("slau", "weno"), ("ausm", "muscl")])
+def test_sst2003(flux, recon):
+    """The 2003 form of SST: strain-rate eddy viscosity with 1e-10 floors
+    (src/viscosity.f90:215-275), production limited to 10 D_k and fixed
+    gammas in the source (src/source.f90:205-211); everything else as SST."""
+    case, nodes, q = C.sst_bump(14, 10, 6)
+    case = dataclasses.replace(case, turb="sst2003", flux=flux, recon=recon)
+    compare(case, nodes, q, it=3, wall_dist=True, extra=("mut", "f1"))
+    compare_lusgs(case, nodes, q, it=2, wall_dist=True)
+
+
+def test_sst2003_differs_from_sst():
+    """Guard against the 2003 switch being a no-op: on the same state the
+    eddy viscosity and the k and omega residues change."""
+    case, nodes, q = C.sst_bump(14, 10, 6)
+    outs = []
+    for turb in ("sst", "sst2003"):
+        o = Oracle(dataclasses.replace(case, turb=turb))
+        o.set_nodes(nodes)
+        o.wall_dist()
+        o.set_state(q)
+        o.temp_refresh()
+        assert o.residual(1) == 0
+        outs.append((o.arr("mut").copy(), o.arr("res").copy()))
+    assert not np.array_equal(outs[0][0], outs[1][0])
+    assert not np.array_equal(outs[0][1][5:], outs[1][1][5:])
