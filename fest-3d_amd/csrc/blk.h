@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "fest_hip.h"
+
 namespace fest {
 
 struct Blk {
@@ -54,12 +56,12 @@ struct Blk {
   double iPr, itPr, cpR;  // 1/Pr, 1/Pr_t, gamma R/(gamma-1): the heat-flux factors
   double rho_inf, u_inf, v_inf, w_inf, p_inf, tk_inf, tw_inf;
   double c1, c2, c3;
-  double fix[8][6];           // rho,u,v,w,p,tk,tw,wall temperature
+  double fix[9][6];           // rho,u,v,w,p,tk,tw,wall temperature,total pressure
   double global_dt;
   double CFLv;                // CFL of the current call
 };
 
-enum { FX_RHO = 0, FX_U, FX_V, FX_W, FX_P, FX_TK, FX_TW, FX_TWALL };
+enum { FX_RHO = 0, FX_U, FX_V, FX_W, FX_P, FX_TK, FX_TW, FX_TWALL, FX_TP, FX_N };
 enum : unsigned { ERR_NONPHYS = 1u, ERR_NAN_FLUX = 2u, ERR_NAN_GRAD = 4u, ERR_NAN_MU = 8u, ERR_CONFIG = 16u };
 
 __host__ __device__ inline long ci(const Blk &b, int i, int j, int k) {

@@ -536,9 +536,10 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
     int id = bc->id[f];
     // ids >= 0 (interfaces) and -10 (cross-block periodic) take their ghost
     // layers from a group's interface exchange (fest_group_create)
-    if (id == -11) return bad("total-pressure inlets (-11) are not supported");
-    if (id < -10) return bad("unknown boundary id");
-    if (id == -7) return bad("pole boundaries are not supported");
+    if (id < -11) return bad("unknown boundary id");
+    // a k pole on a one-cell k extent would take its normal from the other
+    // k face's pole layers
+    if (id == -7 && f >= 4 && dims->kmx == 2) return bad("a pole needs at least two cells across it");
   }
   if (flow->mu_ref == 0.0 && scheme->turb) return bad("turbulence needs mu_ref != 0");
   int ndev = 0;
@@ -569,6 +570,8 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
   for (int d = 0; d < 3; ++d) { b.lim[d] = scheme->lim[d]; b.tlim[d] = scheme->tlim[d]; }
   // src/boundary/boundary_state_reconstruction.f90:45-47
   b.ppm_flag = (scheme->recon == FEST_RECON_PPM || scheme->recon == FEST_RECON_WENO) ? 1 : 0;
+  for (int f = 0; f < 6; ++f)
+    if (bc->id[f] == -7) b.ppm_flag = 1;
   {
     // XCD-aware block order of the sweep kernels; FEST_XCD_REMAP=0 switches
     // it off (a tuning switch: results do not depend on it)
@@ -597,6 +600,7 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
     b.fix[FX_V][f] = bc->fixed_y_speed[f]; b.fix[FX_W][f] = bc->fixed_z_speed[f];
     b.fix[FX_P][f] = bc->fixed_pressure[f]; b.fix[FX_TK][f] = bc->fixed_tk[f];
     b.fix[FX_TW][f] = bc->fixed_tw[f]; b.fix[FX_TWALL][f] = bc->fixed_wall_temperature[f];
+    b.fix[FX_TP][f] = bc->fixed_Tpressure[f];
   }
   b.global_dt = scheme->global_time_step;
   b.CFLv = 1.0;
@@ -703,6 +707,12 @@ int fest_set_grid(fest_handle *h, const double *nodes, int with_wall_dist) {
   for (int d = 0; d < 3; ++d) {
     dim3 g(cdiv(b.imx + 5 + (d == 0), 64), cdiv(b.jmx + 5 + (d == 1), 4), (unsigned)(b.kmx + 5 + (d == 2)));
     hipLaunchKernelGGL(k_face_metrics, g, dim3(64, 4, 1), 0, h->stream, b, d);
+  }
+  for (int f = 0; f < 6; ++f) {
+    if (b.bc[f] != -7) continue;
+    int d = f >> 1;
+    int na = (d == 0 ? b.jmx : b.imx) + 5, nb = (d == 2 ? b.jmx : b.kmx) + 5;
+    hipLaunchKernelGGL(k_pole_faces, dim3(cdiv(na, 64), cdiv(nb, 4), 4), dim3(64, 4, 1), 0, h->stream, b, f);
   }
   {
     dim3 g(cdiv(b.imx + 5, 64), cdiv(b.jmx + 5, 4), (unsigned)(b.kmx + 5));

@@ -33,19 +33,19 @@ enum { CP_ANTI = 0, CP_FLAT = 1, CP_SYMM = 2 };
 // make the compiler copy the whole argument block to private memory
 struct FaceBC {
   int id;
-  double fix[8];
+  double fix[FX_N];
 };
 __device__ __forceinline__ FaceBC face_params(const Blk &b, int face) {
   FaceBC p;
   p.id = b.bc[0];
 #pragma unroll
-  for (int l = 0; l < 8; ++l) p.fix[l] = b.fix[l][0];
+  for (int l = 0; l < FX_N; ++l) p.fix[l] = b.fix[l][0];
 #pragma unroll
   for (int f = 1; f < 6; ++f)
     if (face == f) {
       p.id = b.bc[f];
 #pragma unroll
-      for (int l = 0; l < 8; ++l) p.fix[l] = b.fix[l][f];
+      for (int l = 0; l < FX_N; ++l) p.fix[l] = b.fix[l][f];
     }
   return p;
 }
@@ -134,9 +134,13 @@ __device__ void omega_wall_col(const Blk &b, int face, int a, int bb) {
   }
 }
 
-// far-field Riemann state of one column (src/boundary/bc_primitive.f90:645-1234);
-// returns Unb and, when write != 0, stores layer 0 of rho,u,v,w,p
-__device__ double farfield_col(const Blk &b, int face, int a, int bb, int write) {
+// far-field Riemann state of one column (src/boundary/bc_primitive.f90:645-1234)
+// or, when total, the total-pressure inlet (bc_primitive.f90:1237-1776):
+// the same Riemann-invariant velocity, then the pressure from the total
+// pressure tp and the boundary Mach number and the density from the
+// boundary speed of sound.  Returns Unb and, when write != 0, stores layer 0
+// of rho,u,v,w,p
+__device__ double riemann_col(const Blk &b, int face, int a, int bb, int write, bool total, double tp) {
   double gm = b.gm;
   double *rho = b.qp, *u = b.qp + b.nc, *v = b.qp + 2 * b.nc, *w = b.qp + 3 * b.nc, *p = b.qp + 4 * b.nc;
   double nx, ny, nz;
@@ -157,17 +161,26 @@ __device__ double farfield_col(const Blk &b, int face, int a, int bb, int write)
     u[cg] = u[c1] + vd * nx;
     v[cg] = v[c1] + vd * ny;
     w[cg] = w[c1] + vd * nz;
-    double sen = p[c1] / pow(rho[c1], gm);
-    rho[cg] = pow(Cb * Cb / (gm * sen), 1. / (gm - 1.));
-    p[cg] = (rho[cg] * Cb * Cb / gm);
+    if (!total) {
+      double sen = p[c1] / pow(rho[c1], gm);
+      rho[cg] = pow(Cb * Cb / (gm * sen), 1. / (gm - 1.));
+      p[cg] = (rho[cg] * Cb * Cb / gm);
+    }
   } else {
     double vd = Unb - Uninf;
     u[cg] = b.u_inf + vd * nx;
     v[cg] = b.v_inf + vd * ny;
     w[cg] = b.w_inf + vd * nz;
-    double sen = b.p_inf / pow(b.rho_inf, gm);
-    rho[cg] = pow(Cb * Cb / (gm * sen), 1. / (gm - 1.));
-    p[cg] = (rho[cg] * Cb * Cb / gm);
+    if (!total) {
+      double sen = b.p_inf / pow(b.rho_inf, gm);
+      rho[cg] = pow(Cb * Cb / (gm * sen), 1. / (gm - 1.));
+      p[cg] = (rho[cg] * Cb * Cb / gm);
+    }
+  }
+  if (total) {
+    double Mb = sqrt(u[cg] * u[cg] + v[cg] * v[cg] + w[cg] * w[cg]) / Cb;
+    p[cg] = tp / pow((1 + 0.5 * (gm - 1.) * Mb * Mb), gm / (gm - 1.));
+    rho[cg] = gm * p[cg] / (Cb * Cb);
   }
   return Unb;
 }
@@ -270,13 +283,16 @@ __global__ void k_bc_face(Blk b, int face0, int current_iter) {
         turb_copycol(b, CP_FLAT, face, a, bb);
       }
       break;
-    case -8: {  // far field
+    case -8:     // far field
+    case -11: {  // total-pressure inlet
+      const bool total = id == -11;
+      const double tp = p.fix[FX_TP];
       if (inner) {
-        farfield_col(b, face, a, bb, 1);
+        riemann_col(b, face, a, bb, 1, total, tp);
         if (b.turb) {
           // the whole-face turbulent copy/fix of the reference ends as set by
           // the last face cell in its loop order
-          double last = farfield_col(b, face, na - 1, nb - 1, 0);
+          double last = riemann_col(b, face, na - 1, nb - 1, 0, total, tp);
           if (last > 0.) turb_copycol(b, CP_FLAT, face, a, bb);
           else turb_fixcol(b, p, face, a, bb);
         }

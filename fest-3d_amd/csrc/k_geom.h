@@ -59,6 +59,27 @@ __global__ void k_face_metrics(Blk b, int d) {
   b.fA[d][f] = A; b.fx[d][f] = nx; b.fy[d][f] = ny; b.fz[d][f] = nz;
 }
 
+// pole face (id -7), src/geometry.f90:101-186 and 227-232: the pole face and
+// its three ghost faces (layers 1..-2, or N..N+3 at a max face) get zero area
+// and the unit normal of the first interior face (2, or N-1), over the whole
+// padded plane.  blockIdx.z is the layer; the source layer is never written.
+__global__ void k_pole_faces(Blk b, int face) {
+  const int d = face >> 1, hi = face & 1;
+  int a = (int)(blockIdx.x * blockDim.x + threadIdx.x) - 2;
+  int c = (int)(blockIdx.y * blockDim.y + threadIdx.y) - 2;
+  int ae = (d == 0 ? b.jmx : b.imx) + 2, ce = (d == 2 ? b.jmx : b.kmx) + 2;
+  if (a > ae || c > ce) return;
+  int N = d == 0 ? b.imx : d == 1 ? b.jmx : b.kmx;
+  int l = (int)blockIdx.z;
+  int n = hi ? N + l : 1 - l, src = hi ? N - 1 : 2;
+  long t, o;
+  if (d == 0) { t = fi(b, 0, n, a, c); o = fi(b, 0, src, a, c); }
+  else if (d == 1) { t = fi(b, 1, a, n, c); o = fi(b, 1, a, src, c); }
+  else { t = fi(b, 2, a, c, n); o = fi(b, 2, a, c, src); }
+  b.fA[d][t] = 0.;
+  b.fx[d][t] = b.fx[d][o]; b.fy[d][t] = b.fy[d][o]; b.fz[d][t] = b.fz[d][o];
+}
+
 __device__ double vol_tet(const double *p1, const double *p2, const double *p3, const double *p4) {
   double v = (p4[0] - p1[0]) * ((p2[1] - p1[1]) * (p3[2] - p1[2]) - (p2[2] - p1[2]) * (p3[1] - p1[1]))
            + (p4[1] - p1[1]) * ((p2[2] - p1[2]) * (p3[0] - p1[0]) - (p2[0] - p1[0]) * (p3[2] - p1[2]))

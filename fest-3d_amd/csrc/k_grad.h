@@ -206,6 +206,7 @@ __global__ void k_grad_bc(Blk b) {
   double sig = hi ? -1.0 : 1.0;
   long gI = gri(b, Ii, Ij, Ik), gG = gri(b, Gi, Gj, Gk);
   double *gx = b.g[0], *gy = b.g[1], *gz = b.g[2];
+  double gg[6][3];
   for (int l = 0; l < b.ng; ++l) {
     double dq = b.qp[(long)(l + 1) * b.nc + cI] - b.qp[(long)(l + 1) * b.nc + cG];
     if (l == 3) dq = b.temp[cI] - b.temp[cG];
@@ -219,6 +220,18 @@ __global__ void k_grad_bc(Blk b) {
     gx[o + gG] = ox + (ix - dot * nx);
     gy[o + gG] = oy + (iy - dot * ny);
     gz[o + gG] = oz + (iz - dot * nz);
+    gg[l][0] = gx[o + gG]; gg[l][1] = gy[o + gG]; gg[l][2] = gz[o + gG];
+  }
+  // A -11 face has no mu_t / F1 ghost rule (src/viscosity.f90:284-339,
+  // 409-465 list -1..-9): its ghosts keep the values calculate_viscosity
+  // computes there, from the ghost gradients just written
+  if (id == -11 && b.ng == 6) {
+    double mut, F1;
+    double rho = b.qp[cG], tk = b.qp[5 * b.nc + cG], tw = b.qp[6 * b.nc + cG];
+    if (b.turb == FEST_TURB_SST2003) sst_mut_f1<true>(rho, tk, tw, b.dist[cG], b.mu[cG], gg, mut, F1);
+    else sst_mut_f1<false>(rho, tk, tw, b.dist[cG], b.mu[cG], gg, mut, F1);
+    b.mut[cG] = mut;
+    b.f1[cG] = F1;
   }
 }
 

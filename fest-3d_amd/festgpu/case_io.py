@@ -93,13 +93,14 @@ def read_mapping(path: str) -> List[Link]:
 def read_bc(path: str, case: Case) -> Case:
     """Fixed values of the six faces (read_bc.f90:16-190)."""
     vals = {k: list(getattr(case, k)) for k in ("fix_rho", "fix_u", "fix_v", "fix_w", "fix_p", "fix_tk",
-                                                  "fix_tw", "fix_twall")}
+                                                  "fix_tw", "fix_twall", "fix_tp")}
     inf = dict(fix_rho=case.rho_inf, fix_u=case.u_inf, fix_v=case.v_inf, fix_w=case.w_inf, fix_p=case.p_inf,
-               fix_tk=case.tk_inf, fix_tw=case.tw_inf, fix_twall=0.0)
+               fix_tk=case.tk_inf, fix_tw=case.tw_inf, fix_twall=0.0, fix_tp=0.0)
     for k in vals:  # fill_fixed_values: the free stream everywhere
         vals[k] = [inf[k]] * 6
     names = {"FIX_DENSITY": "fix_rho", "FIX_X_SPEED": "fix_u", "FIX_Y_SPEED": "fix_v", "FIX_Z_SPEED": "fix_w",
-             "FIX_PRESSURE": "fix_p", "WALL_TEMPERATURE": "fix_twall", "FIX_tk": "fix_tk", "FIX_tw": "fix_tw"}
+             "FIX_PRESSURE": "fix_p", "WALL_TEMPERATURE": "fix_twall", "FIX_tk": "fix_tk", "FIX_tw": "fix_tw",
+             "TOTAL_PRESSURE": "fix_tp"}
     face = -1
     with open(path) as f:
         for line in f:

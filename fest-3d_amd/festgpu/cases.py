@@ -89,6 +89,7 @@ class Case:
     fix_tk: tuple = (0.0,) * 6
     fix_tw: tuple = (0.0,) * 6
     fix_twall: tuple = (0.0,) * 6
+    fix_tp: tuple = (0.0,) * 6      # total pressure of a -11 face
     name: str = "case"
 
     @property

@@ -53,7 +53,7 @@ class Flow(ctypes.Structure):
 class BC(ctypes.Structure):
     _fields_ = [("id", ctypes.c_int32 * 6)] + [(n, ctypes.c_double * 6) for n in
                 ("fixed_density", "fixed_x_speed", "fixed_y_speed", "fixed_z_speed", "fixed_pressure",
-                 "fixed_tk", "fixed_tw", "fixed_wall_temperature")]
+                 "fixed_tk", "fixed_tw", "fixed_wall_temperature", "fixed_Tpressure")]
 
 
 _lib = None
@@ -129,6 +129,7 @@ def c_structs(case: Case):
     b.fixed_tk[:] = list(case.fix_tk)
     b.fixed_tw[:] = list(case.fix_tw)
     b.fixed_wall_temperature[:] = list(case.fix_twall)
+    b.fixed_Tpressure[:] = list(case.fix_tp)
     return e, s, f, b
 
 

@@ -56,6 +56,7 @@ module fest_gpu
     integer(c_int32_t) :: id(6)
     real(c_double) :: fixed_density(6), fixed_x_speed(6), fixed_y_speed(6), fixed_z_speed(6)
     real(c_double) :: fixed_pressure(6), fixed_tk(6), fixed_tw(6), fixed_wall_temperature(6)
+    real(c_double) :: fixed_Tpressure(6)
   end type fest_bc
 
   !< one interface link of a multiblock grid (the reference's map line,

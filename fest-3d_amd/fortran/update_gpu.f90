@@ -136,6 +136,7 @@ contains
     b%fixed_tk = bc%fixed_tk
     b%fixed_tw = bc%fixed_tw
     b%fixed_wall_temperature = bc%fixed_wall_temperature
+    b%fixed_Tpressure = bc%fixed_Tpressure
   end subroutine describe
 
   !< first call: the device block and its geometry, copied out of the

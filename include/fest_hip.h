@@ -80,6 +80,7 @@ typedef struct {
   int32_t id[6];           /* imin,imax,jmin,jmax,kmin,kmax ids          */
   double fixed_density[6], fixed_x_speed[6], fixed_y_speed[6], fixed_z_speed[6];
   double fixed_pressure[6], fixed_tk[6], fixed_tw[6], fixed_wall_temperature[6];
+  double fixed_Tpressure[6];  /* total pressure of a -11 face (read_bc.f90:79-80) */
 } fest_bc;
 
 typedef struct fest_handle fest_handle;

@@ -41,7 +41,8 @@ class CCase(ctypes.Structure):
                 ("fix_rho", ctypes.c_double * 6), ("fix_u", ctypes.c_double * 6),
                 ("fix_v", ctypes.c_double * 6), ("fix_w", ctypes.c_double * 6),
                 ("fix_p", ctypes.c_double * 6), ("fix_tk", ctypes.c_double * 6),
-                ("fix_tw", ctypes.c_double * 6), ("fix_twall", ctypes.c_double * 6)]
+                ("fix_tw", ctypes.c_double * 6), ("fix_twall", ctypes.c_double * 6),
+                ("fix_tp", ctypes.c_double * 6)]
 
 
 def to_ccase(case) -> CCase:
@@ -57,7 +58,7 @@ def to_ccase(case) -> CCase:
     for f in ("global_time_step", "CFL", "rho_inf", "u_inf", "v_inf", "w_inf", "p_inf", "tk_inf",
               "tw_inf", "vel_mag", "MInf", "gm", "R_gas", "mu_ref", "T_ref", "S_ref", "Pr", "tPr"):
         setattr(c, f, float(getattr(case, f)))
-    for f in ("fix_rho", "fix_u", "fix_v", "fix_w", "fix_p", "fix_tk", "fix_tw", "fix_twall"):
+    for f in ("fix_rho", "fix_u", "fix_v", "fix_w", "fix_p", "fix_tk", "fix_tw", "fix_twall", "fix_tp"):
         getattr(c, f)[:] = [float(v) for v in getattr(case, f)]
     return c
 

@@ -301,8 +301,31 @@ int orc_geometry(orc *s) {
     for (size_t f = 0; f < fn[d]; ++f)
       if (A[f] != 0.) { nx[f] = nx[f] / A[f]; ny[f] = ny[f] / A[f]; nz[f] = nz[f] / A[f]; }
   }
-  for (int f = 0; f < 6; ++f)
-    if (s->c.bc_id[f] == -7) return ORC_E_CONFIG; /* pole metrics: not restated */
+  /* pole faces, src/geometry.f90:101-186 and :227-232: the pole face and
+   * its three ghost faces get zero area and the unit normal of the first
+   * interior face (index 2, or N-1 at a max face), over the whole plane */
+  for (int f = 0; f < 6; ++f) {
+    if (s->c.bc_id[f] != -7) continue;
+    /* on a one-cell k extent the two k faces' layers overlap; not restated */
+    if (f >= 4 && KMX == 2) return ORC_E_CONFIG;
+    int d = f / 2, hi = f & 1;
+    int N = d == 0 ? IMX : d == 1 ? JMX : KMX;
+    int ae = (d == 0 ? JMX : IMX) + 2, be = (d == 2 ? JMX : KMX) + 2;
+    int src = hi ? N - 1 : 2;
+    A = s->a[fa[d][0]]; nx = s->a[fa[d][1]]; ny = s->a[fa[d][2]]; nz = s->a[fa[d][3]];
+    for (int l = 0; l < 4; ++l) {
+      int n = hi ? N + l : 1 - l;
+      for (int b = -2; b <= be; ++b)
+        for (int a = -2; a <= ae; ++a) {
+          size_t t, o;
+          if (d == 0) { t = IFI(s, n, a, b); o = IFI(s, src, a, b); }
+          else if (d == 1) { t = JFI(s, a, n, b); o = JFI(s, a, src, b); }
+          else { t = KFI(s, a, b, n); o = KFI(s, a, b, src); }
+          A[t] = 0.;
+          nx[t] = nx[o]; ny[t] = ny[o]; nz[t] = nz[o];
+        }
+    }
+  }
   /* volumes, src/geometry.f90:448-496 */
   double *vol = s->a[ORC_VOL];
   for (size_t n = 0; n < s->ncell; ++n) vol[n] = 1.;
@@ -569,8 +592,12 @@ static void bface_normal(const orc *s, int face, int a, int b, double *nx, doubl
   }
 }
 
-/* src/boundary/bc_primitive.f90:645-1234 */
-static void far_field(orc *s, int face) {
+/* src/boundary/bc_primitive.f90:645-1234 (far field) and, with total != 0,
+ * :1237-1776 (total-pressure inlet): the same Riemann-invariant velocity;
+ * the total-pressure inlet then takes the ghost pressure from the face's
+ * total pressure and the boundary Mach number, and the density from the
+ * boundary speed of sound */
+static void riemann_face(orc *s, int face, int total) {
   const orc_case *c = &s->c;
   double gm = c->gm;
   double *rho = QPL(0), *u = QPL(1), *v = QPL(2), *w = QPL(3), *p = QPL(4);
@@ -597,19 +624,28 @@ static void far_field(orc *s, int face) {
       u[cg] = u[ci] + vel_diff * mx;
       v[cg] = v[ci] + vel_diff * my;
       w[cg] = w[ci] + vel_diff * mz;
-      double sen = p[ci] / pow(rho[ci], gm);
-      rho[cg] = pow(Cb * Cb / (gm * sen), 1. / (gm - 1.));
-      p[cg] = (rho[cg] * Cb * Cb / gm);
+      if (!total) {
+        double sen = p[ci] / pow(rho[ci], gm);
+        rho[cg] = pow(Cb * Cb / (gm * sen), 1. / (gm - 1.));
+        p[cg] = (rho[cg] * Cb * Cb / gm);
+      }
       last_positive = 1;
     } else {
       double vel_diff = Unb - Uninf;
       u[cg] = c->u_inf + vel_diff * mx;
       v[cg] = c->v_inf + vel_diff * my;
       w[cg] = c->w_inf + vel_diff * mz;
-      double sen = c->p_inf / pow(c->rho_inf, gm);
-      rho[cg] = pow(Cb * Cb / (gm * sen), 1. / (gm - 1.));
-      p[cg] = (rho[cg] * Cb * Cb / gm);
+      if (!total) {
+        double sen = c->p_inf / pow(c->rho_inf, gm);
+        rho[cg] = pow(Cb * Cb / (gm * sen), 1. / (gm - 1.));
+        p[cg] = (rho[cg] * Cb * Cb / gm);
+      }
       last_positive = 0;
+    }
+    if (total) {
+      double Mb = sqrt(u[cg] * u[cg] + v[cg] * v[cg] + w[cg] * w[cg]) / Cb;
+      p[cg] = c->fix_tp[face] / pow((1 + 0.5 * (gm - 1.) * Mb * Mb), gm / (gm - 1.));
+      rho[cg] = gm * p[cg] / (Cb * Cb);
     }
   }
   /* the turbulent ghosts end as set by the last face cell in loop order:
@@ -726,9 +762,9 @@ static int populate_ghost_primitive(orc *s, int current_iter) {
         for (int l = 0; l < 5; ++l) copy3(s, QPL(l), CP_FLAT, f);
         turb_copy(s, CP_FLAT, f);
         break;
-      case -8: far_field(s, f); break;
+      case -8: riemann_face(s, f, 0); break;
       case -9: periodic_bc(s, f); break;
-      case -11: return ORC_E_CONFIG; /* total-pressure inlet: not restated */
+      case -11: riemann_face(s, f, 1); break;
       default: break; /* interfaces (>=0) and -10 untouched */
     }
   }

@@ -51,6 +51,7 @@ typedef struct {
   double gm, R_gas, mu_ref, T_ref, S_ref, Pr, tPr;
   double fix_rho[6], fix_u[6], fix_v[6], fix_w[6], fix_p[6];
   double fix_tk[6], fix_tw[6], fix_twall[6];
+  double fix_tp[6]; /* total pressure of a -11 face */
 } orc_case;
 
 typedef struct orc orc;

@@ -42,6 +42,7 @@ module ref_harness
     real(c_double) :: gm, R_gas, mu_ref, T_ref, S_ref, Pr, tPr
     real(c_double) :: fix_rho(6), fix_u(6), fix_v(6), fix_w(6), fix_p(6)
     real(c_double) :: fix_tk(6), fix_tw(6), fix_twall(6)
+    real(c_double) :: fix_tp(6)
   end type hcase
 
   type(extent) :: dims
@@ -115,6 +116,7 @@ contains
     bc%fixed_density = hc%fix_rho; bc%fixed_x_speed = hc%fix_u; bc%fixed_y_speed = hc%fix_v
     bc%fixed_z_speed = hc%fix_w; bc%fixed_pressure = hc%fix_p
     bc%fixed_tk = hc%fix_tk; bc%fixed_tw = hc%fix_tw; bc%fixed_wall_temperature = hc%fix_twall
+    bc%fixed_Tpressure = hc%fix_tp
     if (allocated(bc%make_F_flux_zero)) deallocate(bc%make_F_flux_zero, bc%make_G_flux_zero, bc%make_H_flux_zero)
     allocate(bc%make_F_flux_zero(1:imx), bc%make_G_flux_zero(1:jmx), bc%make_H_flux_zero(1:kmx))
     bc%make_F_flux_zero = 1; bc%make_G_flux_zero = 1; bc%make_H_flux_zero = 1

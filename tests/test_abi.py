@@ -34,14 +34,13 @@ def test_struct_sizes_match_header():
     # fest_extent is four int32; fest_scheme 12 int32 + one double
     assert ctypes.sizeof(native.Extent) == 16
     assert ctypes.sizeof(native.Scheme) == 56
-    assert ctypes.sizeof(native.BC) == 24 + 8 * 48
+    assert ctypes.sizeof(native.BC) == 24 + 8 * 54
 
 
 @pytest.mark.parametrize("change,code", [
     (dict(flux="roe"), None),
-    (dict(bc_id=(-7, -9, -5, -5, -9, -9)), -3),
-    (dict(bc_id=(-11, -9, -5, -5, -9, -9)), -3),
     (dict(bc_id=(-12, -9, -5, -5, -9, -9)), -3),
+    (dict(bc_id=(-9, -9, -5, -5, -7, -9), kmx=2), -3),  # a k pole on a one-cell k extent
     (dict(time_accuracy="RK2", bc_id=(-9, -9, -5, -5, -9, -9), imx=3), -3),
     (dict(kmx=3), -3),  # two cells in k: neither 3-D nor the reference's 2-D
 ])
@@ -61,6 +60,16 @@ def test_precision_modes():
     with pytest.raises(native.FestError) as e:
         native.GpuBlock(case, precision=7)
     assert e.value.code == -3
+
+
+@pytest.mark.parametrize("ids", [(-7, -9, -5, -5, -9, -9), (-11, -4, -5, -5, -9, -9)])
+def test_pole_and_total_pressure_pass_the_configuration_check(ids):
+    """Without a GPU the handle fails on the device, after the check."""
+    case, _, _ = C.laminar_box(8, 6, 5)
+    try:
+        native.GpuBlock(dataclasses.replace(case, bc_id=ids)).close()
+    except native.FestError as e:
+        assert e.code == -4, e
 
 
 def test_two_dimensional_extent_passes_the_configuration_check():

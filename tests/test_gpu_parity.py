@@ -20,6 +20,7 @@ from binding import Oracle
 from festgpu import cases as C
 from festgpu.native import FestError, GpuBlock
 from props import boundary_flux_mismatch
+from bc_cases import POLE, TOTAL_PRESSURE, pole_case, total_pressure_case
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
@@ -297,3 +298,59 @@ def test_sst2003_steps(rule):
     assert relerr(out["qp"], o.arr("qp")) <= TOL
     assert relerr(out["mut"], o.arr("mut")) <= TOL
     assert relerr(out["residue"], o.arr("res")) <= 1e-10
+
+
+@pytest.mark.parametrize("face,turb", TOTAL_PRESSURE)
+def test_total_pressure_inlet(face, turb):
+    """-11 faces: the residual (ghost states, mu_t and F1 at the ghost cells,
+    which keep their computed values), then RK4 and implicit steps."""
+    case, nodes, q = total_pressure_case(face, turb, ni=20, nj=12, nk=6)
+    sst = turb != "none"
+    say("total pressure", face, turb)
+    g, o = pair(case, nodes, q, wall=sst)
+    o.temp_refresh()
+    assert o.residual(3) == 0
+    g.residual(3)
+    names = ("qp", "F", "G", "H", "residue") + (("mut", "f1") if sst else ())
+    out = g.download(*names)
+    assert relerr(out["qp"], o.arr("qp")) <= TOL
+    for n, on in (("F", "F"), ("G", "G"), ("H", "H"), ("residue", "res")):
+        assert relerr(out[n], o.arr(on)) <= TOL, n
+    if sst:
+        inner = (slice(2, -2),) * 3
+        assert relerr(out["mut"][inner], o.arr("mut")[inner]) <= TOL
+        assert relerr(out["f1"][inner], o.arr("f1")[inner]) <= TOL
+    for rule in ("RK4", "implicit"):
+        c2 = dataclasses.replace(case, time_accuracy=rule)
+        g, o = pair(c2, nodes, q, wall=sst)
+        for it in (1, 2):
+            assert o.step(it) == 0
+            g.get_next_solution(it)
+        assert relerr(g.download("qp"), o.arr("qp")) <= TOL, rule
+
+
+@pytest.mark.parametrize("bcs,recon", POLE)
+def test_pole(bcs, recon):
+    """-7 faces: the pole metrics bit for bit, the residual, and RK4 and
+    implicit steps."""
+    case, nodes, q = pole_case(bcs, recon, ni=16, nj=12, nk=8)
+    say("pole", bcs, recon)
+    g, o = pair(case, nodes, q)
+    faces = g.download("faces")
+    for d, p in enumerate("ijk"):
+        for a, n in enumerate(("A", "nx", "ny", "nz")):
+            assert np.array_equal(faces[d][a], o.arr(p + n)), p + n
+    o.temp_refresh()
+    assert o.residual(1) == 0
+    g.residual(1)
+    out = g.download("qp", "F", "G", "H", "residue")
+    assert relerr(out["qp"], o.arr("qp")) <= TOL
+    for n, on in (("F", "F"), ("G", "G"), ("H", "H"), ("residue", "res")):
+        assert relerr(out[n], o.arr(on)) <= TOL, n
+    for rule in ("RK4", "implicit"):
+        c2 = dataclasses.replace(case, time_accuracy=rule)
+        g, o = pair(c2, nodes, q)
+        for it in (1, 2):
+            assert o.step(it) == 0
+            g.get_next_solution(it)
+        assert relerr(g.download("qp"), o.arr("qp")) <= TOL, rule
