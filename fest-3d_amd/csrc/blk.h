@@ -42,7 +42,7 @@ struct Blk {
   double *g[3];
   double *nodes;              // grid nodes (3 planes) when set on device
   double *red;                // reduction partials
-  double *norms;              // n_var+1 absolute norms
+  double *norms;              // n_var+1 absolute norms; from 8, the block's unassembled share
   double *dtmin;              // global time step
   unsigned *err;              // error bits
   // scheme

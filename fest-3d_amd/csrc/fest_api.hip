@@ -624,7 +624,7 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
   h->norm_blocks = (int)std::min<long>(1024, std::max<long>(1, cdiv(b.nres, 256)));
   h->red_cap = std::max<long>(nblk_dt, 8L * h->norm_blocks);
   AL(b.red, h->red_cap);
-  AL(b.norms, 8); AL(b.dtmin, 1);
+  AL(b.norms, 16); AL(b.dtmin, 1);
   if (b.time_acc == FEST_TA_IMPLICIT) {
     // LU-SGS storage over the hyperplanes of (0:imx, 0:jmx, 0:kmx); zero
     // where nothing is written (the sweep values outside the interior)
@@ -1045,6 +1045,28 @@ int fest_group_wall_dist(fest_group *g) {
     if ((rc = wall_nodes(h, wall))) return gfail(g, rc, "%s", h->err.c_str());
   for (fest_handle *h : g->blocks)
     if ((rc = dist_from_walls(h, wall))) return gfail(g, rc, "%s", h->err.c_str());
+  return 0;
+}
+
+int fest_group_resnorm(fest_group *g, double *res_abs) {
+  if (!g || !res_abs) return FEST_E_ARG;
+  if (g->broken) return FEST_E_ARG;
+  const int nv = g->blocks[0]->b.nv;
+  std::vector<double> tot(nv + 1, 0.0), part(nv + 1);
+  for (fest_handle *h : g->blocks) {
+    Blk &b = h->b;
+    hipLaunchKernelGGL(k_norm_partial, dim3(h->norm_blocks), dim3(256), 0, h->stream, b);
+    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(64), 0, h->stream, b, h->norm_blocks);
+    if (hipGetLastError() != hipSuccess) return gfail(g, FEST_E_HIP, "norm launch failed");
+    if (hipMemcpyAsync(part.data(), b.norms + 8, sizeof(double) * (nv + 1), hipMemcpyDeviceToHost, h->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+      return gfail(g, FEST_E_HIP, "norm download failed");
+    // MPI_ALLGATHER then the sum in process order (src/resnorm.f90:193-213)
+    for (int l = 0; l <= nv; ++l) tot[l] += part[l];
+  }
+  res_abs[0] = fabs(tot[0]);
+  for (int l = 1; l <= nv; ++l) res_abs[l] = sqrt(tot[l]);
   return 0;
 }
 

@@ -243,6 +243,10 @@ __global__ void k_norm_final(Blk b, int nblk) {
   scale[7] = b.rho_inf * v * b.tw_inf;
   b.norms[0] = fabs(acc[0] / scale[0]);
   for (int l = 1; l <= b.nv; ++l) b.norms[l] = sqrt(acc[l] / (scale[l] * scale[l]));
+  // the block's share before the cross-block assembly (src/resnorm.f90:
+  // 171-225): the signed mass imbalance and sum(R_l^2)/scale_l^2
+  b.norms[8] = acc[0] / scale[0];
+  for (int l = 1; l <= b.nv; ++l) b.norms[8 + l] = acc[l] / (scale[l] * scale[l]);
 }
 
 }  // namespace fest

@@ -88,13 +88,15 @@ def lib():
         L.fest_group_last_error.restype = ctypes.c_char_p
         L.fest_group_exchange.argtypes = [vp]
         L.fest_group_wall_dist.argtypes = [vp]
+        L.fest_group_resnorm.argtypes = [vp, dp]
         L.fest_group_get_next_solution.argtypes = [vp, ip, ctypes.c_double]
         L.fest_timing.argtypes = [vp, ctypes.c_char_p]
         L.fest_kernel_time.argtypes = [vp, ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_long)]
         for f in ("fest_create", "fest_upload_geometry", "fest_set_grid", "fest_upload_state",
                   "fest_get_next_solution", "fest_residual", "fest_time_step", "fest_download", "fest_resnorm",
                   "fest_device_ptr", "fest_sync", "fest_timing", "fest_kernel_time", "fest_group_create",
-                  "fest_group_exchange", "fest_group_wall_dist", "fest_group_get_next_solution"):
+                  "fest_group_exchange", "fest_group_wall_dist", "fest_group_resnorm",
+                  "fest_group_get_next_solution"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -293,6 +295,12 @@ class GpuGroup:
 
     def exchange(self):
         self._check(lib().fest_group_exchange(self.g))
+
+    def resnorm(self) -> np.ndarray:
+        """Res_abs(0:n_var) of the whole group (find_resnorm)."""
+        out = np.zeros(8)
+        self._check(lib().fest_group_resnorm(self.g, _ptr(out)))
+        return out[: self.blocks[0].case.n_var + 1]
 
     def get_next_solution(self, current_iter: int, cfl: float = None):
         c = self.blocks[0].case.CFL if cfl is None else cfl

@@ -73,7 +73,7 @@ module fest_gpu
             fest_upload_state, fest_get_next_solution, fest_residual, fest_time_step, &
             fest_download, fest_resnorm, fest_sync, fest_check, &
             fest_group_create, fest_group_destroy, fest_group_exchange, fest_group_wall_dist, &
-            fest_group_get_next_solution
+            fest_group_resnorm, fest_group_get_next_solution
 
   interface
     integer(c_int) function fest_create(h, dims, scheme, flow, bc, precision) bind(c, name='fest_create')
@@ -185,6 +185,12 @@ module fest_gpu
       import :: c_int, c_ptr
       type(c_ptr), value :: g
     end function fest_group_wall_dist
+
+    integer(c_int) function fest_group_resnorm(g, res_abs) bind(c, name='fest_group_resnorm')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: g
+      real(c_double), intent(out) :: res_abs(*)
+    end function fest_group_resnorm
 
     integer(c_int) function fest_group_get_next_solution(g, current_iter, cfl) &
         bind(c, name='fest_group_get_next_solution')

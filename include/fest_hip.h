@@ -209,6 +209,11 @@ int fest_group_exchange(fest_group *g);
  * (src/wall/wall.f90:55-111, src/wall/wall_dist.f90:84-131).  Call after
  * fest_set_grid on every block; replaces their single-block distances. */
 int fest_group_wall_dist(fest_group *g);
+/* Absolute residual norms Res_abs(0:n_var) of the whole group, assembled
+ * from every block's last residue as find_resnorm does across processes
+ * (src/resnorm.f90:62-91,171-225): the blocks' signed mass imbalances and
+ * sum(R_l^2)/scale_l^2 summed in block order, then |mass| and square roots. */
+int fest_group_resnorm(fest_group *g, double *res_abs);
 /* get_next_solution (src/update.f90:129-226) on every block, stage by stage,
  * the interface exchange opening every residual evaluation
  * (src/update.f90:534). */

@@ -11,6 +11,9 @@ reference's own report (tests/Report.txt):
               2.872E-03, restarted (control.md "Restart level" 2) from the
               state its time_directories/0002 holds, also copied
 
+The restart logs of the Tfp and SmoothBump dumps are copied too; they pin
+the restart-log writer's line formats.
+
 Run from the repository root:  python tests/golden/copy_cases.py
 """
 import os
@@ -24,7 +27,10 @@ FILES = ["system/control.md", "system/fvscheme.md", "system/flow.md",
          "system/mesh/gridfiles/grid_00.txt", "system/mesh/gridfiles/grid_01.txt"]
 
 RESTART = {"Tfp": ["system/output_control.md", "time_directories/0002/process_00.dat",
-                   "time_directories/0002/process_01.dat"]}
+                   "time_directories/0002/process_01.dat",
+                   "time_directories/0002/restart/process_00", "time_directories/0002/restart/process_01"],
+           "SmoothBump": ["time_directories/0010/restart/process_00",
+                          "time_directories/0010/restart/process_01"]}
 
 if __name__ == "__main__":
     for case in ("SmoothBump", "Lfp", "Tfp"):
