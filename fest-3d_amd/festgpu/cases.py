@@ -27,7 +27,7 @@ import numpy as np
 # flux / reconstruction / turbulence ids shared with the C-ABI
 FLUX = {"ausm": 0, "ausmP": 1, "ausmUP": 2, "slau": 3, "ldfss0": 4, "van_leer": 5}
 RECON = {"none": 0, "muscl": 1, "ppm": 2, "weno": 3}
-TURB = {"none": 0, "sst": 1, "sst2003": 2}
+TURB = {"none": 0, "sst": 1, "sst2003": 2, "sa": 3}
 TIME_ACC = {"none": 0, "RK2": 2, "RK4": 4, "TVDRK2": 12, "TVDRK3": 13, "implicit": 20}
 
 _GOLDEN = np.uint64(0x9E3779B97F4A7C15)
@@ -90,6 +90,8 @@ class Case:
     fix_tw: tuple = (0.0,) * 6
     fix_twall: tuple = (0.0,) * 6
     fix_tp: tuple = (0.0,) * 6      # total pressure of a -11 face
+    tv_inf: float = 0.0             # Spalart-Allmaras free-stream nu-tilde
+    fix_tv: tuple = (0.0,) * 6      # fixed nu-tilde of each face
     name: str = "case"
 
     @property
@@ -100,7 +102,13 @@ class Case:
 
     @property
     def n_var(self) -> int:
-        return 7 if self.sst_family else 5
+        return 7 if self.sst_family else 6 if self.turb == "sa" else 5
+
+    @property
+    def n_grad(self) -> int:
+        """Gradient slots u, v, w, T, then k, omega (SST) or nu-tilde (SA)
+        (src/gradients.f90:236-260)."""
+        return 6 if self.sst_family else 5 if self.turb == "sa" else 4
 
     @property
     def ncells(self) -> int:
@@ -133,6 +141,8 @@ def freestream(case: Case, T_inf=300.0, p_inf=101325.0, mach=0.5, reynolds=1000.
         tu = tu_percent / 100
         case.tk_inf = 1.5 * ((case.vel_mag * tu) ** 2)
         case.tw_inf = rho * case.tk_inf / (case.mu_ref * mu_ratio)
+    if case.turb == "sa":
+        case.tv_inf = mu_ratio * case.mu_ref / rho
     return case
 
 
@@ -160,7 +170,8 @@ def case_from_json(text: str) -> Case:
 
 def freestream_state(case: Case) -> np.ndarray:
     q = np.empty(case.cell_shape(case.n_var))
-    vals = [case.rho_inf, case.u_inf, case.v_inf, case.w_inf, case.p_inf, case.tk_inf, case.tw_inf]
+    vals = [case.rho_inf, case.u_inf, case.v_inf, case.w_inf, case.p_inf]
+    vals += [case.tv_inf] if case.turb == "sa" else [case.tk_inf, case.tw_inf]
     for l in range(case.n_var):
         q[l] = vals[l]
     return q
@@ -276,12 +287,13 @@ def laminar_box(ni=64, nj=64, nk=4, seed=1, flux="ausmP", recon="muscl", time_lo
     return c, nodes, q
 
 
-def sst_bump(ni=256, nj=128, nk=128, seed=2, flux="slau", recon="weno", first=1e-5):
+def sst_bump(ni=256, nj=128, nk=128, seed=2, flux="slau", recon="weno", first=1e-5, turb="sst"):
     """RANS SST channel over a 0.05L sinusoidal bump: WENO5 + SLAU, M=0.2,
     Re=1e6 per L, Tu=0.1%, mu_t/mu=0.01, 0.5% seeded noise.  Boundaries:
     subsonic inlet (-3), subsonic outlet (-4), adiabatic wall at jmin (-5),
-    far field at jmax (-8), periodic in k (-9)."""
-    c = Case(ni + 1, nj + 1, nk + 1, flux=flux, recon=recon, turb="sst", time_local=1,
+    far field at jmax (-8), periodic in k (-9).  turb="sa" gives the same
+    channel with Spalart-Allmaras (nu-tilde = mu_ratio mu / rho)."""
+    c = Case(ni + 1, nj + 1, nk + 1, flux=flux, recon=recon, turb=turb, time_local=1,
              accur=1, lim=(1, 1, 1), tlim=(1, 1, 1), mu_sutherland=1,
              bc_id=(-3, -4, -5, -8, -9, -9), CFL=0.5, name=f"sst_bump_{ni}x{nj}x{nk}")
     freestream(c, mach=0.2, reynolds=1.0e6, tu_percent=0.1, mu_ratio=0.01)
@@ -296,6 +308,7 @@ def sst_bump(ni=256, nj=128, nk=128, seed=2, flux="slau", recon="weno", first=1e
     c.fix_tk = (inf[5],) * 6
     c.fix_tw = (inf[6],) * 6
     c.fix_twall = (0.0,) * 6
+    c.fix_tv = (c.tv_inf,) * 6
     nodes = bump_nodes(c, first=first)
     q = uniform_noise(c, freestream_state(c), seed, 0.005)
     return c, nodes, q

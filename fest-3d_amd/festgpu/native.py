@@ -224,7 +224,7 @@ class GpuBlock:
     # -- downloads in the reference layout
     def shape_of(self, name):
         c = self.case
-        nv, ng = c.n_var, (6 if c.sst_family else 4)
+        nv, ng = c.n_var, c.n_grad
         cell = c.cell_shape()
         return {
             "qp": (nv,) + cell, "temp": cell, "mu": cell, "mut": cell, "f1": cell, "dist": cell,

@@ -42,7 +42,8 @@ class CCase(ctypes.Structure):
                 ("fix_v", ctypes.c_double * 6), ("fix_w", ctypes.c_double * 6),
                 ("fix_p", ctypes.c_double * 6), ("fix_tk", ctypes.c_double * 6),
                 ("fix_tw", ctypes.c_double * 6), ("fix_twall", ctypes.c_double * 6),
-                ("fix_tp", ctypes.c_double * 6)]
+                ("fix_tp", ctypes.c_double * 6), ("tv_inf", ctypes.c_double),
+                ("fix_tv", ctypes.c_double * 6)]
 
 
 def to_ccase(case) -> CCase:
@@ -58,7 +59,8 @@ def to_ccase(case) -> CCase:
     for f in ("global_time_step", "CFL", "rho_inf", "u_inf", "v_inf", "w_inf", "p_inf", "tk_inf",
               "tw_inf", "vel_mag", "MInf", "gm", "R_gas", "mu_ref", "T_ref", "S_ref", "Pr", "tPr"):
         setattr(c, f, float(getattr(case, f)))
-    for f in ("fix_rho", "fix_u", "fix_v", "fix_w", "fix_p", "fix_tk", "fix_tw", "fix_twall", "fix_tp"):
+    c.tv_inf = float(case.tv_inf)
+    for f in ("fix_rho", "fix_u", "fix_v", "fix_w", "fix_p", "fix_tk", "fix_tw", "fix_twall", "fix_tp", "fix_tv"):
         getattr(c, f)[:] = [float(v) for v in getattr(case, f)]
     return c
 
@@ -66,7 +68,7 @@ def to_ccase(case) -> CCase:
 def array_shape(case, name):
     """numpy (C-order) shape of each array in the Fortran layout."""
     imx, jmx, kmx, nv = case.imx, case.jmx, case.kmx, case.n_var
-    ng = 6 if case.sst_family else 4
+    ng = case.n_grad
     cell = (kmx + 5, jmx + 5, imx + 5)
     shapes = {
         "qp": (nv,) + cell, "ustore": (nv,) + cell,

@@ -94,6 +94,22 @@ static inline double sq(double x) { return x * x; }
 /* SST constants, src/global/global_sst.f90:6-14 */
 static const double sigma_k1 = 0.85, sigma_k2 = 1.0, sigma_w1 = 0.5, sigma_w2 = 0.856;
 static const double beta1 = 0.075, beta2 = 0.0828, bstar = 0.09, kappa_sst = 0.41, a1_sst = 0.31;
+/* Spalart-Allmaras, src/global/global_sa.f90; cw1, cv1^3 and cw3^6 are the
+ * reference's parameter expressions evaluated in double */
+static const double cb1 = 0.1355, cb2 = 0.6220, cw2 = 0.3, cw3 = 2.0, cv1 = 7.1, sigma_sa = 2. / 3.,
+                    kappa_sa = 0.41;
+#define CW1_SA ((cb1 / (kappa_sa * kappa_sa)) + ((1 + cb2) / sigma_sa))
+#define CV1_3 ((cv1 * cv1) * cv1)
+#define CW3_6 (((((cw3 * cw3) * cw3) * cw3) * cw3) * cw3)
+/* x**n with an integer constant n, multiplied left to right as amdflang
+ * evaluates it */
+static inline double ipow(double x, int n) {
+  double r = x;
+  for (int m = 1; m < n; ++m) r = r * x;
+  return r;
+}
+#define SST(s) ((s)->c.turb == ORC_TURB_SST || (s)->c.turb == ORC_TURB_SST2003)
+#define SA(s) ((s)->c.turb == ORC_TURB_SA)
 
 /* ----------------------------------------------------------- allocation */
 orc *orc_create(const orc_case *c) {
@@ -101,7 +117,7 @@ orc *orc_create(const orc_case *c) {
   if (!s) return NULL;
   s->c = *c;
   s->imx = c->imx; s->jmx = c->jmx; s->kmx = c->kmx; s->nv = c->n_var;
-  s->ngrad = (c->turb != ORC_TURB_NONE) ? 6 : 4;
+  s->ngrad = (c->turb == ORC_TURB_SA) ? 5 : (c->turb != ORC_TURB_NONE) ? 6 : 4;
   s->pi = s->imx + 5; s->pj = s->jmx + 5; s->pk = s->kmx + 5;
   s->ncell = (size_t)s->pi * s->pj * s->pk;
   s->nif = (size_t)(s->imx + 6) * (s->jmx + 5) * (s->kmx + 5);
@@ -569,13 +585,15 @@ static void set_omega_at_wall(orc *s, int face) {
 }
 
 static void turb_copy(orc *s, int type, int face) {
-  if (s->c.turb != ORC_TURB_NONE) { copy3(s, QPL(5), type, face); copy3(s, QPL(6), type, face); }
+  if (SST(s)) { copy3(s, QPL(5), type, face); copy3(s, QPL(6), type, face); }
+  if (SA(s)) copy3(s, QPL(5), type, face);
 }
 static void turb_fix(orc *s, int face) {
-  if (s->c.turb != ORC_TURB_NONE) {
+  if (SST(s)) {
     fixv(s, QPL(5), s->c.fix_tk[face], face);
     fixv(s, QPL(6), s->c.fix_tw[face], face);
   }
+  if (SA(s)) fixv(s, QPL(5), s->c.fix_tv[face], face);
 }
 
 /* face-normal unit vector stored at the boundary face (outward sign applied
@@ -751,7 +769,8 @@ static int populate_ghost_primitive(orc *s, int current_iter) {
         copy3(s, QPL(4), CP_SYMM, f);
         temp_based_density(s, f);
         copy3(s, QPL(1), CP_ANTI, f); copy3(s, QPL(2), CP_ANTI, f); copy3(s, QPL(3), CP_ANTI, f);
-        if (c->turb != ORC_TURB_NONE) { copy3(s, QPL(5), CP_ANTI, f); set_omega_at_wall(s, f); }
+        if (SST(s)) { copy3(s, QPL(5), CP_ANTI, f); set_omega_at_wall(s, f); }
+        if (SA(s)) copy3(s, QPL(5), CP_ANTI, f); /* no_slip, bc_primitive.f90:540-541 */
         break;
       case -6: /* slip wall */
         copy3(s, QPL(0), CP_SYMM, f); copy3(s, QPL(4), CP_SYMM, f);
@@ -1300,7 +1319,17 @@ static int calculate_viscosity(orc *s) {
       mu[ci] = c->mu_ref * (pow(T / c->T_ref, 1.5)) * ((c->T_ref + c->S_ref) / (T + c->S_ref));
     }
   }
-  if (c->turb != ORC_TURB_NONE) {
+  if (SA(s)) {
+    /* src/viscosity.f90:149-163: mu_t = rho nu~ fv1 */
+    for (int k = 0; k <= KMX; ++k) for (int j = 0; j <= JMX; ++j) for (int i = 0; i <= IMX; ++i) {
+      size_t ci = CI(s, i, j, k);
+      double tv = QP(5, i, j, k), density = QP(0, i, j, k);
+      double xi = tv * density / mu[ci];
+      double fv1 = ipow(xi, 3) / (ipow(xi, 3) + CV1_3);
+      mut[ci] = density * tv * fv1;
+    }
+  }
+  if (SST(s)) {
     for (int k = 0; k <= KMX; ++k) for (int j = 0; j <= JMX; ++j) for (int i = 0; i <= IMX; ++i) {
       size_t ci = CI(s, i, j, k);
       double density = QP(0, i, j, k), tk = QP(5, i, j, k), tw = QP(6, i, j, k);
@@ -1335,6 +1364,8 @@ static int calculate_viscosity(orc *s) {
       double arg1 = fmn(left, right);
       F1[ci] = tanh(arg1 * arg1 * arg1 * arg1);
     }
+  }
+  if (c->turb != ORC_TURB_NONE) {
     for (int f = 0; f < 6; ++f) {
       int id = c->bc_id[f];
       if (id >= 0 || id == -10) continue;
@@ -1344,7 +1375,7 @@ static int calculate_viscosity(orc *s) {
       for (int b = 1; b <= nb; ++b) for (int a = 1; a <= na; ++a) {
         size_t cg = face_cell(s, f, 0, a, b), ci = face_cell(s, f, 1, a, b);
         mut[cg] = anti ? -mut[ci] : mut[ci];
-        F1[cg] = F1[ci];
+        if (SST(s)) F1[cg] = F1[ci];
       }
     }
   }
@@ -1420,6 +1451,20 @@ static void viscous_dir(orc *s, int d, int sst) {
           Fl[4 * nfl + fo] = Fl[4 * nfl + fo] - (area * (((Txx * uface + Txy * vface + Txz * wface + Qx) * nx) +
                                                         ((Txy * uface + Tyy * vface + Tyz * wface + Qy) * ny) +
                                                         ((Txz * uface + Tyz * vface + Tzz * wface + Qz) * nz)));
+        } else if (sst == 2) {
+          /* compute_viscous_fluxes_sa, src/viscous.f90:570-656 */
+          double g0 = 0.5 * (GX(4, im, jm, km) + GX(4, i, j, k));
+          double g1 = 0.5 * (GY(4, im, jm, km) + GY(4, i, j, k));
+          double g2 = 0.5 * (GZ(4, im, jm, km) + GZ(4, i, j, k));
+          double deltv = QP(5, i, j, k) - QP(5, im, jm, km);
+          double nc = (deltv - (g0 * delx + g1 * dely + g2 * delz)) / d_LR;
+          g0 = g0 + (nc * delx / d_LR);
+          g1 = g1 + (nc * dely / d_LR);
+          g2 = g2 + (nc * delz / d_LR);
+          double rhoface = 0.5 * (QP(0, im, jm, km) + QP(0, i, j, k));
+          double mut_f = 0.5 * (QP(5, im, jm, km) + QP(5, i, j, k)) * rhoface;
+          Fl[5 * nfl + fo] =
+              Fl[5 * nfl + fo] - (area * ((mu_f + mut_f) * (g0 * nx + g1 * ny + g2 * nz))) / sigma_sa;
         } else {
           double g[2][3];
           for (int l = 0; l < 2; ++l) {
@@ -1456,11 +1501,15 @@ static void viscous_dir(orc *s, int d, int sst) {
 
 static int compute_viscous_fluxes(orc *s) {
   for (int d = 0; d < 3; ++d) viscous_dir(s, d, 0);
-  if (s->c.turb != ORC_TURB_NONE) {
+  if (SST(s)) {
     viscous_dir(s, 0, 1);
     viscous_dir(s, 1, 1);
     if (KMX != 2) viscous_dir(s, 2, 1);
   }
+  /* the SA flux runs in all three directions, 2-D grids too
+   * (src/viscous.f90:94-97) */
+  if (SA(s))
+    for (int d = 0; d < 3; ++d) viscous_dir(s, d, 2);
   size_t nn[3] = {s->nF, s->nG, s->nH};
   double *arr[3] = {s->a[ORC_F], s->a[ORC_G], s->a[ORC_H]};
   for (int d = 0; d < 3; ++d)
@@ -1478,6 +1527,63 @@ static void compute_residue(orc *s) {
           (F[(size_t)l * s->nF + FI(s, i + 1, j, k)] - F[(size_t)l * s->nF + FI(s, i, j, k)]) +
           (G[(size_t)l * s->nG + GI(s, i, j + 1, k)] - G[(size_t)l * s->nG + GI(s, i, j, k)]) +
           (H[(size_t)l * s->nH + HI(s, i, j, k + 1)] - H[(size_t)l * s->nH + HI(s, i, j, k)]);
+}
+
+/* add_sa_source, src/source.f90:835-983.  The density gradient is a
+ * Green-Gauss sum of its own over the face averages; its k-face normal is
+ * (nx, nx, nx), as the reference writes it (source.f90:887). */
+static void add_sa_source(orc *s) {
+  double *R = s->a[ORC_RES], *vol = s->a[ORC_VOL], *mu = s->a[ORC_MU], *dist = s->a[ORC_DIST];
+  const double cw1 = CW1_SA;
+  for (int k = 1; k <= KMX - 1; ++k) for (int j = 1; j <= JMX - 1; ++j) for (int i = 1; i <= IMX - 1; ++i) {
+    size_t ci = CI(s, i, j, k);
+    double density = QP(0, i, j, k), tv = QP(5, i, j, k);
+    double rf[6] = {QP(0, i - 1, j, k) + density, QP(0, i, j - 1, k) + density, QP(0, i, j, k - 1) + density,
+                    QP(0, i + 1, j, k) + density, QP(0, i, j + 1, k) + density, QP(0, i, j, k + 1) + density};
+    size_t f[6] = {IFI(s, i, j, k), JFI(s, i, j, k), KFI(s, i, j, k),
+                   IFI(s, i + 1, j, k), JFI(s, i, j + 1, k), KFI(s, i, j, k + 1)};
+    const int arr[6][4] = {{ORC_IA, ORC_INX, ORC_INY, ORC_INZ}, {ORC_JA, ORC_JNX, ORC_JNY, ORC_JNZ},
+                           {ORC_KA, ORC_KNX, ORC_KNX, ORC_KNX}, {ORC_IA, ORC_INX, ORC_INY, ORC_INZ},
+                           {ORC_JA, ORC_JNX, ORC_JNY, ORC_JNZ}, {ORC_KA, ORC_KNX, ORC_KNY, ORC_KNZ}};
+    double gr[3];
+    for (int c3 = 0; c3 < 3; ++c3) {
+      double g = -(rf[0] * s->a[arr[0][1 + c3]][f[0]] * s->a[arr[0][0]][f[0]]);
+      g = g - rf[1] * s->a[arr[1][1 + c3]][f[1]] * s->a[arr[1][0]][f[1]];
+      g = g - rf[2] * s->a[arr[2][1 + c3]][f[2]] * s->a[arr[2][0]][f[2]];
+      g = g + rf[3] * s->a[arr[3][1 + c3]][f[3]] * s->a[arr[3][0]][f[3]];
+      g = g + rf[4] * s->a[arr[4][1 + c3]][f[4]] * s->a[arr[4][0]][f[4]];
+      g = g + rf[5] * s->a[arr[5][1 + c3]][f[5]] * s->a[arr[5][0]][f[5]];
+      gr[c3] = g / (2.0 * vol[ci]);
+    }
+    double vort = sqrt(sq(GY(2, i, j, k) - GZ(1, i, j, k)) + sq(GZ(0, i, j, k) - GX(2, i, j, k)) +
+                       sq(GX(1, i, j, k) - GY(0, i, j, k)));
+    double CD1 = cb2 * ((GX(4, i, j, k) * GX(4, i, j, k)) + (GY(4, i, j, k) * GY(4, i, j, k)) +
+                        (GZ(4, i, j, k) * GZ(4, i, j, k)));
+    double CD2 = ((gr[0] * GX(4, i, j, k)) + (gr[1] * GY(4, i, j, k)) + (gr[2] * GZ(4, i, j, k)));
+    double kd = kappa_sa * dist[ci], kd2 = kd * kd;
+    double nu = mu[ci] / density;
+    double xi = tv / nu;
+    double fv1 = ipow(xi, 3) / (ipow(xi, 3) + CV1_3);
+    double fv2 = 1.0 - xi / (1.0 + (xi * fv1));
+    double scap = fmx(vort + (tv * fv2 / (kd2)), 0.3 * vort);
+    double r = fmn(tv / (scap * kd2), 10.0);
+    double g = r + cw2 * (ipow(r, 6) - r);
+    double fw = g * pow((1.0 + CW3_6) / (ipow(g, 6) + CW3_6), 1.0 / 6.0);
+    double td = tv / dist[ci];
+    double D_v = density * cw1 * fw * (td * td);
+    double P_v = density * cb1 * scap * tv;
+    double lamda = density * CD1 / sigma_sa - CD2 * (nu + tv) / sigma_sa;
+    double S_v = (P_v - D_v + lamda) * vol[ci];
+    R[(size_t)5 * s->nR + RI(s, i, j, k)] = R[(size_t)5 * s->nR + RI(s, i, j, k)] - S_v;
+  }
+}
+
+
+static void add_sst_source(orc *s);
+/* add_source_term_residue, src/source.f90:94-155 */
+static void add_source_term_residue(orc *s) {
+  if (SST(s)) add_sst_source(s);
+  if (SA(s)) add_sa_source(s);
 }
 
 /* src/source.f90:158-270; sst2003 limits the production to 10 D_k, floors
@@ -1533,7 +1639,7 @@ int orc_stage(orc *s, int current_iter, int mask) {
   }
   if (mask & 16) {
     compute_residue(s);
-    if (s->c.turb != ORC_TURB_NONE) add_sst_source(s);
+    add_source_term_residue(s);
   }
   return ORC_OK;
 }
@@ -1554,7 +1660,7 @@ int orc_residual(orc *s, int current_iter) {
     if ((st = compute_viscous_fluxes(s))) return st;
   }
   compute_residue(s);
-  if (s->c.turb != ORC_TURB_NONE) add_sst_source(s);
+  add_source_term_residue(s);
   return ORC_OK;
 }
 
@@ -1648,10 +1754,27 @@ static int update_with(orc *s, double TF, double SF, int TU, int use_store) {
     double KE = 0.;
     u1[4] = (u1[4] / (gm - 1.) + 0.5 * (u1[1] * u1[1] + u1[2] * u1[2] + u1[3] * u1[3])) / u1[0] + KE;
     for (int l = 0; l < nv; ++l) R[l] = res[(size_t)l * s->nR + ri];
-    if (c->turb != ORC_TURB_NONE) {
+    if (SST(s)) {
       double beta = beta1 * F1a[ci] + (1. - F1a[ci]) * beta2;
       R[5] = R[5] / (1 + (beta * QP(6, i, j, k) * dt[ri]));
       R[6] = R[6] / (1 + (2 * beta * QP(6, i, j, k) * dt[ri]));
+    }
+    if (SA(s)) {
+      /* src/update.f90:405-420: the point-implicit destruction, with
+       * xi = U1(6) * rho / mu from the conservative nu-tilde */
+      const double *mu = s->a[ORC_MU], *dist = s->a[ORC_DIST];
+      double vort = sqrt(sq(GY(2, i, j, k) - GZ(1, i, j, k)) + sq(GZ(0, i, j, k) - GX(2, i, j, k)) +
+                         sq(GX(1, i, j, k) - GY(0, i, j, k)));
+      double kd = kappa_sa * dist[ci], kd2 = kd * kd;
+      double xi = u1[5] * QP(0, i, j, k) / mu[ci];
+      double fv1 = ipow(xi, 3) / (ipow(xi, 3) + CV1_3);
+      double fv2 = 1.0 - xi / (1 + xi * fv1);
+      double scap = vort + u1[5] * fv2 / (kd2);
+      double rsa = fmn(u1[5] / (scap * kd2), 10.0);
+      double g = rsa + cw2 * (ipow(rsa, 6) - rsa);
+      double fw = g * pow((1.0 + CW3_6) / (ipow(g, 6) + CW3_6), 1.0 / 6.0);
+      double d2 = dist[ci] * dist[ci];
+      R[5] = R[5] / (1. + ((-1.0 * u1[0] * cb1 * scap) + (2.0 * u1[0] * CW1_SA * fw * u1[5] / d2)) * dt[ri]);
     }
     if (use_store) {
       for (int l = 0; l < nv; ++l) Rs[(size_t)l * s->nR + ri] = Rs[(size_t)l * s->nR + ri] + SF * R[l];
@@ -1664,10 +1787,11 @@ static int update_with(orc *s, double TF, double SF, int TU, int use_store) {
     for (int l = 0; l < nv; ++l) if (isnan(u2[l])) bad = 1;
     if (bad) { status = ORC_E_NONPHYSICAL; continue; }
     for (int l = 0; l < 5; ++l) QP(l, i, j, k) = u2[l];
-    if (c->turb != ORC_TURB_NONE) {
+    if (SST(s)) {
       if (u2[5] >= 0.) QP(5, i, j, k) = u2[5];
       if (u2[6] >= 0.) QP(6, i, j, k) = u2[6];
     }
+    if (SA(s)) QP(5, i, j, k) = fmx(u2[5], 1.e-12);
   }
   return status;
 }
@@ -1693,6 +1817,7 @@ static void lu_flux(const orc *s, int nv, const double *ql, const double *qr, co
   U[3] = ql[0] * ql[3];
   U[4] = (ql[4] / (gm - 1.0)) + (0.5 * ql[0] * (ql[1] * ql[1] + ql[2] * ql[2] + ql[3] * ql[3]));
   if (nv == 7) { U[5] = ql[0] * ql[5]; U[6] = ql[0] * ql[6]; }
+  if (nv == 6) U[5] = ql[0] * ql[5];
   for (int l = 0; l < nv; ++l) U[l] = U[l] + du[l];
   W[0] = U[0];
   W[1] = U[1] / U[0];
@@ -1706,6 +1831,7 @@ static void lu_flux(const orc *s, int nv, const double *ql, const double *qr, co
     W[5] = W[5] + 0.5 * (1. - copysign(1., W[5])) * (ql[5] - W[5]);
     W[6] = W[6] + 0.5 * (1. - copysign(1., W[6])) * (ql[6] - W[6]);
   }
+  if (nv == 6) W[5] = fmx(U[5] / U[0], 1e-8); /* SAFlux, src/lusgs.f90:2145-2146 */
   double nx = f->nx, ny = f->ny, nz = f->nz, Area = f->A, Volume = f->vol, mmu = f->mm, tmu = f->tm;
   double Vn = (W[1] * nx) + (W[2] * ny) + (W[3] * nz);
   double uface = 0.5 * (W[1] + P[1]), vface = 0.5 * (W[2] + P[2]), wface = 0.5 * (W[3] + P[3]);
@@ -1718,6 +1844,7 @@ static void lu_flux(const orc *s, int nv, const double *ql, const double *qr, co
   double RhoHt = ((gm / (gm - 1.0)) * W[4]) + HalfRhoUsquare;
   Fx[4] = RhoHt * Vn;
   if (nv == 7) { Fx[5] = W[5] * Fx[0]; Fx[6] = W[6] * Fx[0]; }
+  if (nv == 6) Fx[5] = (W[5] * Fx[0]);
   double mu = mmu + tmu;
   double T1 = W[4] / (W[0] * R_gas), T2 = P[4] / (P[0] * R_gas);
   double dTdx = (T2 - T1) * nx * Area / Volume, dTdy = (T2 - T1) * ny * Area / Volume,
@@ -1750,6 +1877,14 @@ static void lu_flux(const orc *s, int nv, const double *ql, const double *qr, co
     Fx[5] = Fx[5] + (mmu + sk * tmu) * (dtkdx * nx + dtkdy * ny + dtkdz * nz);
     Fx[6] = Fx[6] + (mmu + sw * tmu) * (dtwdx * nx + dtwdy * ny + dtwdz * nz);
   }
+  if (nv == 6) {
+    /* nu-tilde diffusion with the face's 0.25 (rho_L + rho_R)(nu_L + nu_R),
+     * added (src/lusgs.f90:2219, 2254) */
+    double muCap = 0.25 * (P[0] + W[0]) * (P[5] + W[5]);
+    double dtvdx = (P[5] - W[5]) * nx * Area / Volume, dtvdy = (P[5] - W[5]) * ny * Area / Volume,
+           dtvdz = (P[5] - W[5]) * nz * Area / Volume;
+    Fx[5] = Fx[5] + (mmu + muCap) * (dtvdx * nx + dtvdy * ny + dtvdz * nz) / sigma_sa;
+  }
   for (int l = 0; l < nv; ++l) out[l] = Fx[l] * Area;
 }
 
@@ -1777,7 +1912,7 @@ static inline size_t DQI(const orc *s, int i, int j, int k) {
 /* The six faces, neighbour states and centres of cell (i,j,k), in the
  * reference's numbering 1..6 = i-, j-, k-, i+, j+, k+ (src/lusgs.f90:246-312). */
 static void lu_cell(const orc *s, int i, int j, int k, double Q[7][7], double C[7][3], lu_face F[6]) {
-  int nv = s->nv, sst = s->c.turb != ORC_TURB_NONE;
+  int nv = s->nv, turb = s->c.turb != ORC_TURB_NONE, sst = SST(s);
   const int di[7] = {0, -1, 0, 0, 1, 0, 0}, dj[7] = {0, 0, -1, 0, 0, 1, 0}, dk[7] = {0, 0, 0, -1, 0, 0, 1};
   const double *vol = s->a[ORC_VOL], *mu = s->a[ORC_MU], *mut = s->a[ORC_MUT], *F1 = s->a[ORC_F1];
   int lam_mu = s->c.mu_ref != 0.0;
@@ -1805,9 +1940,45 @@ static void lu_cell(const orc *s, int i, int j, int k, double Q[7][7], double C[
     /* mmu points at zeros for inviscid flow and tmu for laminar flow
      * (src/lusgs.f90:115-130) */
     f->mm = lam_mu ? 0.5 * (mu[cn] + mu[c0]) : 0.0;
-    f->tm = sst ? 0.5 * (mut[cn] + mut[c0]) : 0.0;
+    f->tm = turb ? 0.5 * (mut[cn] + mut[c0]) : 0.0;
     f->F1 = sst ? 0.5 * (F1[cn] + F1[c0]) : 0.0;
   }
+}
+
+/* The SA diagonal, src/lusgs.f90:1882-1914 (and 2034-2066 backward): the
+ * convective/viscous D plus the linearised SA production and destruction,
+ * one value for all six equations. */
+static double sa_diagonal(const orc *s, int i, int j, int k, double D) {
+  const double *mu = s->a[ORC_MU], *dist = s->a[ORC_DIST], *vol = s->a[ORC_VOL];
+  size_t c0 = CI(s, i, j, k);
+  double density = QP(0, i, j, k), q6 = QP(5, i, j, k);
+  double Omega = sqrt(sq(GY(2, i, j, k) - GZ(1, i, j, k)) + sq(GZ(0, i, j, k) - GX(2, i, j, k)) +
+                      sq(GX(1, i, j, k) - GY(0, i, j, k)));
+  double dist_i = dist[c0], dist_i_2 = dist_i * dist_i, k2 = kappa_sa * kappa_sa;
+  double nu = mu[c0] / density;
+  double Ji = q6 / nu, Ji_2 = Ji * Ji, Ji_3 = Ji_2 * Ji;
+  double fv1 = (Ji_3) / ((Ji_3) + (CV1_3));
+  double fv2 = 1.0 - Ji / (1.0 + (Ji * fv1));
+  double inv_k2_d2 = 1.0 / (k2 * dist_i_2);
+  double Shat = Omega + q6 * fv2 * inv_k2_d2;
+  Shat = fmx(Shat, 1.0e-10);
+  double inv_Shat = 1.0 / Shat;
+  double jc = Ji_3 + CV1_3;
+  double dfv1 = 3.0 * Ji_2 * CV1_3 / (nu * (jc * jc));
+  double jf = 1.0 + Ji * fv1;
+  double dfv2 = -((1.0 / nu) - Ji_2 * dfv1) / (jf * jf);
+  double dShat = (fv2 + q6 * dfv2) * inv_k2_d2;
+  D = D - cb1 * (q6 * dShat + Shat) * vol[c0];
+  double r = fmn(q6 * inv_Shat * inv_k2_d2, 10.0);
+  double g = r + cw2 * ((ipow(r, 6)) - r);
+  double g_6 = ipow(g, 6);
+  double glim = pow((1.0 + CW3_6) / (g_6 + CW3_6), 1.0 / 6.0);
+  double fw = g * glim;
+  double dr = (Shat - q6 * dShat) * inv_Shat * inv_Shat * inv_k2_d2;
+  double dg = dr * (1.0 + cw2 * (6.0 * (ipow(r, 5)) - 1.0));
+  double dfw = dg * glim * (1.0 - g_6 / (g_6 + CW3_6));
+  D = D + CW1_SA * (dfw * q6 + 2 * fw) * q6 / dist_i_2 * vol[c0];
+  return D;
 }
 
 /* update_laminar_variables / update_SST_variables, src/lusgs.f90:186-488 and
@@ -1815,7 +1986,7 @@ static void lu_cell(const orc *s, int i, int j, int k, double Q[7][7], double C[
  * conservative update of the interior cells. */
 static int lusgs(orc *s) {
   const orc_case *c = &s->c;
-  int nv = s->nv, sst = c->turb != ORC_TURB_NONE;
+  int nv = s->nv, sst = SST(s), sa = SA(s);
   double gm = c->gm;
   size_t ndq = (size_t)(IMX + 1) * (JMX + 1) * (KMX + 1);
   double *dqs = calloc(ndq * (size_t)nv, sizeof(double));
@@ -1855,6 +2026,10 @@ static int lusgs(orc *s) {
             D[5] = (D[5] + (bstar * QP(6, i, j, k)) * vol[c0]);
             D[6] = (D[6] + 2.0 * beta * QP(6, i, j, k) * vol[c0]);
           }
+          if (sa) {
+            double d0 = sa_diagonal(s, i, j, k, D[0]);
+            for (int l = 0; l < nv; ++l) D[l] = d0;
+          }
           for (int l = 0; l < nv; ++l) {
             double dI = New[0][l] - Old[0][l], dJ = New[1][l] - Old[1][l], dK = New[2][l] - Old[2][l];
             if (!pass) {
@@ -1881,6 +2056,7 @@ static int lusgs(orc *s) {
            (0.5 * QP(0, i, j, k) * (QP(1, i, j, k) * QP(1, i, j, k) + QP(2, i, j, k) * QP(2, i, j, k) +
                                     QP(3, i, j, k) * QP(3, i, j, k)));
     if (sst) { U[5] = QP(0, i, j, k) * QP(5, i, j, k); U[6] = QP(0, i, j, k) * QP(6, i, j, k); }
+    if (sa) U[5] = QP(0, i, j, k) * QP(5, i, j, k);
     for (int l = 0; l < nv; ++l) U[l] = U[l] + DQ(l, i, j, k);
     QP(0, i, j, k) = U[0];
     QP(1, i, j, k) = U[1] / U[0];
@@ -1891,6 +2067,7 @@ static int lusgs(orc *s) {
       if (U[5] > 0) QP(5, i, j, k) = U[5] / U[0];
       if (U[6] > 0) QP(6, i, j, k) = U[6] / U[0];
     }
+    if (sa) QP(5, i, j, k) = fmx(U[5] / U[0], 1.e-8);
   }
 #undef DQS
 #undef DQ

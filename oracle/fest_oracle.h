@@ -28,7 +28,7 @@ enum { ORC_FLUX_AUSM = 0, ORC_FLUX_AUSMP = 1, ORC_FLUX_AUSMUP = 2,
        ORC_FLUX_SLAU = 3, ORC_FLUX_LDFSS0 = 4, ORC_FLUX_VANLEER = 5 };
 enum { ORC_RECON_NONE = 0, ORC_RECON_MUSCL = 1, ORC_RECON_PPM = 2,
        ORC_RECON_WENO = 3 };
-enum { ORC_TURB_NONE = 0, ORC_TURB_SST = 1, ORC_TURB_SST2003 = 2 };
+enum { ORC_TURB_NONE = 0, ORC_TURB_SST = 1, ORC_TURB_SST2003 = 2, ORC_TURB_SA = 3 };
 enum { ORC_TA_NONE = 0, ORC_TA_RK2 = 2, ORC_TA_RK4 = 4, ORC_TA_TVDRK2 = 12, ORC_TA_TVDRK3 = 13,
        ORC_TA_IMPLICIT = 20 };
 
@@ -52,6 +52,8 @@ typedef struct {
   double fix_rho[6], fix_u[6], fix_v[6], fix_w[6], fix_p[6];
   double fix_tk[6], fix_tw[6], fix_twall[6];
   double fix_tp[6]; /* total pressure of a -11 face */
+  double tv_inf;     /* Spalart-Allmaras free-stream nu-tilde */
+  double fix_tv[6];  /* fixed nu-tilde of each face */
 } orc_case;
 
 typedef struct orc orc;

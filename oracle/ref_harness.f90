@@ -43,6 +43,7 @@ module ref_harness
     real(c_double) :: fix_rho(6), fix_u(6), fix_v(6), fix_w(6), fix_p(6)
     real(c_double) :: fix_tk(6), fix_tw(6), fix_twall(6)
     real(c_double) :: fix_tp(6)
+    real(c_double) :: tv_inf, fix_tv(6)
   end type hcase
 
   type(extent) :: dims
@@ -79,6 +80,8 @@ contains
       schm%turbulence = 'sst'
     else if (hc%turb == 2) then
       schm%turbulence = 'sst2003'
+    else if (hc%turb == 3) then
+      schm%turbulence = 'sa'
     else
       schm%turbulence = 'none'
     end if
@@ -97,6 +100,7 @@ contains
     flow%density_inf = hc%rho_inf; flow%x_speed_inf = hc%u_inf
     flow%y_speed_inf = hc%v_inf; flow%z_speed_inf = hc%w_inf
     flow%pressure_inf = hc%p_inf; flow%tk_inf = hc%tk_inf; flow%tw_inf = hc%tw_inf
+    flow%tv_inf = hc%tv_inf
     flow%vel_mag = hc%vel_mag; flow%MInf = hc%MInf
     flow%gm = hc%gm; flow%R_gas = hc%R_gas; flow%mu_ref = hc%mu_ref
     flow%T_ref = hc%T_ref; flow%Sutherland_temp = hc%S_ref; flow%Pr = hc%Pr; flow%tPr = hc%tPr
@@ -117,6 +121,7 @@ contains
     bc%fixed_z_speed = hc%fix_w; bc%fixed_pressure = hc%fix_p
     bc%fixed_tk = hc%fix_tk; bc%fixed_tw = hc%fix_tw; bc%fixed_wall_temperature = hc%fix_twall
     bc%fixed_Tpressure = hc%fix_tp
+    bc%fixed_tv = hc%fix_tv
     if (allocated(bc%make_F_flux_zero)) deallocate(bc%make_F_flux_zero, bc%make_G_flux_zero, bc%make_H_flux_zero)
     allocate(bc%make_F_flux_zero(1:imx), bc%make_G_flux_zero(1:jmx), bc%make_H_flux_zero(1:kmx))
     bc%make_F_flux_zero = 1; bc%make_G_flux_zero = 1; bc%make_H_flux_zero = 1
