@@ -54,14 +54,14 @@ struct Blk {
   // flow
   double gm, R, mu_ref, T_ref, S_ref, Pr, tPr, MInf, vel_mag;
   double iPr, itPr, cpR;  // 1/Pr, 1/Pr_t, gamma R/(gamma-1): the heat-flux factors
-  double rho_inf, u_inf, v_inf, w_inf, p_inf, tk_inf, tw_inf;
+  double rho_inf, u_inf, v_inf, w_inf, p_inf, tk_inf, tw_inf, tv_inf;
   double c1, c2, c3;
-  double fix[9][6];           // rho,u,v,w,p,tk,tw,wall temperature,total pressure
+  double fix[10][6];          // rho,u,v,w,p,tk,tw,wall temperature,total pressure,nu-tilde
   double global_dt;
   double CFLv;                // CFL of the current call
 };
 
-enum { FX_RHO = 0, FX_U, FX_V, FX_W, FX_P, FX_TK, FX_TW, FX_TWALL, FX_TP, FX_N };
+enum { FX_RHO = 0, FX_U, FX_V, FX_W, FX_P, FX_TK, FX_TW, FX_TWALL, FX_TP, FX_TV, FX_N };
 enum : unsigned { ERR_NONPHYS = 1u, ERR_NAN_FLUX = 2u, ERR_NAN_GRAD = 4u, ERR_NAN_MU = 8u, ERR_CONFIG = 16u };
 
 __host__ __device__ inline long ci(const Blk &b, int i, int j, int k) {
@@ -132,5 +132,14 @@ __device__ __forceinline__ double fmx(double a, double b) { return a > b ? a : b
 __device__ __forceinline__ double fmn(double a, double b) { return a < b ? a : b; }
 __device__ __forceinline__ double fsgn(double a, double b) { return copysign(fabs(a), b); }
 __device__ __forceinline__ double sqd(double x) { return x * x; }
+
+// Spalart-Allmaras constants (src/global/global_sa.f90); cw1 and cv1^3 as
+// the reference's parameter expressions evaluate in double
+struct Sa {
+  static constexpr double cb1 = 0.1355, cb2 = 0.6220, cw2 = 0.3, cv1 = 7.1, sigma = 2. / 3., kappa = 0.41;
+  static constexpr double cw1 = (cb1 / (kappa * kappa)) + ((1 + cb2) / sigma);
+  static constexpr double cv1_3 = (cv1 * cv1) * cv1;
+  static constexpr double cw3_6 = 64.0;
+};
 
 }  // namespace fest

@@ -198,16 +198,21 @@ void launch_flux_nv(const Blk &b, dim3 g, dim3 t, hipStream_t st, int store_csnd
   }
 }
 // the k faces of a 2-D grid (one cell in k) carry no convective flux
+template <int D, int NV>
+void launch_flux_v(const Blk &b, dim3 g, dim3 t, hipStream_t st, int store_csnd) {
+  if constexpr (D == 2) {
+    if (b.kmx == 2) {
+      launch_flux_nv<D, NV, true>(b, g, t, st, store_csnd);
+      return;
+    }
+  }
+  launch_flux_nv<D, NV, false>(b, g, t, st, store_csnd);
+}
 template <int D>
 void launch_flux(const Blk &b, dim3 g, dim3 t, hipStream_t st, int store_csnd) {
-  const bool flat = D == 2 && b.kmx == 2;
-  if (b.nv == 7) {
-    if (flat) launch_flux_nv<D, 7, true>(b, g, t, st, store_csnd);
-    else launch_flux_nv<D, 7, false>(b, g, t, st, store_csnd);
-  } else {
-    if (flat) launch_flux_nv<D, 5, true>(b, g, t, st, store_csnd);
-    else launch_flux_nv<D, 5, false>(b, g, t, st, store_csnd);
-  }
+  if (b.nv == 7) launch_flux_v<D, 7>(b, g, t, st, store_csnd);
+  else if (b.nv == 6) launch_flux_v<D, 6>(b, g, t, st, store_csnd);
+  else launch_flux_v<D, 5>(b, g, t, st, store_csnd);
 }
 
 // grid of 64 x 4 tiles over an (ni x nj x nk) box (the kernels use tile_ij)
@@ -254,6 +259,7 @@ int launch_residual_front(fest_handle *h, int current_iter, int store_csnd) {
       KgScope ts(h, KG_GRAD);
       if (b.turb == FEST_TURB_SST2003) hipLaunchKernelGGL((k_grad<6, true>), g, t, 0, st, b);
       else if (b.ng == 6) hipLaunchKernelGGL((k_grad<6, false>), g, t, 0, st, b);
+      else if (b.ng == 5) hipLaunchKernelGGL((k_grad<5, false>), g, t, 0, st, b);
       else hipLaunchKernelGGL((k_grad<4, false>), g, t, 0, st, b);
     }
     int mx = std::max(std::max(b.imx, b.jmx), b.kmx);
@@ -261,6 +267,9 @@ int launch_residual_front(fest_handle *h, int current_iter, int store_csnd) {
     {
       KgScope ts(h, KG_GRAD_BC);
       hipLaunchKernelGGL(k_grad_bc, gb, dim3(64, 4, 1), 0, st, b);
+      bool tp = false;
+      for (int f = 0; f < 6; ++f) tp |= b.bc[f] == -11;
+      if (tp && b.ng == 6) hipLaunchKernelGGL(k_mut_ghost_tp, gb, dim3(64, 4, 1), 0, st, b);
     }
     LAUNCHCHK(h);
   }
@@ -347,15 +356,20 @@ int launch_lusgs_nv(fest_handle *h) {
       h->sk_geometry = true;
     }
     const long nc = b.nc, nr = b.nres;
+    // the NV state planes and residue planes (at most 7, one transpose each)
     if (NV == 7)
       to_skew(h, {b.qp, b.qp + nc, b.qp + 2 * nc, b.qp + 3 * nc, b.qp + 4 * nc, b.qp + 5 * nc, b.qp + 6 * nc},
               C::Q, 0, 0, 1);
+    else if (NV == 6)
+      to_skew(h, {b.qp, b.qp + nc, b.qp + 2 * nc, b.qp + 3 * nc, b.qp + 4 * nc, b.qp + 5 * nc}, C::Q, 0, 0, 1);
     else
       to_skew(h, {b.qp, b.qp + nc, b.qp + 2 * nc, b.qp + 3 * nc, b.qp + 4 * nc}, C::Q, 0, 0, 1);
     to_skew(h, {b.mu, b.mut, b.f1}, C::MU, 0, 0, 1);
     if (NV == 7)
       to_skew(h, {b.res, b.res + nr, b.res + 2 * nr, b.res + 3 * nr, b.res + 4 * nr, b.res + 5 * nr, b.res + 6 * nr},
               C::RES, 1, 1, 0);
+    else if (NV == 6)
+      to_skew(h, {b.res, b.res + nr, b.res + 2 * nr, b.res + 3 * nr, b.res + 4 * nr, b.res + 5 * nr}, C::RES, 1, 1, 0);
     else
       to_skew(h, {b.res, b.res + nr, b.res + 2 * nr, b.res + 3 * nr, b.res + 4 * nr}, C::RES, 1, 1, 0);
     // radii, Old fluxes and diagonal over the cells 1..n+1
@@ -379,7 +393,9 @@ int launch_lusgs_nv(fest_handle *h) {
   LAUNCHCHK(h);
   return 0;
 }
-int launch_lusgs(fest_handle *h) { return h->b.nv == 7 ? launch_lusgs_nv<7>(h) : launch_lusgs_nv<5>(h); }
+int launch_lusgs(fest_handle *h) {
+  return h->b.nv == 7 ? launch_lusgs_nv<7>(h) : h->b.nv == 6 ? launch_lusgs_nv<6>(h) : launch_lusgs_nv<5>(h);
+}
 
 // One time step of get_next_solution (src/update.f90:129-226) as a sequence
 // of operations: the step start (Temp refresh, U_store), residual
@@ -522,9 +538,10 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
   // a 2-D grid has one cell in k (kmx == 2), src/face/flux/convective/*.f90
   if (dims->imx < 4 || dims->jmx < 4 || (dims->kmx < 4 && dims->kmx != 2))
     return bad("i and j need at least 3 cells, k at least 3 or exactly 1");
-  if (scheme->turb != FEST_TURB_NONE && scheme->turb != FEST_TURB_SST && scheme->turb != FEST_TURB_SST2003)
+  if (scheme->turb != FEST_TURB_NONE && scheme->turb != FEST_TURB_SST && scheme->turb != FEST_TURB_SST2003 &&
+      scheme->turb != FEST_TURB_SA)
     return bad("turbulence model not supported");
-  int nv_expect = scheme->turb != FEST_TURB_NONE ? 7 : 5;
+  int nv_expect = scheme->turb == FEST_TURB_SA ? 6 : scheme->turb != FEST_TURB_NONE ? 7 : 5;
   if (dims->n_var != nv_expect) return bad("n_var does not match the turbulence model");
   if (scheme->flux < 0 || scheme->flux > 5) return bad("unknown flux scheme");
   if (scheme->recon < 0 || scheme->recon > 3) return bad("unknown reconstruction");
@@ -552,7 +569,7 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
   Blk &b = h->b;
   memset(&b, 0, sizeof b);
   b.imx = dims->imx; b.jmx = dims->jmx; b.kmx = dims->kmx; b.nv = dims->n_var;
-  b.ng = scheme->turb ? 6 : 4;
+  b.ng = scheme->turb == FEST_TURB_SA ? 5 : scheme->turb ? 6 : 4;
   b.pi = b.imx + 5; b.pj = b.jmx + 5; b.pk = b.kmx + 5;
   b.nc = (long)b.pi * b.pj * b.pk;
   b.fpi[0] = b.imx + 6; b.fpj[0] = b.jmx + 5; b.fpk[0] = b.kmx + 5;
@@ -592,6 +609,7 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
   b.vel_mag = flow->vel_mag;
   b.rho_inf = flow->density_inf; b.u_inf = flow->x_speed_inf; b.v_inf = flow->y_speed_inf;
   b.w_inf = flow->z_speed_inf; b.p_inf = flow->pressure_inf; b.tk_inf = flow->tk_inf; b.tw_inf = flow->tw_inf;
+  b.tv_inf = flow->tv_inf;
   b.c2 = 1 + scheme->accur;
   b.c3 = 0.5 * scheme->accur;
   b.c1 = b.c2 - b.c3;
@@ -601,6 +619,7 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
     b.fix[FX_P][f] = bc->fixed_pressure[f]; b.fix[FX_TK][f] = bc->fixed_tk[f];
     b.fix[FX_TW][f] = bc->fixed_tw[f]; b.fix[FX_TWALL][f] = bc->fixed_wall_temperature[f];
     b.fix[FX_TP][f] = bc->fixed_Tpressure[f];
+    b.fix[FX_TV][f] = bc->fixed_tv[f];
   }
   b.global_dt = scheme->global_time_step;
   b.CFLv = 1.0;

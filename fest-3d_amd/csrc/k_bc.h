@@ -72,17 +72,17 @@ __device__ void fixcol(const Blk &b, double *v, double val, int face, int a, int
   for (int g = 0; g < 3; ++g) v[fcell(b, face, -g, a, bb)] = val;
 }
 
+// the turbulence variables: k and omega (SST), nu-tilde (SA)
 __device__ void turb_copycol(const Blk &b, int type, int face, int a, int bb) {
-  if (b.turb) {
-    copy3col(b, b.qp + 5 * b.nc, type, face, a, bb);
-    copy3col(b, b.qp + 6 * b.nc, type, face, a, bb);
-  }
+  if (b.nv >= 6) copy3col(b, b.qp + 5 * b.nc, type, face, a, bb);
+  if (b.nv == 7) copy3col(b, b.qp + 6 * b.nc, type, face, a, bb);
 }
 __device__ void turb_fixcol(const Blk &b, const FaceBC &p, int face, int a, int bb) {
-  if (b.turb) {
+  if (b.nv == 7) {
     fixcol(b, b.qp + 5 * b.nc, p.fix[FX_TK], face, a, bb);
     fixcol(b, b.qp + 6 * b.nc, p.fix[FX_TW], face, a, bb);
   }
+  if (b.nv == 6) fixcol(b, b.qp + 5 * b.nc, p.fix[FX_TV], face, a, bb);
 }
 
 // unit normal of the boundary face at (a, bb)
@@ -263,10 +263,8 @@ __global__ void k_bc_face(Blk b, int face0, int current_iter) {
         copy3col(b, q + 1 * nc, CP_ANTI, face, a, bb);
         copy3col(b, q + 2 * nc, CP_ANTI, face, a, bb);
         copy3col(b, q + 3 * nc, CP_ANTI, face, a, bb);
-        if (b.turb) {
-          copy3col(b, q + 5 * nc, CP_ANTI, face, a, bb);
-          omega_wall_col(b, face, a, bb);
-        }
+        if (b.nv >= 6) copy3col(b, q + 5 * nc, CP_ANTI, face, a, bb);  // k or nu-tilde
+        if (b.nv == 7) omega_wall_col(b, face, a, bb);
       }
       break;
     case -6:  // slip wall

@@ -296,9 +296,10 @@ __device__ __forceinline__ void inviscid_flux(const Blk &b, const double (&qL)[N
 // what the reference subtracts from the inviscid flux of a face between
 // cells cm and c (gradient indices gL, gR) with unit normal n and area A;
 // the laminar and SST parts of one component are summed before the
-// subtraction.  laminar_only keeps the laminar part alone: the k faces of a
-// 2-D grid, where the reference skips the SST viscous flux
-// (src/viscous.f90:101-105)
+// subtraction.  laminar_only keeps the laminar part (with mu_t) alone: the
+// k faces of a 2-D grid, where the reference skips the SST viscous flux
+// (src/viscous.f90:101-105).  NV 6 is Spalart-Allmaras: its nu-tilde flux
+// runs on every face (src/viscous.f90:94-97, 570-656)
 template <int NV>
 __device__ __forceinline__ void visc_face(const Blk &b, long c, long cm, long gL, long gR, double nx, double ny,
                                           double nz, double A, bool laminar_only, double (&v)[NV]) {
@@ -328,7 +329,9 @@ __device__ __forceinline__ void visc_face(const Blk &b, long c, long cm, long gL
     gr[l][2] = gr[l][2] + ncmp * delz;
   }
   double mu_f = 0.5 * (b.mu[cm] + b.mu[c]);
-  double mut_f = (NV == 7 && !laminar_only) ? 0.5 * (b.mut[cm] + b.mut[c]) : 0.0;
+  // compute_viscous_fluxes_laminar takes mu_t whenever the run is turbulent
+  // (src/viscous.f90:270-277)
+  double mut_f = (NV >= 6) ? 0.5 * (b.mut[cm] + b.mut[c]) : 0.0;
   double tm = mu_f + mut_f;
   double dudx = gr[0][0], dudy = gr[0][1], dudz = gr[0][2];
   double dvdx = gr[1][0], dvdy = gr[1][1], dvdz = gr[1][2];
@@ -354,6 +357,20 @@ __device__ __forceinline__ void visc_face(const Blk &b, long c, long cm, long gL
   if constexpr (NV == 7) {
     v[5] = 0.0;
     v[6] = 0.0;
+  }
+  if constexpr (NV == 6) {
+    // nu-tilde diffusion with (mu + rho_f nu~_f) / sigma, sigma = 2/3; the
+    // quotient taken as a product with 1.5 (within an ulp of it)
+    double g0 = 0.5 * (gx[4 * ngr + gL] + gx[4 * ngr + gR]);
+    double g1 = 0.5 * (gy[4 * ngr + gL] + gy[4 * ngr + gR]);
+    double g2 = 0.5 * (gz[4 * ngr + gL] + gz[4 * ngr + gR]);
+    double ncmp = ((qp[5 * nc + c] - qp[5 * nc + cm]) - (g0 * delx + g1 * dely + g2 * delz)) * id2;
+    g0 = g0 + ncmp * delx;
+    g1 = g1 + ncmp * dely;
+    g2 = g2 + ncmp * delz;
+    double rhof = 0.5 * (qp[cm] + qp[c]);
+    double mtv = 0.5 * (qp[5 * nc + cm] + qp[5 * nc + c]) * rhof;
+    v[NV == 6 ? 5 : 0] = (A * ((mu_f + mtv) * (g0 * nx + g1 * ny + g2 * nz))) * 1.5;
   }
   if (NV == 7 && !laminar_only) {
     // SST k / omega diffusion and the -2/3 rho k stress, src/viscous.f90:328-447
@@ -390,7 +407,7 @@ __device__ __forceinline__ void visc_face(const Blk &b, long c, long cm, long gL
 
 // ---- the face kernel// ---- the face kernel
 // D: sweep direction.  Faces: n = 1..N along D, 1..N-1 across.  NV: 5
-// (laminar) or 7 (SST) variables; RECON: the reconstruction.  Compile-time
+// (laminar), 6 (Spalart-Allmaras) or 7 (SST) variables; RECON: the reconstruction.  Compile-time
 // counts keep the per-variable arrays in registers.
 // Registers are fitted to 4 waves per SIMD: 5 spills the SST/WENO variants
 // and measured slower even where it does not spill.

@@ -1,6 +1,7 @@
 // k_grad.h -- Green-Gauss cell gradients (src/gradients.f90:276-482) and the
-// ghost-cell gradient rule (src/gradients.f90:486-676), molecular and SST
-// eddy viscosity with the F1 blending function (src/viscosity.f90:52-546).
+// ghost-cell gradient rule (src/gradients.f90:486-676), molecular, SST (with
+// the F1 blending function) and Spalart-Allmaras eddy viscosity
+// (src/viscosity.f90:52-546), and the SST and SA sources.
 //
 // One thread per cell computes all 3*n_grad gradients (the reference makes
 // 3*n_grad separate passes): the 6 face metrics and the volume are read once
@@ -15,7 +16,7 @@
 
 namespace fest {
 
-// gradient slots: u, v, w, T, k, omega
+// gradient slots: u, v, w, T, then k, omega (SST) or nu-tilde (SA)
 __device__ __forceinline__ const double *grad_var(const Blk &b, int l) {
   return l < 3 ? b.qp + (long)(l + 1) * b.nc : l == 3 ? b.temp : b.qp + (long)(l + 1) * b.nc;
 }
@@ -51,6 +52,40 @@ __device__ __forceinline__ void sst_mut_f1(double rho, double tk, double tw, dou
   double left = fmx(var1, var2);
   double arg1 = fmn(left, right);
   f1 = tanh(arg1 * arg1 * arg1 * arg1);
+}
+
+// SA eddy viscosity rho nu~ fv1 (src/viscosity.f90:149-163)
+__device__ __forceinline__ double sa_mut(double rho, double tv, double mu) {
+  double xi = tv * rho / mu;
+  double x3 = (xi * xi) * xi;
+  return rho * tv * (x3 / (x3 + Sa::cv1_3));
+}
+
+// SA source of an interior cell, times its volume (src/source.f90:835-983).
+// grho is the cell's own Green-Gauss density gradient, whose k-face normal
+// is (nx, nx, nx) as in the reference (source.f90:887).
+__device__ __forceinline__ double sa_source(double rho, double tv, double mu, double dist, double vol,
+                                            const double (&g)[5][3], const double (&grho)[3]) {
+  double vort = sqrt(sqd(g[2][1] - g[1][2]) + sqd(g[0][2] - g[2][0]) + sqd(g[1][0] - g[0][1]));
+  double CD1 = Sa::cb2 * ((g[4][0] * g[4][0]) + (g[4][1] * g[4][1]) + (g[4][2] * g[4][2]));
+  double CD2 = (grho[0] * g[4][0]) + (grho[1] * g[4][1]) + (grho[2] * g[4][2]);
+  double kd = Sa::kappa * dist, kd2 = kd * kd;
+  double nu = mu / rho;
+  double xi = tv / nu;
+  double x3 = (xi * xi) * xi;
+  double fv1 = x3 / (x3 + Sa::cv1_3);
+  double fv2 = 1.0 - xi / (1.0 + (xi * fv1));
+  double scap = fmx(vort + (tv * fv2 / kd2), 0.3 * vort);
+  double r = fmn(tv / (scap * kd2), 10.0);
+  double r6 = ((((r * r) * r) * r) * r) * r;
+  double gg = r + Sa::cw2 * (r6 - r);
+  double g6 = ((((gg * gg) * gg) * gg) * gg) * gg;
+  double fw = gg * pow((1.0 + Sa::cw3_6) / (g6 + Sa::cw3_6), 1.0 / 6.0);
+  double td = tv / dist;
+  double D_v = rho * Sa::cw1 * fw * (td * td);
+  double P_v = rho * Sa::cb1 * scap * tv;
+  double lamda = rho * CD1 / Sa::sigma - CD2 * (nu + tv) / Sa::sigma;
+  return (P_v - D_v + lamda) * vol;
 }
 
 // SST source of an interior cell, times its volume (src/source.f90:158-270);
@@ -163,6 +198,30 @@ __global__ void __launch_bounds__(256) k_grad(Blk b) {
     b.src[b.nres + r] = S_w;
   }
   }
+  if constexpr (NG == 5) {
+  double tv = b.qp[5 * b.nc + c];
+  b.mut[c] = sa_mut(rho, tv, mu);
+  if (i >= 1 && i <= b.imx - 1 && j >= 1 && j <= b.jmx - 1 && k >= 1 && k <= b.kmx - 1) {
+    // the density gradient over the face sums rho_neighbour + rho
+    const double *q0 = b.qp;
+    double rf[6] = {q0[c - si] + rho, q0[c - sj] + rho, q0[c - sk] + rho,
+                    q0[c + si] + rho, q0[c + sj] + rho, q0[c + sk] + rho};
+    double grho[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      // the k-face normal of the lower k face is (nx, nx, nx)
+      double nk0 = n[0][2];
+      double g = -(rf[0] * n[d][0] * Ai0);
+      g = g - rf[1] * n[d][1] * Aj0;
+      g = g - rf[2] * nk0 * Ak0;
+      g = g + rf[3] * n[d][3] * Ai1;
+      g = g + rf[4] * n[d][4] * Aj1;
+      g = g + rf[5] * n[d][5] * Ak1;
+      grho[d] = g / (2.0 * vol);
+    }
+    b.src[ri(b, i, j, k)] = sa_source(rho, tv, mu, b.dist[c], vol, gg, grho);
+  }
+  }
 }
 
 // mu_t / F1 ghost rules (src/viscosity.f90:407-465), face interiors
@@ -171,7 +230,7 @@ __device__ __forceinline__ void visc_ghost(const Blk &b, int face, int a, int bb
   if (!(id == -5 || (id <= -1 && id >= -9))) return;
   long cg = fcell(b, face, 0, a, bb), cn = fcell(b, face, 1, a, bb);
   b.mut[cg] = (id == -5) ? -b.mut[cn] : b.mut[cn];
-  b.f1[cg] = b.f1[cn];
+  if (b.ng == 6) b.f1[cg] = b.f1[cn];
 }
 
 
@@ -206,7 +265,6 @@ __global__ void k_grad_bc(Blk b) {
   double sig = hi ? -1.0 : 1.0;
   long gI = gri(b, Ii, Ij, Ik), gG = gri(b, Gi, Gj, Gk);
   double *gx = b.g[0], *gy = b.g[1], *gz = b.g[2];
-  double gg[6][3];
   for (int l = 0; l < b.ng; ++l) {
     double dq = b.qp[(long)(l + 1) * b.nc + cI] - b.qp[(long)(l + 1) * b.nc + cG];
     if (l == 3) dq = b.temp[cI] - b.temp[cG];
@@ -220,19 +278,39 @@ __global__ void k_grad_bc(Blk b) {
     gx[o + gG] = ox + (ix - dot * nx);
     gy[o + gG] = oy + (iy - dot * ny);
     gz[o + gG] = oz + (iz - dot * nz);
-    gg[l][0] = gx[o + gG]; gg[l][1] = gy[o + gG]; gg[l][2] = gz[o + gG];
   }
-  // A -11 face has no mu_t / F1 ghost rule (src/viscosity.f90:284-339,
-  // 409-465 list -1..-9): its ghosts keep the values calculate_viscosity
-  // computes there, from the ghost gradients just written
-  if (id == -11 && b.ng == 6) {
-    double mut, F1;
-    double rho = b.qp[cG], tk = b.qp[5 * b.nc + cG], tw = b.qp[6 * b.nc + cG];
-    if (b.turb == FEST_TURB_SST2003) sst_mut_f1<true>(rho, tk, tw, b.dist[cG], b.mu[cG], gg, mut, F1);
-    else sst_mut_f1<false>(rho, tk, tw, b.dist[cG], b.mu[cG], gg, mut, F1);
-    b.mut[cG] = mut;
-    b.f1[cG] = F1;
+}
+
+// A -11 face has no mu_t / F1 ghost rule (src/viscosity.f90:284-339,
+// 409-465 list -1..-9): its ghost cells keep the SST values
+// calculate_viscosity computes there, from the ghost gradients k_grad_bc has
+// written.  Launched after k_grad_bc, only when an SST block has a -11 face.
+__global__ void k_mut_ghost_tp(Blk b) {
+  int face = blockIdx.z;
+  int a = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int bb = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  int na = face_na(b, face), nb = face_nb(b, face);
+  if (a > na - 1 || bb > nb - 1) return;
+  if (b.bc[face] != -11 || b.ng != 6) return;
+  int d = face >> 1, hi = face & 1;
+  int N = d == 0 ? b.imx : d == 1 ? b.jmx : b.kmx;
+  int g = hi ? N : 0;  // the ghost layer next to the face
+  int Gi, Gj, Gk;
+  if (d == 0) { Gi = g; Gj = a; Gk = bb; } else if (d == 1) { Gi = a; Gj = g; Gk = bb; } else { Gi = a; Gj = bb; Gk = g; }
+  long cG = ci(b, Gi, Gj, Gk), gG = gri(b, Gi, Gj, Gk);
+  double gg[6][3];
+#pragma unroll
+  for (int l = 0; l < 6; ++l) {
+    gg[l][0] = b.g[0][(long)l * b.ngr + gG];
+    gg[l][1] = b.g[1][(long)l * b.ngr + gG];
+    gg[l][2] = b.g[2][(long)l * b.ngr + gG];
   }
+  double mut, F1;
+  double rho = b.qp[cG], tk = b.qp[5 * b.nc + cG], tw = b.qp[6 * b.nc + cG];
+  if (b.turb == FEST_TURB_SST2003) sst_mut_f1<true>(rho, tk, tw, b.dist[cG], b.mu[cG], gg, mut, F1);
+  else sst_mut_f1<false>(rho, tk, tw, b.dist[cG], b.mu[cG], gg, mut, F1);
+  b.mut[cG] = mut;
+  b.f1[cG] = F1;
 }
 
 }  // namespace fest

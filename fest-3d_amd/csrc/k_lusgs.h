@@ -26,7 +26,7 @@ struct LuFace {
   double A, nx, ny, nz, vol, mm, tm, F1;
 };
 
-// Flux / SSTFlux (src/lusgs.f90:490-609, 1027-1196): the flux through face
+// Flux / SAFlux / SSTFlux (src/lusgs.f90:490-609, 2104-2259, 1027-1196): the flux through face
 // f of the neighbour state ql moved by du (conservative), against the cell's
 // own primitive state qr
 template <int NV>
@@ -43,6 +43,7 @@ __device__ __forceinline__ void lu_flux(const Blk &b, const double (&ql)[NV], co
     U[5] = ql[0] * ql[5];
     U[6] = ql[0] * ql[6];
   }
+  if constexpr (NV == 6) U[5] = ql[0] * ql[5];
 #pragma unroll
   for (int l = 0; l < NV; ++l) U[l] = U[l] + du[l];
   W[0] = U[0];
@@ -57,6 +58,7 @@ __device__ __forceinline__ void lu_flux(const Blk &b, const double (&ql)[NV], co
     W[5] = W[5] + 0.5 * (1. - copysign(1., W[5])) * (ql[5] - W[5]);
     W[6] = W[6] + 0.5 * (1. - copysign(1., W[6])) * (ql[6] - W[6]);
   }
+  if constexpr (NV == 6) W[5] = fmx(U[5] / U[0], 1e-8);
   const double *P = qr;
   const double nx = f.nx, ny = f.ny, nz = f.nz, Area = f.A, Volume = f.vol, mmu = f.mm, tmu = f.tm;
   const double Vn = (W[1] * nx) + (W[2] * ny) + (W[3] * nz);
@@ -73,6 +75,7 @@ __device__ __forceinline__ void lu_flux(const Blk &b, const double (&ql)[NV], co
     Fx[5] = W[5] * Fx[0];
     Fx[6] = W[6] * Fx[0];
   }
+  if constexpr (NV == 6) Fx[5] = W[5] * Fx[0];
   const double mu = mmu + tmu;
   const double T1 = W[4] / (W[0] * R_gas), T2 = P[4] / (P[0] * R_gas);
   const double dTdx = (T2 - T1) * nx * Area / Volume, dTdy = (T2 - T1) * ny * Area / Volume,
@@ -104,6 +107,13 @@ __device__ __forceinline__ void lu_flux(const Blk &b, const double (&ql)[NV], co
     const double sk = 0.85 * F1 + 1.0 * (1.0 - F1), sw = 0.5 * F1 + 0.856 * (1.0 - F1);
     Fx[5] = Fx[5] + (mmu + sk * tmu) * (dtkdx * nx + dtkdy * ny + dtkdz * nz);
     Fx[6] = Fx[6] + (mmu + sw * tmu) * (dtwdx * nx + dtwdy * ny + dtwdz * nz);
+  }
+  if constexpr (NV == 6) {
+    // nu-tilde diffusion with 0.25 (rho_L + rho_R)(nu_L + nu_R), added
+    const double muCap = 0.25 * (P[0] + W[0]) * (P[5] + W[5]);
+    const double dtvdx = (P[5] - W[5]) * nx * Area / Volume, dtvdy = (P[5] - W[5]) * ny * Area / Volume,
+                 dtvdz = (P[5] - W[5]) * nz * Area / Volume;
+    Fx[5] = Fx[5] + (mmu + muCap) * (dtvdx * nx + dtvdy * ny + dtvdz * nz) / Sa::sigma;
   }
 #pragma unroll
   for (int l = 0; l < NV; ++l) out[l] = Fx[l] * Area;
@@ -151,6 +161,40 @@ template <int NV>
 __device__ __forceinline__ void lu_load(const Blk &b, long c, double (&q)[NV]) {
 #pragma unroll
   for (int l = 0; l < NV; ++l) q[l] = b.qp[(long)l * b.nc + c];
+}
+
+// The SA diagonal (src/lusgs.f90:1882-1914, the same backward): D plus the
+// linearised SA production and destruction, one value for all six
+// equations; gi the cell's gradient index
+__device__ __forceinline__ double sa_diagonal(const Blk &b, long c0, long gi, double rho, double q6, double vol,
+                                              double D) {
+  const double *gx = b.g[0], *gy = b.g[1], *gz = b.g[2];
+  const long ngr = b.ngr;
+  const double Omega = sqrt(sqd(gy[2 * ngr + gi] - gz[ngr + gi]) + sqd(gz[gi] - gx[2 * ngr + gi]) +
+                            sqd(gx[ngr + gi] - gy[gi]));
+  const double dist_i = b.dist[c0], dist_i_2 = dist_i * dist_i, k2 = Sa::kappa * Sa::kappa;
+  const double nu = b.mu[c0] / rho;
+  const double Ji = q6 / nu, Ji_2 = Ji * Ji, Ji_3 = Ji_2 * Ji;
+  const double fv1 = Ji_3 / (Ji_3 + Sa::cv1_3);
+  const double fv2 = 1.0 - Ji / (1.0 + (Ji * fv1));
+  const double inv_k2_d2 = 1.0 / (k2 * dist_i_2);
+  const double Shat = fmx(Omega + q6 * fv2 * inv_k2_d2, 1.0e-10);
+  const double inv_Shat = 1.0 / Shat;
+  const double jc = Ji_3 + Sa::cv1_3, jf = 1.0 + Ji * fv1;
+  const double dfv1 = 3.0 * Ji_2 * Sa::cv1_3 / (nu * (jc * jc));
+  const double dfv2 = -((1.0 / nu) - Ji_2 * dfv1) / (jf * jf);
+  const double dShat = (fv2 + q6 * dfv2) * inv_k2_d2;
+  D = D - Sa::cb1 * (q6 * dShat + Shat) * vol;
+  const double r = fmn(q6 * inv_Shat * inv_k2_d2, 10.0);
+  const double r5 = (((r * r) * r) * r) * r;
+  const double g = r + Sa::cw2 * ((r5 * r) - r);
+  const double g_6 = ((((g * g) * g) * g) * g) * g;
+  const double glim = pow((1.0 + Sa::cw3_6) / (g_6 + Sa::cw3_6), 1.0 / 6.0);
+  const double fw = g * glim;
+  const double dr = (Shat - q6 * dShat) * inv_Shat * inv_Shat * inv_k2_d2;
+  const double dg = dr * (1.0 + Sa::cw2 * (6.0 * r5 - 1.0));
+  const double dfw = dg * glim * (1.0 - g_6 / (g_6 + Sa::cw3_6));
+  return D + Sa::cw1 * (dfw * q6 + 2 * fw) * q6 / dist_i_2 * vol;
 }
 
 // Hyperplane-major ("skewed") storage of everything the sweeps read.  Cell
@@ -301,6 +345,7 @@ __global__ void __launch_bounds__(256) k_lusgs_prep(Blk b) {
     const double sumL = L[0] + L[1] + L[2] + L[3] + L[4] + L[5];
     const double vol = b.vol[c0];
     dg = (vol / b.dt[ri(b, i, j, k)]) + 0.5 * sumL;
+    if constexpr (NV == 6) dg = sa_diagonal(b, c0, gri(b, i, j, k), Q0[0], Q0[5], vol, dg);
     if constexpr (NV == 7) {
       const double F1 = b.f1[c0], w = Q0[6];
       const double beta = F1 * 0.075 + (1.0 - F1) * 0.0828;
@@ -428,6 +473,7 @@ __global__ void __launch_bounds__(256) k_lusgs_apply(Blk b) {
     U[5] = q[0] * q[5];
     U[6] = q[0] * q[6];
   }
+  if constexpr (NV == 6) U[5] = q[0] * q[5];
 #pragma unroll
   for (int l = 0; l < NV; ++l) U[l] = U[l] + dq[(long)l * b.sk_p];
   b.qp[c] = U[0];
@@ -439,6 +485,7 @@ __global__ void __launch_bounds__(256) k_lusgs_apply(Blk b) {
     if (U[5] > 0) b.qp[5 * nc + c] = U[5] / U[0];
     if (U[6] > 0) b.qp[6 * nc + c] = U[6] / U[0];
   }
+  if constexpr (NV == 6) b.qp[5 * nc + c] = fmx(U[5] / U[0], 1.e-8);  // src/lusgs.f90:2096-2097
 }
 
 }  // namespace fest

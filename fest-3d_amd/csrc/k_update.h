@@ -124,7 +124,29 @@ __global__ void k_dt_fill(Blk b, int use_fixed) {
 
 enum { UPD_FIRST = 1, UPD_LAST = 2, UPD_WRITE_RES = 4, UPD_APPLY = 8, UPD_STORE = 16 };
 
-// residue + SST source, then (UPD_APPLY) the conservative update of the cell
+// the SA point-implicit destruction factor of the update (src/update.f90:
+// 405-420): xi takes the conservative nu-tilde times the current density
+__device__ __forceinline__ double sa_implicit(const Blk &b, long c, long gi, double u1_0, double u1_5, double dt) {
+  const double *gx = b.g[0], *gy = b.g[1], *gz = b.g[2];
+  const long ngr = b.ngr;
+  double vort = sqrt(sqd(gy[2 * ngr + gi] - gz[ngr + gi]) + sqd(gz[gi] - gx[2 * ngr + gi]) +
+                     sqd(gx[ngr + gi] - gy[gi]));
+  double dist = b.dist[c];
+  double kd = Sa::kappa * dist, kd2 = kd * kd;
+  double xi = u1_5 * b.qp[c] / b.mu[c];
+  double x3 = (xi * xi) * xi;
+  double fv1 = x3 / (x3 + Sa::cv1_3);
+  double fv2 = 1.0 - xi / (1 + xi * fv1);
+  double scap = vort + u1_5 * fv2 / kd2;
+  double rsa = fmn(u1_5 / (scap * kd2), 10.0);
+  double r6 = ((((rsa * rsa) * rsa) * rsa) * rsa) * rsa;
+  double g = rsa + Sa::cw2 * (r6 - rsa);
+  double g6 = ((((g * g) * g) * g) * g) * g;
+  double fw = g * pow((1.0 + Sa::cw3_6) / (g6 + Sa::cw3_6), 1.0 / 6.0);
+  return 1. + ((-1.0 * u1_0 * Sa::cb1 * scap) + (2.0 * u1_0 * Sa::cw1 * fw * u1_5 / (dist * dist))) * dt;
+}
+
+// residue + SST/SA source, then (UPD_APPLY) the conservative update of the cell
 __global__ void __launch_bounds__(256) k_update(Blk b, double TF, double SF, int flags) {
   int i, j;
   if (!flat_ij(b.imx - 1, b.jmx - 1, i, j, plane_block(b.xcd_remap))) return;
@@ -146,11 +168,13 @@ __global__ void __launch_bounds__(256) k_update(Blk b, double TF, double SF, int
     R[l] = (F[F0 + 1] - F[F0]) + (G[G0 + sG] - G[G0]) + (H[H0 + sH] - H[H0]);
   }
   double rho = b.qp[c];
-  if (b.turb) {
+  const bool sst = b.ng == 6, sa = b.ng == 5;
+  if (sst) {
     // SST source (src/source.f90:158-270), computed with the gradients
     R[5] = R[5] - b.src[r];
     R[6] = R[6] - b.src[b.nres + r];
   }
+  if (sa) R[5] = R[5] - b.src[r];  // SA source (src/source.f90:835-983)
   if (flags & UPD_WRITE_RES)
     for (int l = 0; l < nv; ++l) b.res[(long)l * b.nres + r] = R[l];
   if (!(flags & UPD_APPLY)) return;
@@ -162,7 +186,8 @@ __global__ void __launch_bounds__(256) k_update(Blk b, double TF, double SF, int
   for (int l = 1; l < nv; ++l) u1[l] = Qu[(long)l * nc + c] * u1[0];
   u1[4] = (u1[4] / (gm - 1.) + 0.5 * (u1[1] * u1[1] + u1[2] * u1[2] + u1[3] * u1[3])) / u1[0] + 0.0;
   double dt = b.dt[r];
-  if (b.turb) {
+  if (sa) R[5] = R[5] / sa_implicit(b, c, gri(b, i, j, k), u1[0], u1[5], dt);
+  if (sst) {
     double F1 = b.f1[c];
     double beta = 0.075 * F1 + (1. - F1) * 0.0828;
     double w7 = b.qp[6 * nc + c];
@@ -187,10 +212,11 @@ __global__ void __launch_bounds__(256) k_update(Blk b, double TF, double SF, int
     return;
   }
   for (int l = 0; l < 5; ++l) b.qp[(long)l * nc + c] = u2[l];
-  if (b.turb) {
+  if (sst) {
     if (u2[5] >= 0.) b.qp[5 * nc + c] = u2[5];
     if (u2[6] >= 0.) b.qp[6 * nc + c] = u2[6];
   }
+  if (sa) b.qp[5 * nc + c] = fmx(u2[5], 1.e-12);
 }
 
 // per-block sums of residue^2 and of the boundary mass fluxes
@@ -239,7 +265,7 @@ __global__ void k_norm_final(Blk b, int nblk) {
   scale[1] = b.rho_inf * v;
   scale[2] = scale[3] = scale[4] = b.rho_inf * v * v;
   scale[5] = (0.5 * b.rho_inf * (v * v * v) + ((b.gm / (b.gm - 1.)) * b.p_inf));
-  scale[6] = b.rho_inf * v * b.tk_inf;
+  scale[6] = b.rho_inf * v * (b.ng == 5 ? b.tv_inf : b.tk_inf);  // nu-tilde for SA (resnorm.f90:158)
   scale[7] = b.rho_inf * v * b.tw_inf;
   b.norms[0] = fabs(acc[0] / scale[0]);
   for (int l = 1; l <= b.nv; ++l) b.norms[l] = sqrt(acc[l] / (scale[l] * scale[l]));

@@ -47,13 +47,13 @@ class Flow(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
                 ("density_inf", "x_speed_inf", "y_speed_inf", "z_speed_inf", "pressure_inf", "tk_inf", "tw_inf",
                  "vel_mag", "MInf", "gm", "R_gas", "mu_ref", "T_ref", "Sutherland_temp", "Pr", "tPr")] + \
-               [("mu_sutherland", ctypes.c_int32)]
+               [("mu_sutherland", ctypes.c_int32), ("tv_inf", ctypes.c_double)]
 
 
 class BC(ctypes.Structure):
     _fields_ = [("id", ctypes.c_int32 * 6)] + [(n, ctypes.c_double * 6) for n in
                 ("fixed_density", "fixed_x_speed", "fixed_y_speed", "fixed_z_speed", "fixed_pressure",
-                 "fixed_tk", "fixed_tw", "fixed_wall_temperature", "fixed_Tpressure")]
+                 "fixed_tk", "fixed_tw", "fixed_wall_temperature", "fixed_Tpressure", "fixed_tv")]
 
 
 _lib = None
@@ -121,6 +121,7 @@ def c_structs(case: Case):
     f.pressure_inf, f.tk_inf, f.tw_inf, f.vel_mag, f.MInf = case.p_inf, case.tk_inf, case.tw_inf, case.vel_mag, case.MInf
     f.gm, f.R_gas, f.mu_ref, f.T_ref, f.Sutherland_temp = case.gm, case.R_gas, case.mu_ref, case.T_ref, case.S_ref
     f.Pr, f.tPr, f.mu_sutherland = case.Pr, case.tPr, case.mu_sutherland
+    f.tv_inf = case.tv_inf
     b = BC()
     b.id[:] = list(case.bc_id)
     b.fixed_density[:] = list(case.fix_rho)
@@ -132,6 +133,7 @@ def c_structs(case: Case):
     b.fixed_tw[:] = list(case.fix_tw)
     b.fixed_wall_temperature[:] = list(case.fix_twall)
     b.fixed_Tpressure[:] = list(case.fix_tp)
+    b.fixed_tv[:] = list(case.fix_tv)
     return e, s, f, b
 
 

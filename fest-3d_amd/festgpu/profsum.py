@@ -32,7 +32,7 @@ PROBE_BYTES = 512 * 2 ** 20  # copy_probe: read and written per launch
 
 GROUPS = [  # (substring of the demangled kernel name, kernel group)
     ("k_flux<0", "flux_i"), ("k_flux<1", "flux_j"), ("k_flux<2", "flux_k"),
-    ("k_grad<", "grad"), ("k_grad_bc", "grad_bc"), ("k_update", "update"),
+    ("k_grad<", "grad"), ("k_grad_bc", "grad_bc"), ("k_mut_ghost_tp", "grad_bc"), ("k_update", "update"),
     ("k_bc_face", "bc_face"), ("k_bc_edges", "bc_edges"), ("k_step_begin", "step_begin"),
     ("k_dt_min", "time_step"), ("k_dt_fill", "time_step"), ("k_dt", "time_step"),
     ("k_norm", "resnorm"), ("k_node_dist", "setup"), ("k_cell_dist", "setup"), ("k_cells", "setup"),

@@ -22,7 +22,8 @@ module fest_gpu
       FEST_FLUX_AUSMUP = 2, FEST_FLUX_SLAU = 3, FEST_FLUX_LDFSS0 = 4, FEST_FLUX_VANLEER = 5
   integer(c_int32_t), parameter, public :: FEST_RECON_NONE = 0, FEST_RECON_MUSCL = 1, &
       FEST_RECON_PPM = 2, FEST_RECON_WENO = 3
-  integer(c_int32_t), parameter, public :: FEST_TURB_NONE = 0, FEST_TURB_SST = 1, FEST_TURB_SST2003 = 2
+  integer(c_int32_t), parameter, public :: FEST_TURB_NONE = 0, FEST_TURB_SST = 1, FEST_TURB_SST2003 = 2, &
+                                           FEST_TURB_SA = 3
   integer(c_int32_t), parameter, public :: FEST_TA_NONE = 0, FEST_TA_RK2 = 2, FEST_TA_RK4 = 4, &
       FEST_TA_TVDRK2 = 12, FEST_TA_TVDRK3 = 13, FEST_TA_IMPLICIT = 20
 
@@ -49,6 +50,7 @@ module fest_gpu
     real(c_double) :: tk_inf, tw_inf, vel_mag, MInf
     real(c_double) :: gm, R_gas, mu_ref, T_ref, Sutherland_temp, Pr, tPr
     integer(c_int32_t) :: mu_sutherland
+    real(c_double) :: tv_inf
   end type fest_flow
 
   !< hot-path fields of type(boundarytype), src/vartypes.f90:293-365
@@ -57,6 +59,7 @@ module fest_gpu
     real(c_double) :: fixed_density(6), fixed_x_speed(6), fixed_y_speed(6), fixed_z_speed(6)
     real(c_double) :: fixed_pressure(6), fixed_tk(6), fixed_tw(6), fixed_wall_temperature(6)
     real(c_double) :: fixed_Tpressure(6)
+    real(c_double) :: fixed_tv(6)
   end type fest_bc
 
   !< one interface link of a multiblock grid (the reference's map line,

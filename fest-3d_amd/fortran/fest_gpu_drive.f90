@@ -56,9 +56,10 @@ program fest_gpu_drive
   read(u) s%global_time_step, cfl
   read(u) f%density_inf, f%x_speed_inf, f%y_speed_inf, f%z_speed_inf, f%pressure_inf, &
           f%tk_inf, f%tw_inf, f%vel_mag, f%MInf, f%gm, f%R_gas, f%mu_ref, f%T_ref, &
-          f%Sutherland_temp, f%Pr, f%tPr
+          f%Sutherland_temp, f%Pr, f%tPr, f%tv_inf
   read(u) b%fixed_density, b%fixed_x_speed, b%fixed_y_speed, b%fixed_z_speed, &
-          b%fixed_pressure, b%fixed_tk, b%fixed_tw, b%fixed_wall_temperature, b%fixed_Tpressure
+          b%fixed_pressure, b%fixed_tk, b%fixed_tw, b%fixed_wall_temperature, b%fixed_Tpressure, &
+          b%fixed_tv
   allocate(nodes(-2:e%imx+3, -2:e%jmx+3, -2:e%kmx+3, 3))
   allocate(qp(-2:e%imx+2, -2:e%jmx+2, -2:e%kmx+2, e%n_var))
   allocate(res(1:e%imx-1, 1:e%jmx-1, 1:e%kmx-1, e%n_var))

@@ -91,6 +91,7 @@ contains
       case ("none"); s%turb = FEST_TURB_NONE
       case ("sst");  s%turb = FEST_TURB_SST
       case ("sst2003");  s%turb = FEST_TURB_SST2003
+      case ("sa");  s%turb = FEST_TURB_SA
       case default
         print *, "update: turbulence model '", trim(scheme%turbulence), "' is not on the device path"
         stop 1
@@ -126,6 +127,7 @@ contains
     f%Pr = flow%Pr
     f%tPr = flow%tPr
     f%mu_sutherland = merge(1, 0, trim(flow%mu_variation) == 'sutherland_law')
+    f%tv_inf = flow%tv_inf
 
     b%id = bc%id
     b%fixed_density = bc%fixed_density
@@ -137,6 +139,7 @@ contains
     b%fixed_tw = bc%fixed_tw
     b%fixed_wall_temperature = bc%fixed_wall_temperature
     b%fixed_Tpressure = bc%fixed_Tpressure
+    b%fixed_tv = bc%fixed_tv
   end subroutine describe
 
   !< first call: the device block and its geometry, copied out of the

@@ -2143,7 +2143,8 @@ void orc_resnorm(orc *s, double *out) {
   scale[1] = c->rho_inf * c->vel_mag;
   scale[2] = scale[3] = scale[4] = c->rho_inf * c->vel_mag * c->vel_mag;
   scale[5] = (0.5 * c->rho_inf * (c->vel_mag * c->vel_mag * c->vel_mag) + ((c->gm / (c->gm - 1.)) * c->p_inf));
-  scale[6] = c->rho_inf * c->vel_mag * c->tk_inf;
+  /* nu-tilde for SA, k for SST (src/resnorm.f90:150-158) */
+  scale[6] = c->rho_inf * c->vel_mag * (c->turb == ORC_TURB_SA ? c->tv_inf : c->tk_inf);
   scale[7] = c->rho_inf * c->vel_mag * c->tw_inf;
   double *R = s->a[ORC_RES];
   for (int l = 0; l < s->nv; ++l) {

@@ -258,12 +258,16 @@ def test_nonphysical_state_is_reported():
 
 
 @pytest.mark.parametrize("rule", ["RK4", "implicit"])
-@pytest.mark.parametrize("kind", ["inviscid", "laminar", "sst"])
+@pytest.mark.parametrize("kind", ["inviscid", "laminar", "sst", "sst_slip_k", "sa"])
 def test_two_dimensional_grids(rule, kind):
     """One cell in k (kmx == 2, the reference's 2-D mode): no convective k
-    flux, no z gradients, no SST viscous k flux; two steps."""
-    if kind == "sst":
-        case, nodes, q = C.sst_bump(20, 12, 1)
+    flux, no z gradients, no SST viscous k flux (the laminar one keeps mu_t;
+    with slip walls in k the two k faces differ, so a wrong k flux shows in
+    the residue); two steps."""
+    if kind in ("sst", "sst_slip_k", "sa"):
+        case, nodes, q = C.sst_bump(20, 12, 1, turb="sa" if kind == "sa" else "sst")
+        if kind == "sst_slip_k":
+            case = dataclasses.replace(case, bc_id=case.bc_id[:4] + (-6, -6))
         wall = True
     else:
         case, _, q = C.laminar_box(16, 12, 1)
@@ -281,6 +285,8 @@ def test_two_dimensional_grids(rule, kind):
     assert relerr(out["qp"][:, 3:-3, 3:-3, 3:-3], o.arr("qp")[:, 3:-3, 3:-3, 3:-3]) <= TOL
     assert relerr(out["residue"], o.arr("res")) <= 1e-10
     assert not out["H"][0].any()
+    if kind != "inviscid":
+        assert np.abs(out["H"] - o.arr("H")).max() <= TOL * np.abs(o.arr("F")).max()
 
 
 @pytest.mark.parametrize("rule", ["RK4", "implicit"])
@@ -354,3 +360,46 @@ def test_pole(bcs, recon):
             assert o.step(it) == 0
             g.get_next_solution(it)
         assert relerr(g.download("qp"), o.arr("qp")) <= TOL, rule
+
+
+SA_SCHEMES = [("slau", "weno"), ("ausm", "muscl"), ("van_leer", "none")]
+
+
+@pytest.mark.parametrize("flux,recon", SA_SCHEMES)
+def test_spalart_allmaras_residual(flux, recon):
+    """SA residual: nu-tilde ghosts, mu_t = rho nu~ fv1, the nu-tilde
+    viscous flux and the SA source, against the oracle (pinned bit for bit
+    to the reference build)."""
+    case, nodes, q = C.sst_bump(20, 12, 8, flux=flux, recon=recon, turb="sa")
+    say("residual sa", flux, recon)
+    g, o = pair(case, nodes, q, wall=True)
+    o.temp_refresh()
+    assert o.residual(3) == 0
+    g.residual(3)
+    out = g.download("qp", "F", "G", "H", "residue", "mu", "mut", "grads")
+    assert relerr(out["qp"], o.arr("qp")) <= TOL
+    for n, on in (("F", "F"), ("G", "G"), ("H", "H"), ("residue", "res"), ("mu", "mu")):
+        assert relerr(out[n], o.arr(on)) <= TOL, n
+    for d, n in enumerate(("gx", "gy", "gz")):
+        assert relerr(out["grads"][d], o.arr(n)) <= TOL, n
+    inner = (slice(2, -2),) * 3
+    assert relerr(out["mut"][inner], o.arr("mut")[inner]) <= TOL
+
+
+@pytest.mark.parametrize("rule", ["RK4", "implicit"])
+@pytest.mark.parametrize("bcs", [(-3, -4, -5, -8, -9, -9), (-11, -4, -5, -5, -7, -7)])
+def test_spalart_allmaras_steps(rule, bcs):
+    """Two RK4 steps (the point-implicit SA destruction) or two LU-SGS steps
+    (the SA diagonal), from iteration 1 where the inlets fix nu-tilde."""
+    case, nodes, q = C.sst_bump(20, 12, 6, turb="sa")
+    case = dataclasses.replace(case, time_accuracy=rule, bc_id=bcs, fix_tv=(1.3 * case.tv_inf,) * 6,
+                               fix_tp=(1.01 * case.p_inf,) * 6)
+    say("sa steps", rule, bcs)
+    g, o = pair(case, nodes, q, wall=True)
+    for it in (1, 2):
+        assert o.step(it) == 0
+        g.get_next_solution(it)
+    out = g.download("qp", "residue")
+    assert relerr(out["qp"][:, 3:-3, 3:-3, 3:-3], o.arr("qp")[:, 3:-3, 3:-3, 3:-3]) <= TOL
+    assert relerr(out["residue"], o.arr("res")) <= 1e-10
+    np.testing.assert_allclose(g.resnorm(), o.resnorm(), rtol=1e-10)
