@@ -55,6 +55,8 @@ struct Blk {
   double gm, R, mu_ref, T_ref, S_ref, Pr, tPr, MInf, vel_mag;
   double iPr, itPr, cpR;  // 1/Pr, 1/Pr_t, gamma R/(gamma-1): the heat-flux factors
   double rho_inf, u_inf, v_inf, w_inf, p_inf, tk_inf, tw_inf, tv_inf;
+  int trans;                  // 1: the 'bc' intermittency in the turbulence source
+  double tu_inf, Re;          // free-stream turbulence intensity (percent), rho V / mu_ref
   double c1, c2, c3;
   double fix[10][6];          // rho,u,v,w,p,tk,tw,wall temperature,total pressure,nu-tilde
   double global_dt;

@@ -257,10 +257,19 @@ int launch_residual_front(fest_handle *h, int current_iter, int store_csnd) {
     box(b.imx + 1, b.jmx + 1, b.kmx + 1, g, t, b.xcd_remap);
     {
       KgScope ts(h, KG_GRAD);
-      if (b.turb == FEST_TURB_SST2003) hipLaunchKernelGGL((k_grad<6, true>), g, t, 0, st, b);
-      else if (b.ng == 6) hipLaunchKernelGGL((k_grad<6, false>), g, t, 0, st, b);
-      else if (b.ng == 5) hipLaunchKernelGGL((k_grad<5, false>), g, t, 0, st, b);
-      else hipLaunchKernelGGL((k_grad<4, false>), g, t, 0, st, b);
+      if (b.trans) {
+        if (b.turb == FEST_TURB_SST2003) hipLaunchKernelGGL((k_grad<6, true, true>), g, t, 0, st, b);
+        else if (b.ng == 6) hipLaunchKernelGGL((k_grad<6, false, true>), g, t, 0, st, b);
+        else hipLaunchKernelGGL((k_grad<5, false, true>), g, t, 0, st, b);
+      } else if (b.turb == FEST_TURB_SST2003) {
+        hipLaunchKernelGGL((k_grad<6, true>), g, t, 0, st, b);
+      } else if (b.ng == 6) {
+        hipLaunchKernelGGL((k_grad<6, false>), g, t, 0, st, b);
+      } else if (b.ng == 5) {
+        hipLaunchKernelGGL((k_grad<5, false>), g, t, 0, st, b);
+      } else {
+        hipLaunchKernelGGL((k_grad<4, false>), g, t, 0, st, b);
+      }
     }
     int mx = std::max(std::max(b.imx, b.jmx), b.kmx);
     dim3 gb(cdiv(mx, 64), cdiv(mx, 4), 6);
@@ -542,6 +551,10 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
       scheme->turb != FEST_TURB_SA)
     return bad("turbulence model not supported");
   int nv_expect = scheme->turb == FEST_TURB_SA ? 6 : scheme->turb != FEST_TURB_NONE ? 7 : 5;
+  if (scheme->transition != FEST_TRANS_NONE && scheme->transition != FEST_TRANS_BC)
+    return bad("transition model not supported (none, bc)");
+  if (scheme->transition == FEST_TRANS_BC && (scheme->turb == FEST_TURB_NONE || flow->mu_ref == 0.0))
+    return bad("the bc transition needs a turbulence model and mu_ref != 0");
   if (dims->n_var != nv_expect) return bad("n_var does not match the turbulence model");
   if (scheme->flux < 0 || scheme->flux > 5) return bad("unknown flux scheme");
   if (scheme->recon < 0 || scheme->recon > 3) return bad("unknown reconstruction");
@@ -610,6 +623,9 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
   b.rho_inf = flow->density_inf; b.u_inf = flow->x_speed_inf; b.v_inf = flow->y_speed_inf;
   b.w_inf = flow->z_speed_inf; b.p_inf = flow->pressure_inf; b.tk_inf = flow->tk_inf; b.tw_inf = flow->tw_inf;
   b.tv_inf = flow->tv_inf;
+  b.trans = scheme->transition;
+  b.tu_inf = flow->tu_inf;
+  b.Re = flow->density_inf * flow->vel_mag * 1.0 / flow->mu_ref;  // state.f90:89
   b.c2 = 1 + scheme->accur;
   b.c3 = 0.5 * scheme->accur;
   b.c1 = b.c2 - b.c3;

@@ -88,6 +88,71 @@ __device__ __forceinline__ double sa_source(double rho, double tv, double mu, do
   return (P_v - D_v + lamda) * vol;
 }
 
+// The 'bc' intermittency 1 - exp(-(term1 + term2)) (src/source.f90:566-580,
+// 1128-1142)
+__device__ __forceinline__ double gamma_bc(const Blk &b, double nu_t, double vmag, double dist, double re_v) {
+  const double chi_1 = 0.002, chi_2 = 5.0;
+  double nu_cr = chi_2 / b.Re;
+  double nu_bc = nu_t / (vmag * dist);
+  double re_theta = re_v / 2.193;
+  double re_theta_t = (803.73 * pow(b.tu_inf + 0.6067, -1.027));
+  double term1 = sqrt(fmx(re_theta - re_theta_t, 0.) / (chi_1 * re_theta_t));
+  double term2 = sqrt(fmx(nu_bc - nu_cr, 0.0) / nu_cr);
+  return 1.0 - exp(-(term1 + term2));
+}
+
+// SA source with the 'bc' transition (src/source.f90:985-1194): production
+// times gamma_BC and destruction in kinematic form, the cross diffusion as
+// in sa_source
+__device__ __forceinline__ double sa_bc_source(const Blk &b, double rho, double vmag, double tv, double mu,
+                                               double dist, double vol, const double (&g)[5][3],
+                                               const double (&grho)[3]) {
+  double Omega = sqrt(sqd(g[2][1] - g[1][2]) + sqd(g[0][2] - g[2][0]) + sqd(g[1][0] - g[0][1]));
+  double CD1 = Sa::cb2 * ((g[4][0] * g[4][0]) + (g[4][1] * g[4][1]) + (g[4][2] * g[4][2]));
+  double CD2 = (grho[0] * g[4][0]) + (grho[1] * g[4][1]) + (grho[2] * g[4][2]);
+  double d2 = dist * dist, k2 = Sa::kappa * Sa::kappa;
+  double nu = mu / rho;
+  double Ji = tv / nu, Ji_3 = (Ji * Ji) * Ji;
+  double fv1 = Ji_3 / (Ji_3 + Sa::cv1_3);
+  double fv2 = 1.0 - Ji / (1.0 + (Ji * fv1));
+  double inv_k2_d2 = 1.0 / (k2 * d2);
+  double Shat = fmx(Omega + tv * fv2 * inv_k2_d2, 1.0e-10);
+  double inv_Shat = 1.0 / Shat;
+  double gBC = gamma_bc(b, tv * fv1, vmag, dist, d2 * Omega / nu);
+  double Production = gBC * Sa::cb1 * Shat * tv * vol;
+  double r = fmn(tv * inv_Shat * inv_k2_d2, 10.0);
+  double r6 = ((((r * r) * r) * r) * r) * r;
+  double gg = r + Sa::cw2 * (r6 - r);
+  double g6 = ((((gg * gg) * gg) * gg) * gg) * gg;
+  double fw = gg * pow((1.0 + Sa::cw3_6) / (g6 + Sa::cw3_6), 1.0 / 6.0);
+  double Destruction = (Sa::cw1 * fw * tv * tv / d2) * vol;
+  double lamda = (rho * CD1 / Sa::sigma - CD2 * (nu + tv) / Sa::sigma) * vol;
+  return Production - Destruction + lamda;
+}
+
+// SST source with the 'bc' transition (src/source.f90:467-604): production
+// mu_t Omega^2 without the divergence term, limited at 20 D_k, the original
+// gama1/gama2 for either SST form, P_k scaled by gamma_BC after P_omega
+__device__ __forceinline__ void sst_bc_source(const Blk &b, double rho, double vmag, double tk, double tw,
+                                              double mut, double F1, double mu, double dist, double vol,
+                                              const double (&g)[6][3], double &S_k, double &S_w) {
+  const double sigma_w1 = 0.5, sigma_w2 = 0.856, beta1 = 0.075, beta2 = 0.0828, bstar = 0.09, kap = 0.41;
+  const double gama1 = (beta1 / bstar) - ((sigma_w1 * (kap * kap)) / sqrt(bstar));
+  const double gama2 = (beta2 / bstar) - ((sigma_w2 * (kap * kap)) / sqrt(bstar));
+  double vort = sqrt(sqd(g[2][1] - g[1][2]) + sqd(g[0][2] - g[2][0]) + sqd(g[1][0] - g[0][1]));
+  double CD = 2 * rho * sigma_w2 * (g[4][0] * g[5][0] + g[4][1] * g[5][1] + g[4][2] * g[5][2]) / tw;
+  double gama = gama1 * F1 + gama2 * (1. - F1);
+  double beta = beta1 * F1 + beta2 * (1. - F1);
+  double D_k = bstar * rho * tw * tk;
+  double D_w = beta * rho * (tw * tw);
+  double P_k = fmn(mut * (vort * vort), 20.0 * D_k);
+  double P_w = (rho * gama / mut) * P_k;
+  double lamda = (1. - F1) * CD;
+  P_k = gamma_bc(b, mut / rho, vmag, dist, rho * dist * dist * vort / mu) * P_k;
+  S_k = (P_k - D_k) * vol;
+  S_w = (P_w - D_w + lamda) * vol;
+}
+
 // SST source of an interior cell, times its volume (src/source.f90:158-270);
 // S2003: production limited to 10 D_k, CD floored at 1e-10, gama1 = 5/9,
 // gama2 = 0.44 (src/source.f90:205-211)
@@ -119,7 +184,7 @@ __device__ __forceinline__ void sst_source(double rho, double tk, double tw, dou
 // source.  Interior gradients are final here: the ghost-gradient rule only
 // writes ghost cells.  NG is the number of gradient slots (4 laminar, 6
 // SST); a compile-time count keeps all 3*NG gradients in registers.
-template <int NG, bool S2003>
+template <int NG, bool S2003, bool BC = false>
 __global__ void __launch_bounds__(256) k_grad(Blk b) {
   int i, j;
   if (!flat_ij(b.imx + 1, b.jmx + 1, i, j, plane_block(b.xcd_remap))) return;
@@ -192,7 +257,12 @@ __global__ void __launch_bounds__(256) k_grad(Blk b) {
   b.f1[c] = F1;
   if (i >= 1 && i <= b.imx - 1 && j >= 1 && j <= b.jmx - 1 && k >= 1 && k <= b.kmx - 1) {
     double S_k, S_w;
-    sst_source<S2003>(rho, tk, tw, mut, F1, vol, gg, S_k, S_w);
+    if constexpr (BC) {
+      const double u = b.qp[b.nc + c], v = b.qp[2 * b.nc + c], w = b.qp[3 * b.nc + c];
+      sst_bc_source(b, rho, sqrt(u * u + v * v + w * w), tk, tw, mut, F1, mu, b.dist[c], vol, gg, S_k, S_w);
+    } else {
+      sst_source<S2003>(rho, tk, tw, mut, F1, vol, gg, S_k, S_w);
+    }
     long r = ri(b, i, j, k);
     b.src[r] = S_k;
     b.src[b.nres + r] = S_w;
@@ -219,7 +289,12 @@ __global__ void __launch_bounds__(256) k_grad(Blk b) {
       g = g + rf[5] * n[d][5] * Ak1;
       grho[d] = g / (2.0 * vol);
     }
-    b.src[ri(b, i, j, k)] = sa_source(rho, tv, mu, b.dist[c], vol, gg, grho);
+    if constexpr (BC) {
+      const double u = b.qp[b.nc + c], v = b.qp[2 * b.nc + c], w = b.qp[3 * b.nc + c];
+      b.src[ri(b, i, j, k)] = sa_bc_source(b, rho, sqrt(u * u + v * v + w * w), tv, mu, b.dist[c], vol, gg, grho);
+    } else {
+      b.src[ri(b, i, j, k)] = sa_source(rho, tv, mu, b.dist[c], vol, gg, grho);
+    }
   }
   }
 }

@@ -28,6 +28,7 @@ import numpy as np
 FLUX = {"ausm": 0, "ausmP": 1, "ausmUP": 2, "slau": 3, "ldfss0": 4, "van_leer": 5}
 RECON = {"none": 0, "muscl": 1, "ppm": 2, "weno": 3}
 TURB = {"none": 0, "sst": 1, "sst2003": 2, "sa": 3}
+TRANSITION = {"none": 0, "bc": 1}
 TIME_ACC = {"none": 0, "RK2": 2, "RK4": 4, "TVDRK2": 12, "TVDRK3": 13, "implicit": 20}
 
 _GOLDEN = np.uint64(0x9E3779B97F4A7C15)
@@ -92,6 +93,8 @@ class Case:
     fix_tp: tuple = (0.0,) * 6      # total pressure of a -11 face
     tv_inf: float = 0.0             # Spalart-Allmaras free-stream nu-tilde
     fix_tv: tuple = (0.0,) * 6      # fixed nu-tilde of each face
+    transition: str = "none"       # 'bc': the intermittency of the SA and SST sources
+    tu_inf: float = 0.0             # free-stream turbulence intensity, percent
     name: str = "case"
 
     @property
@@ -143,6 +146,7 @@ def freestream(case: Case, T_inf=300.0, p_inf=101325.0, mach=0.5, reynolds=1000.
         case.tw_inf = rho * case.tk_inf / (case.mu_ref * mu_ratio)
     if case.turb == "sa":
         case.tv_inf = mu_ratio * case.mu_ref / rho
+    case.tu_inf = tu_percent
     return case
 
 

@@ -18,10 +18,11 @@ DRIVER = os.path.join(HERE, "fest_gpu_drive")
 def write_input(path, case: Case, nodes: np.ndarray, qp: np.ndarray, nsteps: int, wall_dist: bool):
     e, s, f, b = c_structs(case)
     hdr = [e.imx, e.jmx, e.kmx, e.n_var, s.flux, s.recon, s.turb, s.time_local, s.time_accuracy, s.accur,
-           *s.lim, *s.tlim, f.mu_sutherland, *b.id, nsteps, int(wall_dist)]
-    assert len(hdr) == 25
+           *s.lim, *s.tlim, f.mu_sutherland, *b.id, nsteps, int(wall_dist), s.transition]
+    assert len(hdr) == 26
     flow = [f.density_inf, f.x_speed_inf, f.y_speed_inf, f.z_speed_inf, f.pressure_inf, f.tk_inf, f.tw_inf,
-            f.vel_mag, f.MInf, f.gm, f.R_gas, f.mu_ref, f.T_ref, f.Sutherland_temp, f.Pr, f.tPr, f.tv_inf]
+            f.vel_mag, f.MInf, f.gm, f.R_gas, f.mu_ref, f.T_ref, f.Sutherland_temp, f.Pr, f.tPr, f.tv_inf,
+            f.tu_inf]
     fixed = [b.fixed_density, b.fixed_x_speed, b.fixed_y_speed, b.fixed_z_speed, b.fixed_pressure, b.fixed_tk,
              b.fixed_tw, b.fixed_wall_temperature, b.fixed_Tpressure, b.fixed_tv]
     with open(path, "wb") as fh:

@@ -15,7 +15,7 @@ import os
 
 import numpy as np
 
-from .cases import FLUX, RECON, TURB, TIME_ACC, Case
+from .cases import FLUX, RECON, TURB, TIME_ACC, TRANSITION, Case
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_DIR, "libfest_hip.so")
@@ -40,14 +40,15 @@ class Extent(ctypes.Structure):
 class Scheme(ctypes.Structure):
     _fields_ = [("flux", ctypes.c_int32), ("recon", ctypes.c_int32), ("turb", ctypes.c_int32),
                 ("time_local", ctypes.c_int32), ("time_accuracy", ctypes.c_int32), ("accur", ctypes.c_int32),
-                ("lim", ctypes.c_int32 * 3), ("tlim", ctypes.c_int32 * 3), ("global_time_step", ctypes.c_double)]
+                ("lim", ctypes.c_int32 * 3), ("tlim", ctypes.c_int32 * 3), ("global_time_step", ctypes.c_double),
+                ("transition", ctypes.c_int32)]
 
 
 class Flow(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
                 ("density_inf", "x_speed_inf", "y_speed_inf", "z_speed_inf", "pressure_inf", "tk_inf", "tw_inf",
                  "vel_mag", "MInf", "gm", "R_gas", "mu_ref", "T_ref", "Sutherland_temp", "Pr", "tPr")] + \
-               [("mu_sutherland", ctypes.c_int32), ("tv_inf", ctypes.c_double)]
+               [("mu_sutherland", ctypes.c_int32), ("tv_inf", ctypes.c_double), ("tu_inf", ctypes.c_double)]
 
 
 class BC(ctypes.Structure):
@@ -116,12 +117,14 @@ def c_structs(case: Case):
     s.lim[:] = list(case.lim)
     s.tlim[:] = list(case.tlim)
     s.global_time_step = case.global_time_step
+    s.transition = TRANSITION[case.transition]
     f = Flow()
     f.density_inf, f.x_speed_inf, f.y_speed_inf, f.z_speed_inf = case.rho_inf, case.u_inf, case.v_inf, case.w_inf
     f.pressure_inf, f.tk_inf, f.tw_inf, f.vel_mag, f.MInf = case.p_inf, case.tk_inf, case.tw_inf, case.vel_mag, case.MInf
     f.gm, f.R_gas, f.mu_ref, f.T_ref, f.Sutherland_temp = case.gm, case.R_gas, case.mu_ref, case.T_ref, case.S_ref
     f.Pr, f.tPr, f.mu_sutherland = case.Pr, case.tPr, case.mu_sutherland
     f.tv_inf = case.tv_inf
+    f.tu_inf = case.tu_inf
     b = BC()
     b.id[:] = list(case.bc_id)
     b.fixed_density[:] = list(case.fix_rho)

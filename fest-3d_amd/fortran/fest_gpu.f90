@@ -22,6 +22,7 @@ module fest_gpu
       FEST_FLUX_AUSMUP = 2, FEST_FLUX_SLAU = 3, FEST_FLUX_LDFSS0 = 4, FEST_FLUX_VANLEER = 5
   integer(c_int32_t), parameter, public :: FEST_RECON_NONE = 0, FEST_RECON_MUSCL = 1, &
       FEST_RECON_PPM = 2, FEST_RECON_WENO = 3
+  integer(c_int32_t), parameter, public :: FEST_TRANS_NONE = 0, FEST_TRANS_BC = 1
   integer(c_int32_t), parameter, public :: FEST_TURB_NONE = 0, FEST_TURB_SST = 1, FEST_TURB_SST2003 = 2, &
                                            FEST_TURB_SA = 3
   integer(c_int32_t), parameter, public :: FEST_TA_NONE = 0, FEST_TA_RK2 = 2, FEST_TA_RK4 = 4, &
@@ -42,6 +43,7 @@ module fest_gpu
     integer(c_int32_t) :: flux, recon, turb, time_local, time_accuracy, accur
     integer(c_int32_t) :: lim(3), tlim(3)
     real(c_double) :: global_time_step
+    integer(c_int32_t) :: transition
   end type fest_scheme
 
   !< hot-path fields of type(flowtype), src/vartypes.f90:239-290
@@ -50,7 +52,7 @@ module fest_gpu
     real(c_double) :: tk_inf, tw_inf, vel_mag, MInf
     real(c_double) :: gm, R_gas, mu_ref, T_ref, Sutherland_temp, Pr, tPr
     integer(c_int32_t) :: mu_sutherland
-    real(c_double) :: tv_inf
+    real(c_double) :: tv_inf, tu_inf
   end type fest_flow
 
   !< hot-path fields of type(boundarytype), src/vartypes.f90:293-365

@@ -9,12 +9,12 @@
 !< IN (stream, native endianness), in this order:
 !<   int32  imx, jmx, kmx, n_var, flux, recon, turb, time_local,
 !<          time_accuracy, accur, lim(3), tlim(3), mu_sutherland, bc_id(6),
-!<          nsteps, with_wall_dist
+!<          nsteps, with_wall_dist, transition
 !<   real64 global_time_step, CFL, density_inf, x_speed_inf, y_speed_inf,
 !<          z_speed_inf, pressure_inf, tk_inf, tw_inf, vel_mag, MInf, gm,
-!<          R_gas, mu_ref, T_ref, Sutherland_temp, Pr, tPr,
+!<          R_gas, mu_ref, T_ref, Sutherland_temp, Pr, tPr, tv_inf, tu_inf,
 !<          fixed_{density, x_speed, y_speed, z_speed, pressure, tk, tw,
-!<          wall_temperature}(6)
+!<          wall_temperature, Tpressure, tv}(6)
 !<   real64 nodes(-2:imx+3,-2:jmx+3,-2:kmx+3,3), interior 1..imx filled
 !<   real64 qp(-2:imx+2,-2:jmx+2,-2:kmx+2,n_var)
 !< OUT: real64 qp (same extent), residue(1:imx-1,1:jmx-1,1:kmx-1,n_var),
@@ -28,7 +28,7 @@ program fest_gpu_drive
   type(fest_flow) :: f
   type(fest_bc) :: b
   type(c_ptr) :: h
-  integer(c_int32_t) :: hdr(25)
+  integer(c_int32_t) :: hdr(26)
   real(c_double) :: cfl
   real(c_double), allocatable, target :: nodes(:,:,:,:), qp(:,:,:,:), res(:,:,:,:)
   real(c_double) :: norms(0:7)
@@ -53,10 +53,11 @@ program fest_gpu_drive
   b%id = hdr(18:23)
   nsteps = hdr(24)
   wall = hdr(25)
+  s%transition = hdr(26)
   read(u) s%global_time_step, cfl
   read(u) f%density_inf, f%x_speed_inf, f%y_speed_inf, f%z_speed_inf, f%pressure_inf, &
           f%tk_inf, f%tw_inf, f%vel_mag, f%MInf, f%gm, f%R_gas, f%mu_ref, f%T_ref, &
-          f%Sutherland_temp, f%Pr, f%tPr, f%tv_inf
+          f%Sutherland_temp, f%Pr, f%tPr, f%tv_inf, f%tu_inf
   read(u) b%fixed_density, b%fixed_x_speed, b%fixed_y_speed, b%fixed_z_speed, &
           b%fixed_pressure, b%fixed_tk, b%fixed_tw, b%fixed_wall_temperature, b%fixed_Tpressure, &
           b%fixed_tv

@@ -109,6 +109,13 @@ contains
     s%lim = [scheme%ilimiter_switch, scheme%jlimiter_switch, scheme%klimiter_switch]
     s%tlim = [scheme%itlimiter_switch, scheme%jtlimiter_switch, scheme%ktlimiter_switch]
     s%global_time_step = scheme%global_time_step
+    select case (trim(scheme%transition))
+      case ("none"); s%transition = FEST_TRANS_NONE
+      case ("bc"); s%transition = FEST_TRANS_BC
+      case default
+        print *, "update: transition model '", trim(scheme%transition), "' is not on the device path"
+        stop 1
+    end select
 
     f%density_inf = flow%density_inf
     f%x_speed_inf = flow%x_speed_inf
@@ -128,6 +135,7 @@ contains
     f%tPr = flow%tPr
     f%mu_sutherland = merge(1, 0, trim(flow%mu_variation) == 'sutherland_law')
     f%tv_inf = flow%tv_inf
+    f%tu_inf = flow%tu_inf
 
     b%id = bc%id
     b%fixed_density = bc%fixed_density

@@ -49,6 +49,7 @@ enum { FEST_RECON_NONE = 0, FEST_RECON_MUSCL = 1, FEST_RECON_PPM = 2,
        FEST_RECON_WENO = 3 };
 enum { FEST_TURB_NONE = 0, FEST_TURB_SST = 1, FEST_TURB_SST2003 = 2 /* src/viscosity.f90:215, source.f90:205 */,
        FEST_TURB_SA = 3 /* 'sa', src/viscosity.f90:149, source.f90:835, lusgs.f90:1680 */ };
+enum { FEST_TRANS_NONE = 0, FEST_TRANS_BC = 1 /* the 'bc' intermittency of the SA and SST sources */ };
 enum { FEST_TA_NONE = 0, FEST_TA_RK2 = 2, FEST_TA_RK4 = 4, FEST_TA_TVDRK2 = 12, FEST_TA_TVDRK3 = 13,
        FEST_TA_IMPLICIT = 20 /* LU-SGS, src/lusgs.f90 */ };
 
@@ -66,6 +67,7 @@ typedef struct {
   int32_t lim[3];          /* i/j/klimiter_switch                        */
   int32_t tlim[3];         /* i/j/ktlimiter_switch                       */
   double global_time_step; /* global_time_step (used when > 0, 'g')      */
+  int32_t transition;      /* transition -> FEST_TRANS_* (source.f90:125-145) */
 } fest_scheme;
 
 /* the hot-path fields of type(flowtype), src/vartypes.f90:239-290 */
@@ -75,6 +77,7 @@ typedef struct {
   double gm, R_gas, mu_ref, T_ref, Sutherland_temp, Pr, tPr;
   int32_t mu_sutherland;   /* mu_variation: 0 'constant', 1 'sutherland_law' */
   double tv_inf;           /* Spalart-Allmaras free-stream nu-tilde (state.f90:106) */
+  double tu_inf;           /* free-stream turbulence intensity, percent (vartypes.f90) */
 } fest_flow;
 
 /* the hot-path fields of type(boundarytype), src/vartypes.f90:293-365 */

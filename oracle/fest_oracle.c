@@ -1579,11 +1579,138 @@ static void add_sa_source(orc *s) {
 }
 
 
+/* The density gradient of the SA sources: a Green-Gauss sum over the face
+ * averages rho_neighbour + rho, whose k-face normal is (nx, nx, nx)
+ * (source.f90:887, 1039) */
+static void sa_rho_gradient(const orc *s, int i, int j, int k, double gr[3]) {
+  double density = QP(0, i, j, k);
+  double rf[6] = {QP(0, i - 1, j, k) + density, QP(0, i, j - 1, k) + density, QP(0, i, j, k - 1) + density,
+                  QP(0, i + 1, j, k) + density, QP(0, i, j + 1, k) + density, QP(0, i, j, k + 1) + density};
+  size_t f[6] = {IFI(s, i, j, k), JFI(s, i, j, k), KFI(s, i, j, k),
+                 IFI(s, i + 1, j, k), JFI(s, i, j + 1, k), KFI(s, i, j, k + 1)};
+  const int arr[6][4] = {{ORC_IA, ORC_INX, ORC_INY, ORC_INZ}, {ORC_JA, ORC_JNX, ORC_JNY, ORC_JNZ},
+                         {ORC_KA, ORC_KNX, ORC_KNX, ORC_KNX}, {ORC_IA, ORC_INX, ORC_INY, ORC_INZ},
+                         {ORC_JA, ORC_JNX, ORC_JNY, ORC_JNZ}, {ORC_KA, ORC_KNX, ORC_KNY, ORC_KNZ}};
+  double vol = s->a[ORC_VOL][CI(s, i, j, k)];
+  for (int c3 = 0; c3 < 3; ++c3) {
+    double g = -(rf[0] * s->a[arr[0][1 + c3]][f[0]] * s->a[arr[0][0]][f[0]]);
+    g = g - rf[1] * s->a[arr[1][1 + c3]][f[1]] * s->a[arr[1][0]][f[1]];
+    g = g - rf[2] * s->a[arr[2][1 + c3]][f[2]] * s->a[arr[2][0]][f[2]];
+    g = g + rf[3] * s->a[arr[3][1 + c3]][f[3]] * s->a[arr[3][0]][f[3]];
+    g = g + rf[4] * s->a[arr[4][1 + c3]][f[4]] * s->a[arr[4][0]][f[4]];
+    g = g + rf[5] * s->a[arr[5][1 + c3]][f[5]] * s->a[arr[5][0]][f[5]];
+    gr[c3] = g / (2 * vol);
+  }
+}
+
+/* The 'bc' intermittency gamma_BC = 1 - exp(-(term1 + term2)) of
+ * source.f90:1128-1142 and 566-580; nu_t, re_v from the caller */
+static double gamma_bc(double nu_t, double vmag, double dist, double re_v, double Re, double tu) {
+  double chi_1 = 0.002, chi_2 = 5.0;
+  double nu_cr = chi_2 / Re;
+  double nu_bc = nu_t / (vmag * dist);
+  double re_theta = re_v / 2.193;
+  double re_theta_t = (803.73 * pow(tu + 0.6067, -1.027));
+  double term1 = sqrt(fmx(re_theta - re_theta_t, 0.) / (chi_1 * re_theta_t));
+  double term2 = sqrt(fmx(nu_bc - nu_cr, 0.0) / nu_cr);
+  return 1.0 - exp(-(term1 + term2));
+}
+
+static double reynolds(const orc *s) {
+  /* state.f90:89 */
+  return s->c.rho_inf * s->c.vel_mag * 1.0 / s->c.mu_ref;
+}
+
+/* add_saBC_source, src/source.f90:985-1194: production (kinematic, times
+ * gamma_BC) and destruction without density, the cross diffusion with */
+static void add_sa_bc_source(orc *s) {
+  double *R = s->a[ORC_RES], *vol = s->a[ORC_VOL], *mu = s->a[ORC_MU], *dist = s->a[ORC_DIST];
+  const double cw1 = CW1_SA, Re = reynolds(s), tu = s->c.tu_inf;
+  for (int k = 1; k <= KMX - 1; ++k) for (int j = 1; j <= JMX - 1; ++j) for (int i = 1; i <= IMX - 1; ++i) {
+    size_t ci = CI(s, i, j, k);
+    double density = QP(0, i, j, k), u = QP(1, i, j, k), v = QP(2, i, j, k), w = QP(3, i, j, k);
+    double tv = QP(5, i, j, k);
+    double vmag = sqrt(u * u + v * v + w * w);
+    double gr[3];
+    sa_rho_gradient(s, i, j, k, gr);
+    double Omega = sqrt(sq(GY(2, i, j, k) - GZ(1, i, j, k)) + sq(GZ(0, i, j, k) - GX(2, i, j, k)) +
+                        sq(GX(1, i, j, k) - GY(0, i, j, k)));
+    double CD1 = cb2 * ((GX(4, i, j, k) * GX(4, i, j, k)) + (GY(4, i, j, k) * GY(4, i, j, k)) +
+                        (GZ(4, i, j, k) * GZ(4, i, j, k)));
+    double CD2 = ((gr[0] * GX(4, i, j, k)) + (gr[1] * GY(4, i, j, k)) + (gr[2] * GZ(4, i, j, k)));
+    double dist_i = dist[ci], dist_i_2 = dist_i * dist_i, k2 = kappa_sa * kappa_sa;
+    double nu = mu[ci] / density;
+    double Ji = tv / nu, Ji_2 = Ji * Ji, Ji_3 = Ji_2 * Ji;
+    double fv1 = (Ji_3) / ((Ji_3) + (CV1_3));
+    double fv2 = 1.0 - Ji / (1.0 + (Ji * fv1));
+    double S = Omega;
+    double inv_k2_d2 = 1.0 / (k2 * dist_i_2);
+    double Shat = S + tv * fv2 * inv_k2_d2;
+    Shat = fmx(Shat, 1.0e-10);
+    double inv_Shat = 1.0 / Shat;
+    double nu_t = tv * fv1;
+    double re_v = dist_i_2 * Omega / nu;
+    double gBC = gamma_bc(nu_t, vmag, dist_i, re_v, Re, tu);
+    double Production = gBC * cb1 * Shat * tv * vol[ci];
+    double r = fmn(tv * inv_Shat * inv_k2_d2, 10.0);
+    double g = r + cw2 * ((ipow(r, 6)) - r);
+    double g_6 = ipow(g, 6);
+    double glim = pow((1.0 + CW3_6) / (g_6 + CW3_6), 1.0 / 6.0);
+    double fw = g * glim;
+    double Destruction = (cw1 * fw * tv * tv / dist_i_2) * (vol[ci]);
+    double lamda = (density * CD1 / sigma_sa - CD2 * (nu + tv) / sigma_sa) * vol[ci];
+    double S_v = (Production - Destruction + lamda);
+    R[(size_t)5 * s->nR + RI(s, i, j, k)] = R[(size_t)5 * s->nR + RI(s, i, j, k)] - S_v;
+  }
+}
+
+/* add_sst_bc_source, src/source.f90:467-604: the SST source with the
+ * production mu_t Omega^2 (no divergence term), limited at 20 D_k, the
+ * original gama1/gama2 whatever the SST form, and P_k scaled by gamma_BC
+ * after P_omega is formed */
+static void add_sst_bc_source(orc *s) {
+  double *R = s->a[ORC_RES], *mut = s->a[ORC_MUT], *F1a = s->a[ORC_F1], *vol = s->a[ORC_VOL];
+  double *mu = s->a[ORC_MU], *dist = s->a[ORC_DIST];
+  const double gama1 = (beta1 / bstar) - ((sigma_w1 * (kappa_sst * kappa_sst)) / sqrt(bstar));
+  const double gama2 = (beta2 / bstar) - ((sigma_w2 * (kappa_sst * kappa_sst)) / sqrt(bstar));
+  const double Re = reynolds(s), tu = s->c.tu_inf;
+  for (int k = 1; k <= KMX - 1; ++k) for (int j = 1; j <= JMX - 1; ++j) for (int i = 1; i <= IMX - 1; ++i) {
+    size_t ci = CI(s, i, j, k);
+    double density = QP(0, i, j, k), tk = QP(5, i, j, k), tw = QP(6, i, j, k);
+    double vort = sqrt(sq(GY(2, i, j, k) - GZ(1, i, j, k)) + sq(GZ(0, i, j, k) - GX(2, i, j, k)) +
+                       sq(GX(1, i, j, k) - GY(0, i, j, k)));
+    double CD = 2 * density * sigma_w2 * (GX(4, i, j, k) * GX(5, i, j, k) + GY(4, i, j, k) * GY(5, i, j, k) +
+                                          GZ(4, i, j, k) * GZ(5, i, j, k)) / tw;
+    double F1 = F1a[ci];
+    double gama = gama1 * F1 + gama2 * (1. - F1);
+    double beta = beta1 * F1 + beta2 * (1. - F1);
+    double D_k = bstar * density * tw * tk;
+    double D_w = beta * density * (tw * tw);
+    double P_k = mut[ci] * (vort * vort);
+    P_k = fmn(P_k, 20.0 * D_k);
+    double P_w = (density * gama / mut[ci]) * P_k;
+    double lamda = (1. - F1) * CD;
+    double u = QP(1, i, j, k), v = QP(2, i, j, k), w = QP(3, i, j, k);
+    double vmag = sqrt(u * u + v * v + w * w);
+    double nu_t = mut[ci] / density;
+    double re_v = density * dist[ci] * dist[ci] * vort / mu[ci];
+    double gBC = gamma_bc(nu_t, vmag, dist[ci], re_v, Re, tu);
+    P_k = gBC * P_k;
+    double S_k = P_k - D_k;
+    double S_w = P_w - D_w + lamda;
+    S_k = S_k * vol[ci];
+    S_w = S_w * vol[ci];
+    R[(size_t)5 * s->nR + RI(s, i, j, k)] = R[(size_t)5 * s->nR + RI(s, i, j, k)] - S_k;
+    R[(size_t)6 * s->nR + RI(s, i, j, k)] = R[(size_t)6 * s->nR + RI(s, i, j, k)] - S_w;
+  }
+}
+
 static void add_sst_source(orc *s);
 /* add_source_term_residue, src/source.f90:94-155 */
 static void add_source_term_residue(orc *s) {
-  if (SST(s)) add_sst_source(s);
-  if (SA(s)) add_sa_source(s);
+  int bc = s->c.transition == 1;
+  if (SST(s)) { if (bc) add_sst_bc_source(s); else add_sst_source(s); }
+  if (SA(s)) { if (bc) add_sa_bc_source(s); else add_sa_source(s); }
 }
 
 /* src/source.f90:158-270; sst2003 limits the production to 10 D_k, floors

@@ -54,6 +54,8 @@ typedef struct {
   double fix_tp[6]; /* total pressure of a -11 face */
   double tv_inf;     /* Spalart-Allmaras free-stream nu-tilde */
   double fix_tv[6];  /* fixed nu-tilde of each face */
+  int32_t transition; /* 0 none, 1 'bc' */
+  double tu_inf;      /* free-stream turbulence intensity, percent */
 } orc_case;
 
 typedef struct orc orc;

@@ -44,6 +44,8 @@ module ref_harness
     real(c_double) :: fix_tk(6), fix_tw(6), fix_twall(6)
     real(c_double) :: fix_tp(6)
     real(c_double) :: tv_inf, fix_tv(6)
+    integer(c_int32_t) :: transition
+    real(c_double) :: tu_inf
   end type hcase
 
   type(extent) :: dims
@@ -86,6 +88,7 @@ contains
       schm%turbulence = 'none'
     end if
     schm%transition = 'none'
+    if (hc%transition == 1) schm%transition = 'bc'
     schm%accur = hc%accur
     schm%ilimiter_switch = hc%lim(1); schm%jlimiter_switch = hc%lim(2); schm%klimiter_switch = hc%lim(3)
     schm%itlimiter_switch = hc%tlim(1); schm%jtlimiter_switch = hc%tlim(2); schm%ktlimiter_switch = hc%tlim(3)
@@ -101,8 +104,11 @@ contains
     flow%y_speed_inf = hc%v_inf; flow%z_speed_inf = hc%w_inf
     flow%pressure_inf = hc%p_inf; flow%tk_inf = hc%tk_inf; flow%tw_inf = hc%tw_inf
     flow%tv_inf = hc%tv_inf
+    flow%tu_inf = hc%tu_inf
     flow%vel_mag = hc%vel_mag; flow%MInf = hc%MInf
     flow%gm = hc%gm; flow%R_gas = hc%R_gas; flow%mu_ref = hc%mu_ref
+    ! set as state.f90:89 sets it
+    if (hc%mu_ref /= 0.0) flow%Reynolds_number = flow%density_inf*flow%vel_mag*1.0/flow%mu_ref
     flow%T_ref = hc%T_ref; flow%Sutherland_temp = hc%S_ref; flow%Pr = hc%Pr; flow%tPr = hc%tPr
     if (hc%mu_sutherland == 1) then
       flow%mu_variation = 'sutherland_law'

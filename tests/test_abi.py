@@ -31,12 +31,12 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_struct_sizes_match_header():
-    # fest_extent is four int32; fest_scheme 12 int32 + one double
+    # fest_extent is four int32; fest_scheme 12 int32, one double, the transition int32 (padded)
     assert ctypes.sizeof(native.Extent) == 16
-    assert ctypes.sizeof(native.Scheme) == 56
+    assert ctypes.sizeof(native.Scheme) == 64  # + transition and padding
     assert ctypes.sizeof(native.BC) == 24 + 8 * 60
     # 16 doubles, mu_sutherland (int32, then 4 bytes of padding), tv_inf
-    assert ctypes.sizeof(native.Flow) == 8 * 16 + 8 + 8
+    assert ctypes.sizeof(native.Flow) == 8 * 16 + 8 + 8 + 8
 
 
 @pytest.mark.parametrize("change,code", [

@@ -403,3 +403,23 @@ def test_spalart_allmaras_steps(rule, bcs):
     assert relerr(out["qp"][:, 3:-3, 3:-3, 3:-3], o.arr("qp")[:, 3:-3, 3:-3, 3:-3]) <= TOL
     assert relerr(out["residue"], o.arr("res")) <= 1e-10
     np.testing.assert_allclose(g.resnorm(), o.resnorm(), rtol=1e-10)
+
+
+@pytest.mark.parametrize("turb", ["sa", "sst", "sst2003"])
+@pytest.mark.parametrize("rule", ["RK4", "implicit"])
+def test_transition_bc(turb, rule):
+    """The 'bc' intermittency in the SA and SST sources: residual and two
+    steps against the oracle (pinned bit for bit to the reference build)."""
+    case, nodes, q = C.sst_bump(20, 12, 6, flux="ausm", recon="muscl", turb=turb)
+    case = dataclasses.replace(case, transition="bc", tu_inf=0.8, time_accuracy=rule)
+    say("bc transition", turb, rule)
+    g, o = pair(case, nodes, q, wall=True)
+    o.temp_refresh()
+    assert o.residual(3) == 0
+    g.residual(3)
+    assert relerr(g.download("residue"), o.arr("res")) <= 1e-10
+    g, o = pair(case, nodes, q, wall=True)
+    for it in (1, 2):
+        assert o.step(it) == 0
+        g.get_next_solution(it)
+    assert relerr(g.download("qp")[:, 3:-3, 3:-3, 3:-3], o.arr("qp")[:, 3:-3, 3:-3, 3:-3]) <= TOL

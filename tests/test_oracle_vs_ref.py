@@ -253,3 +253,34 @@ def test_spalart_allmaras_two_dimensional():
     """One cell in k: the SA viscous flux still runs in k (viscous.f90:94-97)."""
     case, nodes, q = C.sst_bump(14, 10, 1, flux="slau", recon="muscl", turb="sa")
     compare_2d(case, nodes, q, it=2, wall_dist=True, extra=("mut",))
+
+
+@pytest.mark.parametrize("turb", ["sa", "sst", "sst2003"])
+def test_transition_bc(turb):
+    """The 'bc' transition: the intermittency gamma_BC in the SA and SST
+    sources (src/source.f90:467-604, 985-1194), with the free-stream
+    turbulence intensity and the Reynolds number of the free stream."""
+    case, nodes, q = C.sst_bump(14, 10, 6, flux="ausm", recon="muscl", turb=turb)
+    case = dataclasses.replace(case, transition="bc", tu_inf=0.8)
+    extra = ("mut",) if turb == "sa" else ("mut", "f1")
+    compare(case, nodes, q, it=3, wall_dist=True, extra=extra)
+    compare_lusgs(case, nodes, q, it=2, wall_dist=True)
+
+
+def test_transition_bc_changes_the_turbulence_residue():
+    """Guard against the bc switch being a no-op on the cases above: the
+    intermittency changes the first turbulence equation in every cell."""
+    from binding import Oracle
+    for turb in ("sa", "sst"):
+        res = {}
+        for tr in ("none", "bc"):
+            case, nodes, q = C.sst_bump(14, 10, 6, flux="ausm", recon="muscl", turb=turb)
+            case = dataclasses.replace(case, transition=tr, tu_inf=0.8)
+            o = Oracle(case)
+            o.set_nodes(nodes)
+            o.wall_dist()
+            o.set_state(q)
+            o.temp_refresh()
+            assert o.residual(3) == 0
+            res[tr] = o.arr("res")[5].copy()
+        assert (res["bc"] != res["none"]).all(), turb
