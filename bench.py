@@ -84,6 +84,46 @@ def cpu_baseline(case_fn, full_dist, budget_s=25.0, implicit=False):
                       f"({case.ncells} cells), oracle/fest_oracle.c single thread, {dt:.1f} s"}
 
 
+def fp32_flux_run(case, nodes, q, g64, warmup, steps, implicit, breakdown64):
+    """The same workload with the face reconstruction and flux arithmetic in
+    FP32 (FEST_PREC_FP32_FLUX): its rate over the same steps from the same
+    initial state, and its error against the FP64 run after them."""
+    import numpy as np
+    from festgpu.native import GpuBlock
+    g = GpuBlock(case, precision=1)
+    g.set_grid(nodes, wall_dist=case.sst_family)
+    g.upload_state(q)
+    it = 1
+    g.timing("*")
+    for _ in range(warmup):
+        g.get_next_solution(it)
+        it += 1
+    g.sync()
+    flux32 = sum(g.kernel_time(grp)[0] for grp in ("flux_i", "flux_j", "flux_k"))
+    g.timing(None)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.get_next_solution(it)
+        it += 1
+    g.sync()
+    el = time.perf_counter() - t0
+    a, b = g.download("qp", "residue"), g64.download("qp", "residue")
+    g.close()
+    inner = (slice(None), slice(3, -3), slice(3, -3), slice(3, -3))
+    qa, qb = a["qp"][inner], b["qp"][inner]
+    speed = np.sqrt(qb[1] ** 2 + qb[2] ** 2 + qb[3] ** 2).max()
+    err_q = [float(np.abs(qa[l] - qb[l]).max() / (speed if l in (1, 2, 3) else np.abs(qb[l]).max()))
+             for l in range(qb.shape[0])]
+    err_r = float(np.abs(a["residue"] - b["residue"]).max() / np.abs(b["residue"]).max())
+    stages = (1 if implicit else 4) * steps
+    flux64 = sum(breakdown64[grp] for grp in ("flux_i", "flux_j", "flux_k"))
+    return {"value": case.ncells * stages / el, "ms_per_step": 1e3 * el / steps,
+            "flux_ms_per_step": flux32 / max(warmup, 1), "flux_ms_per_step_fp64": flux64 / max(warmup, 1),
+            "max_rel_err_state_per_var": err_q, "max_rel_err_residue": err_r,
+            "note": f"state after {warmup + steps} steps from the same initial state vs the FP64 run; "
+                    "errors relative to each variable's largest magnitude, the velocity components' to the largest speed"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +133,8 @@ def main():
     ap.add_argument("--size", default="256x128x128")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--time-accuracy", default="RK4", choices=["RK4", "implicit"])
+    ap.add_argument("--no-fp32-flux", action="store_true",
+                    help="skip the run with FP32 face reconstruction and flux arithmetic")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -158,6 +200,8 @@ def main():
         "kernel_ms_per_step": ({grp: breakdown[grp] / args.warmup for grp in breakdown}
                                if args.warmup > 0 else None),
     }
+    if not args.no_fp32_flux:
+        out["fp32_flux"] = fp32_flux_run(case, nodes, q, g, args.warmup, args.steps, implicit, breakdown)
     if not args.no_cpu_baseline:
         dist = g.download("dist") if case.sst_family else None
         out["cpu_baseline"] = cpu_baseline(case_fn, dist, implicit=implicit)

@@ -56,6 +56,7 @@ struct Blk {
   double iPr, itPr, cpR;  // 1/Pr, 1/Pr_t, gamma R/(gamma-1): the heat-flux factors
   double rho_inf, u_inf, v_inf, w_inf, p_inf, tk_inf, tw_inf, tv_inf;
   int trans;                  // 1: the 'bc' intermittency in the turbulence source
+  int fp32;                   // 1: face reconstruction and flux arithmetic in FP32
   double tu_inf, Re;          // free-stream turbulence intensity (percent), rho V / mu_ref
   double c1, c2, c3;
   double fix[10][6];          // rho,u,v,w,p,tk,tw,wall temperature,total pressure,nu-tilde
@@ -134,6 +135,11 @@ __device__ __forceinline__ double fmx(double a, double b) { return a > b ? a : b
 __device__ __forceinline__ double fmn(double a, double b) { return a < b ? a : b; }
 __device__ __forceinline__ double fsgn(double a, double b) { return copysign(fabs(a), b); }
 __device__ __forceinline__ double sqd(double x) { return x * x; }
+// FP32 forms for the single-precision face arithmetic
+__device__ __forceinline__ float fmx(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ float fmn(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float fsgn(float a, float b) { return __builtin_copysignf(__builtin_fabsf(a), b); }
+__device__ __forceinline__ float sqd(float x) { return x * x; }
 
 // Spalart-Allmaras constants (src/global/global_sa.f90); cw1 and cv1^3 as
 // the reference's parameter expressions evaluate in double

@@ -188,14 +188,21 @@ int check_err(fest_handle *h) {
 
 // the flux kernel of sweep D, specialised on the variable count and the
 // reconstruction
+template <int D, int NV, bool FLAT, typename T>
+void launch_flux_t(const Blk &b, dim3 g, dim3 t, hipStream_t st, int store_csnd) {
+  switch (b.recon) {
+    case 0: hipLaunchKernelGGL((k_flux<D, NV, 0, FLAT, T>), g, t, 0, st, b, store_csnd); break;
+    case 1: hipLaunchKernelGGL((k_flux<D, NV, 1, FLAT, T>), g, t, 0, st, b, store_csnd); break;
+    case 2: hipLaunchKernelGGL((k_flux<D, NV, 2, FLAT, T>), g, t, 0, st, b, store_csnd); break;
+    default: hipLaunchKernelGGL((k_flux<D, NV, 3, FLAT, T>), g, t, 0, st, b, store_csnd); break;
+  }
+}
+// FP64 face arithmetic, or FP32 when the handle was created with
+// FEST_PREC_FP32_FLUX
 template <int D, int NV, bool FLAT>
 void launch_flux_nv(const Blk &b, dim3 g, dim3 t, hipStream_t st, int store_csnd) {
-  switch (b.recon) {
-    case 0: hipLaunchKernelGGL((k_flux<D, NV, 0, FLAT>), g, t, 0, st, b, store_csnd); break;
-    case 1: hipLaunchKernelGGL((k_flux<D, NV, 1, FLAT>), g, t, 0, st, b, store_csnd); break;
-    case 2: hipLaunchKernelGGL((k_flux<D, NV, 2, FLAT>), g, t, 0, st, b, store_csnd); break;
-    default: hipLaunchKernelGGL((k_flux<D, NV, 3, FLAT>), g, t, 0, st, b, store_csnd); break;
-  }
+  if (b.fp32) launch_flux_t<D, NV, FLAT, float>(b, g, t, st, store_csnd);
+  else launch_flux_t<D, NV, FLAT, double>(b, g, t, st, store_csnd);
 }
 // the k faces of a 2-D grid (one cell in k) carry no convective flux
 template <int D, int NV>
@@ -543,7 +550,8 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
   fest_handle *h = new fest_handle();
   h->dims = *dims; h->scheme = *scheme; h->flow = *flow; h->bc = *bc;
   auto bad = [&](const char *msg) { h->err = msg; *out = h; return FEST_E_CONFIG; };
-  if (precision != 0) return bad("only the plain FP64 path (precision 0) is built");
+  if (precision != FEST_PREC_FP64 && precision != FEST_PREC_FP32_FLUX)
+    return bad("precision must be 0 (FP64) or 1 (FP32 face arithmetic)");
   // a 2-D grid has one cell in k (kmx == 2), src/face/flux/convective/*.f90
   if (dims->imx < 4 || dims->jmx < 4 || (dims->kmx < 4 && dims->kmx != 2))
     return bad("i and j need at least 3 cells, k at least 3 or exactly 1");
@@ -624,6 +632,7 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
   b.w_inf = flow->z_speed_inf; b.p_inf = flow->pressure_inf; b.tk_inf = flow->tk_inf; b.tw_inf = flow->tw_inf;
   b.tv_inf = flow->tv_inf;
   b.trans = scheme->transition;
+  b.fp32 = precision == FEST_PREC_FP32_FLUX;
   b.tu_inf = flow->tu_inf;
   b.Re = flow->density_inf * flow->vel_mag * 1.0 / flow->mu_ref;  // state.f90:89
   b.c2 = 1 + scheme->accur;

@@ -23,6 +23,7 @@ module fest_gpu
   integer(c_int32_t), parameter, public :: FEST_RECON_NONE = 0, FEST_RECON_MUSCL = 1, &
       FEST_RECON_PPM = 2, FEST_RECON_WENO = 3
   integer(c_int32_t), parameter, public :: FEST_TRANS_NONE = 0, FEST_TRANS_BC = 1
+  integer(c_int32_t), parameter, public :: FEST_PREC_FP64 = 0, FEST_PREC_FP32_FLUX = 1
   integer(c_int32_t), parameter, public :: FEST_TURB_NONE = 0, FEST_TURB_SST = 1, FEST_TURB_SST2003 = 2, &
                                            FEST_TURB_SA = 3
   integer(c_int32_t), parameter, public :: FEST_TA_NONE = 0, FEST_TA_RK2 = 2, FEST_TA_RK4 = 4, &

@@ -91,7 +91,13 @@ typedef struct {
 
 typedef struct fest_handle fest_handle;
 
-/* Allocates the device-resident block.  precision: 0 = plain FP64.
+/* precision of fest_create: the plain FP64 path, or face reconstruction and
+ * flux arithmetic in FP32 with metrics, gradients, the residue, the RK
+ * update, the sources and the norms in FP64 (no counterpart in the
+ * reference, which is FP64 throughout) */
+enum { FEST_PREC_FP64 = 0, FEST_PREC_FP32_FLUX = 1 };
+
+/* Allocates the device-resident block.  precision: FEST_PREC_*.
  * Replaces the allocations of setup_update/setup_scheme/setup_gradients/
  * setup_viscosity (src/update.f90:85-126, src/face/flux/convective/
  * scheme.f90:29-65, src/gradients.f90:102-200, src/viscosity.f90:548-592). */

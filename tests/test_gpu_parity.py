@@ -423,3 +423,53 @@ def test_transition_bc(turb, rule):
         assert o.step(it) == 0
         g.get_next_solution(it)
     assert relerr(g.download("qp")[:, 3:-3, 3:-3, 3:-3], o.arr("qp")[:, 3:-3, 3:-3, 3:-3]) <= TOL
+
+
+FP32_CASES = [("sst", "slau", "weno"), ("sst", "ausmUP", "ppm"), ("none", "ausmP", "muscl"), ("sa", "ldfss0", "weno")]
+
+
+@pytest.mark.parametrize("turb,flux,recon", FP32_CASES)
+def test_fp32_flux_mode(turb, flux, recon):
+    """FEST_PREC_FP32_FLUX: reconstruction and face-flux arithmetic in FP32,
+    the rest FP64.  The residue and the state after two RK4 steps against the
+    FP64 oracle, within the bounds stated here; and the mode must differ from
+    the FP64 path (the switch is live)."""
+    if turb == "none":
+        case, _, q = C.laminar_box(20, 12, 8, flux=flux, recon=recon)
+        nodes, wall = C.cartesian_nodes(case, warp=0.03), False
+    else:
+        case, nodes, q = C.sst_bump(20, 12, 8, flux=flux, recon=recon, turb=turb)
+        wall = True
+    say("fp32 flux", turb, flux, recon)
+    o = Oracle(case)
+    o.set_nodes(nodes)
+    if wall:
+        o.wall_dist()
+    o.set_state(q)
+    o.temp_refresh()
+    assert o.residual(3) == 0
+    g = GpuBlock(case, precision=1)
+    g.set_grid(nodes, wall_dist=wall)
+    g.upload_state(q)
+    g.residual(3)
+    e_res = relerr(g.download("residue"), o.arr("res"))
+    g2, o2 = pair(case, nodes, q, wall=wall)
+    g = GpuBlock(case, precision=1)
+    g.set_grid(nodes, wall_dist=wall)
+    g.upload_state(q)
+    for it in (1, 2):
+        assert o2.step(it) == 0
+        g.get_next_solution(it)
+    inner = (slice(None), slice(3, -3), slice(3, -3), slice(3, -3))
+    out = g.download("qp")[inner]
+    ref = o2.arr("qp")[inner]
+    # each variable against its largest magnitude; the velocity components
+    # against the largest speed
+    speed = np.sqrt(ref[1] ** 2 + ref[2] ** 2 + ref[3] ** 2).max()
+    per = [float(np.abs(out[l] - ref[l]).max() / (speed if l in (1, 2, 3) else np.abs(ref[l]).max()))
+           for l in range(ref.shape[0])]
+    e_q = max(per)
+    say(f"  residue rel err {e_res:.3e}  state rel err after 2 steps {e_q:.3e} per variable "
+        + " ".join(f"{e:.2e}" for e in per))
+    assert 1e-12 < e_res <= 1e-4
+    assert 1e-14 < e_q <= 1e-5
