@@ -289,7 +289,9 @@ int launch_residual_front(fest_handle *h, int current_iter, int store_csnd) {
     }
     LAUNCHCHK(h);
   }
-  box(b.imx, b.jmx - 1, b.kmx - 1, g, t, b.xcd_remap);
+  // the i sweep: blocks of FLUX_I_SPAN consecutive padded-plane offsets
+  g = dim3(plane_slots(cdiv(flux_i_span(b), FLUX_I_SPAN), b.xcd_remap), (unsigned)(b.kmx - 1), 1);
+  t = dim3(FLUX_I_SPAN, 1, 1);
   {
     KgScope ts(h, KG_FLUX_I);
     launch_flux<0>(b, g, t, st, store_csnd);

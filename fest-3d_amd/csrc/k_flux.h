@@ -441,20 +441,56 @@ struct FaceMath {
 // counts keep the per-variable arrays in registers.
 // Registers are fitted to 4 waves per SIMD: 5 spills the SST/WENO variants
 // and measured slower even where it does not spill.
+// The i sweep stages its stencils in LDS: a block takes 256 consecutive
+// offsets p of the padded (i, j) plane of one k, i.e. the cells whose lower
+// i face it computes.  The stencil of every face, cells p-3..p+2, is then a
+// contiguous run of that plane, rows included, so one coalesced load per
+// variable fills sq[l][0..260] (offsets p0-3..p0+257) and each face reads its
+// six cells from LDS.  Offsets that are not faces (ghost columns and rows,
+// the tail) take part in the load and the barrier, then leave.
+constexpr int FLUX_I_SPAN = 256;
+__host__ __device__ inline long flux_i_first(const Blk &b) { return 3 + 3L * b.pi; }
+__host__ __device__ inline long flux_i_span(const Blk &b) {
+  return (b.imx + 2) + (long)(b.jmx + 1) * b.pi - flux_i_first(b) + 1;
+}
+
 template <int D, int NV, int RECON, bool FLAT = false, typename T = double>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_flux(Blk b, int store_csnd) {
   using M = FaceMath<T>;
   const int ii = D == 0, jj = D == 1, kk = D == 2;
   int i, j;
-  // i sweep: flat rows (imx faces per row); j, k sweeps: 64 x 4 tiles
   const int blk = plane_block(b.xcd_remap);
-  const bool in = (D == 0) ? flat_ij(b.imx - 1 + ii, b.jmx - 1 + jj, i, j, blk)
-                           : tile_ij(b.imx - 1 + ii, b.jmx - 1 + jj, i, j, blk);
-  if (!in) return;
-  i += 1;
-  j += 1;
   const int k = blockIdx.y + 1;
-  if (k > b.kmx - 1 + kk) return;
+  const double *lds_row = nullptr;  // the i sweep's stencil row in LDS
+  if constexpr (D == 0) {
+    __shared__ double sq[NV][FLUX_I_SPAN + 5];
+    const int t = (int)threadIdx.x;
+    const long p0 = flux_i_first(b) + (long)blk * FLUX_I_SPAN;
+    const long p = p0 + t;
+    const long plane = (long)b.pi * b.pj;
+    i = (int)(p % b.pi) - 2;
+    j = (int)(p / b.pi) - 2;
+    const bool active = p < flux_i_first(b) + flux_i_span(b) && i >= 1 && i <= b.imx && j >= 1 &&
+                        j <= b.jmx - 1 && k <= b.kmx - 1;
+    const long kbase = plane * (long)(k + 2);
+#pragma unroll
+    for (int l = 0; l < NV; ++l) {
+      const double *ql = b.qp + (long)l * b.nc + kbase;
+      for (int e = t; e < FLUX_I_SPAN + 5; e += FLUX_I_SPAN) {
+        const long pe = p0 - 3 + e;
+        sq[l][e] = (pe < plane && k <= b.kmx - 1) ? ql[pe] : 0.0;
+      }
+    }
+    __syncthreads();
+    if (!active) return;
+    lds_row = &sq[0][t + 3];
+  } else {
+    // j, k sweeps: 64 x 4 tiles
+    if (!tile_ij(b.imx - 1 + ii, b.jmx - 1 + jj, i, j, blk)) return;
+    i += 1;
+    j += 1;
+    if (k > b.kmx - 1 + kk) return;
+  }
   const int N = D == 0 ? b.imx : D == 1 ? b.jmx : b.kmx;
   const int n = D == 0 ? i : D == 1 ? j : k;
   const long s = cstride(b, D);
@@ -473,8 +509,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     T qL[NV], qR[NV];
 #pragma unroll
     for (int l = 0; l < nv; ++l) {
-      const double *u = b.qp + (long)l * b.nc + c;
-      M::template face_states<RECON>(b, D, n, N, l, u, s, qL[l], qR[l]);
+      if constexpr (D == 0) {
+        M::template face_states<RECON>(b, D, n, N, l, lds_row + l * (FLUX_I_SPAN + 5), 1, qL[l], qR[l]);
+      } else {
+        const double *u = b.qp + (long)l * b.nc + c;
+        M::template face_states<RECON>(b, D, n, N, l, u, s, qL[l], qR[l]);
+      }
     }
     if (store_csnd) csf = T(0.5) * (tsqrt(T(b.gm) * qL[4] / qL[0]) + tsqrt(T(b.gm) * qR[4] / qR[0]));
     const T zero = T((n == 1) ? (double)b.mz[D][0] : (n == N) ? (double)b.mz[D][1] : 1.0);
