@@ -37,15 +37,16 @@ PROFILES = os.path.join(ROOT, "profiles")
 
 
 def measured_traffic(case, group):
-    """HBM bytes per launch of `group` from the committed counter summary
-    profiles/hbm_traffic_<workload>_<size>.md (its table rows read
-    `| group | kernel | fetch B | write B | traffic B |`), else None."""
+    """HBM bytes per launch of `group`'s FP64 kernel from the committed
+    counter summary profiles/hbm_traffic_<workload>_<size>.md (its table rows
+    read `| group | kernel | fetch B | write B | traffic B |`; the FP32 face
+    kernels, `k_flux<..., float>`, have rows of their own), else None."""
     path = os.path.join(PROFILES, f"hbm_traffic_{case.name}.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 5 and cells[0] == group:
+        if len(cells) == 5 and cells[0] == group and "float" not in cells[1]:
             try:
                 return float(cells[4])
             except ValueError:

@@ -101,7 +101,7 @@ def traffic(fetch_dir, write_dir, probe_fetch, probe_write, case_name):
              f"per launch for {PROBE_BYTES} B read and written: scale factors fetch {kf:.4f}, write {kw:.4f}",
              "",
              "| group | kernel | fetch B | write B | traffic B |", "|---|---|---|---|---|"]
-    extra = ["", "| group | algorithmic B | traffic / algorithmic |", "|---|---|---|"]
+    extra = ["", "| group | kernel | algorithmic B | traffic / algorithmic |", "|---|---|---|---|"]
     for k in sorted(fe, key=lambda k: -fe[k]):
         grp = group_of(k)
         if grp in ("copy_probe", "-"):
@@ -111,7 +111,7 @@ def traffic(fetch_dir, write_dir, probe_fetch, probe_write, case_name):
         lines.append(f"| {grp} | {k} | {f_b:.4g} | {w_b:.4g} | {f_b + w_b:.6g} |")
         alg = R.bytes_per_launch(case, grp) if case is not None else None
         if alg:
-            extra.append(f"| {grp} | {alg:.6g} | {(f_b + w_b) / alg:.3f} |")
+            extra.append(f"| {grp} | {k} | {alg:.6g} | {(f_b + w_b) / alg:.3f} |")
     return "\n".join(lines + extra)
 
 
