@@ -27,7 +27,7 @@ import numpy as np
 # flux / reconstruction / turbulence ids shared with the C-ABI
 FLUX = {"ausm": 0, "ausmP": 1, "ausmUP": 2, "slau": 3, "ldfss0": 4, "van_leer": 5}
 RECON = {"none": 0, "muscl": 1, "ppm": 2, "weno": 3}
-TURB = {"none": 0, "sst": 1, "sst2003": 2, "sa": 3}
+TURB = {"none": 0, "sst": 1, "sst2003": 2, "sa": 3, "kkl": 4}
 TRANSITION = {"none": 0, "bc": 1}
 TIME_ACC = {"none": 0, "RK2": 2, "RK4": 4, "TVDRK2": 12, "TVDRK3": 13, "implicit": 20}
 
@@ -95,6 +95,8 @@ class Case:
     fix_tv: tuple = (0.0,) * 6      # fixed nu-tilde of each face
     transition: str = "none"       # 'bc': the intermittency of the SA and SST sources
     tu_inf: float = 0.0             # free-stream turbulence intensity, percent
+    tkl_inf: float = 0.0            # k-kL free-stream kL
+    fix_tkl: tuple = (0.0,) * 6     # fixed kL of each face (k-kL)
     name: str = "case"
 
     @property
@@ -104,14 +106,19 @@ class Case:
         return self.turb in ("sst", "sst2003")
 
     @property
+    def two_equation(self) -> bool:
+        """SST or k-kL: two turbulence variables, 6 and 7."""
+        return self.sst_family or self.turb == "kkl"
+
+    @property
     def n_var(self) -> int:
-        return 7 if self.sst_family else 6 if self.turb == "sa" else 5
+        return 7 if self.two_equation else 6 if self.turb == "sa" else 5
 
     @property
     def n_grad(self) -> int:
-        """Gradient slots u, v, w, T, then k, omega (SST) or nu-tilde (SA)
-        (src/gradients.f90:236-260)."""
-        return 6 if self.sst_family else 5 if self.turb == "sa" else 4
+        """Gradient slots u, v, w, T, then k, omega (SST), k, kL (k-kL) or
+        nu-tilde (SA) (src/gradients.f90:236-260)."""
+        return 6 if self.two_equation else 5 if self.turb == "sa" else 4
 
     @property
     def ncells(self) -> int:
@@ -146,6 +153,10 @@ def freestream(case: Case, T_inf=300.0, p_inf=101325.0, mach=0.5, reynolds=1000.
         case.tw_inf = rho * case.tk_inf / (case.mu_ref * mu_ratio)
     if case.turb == "sa":
         case.tv_inf = mu_ratio * case.mu_ref / rho
+    if case.turb == "kkl":
+        # src/state.f90:101-103: both from the free-stream sound speed
+        case.tk_inf = 9 * (1e-9) * (c ** 2)
+        case.tkl_inf = 1.5589 * (1e-6) * (case.mu_ref * c) / rho
     case.tu_inf = tu_percent
     return case
 
@@ -175,7 +186,8 @@ def case_from_json(text: str) -> Case:
 def freestream_state(case: Case) -> np.ndarray:
     q = np.empty(case.cell_shape(case.n_var))
     vals = [case.rho_inf, case.u_inf, case.v_inf, case.w_inf, case.p_inf]
-    vals += [case.tv_inf] if case.turb == "sa" else [case.tk_inf, case.tw_inf]
+    vals += ([case.tv_inf] if case.turb == "sa" else [case.tk_inf, case.tkl_inf] if case.turb == "kkl"
+             else [case.tk_inf, case.tw_inf])
     for l in range(case.n_var):
         q[l] = vals[l]
     return q
@@ -296,7 +308,8 @@ def sst_bump(ni=256, nj=128, nk=128, seed=2, flux="slau", recon="weno", first=1e
     Re=1e6 per L, Tu=0.1%, mu_t/mu=0.01, 0.5% seeded noise.  Boundaries:
     subsonic inlet (-3), subsonic outlet (-4), adiabatic wall at jmin (-5),
     far field at jmax (-8), periodic in k (-9).  turb="sa" gives the same
-    channel with Spalart-Allmaras (nu-tilde = mu_ratio mu / rho)."""
+    channel with Spalart-Allmaras (nu-tilde = mu_ratio mu / rho), turb="kkl"
+    with k-kL (whose fixed omega values, never read, stay 0)."""
     c = Case(ni + 1, nj + 1, nk + 1, flux=flux, recon=recon, turb=turb, time_local=1,
              accur=1, lim=(1, 1, 1), tlim=(1, 1, 1), mu_sutherland=1,
              bc_id=(-3, -4, -5, -8, -9, -9), CFL=0.5, name=f"sst_bump_{ni}x{nj}x{nk}")
@@ -313,6 +326,7 @@ def sst_bump(ni=256, nj=128, nk=128, seed=2, flux="slau", recon="weno", first=1e
     c.fix_tw = (inf[6],) * 6
     c.fix_twall = (0.0,) * 6
     c.fix_tv = (c.tv_inf,) * 6
+    c.fix_tkl = (c.tkl_inf,) * 6
     nodes = bump_nodes(c, first=first)
     q = uniform_noise(c, freestream_state(c), seed, 0.005)
     return c, nodes, q

@@ -110,6 +110,8 @@ def block_fields(case: Case, qp: np.ndarray, mu=None, mut=None, dist=None, f1=No
            "Density": qp[0][inner], "Pressure": qp[4][inner]}
     if case.sst_family:
         out["TKE"], out["Omega"] = qp[5][inner], qp[6][inner]
+    if case.turb == "kkl":
+        out["TKE"], out["Kl"] = qp[5][inner], qp[6][inner]
     for name, a in (("Mu", mu), ("Mu_t", mut), ("Wall_distance", dist), ("F1", f1)):
         if a is not None:
             out[name] = a[inner]
@@ -315,9 +317,10 @@ class ResnormFile:
         sc[1] = rho * vm
         sc[2] = sc[3] = sc[4] = rho * vm * vm
         sc[5] = (0.5 * rho * vm ** 3 + ((gm / (gm - 1.0)) * case.p_inf))
-        if case.sst_family:
+        if case.two_equation:
+            # src/resnorm.f90:147-153
             sc[6] = rho * vm * case.tk_inf
-            sc[7] = rho * vm * case.tw_inf
+            sc[7] = rho * vm * (case.tkl_inf if case.turb == "kkl" else case.tw_inf)
         self.scale = sc
         self.restarted = previous_res is not None
         self.save = np.array(previous_res, dtype=np.float64) if previous_res is not None else np.zeros(nv + 1)

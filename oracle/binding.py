@@ -43,7 +43,8 @@ class CCase(ctypes.Structure):
                 ("fix_p", ctypes.c_double * 6), ("fix_tk", ctypes.c_double * 6),
                 ("fix_tw", ctypes.c_double * 6), ("fix_twall", ctypes.c_double * 6),
                 ("fix_tp", ctypes.c_double * 6), ("tv_inf", ctypes.c_double),
-                ("fix_tv", ctypes.c_double * 6), ("transition", ctypes.c_int32), ("tu_inf", ctypes.c_double)]
+                ("fix_tv", ctypes.c_double * 6), ("transition", ctypes.c_int32), ("tu_inf", ctypes.c_double),
+                ("tkl_inf", ctypes.c_double), ("fix_tkl", ctypes.c_double * 6)]
 
 
 def to_ccase(case) -> CCase:
@@ -62,7 +63,8 @@ def to_ccase(case) -> CCase:
     c.tv_inf = float(case.tv_inf)
     c.transition = {"none": 0, "bc": 1}[case.transition]
     c.tu_inf = float(case.tu_inf)
-    for f in ("fix_rho", "fix_u", "fix_v", "fix_w", "fix_p", "fix_tk", "fix_tw", "fix_twall", "fix_tp", "fix_tv"):
+    c.tkl_inf = float(case.tkl_inf)
+    for f in ("fix_rho", "fix_u", "fix_v", "fix_w", "fix_p", "fix_tk", "fix_tw", "fix_twall", "fix_tp", "fix_tv", "fix_tkl"):
         getattr(c, f)[:] = [float(v) for v in getattr(case, f)]
     return c
 

@@ -110,6 +110,10 @@ static inline double ipow(double x, int n) {
 }
 #define SST(s) ((s)->c.turb == ORC_TURB_SST || (s)->c.turb == ORC_TURB_SST2003)
 #define SA(s) ((s)->c.turb == ORC_TURB_SA)
+#define KKL(s) ((s)->c.turb == ORC_TURB_KKL)
+/* src/global/global_kkl.f90:5-14; sigma_k = sigma_phi = 1 */
+static const double zeta1_kkl = 1.2, zeta2_kkl = 0.97, zeta3_kkl = 0.13, cmu_kkl = 0.09, kappa_kkl = 0.41,
+                    c11_kkl = 10.0, c12_kkl = 1.3, cd1_kkl = 4.7;
 
 /* ----------------------------------------------------------- allocation */
 orc *orc_create(const orc_case *c) {
@@ -585,16 +589,24 @@ static void set_omega_at_wall(orc *s, int face) {
 }
 
 static void turb_copy(orc *s, int type, int face) {
-  if (SST(s)) { copy3(s, QPL(5), type, face); copy3(s, QPL(6), type, face); }
+  if (SST(s) || KKL(s)) { copy3(s, QPL(5), type, face); copy3(s, QPL(6), type, face); }
   if (SA(s)) copy3(s, QPL(5), type, face);
 }
-static void turb_fix(orc *s, int face) {
+/* kl_tw: the subsonic inlet fixes k-kL's kL from the omega values
+ * (src/boundary/bc_primitive.f90:333-336), which k-kL never reads from
+ * the boundary file (read_bc.f90:98-108), so they stay 0 */
+static void turb_fix_kl(orc *s, int face, int kl_tw) {
   if (SST(s)) {
     fixv(s, QPL(5), s->c.fix_tk[face], face);
     fixv(s, QPL(6), s->c.fix_tw[face], face);
   }
+  if (KKL(s)) {
+    fixv(s, QPL(5), s->c.fix_tk[face], face);
+    fixv(s, QPL(6), kl_tw ? s->c.fix_tw[face] : s->c.fix_tkl[face], face);
+  }
   if (SA(s)) fixv(s, QPL(5), s->c.fix_tv[face], face);
 }
+static void turb_fix(orc *s, int face) { turb_fix_kl(s, face, 0); }
 
 /* face-normal unit vector stored at the boundary face (outward sign applied
  * by the caller) */
@@ -756,7 +768,7 @@ static int populate_ghost_primitive(orc *s, int current_iter) {
         if (current_iter <= 2) {
           fixv(s, QPL(0), c->fix_rho[f], f); fixv(s, QPL(1), c->fix_u[f], f);
           fixv(s, QPL(2), c->fix_v[f], f); fixv(s, QPL(3), c->fix_w[f], f);
-          turb_fix(s, f);
+          turb_fix_kl(s, f, 1);
         }
         copy3(s, QPL(4), CP_FLAT, f);
         break;
@@ -771,6 +783,7 @@ static int populate_ghost_primitive(orc *s, int current_iter) {
         copy3(s, QPL(1), CP_ANTI, f); copy3(s, QPL(2), CP_ANTI, f); copy3(s, QPL(3), CP_ANTI, f);
         if (SST(s)) { copy3(s, QPL(5), CP_ANTI, f); set_omega_at_wall(s, f); }
         if (SA(s)) copy3(s, QPL(5), CP_ANTI, f); /* no_slip, bc_primitive.f90:540-541 */
+        if (KKL(s)) { copy3(s, QPL(5), CP_ANTI, f); copy3(s, QPL(6), CP_ANTI, f); } /* :548-550 */
         break;
       case -6: /* slip wall */
         copy3(s, QPL(0), CP_SYMM, f); copy3(s, QPL(4), CP_SYMM, f);
@@ -1365,12 +1378,24 @@ static int calculate_viscosity(orc *s) {
       F1[ci] = tanh(arg1 * arg1 * arg1 * arg1);
     }
   }
+  if (KKL(s)) {
+    /* src/viscosity.f90:469-483: mu_t = cmu^(1/4) rho kL / sqrt(k), zero
+     * where k or kL is below 1e-14 */
+    double cc = pow(cmu_kkl, 0.25);
+    for (int k = 0; k <= KMX; ++k) for (int j = 0; j <= JMX; ++j) for (int i = 0; i <= IMX; ++i) {
+      size_t ci = CI(s, i, j, k);
+      double density = QP(0, i, j, k), tk = QP(5, i, j, k), tkl = QP(6, i, j, k);
+      mut[ci] = cc * density * tkl / (fmx(sqrt(tk), 1.e-20));
+      if (tkl < 1.e-14 || tk < 1.e-14) mut[ci] = 0.0;
+    }
+  }
   if (c->turb != ORC_TURB_NONE) {
     for (int f = 0; f < 6; ++f) {
       int id = c->bc_id[f];
       if (id >= 0 || id == -10) continue;
       int anti = (id == -5);
       if (!(id == -5 || (id <= -1 && id >= -9))) continue;
+      if (KKL(s) && id == -7) continue; /* k-kL: -4..-1, -6, -8, -9 symm and -5 anti (:485-531) */
       int na, nb; face_range(s, f, &na, &nb);
       for (int b = 1; b <= nb; ++b) for (int a = 1; a <= na; ++a) {
         size_t cg = face_cell(s, f, 0, a, b), ci = face_cell(s, f, 1, a, b);
@@ -1485,6 +1510,7 @@ static void viscous_dir(orc *s, int d, int sst) {
           double F1 = 0.5 * (F1a[cL] + F1a[cR]);
           double sigma_kf = sigma_k1 * F1 + sigma_k2 * (1.0 - F1);
           double sigma_wf = sigma_w1 * F1 + sigma_w2 * (1.0 - F1);
+          if (sst == 3) sigma_kf = sigma_wf = 1.0; /* compute_viscous_fluxes_kkl, src/viscous.f90:450-567 */
           double rhoface = 0.5 * (QP(0, im, jm, km) + QP(0, i, j, k));
           double tkface = 0.5 * (QP(5, im, jm, km) + QP(5, i, j, k));
           double Txx = -(2.0 * rhoface * tkface / 3.0);
@@ -1510,6 +1536,9 @@ static int compute_viscous_fluxes(orc *s) {
    * (src/viscous.f90:94-97) */
   if (SA(s))
     for (int d = 0; d < 3; ++d) viscous_dir(s, d, 2);
+  /* so does the k-kL flux (src/viscous.f90:106-109) */
+  if (KKL(s))
+    for (int d = 0; d < 3; ++d) viscous_dir(s, d, 3);
   size_t nn[3] = {s->nF, s->nG, s->nH};
   double *arr[3] = {s->a[ORC_F], s->a[ORC_G], s->a[ORC_H]};
   for (int d = 0; d < 3; ++d)
@@ -1706,11 +1735,83 @@ static void add_sst_bc_source(orc *s) {
 }
 
 static void add_sst_source(orc *s);
+
+/* src/source.f90:607-832 (add_kkl_source): production limited to 20 D_k,
+ * the von Karman length from the second velocity derivatives (each a
+ * Green-Gauss sum over the six faces of the face-averaged first
+ * derivatives, the ghost-cell gradients included), its limiters, and the
+ * near-wall damping of both equations */
+static void add_kkl_source(orc *s) {
+  const double *mu = s->a[ORC_MU], *mut = s->a[ORC_MUT], *dist = s->a[ORC_DIST], *vol = s->a[ORC_VOL];
+  const double *iA = s->a[ORC_IA], *jA = s->a[ORC_JA], *kA = s->a[ORC_KA];
+  const double *inx[3] = {s->a[ORC_INX], s->a[ORC_INY], s->a[ORC_INZ]};
+  const double *jnx[3] = {s->a[ORC_JNX], s->a[ORC_JNY], s->a[ORC_JNZ]};
+  const double *knx[3] = {s->a[ORC_KNX], s->a[ORC_KNY], s->a[ORC_KNZ]};
+  double *res = s->a[ORC_RES];
+  for (int k = 1; k <= KMX - 1; ++k) for (int j = 1; j <= JMX - 1; ++j) for (int i = 1; i <= IMX - 1; ++i) {
+    size_t ci = CI(s, i, j, k), ri = RI(s, i, j, k);
+    double density = QP(0, i, j, k), tk = QP(5, i, j, k), tkl = QP(6, i, j, k);
+    /* g[v][d]: derivative of velocity v along d */
+    double g[3][3];
+    for (int v = 0; v < 3; ++v) { g[v][0] = GX(v, i, j, k); g[v][1] = GY(v, i, j, k); g[v][2] = GZ(v, i, j, k); }
+    double S[3][3];
+    for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) S[a][b] = 0.5 * (g[a][b] + g[b][a]);
+    double delv = g[0][0] + g[1][1] + g[2][2];
+    double P_k = 0.;
+    for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) {
+      double tau = mut[ci] * (2 * S[a][b]);
+      if (a == b) tau = mut[ci] * (2 * S[a][b] - (2.0 / 3.0) * delv) - (2.0 / 3.0) * density * tk;
+      P_k = P_k + tau * g[a][b];
+    }
+    double D_k = (pow(cmu_kkl, 0.75)) * density * (pow(tk, 2.5)) / fmx(tkl, 1.e-20);
+    P_k = fmn(P_k, 20 * D_k);
+    /* d2[v][d]: second derivative of velocity v along d, twice */
+    double lap[3];
+    for (int v = 0; v < 3; ++v) {
+      lap[v] = 0.;
+      for (int d = 0; d < 3; ++d) {
+#define GD(v_, d_, i_, j_, k_) ((d_) == 0 ? GX(v_, i_, j_, k_) : (d_) == 1 ? GY(v_, i_, j_, k_) : GZ(v_, i_, j_, k_))
+        double c0 = GD(v, d, i, j, k);
+        double t = (-(GD(v, d, i - 1, j, k) + c0) * inx[d][IFI(s, i, j, k)] * iA[IFI(s, i, j, k)]
+                    - (GD(v, d, i, j - 1, k) + c0) * jnx[d][JFI(s, i, j, k)] * jA[JFI(s, i, j, k)]
+                    - (GD(v, d, i, j, k - 1) + c0) * knx[d][KFI(s, i, j, k)] * kA[KFI(s, i, j, k)]
+                    + (GD(v, d, i + 1, j, k) + c0) * inx[d][IFI(s, i + 1, j, k)] * iA[IFI(s, i + 1, j, k)]
+                    + (GD(v, d, i, j + 1, k) + c0) * jnx[d][JFI(s, i, j + 1, k)] * jA[JFI(s, i, j + 1, k)]
+                    + (GD(v, d, i, j, k + 1) + c0) * knx[d][KFI(s, i, j, k + 1)] * kA[KFI(s, i, j, k + 1)]) /
+                   (2 * vol[ci]);
+#undef GD
+        lap[v] = lap[v] + t;
+      }
+    }
+    double udd = sqrt(sq(lap[0]) + sq(lap[1]) + sq(lap[2]));
+    double ss = 0.;
+    for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) ss = ss + sq(S[a][b]);
+    double ud = sqrt(2 * ss);
+    double Lvk = kappa_kkl * fabs(ud / fmx(udd, 1.e-20));
+    double fp = fmn(fmx(P_k / D_k, 0.5), 1.0);
+    Lvk = fmx(Lvk, tkl / fmx((tk * c11_kkl), 1.e-20));
+    Lvk = fmn(Lvk, c12_kkl * kappa_kkl * dist[ci] * fp);
+    double eta = density * dist[ci] * sqrt(0.3 * tk) / (20 * mu[ci]);
+    double fphi = (1 + cd1_kkl * eta) / (1 + ipow(eta, 4));
+    double cphi2 = zeta3_kkl;
+    double cphi1 = (zeta1_kkl - zeta2_kkl * (sq(tkl / fmx(tk * Lvk, 1.e-20))));
+    double P_kl = cphi1 * tkl * P_k / fmx(tk, 1.e-20);
+    double D_kl = cphi2 * density * (pow(tk, 1.5));
+    double d2 = sq(dist[ci]);
+    double S_k = P_k - D_k - 2 * mu[ci] * tk / d2;
+    double S_kl = P_kl - D_kl - 6 * mu[ci] * tkl * fphi / d2;
+    S_k = S_k * vol[ci];
+    S_kl = S_kl * vol[ci];
+    res[5 * s->nR + ri] = res[5 * s->nR + ri] - S_k;
+    res[6 * s->nR + ri] = res[6 * s->nR + ri] - S_kl;
+  }
+}
 /* add_source_term_residue, src/source.f90:94-155 */
 static void add_source_term_residue(orc *s) {
   int bc = s->c.transition == 1;
   if (SST(s)) { if (bc) add_sst_bc_source(s); else add_sst_source(s); }
   if (SA(s)) { if (bc) add_sa_bc_source(s); else add_sa_source(s); }
+  if (KKL(s)) add_kkl_source(s); /* no transition model (src/source.f90:147-150) */
 }
 
 /* src/source.f90:158-270; sst2003 limits the production to 10 D_k, floors
@@ -1903,6 +2004,17 @@ static int update_with(orc *s, double TF, double SF, int TU, int use_store) {
       double d2 = dist[ci] * dist[ci];
       R[5] = R[5] / (1. + ((-1.0 * u1[0] * cb1 * scap) + (2.0 * u1[0] * CW1_SA * fw * u1[5] / d2)) * dt[ri]);
     }
+    if (KKL(s)) {
+      /* src/update.f90:399-404: the point-implicit destruction, with
+       * sqrt(rho) in the k factor */
+      const double *mu = s->a[ORC_MU], *dist = s->a[ORC_DIST];
+      double eta = u1[0] * dist[ci] * (sqrt(0.3 * u1[5]) / (20 * mu[ci]));
+      double fphi = (1 + cd1_kkl * eta) / (1 + ipow(eta, 4));
+      double d2 = sq(dist[ci]);
+      R[5] = R[5] / (1. + ((2.5 * ((pow(cmu_kkl, 0.75)) * sqrt(u1[0]) * (pow(u1[5], 1.5)) / fmx(u1[6], 1.e-20)) +
+                            (2 * mu[ci] / d2)) * dt[ri]));
+      R[6] = R[6] / (1. + (6 * mu[ci] * fphi / d2) * dt[ri]);
+    }
     if (use_store) {
       for (int l = 0; l < nv; ++l) Rs[(size_t)l * s->nR + ri] = Rs[(size_t)l * s->nR + ri] + SF * R[l];
       if (TU) for (int l = 0; l < nv; ++l) R[l] = Rs[(size_t)l * s->nR + ri];
@@ -1914,7 +2026,7 @@ static int update_with(orc *s, double TF, double SF, int TU, int use_store) {
     for (int l = 0; l < nv; ++l) if (isnan(u2[l])) bad = 1;
     if (bad) { status = ORC_E_NONPHYSICAL; continue; }
     for (int l = 0; l < 5; ++l) QP(l, i, j, k) = u2[l];
-    if (SST(s)) {
+    if (SST(s) || KKL(s)) {
       if (u2[5] >= 0.) QP(5, i, j, k) = u2[5];
       if (u2[6] >= 0.) QP(6, i, j, k) = u2[6];
     }
@@ -2114,6 +2226,7 @@ static double sa_diagonal(const orc *s, int i, int j, int k, double D) {
 static int lusgs(orc *s) {
   const orc_case *c = &s->c;
   int nv = s->nv, sst = SST(s), sa = SA(s);
+  if (KKL(s)) return ORC_E_CONFIG; /* k-kL's LU-SGS (src/lusgs.f90:1198-1677) is not restated */
   double gm = c->gm;
   size_t ndq = (size_t)(IMX + 1) * (JMX + 1) * (KMX + 1);
   double *dqs = calloc(ndq * (size_t)nv, sizeof(double));
@@ -2272,7 +2385,7 @@ void orc_resnorm(orc *s, double *out) {
   scale[5] = (0.5 * c->rho_inf * (c->vel_mag * c->vel_mag * c->vel_mag) + ((c->gm / (c->gm - 1.)) * c->p_inf));
   /* nu-tilde for SA, k for SST (src/resnorm.f90:150-158) */
   scale[6] = c->rho_inf * c->vel_mag * (c->turb == ORC_TURB_SA ? c->tv_inf : c->tk_inf);
-  scale[7] = c->rho_inf * c->vel_mag * c->tw_inf;
+  scale[7] = c->rho_inf * c->vel_mag * (c->turb == ORC_TURB_KKL ? c->tkl_inf : c->tw_inf);
   double *R = s->a[ORC_RES];
   for (int l = 0; l < s->nv; ++l) {
     double sum = 0.;

@@ -28,7 +28,7 @@ enum { ORC_FLUX_AUSM = 0, ORC_FLUX_AUSMP = 1, ORC_FLUX_AUSMUP = 2,
        ORC_FLUX_SLAU = 3, ORC_FLUX_LDFSS0 = 4, ORC_FLUX_VANLEER = 5 };
 enum { ORC_RECON_NONE = 0, ORC_RECON_MUSCL = 1, ORC_RECON_PPM = 2,
        ORC_RECON_WENO = 3 };
-enum { ORC_TURB_NONE = 0, ORC_TURB_SST = 1, ORC_TURB_SST2003 = 2, ORC_TURB_SA = 3 };
+enum { ORC_TURB_NONE = 0, ORC_TURB_SST = 1, ORC_TURB_SST2003 = 2, ORC_TURB_SA = 3, ORC_TURB_KKL = 4 };
 enum { ORC_TA_NONE = 0, ORC_TA_RK2 = 2, ORC_TA_RK4 = 4, ORC_TA_TVDRK2 = 12, ORC_TA_TVDRK3 = 13,
        ORC_TA_IMPLICIT = 20 };
 
@@ -56,6 +56,8 @@ typedef struct {
   double fix_tv[6];  /* fixed nu-tilde of each face */
   int32_t transition; /* 0 none, 1 'bc' */
   double tu_inf;      /* free-stream turbulence intensity, percent */
+  double tkl_inf;     /* free-stream kL (k-kL) */
+  double fix_tkl[6];  /* fixed kL of each face (k-kL) */
 } orc_case;
 
 typedef struct orc orc;

@@ -46,6 +46,7 @@ module ref_harness
     real(c_double) :: tv_inf, fix_tv(6)
     integer(c_int32_t) :: transition
     real(c_double) :: tu_inf
+    real(c_double) :: tkl_inf, fix_tkl(6)
   end type hcase
 
   type(extent) :: dims
@@ -84,6 +85,8 @@ contains
       schm%turbulence = 'sst2003'
     else if (hc%turb == 3) then
       schm%turbulence = 'sa'
+    else if (hc%turb == 4) then
+      schm%turbulence = 'kkl'
     else
       schm%turbulence = 'none'
     end if
@@ -104,6 +107,7 @@ contains
     flow%y_speed_inf = hc%v_inf; flow%z_speed_inf = hc%w_inf
     flow%pressure_inf = hc%p_inf; flow%tk_inf = hc%tk_inf; flow%tw_inf = hc%tw_inf
     flow%tv_inf = hc%tv_inf
+    flow%tkl_inf = hc%tkl_inf
     flow%tu_inf = hc%tu_inf
     flow%vel_mag = hc%vel_mag; flow%MInf = hc%MInf
     flow%gm = hc%gm; flow%R_gas = hc%R_gas; flow%mu_ref = hc%mu_ref
@@ -128,6 +132,7 @@ contains
     bc%fixed_tk = hc%fix_tk; bc%fixed_tw = hc%fix_tw; bc%fixed_wall_temperature = hc%fix_twall
     bc%fixed_Tpressure = hc%fix_tp
     bc%fixed_tv = hc%fix_tv
+    bc%fixed_tkl = hc%fix_tkl
     if (allocated(bc%make_F_flux_zero)) deallocate(bc%make_F_flux_zero, bc%make_G_flux_zero, bc%make_H_flux_zero)
     allocate(bc%make_F_flux_zero(1:imx), bc%make_G_flux_zero(1:jmx), bc%make_H_flux_zero(1:kmx))
     bc%make_F_flux_zero = 1; bc%make_G_flux_zero = 1; bc%make_H_flux_zero = 1

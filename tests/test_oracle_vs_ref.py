@@ -255,6 +255,48 @@ def test_spalart_allmaras_two_dimensional():
     compare_2d(case, nodes, q, it=2, wall_dist=True, extra=("mut",))
 
 
+@pytest.mark.parametrize("flux,recon", [("slau", "weno"), ("ausm", "muscl"), ("van_leer", "none")])
+def test_k_kl(flux, recon):
+    """k-kL on the bump channel: k and kL ghosts (both anti-symmetric at the
+    wall), mu_t = cmu^(1/4) rho kL / sqrt(k), the k-kL viscous flux, and the
+    source with its von Karman length from second velocity derivatives
+    built from the neighbours' gradients (src/source.f90:607-832)."""
+    case, nodes, q = C.sst_bump(14, 10, 6, flux=flux, recon=recon, turb="kkl")
+    compare(case, nodes, q, it=3, wall_dist=True, extra=("mut",))
+
+
+@pytest.mark.parametrize("bcs", [(-3, -4, -5, -8, -9, -9), (-1, -2, -5, -6, -6, -6), (-11, -4, -5, -5, -7, -7)])
+def test_k_kl_boundaries(bcs):
+    """Iteration 1, where the inlets impose the fixed values: kL from the
+    fixed kL values except at the subsonic inlet, which takes the omega
+    values (0 under k-kL); the mu_t ghosts skip the poles."""
+    case, nodes, q = C.sst_bump(14, 10, 6, flux="ausmUP", recon="ppm", turb="kkl")
+    case = dataclasses.replace(case, bc_id=bcs, fix_tk=(1.2 * case.tk_inf,) * 6,
+                               fix_tkl=(1.3 * case.tkl_inf,) * 6, fix_tp=(1.01 * case.p_inf,) * 6)
+    compare(case, nodes, q, it=1, wall_dist=True, extra=("mut",))
+
+
+def test_k_kl_two_dimensional():
+    """One cell in k: the k-kL viscous flux still runs in k (viscous.f90:106-109)."""
+    case, nodes, q = C.sst_bump(14, 10, 1, flux="slau", recon="muscl", turb="kkl")
+    compare_2d(case, nodes, q, it=2, wall_dist=True, extra=("mut",))
+
+
+def test_k_kl_subsonic_inlet_takes_the_omega_values():
+    """Guard on the quirk above: with the subsonic inlet at imin, the kL
+    ghosts there are the (zero) omega values, not the fixed kL values."""
+    case, nodes, q = C.sst_bump(14, 10, 6, flux="ausm", recon="muscl", turb="kkl")
+    o = Oracle(case)
+    o.set_nodes(nodes)
+    o.wall_dist()
+    o.set_state(q)
+    o.temp_refresh()
+    assert o.residual(1) == 0
+    kl = o.arr("qp")[6]
+    assert np.all(kl[3:-3, 3:-3, 0:3] == 0.0)
+    assert np.all(kl[3:-3, 3:-3, -3:] > 0.0)
+
+
 @pytest.mark.parametrize("turb", ["sa", "sst", "sst2003"])
 def test_transition_bc(turb):
     """The 'bc' transition: the intermittency gamma_BC in the SA and SST
