@@ -18,6 +18,10 @@
 
 namespace fest {
 
+// the fixed values of each face: rho, u, v, w, p, tk, tw, wall
+// temperature, total pressure, nu-tilde, kL
+enum { FX_RHO = 0, FX_U, FX_V, FX_W, FX_P, FX_TK, FX_TW, FX_TWALL, FX_TP, FX_TV, FX_TKL, FX_N };
+
 struct Blk {
   int imx, jmx, kmx, nv, ng;
   int pi, pj, pk;
@@ -56,15 +60,17 @@ struct Blk {
   double iPr, itPr, cpR;  // 1/Pr, 1/Pr_t, gamma R/(gamma-1): the heat-flux factors
   double rho_inf, u_inf, v_inf, w_inf, p_inf, tk_inf, tw_inf, tv_inf;
   int trans;                  // 1: the 'bc' intermittency in the turbulence source
+  int kkl;                    // k-kL: variables 6, 7 are k and kL
+  double tkl_inf;
+  double sigma_w2;            // SST's 0.856; 1 under k-kL, whose F1 stays 0
   int fp32;                   // 1: face reconstruction and flux arithmetic in FP32
   double tu_inf, Re;          // free-stream turbulence intensity (percent), rho V / mu_ref
   double c1, c2, c3;
-  double fix[10][6];          // rho,u,v,w,p,tk,tw,wall temperature,total pressure,nu-tilde
+  double fix[FX_N][6];
   double global_dt;
   double CFLv;                // CFL of the current call
 };
 
-enum { FX_RHO = 0, FX_U, FX_V, FX_W, FX_P, FX_TK, FX_TW, FX_TWALL, FX_TP, FX_TV, FX_N };
 enum : unsigned { ERR_NONPHYS = 1u, ERR_NAN_FLUX = 2u, ERR_NAN_GRAD = 4u, ERR_NAN_MU = 8u, ERR_CONFIG = 16u };
 
 __host__ __device__ inline long ci(const Blk &b, int i, int j, int k) {

@@ -270,6 +270,8 @@ int launch_residual_front(fest_handle *h, int current_iter, int store_csnd) {
         else hipLaunchKernelGGL((k_grad<5, false, true>), g, t, 0, st, b);
       } else if (b.turb == FEST_TURB_SST2003) {
         hipLaunchKernelGGL((k_grad<6, true>), g, t, 0, st, b);
+      } else if (b.kkl) {
+        hipLaunchKernelGGL((k_grad<6, false, false, true>), g, t, 0, st, b);
       } else if (b.ng == 6) {
         hipLaunchKernelGGL((k_grad<6, false>), g, t, 0, st, b);
       } else if (b.ng == 5) {
@@ -285,7 +287,14 @@ int launch_residual_front(fest_handle *h, int current_iter, int store_csnd) {
       hipLaunchKernelGGL(k_grad_bc, gb, dim3(64, 4, 1), 0, st, b);
       bool tp = false;
       for (int f = 0; f < 6; ++f) tp |= b.bc[f] == -11;
-      if (tp && b.ng == 6) hipLaunchKernelGGL(k_mut_ghost_tp, gb, dim3(64, 4, 1), 0, st, b);
+      // k-kL's mu_t needs no gradients: the ghosts keep the value from the
+      // ghost state
+      if (tp && b.ng == 6 && !b.kkl) hipLaunchKernelGGL(k_mut_ghost_tp, gb, dim3(64, 4, 1), 0, st, b);
+    }
+    if (b.kkl) {
+      box(b.imx - 1, b.jmx - 1, b.kmx - 1, g, t, b.xcd_remap);
+      KgScope ts(h, KG_GRAD_BC);
+      hipLaunchKernelGGL(k_kkl_source, g, t, 0, st, b);
     }
     LAUNCHCHK(h);
   }
@@ -558,8 +567,12 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
   if (dims->imx < 4 || dims->jmx < 4 || (dims->kmx < 4 && dims->kmx != 2))
     return bad("i and j need at least 3 cells, k at least 3 or exactly 1");
   if (scheme->turb != FEST_TURB_NONE && scheme->turb != FEST_TURB_SST && scheme->turb != FEST_TURB_SST2003 &&
-      scheme->turb != FEST_TURB_SA)
+      scheme->turb != FEST_TURB_SA && scheme->turb != FEST_TURB_KKL)
     return bad("turbulence model not supported");
+  if (scheme->turb == FEST_TURB_KKL && scheme->time_accuracy == FEST_TA_IMPLICIT)
+    return bad("k-kL with LU-SGS (src/lusgs.f90:1198-1677) is not supported: use an RK scheme");
+  if (scheme->turb == FEST_TURB_KKL && scheme->transition != FEST_TRANS_NONE)
+    return bad("k-kL has no transition model (src/source.f90:147-150)");
   int nv_expect = scheme->turb == FEST_TURB_SA ? 6 : scheme->turb != FEST_TURB_NONE ? 7 : 5;
   if (scheme->transition != FEST_TRANS_NONE && scheme->transition != FEST_TRANS_BC)
     return bad("transition model not supported (none, bc)");
@@ -633,6 +646,9 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
   b.rho_inf = flow->density_inf; b.u_inf = flow->x_speed_inf; b.v_inf = flow->y_speed_inf;
   b.w_inf = flow->z_speed_inf; b.p_inf = flow->pressure_inf; b.tk_inf = flow->tk_inf; b.tw_inf = flow->tw_inf;
   b.tv_inf = flow->tv_inf;
+  b.kkl = scheme->turb == FEST_TURB_KKL;
+  b.sigma_w2 = b.kkl ? 1.0 : 0.856;
+  b.tkl_inf = flow->tkl_inf;
   b.trans = scheme->transition;
   b.fp32 = precision == FEST_PREC_FP32_FLUX;
   b.tu_inf = flow->tu_inf;
@@ -647,6 +663,7 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
     b.fix[FX_TW][f] = bc->fixed_tw[f]; b.fix[FX_TWALL][f] = bc->fixed_wall_temperature[f];
     b.fix[FX_TP][f] = bc->fixed_Tpressure[f];
     b.fix[FX_TV][f] = bc->fixed_tv[f];
+    b.fix[FX_TKL][f] = bc->fixed_tkl[f];
   }
   b.global_dt = scheme->global_time_step;
   b.CFLv = 1.0;

@@ -72,15 +72,18 @@ __device__ void fixcol(const Blk &b, double *v, double val, int face, int a, int
   for (int g = 0; g < 3; ++g) v[fcell(b, face, -g, a, bb)] = val;
 }
 
-// the turbulence variables: k and omega (SST), nu-tilde (SA)
+// the turbulence variables: k and omega (SST), k and kL (k-kL), nu-tilde
+// (SA).  kl_tw: the subsonic inlet fixes k-kL's kL from the omega values
+// (src/boundary/bc_primitive.f90:333-336), which k-kL never reads from the
+// boundary file (read_bc.f90:98-108)
 __device__ void turb_copycol(const Blk &b, int type, int face, int a, int bb) {
   if (b.nv >= 6) copy3col(b, b.qp + 5 * b.nc, type, face, a, bb);
   if (b.nv == 7) copy3col(b, b.qp + 6 * b.nc, type, face, a, bb);
 }
-__device__ void turb_fixcol(const Blk &b, const FaceBC &p, int face, int a, int bb) {
+__device__ void turb_fixcol(const Blk &b, const FaceBC &p, int face, int a, int bb, bool kl_tw = false) {
   if (b.nv == 7) {
     fixcol(b, b.qp + 5 * b.nc, p.fix[FX_TK], face, a, bb);
-    fixcol(b, b.qp + 6 * b.nc, p.fix[FX_TW], face, a, bb);
+    fixcol(b, b.qp + 6 * b.nc, (b.kkl && !kl_tw) ? p.fix[FX_TKL] : p.fix[FX_TW], face, a, bb);
   }
   if (b.nv == 6) fixcol(b, b.qp + 5 * b.nc, p.fix[FX_TV], face, a, bb);
 }
@@ -244,7 +247,7 @@ __global__ void k_bc_face(Blk b, int face0, int current_iter) {
       if (inner) {
         if (current_iter <= 2) {
           for (int l = 0; l < 4; ++l) fixcol(b, q + l * nc, p.fix[l], face, a, bb);
-          turb_fixcol(b, p, face, a, bb);
+          turb_fixcol(b, p, face, a, bb, true);
         }
         copy3col(b, q + 4 * nc, CP_FLAT, face, a, bb);
       }
@@ -264,7 +267,8 @@ __global__ void k_bc_face(Blk b, int face0, int current_iter) {
         copy3col(b, q + 2 * nc, CP_ANTI, face, a, bb);
         copy3col(b, q + 3 * nc, CP_ANTI, face, a, bb);
         if (b.nv >= 6) copy3col(b, q + 5 * nc, CP_ANTI, face, a, bb);  // k or nu-tilde
-        if (b.nv == 7) omega_wall_col(b, face, a, bb);
+        if (b.nv == 7 && b.kkl) copy3col(b, q + 6 * nc, CP_ANTI, face, a, bb);  // kL, :548-550
+        else if (b.nv == 7) omega_wall_col(b, face, a, bb);
       }
       break;
     case -6:  // slip wall

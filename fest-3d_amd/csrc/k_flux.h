@@ -400,8 +400,10 @@ struct FaceMath {
       T mtv = T(0.5) * (T(qp[5 * nc + cm]) + T(qp[5 * nc + c])) * rhof;
       v[NV == 6 ? 5 : 0] = (A * ((mu_f + mtv) * (g0 * nx + g1 * ny + g2 * nz))) * T(1.5);
     }
-    if (NV == 7 && !laminar_only) {
-      // SST k / omega diffusion and the -2/3 rho k stress, src/viscous.f90:328-447
+    if (NV == 7 && (!laminar_only || b.kkl)) {
+      // SST k / omega diffusion and the -2/3 rho k stress, src/viscous.f90:328-447;
+      // k-kL's has sigma_k = sigma_phi = 1 and runs on the k faces of a 2-D
+      // grid too (:450-567, :106-109)
       T gt[2][3];
 #pragma unroll
       for (int l = 0; l < 2; ++l) {
@@ -417,9 +419,10 @@ struct FaceMath {
         gt[l][1] = gt[l][1] + ncmp * dely;
         gt[l][2] = gt[l][2] + ncmp * delz;
       }
+      // k-kL leaves F1 at 0 and sets sigma_w2 to 1, so both blends are 1
       T F1 = T(0.5) * (T(b.f1[cm]) + T(b.f1[c]));
       T sigma_kf = T(0.85) * F1 + T(1.0) * (T(1.0) - F1);
-      T sigma_wf = T(0.5) * F1 + T(0.856) * (T(1.0) - F1);
+      T sigma_wf = T(0.5) * F1 + T(b.sigma_w2) * (T(1.0) - F1);
       T rhof = T(0.5) * (T(qp[cm]) + T(qp[c]));
       T tkf = T(0.5) * (T(qp[5 * nc + cm]) + T(qp[5 * nc + c]));
       T Tkk = -(T(2.0) * rhof * tkf * (T(1.0) / T(3.0)));

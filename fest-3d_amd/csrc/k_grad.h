@@ -184,7 +184,7 @@ __device__ __forceinline__ void sst_source(double rho, double tk, double tw, dou
 // source.  Interior gradients are final here: the ghost-gradient rule only
 // writes ghost cells.  NG is the number of gradient slots (4 laminar, 6
 // SST); a compile-time count keeps all 3*NG gradients in registers.
-template <int NG, bool S2003, bool BC = false>
+template <int NG, bool S2003, bool BC = false, bool KKL = false>
 __global__ void __launch_bounds__(256) k_grad(Blk b) {
   int i, j;
   if (!flat_ij(b.imx + 1, b.jmx + 1, i, j, plane_block(b.xcd_remap))) return;
@@ -249,6 +249,15 @@ __global__ void __launch_bounds__(256) k_grad(Blk b) {
   } else {
     mu = b.mu[c];
   }
+  if constexpr (KKL) {
+    // k-kL mu_t (src/viscosity.f90:469-483); its source reads neighbouring
+    // gradients and runs after the ghost-cell gradients (k_kkl_source)
+    const double tk = b.qp[5 * b.nc + c], tkl = b.qp[6 * b.nc + c];
+    double mut = pow(0.09, 0.25) * rho * tkl / (fmx(sqrt(tk), 1.e-20));
+    if (tkl < 1.e-14 || tk < 1.e-14) mut = 0.0;
+    b.mut[c] = mut;
+    return;
+  }
   if constexpr (NG == 6) {
   double tk = b.qp[5 * b.nc + c], tw = b.qp[6 * b.nc + c];
   double mut, F1;
@@ -303,6 +312,7 @@ __global__ void __launch_bounds__(256) k_grad(Blk b) {
 __device__ __forceinline__ void visc_ghost(const Blk &b, int face, int a, int bb) {
   int id = b.bc[face];
   if (!(id == -5 || (id <= -1 && id >= -9))) return;
+  if (b.kkl && id == -7) return;  // k-kL: -4..-1, -6, -8, -9 and -5 (src/viscosity.f90:485-531)
   long cg = fcell(b, face, 0, a, bb), cn = fcell(b, face, 1, a, bb);
   b.mut[cg] = (id == -5) ? -b.mut[cn] : b.mut[cn];
   if (b.ng == 6) b.f1[cg] = b.f1[cn];
@@ -386,6 +396,90 @@ __global__ void k_mut_ghost_tp(Blk b) {
   else sst_mut_f1<false>(rho, tk, tw, b.dist[cG], b.mu[cG], gg, mut, F1);
   b.mut[cG] = mut;
   b.f1[cG] = F1;
+}
+
+// k-kL source (src/source.f90:607-832), after the ghost-cell gradients:
+// production from the full stress with -2/3 rho k, limited to 20 D_k; the
+// von Karman length from the Laplacian of the velocity, each second
+// derivative a Green-Gauss sum over the six faces of the face-averaged
+// first derivatives (ghost cells' included); its limiters, and the
+// near-wall damping of both equations.  Writes S_k, S_kL (times the volume)
+// where the SST source goes.
+__global__ void __launch_bounds__(256) k_kkl_source(Blk b) {
+  int i, j;
+  if (!flat_ij(b.imx - 1, b.jmx - 1, i, j, plane_block(b.xcd_remap))) return;
+  i += 1;
+  j += 1;
+  const int k = blockIdx.y + 1;
+  if (k > b.kmx - 1) return;
+  const long c = ci(b, i, j, k), r = ri(b, i, j, k), gi = gri(b, i, j, k), nc = b.nc, ngr = b.ngr;
+  const double rho = b.qp[c], tk = b.qp[5 * nc + c], tkl = b.qp[6 * nc + c];
+  const double mut = b.mut[c], mu = b.mu[c], dist = b.dist[c], vol = b.vol[c];
+  double g[3][3];  // g[v][d]: derivative of velocity v along d
+#pragma unroll
+  for (int v = 0; v < 3; ++v)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) g[v][d] = b.g[d][(long)v * ngr + gi];
+  double S[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int e = 0; e < 3; ++e) S[a][e] = 0.5 * (g[a][e] + g[e][a]);
+  const double delv = g[0][0] + g[1][1] + g[2][2];
+  double P_k = 0.;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      double tau = mut * (2 * S[a][e]);
+      if (a == e) tau = mut * (2 * S[a][e] - (2.0 / 3.0) * delv) - (2.0 / 3.0) * rho * tk;
+      P_k = P_k + tau * g[a][e];
+    }
+  const double D_k = (pow(0.09, 0.75)) * rho * (pow(tk, 2.5)) / fmx(tkl, 1.e-20);
+  P_k = fmn(P_k, 20 * D_k);
+  const long f0 = fi(b, 0, i, j, k), f1 = fi(b, 0, i + 1, j, k), g0 = fi(b, 1, i, j, k), g1 = fi(b, 1, i, j + 1, k);
+  const long h0 = fi(b, 2, i, j, k), h1 = fi(b, 2, i, j, k + 1);
+  const long nb[6] = {gri(b, i - 1, j, k), gri(b, i, j - 1, k), gri(b, i, j, k - 1),
+                      gri(b, i + 1, j, k), gri(b, i, j + 1, k), gri(b, i, j, k + 1)};
+  double lap[3];
+#pragma unroll
+  for (int v = 0; v < 3; ++v) {
+    lap[v] = 0.;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double *G = b.g[d] + (long)v * ngr;
+      const double *const *nn = d == 0 ? b.fx : d == 1 ? b.fy : b.fz;
+      const double c0 = G[gi];
+      double t = (-(G[nb[0]] + c0) * nn[0][f0] * b.fA[0][f0] - (G[nb[1]] + c0) * nn[1][g0] * b.fA[1][g0] -
+                  (G[nb[2]] + c0) * nn[2][h0] * b.fA[2][h0] + (G[nb[3]] + c0) * nn[0][f1] * b.fA[0][f1] +
+                  (G[nb[4]] + c0) * nn[1][g1] * b.fA[1][g1] + (G[nb[5]] + c0) * nn[2][h1] * b.fA[2][h1]) /
+                 (2 * vol);
+      lap[v] = lap[v] + t;
+    }
+  }
+  const double udd = sqrt(sqd(lap[0]) + sqd(lap[1]) + sqd(lap[2]));
+  double ss = 0.;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int e = 0; e < 3; ++e) ss = ss + sqd(S[a][e]);
+  const double ud = sqrt(2 * ss);
+  const double kappa = 0.41;
+  double Lvk = kappa * fabs(ud / fmx(udd, 1.e-20));
+  const double fp = fmn(fmx(P_k / D_k, 0.5), 1.0);
+  Lvk = fmx(Lvk, tkl / fmx((tk * 10.0), 1.e-20));
+  Lvk = fmn(Lvk, 1.3 * kappa * dist * fp);
+  const double eta = rho * dist * sqrt(0.3 * tk) / (20 * mu);
+  const double e2 = eta * eta;
+  const double fphi = (1 + 4.7 * eta) / (1 + (e2 * eta) * eta);
+  const double cphi1 = (1.2 - 0.97 * (sqd(tkl / fmx(tk * Lvk, 1.e-20))));
+  const double P_kl = cphi1 * tkl * P_k / fmx(tk, 1.e-20);
+  const double D_kl = 0.13 * rho * (pow(tk, 1.5));
+  const double d2 = sqd(dist);
+  const double S_k = P_k - D_k - 2 * mu * tk / d2;
+  const double S_kl = P_kl - D_kl - 6 * mu * tkl * fphi / d2;
+  b.src[r] = S_k * vol;
+  b.src[b.nres + r] = S_kl * vol;
 }
 
 }  // namespace fest

@@ -187,7 +187,17 @@ __global__ void __launch_bounds__(256) k_update(Blk b, double TF, double SF, int
   u1[4] = (u1[4] / (gm - 1.) + 0.5 * (u1[1] * u1[1] + u1[2] * u1[2] + u1[3] * u1[3])) / u1[0] + 0.0;
   double dt = b.dt[r];
   if (sa) R[5] = R[5] / sa_implicit(b, c, gri(b, i, j, k), u1[0], u1[5], dt);
-  if (sst) {
+  if (sst && b.kkl) {
+    // k-kL point-implicit destruction, sqrt(rho) in the k factor
+    // (src/update.f90:399-404)
+    const double mu = b.mu[c], dist = b.dist[c];
+    const double eta = u1[0] * dist * (sqrt(0.3 * u1[5]) / (20 * mu));
+    const double fphi = (1 + 4.7 * eta) / (1 + ((eta * eta) * eta) * eta);
+    const double d2 = sqd(dist);
+    R[5] = R[5] / (1. + ((2.5 * ((pow(0.09, 0.75)) * sqrt(u1[0]) * (pow(u1[5], 1.5)) / fmx(u1[6], 1.e-20)) +
+                          (2 * mu / d2)) * dt));
+    R[6] = R[6] / (1. + (6 * mu * fphi / d2) * dt);
+  } else if (sst) {
     double F1 = b.f1[c];
     double beta = 0.075 * F1 + (1. - F1) * 0.0828;
     double w7 = b.qp[6 * nc + c];
@@ -266,7 +276,7 @@ __global__ void k_norm_final(Blk b, int nblk) {
   scale[2] = scale[3] = scale[4] = b.rho_inf * v * v;
   scale[5] = (0.5 * b.rho_inf * (v * v * v) + ((b.gm / (b.gm - 1.)) * b.p_inf));
   scale[6] = b.rho_inf * v * (b.ng == 5 ? b.tv_inf : b.tk_inf);  // nu-tilde for SA (resnorm.f90:158)
-  scale[7] = b.rho_inf * v * b.tw_inf;
+  scale[7] = b.rho_inf * v * (b.kkl ? b.tkl_inf : b.tw_inf);  // kL for k-kL (resnorm.f90:153)
   b.norms[0] = fabs(acc[0] / scale[0]);
   for (int l = 1; l <= b.nv; ++l) b.norms[l] = sqrt(acc[l] / (scale[l] * scale[l]));
   // the block's share before the cross-block assembly (src/resnorm.f90:

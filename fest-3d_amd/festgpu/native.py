@@ -48,13 +48,14 @@ class Flow(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
                 ("density_inf", "x_speed_inf", "y_speed_inf", "z_speed_inf", "pressure_inf", "tk_inf", "tw_inf",
                  "vel_mag", "MInf", "gm", "R_gas", "mu_ref", "T_ref", "Sutherland_temp", "Pr", "tPr")] + \
-               [("mu_sutherland", ctypes.c_int32), ("tv_inf", ctypes.c_double), ("tu_inf", ctypes.c_double)]
+               [("mu_sutherland", ctypes.c_int32), ("tv_inf", ctypes.c_double), ("tu_inf", ctypes.c_double),
+                ("tkl_inf", ctypes.c_double)]
 
 
 class BC(ctypes.Structure):
     _fields_ = [("id", ctypes.c_int32 * 6)] + [(n, ctypes.c_double * 6) for n in
                 ("fixed_density", "fixed_x_speed", "fixed_y_speed", "fixed_z_speed", "fixed_pressure",
-                 "fixed_tk", "fixed_tw", "fixed_wall_temperature", "fixed_Tpressure", "fixed_tv")]
+                 "fixed_tk", "fixed_tw", "fixed_wall_temperature", "fixed_Tpressure", "fixed_tv", "fixed_tkl")]
 
 
 _lib = None
@@ -125,6 +126,7 @@ def c_structs(case: Case):
     f.Pr, f.tPr, f.mu_sutherland = case.Pr, case.tPr, case.mu_sutherland
     f.tv_inf = case.tv_inf
     f.tu_inf = case.tu_inf
+    f.tkl_inf = case.tkl_inf
     b = BC()
     b.id[:] = list(case.bc_id)
     b.fixed_density[:] = list(case.fix_rho)
@@ -137,6 +139,7 @@ def c_structs(case: Case):
     b.fixed_wall_temperature[:] = list(case.fix_twall)
     b.fixed_Tpressure[:] = list(case.fix_tp)
     b.fixed_tv[:] = list(case.fix_tv)
+    b.fixed_tkl[:] = list(case.fix_tkl)
     return e, s, f, b
 
 

@@ -25,7 +25,7 @@ module fest_gpu
   integer(c_int32_t), parameter, public :: FEST_TRANS_NONE = 0, FEST_TRANS_BC = 1
   integer(c_int32_t), parameter, public :: FEST_PREC_FP64 = 0, FEST_PREC_FP32_FLUX = 1
   integer(c_int32_t), parameter, public :: FEST_TURB_NONE = 0, FEST_TURB_SST = 1, FEST_TURB_SST2003 = 2, &
-                                           FEST_TURB_SA = 3
+                                           FEST_TURB_SA = 3, FEST_TURB_KKL = 4
   integer(c_int32_t), parameter, public :: FEST_TA_NONE = 0, FEST_TA_RK2 = 2, FEST_TA_RK4 = 4, &
       FEST_TA_TVDRK2 = 12, FEST_TA_TVDRK3 = 13, FEST_TA_IMPLICIT = 20
 
@@ -54,6 +54,7 @@ module fest_gpu
     real(c_double) :: gm, R_gas, mu_ref, T_ref, Sutherland_temp, Pr, tPr
     integer(c_int32_t) :: mu_sutherland
     real(c_double) :: tv_inf, tu_inf
+    real(c_double) :: tkl_inf
   end type fest_flow
 
   !< hot-path fields of type(boundarytype), src/vartypes.f90:293-365
@@ -63,6 +64,7 @@ module fest_gpu
     real(c_double) :: fixed_pressure(6), fixed_tk(6), fixed_tw(6), fixed_wall_temperature(6)
     real(c_double) :: fixed_Tpressure(6)
     real(c_double) :: fixed_tv(6)
+    real(c_double) :: fixed_tkl(6)
   end type fest_bc
 
   !< one interface link of a multiblock grid (the reference's map line,

@@ -92,6 +92,7 @@ contains
       case ("sst");  s%turb = FEST_TURB_SST
       case ("sst2003");  s%turb = FEST_TURB_SST2003
       case ("sa");  s%turb = FEST_TURB_SA
+      case ("kkl");  s%turb = FEST_TURB_KKL
       case default
         print *, "update: turbulence model '", trim(scheme%turbulence), "' is not on the device path"
         stop 1
@@ -136,6 +137,7 @@ contains
     f%mu_sutherland = merge(1, 0, trim(flow%mu_variation) == 'sutherland_law')
     f%tv_inf = flow%tv_inf
     f%tu_inf = flow%tu_inf
+    f%tkl_inf = flow%tkl_inf
 
     b%id = bc%id
     b%fixed_density = bc%fixed_density
@@ -148,6 +150,7 @@ contains
     b%fixed_wall_temperature = bc%fixed_wall_temperature
     b%fixed_Tpressure = bc%fixed_Tpressure
     b%fixed_tv = bc%fixed_tv
+    b%fixed_tkl = bc%fixed_tkl
   end subroutine describe
 
   !< first call: the device block and its geometry, copied out of the

@@ -48,7 +48,8 @@ enum { FEST_FLUX_AUSM = 0, FEST_FLUX_AUSMP = 1, FEST_FLUX_AUSMUP = 2,
 enum { FEST_RECON_NONE = 0, FEST_RECON_MUSCL = 1, FEST_RECON_PPM = 2,
        FEST_RECON_WENO = 3 };
 enum { FEST_TURB_NONE = 0, FEST_TURB_SST = 1, FEST_TURB_SST2003 = 2 /* src/viscosity.f90:215, source.f90:205 */,
-       FEST_TURB_SA = 3 /* 'sa', src/viscosity.f90:149, source.f90:835, lusgs.f90:1680 */ };
+       FEST_TURB_SA = 3 /* 'sa', src/viscosity.f90:149, source.f90:835, lusgs.f90:1680 */,
+       FEST_TURB_KKL = 4 /* 'kkl', src/viscosity.f90:469, source.f90:607, viscous.f90:450; RK only */ };
 enum { FEST_TRANS_NONE = 0, FEST_TRANS_BC = 1 /* the 'bc' intermittency of the SA and SST sources */ };
 enum { FEST_TA_NONE = 0, FEST_TA_RK2 = 2, FEST_TA_RK4 = 4, FEST_TA_TVDRK2 = 12, FEST_TA_TVDRK3 = 13,
        FEST_TA_IMPLICIT = 20 /* LU-SGS, src/lusgs.f90 */ };
@@ -78,6 +79,7 @@ typedef struct {
   int32_t mu_sutherland;   /* mu_variation: 0 'constant', 1 'sutherland_law' */
   double tv_inf;           /* Spalart-Allmaras free-stream nu-tilde (state.f90:106) */
   double tu_inf;           /* free-stream turbulence intensity, percent (vartypes.f90) */
+  double tkl_inf;          /* k-kL free-stream kL (state.f90:103) */
 } fest_flow;
 
 /* the hot-path fields of type(boundarytype), src/vartypes.f90:293-365 */
@@ -87,6 +89,7 @@ typedef struct {
   double fixed_pressure[6], fixed_tk[6], fixed_tw[6], fixed_wall_temperature[6];
   double fixed_Tpressure[6];  /* total pressure of a -11 face (read_bc.f90:79-80) */
   double fixed_tv[6];         /* fixed nu-tilde of each face (read_bc.f90:106-113) */
+  double fixed_tkl[6];        /* fixed kL of each face (read_bc.f90:103-106, 198-202) */
 } fest_bc;
 
 typedef struct fest_handle fest_handle;

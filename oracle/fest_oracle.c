@@ -13,6 +13,8 @@
  * against closed forms for the parts the reference build cannot run
  * (src/time.f90 and src/update.f90 use MPI and do not link here): the RK4
  * amplification polynomial, the uniform-grid time step and the norms.
+ * Parity unpinned: the point-implicit turbulence factors of the RK update
+ * (SA, src/update.f90:405-420; k-kL, :399-404), which only update.f90 holds.
  *
  * Semantics choices that the reference leaves to the compiler:
  *   - MAX/MIN with a NaN argument follow amdflang's lowering, `a>b ? a : b`

@@ -34,9 +34,9 @@ def test_struct_sizes_match_header():
     # fest_extent is four int32; fest_scheme 12 int32, one double, the transition int32 (padded)
     assert ctypes.sizeof(native.Extent) == 16
     assert ctypes.sizeof(native.Scheme) == 64  # + transition and padding
-    assert ctypes.sizeof(native.BC) == 24 + 8 * 60
-    # 16 doubles, mu_sutherland (int32, then 4 bytes of padding), tv_inf
-    assert ctypes.sizeof(native.Flow) == 8 * 16 + 8 + 8 + 8
+    assert ctypes.sizeof(native.BC) == 24 + 8 * 66
+    # 16 doubles, mu_sutherland (int32, then 4 bytes of padding), tv_inf, tu_inf, tkl_inf
+    assert ctypes.sizeof(native.Flow) == 8 * 16 + 8 + 8 + 8 + 8
 
 
 @pytest.mark.parametrize("change,code", [
@@ -45,6 +45,8 @@ def test_struct_sizes_match_header():
     (dict(bc_id=(-9, -9, -5, -5, -7, -9), kmx=2), -3),  # a k pole on a one-cell k extent
     (dict(time_accuracy="RK2", bc_id=(-9, -9, -5, -5, -9, -9), imx=3), -3),
     (dict(kmx=3), -3),  # two cells in k: neither 3-D nor the reference's 2-D
+    (dict(turb="kkl", time_accuracy="implicit", bc_id=(-9, -9, -5, -5, -9, -9)), -3),  # k-kL has RK only
+    (dict(turb="kkl", transition="bc", bc_id=(-9, -9, -5, -5, -9, -9)), -3),
 ])
 def test_config_errors(change, code):
     case, _, _ = C.laminar_box(8, 6, 5)

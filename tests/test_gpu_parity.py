@@ -405,6 +405,64 @@ def test_spalart_allmaras_steps(rule, bcs):
     np.testing.assert_allclose(g.resnorm(), o.resnorm(), rtol=1e-10)
 
 
+@pytest.mark.parametrize("flux,recon", [("slau", "weno"), ("ausm", "muscl")])
+def test_k_kl_residual(flux, recon):
+    """k-kL residual: k and kL ghosts, mu_t = cmu^(1/4) rho kL / sqrt(k), the
+    k-kL viscous flux and its source (second velocity derivatives from the
+    neighbours' gradients), against the oracle (pinned bit for bit to the
+    reference build)."""
+    case, nodes, q = C.sst_bump(20, 12, 8, flux=flux, recon=recon, turb="kkl")
+    say("residual kkl", flux, recon)
+    g, o = pair(case, nodes, q, wall=True)
+    o.temp_refresh()
+    assert o.residual(3) == 0
+    g.residual(3)
+    out = g.download("qp", "F", "G", "H", "residue", "mu", "mut", "grads")
+    assert relerr(out["qp"], o.arr("qp")) <= TOL
+    for n, on in (("F", "F"), ("G", "G"), ("H", "H"), ("residue", "res"), ("mu", "mu")):
+        assert relerr(out[n], o.arr(on)) <= TOL, n
+    for d, n in enumerate(("gx", "gy", "gz")):
+        assert relerr(out["grads"][d], o.arr(n)) <= TOL, n
+    inner = (slice(2, -2),) * 3
+    assert relerr(out["mut"][inner], o.arr("mut")[inner]) <= TOL
+
+
+@pytest.mark.parametrize("bcs", [(-3, -4, -5, -8, -9, -9), (-1, -2, -5, -6, -6, -6), (-11, -4, -5, -5, -7, -7)])
+def test_k_kl_steps(bcs):
+    """Two RK4 steps (the point-implicit k-kL destruction) from iteration 1,
+    where the inlets fix k and kL (kL from the omega values, 0, at the
+    subsonic inlet)."""
+    case, nodes, q = C.sst_bump(20, 12, 6, turb="kkl")
+    case = dataclasses.replace(case, bc_id=bcs, fix_tk=(1.2 * case.tk_inf,) * 6,
+                               fix_tkl=(1.3 * case.tkl_inf,) * 6, fix_tp=(1.01 * case.p_inf,) * 6)
+    say("kkl steps", bcs)
+    g, o = pair(case, nodes, q, wall=True)
+    for it in (1, 2):
+        assert o.step(it) == 0
+        g.get_next_solution(it)
+    out = g.download("qp", "residue")
+    assert relerr(out["qp"][:, 3:-3, 3:-3, 3:-3], o.arr("qp")[:, 3:-3, 3:-3, 3:-3]) <= TOL
+    assert relerr(out["residue"], o.arr("res")) <= 1e-10
+    np.testing.assert_allclose(g.resnorm(), o.resnorm(), rtol=1e-10)
+
+
+def test_k_kl_two_dimensional():
+    """One cell in k with slip walls in k: the k-kL viscous flux runs on the
+    k faces too; two RK4 steps."""
+    case, nodes, q = C.sst_bump(20, 12, 1, turb="kkl")
+    case = dataclasses.replace(case, bc_id=case.bc_id[:4] + (-6, -6))
+    say("2-D kkl")
+    g, o = pair(case, nodes, q, wall=True)
+    for it in range(1, 3):
+        assert o.step(it) == 0
+        g.get_next_solution(it)
+    out = g.download("qp", "residue", "H")
+    assert relerr(out["qp"][:, 3:-3, 3:-3, 3:-3], o.arr("qp")[:, 3:-3, 3:-3, 3:-3]) <= TOL
+    assert relerr(out["residue"], o.arr("res")) <= 1e-10
+    assert np.abs(out["H"] - o.arr("H")).max() <= TOL * np.abs(o.arr("F")).max()
+    assert np.abs(o.arr("H")[5:7]).max() > 0.0
+
+
 @pytest.mark.parametrize("turb", ["sa", "sst", "sst2003"])
 @pytest.mark.parametrize("rule", ["RK4", "implicit"])
 def test_transition_bc(turb, rule):
