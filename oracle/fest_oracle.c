@@ -2068,9 +2068,15 @@ static void lu_flux(const orc *s, int nv, const double *ql, const double *qr, co
   if (nv == 7) {
     W[5] = U[5] / U[0];
     W[6] = U[6] / U[0];
-    /* a negative k or omega falls back to the neighbour's old value */
-    W[5] = W[5] + 0.5 * (1. - copysign(1., W[5])) * (ql[5] - W[5]);
-    W[6] = W[6] + 0.5 * (1. - copysign(1., W[6])) * (ql[6] - W[6]);
+    if (KKL(s)) {
+      /* KKLFlux floors k and kL (src/lusgs.f90:1605-1606) */
+      W[5] = fmx(W[5], 1e-8);
+      W[6] = fmx(W[6], 1e-8);
+    } else {
+      /* a negative k or omega falls back to the neighbour's old value */
+      W[5] = W[5] + 0.5 * (1. - copysign(1., W[5])) * (ql[5] - W[5]);
+      W[6] = W[6] + 0.5 * (1. - copysign(1., W[6])) * (ql[6] - W[6]);
+    }
   }
   if (nv == 6) W[5] = fmx(U[5] / U[0], 1e-8); /* SAFlux, src/lusgs.f90:2145-2146 */
   double nx = f->nx, ny = f->ny, nz = f->nz, Area = f->A, Volume = f->vol, mmu = f->mm, tmu = f->tm;
@@ -2115,6 +2121,7 @@ static void lu_flux(const orc *s, int nv, const double *ql, const double *qr, co
     double dtwdx = (P[6] - W[6]) * nx * Area / Volume, dtwdy = (P[6] - W[6]) * ny * Area / Volume,
            dtwdz = (P[6] - W[6]) * nz * Area / Volume;
     double sk = sigma_k1 * F1 + sigma_k2 * (1.0 - F1), sw = sigma_w1 * F1 + sigma_w2 * (1.0 - F1);
+    if (KKL(s)) sk = sw = 1.0; /* sigma_k, sigma_phi (src/lusgs.f90:1671-1672) */
     Fx[5] = Fx[5] + (mmu + sk * tmu) * (dtkdx * nx + dtkdy * ny + dtkdz * nz);
     Fx[6] = Fx[6] + (mmu + sw * tmu) * (dtwdx * nx + dtwdy * ny + dtwdz * nz);
   }
@@ -2227,8 +2234,7 @@ static double sa_diagonal(const orc *s, int i, int j, int k, double D) {
  * conservative update of the interior cells. */
 static int lusgs(orc *s) {
   const orc_case *c = &s->c;
-  int nv = s->nv, sst = SST(s), sa = SA(s);
-  if (KKL(s)) return ORC_E_CONFIG; /* k-kL's LU-SGS (src/lusgs.f90:1198-1677) is not restated */
+  int nv = s->nv, sst = SST(s), sa = SA(s), kkl = KKL(s);
   double gm = c->gm;
   size_t ndq = (size_t)(IMX + 1) * (JMX + 1) * (KMX + 1);
   double *dqs = calloc(ndq * (size_t)nv, sizeof(double));
@@ -2272,6 +2278,15 @@ static int lusgs(orc *s) {
             double d0 = sa_diagonal(s, i, j, k, D[0]);
             for (int l = 0; l < nv; ++l) D[l] = d0;
           }
+          if (kkl) {
+            /* src/lusgs.f90:1359-1361 and 1468-1470 */
+            const double *mu = s->a[ORC_MU], *dist = s->a[ORC_DIST];
+            double d2 = dist[c0] * dist[c0];
+            D[5] = D[5] + (2.5 * (pow(cmu_kkl, 0.75)) * QP(0, i, j, k) * (pow(QP(5, i, j, k), 1.5)) * vol[c0] /
+                           QP(6, i, j, k));
+            D[5] = D[5] + (2 * mu[c0] * vol[c0] / d2);
+            D[6] = D[6] + (6 * mu[c0] * vol[c0] / d2);
+          }
           for (int l = 0; l < nv; ++l) {
             double dI = New[0][l] - Old[0][l], dJ = New[1][l] - Old[1][l], dK = New[2][l] - Old[2][l];
             if (!pass) {
@@ -2297,7 +2312,7 @@ static int lusgs(orc *s) {
     U[4] = (QP(4, i, j, k) / (gm - 1.0)) +
            (0.5 * QP(0, i, j, k) * (QP(1, i, j, k) * QP(1, i, j, k) + QP(2, i, j, k) * QP(2, i, j, k) +
                                     QP(3, i, j, k) * QP(3, i, j, k)));
-    if (sst) { U[5] = QP(0, i, j, k) * QP(5, i, j, k); U[6] = QP(0, i, j, k) * QP(6, i, j, k); }
+    if (sst || kkl) { U[5] = QP(0, i, j, k) * QP(5, i, j, k); U[6] = QP(0, i, j, k) * QP(6, i, j, k); }
     if (sa) U[5] = QP(0, i, j, k) * QP(5, i, j, k);
     for (int l = 0; l < nv; ++l) U[l] = U[l] + DQ(l, i, j, k);
     QP(0, i, j, k) = U[0];
@@ -2305,9 +2320,13 @@ static int lusgs(orc *s) {
     QP(2, i, j, k) = U[2] / U[0];
     QP(3, i, j, k) = U[3] / U[0];
     QP(4, i, j, k) = (gm - 1.0) * (U[4] - (0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / U[0]));
-    if (sst) {
+    if (sst && !KKL(s)) {
       if (U[5] > 0) QP(5, i, j, k) = U[5] / U[0];
       if (U[6] > 0) QP(6, i, j, k) = U[6] / U[0];
+    }
+    if (KKL(s)) { /* src/lusgs.f90:1502-1505 */
+      QP(5, i, j, k) = fmx(U[5] / U[0], 1.e-8);
+      QP(6, i, j, k) = fmx(U[6] / U[0], 1.e-8);
     }
     if (sa) QP(5, i, j, k) = fmx(U[5] / U[0], 1.e-8);
   }

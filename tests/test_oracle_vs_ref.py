@@ -260,9 +260,12 @@ def test_k_kl(flux, recon):
     """k-kL on the bump channel: k and kL ghosts (both anti-symmetric at the
     wall), mu_t = cmu^(1/4) rho kL / sqrt(k), the k-kL viscous flux, and the
     source with its von Karman length from second velocity derivatives
-    built from the neighbours' gradients (src/source.f90:607-832)."""
+    built from the neighbours' gradients (src/source.f90:607-832); and the
+    k-kL LU-SGS (its flux floors, sigma 1, the source terms on the
+    diagonal, src/lusgs.f90:1198-1677)."""
     case, nodes, q = C.sst_bump(14, 10, 6, flux=flux, recon=recon, turb="kkl")
     compare(case, nodes, q, it=3, wall_dist=True, extra=("mut",))
+    compare_lusgs(case, nodes, q, it=2, wall_dist=True)
 
 
 @pytest.mark.parametrize("bcs", [(-3, -4, -5, -8, -9, -9), (-1, -2, -5, -6, -6, -6), (-11, -4, -5, -5, -7, -7)])
@@ -274,6 +277,7 @@ def test_k_kl_boundaries(bcs):
     case = dataclasses.replace(case, bc_id=bcs, fix_tk=(1.2 * case.tk_inf,) * 6,
                                fix_tkl=(1.3 * case.tkl_inf,) * 6, fix_tp=(1.01 * case.p_inf,) * 6)
     compare(case, nodes, q, it=1, wall_dist=True, extra=("mut",))
+    compare_lusgs(case, nodes, q, it=1, wall_dist=True)
 
 
 def test_k_kl_two_dimensional():
