@@ -569,8 +569,6 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
   if (scheme->turb != FEST_TURB_NONE && scheme->turb != FEST_TURB_SST && scheme->turb != FEST_TURB_SST2003 &&
       scheme->turb != FEST_TURB_SA && scheme->turb != FEST_TURB_KKL)
     return bad("turbulence model not supported");
-  if (scheme->turb == FEST_TURB_KKL && scheme->time_accuracy == FEST_TA_IMPLICIT)
-    return bad("k-kL with LU-SGS (src/lusgs.f90:1198-1677) is not supported: use an RK scheme");
   if (scheme->turb == FEST_TURB_KKL && scheme->transition != FEST_TRANS_NONE)
     return bad("k-kL has no transition model (src/source.f90:147-150)");
   int nv_expect = scheme->turb == FEST_TURB_SA ? 6 : scheme->turb != FEST_TURB_NONE ? 7 : 5;

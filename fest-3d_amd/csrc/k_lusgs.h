@@ -54,9 +54,15 @@ __device__ __forceinline__ void lu_flux(const Blk &b, const double (&ql)[NV], co
   if constexpr (NV == 7) {
     W[5] = U[5] / U[0];
     W[6] = U[6] / U[0];
-    // a negative k or omega falls back to the neighbour's old value
-    W[5] = W[5] + 0.5 * (1. - copysign(1., W[5])) * (ql[5] - W[5]);
-    W[6] = W[6] + 0.5 * (1. - copysign(1., W[6])) * (ql[6] - W[6]);
+    if (b.kkl) {
+      // KKLFlux floors k and kL (src/lusgs.f90:1605-1606)
+      W[5] = fmx(W[5], 1e-8);
+      W[6] = fmx(W[6], 1e-8);
+    } else {
+      // a negative k or omega falls back to the neighbour's old value
+      W[5] = W[5] + 0.5 * (1. - copysign(1., W[5])) * (ql[5] - W[5]);
+      W[6] = W[6] + 0.5 * (1. - copysign(1., W[6])) * (ql[6] - W[6]);
+    }
   }
   if constexpr (NV == 6) W[5] = fmx(U[5] / U[0], 1e-8);
   const double *P = qr;
@@ -104,7 +110,8 @@ __device__ __forceinline__ void lu_flux(const Blk &b, const double (&ql)[NV], co
                  dtkdz = (P[5] - W[5]) * nz * Area / Volume;
     const double dtwdx = (P[6] - W[6]) * nx * Area / Volume, dtwdy = (P[6] - W[6]) * ny * Area / Volume,
                  dtwdz = (P[6] - W[6]) * nz * Area / Volume;
-    const double sk = 0.85 * F1 + 1.0 * (1.0 - F1), sw = 0.5 * F1 + 0.856 * (1.0 - F1);
+    // k-kL: F1 stays 0 and sigma_w2 is 1, so sk = sw = 1 (src/lusgs.f90:1671-1672)
+    const double sk = 0.85 * F1 + 1.0 * (1.0 - F1), sw = 0.5 * F1 + b.sigma_w2 * (1.0 - F1);
     Fx[5] = Fx[5] + (mmu + sk * tmu) * (dtkdx * nx + dtkdy * ny + dtkdz * nz);
     Fx[6] = Fx[6] + (mmu + sw * tmu) * (dtwdx * nx + dtwdy * ny + dtwdz * nz);
   }
@@ -347,10 +354,18 @@ __global__ void __launch_bounds__(256) k_lusgs_prep(Blk b) {
     dg = (vol / b.dt[ri(b, i, j, k)]) + 0.5 * sumL;
     if constexpr (NV == 6) dg = sa_diagonal(b, c0, gri(b, i, j, k), Q0[0], Q0[5], vol, dg);
     if constexpr (NV == 7) {
-      const double F1 = b.f1[c0], w = Q0[6];
-      const double beta = F1 * 0.075 + (1.0 - F1) * 0.0828;
-      d5 = (dg + (0.09 * w) * vol);
-      d6 = (dg + 2.0 * beta * w * vol);
+      if (b.kkl) {
+        // the k-kL source terms (src/lusgs.f90:1359-1361)
+        const double mu = b.mu[c0], d2 = b.dist[c0] * b.dist[c0];
+        d5 = dg + (2.5 * (pow(0.09, 0.75)) * Q0[0] * (pow(Q0[5], 1.5)) * vol / Q0[6]);
+        d5 = d5 + (2 * mu * vol / d2);
+        d6 = dg + (6 * mu * vol / d2);
+      } else {
+        const double F1 = b.f1[c0], w = Q0[6];
+        const double beta = F1 * 0.075 + (1.0 - F1) * 0.0828;
+        d5 = (dg + (0.09 * w) * vol);
+        d6 = (dg + 2.0 * beta * w * vol);
+      }
     }
   }
   tile[0][jj][ii] = dg;
@@ -482,8 +497,13 @@ __global__ void __launch_bounds__(256) k_lusgs_apply(Blk b) {
   b.qp[3 * nc + c] = U[3] / U[0];
   b.qp[4 * nc + c] = (gm - 1.0) * (U[4] - (0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / U[0]));
   if constexpr (NV == 7) {
-    if (U[5] > 0) b.qp[5 * nc + c] = U[5] / U[0];
-    if (U[6] > 0) b.qp[6 * nc + c] = U[6] / U[0];
+    if (b.kkl) {  // src/lusgs.f90:1502-1505
+      b.qp[5 * nc + c] = fmx(U[5] / U[0], 1.e-8);
+      b.qp[6 * nc + c] = fmx(U[6] / U[0], 1.e-8);
+    } else {
+      if (U[5] > 0) b.qp[5 * nc + c] = U[5] / U[0];
+      if (U[6] > 0) b.qp[6 * nc + c] = U[6] / U[0];
+    }
   }
   if constexpr (NV == 6) b.qp[5 * nc + c] = fmx(U[5] / U[0], 1.e-8);  // src/lusgs.f90:2096-2097
 }

@@ -49,7 +49,7 @@ enum { FEST_RECON_NONE = 0, FEST_RECON_MUSCL = 1, FEST_RECON_PPM = 2,
        FEST_RECON_WENO = 3 };
 enum { FEST_TURB_NONE = 0, FEST_TURB_SST = 1, FEST_TURB_SST2003 = 2 /* src/viscosity.f90:215, source.f90:205 */,
        FEST_TURB_SA = 3 /* 'sa', src/viscosity.f90:149, source.f90:835, lusgs.f90:1680 */,
-       FEST_TURB_KKL = 4 /* 'kkl', src/viscosity.f90:469, source.f90:607, viscous.f90:450; RK only */ };
+       FEST_TURB_KKL = 4 /* 'kkl', src/viscosity.f90:469, source.f90:607, viscous.f90:450, lusgs.f90:1198 */ };
 enum { FEST_TRANS_NONE = 0, FEST_TRANS_BC = 1 /* the 'bc' intermittency of the SA and SST sources */ };
 enum { FEST_TA_NONE = 0, FEST_TA_RK2 = 2, FEST_TA_RK4 = 4, FEST_TA_TVDRK2 = 12, FEST_TA_TVDRK3 = 13,
        FEST_TA_IMPLICIT = 20 /* LU-SGS, src/lusgs.f90 */ };

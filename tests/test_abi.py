@@ -45,7 +45,6 @@ def test_struct_sizes_match_header():
     (dict(bc_id=(-9, -9, -5, -5, -7, -9), kmx=2), -3),  # a k pole on a one-cell k extent
     (dict(time_accuracy="RK2", bc_id=(-9, -9, -5, -5, -9, -9), imx=3), -3),
     (dict(kmx=3), -3),  # two cells in k: neither 3-D nor the reference's 2-D
-    (dict(turb="kkl", time_accuracy="implicit", bc_id=(-9, -9, -5, -5, -9, -9)), -3),  # k-kL has RK only
     (dict(turb="kkl", transition="bc", bc_id=(-9, -9, -5, -5, -9, -9)), -3),
 ])
 def test_config_errors(change, code):

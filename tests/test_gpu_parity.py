@@ -427,15 +427,17 @@ def test_k_kl_residual(flux, recon):
     assert relerr(out["mut"][inner], o.arr("mut")[inner]) <= TOL
 
 
+@pytest.mark.parametrize("rule", ["RK4", "implicit"])
 @pytest.mark.parametrize("bcs", [(-3, -4, -5, -8, -9, -9), (-1, -2, -5, -6, -6, -6), (-11, -4, -5, -5, -7, -7)])
-def test_k_kl_steps(bcs):
-    """Two RK4 steps (the point-implicit k-kL destruction) from iteration 1,
-    where the inlets fix k and kL (kL from the omega values, 0, at the
+def test_k_kl_steps(rule, bcs):
+    """Two RK4 steps (the point-implicit k-kL destruction) or two LU-SGS
+    steps (its flux floors and source terms on the diagonal) from iteration
+    1, where the inlets fix k and kL (kL from the omega values, 0, at the
     subsonic inlet)."""
     case, nodes, q = C.sst_bump(20, 12, 6, turb="kkl")
-    case = dataclasses.replace(case, bc_id=bcs, fix_tk=(1.2 * case.tk_inf,) * 6,
+    case = dataclasses.replace(case, time_accuracy=rule, bc_id=bcs, fix_tk=(1.2 * case.tk_inf,) * 6,
                                fix_tkl=(1.3 * case.tkl_inf,) * 6, fix_tp=(1.01 * case.p_inf,) * 6)
-    say("kkl steps", bcs)
+    say("kkl steps", rule, bcs)
     g, o = pair(case, nodes, q, wall=True)
     for it in (1, 2):
         assert o.step(it) == 0
