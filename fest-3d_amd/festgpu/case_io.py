@@ -91,16 +91,21 @@ def read_mapping(path: str) -> List[Link]:
 
 
 def read_bc(path: str, case: Case) -> Case:
-    """Fixed values of the six faces (read_bc.f90:16-190)."""
+    """Fixed values of the six faces (read_bc.f90:16-190).  Under k-kL the
+    omega values are neither filled nor read (read_bc.f90:98-108, 198-202):
+    they stay 0."""
     vals = {k: list(getattr(case, k)) for k in ("fix_rho", "fix_u", "fix_v", "fix_w", "fix_p", "fix_tk",
-                                                  "fix_tw", "fix_twall", "fix_tp")}
+                                                  "fix_tw", "fix_twall", "fix_tp", "fix_tv", "fix_tkl")}
     inf = dict(fix_rho=case.rho_inf, fix_u=case.u_inf, fix_v=case.v_inf, fix_w=case.w_inf, fix_p=case.p_inf,
-               fix_tk=case.tk_inf, fix_tw=case.tw_inf, fix_twall=0.0, fix_tp=0.0)
+               fix_tk=case.tk_inf, fix_tw=0.0 if case.turb == "kkl" else case.tw_inf, fix_twall=0.0, fix_tp=0.0,
+               fix_tv=case.tv_inf, fix_tkl=case.tkl_inf)
     for k in vals:  # fill_fixed_values: the free stream everywhere
         vals[k] = [inf[k]] * 6
     names = {"FIX_DENSITY": "fix_rho", "FIX_X_SPEED": "fix_u", "FIX_Y_SPEED": "fix_v", "FIX_Z_SPEED": "fix_w",
              "FIX_PRESSURE": "fix_p", "WALL_TEMPERATURE": "fix_twall", "FIX_tk": "fix_tk", "FIX_tw": "fix_tw",
-             "TOTAL_PRESSURE": "fix_tp"}
+             "TOTAL_PRESSURE": "fix_tp", "FIX_tv": "fix_tv", "FIX_tkl": "fix_tkl"}
+    if case.turb == "kkl":
+        del names["FIX_tw"]
     face = -1
     with open(path) as f:
         for line in f:
@@ -153,6 +158,12 @@ def read_case(root: str):
         ratio = float(_first(flow, "Free Stream Viscosity Ratio"))
         tk = 1.5 * ((vel * tu) ** 2)
         base = dataclasses.replace(base, tk_inf=tk, tw_inf=rho * tk / (mu_ref * ratio))
+    if turb == "sa":  # src/state.f90:105-106
+        ratio = float(_first(flow, "Free Stream Viscosity Ratio"))
+        base = dataclasses.replace(base, tv_inf=ratio * mu_ref / rho)
+    if turb == "kkl":  # src/state.f90:101-103, from the free-stream sound speed
+        a = math.sqrt(gm * p / rho)
+        base = dataclasses.replace(base, tk_inf=9 * (1e-9) * (a ** 2), tkl_inf=1.5589 * (1e-6) * (mu_ref * a) / rho)
     mesh = os.path.join(sysd, "mesh")
     blocks = []
     for b, (gfile, bcfile, ids) in enumerate(read_layout(os.path.join(mesh, "layout", "layout.md"))):
