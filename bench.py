@@ -54,22 +54,58 @@ def measured_traffic(case, group):
     return None
 
 
-def cpu_baseline(case_fn, full_dist, budget_s=25.0, implicit=False):
-    """The oracle (a single-threaded C restatement of the reference) timed on
-    the host on a bounded sample: RK4 steps of the first 64 i-cells of the
-    same grid, wall distance taken from the device.  Checker code, used here
-    only as the baseline."""
+def host_info():
+    """Host core count (nproc / os.cpu_count) and CPU model (lscpu)."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        import subprocess
+        info["nproc"] = int(subprocess.run(["nproc"], capture_output=True, text=True).stdout.strip())
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True).stdout.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                info[k.strip()] = v.strip()
+    except (OSError, ValueError):
+        pass
+    return info
+
+
+def cpu_baseline(case_fn, ni, nj, nk, full_dist, budget_s=20.0, implicit=False):
+    """The oracle (the C restatement of the reference, checker code, used
+    here only as the baseline) timed on the host's cores on bounded samples
+    of the same workload, in the two modes of the reference's own CPU runs:
+
+      multi-core: the grid cut along i into one block per core, one thread
+        per block, the 3-layer interface exchange at every stage (the
+        reference's MPI decomposition, one rank per block);
+      single-core: the first 64 i-cells of the grid as one block on one core
+        (what the reference does with a single-block case).
+
+    Beside them, the reference's own compiled modules (oracle/_ref, built
+    from its sources here) and the oracle each time one residual evaluation
+    of the single-core sample, giving the speed ratio of the restatement to
+    the reference.  Wall distance comes from the device run."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from binding import Oracle
     import dataclasses
-    case, nodes, q = case_fn(64, 128, 128)
+    import numpy as np
+    from binding import Oracle, group_step_mt, time_reference_residual
+    from festgpu import multiblock as M
+    host = host_info()
+    per, unit, what = (1, "cell-implicit-steps/s", "implicit (LU-SGS)") if implicit else \
+        (4, "cell-RK-stages/s", "RK4")
+    out = {"host": host}
+    # single core, one block
+    nis = min(64, ni)
+    case, nodes, q = case_fn(nis, nj, nk)
     if implicit:
         case = dataclasses.replace(case, time_accuracy="implicit")
     o = Oracle(case)
     o.set_nodes(nodes)
     d = o.arr("dist")
-    d[...] = full_dist[:, :, : d.shape[2]]
+    if full_dist is not None:
+        d[...] = full_dist[:, :, : d.shape[2]]
     o.set_state(q)
+    o.temp_refresh()
+    temp0, dist0 = o.arr("temp").copy(), o.arr("dist").copy()
     t0 = time.perf_counter()
     steps = 0
     while True:
@@ -77,12 +113,71 @@ def cpu_baseline(case_fn, full_dist, budget_s=25.0, implicit=False):
         steps += 1
         if time.perf_counter() - t0 > budget_s or steps >= 4:
             break
-    dt = time.perf_counter() - t0
-    per, unit, what = (1, "cell-implicit-steps/s", "implicit (LU-SGS)") if implicit else \
-        (4, "cell-RK-stages/s", "RK4")
-    return {"value": case.ncells * per * steps / dt, "unit": unit, "cores": 1, "kind": "port",
-            "sample": f"{steps} {what} step(s) of the 64x128x128-cell slab of the same workload "
-                      f"({case.ncells} cells), oracle/fest_oracle.c single thread, {dt:.1f} s"}
+    dt1 = time.perf_counter() - t0
+    single = {"value": case.ncells * per * steps / dt1, "unit": unit, "cores": 1, "kind": "port",
+              "sample": f"{steps} {what} step(s) of the {nis}x{nj}x{nk}-cell slab of the same workload, "
+                        f"one block, oracle/fest_oracle.c on one thread, {dt1:.1f} s"}
+    # the reference's own modules against the oracle, one residual each
+    ref = None
+    try:
+        o2 = Oracle(case)
+        o2.set_nodes(nodes)
+        o2.arr("dist")[...] = dist0
+        o2.set_state(q)
+        o2.temp_refresh()
+        t0 = time.perf_counter()
+        nrep = 0
+        while nrep < 3 and (nrep == 0 or time.perf_counter() - t0 < 8.0):
+            assert o2.residual(1) == 0
+            nrep += 1
+        t_orc = (time.perf_counter() - t0) / nrep
+        r = time_reference_residual(case, o2.arr("nodes").copy(), q, temp0, dist0, reps=3, budget=8.0)
+        if r is not None:
+            ref = {"reference_s_per_residual": r["seconds"], "oracle_s_per_residual": t_orc,
+                   "oracle_over_reference_speed": r["seconds"] / t_orc,
+                   "reference_cell_residuals_per_s": case.ncells / r["seconds"],
+                   "sample": f"one residual evaluation of the {nis}x{nj}x{nk}-cell slab on one core: the "
+                             "reference's own modules (amdflang -O2, ref_stage 5) against the oracle"}
+    except (RuntimeError, OSError) as e:
+        ref = {"error": str(e)[:200]}
+    del o
+    # multi-core: one block per core
+    if implicit:
+        out.update(single)
+        out["single_core"] = single
+        out["reference_modules"] = ref
+        return out
+    ncores = max(1, min(16, os.cpu_count() or 1, (ni - 1) // 4))
+    case, nodes, q = case_fn(ni, nj, nk)
+    step_i = ni // ncores
+    cuts = [1 + step_i * b for b in range(1, ncores)]
+    blocks, links = M.split_i(case, nodes, q, cuts)
+    os_ = []
+    for b, (cb, nb_, qb) in enumerate(blocks):
+        ob = Oracle(cb)
+        ob.set_nodes(nb_)
+        if full_dist is not None:
+            # the block's cells -2..imx_b+2 are the grid's cells s-3..; padded
+            # index = cell index + 2
+            s0 = ([1] + cuts)[b]
+            ob.arr("dist")[...] = full_dist[:, :, s0 - 1: s0 - 1 + cb.imx + 5]
+        ob.set_state(qb)
+        os_.append(ob)
+    t0 = time.perf_counter()
+    steps = 0
+    while True:
+        assert group_step_mt(os_, links, steps + 1, ncores) == 0
+        steps += 1
+        if time.perf_counter() - t0 > budget_s or steps >= 3:
+            break
+    dtm = time.perf_counter() - t0
+    out.update({"value": case.ncells * per * steps / dtm, "unit": unit, "cores": ncores, "kind": "port",
+                "sample": f"{steps} {what} step(s) of the whole {ni}x{nj}x{nk}-cell workload cut along i into "
+                          f"{ncores} blocks, one thread per block, interface exchange every stage; "
+                          f"oracle/fest_oracle.c, {dtm:.1f} s"})
+    out["single_core"] = single
+    out["reference_modules"] = ref
+    return out
 
 
 def fp32_flux_run(case, nodes, q, g64, warmup, steps, implicit, breakdown64):
@@ -168,7 +263,10 @@ def main():
     g.timing(dominant)
     t0 = time.perf_counter()
     for _ in range(args.steps):
+        # the reference's iteration: get_next_solution, then find_resnorm
+        # (solver.f90:184-185)
         g.get_next_solution(it)
+        g.resnorm()
         it += 1
     g.sync()
     el = time.perf_counter() - t0
@@ -205,7 +303,7 @@ def main():
         out["fp32_flux"] = fp32_flux_run(case, nodes, q, g, args.warmup, args.steps, implicit, breakdown)
     if not args.no_cpu_baseline:
         dist = g.download("dist") if case.sst_family else None
-        out["cpu_baseline"] = cpu_baseline(case_fn, dist, implicit=implicit)
+        out["cpu_baseline"] = cpu_baseline(case_fn, ni, nj, nk, dist, implicit=implicit)
     print(json.dumps(out))
 
 

@@ -53,6 +53,8 @@ struct Blk {
   int flux_scheme, recon, turb, ppm_flag, time_local, time_acc, accur, mu_suth;
   int xcd_remap;  // XCD-aware block order of the sweep kernels (plane_block)
   int lim[3], tlim[3];
+  int pbs[3];                 // pressure-based switch per direction (MUSCL, PPM)
+  int valb;                   // 1: van Albada MUSCL limiter
   int bc[6];
   int mz[3][2];               // make_*_flux_zero at the first and last face
   // flow

@@ -194,7 +194,8 @@ void launch_flux_t(const Blk &b, dim3 g, dim3 t, hipStream_t st, int store_csnd)
     case 0: hipLaunchKernelGGL((k_flux<D, NV, 0, FLAT, T>), g, t, 0, st, b, store_csnd); break;
     case 1: hipLaunchKernelGGL((k_flux<D, NV, 1, FLAT, T>), g, t, 0, st, b, store_csnd); break;
     case 2: hipLaunchKernelGGL((k_flux<D, NV, 2, FLAT, T>), g, t, 0, st, b, store_csnd); break;
-    default: hipLaunchKernelGGL((k_flux<D, NV, 3, FLAT, T>), g, t, 0, st, b, store_csnd); break;
+    case 3: hipLaunchKernelGGL((k_flux<D, NV, 3, FLAT, T>), g, t, 0, st, b, store_csnd); break;
+    default: hipLaunchKernelGGL((k_flux<D, NV, 4, FLAT, T>), g, t, 0, st, b, store_csnd); break;
   }
 }
 // FP64 face arithmetic, or FP32 when the handle was created with
@@ -578,7 +579,11 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
     return bad("the bc transition needs a turbulence model and mu_ref != 0");
   if (dims->n_var != nv_expect) return bad("n_var does not match the turbulence model");
   if (scheme->flux < 0 || scheme->flux > 5) return bad("unknown flux scheme");
-  if (scheme->recon < 0 || scheme->recon > 3) return bad("unknown reconstruction");
+  if (scheme->recon < 0 || scheme->recon > 4) return bad("unknown reconstruction");
+  if (scheme->muscl_limiter != FEST_LIM_KOREN && scheme->muscl_limiter != FEST_LIM_VAN_ALBADA)
+    return bad("unknown MUSCL limiter");
+  for (int d = 0; d < 3; ++d)
+    if (scheme->pbs[d] != 0 && scheme->pbs[d] != 1) return bad("pressure-based switch must be 0 or 1");
   if (scheme->time_accuracy != FEST_TA_RK4 && scheme->time_accuracy != FEST_TA_RK2 &&
       scheme->time_accuracy != FEST_TA_NONE && scheme->time_accuracy != FEST_TA_TVDRK2 &&
       scheme->time_accuracy != FEST_TA_TVDRK3 && scheme->time_accuracy != FEST_TA_IMPLICIT)
@@ -619,8 +624,14 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
   b.flux_scheme = scheme->flux; b.recon = scheme->recon; b.turb = scheme->turb;
   b.time_local = scheme->time_local; b.time_acc = scheme->time_accuracy; b.accur = scheme->accur;
   for (int d = 0; d < 3; ++d) { b.lim[d] = scheme->lim[d]; b.tlim[d] = scheme->tlim[d]; }
+  // the switch acts under MUSCL and PPM only (compute_face_interpolant,
+  // src/face/state/face_interpolant.f90:91-104)
+  for (int d = 0; d < 3; ++d)
+    b.pbs[d] = (scheme->recon == FEST_RECON_MUSCL || scheme->recon == FEST_RECON_PPM) ? scheme->pbs[d] : 0;
+  b.valb = scheme->muscl_limiter == FEST_LIM_VAN_ALBADA;
   // src/boundary/boundary_state_reconstruction.f90:45-47
-  b.ppm_flag = (scheme->recon == FEST_RECON_PPM || scheme->recon == FEST_RECON_WENO) ? 1 : 0;
+  b.ppm_flag = (scheme->recon == FEST_RECON_PPM || scheme->recon == FEST_RECON_WENO ||
+                scheme->recon == FEST_RECON_WENO_NM) ? 1 : 0;
   for (int f = 0; f < 6; ++f)
     if (bc->id[f] == -7) b.ppm_flag = 1;
   {

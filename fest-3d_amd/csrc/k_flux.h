@@ -36,14 +36,16 @@ template <typename T>
 struct FaceMath {
   // ---- reconstruction of one cell from its neighbours um (below) and up
   // (above); upv is the cell's upper-face value, lo its lower-face value
-  static __device__ __forceinline__ void muscl_cell(T um, T u0, T up, int sw, T &upv, T &lo) {
+  // valb: van Albada (r^2+r)/(r^2+1), the limiter muscl.f90:180,186 leaves
+  // commented out, in place of the Koren-alpha form
+  static __device__ __forceinline__ void muscl_cell(T um, T u0, T up, int sw, T &upv, T &lo, bool valb = false) {
     const T alpha = T(2.) / T(3.), phi = T(1.0), kappa = T(1.) / T(3.), eps = T(1e-14);
     T fd = up - u0;
     T bd = u0 - um;
     T r = fd / (bd + fsgn(eps, bd));
-    T psi1 = fmx(T(0.), fmn(fmn(2 * r, alpha * (r - T(1.0)) + T(1.0)), T(2.)));
+    T psi1 = valb ? ((r * r) + r) / ((r * r) + T(1.0)) : fmx(T(0.), fmn(fmn(2 * r, alpha * (r - T(1.0)) + T(1.0)), T(2.)));
     r = bd / (fd + fsgn(eps, fd));
-    T psi2 = fmx(T(0.), fmn(fmn(2 * r, alpha * (r - T(1.0)) + T(1.0)), T(2.)));
+    T psi2 = valb ? ((r * r) + r) / ((r * r) + T(1.0)) : fmx(T(0.), fmn(fmn(2 * r, alpha * (r - T(1.0)) + T(1.0)), T(2.)));
     psi1 = (1 - (1 - psi1) * sw);
     psi2 = (1 - (1 - psi2) * sw);
     upv = u0 + T(0.25) * phi * (((T(1.) - kappa) * psi1 * bd) + ((T(1.) + kappa) * psi2 * fd));
@@ -103,6 +105,38 @@ struct FaceMath {
                         T(-1.0) * Up1 + T(5.0) * U0 + T(2.0) * Um1, T(2.0) * U0 + T(5.0) * Um1 - T(1.0) * Um2);
   }
 
+  // WENO for non-uniform grids (weno_NM.f90:59-121): a21, a10, a01, a12
+  // are the volume blends V(m)/(V(m-1)+V(m)) of the cell's stencil, m = -1..2
+  // relative to the cell.  Polynomials and indicators as the reference, the
+  // weights combined with one division (weno_combine; 2Q/6 = P/3 exactly).
+  static __device__ __forceinline__ T weno_nm(T Um2, T Um1, T U0, T Up1, T Up2, T a21, T a10, T a01, T a12,
+                                              bool upper) {
+    const T eps = T(1e-6);
+    T U01 = (T(1.0) - a01) * U0 + a01 * Up1;
+    T U12 = (T(1.0) - a12) * Up1 + a12 * Up2;
+    T U10 = (T(1.0) - a10) * Um1 + a10 * U0;
+    T U21 = (T(1.0) - a21) * Um2 + a21 * Um1;
+    T Q1, Q2, Q3, B1, B2, B3;
+    if (upper) {
+      T U00 = Um1 + (T(1.0) - a21) * (Um1 - Um2);
+      Q1 = T(6.0) * U0 - T(1.0) * U10 - T(2.0) * U00;
+      Q2 = T(-1.0) * U10 + T(2.0) * U0 + T(2.0) * U01;
+      Q3 = T(2.0) * U01 + T(2.0) * Up1 - T(1.0) * U12;
+      B1 = (T(13.0) / T(12.0)) * sqd(T(2) * U10 - T(2.0) * U00) + (T(1.0) / T(4.0)) * sqd(T(4) * U0 - T(2.0) * U10 - T(2.0) * U00);
+      B2 = (T(13.0) / T(12.0)) * sqd(T(2) * U10 - T(4.0) * U0 + T(2) * U01) + (T(1.0) / T(4.0)) * sqd(T(-2) * U10 + T(2.0) * U01);
+      B3 = (T(13.0) / T(12.0)) * sqd(T(2) * U01 - T(4.0) * Up1 + T(2) * U12) + (T(1.0) / T(4.0)) * sqd(T(-6) * U01 + T(8.0) * Up1 - T(2.0) * U12);
+    } else {
+      T U11 = Up1 + a12 * (Up1 - Up2);
+      Q1 = T(6.0) * U0 - T(1.0) * U01 - T(2.0) * U11;
+      Q2 = T(-1.0) * U01 + T(2.0) * U0 + T(2.0) * U10;
+      Q3 = T(2.0) * U10 + T(2.0) * Um1 - T(1.0) * U21;
+      B1 = (T(13.0) / T(12.0)) * sqd(T(2) * U01 - T(2.0) * U11) + (T(1.0) / T(4.0)) * sqd(T(4) * U0 - T(2.0) * U01 - T(2.0) * U11);
+      B2 = (T(13.0) / T(12.0)) * sqd(T(2) * U01 - T(4.0) * U0 + T(2) * U10) + (T(1.0) / T(4.0)) * sqd(T(-2) * U01 + T(2.0) * U10);
+      B3 = (T(13.0) / T(12.0)) * sqd(T(2) * U10 - T(4.0) * Um1 + T(2) * U21) + (T(1.0) / T(4.0)) * sqd(T(-6) * U10 + T(8.0) * Um1 - T(2.0) * U21);
+    }
+    return weno_combine(sqd(eps + B1), sqd(eps + B2), sqd(eps + B3), T(2) * Q1, T(2) * Q2, T(2) * Q3);
+  }
+
   // PPM face estimate at the face between cells m-1 and m
   static __device__ __forceinline__ T ppm_face(T qm2, T qm1, T qm, T qp1) {
     return (T(7.) * (qm + qm1) - (qp1 + qm2)) / T(12.);
@@ -125,9 +159,36 @@ struct FaceMath {
   // is cell n-3+m: the reconstructions read cells n-1..n (first order),
   // n-2..n+1 (MUSCL) or n-3..n+2 (PPM, WENO5), all inside the padded array
   // for faces 1..N.
+  // per-face values shared by the variables: the pressure-based switch
+  // factors of the two cells (muscl.f90:37-112, ppm.f90:109-186) and the
+  // WENO-NM volume blends A(m) = V(m)/(V(m-1)+V(m)), m = n-2..n+2
+  struct Aux {
+    bool pb;
+    T pdL, pdR;
+    T A[5];
+  };
+  // pdif of the cells n-1 and n from the pressure stencil p(m) = cell
+  // n-3+m; the ghost cells 0 and N take the first and last interior
+  // cell's value
+  static __device__ __forceinline__ void pb_factors(const Blk &b, int n, int N, const double *p, long s, Aux &ax) {
+    auto w = [p, s](int m) { return T(p[(long)(m - 3) * s]); };
+    const T pin = T(b.p_inf);
+    T pd2 = (n == 1) ? tabs(w(4) - w(2)) : tabs(w(3) - w(1));
+    ax.pdL = T(1) - (pd2 / (pd2 + pin));
+    pd2 = (n == N) ? tabs(w(3) - w(1)) : tabs(w(4) - w(2));
+    ax.pdR = T(1) - (pd2 / (pd2 + pin));
+  }
+  static __device__ __forceinline__ void nm_blends(const double *vol, long s, Aux &ax) {
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+      const T v1 = T(vol[(long)(m - 2) * s]), v0 = T(vol[(long)(m - 3) * s]);
+      ax.A[m] = v1 / (v0 + v1);
+    }
+  }
+
   template <int RECON>
   static __device__ __forceinline__ void face_states(const Blk &b, int d, int n, int N, int l, const double *u,
-                                                     long s, T &L, T &R) {
+                                                     long s, const Aux &ax, T &L, T &R) {
     constexpr int recon = RECON;
     // cell n-3+m, read where it is used
     auto w = [u, s](int m) { return T(u[(long)(m - 3) * s]); };
@@ -150,13 +211,18 @@ struct FaceMath {
       if constexpr (recon == 0) {
         L = w(2);
       } else if constexpr (recon == 1) {
-        muscl_cell(w(1), w(2), w(3), sw, L, dummy);
+        muscl_cell(w(1), w(2), w(3), sw, L, dummy, b.valb);
       } else if constexpr (recon == 2) {
         T Lc = ppm_face(w(1), w(2), w(3), w(4)), Rc = ppm_face(w(0), w(1), w(2), w(3));
         if (b.lim[d] == 1) ppm_extrema(w(2), Lc, Rc);
         L = Lc;
-      } else {
+      } else if constexpr (recon == 3) {
         L = weno_up(w(0), w(1), w(2), w(3), w(4));
+      } else {
+        L = weno_nm(w(0), w(1), w(2), w(3), w(4), ax.A[0], ax.A[1], ax.A[2], ax.A[3], true);
+      }
+      if constexpr (recon == 1 || recon == 2) {
+        if (ax.pb) L = w(2) + (ax.pdL * (L - w(2)));
       }
     }
     // right state: lower face of cell n
@@ -170,13 +236,18 @@ struct FaceMath {
       if constexpr (recon == 0) {
         R = w(3);
       } else if constexpr (recon == 1) {
-        muscl_cell(w(2), w(3), w(4), sw, dummy, R);
+        muscl_cell(w(2), w(3), w(4), sw, dummy, R, b.valb);
       } else if constexpr (recon == 2) {
         T Lc = ppm_face(w(2), w(3), w(4), w(5)), Rc = ppm_face(w(1), w(2), w(3), w(4));
         if (b.lim[d] == 1) ppm_extrema(w(3), Lc, Rc);
         R = Rc;
-      } else {
+      } else if constexpr (recon == 3) {
         R = weno_lo(w(1), w(2), w(3), w(4), w(5));
+      } else {
+        R = weno_nm(w(1), w(2), w(3), w(4), w(5), ax.A[1], ax.A[2], ax.A[3], ax.A[4], false);
+      }
+      if constexpr (recon == 1 || recon == 2) {
+        if (ax.pb) R = w(3) - (ax.pdR * (w(3) - R));
       }
     }
   }
@@ -510,13 +581,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   const T nx = T(b.fx[D][fo]), ny = T(b.fy[D][fo]), nz = T(b.fz[D][fo]), A = T(b.fA[D][fo]);
   {
     T qL[NV], qR[NV];
+    typename M::Aux ax;
+    ax.pb = (RECON == 1 || RECON == 2) && b.pbs[D];
+    if (ax.pb) {
+      if constexpr (D == 0) M::pb_factors(b, n, N, lds_row + 4 * (FLUX_I_SPAN + 5), 1, ax);
+      else M::pb_factors(b, n, N, b.qp + 4 * b.nc + c, s, ax);
+    }
+    if constexpr (RECON == 4) M::nm_blends(b.vol + c, s, ax);
 #pragma unroll
     for (int l = 0; l < nv; ++l) {
       if constexpr (D == 0) {
-        M::template face_states<RECON>(b, D, n, N, l, lds_row + l * (FLUX_I_SPAN + 5), 1, qL[l], qR[l]);
+        M::template face_states<RECON>(b, D, n, N, l, lds_row + l * (FLUX_I_SPAN + 5), 1, ax, qL[l], qR[l]);
       } else {
         const double *u = b.qp + (long)l * b.nc + c;
-        M::template face_states<RECON>(b, D, n, N, l, u, s, qL[l], qR[l]);
+        M::template face_states<RECON>(b, D, n, N, l, u, s, ax, qL[l], qR[l]);
       }
     }
     if (store_csnd) csf = T(0.5) * (tsqrt(T(b.gm) * qL[4] / qL[0]) + tsqrt(T(b.gm) * qR[4] / qR[0]));

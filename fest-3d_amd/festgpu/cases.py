@@ -26,7 +26,8 @@ import numpy as np
 
 # flux / reconstruction / turbulence ids shared with the C-ABI
 FLUX = {"ausm": 0, "ausmP": 1, "ausmUP": 2, "slau": 3, "ldfss0": 4, "van_leer": 5}
-RECON = {"none": 0, "muscl": 1, "ppm": 2, "weno": 3}
+RECON = {"none": 0, "muscl": 1, "ppm": 2, "weno": 3, "weno_NM": 4}
+LIMITER = {"koren": 0, "van_albada": 1}
 TURB = {"none": 0, "sst": 1, "sst2003": 2, "sa": 3, "kkl": 4}
 TRANSITION = {"none": 0, "bc": 1}
 TIME_ACC = {"none": 0, "RK2": 2, "RK4": 4, "TVDRK2": 12, "TVDRK3": 13, "implicit": 20}
@@ -97,6 +98,8 @@ class Case:
     tu_inf: float = 0.0             # free-stream turbulence intensity, percent
     tkl_inf: float = 0.0            # k-kL free-stream kL
     fix_tkl: tuple = (0.0,) * 6     # fixed kL of each face (k-kL)
+    pbs: tuple = (0, 0, 0)          # i/j/kPB_switch: the pressure-based switch of MUSCL and PPM
+    limiter: str = "koren"          # MUSCL limiter: the reference's Koren-alpha, or "van_albada"
     name: str = "case"
 
     @property

@@ -18,8 +18,9 @@ DRIVER = os.path.join(HERE, "fest_gpu_drive")
 def write_input(path, case: Case, nodes: np.ndarray, qp: np.ndarray, nsteps: int, wall_dist: bool):
     e, s, f, b = c_structs(case)
     hdr = [e.imx, e.jmx, e.kmx, e.n_var, s.flux, s.recon, s.turb, s.time_local, s.time_accuracy, s.accur,
-           *s.lim, *s.tlim, f.mu_sutherland, *b.id, nsteps, int(wall_dist), s.transition]
-    assert len(hdr) == 26
+           *s.lim, *s.tlim, f.mu_sutherland, *b.id, nsteps, int(wall_dist), s.transition, *s.pbs,
+           s.muscl_limiter]
+    assert len(hdr) == 30
     flow = [f.density_inf, f.x_speed_inf, f.y_speed_inf, f.z_speed_inf, f.pressure_inf, f.tk_inf, f.tw_inf,
             f.vel_mag, f.MInf, f.gm, f.R_gas, f.mu_ref, f.T_ref, f.Sutherland_temp, f.Pr, f.tPr, f.tv_inf,
             f.tu_inf, f.tkl_inf]

@@ -15,7 +15,7 @@ import os
 
 import numpy as np
 
-from .cases import FLUX, RECON, TURB, TIME_ACC, TRANSITION, Case
+from .cases import FLUX, LIMITER, RECON, TURB, TIME_ACC, TRANSITION, Case
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_DIR, "libfest_hip.so")
@@ -41,7 +41,7 @@ class Scheme(ctypes.Structure):
     _fields_ = [("flux", ctypes.c_int32), ("recon", ctypes.c_int32), ("turb", ctypes.c_int32),
                 ("time_local", ctypes.c_int32), ("time_accuracy", ctypes.c_int32), ("accur", ctypes.c_int32),
                 ("lim", ctypes.c_int32 * 3), ("tlim", ctypes.c_int32 * 3), ("global_time_step", ctypes.c_double),
-                ("transition", ctypes.c_int32)]
+                ("transition", ctypes.c_int32), ("pbs", ctypes.c_int32 * 3), ("muscl_limiter", ctypes.c_int32)]
 
 
 class Flow(ctypes.Structure):
@@ -119,6 +119,8 @@ def c_structs(case: Case):
     s.tlim[:] = list(case.tlim)
     s.global_time_step = case.global_time_step
     s.transition = TRANSITION[case.transition]
+    s.pbs[:] = list(case.pbs)
+    s.muscl_limiter = LIMITER[case.limiter]
     f = Flow()
     f.density_inf, f.x_speed_inf, f.y_speed_inf, f.z_speed_inf = case.rho_inf, case.u_inf, case.v_inf, case.w_inf
     f.pressure_inf, f.tk_inf, f.tw_inf, f.vel_mag, f.MInf = case.p_inf, case.tk_inf, case.tw_inf, case.vel_mag, case.MInf

@@ -21,7 +21,8 @@ module fest_gpu
   integer(c_int32_t), parameter, public :: FEST_FLUX_AUSM = 0, FEST_FLUX_AUSMP = 1, &
       FEST_FLUX_AUSMUP = 2, FEST_FLUX_SLAU = 3, FEST_FLUX_LDFSS0 = 4, FEST_FLUX_VANLEER = 5
   integer(c_int32_t), parameter, public :: FEST_RECON_NONE = 0, FEST_RECON_MUSCL = 1, &
-      FEST_RECON_PPM = 2, FEST_RECON_WENO = 3
+      FEST_RECON_PPM = 2, FEST_RECON_WENO = 3, FEST_RECON_WENO_NM = 4
+  integer(c_int32_t), parameter, public :: FEST_LIM_KOREN = 0, FEST_LIM_VAN_ALBADA = 1
   integer(c_int32_t), parameter, public :: FEST_TRANS_NONE = 0, FEST_TRANS_BC = 1
   integer(c_int32_t), parameter, public :: FEST_PREC_FP64 = 0, FEST_PREC_FP32_FLUX = 1
   integer(c_int32_t), parameter, public :: FEST_TURB_NONE = 0, FEST_TURB_SST = 1, FEST_TURB_SST2003 = 2, &
@@ -45,6 +46,8 @@ module fest_gpu
     integer(c_int32_t) :: lim(3), tlim(3)
     real(c_double) :: global_time_step
     integer(c_int32_t) :: transition
+    integer(c_int32_t) :: pbs(3)
+    integer(c_int32_t) :: muscl_limiter
   end type fest_scheme
 
   !< hot-path fields of type(flowtype), src/vartypes.f90:239-290

@@ -9,7 +9,7 @@
 !< IN (stream, native endianness), in this order:
 !<   int32  imx, jmx, kmx, n_var, flux, recon, turb, time_local,
 !<          time_accuracy, accur, lim(3), tlim(3), mu_sutherland, bc_id(6),
-!<          nsteps, with_wall_dist, transition
+!<          nsteps, with_wall_dist, transition, pbs(3), muscl_limiter
 !<   real64 global_time_step, CFL, density_inf, x_speed_inf, y_speed_inf,
 !<          z_speed_inf, pressure_inf, tk_inf, tw_inf, vel_mag, MInf, gm,
 !<          R_gas, mu_ref, T_ref, Sutherland_temp, Pr, tPr, tv_inf, tu_inf,
@@ -29,7 +29,7 @@ program fest_gpu_drive
   type(fest_flow) :: f
   type(fest_bc) :: b
   type(c_ptr) :: h
-  integer(c_int32_t) :: hdr(26)
+  integer(c_int32_t) :: hdr(30)
   real(c_double) :: cfl
   real(c_double), allocatable, target :: nodes(:,:,:,:), qp(:,:,:,:), res(:,:,:,:)
   real(c_double) :: norms(0:7)
@@ -55,6 +55,8 @@ program fest_gpu_drive
   nsteps = hdr(24)
   wall = hdr(25)
   s%transition = hdr(26)
+  s%pbs = hdr(27:29)
+  s%muscl_limiter = hdr(30)
   read(u) s%global_time_step, cfl
   read(u) f%density_inf, f%x_speed_inf, f%y_speed_inf, f%z_speed_inf, f%pressure_inf, &
           f%tk_inf, f%tw_inf, f%vel_mag, f%MInf, f%gm, f%R_gas, f%mu_ref, f%T_ref, &

@@ -83,6 +83,7 @@ contains
       case ("muscl"); s%recon = FEST_RECON_MUSCL
       case ("ppm");   s%recon = FEST_RECON_PPM
       case ("weno");  s%recon = FEST_RECON_WENO
+      case ("weno_NM");  s%recon = FEST_RECON_WENO_NM
       case default
         print *, "update: interpolant '", trim(scheme%interpolant), "' is not on the device path"
         stop 1
@@ -109,6 +110,8 @@ contains
     s%accur = scheme%accur
     s%lim = [scheme%ilimiter_switch, scheme%jlimiter_switch, scheme%klimiter_switch]
     s%tlim = [scheme%itlimiter_switch, scheme%jtlimiter_switch, scheme%ktlimiter_switch]
+    s%pbs = [scheme%iPB_switch, scheme%jPB_switch, scheme%kPB_switch]   ! src/face/state/muscl.f90:231-241
+    s%muscl_limiter = FEST_LIM_KOREN
     s%global_time_step = scheme%global_time_step
     select case (trim(scheme%transition))
       case ("none"); s%transition = FEST_TRANS_NONE

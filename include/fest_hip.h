@@ -46,7 +46,11 @@ extern "C" {
 enum { FEST_FLUX_AUSM = 0, FEST_FLUX_AUSMP = 1, FEST_FLUX_AUSMUP = 2,
        FEST_FLUX_SLAU = 3, FEST_FLUX_LDFSS0 = 4, FEST_FLUX_VANLEER = 5 };
 enum { FEST_RECON_NONE = 0, FEST_RECON_MUSCL = 1, FEST_RECON_PPM = 2,
-       FEST_RECON_WENO = 3 };
+       FEST_RECON_WENO = 3, FEST_RECON_WENO_NM = 4 /* 'weno_NM', src/face/state/weno_NM.f90 */ };
+/* MUSCL limiter: the reference's Koren-alpha form (src/face/state/
+ * muscl.f90:178,184), or van Albada (r^2+r)/(r^2+1), the form the reference
+ * leaves commented out at muscl.f90:180,186 */
+enum { FEST_LIM_KOREN = 0, FEST_LIM_VAN_ALBADA = 1 };
 enum { FEST_TURB_NONE = 0, FEST_TURB_SST = 1, FEST_TURB_SST2003 = 2 /* src/viscosity.f90:215, source.f90:205 */,
        FEST_TURB_SA = 3 /* 'sa', src/viscosity.f90:149, source.f90:835, lusgs.f90:1680 */,
        FEST_TURB_KKL = 4 /* 'kkl', src/viscosity.f90:469, source.f90:607, viscous.f90:450, lusgs.f90:1198 */ };
@@ -69,6 +73,9 @@ typedef struct {
   int32_t tlim[3];         /* i/j/ktlimiter_switch                       */
   double global_time_step; /* global_time_step (used when > 0, 'g')      */
   int32_t transition;      /* transition -> FEST_TRANS_* (source.f90:125-145) */
+  int32_t pbs[3];          /* i/j/kPB_switch: pressure-based switch of MUSCL
+                              and PPM (muscl.f90:37-112, ppm.f90:109-186)  */
+  int32_t muscl_limiter;   /* FEST_LIM_*                                 */
 } fest_scheme;
 
 /* the hot-path fields of type(flowtype), src/vartypes.f90:239-290 */

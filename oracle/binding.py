@@ -44,7 +44,8 @@ class CCase(ctypes.Structure):
                 ("fix_tw", ctypes.c_double * 6), ("fix_twall", ctypes.c_double * 6),
                 ("fix_tp", ctypes.c_double * 6), ("tv_inf", ctypes.c_double),
                 ("fix_tv", ctypes.c_double * 6), ("transition", ctypes.c_int32), ("tu_inf", ctypes.c_double),
-                ("tkl_inf", ctypes.c_double), ("fix_tkl", ctypes.c_double * 6)]
+                ("tkl_inf", ctypes.c_double), ("fix_tkl", ctypes.c_double * 6),
+                ("pbs", ctypes.c_int32 * 3), ("muscl_limiter", ctypes.c_int32)]
 
 
 def to_ccase(case) -> CCase:
@@ -64,6 +65,8 @@ def to_ccase(case) -> CCase:
     c.transition = {"none": 0, "bc": 1}[case.transition]
     c.tu_inf = float(case.tu_inf)
     c.tkl_inf = float(case.tkl_inf)
+    c.pbs[:] = [int(v) for v in case.pbs]
+    c.muscl_limiter = {"koren": 0, "van_albada": 1}[case.limiter]
     for f in ("fix_rho", "fix_u", "fix_v", "fix_w", "fix_p", "fix_tk", "fix_tw", "fix_twall", "fix_tp", "fix_tv", "fix_tkl"):
         getattr(c, f)[:] = [float(v) for v in getattr(case, f)]
     return c
@@ -132,6 +135,14 @@ def group_step(blocks, links, current_iter):
     return Oracle.lib().orc_group_step(hs, len(blocks), _links(links), len(links), int(current_iter))
 
 
+def group_step_mt(blocks, links, current_iter, nthreads):
+    """group_step with the blocks of each stage spread over `nthreads`
+    threads (the GIL is released for the whole call)."""
+    hs = (ctypes.c_void_p * len(blocks))(*[o.h for o in blocks])
+    return Oracle.lib().orc_group_step_mt(hs, len(blocks), _links(links), len(links), int(current_iter),
+                                          int(nthreads))
+
+
 class Oracle:
     """One block of the C restatement."""
 
@@ -164,6 +175,9 @@ class Oracle:
             L.orc_group_step.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.POINTER(CLink),
                                          ctypes.c_int, ctypes.c_int]
             L.orc_group_step.restype = ctypes.c_int
+            L.orc_group_step_mt.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.POINTER(CLink),
+                                            ctypes.c_int, ctypes.c_int, ctypes.c_int]
+            L.orc_group_step_mt.restype = ctypes.c_int
             cls._lib = L
         return cls._lib
 
@@ -288,5 +302,58 @@ def run_reference(case, nodes, qp, temp, stage, want, it=1, dist=None, wall_dist
         r = subprocess.run([sys.executable, script, HERE, pkg, jin, jout], capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"reference run failed ({r.returncode}):\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+        with open(jout, "rb") as f:
+            return pickle.load(f)
+
+
+_REF_TIME_SCRIPT = r"""
+import ctypes, sys, time, numpy as np, pickle
+sys.path.insert(0, sys.argv[1]); sys.path.insert(0, sys.argv[2])
+from binding import to_ccase, REF_SO
+job = pickle.load(open(sys.argv[3], 'rb'))
+case = job['case']
+L = ctypes.CDLL(REF_SO)
+c = to_ccase(case)
+L.ref_init(ctypes.byref(c))
+dp = ctypes.POINTER(ctypes.c_double)
+def ptr(a): return a.ctypes.data_as(dp)
+nodes = np.ascontiguousarray(job['nodes'], dtype=np.float64)
+L.ref_set_nodes(ptr(nodes))
+d = np.ascontiguousarray(job['dist'], dtype=np.float64); L.ref_set_dist(ptr(d))
+q = np.ascontiguousarray(job['qp'], dtype=np.float64)
+t = np.ascontiguousarray(job['temp'], dtype=np.float64)
+L.ref_set_state(ptr(q), ptr(t))
+L.ref_stage(ctypes.c_int(1), ctypes.c_int(5))   # first touch of every array
+t0 = time.perf_counter()
+n = 0
+while True:
+    L.ref_stage(ctypes.c_int(1), ctypes.c_int(5))
+    n += 1
+    if time.perf_counter() - t0 > job['budget'] or n >= job['reps']:
+        break
+pickle.dump({'seconds': (time.perf_counter() - t0) / n, 'reps': n}, open(sys.argv[4], 'wb'))
+"""
+
+
+def time_reference_residual(case, nodes, qp, temp, dist, reps=3, budget=10.0, tmpdir="/tmp"):
+    """Seconds per residual evaluation of the reference's own modules
+    (ref_stage(.., 5): ghost cells, face states, fluxes, gradients,
+    viscosity, viscous fluxes, residue, source), run in a child process on
+    one core; None when the reference build is absent."""
+    if not ref_available():
+        return None
+    import pickle
+    import tempfile
+    pkg = os.path.join(os.path.dirname(HERE), "fest-3d_amd")
+    with tempfile.TemporaryDirectory(dir=tmpdir) as td:
+        jin, jout = os.path.join(td, "in.pkl"), os.path.join(td, "out.pkl")
+        script = os.path.join(td, "run.py")
+        with open(script, "w") as f:
+            f.write(_REF_TIME_SCRIPT)
+        with open(jin, "wb") as f:
+            pickle.dump(dict(case=case, nodes=nodes, qp=qp, temp=temp, dist=dist, reps=reps, budget=budget), f)
+        r = subprocess.run([sys.executable, script, HERE, pkg, jin, jout], capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference timing failed ({r.returncode}):\n{r.stderr[-2000:]}")
         with open(jout, "rb") as f:
             return pickle.load(f)

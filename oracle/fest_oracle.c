@@ -25,6 +25,7 @@
 #include "fest_oracle.h"
 #include <math.h>
 #include <stdio.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -180,7 +181,7 @@ orc *orc_create(const orc_case *c) {
   if (WALLISH(c->bc_id[5])) s->mhz[s->kmx] = 0;
 #undef WALLISH
   /* src/boundary/boundary_state_reconstruction.f90:45-47 */
-  s->ppm_flag = (c->recon == ORC_RECON_PPM || c->recon == ORC_RECON_WENO) ? 1 : 0;
+  s->ppm_flag = (c->recon == ORC_RECON_PPM || c->recon == ORC_RECON_WENO || c->recon == ORC_RECON_WENO_NM) ? 1 : 0;
   for (int f = 0; f < 6; ++f) if (c->bc_id[f] == -7) s->ppm_flag = 1;
   return s;
 }
@@ -840,10 +841,13 @@ static fstates states_of(orc *s, int d) {
   return f;
 }
 
-/* MUSCL, src/face/state/muscl.f90:115-200 */
+/* MUSCL, src/face/state/muscl.f90:115-200.  With the van Albada option the
+ * limiter is (r^2 + r)/(r^2 + 1), the form muscl.f90:180,186 carries as a
+ * comment; the Koren-alpha limiter is the reference's. */
 static void muscl_dir(orc *s, int d) {
   fstates fs = states_of(s, d);
   int ii = d == 0, jj = d == 1, kk = d == 2;
+  const int valb = s->c.muscl_limiter == ORC_LIM_VAN_ALBADA;
   double alpha = 2. / 3., phi = 1.0, kappa = 1. / 3., eps = 1e-14;
   int sw = s->c.lim[d];
   for (int l = 0; l < s->nv; ++l) {
@@ -856,9 +860,9 @@ static void muscl_dir(orc *s, int d) {
           double fd = q[CI(s, i + ii, j + jj, k + kk)] - q[CI(s, i, j, k)];
           double bd = q[CI(s, i, j, k)] - q[CI(s, i - ii, j - jj, k - kk)];
           double r = fd / (bd + fsign(eps, bd));
-          double psi1 = fmx(0., fmn(fmn(2 * r, alpha * (r - 1.0) + 1.0), 2.));
+          double psi1 = valb ? ((r * r) + r) / ((r * r) + 1.0) : fmx(0., fmn(fmn(2 * r, alpha * (r - 1.0) + 1.0), 2.));
           r = bd / (fd + fsign(eps, fd));
-          double psi2 = fmx(0., fmn(fmn(2 * r, alpha * (r - 1.0) + 1.0), 2.));
+          double psi2 = valb ? ((r * r) + r) / ((r * r) + 1.0) : fmx(0., fmn(fmn(2 * r, alpha * (r - 1.0) + 1.0), 2.));
           psi1 = (1 - (1 - psi1) * sw);
           psi2 = (1 - (1 - psi2) * sw);
           Lp[fs.idx(s, i + ii, j + jj, k + kk)] = q[CI(s, i, j, k)] + 0.25 * phi *
@@ -934,6 +938,100 @@ static void ppm_dir(orc *s, int d) {
           }
     }
   }
+}
+
+/* WENO for non-uniform grids, src/face/state/weno_NM.f90:21-128: the
+ * stencil values are blended with volume weights before the Jiang-Shu
+ * polynomials and smoothness indicators; sums in element order. */
+static void weno_nm_dir(orc *s, int d) {
+  fstates fs = states_of(s, d);
+  int ii = d == 0, jj = d == 1, kk = d == 2;
+  const double g[3] = {1.0 / 10.0, 6.0 / 10.0, 3.0 / 10.0}, eps = 1e-6;
+  const double *vol = s->a[ORC_VOL];
+  for (int l = 0; l < s->nv; ++l) {
+    double *q = QPL(l);
+    double *Lp = fs.L + (size_t)l * fs.n, *Rp = fs.R + (size_t)l * fs.n;
+    for (int k = 1 - kk; k <= KMX - 1 + kk; ++k)
+      for (int j = 1 - jj; j <= JMX - 1 + jj; ++j)
+        for (int i = 1 - ii; i <= IMX - 1 + ii; ++i) {
+          double U[5], V[5];
+          for (int m = -2; m <= 2; ++m) {
+            size_t c = CI(s, i + m * ii, j + m * jj, k + m * kk);
+            U[m + 2] = q[c];
+            V[m + 2] = vol[c];
+          }
+          double alpha12 = V[4] / (V[3] + V[4]);
+          double alpha01 = V[3] / (V[2] + V[3]);
+          double alpha10 = V[2] / (V[1] + V[2]);
+          double alpha21 = V[1] / (V[0] + V[1]);
+          double U01 = (1.0 - alpha01) * U[2] + alpha01 * U[3];
+          double U12 = (1.0 - alpha12) * U[3] + alpha12 * U[4];
+          double U10 = (1.0 - alpha10) * U[1] + alpha10 * U[2];
+          double U21 = (1.0 - alpha21) * U[0] + alpha21 * U[1];
+          double U00 = U[1] + (1.0 - alpha21) * (U[1] - U[0]);
+          double U11 = U[3] + alpha12 * (U[3] - U[4]);
+          double P[3], B[3], w[3];
+          P[0] = (6.0 * U[2] - 1.0 * U10 - 2.0 * U00) / 3.0;
+          P[1] = (-1.0 * U10 + 2.0 * U[2] + 2.0 * U01) / 3.0;
+          P[2] = (2.0 * U01 + 2.0 * U[3] - 1.0 * U12) / 3.0;
+          B[0] = (13.0 / 12.0) * sq(2 * U10 - 2.0 * U00) + (1.0 / 4.0) * sq(4 * U[2] - 2.0 * U10 - 2.0 * U00);
+          B[1] = (13.0 / 12.0) * sq(2 * U10 - 4.0 * U[2] + 2 * U01) + (1.0 / 4.0) * sq(-2 * U10 + 2.0 * U01);
+          B[2] = (13.0 / 12.0) * sq(2 * U01 - 4.0 * U[3] + 2 * U12) + (1.0 / 4.0) * sq(-6 * U01 + 8.0 * U[3] - 2.0 * U12);
+          for (int m = 0; m < 3; ++m) w[m] = g[m] / sq(eps + B[m]);
+          Lp[fs.idx(s, i + ii, j + jj, k + kk)] = ((w[0] * P[0] + w[1] * P[1]) + w[2] * P[2]) / ((w[0] + w[1]) + w[2]);
+          P[0] = (6.0 * U[2] - 1.0 * U01 - 2.0 * U11) / 3.0;
+          P[1] = (-1.0 * U01 + 2.0 * U[2] + 2.0 * U10) / 3.0;
+          P[2] = (2.0 * U10 + 2.0 * U[1] - 1.0 * U21) / 3.0;
+          B[0] = (13.0 / 12.0) * sq(2 * U01 - 2.0 * U11) + (1.0 / 4.0) * sq(4 * U[2] - 2.0 * U01 - 2.0 * U11);
+          B[1] = (13.0 / 12.0) * sq(2 * U01 - 4.0 * U[2] + 2 * U10) + (1.0 / 4.0) * sq(-2 * U01 + 2.0 * U10);
+          B[2] = (13.0 / 12.0) * sq(2 * U10 - 4.0 * U[1] + 2 * U21) + (1.0 / 4.0) * sq(-6 * U10 + 8.0 * U[1] - 2.0 * U21);
+          for (int m = 0; m < 3; ++m) w[m] = g[m] / sq(eps + B[m]);
+          Rp[fs.idx(s, i, j, k)] = ((w[0] * P[0] + w[1] * P[1]) + w[2] * P[2]) / ((w[0] + w[1]) + w[2]);
+        }
+  }
+}
+
+/* Pressure-based switch, src/face/state/muscl.f90:37-112 (and its copy in
+ * ppm.f90:109-186): pdif = 1 - |p(c+1) - p(c-1)| / (|...| + p_inf) over the
+ * interior cells along d, copied into the two ghost cells along d from the
+ * first and last interior cell; then every face state of faces 1..N is
+ * pulled towards its cell value by that cell's pdif. */
+static void pb_switch_dir(orc *s, int d) {
+  fstates fs = states_of(s, d);
+  int ii = d == 0, jj = d == 1, kk = d == 2;
+  const int N = d == 0 ? IMX : d == 1 ? JMX : KMX;
+  const double *p = QPL(4);
+  /* pdif over (0:imx,0:jmx,0:kmx) */
+  const int ni = IMX + 1, nj = JMX + 1;
+  double *pdif = (double *)malloc(sizeof(double) * (size_t)ni * nj * (KMX + 1));
+#define PD(i, j, k) pdif[(size_t)(i) + (size_t)ni * ((size_t)(j) + (size_t)nj * (size_t)(k))]
+  for (int k = 1; k <= KMX - 1; ++k)
+    for (int j = 1; j <= JMX - 1; ++j)
+      for (int i = 1; i <= IMX - 1; ++i) {
+        double pd2 = fabs(p[CI(s, i + ii, j + jj, k + kk)] - p[CI(s, i - ii, j - jj, k - kk)]);
+        PD(i, j, k) = 1 - (pd2 / (pd2 + s->c.p_inf));
+      }
+  for (int k = 1; k <= KMX - 1; ++k)
+    for (int j = 1; j <= JMX - 1; ++j)
+      for (int i = 1; i <= IMX - 1; ++i) {
+        int n = d == 0 ? i : d == 1 ? j : k;
+        if (n == 1) PD(i - ii, j - jj, k - kk) = PD(i, j, k);
+        if (n == N - 1) PD(i + ii, j + jj, k + kk) = PD(i, j, k);
+      }
+  for (int l = 0; l < s->nv; ++l) {
+    const double *q = QPL(l);
+    double *Lp = fs.L + (size_t)l * fs.n, *Rp = fs.R + (size_t)l * fs.n;
+    for (int k = 1; k <= KMX - 1 + kk; ++k)
+      for (int j = 1; j <= JMX - 1 + jj; ++j)
+        for (int i = 1; i <= IMX - 1 + ii; ++i) {
+          size_t f = fs.idx(s, i, j, k);
+          size_t cm = CI(s, i - ii, j - jj, k - kk), c = CI(s, i, j, k);
+          Lp[f] = q[cm] + (PD(i - ii, j - jj, k - kk) * (Lp[f] - q[cm]));
+          Rp[f] = q[c] - (PD(i, j, k) * (q[c] - Rp[f]));
+        }
+  }
+#undef PD
+  free(pdif);
 }
 
 /* first order, src/face/state/face_interpolant.f90:61-76 */
@@ -1026,8 +1124,11 @@ static int compute_face_interpolant(orc *s) {
       case ORC_RECON_MUSCL: muscl_dir(s, d); break;
       case ORC_RECON_PPM: ppm_dir(s, d); break;
       case ORC_RECON_WENO: weno_dir(s, d); break;
+      case ORC_RECON_WENO_NM: weno_nm_dir(s, d); break;
       default: return ORC_E_CONFIG;
     }
+    /* compute_muscl_states / compute_ppm_states only */
+    if (s->c.pbs[d] == 1 && (s->c.recon == ORC_RECON_MUSCL || s->c.recon == ORC_RECON_PPM)) pb_switch_dir(s, d);
   }
   /* src/boundary/boundary_state_reconstruction.f90:28-75 */
   if (s->c.recon != ORC_RECON_NONE)
@@ -2554,6 +2655,70 @@ int orc_group_step(orc **blocks, int nb, const orc_link *links, int nl, int curr
       double TF = nst == 4 ? TF4[stage] : TF2[stage], SF = nst == 4 ? SF4[stage] : SF2[stage];
       if ((st = update_with(s, TF, SF, stage == nst - 1, 1))) return st;
     }
+  }
+  return ORC_OK;
+}
+
+/* ------------------------------------------------ threaded group step */
+typedef struct {
+  orc **blocks;
+  int nb, t, nt, stage, nst, current_iter;
+  int st;
+} mt_job;
+
+static void *mt_stage(void *arg) {
+  mt_job *j = (mt_job *)arg;
+  const double TF4[4] = {0.5, 0.5, 1.0, 1. / 6.}, SF4[4] = {1., 2., 2., 1.};
+  const double TF2[2] = {0.5, 0.5}, SF2[2] = {1., 1.};
+  j->st = ORC_OK;
+  for (int b = j->t; b < j->nb; b += j->nt) {
+    orc *s = j->blocks[b];
+    int st;
+    if (j->stage < 0) {
+      /* step start: Temp refresh, R_store = 0, U_store = qp */
+      orc_temp_refresh(s);
+      for (size_t n = 0; n < s->nR * (size_t)s->nv; ++n) s->a[ORC_RSTORE][n] = 0.;
+      memcpy(s->a[ORC_USTORE], s->a[ORC_QP], s->ncell * (size_t)s->nv * sizeof(double));
+      continue;
+    }
+    if ((st = orc_residual(s, j->current_iter))) { j->st = st; return NULL; }
+    if (j->stage == 0 && (st = orc_time_step(s))) { j->st = st; return NULL; }
+    double TF = j->nst == 4 ? TF4[j->stage] : TF2[j->stage], SF = j->nst == 4 ? SF4[j->stage] : SF2[j->stage];
+    if ((st = update_with(s, TF, SF, j->stage == j->nst - 1, 1))) { j->st = st; return NULL; }
+  }
+  return NULL;
+}
+
+static int mt_run(orc **blocks, int nb, int nt, int stage, int nst, int current_iter) {
+  pthread_t th[256];
+  mt_job jobs[256];
+  int created[256];
+  if (nt > 256) nt = 256;
+  if (nt > nb) nt = nb;
+  int st = ORC_OK;
+  for (int t = 0; t < nt; ++t) {
+    jobs[t] = (mt_job){blocks, nb, t, nt, stage, nst, current_iter, ORC_OK};
+    created[t] = pthread_create(&th[t], NULL, mt_stage, &jobs[t]) == 0;
+    if (!created[t]) mt_stage(&jobs[t]); /* no thread: run its share here */
+  }
+  for (int t = 0; t < nt; ++t) {
+    if (created[t]) pthread_join(th[t], NULL);
+    if (jobs[t].st && !st) st = jobs[t].st;
+  }
+  return st;
+}
+
+int orc_group_step_mt(orc **blocks, int nb, const orc_link *links, int nl, int current_iter, int nthreads) {
+  int ta = blocks[0]->c.time_accuracy;
+  for (int b = 0; b < nb; ++b)
+    if (blocks[b]->c.time_accuracy != ta) return ORC_E_CONFIG;
+  int nst = ta == ORC_TA_RK4 ? 4 : ta == ORC_TA_RK2 ? 2 : 0;
+  if (!nst || nthreads < 1) return ORC_E_CONFIG;
+  int st;
+  if ((st = mt_run(blocks, nb, nthreads, -1, nst, current_iter))) return st;
+  for (int stage = 0; stage < nst; ++stage) {
+    if ((st = orc_exchange(blocks, nb, links, nl))) return st;
+    if ((st = mt_run(blocks, nb, nthreads, stage, nst, current_iter))) return st;
   }
   return ORC_OK;
 }

@@ -27,7 +27,10 @@ extern "C" {
 enum { ORC_FLUX_AUSM = 0, ORC_FLUX_AUSMP = 1, ORC_FLUX_AUSMUP = 2,
        ORC_FLUX_SLAU = 3, ORC_FLUX_LDFSS0 = 4, ORC_FLUX_VANLEER = 5 };
 enum { ORC_RECON_NONE = 0, ORC_RECON_MUSCL = 1, ORC_RECON_PPM = 2,
-       ORC_RECON_WENO = 3 };
+       ORC_RECON_WENO = 3, ORC_RECON_WENO_NM = 4 };
+/* the MUSCL limiter: the reference's Koren-alpha form, or van Albada
+ * (the form muscl.f90:180,186 leaves commented out) */
+enum { ORC_LIM_KOREN = 0, ORC_LIM_VAN_ALBADA = 1 };
 enum { ORC_TURB_NONE = 0, ORC_TURB_SST = 1, ORC_TURB_SST2003 = 2, ORC_TURB_SA = 3, ORC_TURB_KKL = 4 };
 enum { ORC_TA_NONE = 0, ORC_TA_RK2 = 2, ORC_TA_RK4 = 4, ORC_TA_TVDRK2 = 12, ORC_TA_TVDRK3 = 13,
        ORC_TA_IMPLICIT = 20 };
@@ -58,6 +61,8 @@ typedef struct {
   double tu_inf;      /* free-stream turbulence intensity, percent */
   double tkl_inf;     /* free-stream kL (k-kL) */
   double fix_tkl[6];  /* fixed kL of each face (k-kL) */
+  int32_t pbs[3];     /* i/j/kPB_switch: pressure-based switch of MUSCL and PPM */
+  int32_t muscl_limiter; /* ORC_LIM_* */
 } orc_case;
 
 typedef struct orc orc;
@@ -124,6 +129,12 @@ int orc_exchange(orc **blocks, int nblocks, const orc_link *links, int nlinks);
  * the interface exchange opening every residual evaluation
  * (src/update.f90:129-226, 534). */
 int orc_group_step(orc **blocks, int nblocks, const orc_link *links, int nlinks, int current_iter);
+/* The same RK step with the blocks of each stage spread over `nthreads`
+ * threads (one block per thread at a time), as the reference spreads its
+ * blocks over MPI processes; the exchange stays serial.  RK2/RK4 only.
+ * Results equal orc_group_step's bit for bit (blocks share no state). */
+int orc_group_step_mt(orc **blocks, int nblocks, const orc_link *links, int nlinks, int current_iter,
+                      int nthreads);
 
 #ifdef __cplusplus
 }

@@ -47,6 +47,7 @@ module ref_harness
     integer(c_int32_t) :: transition
     real(c_double) :: tu_inf
     real(c_double) :: tkl_inf, fix_tkl(6)
+    integer(c_int32_t) :: pbs(3), muscl_limiter
   end type hcase
 
   type(extent) :: dims
@@ -71,7 +72,8 @@ contains
     type(hcase), intent(in) :: hc
     character(len=16), parameter :: fluxes(0:5) = [character(len=16) :: 'ausm', 'ausmP', 'ausmUP', &
                                                    'slau', 'ldfss0', 'van_leer']
-    character(len=16), parameter :: recons(0:3) = [character(len=16) :: 'none', 'muscl', 'ppm', 'weno']
+    character(len=16), parameter :: recons(0:4) = [character(len=16) :: 'none', 'muscl', 'ppm', 'weno', &
+                                                   'weno_NM']
     imx = hc%imx; jmx = hc%jmx; kmx = hc%kmx; nv = hc%n_var
     dims%imx = imx; dims%jmx = jmx; dims%kmx = kmx; dims%n_var = nv
     control%n_var = nv
@@ -95,7 +97,7 @@ contains
     schm%accur = hc%accur
     schm%ilimiter_switch = hc%lim(1); schm%jlimiter_switch = hc%lim(2); schm%klimiter_switch = hc%lim(3)
     schm%itlimiter_switch = hc%tlim(1); schm%jtlimiter_switch = hc%tlim(2); schm%ktlimiter_switch = hc%tlim(3)
-    schm%iPB_switch = 0; schm%jPB_switch = 0; schm%kPB_switch = 0
+    schm%iPB_switch = hc%pbs(1); schm%jPB_switch = hc%pbs(2); schm%kPB_switch = hc%pbs(3)
     if (hc%time_local == 1) then
       schm%time_stepping_method = 'l'
     else

@@ -31,9 +31,10 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_struct_sizes_match_header():
-    # fest_extent is four int32; fest_scheme 12 int32, one double, the transition int32 (padded)
+    # fest_extent is four int32; fest_scheme 12 int32, one double, then the
+    # transition, pbs(3) and muscl_limiter int32 (padded to 8)
     assert ctypes.sizeof(native.Extent) == 16
-    assert ctypes.sizeof(native.Scheme) == 64  # + transition and padding
+    assert ctypes.sizeof(native.Scheme) == 80
     assert ctypes.sizeof(native.BC) == 24 + 8 * 66
     # 16 doubles, mu_sutherland (int32, then 4 bytes of padding), tv_inf, tu_inf, tkl_inf
     assert ctypes.sizeof(native.Flow) == 8 * 16 + 8 + 8 + 8 + 8

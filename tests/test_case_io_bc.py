@@ -26,3 +26,39 @@ def test_sst_reads_omega(tmp_path):
     p.write_text(BC)
     r = case_io.read_bc(str(p), case)
     assert r.fix_tw == (9.0,) + (case.tw_inf,) * 5
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _case_copy(tmp_path, name, edit):
+    import shutil
+    dst = tmp_path / name
+    shutil.copytree(os.path.join(GOLDEN, name, "system"), dst / "system")
+    p = dst / "system" / "fvscheme.md"
+    p.write_text(edit(p.read_text()))
+    return str(dst)
+
+
+def test_reads_transition_model_and_intensity(tmp_path):
+    """'## Transition model' and the free-stream intensity (percent) reach
+    the case (read.f90:219-222 and read_flow)."""
+    import pytest
+    root = _case_copy(tmp_path, "Tfp", lambda s: s.replace("## Transition model\nnone", "## Transition model\nbc"))
+    blocks, _, _ = case_io.read_case(root)
+    c = blocks[0][0]
+    assert c.transition == "bc"
+    flow = case_io.read_md(os.path.join(root, "system", "flow.md"))
+    assert c.tu_inf == float(case_io._first(flow, "Free Stream Turbulence Intensity")) and c.tu_inf > 0
+    root2 = _case_copy(tmp_path / "b", "Tfp",
+                       lambda s: s.replace("## Transition model\nnone", "## Transition model\nlctm2015"))
+    with pytest.raises(ValueError, match="lctm2015"):
+        case_io.read_case(root2)
+
+
+def test_reads_pressure_based_switch(tmp_path):
+    """The last three numbers of the limiter line are iPB/jPB/kPB_switch
+    (read.f90:198-201)."""
+    root = _case_copy(tmp_path, "Tfp", lambda s: s.replace("0 0 0  0 0 0", "1 1 1  1 0 1"))
+    blocks, _, _ = case_io.read_case(root)
+    assert blocks[0][0].lim == (1, 1, 1) and blocks[0][0].pbs == (1, 0, 1)

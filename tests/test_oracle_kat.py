@@ -88,6 +88,7 @@ def test_flux_zero_mass_flag():
 def states_1d(recon, lim):
     case = tiny_case(5, 2, 2, recon=recon, lim=(lim,) * 3, tlim=(lim,) * 3)
     o = Oracle(case)
+    o.arr("vol")[...] = 1.0  # cells(:,:,:)%volume = 1 (test_weno_NM.f90:33)
     q = o.arr("qp")
     q[...] = 1.0
     for l in range(5):
@@ -552,3 +553,117 @@ def test_resnorm_closed_form():
     F, G, H = o.arr("F")[0], o.arr("G")[0], o.arr("H")[0]
     m = F[:, :, 0].sum() - F[:, :, -1].sum() + G[:, 0, :].sum() - G[:, -1, :].sum() + H[0].sum() - H[-1].sum()
     assert out[0] == pytest.approx(abs(m), rel=1e-10)
+
+
+# ------------------------------------------------ MUSCL limiters, switches
+def _muscl_row(r, limiter, lim=1):
+    """Upper- and lower-face states of cell 2 of a row with q(1) = 0,
+    q(2) = 1, q(3) = 1 + r: backward difference 1, forward difference r."""
+    case = tiny_case(6, 2, 2, recon="muscl", lim=(lim,) * 3, tlim=(lim,) * 3, limiter=limiter)
+    o = Oracle(case)
+    q = o.arr("qp")
+    q[...] = 1.0
+    row = np.full(11, 7.0)
+    row[:3] = (-2.0, -1.0, 0.0)   # cells -2 .. 0
+    row[3:6] = (0.0, 1.0, 1.0 + r)  # cells 1, 2, 3
+    for l in range(5):
+        q[l, 3, 3, :] = row
+    o.stage(1, 2)
+    return o.arr("xl")[0, 0, 0, 3], o.arr("xr")[0, 0, 0, 2]
+
+
+@pytest.mark.parametrize("r", [-2.0, -0.5, 0.0, 0.5, 1.0, 3.0, 10.0])
+def test_van_albada_closed_form(r):
+    """psi(r) = (r^2 + r)/(r^2 + 1) has psi(1/r) = psi(r)/r, so both
+    kappa = 1/3 face states reduce to q -+ psi(r) bd / 2."""
+    va = (r * r + r) / (r * r + 1.0)
+    L, R = _muscl_row(r, "van_albada")
+    assert L == pytest.approx(1.0 + 0.5 * va, rel=1e-12, abs=1e-13)
+    assert R == pytest.approx(1.0 - 0.5 * va, rel=1e-12, abs=1e-13)
+
+
+@pytest.mark.parametrize("r,psi,psi_inv", [(-1.0, 0.0, 0.0), (0.25, 0.5, 2.0), (1.0, 1.0, 1.0), (3.0, 2.0, 5.0 / 9.0),
+                                           (10.0, 2.0, 0.2)])
+def test_koren_closed_form(r, psi, psi_inv):
+    """Koren-alpha limiter max(0, min(2r, (2r+1)/3, 2)): 0 for r <= 0, 1 at
+    r = 1, 2 from r = 2.5; face states q + (2/3 psi(r) + 4/3 r psi(1/r)) bd/4
+    and q - (4/3 psi(r) + 2/3 r psi(1/r)) bd/4."""
+    L, R = _muscl_row(r, "koren")
+    assert L == pytest.approx(1.0 + 0.25 * (2 / 3 * psi + 4 / 3 * r * psi_inv), rel=1e-12, abs=1e-13)
+    assert R == pytest.approx(1.0 - 0.25 * (4 / 3 * psi + 2 / 3 * r * psi_inv), rel=1e-12, abs=1e-13)
+    # limiter switch off: psi = 1, the unlimited kappa = 1/3 states
+    L0, R0 = _muscl_row(r, "koren", lim=0)
+    assert L0 == pytest.approx(1.0 + 0.25 * (2 / 3 + 4 / 3 * r), rel=1e-13)
+    assert R0 == pytest.approx(1.0 - 0.25 * (4 / 3 + 2 / 3 * r), rel=1e-13)
+
+
+def test_weno_nm_kat():
+    L, R = states_1d("weno_NM", 1)  # tests/test_weno_NM.f90:19-46, unit volumes
+    assert 0.48 <= L[1] < 0.52 and 0.48 < R[1] < 0.52
+    assert 1.48 <= L[2] < 1.52 and 1.48 < R[2] < 1.52
+    assert 2.4 <= L[3] < 2.5 and 2.4 < R[3] < 2.5
+    assert 3.6 <= L[4] < 3.7 and 3.8 < R[4] < 4.1
+    assert 5.9 <= L[5] < 6.1 and 6.9 < R[5] < 7.1
+
+
+def test_weno_nm_is_weno5_on_equal_volumes():
+    """With equal volumes every blend weight is 1/2 and WENO-NM's
+    polynomials and smoothness indicators are WENO5's (weno.f90:59-87)
+    rewritten: the two agree to rounding, on a wiggly profile too."""
+    case, nodes, q = C.laminar_box(10, 8, 6)
+    out = {}
+    for recon in ("weno", "weno_NM"):
+        o = Oracle(dataclasses.replace(case, recon=recon))
+        o.set_nodes(nodes)
+        o.set_state(q)
+        o.stage(1, 2)
+        out[recon] = [o.arr(n).copy() for n in ("xl", "xr", "yl", "yr", "zl", "zr")]
+    for a, b in zip(out["weno"], out["weno_NM"]):
+        np.testing.assert_allclose(b, a, rtol=1e-13, atol=0)
+    # on the stretched bump grid the volumes differ and so do the states
+    c2, n2, q2 = C.sst_bump(10, 8, 4)
+    yl = {}
+    for recon in ("weno", "weno_NM"):
+        o = Oracle(dataclasses.replace(c2, recon=recon))
+        o.set_nodes(n2)
+        o.set_state(q2)
+        o.stage(1, 2)
+        yl[recon] = o.arr("yl").copy()
+    assert np.abs(yl["weno"] - yl["weno_NM"]).max() > 1e-6 * np.abs(yl["weno"]).max()
+
+
+def _pb_states(case, nodes, q, pbs):
+    o = Oracle(dataclasses.replace(case, pbs=pbs))
+    o.set_nodes(nodes)
+    o.set_state(q)
+    o.stage(1, 2)
+    return [o.arr(n).copy() for n in ("xl", "xr", "yl", "yr", "zl", "zr")]
+
+
+@pytest.mark.parametrize("recon", ["muscl", "ppm"])
+def test_pressure_switch_limits(recon):
+    """pdif = 1 - |dp|/(|dp| + p_inf): with uniform pressure it is 1 and the
+    switch changes nothing; with pressure jumps far above p_inf it tends to
+    0 and the face states become the first-order cell values."""
+    case, _, q = C.laminar_box(8, 6, 4)
+    case = dataclasses.replace(case, recon=recon, bc_id=(0,) * 6)
+    nodes = C.cartesian_nodes(case)
+    q = q.copy()
+    q[4] = case.p_inf
+    a, b = _pb_states(case, nodes, q, (0, 0, 0)), _pb_states(case, nodes, q, (1, 1, 1))
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    # pressure jumps of 1e12 p_inf between every i-neighbour pair two
+    # apart (period 4 in i): pdif ~ 1e-12
+    kk, jj, ii = np.indices(q[4].shape)
+    q[4] = case.p_inf * (1.0 + 1e12 * ((ii // 2) % 2))
+    L, R = _pb_states(case, nodes, q, (1, 0, 0))[:2]
+    u = q[1]
+    # i faces 1..imx of interior rows: left = cell f-1, right = cell f
+    for f in range(1, case.imx + 1):
+        np.testing.assert_allclose(L[1, :, :, f], u[3:-3, 3:-3, f + 1], rtol=1e-10)
+        np.testing.assert_allclose(R[1, :, :, f], u[3:-3, 3:-3, f + 2], rtol=1e-10)
+    # the j states are untouched when only the i switch is on
+    a0 = _pb_states(case, nodes, q, (0, 0, 0))
+    assert np.array_equal(a0[2], _pb_states(case, nodes, q, (1, 0, 0))[2])
+    assert not np.array_equal(a0[0], L)

@@ -330,3 +330,51 @@ def test_transition_bc_changes_the_turbulence_residue():
             assert o.residual(3) == 0
             res[tr] = o.arr("res")[5].copy()
         assert (res["bc"] != res["none"]).all(), turb
+
+
+@pytest.mark.parametrize("flux", ["ausmP", "slau", "ldfss0"])
+def test_weno_nm(flux):
+    """WENO for non-uniform grids (src/face/state/weno_NM.f90:21-147) on the
+    stretched bump grid, SST with wall distance."""
+    case, nodes, q = C.sst_bump(14, 10, 6)
+    case = dataclasses.replace(case, flux=flux, recon="weno_NM")
+    compare(case, nodes, q, it=3, wall_dist=True, extra=("mut", "f1"))
+
+
+def test_weno_nm_laminar_boundaries():
+    case, _, q = C.laminar_box(10, 8, 6)
+    nodes = C.cartesian_nodes(case, warp=0.05)
+    case = dataclasses.replace(case, recon="weno_NM", bc_id=(-3, -4, -6, -5, -8, -8),
+                               fix_twall=(0, 0, 0, 300, 0, 0), fix_rho=(case.rho_inf * 1.01,) * 6,
+                               fix_u=(case.u_inf,) * 6, fix_v=(3.0,) * 6, fix_w=(0.0,) * 6,
+                               fix_p=(case.p_inf * 0.99,) * 6)
+    compare(case, nodes, q, it=1)
+
+
+@pytest.mark.parametrize("recon", ["muscl", "ppm"])
+@pytest.mark.parametrize("pbs", [(1, 1, 1), (1, 0, 1), (0, 1, 0)])
+def test_pressure_based_switch(recon, pbs):
+    """The pressure-based switch of MUSCL and PPM (src/face/state/muscl.f90:
+    37-112, ppm.f90:109-186) with the boundary overrides after it, SST."""
+    case, nodes, q = C.sst_bump(14, 10, 6)
+    case = dataclasses.replace(case, flux="ausmUP", recon=recon, pbs=pbs)
+    compare(case, nodes, q, it=3, wall_dist=True, extra=("mut", "f1"))
+
+
+@pytest.mark.parametrize("recon", ["muscl", "ppm", "weno"])
+def test_pressure_based_switch_laminar(recon):
+    """Laminar box with walls, far field and inlets; under WENO the switch
+    is read and not applied, as in compute_face_interpolant."""
+    case, _, q = C.laminar_box(10, 8, 6)
+    nodes = C.cartesian_nodes(case, warp=0.05)
+    case = dataclasses.replace(case, recon=recon, pbs=(1, 1, 1), bc_id=(-1, -2, -5, -6, -9, -9),
+                               fix_twall=(0, 0, 310, 0, 0, 0), fix_rho=(case.rho_inf * 1.01,) * 6,
+                               fix_u=(case.u_inf,) * 6, fix_v=(3.0,) * 6, fix_w=(0.0,) * 6,
+                               fix_p=(case.p_inf * 0.99,) * 6, accur=1)
+    compare(case, nodes, q, it=1)
+
+
+def test_lusgs_weno_nm_pressure_switch():
+    case, nodes, q = C.sst_bump(12, 8, 5)
+    compare_lusgs(dataclasses.replace(case, recon="weno_NM"), nodes, q, it=2, wall_dist=True)
+    compare_lusgs(dataclasses.replace(case, recon="muscl", pbs=(1, 1, 1)), nodes, q, it=2, wall_dist=True)

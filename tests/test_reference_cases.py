@@ -32,8 +32,14 @@ import pytest
 
 from binding import Oracle, group_exchange, group_step, group_wall_dist
 from festgpu import case_io as IO
+import postproc as PP
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+# The report's own tolerance is 2% (tests/Report.txt); the checks below hold
+# every case to 0.2% of the reported value, and to the four digits this
+# path reaches as a regression value.  Tfp's 0.12% gap to the report is
+# open (its cause is not found).
+REPORT_RTOL = 2e-3
 CASES = {
     # case: (measure, reported value, value this path reaches, plate length)
     "SmoothBump": ("entropy", "7.883E-07", "7.883E-07", None),
@@ -57,8 +63,8 @@ def measure(name, blocks, qps, vols, grads=None, mus=None):
     kind, _, _, length = CASES[name]
     cases = [c for c, _ in blocks]
     if kind == "entropy":
-        return IO.entropy_error(qps, vols, cases[0])
-    return IO.drag_coefficient(cases, [n for _, n in blocks], qps, grads, mus, length)
+        return PP.entropy_error(qps, vols, cases[0])
+    return PP.drag_coefficient(cases, [n for _, n in blocks], qps, grads, mus, length)
 
 
 def oracle_run(name, n=None):
@@ -145,9 +151,9 @@ def test_group_wall_distance_is_decomposition_invariant():
 def test_oracle_reproduces_reference_report(name):
     blocks, links, ctl, os_ = oracle_run(name)
     value = oracle_measure(name, blocks, links, os_, ctl["max_iter"] + 1)
-    assert "{:.3E}".format(value) == CASES[name][2], value
     reported = float(CASES[name][1])
-    assert abs(value - reported) / reported < 2e-3, value
+    assert abs(value - reported) / reported < REPORT_RTOL, value
+    assert "{:.3E}".format(value) == CASES[name][2], value
 
 
 @pytest.mark.gpu
@@ -185,5 +191,7 @@ def test_gpu_reproduces_reference_report(name):
     vols = [o.arr("vol")[3:-3, 3:-3, 3:-3].copy() for o in os_]
     value = measure(name, blocks, [x["qp"] for x in out], vols, [x["grads"] for x in out], [x["mu"] for x in out])
     say(name, "measure", value)
+    reported = float(CASES[name][1])
+    assert abs(value - reported) / reported < REPORT_RTOL, value
     assert "{:.3E}".format(value) == CASES[name][2], value
     group.close()
