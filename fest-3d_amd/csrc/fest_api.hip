@@ -39,6 +39,28 @@ struct KgTimer {
   long launches = 0;
 };
 
+// A time step captured from the stream into a HIP graph and replayed with
+// one launch.  Kernel arguments are baked in at capture, so a graph is kept
+// per inflow-fix class and re-captured when the CFL number changes or the
+// arrays it was captured on move.  The first call of a class runs eagerly
+// (one-time set-up such as the LU-SGS geometry copy stays out of the graph);
+// the second captures.
+struct StepGraphs {
+  bool enabled = true;
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  double cfl[2] = {0.0, 0.0};
+  const void *key[2] = {nullptr, nullptr};
+  int seen[2] = {0, 0};
+  void reset() {
+    for (int k = 0; k < 2; ++k) {
+      if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
+      exec[k] = nullptr;
+      seen[k] = 0;
+      key[k] = nullptr;
+    }
+  }
+};
+
 struct fest_handle {
   Blk b;
   hipStream_t stream = nullptr;      // the stream every launch goes to
@@ -59,6 +81,11 @@ struct fest_handle {
   unsigned timed = 0;
   KgTimer timers[KG_COUNT];
   std::vector<hipEvent_t> ev_free;
+  // the time step as captured HIP graphs, one per inflow-fix class
+  // (iterations <= 2 and later, the only way current_iter enters a kernel)
+  StepGraphs graphs;
+  int check_every = 1;  // read the device error word after every n-th step
+  int unchecked = 0;
 };
 
 // a group's view of its blocks; a block destroyed while in a group detaches
@@ -71,6 +98,9 @@ struct fest_group {
   hipStream_t stream = nullptr;
   bool broken = false;
   std::string err;
+  StepGraphs graphs;
+  int check_every = 1;
+  int unchecked = 0;
 };
 
 namespace {
@@ -494,6 +524,56 @@ int run_op(fest_handle *h, const StepOp &op, int current_iter) {
   }
 }
 
+// Runs `step` (which only launches kernels on `st`) through the graph of
+// the inflow-fix class of current_iter, capturing it on the class's second
+// call; eagerly when graphs are off.  key identifies the arrays and
+// constants the kernels were captured with.
+template <typename F>
+int run_graphed(StepGraphs &G, hipStream_t st, int current_iter, double cfl, const void *key, bool eager, F step,
+                std::string &err) {
+  const int k = current_iter <= 2 ? 0 : 1;
+  if (eager || !G.enabled) return step();
+  if (G.exec[k] && (G.cfl[k] != cfl || G.key[k] != key)) {
+    (void)hipGraphExecDestroy(G.exec[k]);
+    G.exec[k] = nullptr;
+    G.seen[k] = 0;
+  }
+  if (!G.exec[k]) {
+    if (G.seen[k]++ == 0) return step();
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+      err = std::string("hipStreamBeginCapture: ") + hipGetErrorString(e);
+      return FEST_E_HIP;
+    }
+    int rc = step();
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(st, &graph);
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (e != hipSuccess) {
+      err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
+      return FEST_E_HIP;
+    }
+    e = hipGraphInstantiate(&G.exec[k], graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      G.exec[k] = nullptr;
+      err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
+      return FEST_E_HIP;
+    }
+    G.cfl[k] = cfl;
+    G.key[k] = key;
+  }
+  hipError_t e = hipGraphLaunch(G.exec[k], st);
+  if (e != hipSuccess) {
+    err = std::string("hipGraphLaunch: ") + hipGetErrorString(e);
+    return FEST_E_HIP;
+  }
+  return 0;
+}
+
 // surface nodes of the no-slip faces of one block, in the order of
 // src/wall/wall.f90:198-288, rounded like the ES18.10E3 surface-node file
 // the reference writes and reads back (src/wall/wall.f90:101)
@@ -738,6 +818,7 @@ void fest_destroy(fest_handle *h) {
       (void)hipEventDestroy(pr.second);
     }
   for (hipEvent_t e : h->ev_free) (void)hipEventDestroy(e);
+  h->graphs.reset();
   for (void *p : h->allocs) (void)hipFree(p);
   if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
   delete h;
@@ -763,6 +844,7 @@ int fest_upload_geometry(fest_handle *h, const double *cells4, const double *if4
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->have_grid = true;
   h->sk_geometry = false;
+  h->graphs.reset();
   return 0;
 }
 
@@ -800,6 +882,7 @@ int fest_set_grid(fest_handle *h, const double *nodes, int with_wall_dist) {
   if (rc) return fail(h, FEST_E_NONPHYSICAL, "non-positive cell volume in the grid");
   h->have_grid = true;
   h->sk_geometry = false;
+  h->graphs.reset();
   return 0;
 }
 
@@ -838,10 +921,36 @@ int fest_get_next_solution(fest_handle *h, int current_iter, double cfl) {
   if (!h) return FEST_E_ARG;
   if (!h->have_grid || !h->have_state) return fail(h, FEST_E_ARG, "grid and state must be set first");
   h->b.CFLv = cfl;
-  int rc;
-  for (const StepOp &op : step_plan(h->b.time_acc))
-    if ((rc = run_op(h, op, current_iter))) return rc;
+  // kernels see current_iter only through current_iter <= 2 (the inflow
+  // fix, src/boundary/bc_primitive.f90:236), so one value per class
+  const int itn = current_iter <= 2 ? 1 : 3;
+  auto step = [h, itn]() -> int {
+    int rc;
+    for (const StepOp &op : step_plan(h->b.time_acc))
+      if ((rc = run_op(h, op, itn))) return rc;
+    return 0;
+  };
+  int rc = run_graphed(h->graphs, h->stream, current_iter, cfl, h->b.qp, h->timed != 0, step, h->err);
+  if (rc) return rc;
+  if (++h->unchecked < h->check_every) return 0;
+  h->unchecked = 0;
   return check_err(h);
+}
+
+int fest_set_option(fest_handle *h, const char *name, long value) {
+  if (!h || !name) return FEST_E_ARG;
+  if (!strcmp(name, "graphs")) {
+    h->graphs.reset();
+    h->graphs.enabled = value != 0;
+    return 0;
+  }
+  if (!strcmp(name, "check_interval")) {
+    if (value < 1) return fail(h, FEST_E_ARG, "check_interval must be >= 1");
+    h->check_every = (int)value;
+    h->unchecked = 0;
+    return 0;
+  }
+  return fail(h, FEST_E_ARG, "unknown option '%s'", name);
 }
 
 int fest_download(fest_handle *h, uint32_t what, double *qp, double *temp, double *residue, double *F, double *G,
@@ -902,8 +1011,9 @@ int fest_resnorm(fest_handle *h, double *res_abs) {
   hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(64), 0, h->stream, b, h->norm_blocks);
   LAUNCHCHK(h);
   HIPCHK(h, hipMemcpyAsync(res_abs, b.norms, sizeof(double) * (b.nv + 1), hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  return 0;
+  // the norms need a wait anyway: deferred device errors are reported here
+  h->unchecked = 0;
+  return check_err(h);
 }
 
 int fest_device_ptr(fest_handle *h, const char *name, void **ptr, size_t *count) {
@@ -1086,6 +1196,7 @@ int fest_group_create(fest_group **out, fest_handle *const *blocks, int nblocks,
 void fest_group_destroy(fest_group *g) {
   if (!g) return;
   if (g->stream) (void)hipStreamSynchronize(g->stream);
+  g->graphs.reset();
   for (fest_handle *h : g->blocks) {
     h->stream = h->own_stream;
     h->group = nullptr;
@@ -1148,10 +1259,36 @@ int fest_group_get_next_solution(fest_group *g, int current_iter, double cfl) {
   for (fest_handle *h : g->blocks)
     if (!h->have_grid || !h->have_state) return gfail(g, FEST_E_ARG, "grid and state must be set on every block");
   for (fest_handle *h : g->blocks) h->b.CFLv = cfl;
-  int rc;
-  for (const StepOp &op : step_plan(g->blocks[0]->b.time_acc))
-    if ((rc = run_group_op(g, op, current_iter))) return rc;
+  const int itn = current_iter <= 2 ? 1 : 3;
+  bool timed = false;
+  for (fest_handle *h : g->blocks) timed |= h->timed != 0;
+  auto step = [g, itn]() -> int {
+    int rc;
+    for (const StepOp &op : step_plan(g->blocks[0]->b.time_acc))
+      if ((rc = run_group_op(g, op, itn))) return rc;
+    return 0;
+  };
+  int rc = run_graphed(g->graphs, g->stream, current_iter, cfl, g->blocks[0]->b.qp, timed, step, g->err);
+  if (rc) return rc;
+  if (++g->unchecked < g->check_every) return 0;
+  g->unchecked = 0;
   return group_check(g);
+}
+
+int fest_group_set_option(fest_group *g, const char *name, long value) {
+  if (!g || !name) return FEST_E_ARG;
+  if (!strcmp(name, "graphs")) {
+    g->graphs.reset();
+    g->graphs.enabled = value != 0;
+    return 0;
+  }
+  if (!strcmp(name, "check_interval")) {
+    if (value < 1) return gfail(g, FEST_E_ARG, "check_interval must be >= 1");
+    g->check_every = (int)value;
+    g->unchecked = 0;
+    return 0;
+  }
+  return gfail(g, FEST_E_ARG, "unknown option '%s'", name);
 }
 
 namespace {
@@ -1169,7 +1306,8 @@ int run_group_op(fest_group *g, const StepOp &op, int current_iter) {
 int fest_sync(fest_handle *h) {
   if (!h) return FEST_E_ARG;
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  return 0;
+  h->unchecked = 0;
+  return check_err(h);
 }
 
 }  // extern "C"

@@ -94,7 +94,10 @@ def lib():
         L.fest_group_get_next_solution.argtypes = [vp, ip, ctypes.c_double]
         L.fest_timing.argtypes = [vp, ctypes.c_char_p]
         L.fest_kernel_time.argtypes = [vp, ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_long)]
-        for f in ("fest_create", "fest_upload_geometry", "fest_set_grid", "fest_upload_state",
+        L.fest_set_option.argtypes = [vp, ctypes.c_char_p, ctypes.c_long]
+        L.fest_group_set_option.argtypes = [vp, ctypes.c_char_p, ctypes.c_long]
+        for f in ("fest_set_option", "fest_group_set_option", "fest_create", "fest_upload_geometry",
+                  "fest_set_grid", "fest_upload_state",
                   "fest_get_next_solution", "fest_residual", "fest_time_step", "fest_download", "fest_resnorm",
                   "fest_device_ptr", "fest_sync", "fest_timing", "fest_kernel_time", "fest_group_create",
                   "fest_group_exchange", "fest_group_wall_dist", "fest_group_resnorm",
@@ -214,6 +217,10 @@ class GpuBlock:
     def sync(self):
         self._check(lib().fest_sync(self.h))
 
+    def set_option(self, name: str, value: int):
+        """fest_set_option: "graphs" (0/1), "check_interval" (n >= 1)."""
+        self._check(lib().fest_set_option(self.h, name.encode(), int(value)))
+
     KERNEL_GROUPS = ("step_begin", "bc_face", "bc_edges", "grad", "grad_bc", "flux_i", "flux_j", "flux_k",
                      "time_step", "update", "lusgs_diag", "lusgs_sweep", "lusgs_apply")
 
@@ -311,6 +318,10 @@ class GpuGroup:
         out = np.zeros(8)
         self._check(lib().fest_group_resnorm(self.g, _ptr(out)))
         return out[: self.blocks[0].case.n_var + 1]
+
+    def set_option(self, name: str, value: int):
+        """fest_group_set_option: "graphs" (0/1), "check_interval" (n >= 1)."""
+        self._check(lib().fest_group_set_option(self.g, name.encode(), int(value)))
 
     def get_next_solution(self, current_iter: int, cfl: float = None):
         c = self.blocks[0].case.CFL if cfl is None else cfl

@@ -7,13 +7,12 @@
       per-kernel VALU and wait figures from --pmc runs with SQ_WAVES,
       SQ_INSTS_VALU, SQ_WAVE_CYCLES, SQ_WAIT_INST_ANY, SQ_INSTS_VMEM_RD,
       GRBM_GUI_ACTIVE (and SQC_ICACHE_REQ/HITS).
-  python -m festgpu.profsum traffic --fetch DIR --write DIR
-                 --probe-fetch DIR --probe-write DIR --case NAME
+  python -m festgpu.profsum traffic --fetch DIR --write DIR --case NAME
       HBM bytes per launch of each kernel from two counter runs of the
       engine (--pmc FETCH_SIZE, --pmc WRITE_SIZE; the counters are KiB per
-      dispatch) and the same two on copy_probe, whose known traffic
-      (512 MiB read, 512 MiB written per launch) gives the scale factors
-      applied to the engine's counters.  The table rows are
+      dispatch), scaled by what the same counters read for the engine's
+      k_step_begin, whose traffic is known exactly (the state read, Temp
+      and the RK base written).  The table rows are
       `| group | kernel | fetch B | write B | traffic B |`.
 
 Run from fest-3d_amd/ (or with it on sys.path).
@@ -28,7 +27,6 @@ from collections import defaultdict
 from . import cases as C
 from . import roofline as R
 
-PROBE_BYTES = 512 * 2 ** 20  # copy_probe: read and written per launch
 
 GROUPS = [  # (substring of the demangled kernel name, kernel group)
     ("k_flux<0", "flux_i"), ("k_flux<1", "flux_j"), ("k_flux<2", "flux_k"),
@@ -37,7 +35,7 @@ GROUPS = [  # (substring of the demangled kernel name, kernel group)
     ("k_dt_min", "time_step"), ("k_dt_fill", "time_step"), ("k_dt", "time_step"),
     ("k_norm", "resnorm"), ("k_node_dist", "setup"), ("k_cell_dist", "setup"), ("k_cells", "setup"),
     ("k_face_metrics", "setup"), ("k_ghost_nodes", "setup"), ("k_fill", "setup"),
-    ("k_copy_probe", "copy_probe"), ("k_lusgs_sweep", "lusgs_sweep"), ("k_lusgs_prep", "lusgs_diag"),
+("k_lusgs_sweep", "lusgs_sweep"), ("k_lusgs_prep", "lusgs_diag"),
     ("k_to_skew", "lusgs_diag"), ("k_lusgs_apply", "lusgs_apply"), ("k_exchange", "exchange"),
 ]
 
@@ -90,21 +88,42 @@ def counters(d, counter):
     return {k: sum(v) / len(v) for k, v in acc.items()}
 
 
-def traffic(fetch_dir, write_dir, probe_fetch, probe_write, case_name):
+def counter_max(d, counter, kernel):
+    """Largest counter value of one dispatch of `kernel`."""
+    best = None
+    for path in _csvs(d, "counter_collection.csv"):
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if r.get("Counter_Name") == counter and short(r["Kernel_Name"]) == kernel:
+                    v = float(r["Counter_Value"])
+                    best = v if best is None else max(best, v)
+    return best
+
+
+def traffic(fetch_dir, write_dir, case_name):
+    """HBM bytes per launch of every kernel.  The counters' scale comes from
+    the engine's own k_step_begin at the start of an RK4 step, whose traffic
+    is exact: it streams the n_var state planes in and writes Temp and the
+    n_var RK-base planes (8 B per value, nc padded cells per plane).  Its
+    largest dispatch is that one (the state upload launches it without the
+    RK-base copy)."""
     fe, wr = counters(fetch_dir, "FETCH_SIZE"), counters(write_dir, "WRITE_SIZE")
-    pf, pw = counters(probe_fetch, "FETCH_SIZE"), counters(probe_write, "WRITE_SIZE")
-    kf = PROBE_BYTES / (1024.0 * pf["k_copy_probe"])
-    kw = PROBE_BYTES / (1024.0 * pw["k_copy_probe"])
     workload, size = case_name.rsplit("_", 1)
     case = C.WORKLOADS[workload](*(int(v) for v in size.split("x")))[0]
-    lines = [f"copy_probe: FETCH_SIZE {pf['k_copy_probe']:.0f} KiB and WRITE_SIZE {pw['k_copy_probe']:.0f} KiB "
-             f"per launch for {PROBE_BYTES} B read and written: scale factors fetch {kf:.4f}, write {kw:.4f}",
+    nc = (case.imx + 5) * (case.jmx + 5) * (case.kmx + 5)
+    known_f, known_w = 8.0 * nc * case.n_var, 8.0 * nc * (case.n_var + 1)
+    pf = counter_max(fetch_dir, "FETCH_SIZE", "k_step_begin")
+    pw = counter_max(write_dir, "WRITE_SIZE", "k_step_begin")
+    kf = known_f / (1024.0 * pf)
+    kw = known_w / (1024.0 * pw)
+    lines = [f"k_step_begin (RK-base copy): FETCH_SIZE {pf:.0f} KiB and WRITE_SIZE {pw:.0f} KiB per launch for "
+             f"{known_f:.0f} B read and {known_w:.0f} B written: scale factors fetch {kf:.4f}, write {kw:.4f}",
              "",
              "| group | kernel | fetch B | write B | traffic B |", "|---|---|---|---|---|"]
     extra = ["", "| group | kernel | algorithmic B | traffic / algorithmic |", "|---|---|---|---|"]
     for k in sorted(fe, key=lambda k: -fe[k]):
         grp = group_of(k)
-        if grp in ("copy_probe", "-"):
+        if grp == "-":
             continue
         f_b = fe[k] * 1024.0 * kf
         w_b = wr.get(k, 0.0) * 1024.0 * kw
@@ -130,7 +149,7 @@ def sq(d):
              "|---|---|---|---|---|---|---|---|"]
     for k in sorted(acc):
         grp = group_of(k)
-        if grp in ("setup", "-", "copy_probe", "resnorm"):
+        if grp in ("setup", "-", "resnorm"):
             continue
         m = {c: sum(v) / len(v) for c, v in acc[k].items()}
         waves = m.get("SQ_WAVES", 0.0)
@@ -159,8 +178,6 @@ def main(argv=None):
     t = sub.add_parser("traffic")
     t.add_argument("--fetch", required=True)
     t.add_argument("--write", required=True)
-    t.add_argument("--probe-fetch", required=True)
-    t.add_argument("--probe-write", required=True)
     t.add_argument("--case", required=True)
     a = ap.parse_args(argv)
     if a.cmd == "stats":
@@ -168,7 +185,7 @@ def main(argv=None):
     elif a.cmd == "sq":
         print("\n\n".join(sq(d) for d in a.dir))
     else:
-        print(traffic(a.fetch, a.write, a.probe_fetch, a.probe_write, a.case))
+        print(traffic(a.fetch, a.write, a.case))
 
 
 if __name__ == "__main__":

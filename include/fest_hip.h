@@ -189,8 +189,24 @@ int fest_device_ptr(fest_handle *h, const char *name, void **ptr, size_t *count)
 int fest_timing(fest_handle *h, const char *groups);
 int fest_kernel_time(fest_handle *h, const char *group, double *total_ms, long *launches);
 
-/* Synchronise the handle's stream. */
+/* Synchronise the handle's stream and report a deferred device error
+ * (FEST_E_NONPHYSICAL / FEST_E_NAN raised since the last check). */
 int fest_sync(fest_handle *h);
+
+/* Run-time options (no counterpart in the reference):
+ *   "graphs"          1 (default): fest_get_next_solution replays the time
+ *                     step as a captured HIP graph, one launch per step
+ *                     (captured on the second call of each inflow-fix class,
+ *                     iterations <= 2 and later; re-captured when the CFL
+ *                     number changes); 0: launch kernel by kernel.  Steps
+ *                     with kernel timing on (fest_timing) run kernel by
+ *                     kernel.
+ *   "check_interval"  n >= 1 (default 1): read the device's error word after
+ *                     every n-th fest_get_next_solution, so that the host
+ *                     waits for the device only then.  With n > 1 an error
+ *                     is reported by the next check, fest_resnorm or
+ *                     fest_sync; the reference stops at once (n = 1). */
+int fest_set_option(fest_handle *h, const char *name, long value);
 
 /* ---- multiblock grids
  * One interface link (the reference's interface map, src/mapping.f90:64-258):
@@ -240,6 +256,9 @@ int fest_group_resnorm(fest_group *g, double *res_abs);
  * the interface exchange opening every residual evaluation
  * (src/update.f90:534). */
 int fest_group_get_next_solution(fest_group *g, int current_iter, double cfl);
+/* fest_set_option's "graphs" and "check_interval" for a group's step: the
+ * exchange and every block's kernels of one step form one graph. */
+int fest_group_set_option(fest_group *g, const char *name, long value);
 
 #ifdef __cplusplus
 }

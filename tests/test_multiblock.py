@@ -28,7 +28,7 @@ import dataclasses
 import numpy as np
 import pytest
 
-from binding import Oracle, group_exchange, group_step
+from binding import Oracle, group_exchange, group_step, group_wall_dist
 from festgpu import cases as C
 from festgpu import multiblock as M
 
@@ -229,3 +229,26 @@ def test_gpu_group_rejects_bad_map():
     gs = gpu_blocks(blocks)
     with pytest.raises(FestError):
         GpuGroup(gs, [links[0]._replace(lo2=2)])
+
+
+def test_threaded_group_step_equals_serial():
+    """orc_group_step_mt (the CPU baseline's one-thread-per-block run) gives
+    the serial group step's state bit for bit: the blocks share nothing."""
+    from binding import group_step_mt
+    case, nodes, q = C.sst_bump(24, 10, 6)
+    blocks, links = M.split_i(case, nodes, q, [7, 13, 19])
+    runs = []
+    for mt in (False, True):
+        os_ = []
+        for c, n, qb in blocks:
+            o = Oracle(c)
+            o.set_nodes(n)
+            o.set_state(qb)
+            os_.append(o)
+        assert group_wall_dist(os_) == 0
+        for it in range(1, 4):
+            st = group_step_mt(os_, links, it, 3) if mt else group_step(os_, links, it)
+            assert st == 0
+        runs.append([o.arr("qp").copy() for o in os_])
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)

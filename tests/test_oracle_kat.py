@@ -269,6 +269,41 @@ def test_time_step_uniform_closed_form(viscous):
         np.testing.assert_allclose(o.arr("dt"), 0.7 * (2 / (ls + 2 * 0.7 * V / dt_inv)) * V, rtol=1e-13)
 
 
+def test_time_step_uniform_closed_form_sst():
+    """add_turbulent_time (src/time.f90:450-531) on a uniform SST state of a
+    uniform periodic grid: every gradient is zero, so the vorticity is zero
+    and the eddy viscosity is rho a1 k / (a1 omega) = rho k / omega exactly
+    (viscosity.f90:343-388); the turbulent pass repeats the viscous one
+    with mu_t and Pr_t on the viscous time step.  Pr != Pr_t and
+    mu != mu_t, so a swap of either fails."""
+    mu, Pr, tPr, h = 2e-3, 0.72, 0.9, 0.01
+    case = tiny_case(6, 5, 5, bc_id=(-9,) * 6, time_local=1, CFL=0.7, flux="ausmP", turb="sst",
+                     mu_ref=mu, Pr=Pr, tPr=tPr, tk_inf=500.0, tw_inf=600.0, lim=(1, 1, 1), tlim=(1, 1, 1))
+    o = Oracle(case)
+    o.set_nodes(C.cartesian_nodes(case, spacing=(h, h, h)))
+    o.arr("dist")[...] = 1.0
+    rho, u, v, w, p, k, om = 1.2, 30.0, -20.0, 10.0, 1.0e5, 500.0, 600.0
+    q = o.arr("qp")
+    for l, val in enumerate((rho, u, v, w, p, k, om)):
+        q[l] = val
+    o.temp_refresh()
+    assert o.residual(1) == 0
+    inner = (slice(3, -3),) * 3
+    mut = rho * k / om
+    np.testing.assert_allclose(o.arr("mut")[inner], mut, rtol=1e-14)
+    o.time_step()
+    c = math.sqrt(1.4 * p / rho)
+    dt_inv = 0.7 * h / (2 * (abs(u) + abs(v) + abs(w)) + 6 * c)
+    V = h ** 3
+    ls = 1.4 * (6 * h * h * mu / (rho * h)) / Pr
+    dt_v = 0.7 * (2 / (ls + 2 * 0.7 * V / dt_inv)) * V
+    lt = 1.4 * (6 * h * h * mut / (rho * h)) / tPr
+    dt_t = 0.7 * (2 / (lt + 2 * 0.7 * V / dt_v)) * V
+    np.testing.assert_allclose(o.arr("dt"), dt_t, rtol=1e-13)
+    # the pass changes the step: mu_t is not negligible here
+    assert abs(dt_t - dt_v) / dt_v > 1e-2
+
+
 # ----------------------------------------------------------- invariants
 @pytest.mark.parametrize("scheme", list(FLUX_KAT))
 @pytest.mark.parametrize("recon", ["none", "muscl", "ppm", "weno"])
