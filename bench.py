@@ -74,7 +74,7 @@ def cpu_baseline(case_fn, ni, nj, nk, full_dist, budget_s=20.0, implicit=False):
     here only as the baseline) timed on the host's cores on bounded samples
     of the same workload, in the two modes of the reference's own CPU runs:
 
-      multi-core: the grid cut along i into one block per core, one thread
+      multi-core: the grid cut along k into one block per core, one thread
         per block, the 3-layer interface exchange at every stage (the
         reference's MPI decomposition, one rank per block);
       single-core: the first 64 i-cells of the grid as one block on one core
@@ -147,20 +147,23 @@ def cpu_baseline(case_fn, ni, nj, nk, full_dist, budget_s=20.0, implicit=False):
         out["single_core"] = single
         out["reference_modules"] = ref
         return out
-    ncores = max(1, min(16, os.cpu_count() or 1, (ni - 1) // 4))
+    # cut along k, the uniform spanwise direction (its periodic faces become
+    # a cross-block periodic pair): extrapolated ghost nodes at the cuts
+    # coincide with the neighbours' nodes there
+    ncores = max(1, min(16, os.cpu_count() or 1, (nk - 1) // 4))
     case, nodes, q = case_fn(ni, nj, nk)
-    step_i = ni // ncores
-    cuts = [1 + step_i * b for b in range(1, ncores)]
-    blocks, links = M.split_i(case, nodes, q, cuts)
+    step_k = nk // ncores
+    cuts = [1 + step_k * b for b in range(1, ncores)]
+    blocks, links = M.split(case, nodes, q, 2, cuts)
     os_ = []
     for b, (cb, nb_, qb) in enumerate(blocks):
         ob = Oracle(cb)
         ob.set_nodes(nb_)
         if full_dist is not None:
-            # the block's cells -2..imx_b+2 are the grid's cells s-3..; padded
+            # the block's cells -2..kmx_b+2 are the grid's cells s-3..; padded
             # index = cell index + 2
             s0 = ([1] + cuts)[b]
-            ob.arr("dist")[...] = full_dist[:, :, s0 - 1: s0 - 1 + cb.imx + 5]
+            ob.arr("dist")[...] = full_dist[s0 - 1: s0 - 1 + cb.kmx + 5]
         ob.set_state(qb)
         os_.append(ob)
     t0 = time.perf_counter()
@@ -182,7 +185,7 @@ def cpu_baseline(case_fn, ni, nj, nk, full_dist, budget_s=20.0, implicit=False):
         print(f"cpu_baseline: multi-core oracle status {st} at step {steps + 1}", file=sys.stderr)
         return out
     out.update({"value": case.ncells * per * steps / dtm, "unit": unit, "cores": ncores, "kind": "port",
-                "sample": f"{steps} {what} step(s) of the whole {ni}x{nj}x{nk}-cell workload cut along i into "
+                "sample": f"{steps} {what} step(s) of the whole {ni}x{nj}x{nk}-cell workload cut along k into "
                           f"{ncores} blocks, one thread per block, interface exchange every stage; "
                           f"oracle/fest_oracle.c, {dtm:.1f} s"})
     return out

@@ -44,6 +44,12 @@ struct Blk {
   long sk_p, sk_n;            // positions per hyperplane; hyperplanes x sk_p
   double *sk;                 // 3 nv + 23 components x sk_n
   double *g[3];
+  // FP32 mode: FP32 copies of the padded state (refreshed after the ghost
+  // cells of every residual evaluation) and of the gradients, read by the
+  // face sweeps; g32only: the FP64 gradients are not written
+  float *q32;
+  float *g32[3];
+  int g32only;
   double *nodes;              // grid nodes (3 planes) when set on device
   double *red;                // reduction partials
   double *norms;              // n_var+1 absolute norms; from 8, the block's unassembled share

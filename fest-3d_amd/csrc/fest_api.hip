@@ -289,6 +289,10 @@ int launch_residual_front(fest_handle *h, int current_iter, int store_csnd) {
     hipLaunchKernelGGL(k_bc_edges, dim3(cdiv(4L * (b.kmx + 5) * b.nv, 256)), dim3(256), 0, st, b, 1);
     hipLaunchKernelGGL(k_bc_edges, dim3(1), dim3(64), 0, st, b, 2);
   }
+  if (b.fp32) {
+    KgScope ts(h, KG_BC_EDGES);
+    hipLaunchKernelGGL(k_to_f32, dim3(cdiv(b.nc * b.nv, 256)), dim3(256), 0, st, b);
+  }
   LAUNCHCHK(h);
   dim3 g, t;
   if (b.mu_ref != 0.0) {
@@ -740,6 +744,11 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
   b.tkl_inf = flow->tkl_inf;
   b.trans = scheme->transition;
   b.fp32 = precision == FEST_PREC_FP32_FLUX;
+  // FP32 mode keeps only FP32 gradients where nothing but the ghost rule
+  // and the face sweeps read them (no SA/k-kL source or LU-SGS diagonal)
+  b.g32only = b.fp32 && scheme->time_accuracy != FEST_TA_IMPLICIT &&
+              (scheme->turb == FEST_TURB_NONE || scheme->turb == FEST_TURB_SST ||
+               scheme->turb == FEST_TURB_SST2003);
   b.tu_inf = flow->tu_inf;
   b.Re = flow->density_inf * flow->vel_mag * 1.0 / flow->mu_ref;  // state.f90:89
   b.c2 = 1 + scheme->accur;
@@ -768,6 +777,14 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
     AL(b.fA[d], b.nfc[d]); AL(b.fx[d], b.nfc[d]); AL(b.fy[d], b.nfc[d]); AL(b.fz[d], b.nfc[d]);
     AL(b.flux[d], b.nfl[d] * b.nv); AL(b.csnd[d], b.nfl[d]);
     AL(b.g[d], b.ngr * b.ng);
+  }
+  if (b.fp32) {
+    // FP32 copies: (nc * nv) state values and 3 x (ngr * ng) gradients
+    double *q32 = nullptr, *g32 = nullptr;
+    AL(q32, (b.nc * b.nv + 1) / 2);
+    AL(g32, (3 * b.ngr * b.ng + 1) / 2);
+    b.q32 = (float *)q32;
+    for (int d = 0; d < 3; ++d) b.g32[d] = (float *)g32 + (long)d * b.ngr * b.ng;
   }
   AL(b.res, b.nres * b.nv); AL(b.dt, b.nres); AL(b.src, 2 * b.nres); AL(b.ust, b.nc * b.nv); AL(b.rst, b.nres * b.nv);
   AL(b.nodes, 3L * (b.imx + 6) * (b.jmx + 6) * (b.kmx + 6));
@@ -980,6 +997,10 @@ int fest_download(fest_handle *h, uint32_t what, double *qp, double *temp, doubl
 #undef DL
   if (what & FEST_DL_GRADS) {
     if (!grads) return FEST_E_ARG;
+    if (b.g32only) {
+      hipLaunchKernelGGL(k_g32_widen, dim3(cdiv(b.ngr * b.ng, 256)), dim3(256), 0, st, b);
+      LAUNCHCHK(h);
+    }
     for (int d = 0; d < 3; ++d)
       if ((rc = cp(grads + (size_t)d * b.ngr * b.ng, b.g[d], (size_t)(b.ngr * b.ng)))) return rc;
   }
@@ -1008,7 +1029,7 @@ int fest_resnorm(fest_handle *h, double *res_abs) {
   if (!h || !res_abs) return FEST_E_ARG;
   Blk &b = h->b;
   hipLaunchKernelGGL(k_norm_partial, dim3(h->norm_blocks), dim3(256), 0, h->stream, b);
-  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(64), 0, h->stream, b, h->norm_blocks);
+  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, h->stream, b, h->norm_blocks);
   LAUNCHCHK(h);
   HIPCHK(h, hipMemcpyAsync(res_abs, b.norms, sizeof(double) * (b.nv + 1), hipMemcpyDeviceToHost, h->stream));
   // the norms need a wait anyway: deferred device errors are reported here
@@ -1239,7 +1260,7 @@ int fest_group_resnorm(fest_group *g, double *res_abs) {
   for (fest_handle *h : g->blocks) {
     Blk &b = h->b;
     hipLaunchKernelGGL(k_norm_partial, dim3(h->norm_blocks), dim3(256), 0, h->stream, b);
-    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(64), 0, h->stream, b, h->norm_blocks);
+    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, h->stream, b, h->norm_blocks);
     if (hipGetLastError() != hipSuccess) return gfail(g, FEST_E_HIP, "norm launch failed");
     if (hipMemcpyAsync(part.data(), b.norms + 8, sizeof(double) * (nv + 1), hipMemcpyDeviceToHost, h->stream) !=
             hipSuccess ||

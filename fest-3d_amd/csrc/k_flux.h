@@ -23,12 +23,19 @@
 // the unfused reference order by rounding only, far inside the 1e-12 parity
 // bound, and the FP64 instruction count, which bounds these kernels, drops
 #pragma clang fp contract(fast)
+#include <type_traits>
+
 #include "blk.h"
 
 namespace fest {
 
+// FP64: the IEEE operations.  FP32 (the mixed-precision mode): the
+// hardware reciprocal and square root (v_rcp_f32, v_sqrt_f32, 1 ulp), whose
+// error is far below the FP32 face arithmetic's own.
 __device__ __forceinline__ double tsqrt(double x) { return sqrt(x); }
-__device__ __forceinline__ float tsqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ float tsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double tdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float tdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ double tabs(double x) { return fabs(x); }
 __device__ __forceinline__ float tabs(float x) { return __builtin_fabsf(x); }
 
@@ -42,10 +49,10 @@ struct FaceMath {
     const T alpha = T(2.) / T(3.), phi = T(1.0), kappa = T(1.) / T(3.), eps = T(1e-14);
     T fd = up - u0;
     T bd = u0 - um;
-    T r = fd / (bd + fsgn(eps, bd));
-    T psi1 = valb ? ((r * r) + r) / ((r * r) + T(1.0)) : fmx(T(0.), fmn(fmn(2 * r, alpha * (r - T(1.0)) + T(1.0)), T(2.)));
-    r = bd / (fd + fsgn(eps, fd));
-    T psi2 = valb ? ((r * r) + r) / ((r * r) + T(1.0)) : fmx(T(0.), fmn(fmn(2 * r, alpha * (r - T(1.0)) + T(1.0)), T(2.)));
+    T r = tdiv(fd, bd + fsgn(eps, bd));
+    T psi1 = valb ? tdiv((r * r) + r, (r * r) + T(1.0)) : fmx(T(0.), fmn(fmn(2 * r, alpha * (r - T(1.0)) + T(1.0)), T(2.)));
+    r = tdiv(bd, fd + fsgn(eps, fd));
+    T psi2 = valb ? tdiv((r * r) + r, (r * r) + T(1.0)) : fmx(T(0.), fmn(fmn(2 * r, alpha * (r - T(1.0)) + T(1.0)), T(2.)));
     psi1 = (1 - (1 - psi1) * sw);
     psi2 = (1 - (1 - psi2) * sw);
     upv = u0 + T(0.25) * phi * (((T(1.) - kappa) * psi1 * bd) + ((T(1.) + kappa) * psi2 * fd));
@@ -57,10 +64,10 @@ struct FaceMath {
     const T phi = T(1.0), kappa = T(1.) / T(3.);
     T fd = up - u0;
     T bd = u0 - um;
-    T r = fd / bd;
+    T r = tdiv(fd, bd);
     T psi1 = fmx(T(0.), fmn(fmn(2 * r, (2 + r) / T(3.)), T(2.)));
     psi1 = (1 - (1 - psi1) * sw);
-    r = bd / fd;
+    r = tdiv(bd, fd);
     T psi2 = fmx(T(0.), fmn(fmn(2 * r, (2 + r) / T(3.)), T(2.)));
     psi2 = (1 - (1 - psi2) * sw);
     upv = u0 + T(0.25) * phi * (((T(1.) - kappa) * psi1 * bd) + ((T(1.) + kappa) * psi2 * fd));
@@ -79,10 +86,10 @@ struct FaceMath {
     const T g1 = T(1.0) / T(10.0), g2 = T(6.0) / T(10.0), g3 = T(3.0) / T(10.0);
     if constexpr (sizeof(T) == sizeof(double)) {
       T e1 = g1 * (D2 * D3), e2 = g2 * (D1 * D3), e3 = g3 * (D1 * D2);
-      return (e1 * Q1 + e2 * Q2 + e3 * Q3) / (T(6.0) * (e1 + e2 + e3));
+      return tdiv(e1 * Q1 + e2 * Q2 + e3 * Q3, T(6.0) * (e1 + e2 + e3));
     } else {
-      T w1 = g1 / D1, w2 = g2 / D2, w3 = g3 / D3;
-      return (w1 * Q1 + w2 * Q2 + w3 * Q3) / (T(6.0) * (w1 + w2 + w3));
+      T w1 = tdiv(g1, D1), w2 = tdiv(g2, D2), w3 = tdiv(g3, D3);
+      return tdiv(w1 * Q1 + w2 * Q2 + w3 * Q3, T(6.0) * (w1 + w2 + w3));
     }
   }
   static __device__ __forceinline__ void weno_betas(T Um2, T Um1, T U0, T Up1, T Up2, T &D1, T &D2, T &D3) {
@@ -170,24 +177,25 @@ struct FaceMath {
   // pdif of the cells n-1 and n from the pressure stencil p(m) = cell
   // n-3+m; the ghost cells 0 and N take the first and last interior
   // cell's value
-  static __device__ __forceinline__ void pb_factors(const Blk &b, int n, int N, const double *p, long s, Aux &ax) {
+  template <typename Q>
+  static __device__ __forceinline__ void pb_factors(const Blk &b, int n, int N, const Q *p, long s, Aux &ax) {
     auto w = [p, s](int m) { return T(p[(long)(m - 3) * s]); };
     const T pin = T(b.p_inf);
     T pd2 = (n == 1) ? tabs(w(4) - w(2)) : tabs(w(3) - w(1));
-    ax.pdL = T(1) - (pd2 / (pd2 + pin));
+    ax.pdL = T(1) - tdiv(pd2, pd2 + pin);
     pd2 = (n == N) ? tabs(w(3) - w(1)) : tabs(w(4) - w(2));
-    ax.pdR = T(1) - (pd2 / (pd2 + pin));
+    ax.pdR = T(1) - tdiv(pd2, pd2 + pin);
   }
   static __device__ __forceinline__ void nm_blends(const double *vol, long s, Aux &ax) {
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
       const T v1 = T(vol[(long)(m - 2) * s]), v0 = T(vol[(long)(m - 3) * s]);
-      ax.A[m] = v1 / (v0 + v1);
+      ax.A[m] = tdiv(v1, v0 + v1);
     }
   }
 
-  template <int RECON>
-  static __device__ __forceinline__ void face_states(const Blk &b, int d, int n, int N, int l, const double *u,
+  template <int RECON, typename Q>
+  static __device__ __forceinline__ void face_states(const Blk &b, int d, int n, int N, int l, const Q *u,
                                                      long s, const Aux &ax, T &L, T &R) {
     constexpr int recon = RECON;
     // cell n-3+m, read where it is used
@@ -259,22 +267,25 @@ struct FaceMath {
   static __device__ __forceinline__ void inviscid_flux(const Blk &b, const T (&qL)[NV], const T (&qR)[NV], T nx, T ny,
                                                        T nz, T A, T zero, T (&out)[NV]) {
     const T gm = T(b.gm);
+    // gamma/(gamma-1): the reference forms (gm/(gm-1))*p/rho, i.e. the
+    // product first, then the quotient by rho
+    const T gmr = gm / (gm - T(1.));
     constexpr int nv = NV;
     T rL = qL[0], uL = qL[1], vL = qL[2], wL = qL[3], pL = qL[4];
     T rR = qR[0], uR = qR[1], vR = qR[2], wR = qR[3], pR = qR[4];
     const int sch = b.flux_scheme;
     if (sch == 1 || sch == 2 || sch == 3) {
-      T HL = (T(0.5) * (uL * uL + vL * vL + wL * wL)) + ((gm / (gm - T(1.))) * pL / rL);
-      T HR = (T(0.5) * (uR * uR + vR * vR + wR * wR)) + ((gm / (gm - T(1.))) * pR / rR);
+      T HL = (T(0.5) * (uL * uL + vL * vL + wL * wL)) + tdiv(gmr * pL, rL);
+      T HR = (T(0.5) * (uR * uR + vR * vR + wR * wR)) + tdiv(gmr * pR, rR);
       T VnL = uL * nx + vL * ny + wL * nz;
       T VnR = uR * nx + vR * ny + wR * nz;
       T mass, pbar;
       if (sch == 1) {  // AUSM+, ausmP.f90:105-158
-        T cs = tsqrt(T(2.0) * (gm - T(1.0)) * (T(0.5) * (HL + HR)) / (gm + T(1.0)));
-        T cL = cs * cs / (fmx(cs, tabs(VnL)));
-        T cR = cs * cs / (fmx(cs, tabs(VnR)));
+        T cs = tsqrt(tdiv(T(2.0) * (gm - T(1.0)) * (T(0.5) * (HL + HR)), gm + T(1.0)));
+        T cL = tdiv(cs * cs, fmx(cs, tabs(VnL)));
+        T cR = tdiv(cs * cs, fmx(cs, tabs(VnR)));
         T C = fmn(cL, cR);
-        const T iC = T(1.0) / C;  // one reciprocal for the quotients by C
+        const T iC = tdiv(T(1.0), C);  // one reciprocal for the quotients by C
         T ML = VnL * iC, MR = VnR * iC;
         T aL = alpha_sub(ML), aR = alpha_sub(MR);
         T FmL = (T(0.5) * (T(1.0) + fsgn(T(1.0), ML)) * (T(1.0) - aL) * ML) + aL * T(0.25) * sqd(T(1.0) + ML);
@@ -291,13 +302,13 @@ struct FaceMath {
       } else if (sch == 2) {  // AUSM+-up, ausmUP.f90:113-189 (uses the freestream Mach)
         const T Kp = T(0.25), Ku = T(0.75), sigma = T(1.0);
         const T MInf = T(b.MInf);
-        T cs = tsqrt(T(2.0) * (gm - T(1.0)) * (T(0.5) * (HL + HR)) / (gm + T(1.0)));
-        T cL = cs * cs / (fmx(cs, VnL));
-        T cR = cs * cs / (fmx(cs, -VnR));
+        T cs = tsqrt(tdiv(T(2.0) * (gm - T(1.0)) * (T(0.5) * (HL + HR)), gm + T(1.0)));
+        T cL = tdiv(cs * cs, fmx(cs, VnL));
+        T cR = tdiv(cs * cs, fmx(cs, -VnR));
         T C = fmn(cL, cR);
-        const T iC = T(1.0) / C;  // one reciprocal for the quotients by C
+        const T iC = tdiv(T(1.0), C);  // one reciprocal for the quotients by C
         T ML = VnL * iC, MR = VnR * iC;
-        T Mb = tsqrt(T(0.5) * ((VnL * VnL) + (VnR * VnR)) / (C * C));
+        T Mb = tsqrt(tdiv(T(0.5) * ((VnL * VnL) + (VnR * VnR)), C * C));
         T Mo = tsqrt(fmn(T(1.0), fmx(Mb * Mb, MInf * MInf)));
         T fna = Mo * (T(2.0) - Mo);
         T alfa = T(3.0) * (T(-4.0) + (T(5.0) * fna * fna)) / T(16.0);
@@ -311,15 +322,15 @@ struct FaceMath {
         FmR = FmR - aR * T(0.125) * sqd(MR * MR - T(1.0));
         betaR = betaR - aR * alfa * sqd(MR * MR - T(1.0)) * MR;
         T Pu = -Ku * betaL * betaR * (rL + rR) * fna * C * (VnR - VnL);
-        T Mp = T(-2.0) * Kp * fmx(T(1.0) - (sigma * Mb * Mb), T(0.0)) * (pR - pL) / (fna * (rL + rR) * C * C);
+        T Mp = tdiv(T(-2.0) * Kp * fmx(T(1.0) - (sigma * Mb * Mb), T(0.0)) * (pR - pL), fna * (rL + rR) * C * C);
         T Mface = FmL + FmR + Mp;
         pbar = betaL * pL + betaR * pR + Pu;
         mass = (Mface > T(0.0)) ? Mface * C * rL : Mface * C * rR;
       } else {  // SLAU, slau.f90:108-161
-        T cL = tsqrt(gm * pL / rL), cR = tsqrt(gm * pR / rR);
+        T cL = tsqrt(tdiv(gm * pL, rL)), cR = tsqrt(tdiv(gm * pR, rR));
         T C = T(0.5) * (cL + cR);
         T delp = pR - pL;
-        const T iC = T(1.0) / C;  // one reciprocal for the quotients by C
+        const T iC = tdiv(T(1.0), C);  // one reciprocal for the quotients by C
         T ML = VnL * iC, MR = VnR * iC;
         T aL = alpha_sub(ML), aR = alpha_sub(MR);
         T betaL = (T(1.0) - aL) * T(0.5) * (T(1.0) + fsgn(T(1.0), ML)) + (aL) * T(0.25) * (T(2.0) - ML) * sqd(ML + T(1.0));
@@ -327,7 +338,7 @@ struct FaceMath {
         T vt = tsqrt(T(0.5) * ((uL * uL) + (vL * vL) + (wL * wL) + (uR * uR) + (vR * vR) + (wR * wR)));
         T Mcap = fmn(T(1.0), vt * iC);
         T Xi = sqd(T(1.0) - Mcap);
-        T Vnabs = (rL * tabs(VnL) + rR * tabs(VnR)) / (rL + rR);
+        T Vnabs = tdiv(rL * tabs(VnL) + rR * tabs(VnR), rL + rR);
         T fnG = T(-1.0) * fmx(fmn(ML, T(0.0)), T(-1.0)) * fmn(fmx(MR, T(0.0)), T(1.0));
         pbar = T(0.5) * ((pL + pR) + (betaL - betaR) * (pL - pR) + (T(1.0) - Xi) * (betaL + betaR - T(1.0)) * (pL + pR));
         T VnabsL = (T(1.0) - fnG) * Vnabs + fnG * tabs(VnL);
@@ -346,15 +357,15 @@ struct FaceMath {
       return;
     }
     // AUSM (0), LDFSS(0) (4), van Leer (5): ausm.f90, ldfss0.f90, van_leer.f90
-    T ssa = T(0.5) * (tsqrt(gm * pL / rL) + tsqrt(gm * pR / rR));
-    T MpL = (uL * nx + vL * ny + wL * nz) / ssa;
+    T ssa = T(0.5) * (tsqrt(tdiv(gm * pL, rL)) + tsqrt(tdiv(gm * pR, rR)));
+    T MpL = tdiv(uL * nx + vL * ny + wL * nz, ssa);
     T ap = T(0.5) * (T(1.0) + fsgn(T(1.0), MpL));
     T bl = tabs(MpL) < T(1.0) ? T(-1.0) : T(0.0);
     T Mplus = T(0.25) * sqd(T(1.) + MpL);
     T Dplus = T(0.25) * sqd(T(1.) + MpL) * (T(2.) - MpL);
     T cplus = (ap * (T(1.0) + bl) * MpL) - bl * Mplus;
     T sDp = (ap * (T(1.) + bl)) - (bl * Dplus);
-    T MpR = (uR * nx + vR * ny + wR * nz) / ssa;
+    T MpR = tdiv(uR * nx + vR * ny + wR * nz, ssa);
     T am = T(0.5) * (T(1.0) - fsgn(T(1.0), MpR));
     T br = tabs(MpR) < T(1.0) ? T(-1.0) : T(0.0);
     T Mminus = T(-0.25) * sqd(T(1.) - MpR);
@@ -367,8 +378,8 @@ struct FaceMath {
       cminus = fmn(T(0.), tc);
     } else if (sch == 4) {
       T Ml = T(0.25) * bl * br * sqd(tsqrt((MpL * MpL + MpR * MpR) * T(0.5)) - 1);
-      T Mpl = Ml * (1 - (pL - pR) / (2 * rL * (ssa * ssa)));
-      T Mml = Ml * (1 - (pL - pR) / (2 * rR * (ssa * ssa)));
+      T Mpl = Ml * (1 - tdiv(pL - pR, 2 * rL * (ssa * ssa)));
+      T Mml = Ml * (1 - tdiv(pL - pR, 2 * rR * (ssa * ssa)));
       cplus = cplus - Mpl;
       cminus = cminus + Mml;
     }
@@ -380,11 +391,11 @@ struct FaceMath {
     Fp[1] = (Fp0 * uL) + (sDp * pL * nx);
     Fp[2] = (Fp0 * vL) + (sDp * pL * ny);
     Fp[3] = (Fp0 * wL) + (sDp * pL * nz);
-    Fp[4] = Fp0 * ((T(0.5) * (uL * uL + vL * vL + wL * wL)) + ((gm / (gm - T(1.))) * pL / rL));
+    Fp[4] = Fp0 * ((T(0.5) * (uL * uL + vL * vL + wL * wL)) + tdiv(gmr * pL, rL));
     Fm[1] = (Fm0 * uR) + (sDm * pR * nx);
     Fm[2] = (Fm0 * vR) + (sDm * pR * ny);
     Fm[3] = (Fm0 * wR) + (sDm * pR * nz);
-    Fm[4] = Fm0 * ((T(0.5) * (uR * uR + vR * vR + wR * wR)) + ((gm / (gm - T(1.))) * pR / rR));
+    Fm[4] = Fm0 * ((T(0.5) * (uR * uR + vR * vR + wR * wR)) + tdiv(gmr * pR, rR));
 #pragma unroll
     for (int l = 0; l < 5; ++l) out[l] = Fp[l] * A + Fm[l] * A;
 #pragma unroll
@@ -406,8 +417,15 @@ struct FaceMath {
     const long nc = b.nc;
     const T delx = T(b.cx[c] - b.cx[cm]), dely = T(b.cy[c] - b.cy[cm]), delz = T(b.cz[c] - b.cz[cm]);
     T d2 = delx * delx + dely * dely + delz * delz;
-    T id2 = T(1.0) / d2;  // normal_comp*del/d_LR = (del_phi - grad.del)*del/d_LR^2
-    const double *gx = b.g[0], *gy = b.g[1], *gz = b.g[2];
+    T id2 = tdiv(T(1.0), d2);  // normal_comp*del/d_LR = (del_phi - grad.del)*del/d_LR^2
+    // FP32 mode reads the FP32 copy of the gradients
+    using GS = typename std::conditional<sizeof(T) == sizeof(float), const float, const double>::type;
+    GS *gx, *gy, *gz;
+    if constexpr (sizeof(T) == sizeof(float)) {
+      gx = b.g32[0]; gy = b.g32[1]; gz = b.g32[2];
+    } else {
+      gx = b.g[0]; gy = b.g[1]; gz = b.g[2];
+    }
     const long ngr = b.ngr;
     T gr[4][3];
 #pragma unroll
@@ -535,9 +553,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   int i, j;
   const int blk = plane_block(b.xcd_remap);
   const int k = blockIdx.y + 1;
-  const double *lds_row = nullptr;  // the i sweep's stencil row in LDS
+  // the stencil's storage: the FP64 state, or its FP32 copy in FP32 mode
+  using QS = typename std::conditional<sizeof(T) == sizeof(float), float, double>::type;
+  const QS *qsrc;
+  if constexpr (sizeof(T) == sizeof(float)) qsrc = b.q32;
+  else qsrc = b.qp;
+  const QS *lds_row = nullptr;  // the i sweep's stencil row in LDS
   if constexpr (D == 0) {
-    __shared__ double sq[NV][FLUX_I_SPAN + 5];
+    __shared__ QS sq[NV][FLUX_I_SPAN + 5];
     const int t = (int)threadIdx.x;
     const long p0 = flux_i_first(b) + (long)blk * FLUX_I_SPAN;
     const long p = p0 + t;
@@ -549,10 +572,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     const long kbase = plane * (long)(k + 2);
 #pragma unroll
     for (int l = 0; l < NV; ++l) {
-      const double *ql = b.qp + (long)l * b.nc + kbase;
+      const QS *ql = qsrc + (long)l * b.nc + kbase;
       for (int e = t; e < FLUX_I_SPAN + 5; e += FLUX_I_SPAN) {
         const long pe = p0 - 3 + e;
-        sq[l][e] = (pe < plane && k <= b.kmx - 1) ? ql[pe] : 0.0;
+        sq[l][e] = (pe < plane && k <= b.kmx - 1) ? ql[pe] : QS(0);
       }
     }
     __syncthreads();
@@ -585,7 +608,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     ax.pb = (RECON == 1 || RECON == 2) && b.pbs[D];
     if (ax.pb) {
       if constexpr (D == 0) M::pb_factors(b, n, N, lds_row + 4 * (FLUX_I_SPAN + 5), 1, ax);
-      else M::pb_factors(b, n, N, b.qp + 4 * b.nc + c, s, ax);
+      else M::pb_factors(b, n, N, qsrc + 4 * b.nc + c, s, ax);
     }
     if constexpr (RECON == 4) M::nm_blends(b.vol + c, s, ax);
 #pragma unroll
@@ -593,7 +616,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
       if constexpr (D == 0) {
         M::template face_states<RECON>(b, D, n, N, l, lds_row + l * (FLUX_I_SPAN + 5), 1, ax, qL[l], qR[l]);
       } else {
-        const double *u = b.qp + (long)l * b.nc + c;
+        const QS *u = qsrc + (long)l * b.nc + c;
         M::template face_states<RECON>(b, D, n, N, l, u, s, ax, qL[l], qR[l]);
       }
     }

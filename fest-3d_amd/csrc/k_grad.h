@@ -234,7 +234,8 @@ __global__ void __launch_bounds__(256) k_grad(Blk b) {
       }
       bad |= isnan(g);
       gg[l][d] = g;
-      b.g[d][(long)l * b.ngr + gidx] = g;
+      if (!b.g32only) b.g[d][(long)l * b.ngr + gidx] = g;
+      if (b.fp32) b.g32[d][(long)l * b.ngr + gidx] = (float)g;
     }
   }
   if (bad) atomicOr(b.err, ERR_NAN_GRAD);
@@ -350,20 +351,48 @@ __global__ void k_grad_bc(Blk b) {
   double sig = hi ? -1.0 : 1.0;
   long gI = gri(b, Ii, Ij, Ik), gG = gri(b, Gi, Gj, Gk);
   double *gx = b.g[0], *gy = b.g[1], *gz = b.g[2];
+  float *hx = b.g32[0], *hy = b.g32[1], *hz = b.g32[2];
+  // the interior gradient: FP64, or its FP32 copy when that is all there is
+  auto gin = [&](int c, long o) -> double {
+    if (b.g32only) return (double)(c == 0 ? hx : c == 1 ? hy : hz)[o];
+    return (c == 0 ? gx : c == 1 ? gy : gz)[o];
+  };
   for (int l = 0; l < b.ng; ++l) {
     double dq = b.qp[(long)(l + 1) * b.nc + cI] - b.qp[(long)(l + 1) * b.nc + cG];
     if (l == 3) dq = b.temp[cI] - b.temp[cG];
     double ox = sig * dq * c_x, oy = sig * dq * c_y, oz = sig * dq * c_z;
     long o = (long)l * b.ngr;
+    double ix = gin(0, o + gI), iy = gin(1, o + gI), iz = gin(2, o + gI);
     if (l == 3 && id == -5 && (Tw < 1. && Tw >= 0.)) {
-      ox = -gx[o + gI]; oy = -gy[o + gI]; oz = -gz[o + gI];
+      ox = -ix; oy = -iy; oz = -iz;
     }
-    double ix = gx[o + gI], iy = gy[o + gI], iz = gz[o + gI];
     double dot = (ix * nx) + (iy * ny) + (iz * nz);
-    gx[o + gG] = ox + (ix - dot * nx);
-    gy[o + gG] = oy + (iy - dot * ny);
-    gz[o + gG] = oz + (iz - dot * nz);
+    const double rx = ox + (ix - dot * nx), ry = oy + (iy - dot * ny), rz = oz + (iz - dot * nz);
+    if (!b.g32only) {
+      gx[o + gG] = rx;
+      gy[o + gG] = ry;
+      gz[o + gG] = rz;
+    }
+    if (b.fp32) {
+      hx[o + gG] = (float)rx;
+      hy[o + gG] = (float)ry;
+      hz[o + gG] = (float)rz;
+    }
   }
+}
+
+// FP32 copy of the padded state for the FP32 face sweeps
+__global__ void k_to_f32(Blk b) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= b.nc * b.nv) return;
+  b.q32[t] = (float)b.qp[t];
+}
+
+// FP64 gradients widened from the FP32 copy (downloads in FP32 mode)
+__global__ void k_g32_widen(Blk b) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= b.ngr * b.ng) return;
+  for (int d = 0; d < 3; ++d) b.g[d][t] = (double)b.g32[d][t];
 }
 
 // A -11 face has no mu_t / F1 ghost rule (src/viscosity.f90:284-339,

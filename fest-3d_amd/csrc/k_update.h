@@ -229,46 +229,66 @@ __global__ void __launch_bounds__(256) k_update(Blk b, double TF, double SF, int
   if (sa) b.qp[5 * nc + c] = fmx(u2[5], 1.e-12);
 }
 
-// per-block sums of residue^2 and of the boundary mass fluxes
-__global__ void k_norm_partial(Blk b) {
-  __shared__ double s[256][8];
+// per-block sums of residue^2 and of the boundary mass fluxes: blocks take
+// whole i rows (j, k) of the residue in turn, threads the cells of a row
+__global__ void __launch_bounds__(256) k_norm_partial(Blk b) {
+  __shared__ double s[8][256];
   const int nv = b.nv;
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  long n = b.nres;
-  long stride = (long)gridDim.x * blockDim.x;
-  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
-    for (int l = 0; l < nv; ++l) {
-      double r = b.res[(long)l * n + t];
-      acc[1 + l] += r * r;
+  const int ni = b.imx - 1, nj = b.jmx - 1;
+  const long rows = (long)nj * (b.kmx - 1);
+  for (long r = blockIdx.x; r < rows; r += gridDim.x) {
+    const int j = (int)(r % nj) + 1, k = (int)(r / nj) + 1;
+    const long row = (r) * (long)ni;
+    for (int i = 1 + (int)threadIdx.x; i <= ni; i += blockDim.x) {
+      const long t = row + (i - 1);
+#pragma unroll
+      for (int l = 0; l < 7; ++l)
+        if (l < nv) {
+          double v = b.res[(long)l * b.nres + t];
+          acc[1 + l] += v * v;
+        }
+      if (i == 1) acc[0] += b.flux[0][fli(b, 0, 1, j, k)];
+      if (i == ni) acc[0] -= b.flux[0][fli(b, 0, b.imx, j, k)];
+      if (j == 1) acc[0] += b.flux[1][fli(b, 1, i, 1, k)];
+      if (j == nj) acc[0] -= b.flux[1][fli(b, 1, i, b.jmx, k)];
+      if (k == 1) acc[0] += b.flux[2][fli(b, 2, i, j, 1)];
+      if (k == b.kmx - 1) acc[0] -= b.flux[2][fli(b, 2, i, j, b.kmx)];
     }
-    int i = (int)(t % (b.imx - 1)) + 1;
-    long q = t / (b.imx - 1);
-    int j = (int)(q % (b.jmx - 1)) + 1;
-    int k = (int)(q / (b.jmx - 1)) + 1;
-    if (i == 1) acc[0] += b.flux[0][fli(b, 0, 1, j, k)];
-    if (i == b.imx - 1) acc[0] -= b.flux[0][fli(b, 0, b.imx, j, k)];
-    if (j == 1) acc[0] += b.flux[1][fli(b, 1, i, 1, k)];
-    if (j == b.jmx - 1) acc[0] -= b.flux[1][fli(b, 1, i, b.jmx, k)];
-    if (k == 1) acc[0] += b.flux[2][fli(b, 2, i, j, 1)];
-    if (k == b.kmx - 1) acc[0] -= b.flux[2][fli(b, 2, i, j, b.kmx)];
   }
-  for (int l = 0; l <= nv; ++l) s[threadIdx.x][l] = acc[l];
+#pragma unroll
+  for (int l = 0; l < 8; ++l) s[l][threadIdx.x] = acc[l];
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
     if (threadIdx.x < st)
-      for (int l = 0; l <= nv; ++l) s[threadIdx.x][l] += s[threadIdx.x + st][l];
+#pragma unroll
+      for (int l = 0; l < 8; ++l) s[l][threadIdx.x] += s[l][threadIdx.x + st];
     __syncthreads();
   }
-  if (threadIdx.x == 0)
-    for (int l = 0; l <= nv; ++l) b.red[(long)blockIdx.x * 8 + l] = s[0][l];
+  if (threadIdx.x < 8) b.red[(long)blockIdx.x * 8 + threadIdx.x] = s[threadIdx.x][0];
 }
 
-// final norms, src/resnorm.f90:128-225
-__global__ void k_norm_final(Blk b, int nblk) {
+// final norms, src/resnorm.f90:128-225: 256 threads sum the block
+// partials in a fixed order (strided, then a tree)
+__global__ void __launch_bounds__(256) k_norm_final(Blk b, int nblk) {
+  __shared__ double s[8][256];
+  double part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int t = threadIdx.x; t < nblk; t += blockDim.x)
+#pragma unroll
+    for (int l = 0; l < 8; ++l) part[l] += b.red[(long)t * 8 + l];
+#pragma unroll
+  for (int l = 0; l < 8; ++l) s[l][threadIdx.x] = part[l];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st)
+#pragma unroll
+      for (int l = 0; l < 8; ++l) s[l][threadIdx.x] += s[l][threadIdx.x + st];
+    __syncthreads();
+  }
   if (threadIdx.x != 0) return;
-  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int t = 0; t < nblk; ++t)
-    for (int l = 0; l <= b.nv; ++l) acc[l] += b.red[(long)t * 8 + l];
+  double acc[8];
+#pragma unroll
+  for (int l = 0; l < 8; ++l) acc[l] = s[l][0];
   double v = b.vel_mag;
   double scale[8];
   scale[0] = 1.;

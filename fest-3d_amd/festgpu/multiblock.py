@@ -36,39 +36,53 @@ def aligned_link(blocks, b, face, b2, face2):
 
 
 def split_i(case: Case, nodes: np.ndarray, q: np.ndarray, cuts):
-    """Cut `case` at the i-nodes in `cuts` (1 < cut < imx).  Returns the
-    blocks [(case, nodes, q)] and the links; an i-periodic original (-9 on
-    imin and imax) becomes a cross-block periodic pair (-10, src/boundary/
-    bc.f90:27-32).  Each sub-block's state includes three cells of its
-    neighbours on each cut, i.e. consistent ghost layers."""
-    bounds = [1] + sorted(cuts) + [case.imx]
+    """Cut `case` at the i-nodes in `cuts` (1 < cut < imx); split() along i."""
+    return split(case, nodes, q, 0, cuts)
+
+
+def split(case: Case, nodes: np.ndarray, q: np.ndarray, axis: int, cuts):
+    """Cut `case` at the nodes `cuts` (1 < cut < N) of direction `axis`
+    (0 i, 1 j, 2 k).  Returns the blocks [(case, nodes, q)] and the links; a
+    periodic original (-9 on both faces of the axis) becomes a cross-block
+    periodic pair (-10, src/boundary/bc.f90:27-32).  Each sub-block's state
+    includes three cells of its neighbours on each cut, i.e. consistent
+    ghost layers."""
+    dims = (case.imx, case.jmx, case.kmx)
+    N = dims[axis]
+    bounds = [1] + sorted(cuts) + [N]
     nb = len(bounds) - 1
-    periodic = case.bc_id[0] == -9 and case.bc_id[1] == -9
+    fmin, fmax = 2 * axis, 2 * axis + 1
+    periodic = case.bc_id[fmin] == -9 and case.bc_id[fmax] == -9
+    ax = 3 - axis  # array axis of direction `axis` in (component, k, j, i)
     blocks = []
     for b in range(nb):
         s, e = bounds[b], bounds[b + 1]  # node range of block b
-        imx = e - s + 1
+        n = e - s + 1
         ids = list(case.bc_id)
-        ids[0] = (b - 1) if b > 0 else (-10 if periodic else case.bc_id[0])
-        ids[1] = (b + 1) if b < nb - 1 else (-10 if periodic else case.bc_id[1])
-        cb = dataclasses.replace(case, imx=imx, bc_id=tuple(ids), name=f"{case.name}_b{b}")
+        ids[fmin] = (b - 1) if b > 0 else (-10 if periodic else case.bc_id[fmin])
+        ids[fmax] = (b + 1) if b < nb - 1 else (-10 if periodic else case.bc_id[fmax])
+        ext = dict(zip(("imx", "jmx", "kmx"), dims))
+        ext[("imx", "jmx", "kmx")[axis]] = n
+        cb = dataclasses.replace(case, bc_id=tuple(ids), name=f"{case.name}_b{b}", **ext)
         # nodes: array index = node index + 2; cells: array index = cell index + 2
         nb_nodes = np.zeros(cb.node_shape())
-        nb_nodes[:, :, :, 3:-3] = nodes[:, :, :, s + 2:e + 3]
-        qb = np.zeros((q.shape[0],) + cb.cell_shape())
+        src_n = [slice(None)] * 4
+        dst_n = [slice(None)] * 4
+        src_n[ax] = slice(s + 2, e + 3)
+        dst_n[ax] = slice(3, -3)
+        nb_nodes[tuple(dst_n)] = nodes[tuple(src_n)]
         lo = s - 3 + 2  # cell index s-3 .. e+2
-        src = np.arange(lo, lo + imx + 5)
-        src_c = np.clip(src, 0, q.shape[-1] - 1)
-        qb[...] = q[..., src_c]
+        src = np.clip(np.arange(lo, lo + n + 5), 0, q.shape[ax] - 1)
+        qb = np.take(q, src, axis=ax)
         blocks.append((cb, nb_nodes, qb))
     cases = [c for c, _, _ in blocks]
     links = []
     for b in range(nb - 1):
-        links.append(aligned_link(cases, b, 2, b + 1, 1))
-        links.append(aligned_link(cases, b + 1, 1, b, 2))
+        links.append(aligned_link(cases, b, fmax + 1, b + 1, fmin + 1))
+        links.append(aligned_link(cases, b + 1, fmin + 1, b, fmax + 1))
     if periodic:
-        links.append(aligned_link(cases, 0, 1, nb - 1, 2))
-        links.append(aligned_link(cases, nb - 1, 2, 0, 1))
+        links.append(aligned_link(cases, 0, fmin + 1, nb - 1, fmax + 1))
+        links.append(aligned_link(cases, nb - 1, fmax + 1, 0, fmin + 1))
     return blocks, links
 
 

@@ -252,3 +252,27 @@ def test_threaded_group_step_equals_serial():
         runs.append([o.arr("qp").copy() for o in os_])
     for a, b in zip(*runs):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("axis,cuts", [(1, [5]), (2, [4, 7])])
+def test_split_along_j_and_k_equals_single_block_inviscid(axis, cuts):
+    """split() along j or k: an integer Cartesian block with far-field faces
+    on the cut axis, inviscid, three RK4 steps, bit for bit."""
+    case, _, q = C.laminar_box(8, 10, 10, seed=3)
+    ids = list(case.bc_id)
+    ids[2 * axis], ids[2 * axis + 1] = -8, -8
+    case = dataclasses.replace(case, time_local=1, mu_ref=0.0, bc_id=tuple(ids))
+    nodes = C.cartesian_nodes(case)
+    blocks, links = M.split(case, nodes, q, axis, cuts)
+    single = Oracle(case)
+    single.set_nodes(nodes)
+    single.set_state(q)
+    os_ = oracles(blocks)
+    for it in range(1, 4):
+        assert single.step(it) == 0
+        assert group_step(os_, links, it) == 0
+    bounds = [1] + sorted(cuts) + [(case.imx, case.jmx, case.kmx)[axis]]
+    parts = [o.arr("qp")[:, 3:-3, 3:-3, 3:-3] for o in os_]
+    parts = [np.take(p, np.arange(bounds[b + 1] - bounds[b]), axis=3 - axis) for b, p in enumerate(parts)]
+    got = np.concatenate(parts, axis=3 - axis)
+    assert np.array_equal(got, single.arr("qp")[:, 3:-3, 3:-3, 3:-3])
