@@ -1028,8 +1028,9 @@ int fest_download(fest_handle *h, uint32_t what, double *qp, double *temp, doubl
 int fest_resnorm(fest_handle *h, double *res_abs) {
   if (!h || !res_abs) return FEST_E_ARG;
   Blk &b = h->b;
-  hipLaunchKernelGGL(k_norm_partial, dim3(h->norm_blocks), dim3(256), 0, h->stream, b);
-  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, h->stream, b, h->norm_blocks);
+  hipLaunchKernelGGL(k_norm_partial, dim3(h->norm_blocks), dim3(256), 0, h->stream, b, b.red);
+  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, h->stream, b, h->norm_blocks, (const double *)b.red,
+                     b.norms, 1);
   LAUNCHCHK(h);
   HIPCHK(h, hipMemcpyAsync(res_abs, b.norms, sizeof(double) * (b.nv + 1), hipMemcpyDeviceToHost, h->stream));
   // the norms need a wait anyway: deferred device errors are reported here
@@ -1259,8 +1260,9 @@ int fest_group_resnorm(fest_group *g, double *res_abs) {
   std::vector<double> tot(nv + 1, 0.0), part(nv + 1);
   for (fest_handle *h : g->blocks) {
     Blk &b = h->b;
-    hipLaunchKernelGGL(k_norm_partial, dim3(h->norm_blocks), dim3(256), 0, h->stream, b);
-    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, h->stream, b, h->norm_blocks);
+    hipLaunchKernelGGL(k_norm_partial, dim3(h->norm_blocks), dim3(256), 0, h->stream, b, b.red);
+    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, h->stream, b, h->norm_blocks, (const double *)b.red,
+                       b.norms, 1);
     if (hipGetLastError() != hipSuccess) return gfail(g, FEST_E_HIP, "norm launch failed");
     if (hipMemcpyAsync(part.data(), b.norms + 8, sizeof(double) * (nv + 1), hipMemcpyDeviceToHost, h->stream) !=
             hipSuccess ||

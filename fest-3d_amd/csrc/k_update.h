@@ -231,7 +231,7 @@ __global__ void __launch_bounds__(256) k_update(Blk b, double TF, double SF, int
 
 // per-block sums of residue^2 and of the boundary mass fluxes: blocks take
 // whole i rows (j, k) of the residue in turn, threads the cells of a row
-__global__ void __launch_bounds__(256) k_norm_partial(Blk b) {
+__global__ void __launch_bounds__(256) k_norm_partial(Blk b, double *red) {
   __shared__ double s[8][256];
   const int nv = b.nv;
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -265,17 +265,18 @@ __global__ void __launch_bounds__(256) k_norm_partial(Blk b) {
       for (int l = 0; l < 8; ++l) s[l][threadIdx.x] += s[l][threadIdx.x + st];
     __syncthreads();
   }
-  if (threadIdx.x < 8) b.red[(long)blockIdx.x * 8 + threadIdx.x] = s[threadIdx.x][0];
+  if (threadIdx.x < 8) red[(long)blockIdx.x * 8 + threadIdx.x] = s[threadIdx.x][0];
 }
 
 // final norms, src/resnorm.f90:128-225: 256 threads sum the block
-// partials in a fixed order (strided, then a tree)
-__global__ void __launch_bounds__(256) k_norm_final(Blk b, int nblk) {
+// partials in a fixed order (strided, then a tree).  out receives the n_var+1
+// norms and, with shares, from out[8] the block's unassembled share.
+__global__ void __launch_bounds__(256) k_norm_final(Blk b, int nblk, const double *red, double *out, int shares) {
   __shared__ double s[8][256];
   double part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int t = threadIdx.x; t < nblk; t += blockDim.x)
 #pragma unroll
-    for (int l = 0; l < 8; ++l) part[l] += b.red[(long)t * 8 + l];
+    for (int l = 0; l < 8; ++l) part[l] += red[(long)t * 8 + l];
 #pragma unroll
   for (int l = 0; l < 8; ++l) s[l][threadIdx.x] = part[l];
   __syncthreads();
@@ -297,12 +298,13 @@ __global__ void __launch_bounds__(256) k_norm_final(Blk b, int nblk) {
   scale[5] = (0.5 * b.rho_inf * (v * v * v) + ((b.gm / (b.gm - 1.)) * b.p_inf));
   scale[6] = b.rho_inf * v * (b.ng == 5 ? b.tv_inf : b.tk_inf);  // nu-tilde for SA (resnorm.f90:158)
   scale[7] = b.rho_inf * v * (b.kkl ? b.tkl_inf : b.tw_inf);  // kL for k-kL (resnorm.f90:153)
-  b.norms[0] = fabs(acc[0] / scale[0]);
-  for (int l = 1; l <= b.nv; ++l) b.norms[l] = sqrt(acc[l] / (scale[l] * scale[l]));
+  out[0] = fabs(acc[0] / scale[0]);
+  for (int l = 1; l <= b.nv; ++l) out[l] = sqrt(acc[l] / (scale[l] * scale[l]));
+  if (!shares) return;
   // the block's share before the cross-block assembly (src/resnorm.f90:
   // 171-225): the signed mass imbalance and sum(R_l^2)/scale_l^2
-  b.norms[8] = acc[0] / scale[0];
-  for (int l = 1; l <= b.nv; ++l) b.norms[8 + l] = acc[l] / (scale[l] * scale[l]);
+  out[8] = acc[0] / scale[0];
+  for (int l = 1; l <= b.nv; ++l) out[8 + l] = acc[l] / (scale[l] * scale[l]);
 }
 
 }  // namespace fest
