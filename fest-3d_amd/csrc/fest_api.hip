@@ -103,6 +103,18 @@ struct fest_group {
   int unchecked = 0;
 };
 
+// Scratch of fest_resnorm_device, the allocation-free norm entry point: the
+// per-block partial sums of the norm reduction, 8 doubles for each of at most
+// NORM_BLOCKS_MAX blocks (64 KiB).  Each device holds its own copy until the
+// process ends; host code reaches it only through the address that
+// hipGetSymbolAddress returns for the current device (scratch_dev).  Every
+// call on a device shares that device's copy: calls that overlap on one
+// device race.
+constexpr int NORM_BLOCKS_MAX = 1024;
+constexpr int SCRATCH_DEVICES = 64;
+static __device__ __align__(256) double scratch[8 * NORM_BLOCKS_MAX];
+static double *scratch_dev[SCRATCH_DEVICES];
+
 namespace {
 
 int fail(fest_handle *h, int code, const char *fmt, ...) {
@@ -689,6 +701,26 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
     return FEST_E_HIP;
   }
 
+  {
+    // the device address of the norm scratch, once per device
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= SCRATCH_DEVICES) {
+      h->err = "hipGetDevice failed or device index out of range";
+      *out = h;
+      return FEST_E_HIP;
+    }
+    if (!scratch_dev[dev]) {
+      void *p = nullptr;
+      hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p) {
+        h->err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        *out = h;
+        return FEST_E_HIP;
+      }
+      scratch_dev[dev] = (double *)p;
+    }
+  }
+
   Blk &b = h->b;
   memset(&b, 0, sizeof b);
   b.imx = dims->imx; b.jmx = dims->jmx; b.kmx = dims->kmx; b.nv = dims->n_var;
@@ -1036,6 +1068,23 @@ int fest_resnorm(fest_handle *h, double *res_abs) {
   // the norms need a wait anyway: deferred device errors are reported here
   h->unchecked = 0;
   return check_err(h);
+}
+
+int fest_resnorm_device(fest_handle *h, double *res_abs_dev) {
+  if (!h || !res_abs_dev) return FEST_E_ARG;
+  Blk &b = h->b;
+  int dev = 0;
+  HIPCHK(h, hipGetDevice(&dev));
+  if (dev < 0 || dev >= SCRATCH_DEVICES || !scratch_dev[dev])
+    return fail(h, FEST_E_ARG, "no norm scratch on device %d: create the handle with that device current", dev);
+  if (h->norm_blocks > NORM_BLOCKS_MAX)
+    return fail(h, FEST_E_ARG, "%d norm blocks exceed the scratch's %d", h->norm_blocks, NORM_BLOCKS_MAX);
+  double *red = scratch_dev[dev];
+  hipLaunchKernelGGL(k_norm_partial, dim3(h->norm_blocks), dim3(256), 0, h->stream, b, red);
+  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, h->stream, b, h->norm_blocks, (const double *)red,
+                     res_abs_dev, 0);
+  LAUNCHCHK(h);
+  return 0;
 }
 
 int fest_device_ptr(fest_handle *h, const char *name, void **ptr, size_t *count) {

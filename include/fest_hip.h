@@ -174,6 +174,17 @@ int fest_download(fest_handle *h, uint32_t what, double *qp, double *temp, doubl
  * (src/resnorm.f90:171-225): [|mass imbalance|, sqrt(sum R_l^2)/scale_l]. */
 int fest_resnorm(fest_handle *h, double *res_abs);
 
+/* fest_resnorm without allocation, waiting or host copies: the n_var+1
+ * norms of the last residue go to res_abs_dev, device memory the caller
+ * owns, in stream order (nothing waits for them).  No counterpart in the
+ * reference: it is for callers that cannot allocate -- code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc.  Its partial
+ * sums live in a static per-device scratch of 64 KiB (at most 1024 partial
+ * blocks; larger reductions are rejected with FEST_E_ARG), set up when a
+ * handle is created on the device; calls that overlap on one device race
+ * for it.  The handle's device must be current. */
+int fest_resnorm_device(fest_handle *h, double *res_abs_dev);
+
 /* Device pointers of the resident arrays for callers that keep the state
  * on the device.  name: "qp","temp","residue","F","G","H","dt". */
 int fest_device_ptr(fest_handle *h, const char *name, void **ptr, size_t *count);
