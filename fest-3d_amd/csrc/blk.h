@@ -112,10 +112,17 @@ constexpr int FLAT_BLOCK = 256;
 // blocks e*S/8 .. (e+1)*S/8-1 of every plane, so stencil neighbours along j
 // (adjacent rows) and along k (the same range of the next plane) are read
 // through one L2.  Blocks past the plane's end find no cells.
-__device__ __forceinline__ int plane_block(int remap) {
-  const int s = (int)blockIdx.x;
-  if (!remap) return s;
-  const int S = (int)gridDim.x;
+// The work-group's place in its launch: the hardware block index, or an
+// entry of a group's tile work list (nx = 0 there: no XCD reordering).
+struct Wg {
+  int x, y, z, nx;
+};
+#define FEST_WG_HW (::fest::Wg{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, (int)gridDim.x})
+
+__device__ __forceinline__ int plane_block(const Wg &wg, int remap) {
+  const int s = wg.x;
+  if (!remap || wg.nx == 0) return s;
+  const int S = wg.nx;
   return (s & 7) * (S >> 3) + (s >> 3);
 }
 

@@ -22,6 +22,7 @@
 #include "k_grad.h"
 #include "k_iface.h"
 #include "k_lusgs.h"
+#include "k_multi.h"
 #include "k_update.h"
 
 using namespace fest;
@@ -90,6 +91,11 @@ struct fest_handle {
 
 // a group's view of its blocks; a block destroyed while in a group detaches
 // itself and leaves the group unusable (its links point into the block)
+// the phases of a batched group step (k_multi.h), each one launch over the
+// tiles of every block
+enum { PH_BEGIN = 0, PH_BC_A0, PH_BC_A1, PH_BC_A2, PH_BC_B0, PH_BC_B1, PH_BC_B2, PH_EDGE0, PH_EDGE1, PH_EDGE2,
+       PH_F32, PH_GRAD, PH_GRAD_BC, PH_FLUX_I, PH_FLUX_J, PH_FLUX_K, PH_DT, PH_UPDATE, PH_COUNT };
+
 struct fest_group {
   std::vector<fest_handle *> blocks;
   std::vector<fest_link> links;
@@ -101,6 +107,18 @@ struct fest_group {
   StepGraphs graphs;
   int check_every = 1;
   int unchecked = 0;
+  // batched execution: the blocks' records and each phase's tile work list
+  // on the device; the schedule (batched or not, work-groups per launch,
+  // tile order) is set by fest_group_set_option or fest_group_autotune
+  std::string batch_why;  // empty when one kernel variant fits every block
+  int batched = 0;
+  int grid_cap = 0;       // work-groups per batched launch; 0: one per tile
+  int order = 0;          // 0: tiles block by block; 1: the blocks' tiles interleaved
+  int lists_order = -1;   // order the device lists were built in
+  Blk *d_blks = nullptr;
+  double blk_cfl = -1.0;  // CFL of the uploaded records
+  WorkItem *d_items = nullptr;
+  int ph_off[PH_COUNT] = {}, ph_n[PH_COUNT] = {};
 };
 
 // Scratch of fest_resnorm_device, the allocation-free norm entry point: the
@@ -1188,6 +1206,187 @@ int group_exchange(fest_group *g) {
   return e == hipSuccess ? 0 : gfail(g, FEST_E_HIP, "exchange launch: %s", hipGetErrorString(e));
 }
 
+// the per-block launch grid of a phase (as launch_residual_front and the
+// step operations lay it out); false when the block has no tiles in it
+bool phase_grid(const Blk &b, int ph, dim3 &g) {
+  auto N = [&](int d) { return d == 0 ? b.imx : d == 1 ? b.jmx : b.kmx; };
+  switch (ph) {
+    case PH_BEGIN: g = dim3(cdiv(b.nc, 256), 1, 1); return true;
+    case PH_BC_A0: case PH_BC_A1: case PH_BC_A2: {
+      const int d = ph - PH_BC_A0;
+      g = dim3(cdiv(face_na(b, 2 * d) + 5, 64), cdiv(face_nb(b, 2 * d) + 5, 4), N(d) >= 8 ? 2 : 1);
+      return true;
+    }
+    case PH_BC_B0: case PH_BC_B1: case PH_BC_B2: {
+      const int d = ph - PH_BC_B0;
+      g = dim3(cdiv(face_na(b, 2 * d) + 5, 64), cdiv(face_nb(b, 2 * d) + 5, 4), 1);
+      return N(d) < 8;
+    }
+    case PH_EDGE0: g = dim3(cdiv(4L * (b.imx + 5) * b.nv, 256), 1, 1); return true;
+    case PH_EDGE1: g = dim3(cdiv(4L * (b.kmx + 5) * b.nv, 256), 1, 1); return true;
+    case PH_EDGE2: g = dim3(1, 1, 1); return true;
+    case PH_F32: g = dim3(cdiv(b.nc * b.nv, 256), 1, 1); return b.fp32 != 0;
+    case PH_GRAD: g = dim3(cdiv((long)(b.imx + 1) * (b.jmx + 1), FLAT_BLOCK), b.kmx + 1, 1); return b.mu_ref != 0.0;
+    case PH_GRAD_BC: {
+      const int mx = std::max(std::max(b.imx, b.jmx), b.kmx);
+      g = dim3(cdiv(mx, 64), cdiv(mx, 4), 6);
+      return b.mu_ref != 0.0;
+    }
+    case PH_FLUX_I: g = dim3(cdiv(flux_i_span(b), FLUX_I_SPAN), b.kmx - 1, 1); return true;
+    case PH_FLUX_J: g = dim3(cdiv(b.imx - 1, TILE_I) * cdiv(b.jmx, TILE_J), b.kmx - 1, 1); return true;
+    case PH_FLUX_K: g = dim3(cdiv(b.imx - 1, TILE_I) * cdiv(b.jmx - 1, TILE_J), b.kmx, 1); return true;
+    case PH_DT: case PH_UPDATE:
+      g = dim3(cdiv((long)(b.imx - 1) * (b.jmx - 1), FLAT_BLOCK), b.kmx - 1, 1);
+      return true;
+    default: return false;
+  }
+}
+
+// why one batched kernel variant cannot serve every block ("" when it can)
+std::string batch_reason(const fest_group *g) {
+  const Blk &a = g->blocks[0]->b;
+  for (fest_handle *h : g->blocks) {
+    const Blk &b = h->b;
+    if (b.nv != a.nv || b.recon != a.recon || b.turb != a.turb || b.fp32 != a.fp32 ||
+        (b.mu_ref != 0.0) != (a.mu_ref != 0.0) || b.time_acc != a.time_acc)
+      return "blocks differ in variables, reconstruction, model, precision or integrator";
+    if (b.turb != FEST_TURB_NONE && b.turb != FEST_TURB_SST && b.turb != FEST_TURB_SST2003)
+      return "batched steps cover laminar and SST blocks";
+    if (b.trans) return "batched steps carry no transition model";
+    if (b.kmx == 2) return "batched steps need three-dimensional blocks";
+    if (!b.time_local) return "batched steps use local time steps";
+    if (b.time_acc != FEST_TA_RK4 && b.time_acc != FEST_TA_RK2 && b.time_acc != FEST_TA_NONE)
+      return "batched steps cover RK4, RK2 and the single-stage update";
+    for (int f = 0; f < 6; ++f)
+      if (b.bc[f] == -11) return "the total-pressure inlet is not batched";
+  }
+  return "";
+}
+
+// every phase's work list, in `order`, uploaded as one device array
+int build_lists(fest_group *g, int order) {
+  std::vector<WorkItem> all;
+  for (int ph = 0; ph < PH_COUNT; ++ph) {
+    std::vector<std::vector<WorkItem>> per(g->blocks.size());
+    for (size_t n = 0; n < g->blocks.size(); ++n) {
+      dim3 gr;
+      if (!phase_grid(g->blocks[n]->b, ph, gr)) continue;
+      for (unsigned z = 0; z < gr.z; ++z)
+        for (unsigned y = 0; y < gr.y; ++y)
+          for (unsigned x = 0; x < gr.x; ++x) per[n].push_back(WorkItem{(int)n, (int)x, (int)y, (int)z});
+    }
+    g->ph_off[ph] = (int)all.size();
+    if (order == 0) {
+      for (auto &v : per) all.insert(all.end(), v.begin(), v.end());
+    } else {
+      // round robin over the blocks, one tile of each in turn
+      size_t mx = 0;
+      for (auto &v : per) mx = std::max(mx, v.size());
+      for (size_t t = 0; t < mx; ++t)
+        for (auto &v : per)
+          if (t < v.size()) all.push_back(v[t]);
+    }
+    g->ph_n[ph] = (int)all.size() - g->ph_off[ph];
+  }
+  if (g->d_items) (void)hipFree(g->d_items);
+  g->d_items = nullptr;
+  if (hipMalloc(&g->d_items, sizeof(WorkItem) * std::max<size_t>(1, all.size())) != hipSuccess)
+    return gfail(g, FEST_E_HIP, "hipMalloc of the work lists failed");
+  if (!all.empty() && hipMemcpy(g->d_items, all.data(), sizeof(WorkItem) * all.size(), hipMemcpyHostToDevice) !=
+                          hipSuccess)
+    return gfail(g, FEST_E_HIP, "work list upload failed");
+  g->lists_order = order;
+  return 0;
+}
+
+// the blocks' records on the device (they carry the CFL number of the call)
+int upload_blks(fest_group *g) {
+  std::vector<Blk> v;
+  for (fest_handle *h : g->blocks) v.push_back(h->b);
+  if (!g->d_blks && hipMalloc(&g->d_blks, sizeof(Blk) * v.size()) != hipSuccess)
+    return gfail(g, FEST_E_HIP, "hipMalloc of the block records failed");
+  if (hipMemcpy(g->d_blks, v.data(), sizeof(Blk) * v.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return gfail(g, FEST_E_HIP, "block record upload failed");
+  g->blk_cfl = v[0].CFLv;
+  return 0;
+}
+
+extern "C++" {
+template <int D, int NV, int RECON, typename T>
+void km_flux_launch(fest_group *g, int ph, unsigned grid, int store_csnd) {
+  hipLaunchKernelGGL((km_flux<D, NV, RECON, false, T>), dim3(grid), dim3(256), 0, g->stream, g->d_blks,
+                     g->d_items + g->ph_off[ph], g->ph_n[ph], store_csnd);
+}
+template <int D, int NV, typename T>
+void km_flux_recon(fest_group *g, int ph, unsigned grid, int store_csnd) {
+  switch (g->blocks[0]->b.recon) {
+    case 0: km_flux_launch<D, NV, 0, T>(g, ph, grid, store_csnd); break;
+    case 1: km_flux_launch<D, NV, 1, T>(g, ph, grid, store_csnd); break;
+    case 2: km_flux_launch<D, NV, 2, T>(g, ph, grid, store_csnd); break;
+    case 3: km_flux_launch<D, NV, 3, T>(g, ph, grid, store_csnd); break;
+    default: km_flux_launch<D, NV, 4, T>(g, ph, grid, store_csnd); break;
+  }
+}
+template <int D>
+void km_flux_any(fest_group *g, int ph, unsigned grid, int store_csnd) {
+  const Blk &b = g->blocks[0]->b;
+  if (b.nv == 7) {
+    if (b.fp32) km_flux_recon<D, 7, float>(g, ph, grid, store_csnd);
+    else km_flux_recon<D, 7, double>(g, ph, grid, store_csnd);
+  } else {
+    if (b.fp32) km_flux_recon<D, 5, float>(g, ph, grid, store_csnd);
+    else km_flux_recon<D, 5, double>(g, ph, grid, store_csnd);
+  }
+}
+
+}  // extern "C++"
+
+// one operation on all blocks through the phases' work lists
+int run_group_op_batched(fest_group *g, const StepOp &op, int current_iter) {
+  const Blk &b0 = g->blocks[0]->b;
+  hipStream_t st = g->stream;
+  auto grid = [g](int ph) -> unsigned {
+    return (unsigned)(g->grid_cap > 0 ? std::min(g->ph_n[ph], g->grid_cap) : g->ph_n[ph]);
+  };
+#define KM(ph, kern, bd, ...)                                                                              \
+  if (g->ph_n[ph])                                                                                        \
+  hipLaunchKernelGGL(kern, dim3(grid(ph)), bd, 0, st, (const Blk *)g->d_blks, (const WorkItem *)(g->d_items + g->ph_off[ph]), \
+                     g->ph_n[ph], ##__VA_ARGS__)
+  int rc;
+  switch (op.kind) {
+    case OP_BEGIN: KM(PH_BEGIN, km_step_begin, dim3(256), op.flags); break;
+    case OP_RESID:
+      if ((rc = group_exchange(g))) return rc;
+      for (int d = 0; d < 3; ++d) {
+        KM(PH_BC_A0 + d, km_bc_face, dim3(64, 4, 1), 2 * d, current_iter);
+        KM(PH_BC_B0 + d, km_bc_face, dim3(64, 4, 1), 2 * d + 1, current_iter);
+      }
+      KM(PH_EDGE0, km_bc_edges, dim3(256), 0);
+      KM(PH_EDGE1, km_bc_edges, dim3(256), 1);
+      KM(PH_EDGE2, km_bc_edges, dim3(256), 2);
+      if (b0.fp32) KM(PH_F32, km_to_f32, dim3(256));
+      if (b0.mu_ref != 0.0) {
+        if (b0.turb == FEST_TURB_SST2003) KM(PH_GRAD, (km_grad<6, true>), dim3(256));
+        else if (b0.ng == 6) KM(PH_GRAD, (km_grad<6, false>), dim3(256));
+        else KM(PH_GRAD, (km_grad<4, false>), dim3(256));
+        KM(PH_GRAD_BC, km_grad_bc, dim3(64, 4, 1));
+      }
+      if (g->ph_n[PH_FLUX_I]) km_flux_any<0>(g, PH_FLUX_I, grid(PH_FLUX_I), op.flags);
+      if (g->ph_n[PH_FLUX_J]) km_flux_any<1>(g, PH_FLUX_J, grid(PH_FLUX_J), op.flags);
+      if (g->ph_n[PH_FLUX_K]) km_flux_any<2>(g, PH_FLUX_K, grid(PH_FLUX_K), op.flags);
+      break;
+    case OP_DT: KM(PH_DT, km_dt, dim3(256)); break;
+    case OP_UPDATE: KM(PH_UPDATE, km_update, dim3(256), op.a, op.c, op.flags); break;
+    default:
+      for (fest_handle *h : g->blocks)
+        if ((rc = run_op(h, op, current_iter))) return gfail(g, rc, "%s", h->err.c_str());
+      return 0;
+  }
+#undef KM
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : gfail(g, FEST_E_HIP, "batched launch: %s", hipGetErrorString(e));
+}
+
 int group_check(fest_group *g) {
   for (size_t n = 0; n < g->blocks.size(); ++n) {
     int rc = check_err(g->blocks[n]);
@@ -1261,6 +1460,86 @@ int fest_group_create(fest_group **out, fest_handle *const *blocks, int nblocks,
     h->group = g;
     g->blocks.push_back(h);
   }
+  g->batch_why = batch_reason(g);
+  return 0;
+}
+
+int fest_group_autotune(fest_group *g, int steps, double *ms, int ms_len, int *ncand, int *chosen) {
+  if (!g || steps < 3) return FEST_E_ARG;
+  if (g->broken) return FEST_E_ARG;
+  for (fest_handle *h : g->blocks)
+    if (!h->have_grid || !h->have_state) return gfail(g, FEST_E_ARG, "grid and state must be set on every block");
+  hipDeviceProp_t prop;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+    return gfail(g, FEST_E_HIP, "device properties unavailable");
+  const int cu = prop.multiProcessorCount;
+  struct Cand { int batched, cap, order; };
+  std::vector<Cand> cands = {{0, 0, 0}};
+  if (g->batch_why.empty())
+    for (int order = 0; order < 2; ++order)
+      for (int cap : {0, 16 * cu, 8 * cu, 4 * cu}) cands.push_back({1, cap, order});
+  // snapshot of the state: a step reads only qp (Temp is refreshed from it)
+  std::vector<double *> saved;
+  for (fest_handle *h : g->blocks) {
+    double *p = nullptr;
+    if (hipMalloc(&p, sizeof(double) * h->b.nc * h->b.nv) != hipSuccess) {
+      for (double *q : saved) (void)hipFree(q);
+      return gfail(g, FEST_E_HIP, "hipMalloc of the state snapshot failed");
+    }
+    (void)hipMemcpyAsync(p, h->b.qp, sizeof(double) * h->b.nc * h->b.nv, hipMemcpyDeviceToDevice, g->stream);
+    saved.push_back(p);
+  }
+  auto restore = [&]() {
+    for (size_t n = 0; n < g->blocks.size(); ++n)
+      (void)hipMemcpyAsync(g->blocks[n]->b.qp, saved[n], sizeof(double) * g->blocks[n]->b.nc * g->blocks[n]->b.nv,
+                           hipMemcpyDeviceToDevice, g->stream);
+  };
+  const double cfl = g->blocks[0]->b.CFLv;
+  const int keep_b = g->batched, keep_c = g->grid_cap, keep_o = g->order;
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  int rc = 0, best = 0;
+  std::vector<double> t(cands.size(), 0.0);
+  for (size_t c = 0; c < cands.size() && !rc; ++c) {
+    g->graphs.reset();
+    g->batched = cands[c].batched;
+    g->grid_cap = cands[c].cap;
+    g->order = cands[c].order;
+    g->blk_cfl = -1.0;
+    restore();
+    // the first two steps capture the graph; the rest are timed
+    for (int s = 0; s < steps && !rc; ++s) {
+      if (s == 2) (void)hipEventRecord(e0, g->stream);
+      rc = fest_group_get_next_solution(g, 3 + s, cfl);
+    }
+    (void)hipEventRecord(e1, g->stream);
+    if (hipEventSynchronize(e1) != hipSuccess) rc = gfail(g, FEST_E_HIP, "autotune timing failed");
+    float el = 0.f;
+    (void)hipEventElapsedTime(&el, e0, e1);
+    t[c] = el / (steps - 2);
+    if (t[c] < t[best]) best = (int)c;
+  }
+  restore();
+  (void)hipStreamSynchronize(g->stream);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (double *q : saved) (void)hipFree(q);
+  g->graphs.reset();
+  g->blk_cfl = -1.0;
+  if (rc) {
+    g->batched = keep_b;
+    g->grid_cap = keep_c;
+    g->order = keep_o;
+    return rc;
+  }
+  g->batched = cands[best].batched;
+  g->grid_cap = cands[best].cap;
+  g->order = cands[best].order;
+  if (ncand) *ncand = (int)cands.size();
+  if (chosen) *chosen = best;
+  for (int c = 0; ms && c < (int)cands.size() && c < ms_len; ++c) ms[c] = t[c];
   return 0;
 }
 
@@ -1268,6 +1547,8 @@ void fest_group_destroy(fest_group *g) {
   if (!g) return;
   if (g->stream) (void)hipStreamSynchronize(g->stream);
   g->graphs.reset();
+  if (g->d_blks) (void)hipFree(g->d_blks);
+  if (g->d_items) (void)hipFree(g->d_items);
   for (fest_handle *h : g->blocks) {
     h->stream = h->own_stream;
     h->group = nullptr;
@@ -1331,6 +1612,15 @@ int fest_group_get_next_solution(fest_group *g, int current_iter, double cfl) {
   for (fest_handle *h : g->blocks)
     if (!h->have_grid || !h->have_state) return gfail(g, FEST_E_ARG, "grid and state must be set on every block");
   for (fest_handle *h : g->blocks) h->b.CFLv = cfl;
+  int rc0;
+  if (g->batched) {
+    if (g->lists_order != g->order && (rc0 = build_lists(g, g->order))) return rc0;
+    if (g->blk_cfl != cfl) {
+      // the records carry the CFL number; nothing may still read them
+      if (hipStreamSynchronize(g->stream) != hipSuccess) return gfail(g, FEST_E_HIP, "stream wait failed");
+      if ((rc0 = upload_blks(g))) return rc0;
+    }
+  }
   const int itn = current_iter <= 2 ? 1 : 3;
   bool timed = false;
   for (fest_handle *h : g->blocks) timed |= h->timed != 0;
@@ -1349,6 +1639,25 @@ int fest_group_get_next_solution(fest_group *g, int current_iter, double cfl) {
 
 int fest_group_set_option(fest_group *g, const char *name, long value) {
   if (!g || !name) return FEST_E_ARG;
+  if (!strcmp(name, "batched")) {
+    if (value && !g->batch_why.empty()) return gfail(g, FEST_E_CONFIG, "not batched: %s", g->batch_why.c_str());
+    g->graphs.reset();
+    g->batched = value != 0;
+    g->blk_cfl = -1.0;
+    return 0;
+  }
+  if (!strcmp(name, "grid_cap")) {
+    if (value < 0) return gfail(g, FEST_E_ARG, "grid_cap must be >= 0");
+    g->graphs.reset();
+    g->grid_cap = (int)value;
+    return 0;
+  }
+  if (!strcmp(name, "order")) {
+    if (value != 0 && value != 1) return gfail(g, FEST_E_ARG, "order must be 0 or 1");
+    g->graphs.reset();
+    g->order = (int)value;
+    return 0;
+  }
   if (!strcmp(name, "graphs")) {
     g->graphs.reset();
     g->graphs.enabled = value != 0;
@@ -1368,6 +1677,7 @@ namespace {
 // with the interface exchange (src/update.f90:534)
 int run_group_op(fest_group *g, const StepOp &op, int current_iter) {
   int rc;
+  if (g->batched) return run_group_op_batched(g, op, current_iter);
   if (op.kind == OP_RESID && (rc = group_exchange(g))) return rc;
   for (fest_handle *h : g->blocks)
     if ((rc = run_op(h, op, current_iter))) return gfail(g, rc, "%s", h->err.c_str());

@@ -219,10 +219,10 @@ __device__ void flow_tangency_col(const Blk &b, int face, int a, int bb) {
 // whole plane.
 // blockIdx.z selects face0 + z: the min and max faces of one direction run
 // in one launch when they cannot touch each other's cells (see the caller).
-__global__ void k_bc_face(Blk b, int face0, int current_iter) {
-  const int face = face0 + (int)blockIdx.z;
-  int a = (int)(blockIdx.x * blockDim.x + threadIdx.x) - 2;
-  int bb = (int)(blockIdx.y * blockDim.y + threadIdx.y) - 2;
+__device__ __forceinline__ void k_bc_face_b(const Blk &b, const Wg &wg, int face0, int current_iter) {
+  const int face = face0 + (int)wg.z;
+  int a = (int)(wg.x * blockDim.x + threadIdx.x) - 2;
+  int bb = (int)(wg.y * blockDim.y + threadIdx.y) - 2;
   int na = face_na(b, face), nb = face_nb(b, face);
   if (a > na + 2 || bb > nb + 2) return;
   const FaceBC p = face_params(b, face);
@@ -334,11 +334,14 @@ __global__ void k_bc_face(Blk b, int face0, int current_iter) {
       break;  // interfaces (>=0) and -10 are filled by the interface exchange
   }
 }
+__global__ void k_bc_face(Blk b, int face0, int current_iter) {
+  k_bc_face_b(b, FEST_WG_HW, face0, current_iter);
+}
 
 // edges and corners, src/boundary/bc_primitive.f90:209-224: phase 0 the
 // four lines along i, phase 1 the four lines along k, phase 2 the corners
-__global__ void k_bc_edges(Blk b, int phase) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_bc_edges_b(const Blk &b, const Wg &wg, int phase) {
+  int t = wg.x * blockDim.x + threadIdx.x;
   const int I = b.imx, J = b.jmx, K = b.kmx;
 #define Q(i, j, k) v[ci(b, i, j, k)]
   if (phase == 0) {
@@ -375,6 +378,9 @@ __global__ void k_bc_edges(Blk b, int phase) {
     }
   }
 #undef Q
+}
+__global__ void k_bc_edges(Blk b, int phase) {
+  k_bc_edges_b(b, FEST_WG_HW, phase);
 }
 
 }  // namespace fest

@@ -546,13 +546,13 @@ __host__ __device__ inline long flux_i_span(const Blk &b) {
   return (b.imx + 2) + (long)(b.jmx + 1) * b.pi - flux_i_first(b) + 1;
 }
 
-template <int D, int NV, int RECON, bool FLAT = false, typename T = double>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_flux(Blk b, int store_csnd) {
+template <int D, int NV, int RECON, bool FLAT, typename T>
+__device__ __forceinline__ void k_flux_b(const Blk &b, const Wg &wg, int store_csnd) {
   using M = FaceMath<T>;
   const int ii = D == 0, jj = D == 1, kk = D == 2;
   int i, j;
-  const int blk = plane_block(b.xcd_remap);
-  const int k = blockIdx.y + 1;
+  const int blk = plane_block(wg, b.xcd_remap);
+  const int k = wg.y + 1;
   // the stencil's storage: the FP64 state, or its FP32 copy in FP32 mode
   using QS = typename std::conditional<sizeof(T) == sizeof(float), float, double>::type;
   const QS *qsrc;
@@ -645,6 +645,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   }
   if (bad) atomicOr(b.err, ERR_NAN_FLUX);
   if (store_csnd) b.csnd[D][o] = (double)csf;
+}
+template <int D, int NV, int RECON, bool FLAT = false, typename T = double>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_flux(Blk b, int store_csnd) {
+  k_flux_b<D, NV, RECON, FLAT, T>(b, FEST_WG_HW, store_csnd);
 }
 
 }  // namespace fest

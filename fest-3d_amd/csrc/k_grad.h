@@ -184,11 +184,11 @@ __device__ __forceinline__ void sst_source(double rho, double tk, double tw, dou
 // source.  Interior gradients are final here: the ghost-gradient rule only
 // writes ghost cells.  NG is the number of gradient slots (4 laminar, 6
 // SST); a compile-time count keeps all 3*NG gradients in registers.
-template <int NG, bool S2003, bool BC = false, bool KKL = false>
-__global__ void __launch_bounds__(256) k_grad(Blk b) {
+template <int NG, bool S2003, bool BC, bool KKL>
+__device__ __forceinline__ void k_grad_b(const Blk &b, const Wg &wg) {
   int i, j;
-  if (!flat_ij(b.imx + 1, b.jmx + 1, i, j, plane_block(b.xcd_remap))) return;
-  const int k = blockIdx.y;
+  if (!flat_ij(b.imx + 1, b.jmx + 1, i, j, plane_block(wg, b.xcd_remap))) return;
+  const int k = wg.y;
   if (k > b.kmx) return;
   long c = ci(b, i, j, k);
   long f0 = fi(b, 0, i, j, k), f1 = fi(b, 0, i + 1, j, k);
@@ -308,6 +308,10 @@ __global__ void __launch_bounds__(256) k_grad(Blk b) {
   }
   }
 }
+template <int NG, bool S2003, bool BC = false, bool KKL = false>
+__global__ void __launch_bounds__(256) k_grad(Blk b) {
+  k_grad_b<NG, S2003, BC, KKL>(b, FEST_WG_HW);
+}
 
 // mu_t / F1 ghost rules (src/viscosity.f90:407-465), face interiors
 __device__ __forceinline__ void visc_ghost(const Blk &b, int face, int a, int bb) {
@@ -324,10 +328,10 @@ __device__ __forceinline__ void visc_ghost(const Blk &b, int face, int a, int bb
 // boundary id < 0.  The reference passes Jfaces/Kfaces to a dummy declared
 // with the I-face extents, so the J/K metrics are read at the I-face linear
 // offset (src/gradients.f90:523-588 calling :591); that is reproduced.
-__global__ void k_grad_bc(Blk b) {
-  int face = blockIdx.z;
-  int a = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  int bb = blockIdx.y * blockDim.y + threadIdx.y + 1;
+__device__ __forceinline__ void k_grad_bc_b(const Blk &b, const Wg &wg) {
+  int face = wg.z;
+  int a = wg.x * blockDim.x + threadIdx.x + 1;
+  int bb = wg.y * blockDim.y + threadIdx.y + 1;
   int na = face_na(b, face), nb = face_nb(b, face);
   if (a > na - 1 || bb > nb - 1) return;
   int id = b.bc[face];
@@ -380,12 +384,18 @@ __global__ void k_grad_bc(Blk b) {
     }
   }
 }
+__global__ void k_grad_bc(Blk b) {
+  k_grad_bc_b(b, FEST_WG_HW);
+}
 
 // FP32 copy of the padded state for the FP32 face sweeps
-__global__ void k_to_f32(Blk b) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_to_f32_b(const Blk &b, const Wg &wg) {
+  long t = (long)wg.x * blockDim.x + threadIdx.x;
   if (t >= b.nc * b.nv) return;
   b.q32[t] = (float)b.qp[t];
+}
+__global__ void k_to_f32(Blk b) {
+  k_to_f32_b(b, FEST_WG_HW);
 }
 
 // FP64 gradients widened from the FP32 copy (downloads in FP32 mode)
@@ -399,10 +409,10 @@ __global__ void k_g32_widen(Blk b) {
 // 409-465 list -1..-9): its ghost cells keep the SST values
 // calculate_viscosity computes there, from the ghost gradients k_grad_bc has
 // written.  Launched after k_grad_bc, only when an SST block has a -11 face.
-__global__ void k_mut_ghost_tp(Blk b) {
-  int face = blockIdx.z;
-  int a = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  int bb = blockIdx.y * blockDim.y + threadIdx.y + 1;
+__device__ __forceinline__ void k_mut_ghost_tp_b(const Blk &b, const Wg &wg) {
+  int face = wg.z;
+  int a = wg.x * blockDim.x + threadIdx.x + 1;
+  int bb = wg.y * blockDim.y + threadIdx.y + 1;
   int na = face_na(b, face), nb = face_nb(b, face);
   if (a > na - 1 || bb > nb - 1) return;
   if (b.bc[face] != -11 || b.ng != 6) return;
@@ -426,6 +436,9 @@ __global__ void k_mut_ghost_tp(Blk b) {
   b.mut[cG] = mut;
   b.f1[cG] = F1;
 }
+__global__ void k_mut_ghost_tp(Blk b) {
+  k_mut_ghost_tp_b(b, FEST_WG_HW);
+}
 
 // k-kL source (src/source.f90:607-832), after the ghost-cell gradients:
 // production from the full stress with -2/3 rho k, limited to 20 D_k; the
@@ -434,12 +447,12 @@ __global__ void k_mut_ghost_tp(Blk b) {
 // first derivatives (ghost cells' included); its limiters, and the
 // near-wall damping of both equations.  Writes S_k, S_kL (times the volume)
 // where the SST source goes.
-__global__ void __launch_bounds__(256) k_kkl_source(Blk b) {
+__device__ __forceinline__ void k_kkl_source_b(const Blk &b, const Wg &wg) {
   int i, j;
-  if (!flat_ij(b.imx - 1, b.jmx - 1, i, j, plane_block(b.xcd_remap))) return;
+  if (!flat_ij(b.imx - 1, b.jmx - 1, i, j, plane_block(wg, b.xcd_remap))) return;
   i += 1;
   j += 1;
-  const int k = blockIdx.y + 1;
+  const int k = wg.y + 1;
   if (k > b.kmx - 1) return;
   const long c = ci(b, i, j, k), r = ri(b, i, j, k), gi = gri(b, i, j, k), nc = b.nc, ngr = b.ngr;
   const double rho = b.qp[c], tk = b.qp[5 * nc + c], tkl = b.qp[6 * nc + c];
@@ -509,6 +522,9 @@ __global__ void __launch_bounds__(256) k_kkl_source(Blk b) {
   const double S_kl = P_kl - D_kl - 6 * mu * tkl * fphi / d2;
   b.src[r] = S_k * vol;
   b.src[b.nres + r] = S_kl * vol;
+}
+__global__ void __launch_bounds__(256) k_kkl_source(Blk b) {
+  k_kkl_source_b(b, FEST_WG_HW);
 }
 
 }  // namespace fest

@@ -14,13 +14,16 @@ namespace fest {
 
 // Temp = p/(R rho) on the whole padded array (src/update.f90:170) and the RK
 // base U_store = qp (src/update.f90:178)
-__global__ void k_step_begin(Blk b, int store_u) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_step_begin_b(const Blk &b, const Wg &wg, int store_u) {
+  long t = (long)wg.x * blockDim.x + threadIdx.x;
   if (t >= b.nc) return;
   double rho = b.qp[t];
   b.temp[t] = b.qp[4 * b.nc + t] / (b.R * rho);
   if (store_u)
     for (int l = 0; l < b.nv; ++l) b.ust[(long)l * b.nc + t] = b.qp[(long)l * b.nc + t];
+}
+__global__ void k_step_begin(Blk b, int store_u) {
+  k_step_begin_b(b, FEST_WG_HW, store_u);
 }
 
 // q = a U_store + c q over the whole padded primitive array, the TVD-RK
@@ -34,12 +37,12 @@ __global__ void k_blend(Blk b, double a, double c) {
 __device__ __forceinline__ double sspd(const Blk &b, int d, long o) { return b.csnd[d][o]; }
 
 // local time step of one cell (src/time.f90:122-246, 366-531)
-__global__ void __launch_bounds__(256) k_dt(Blk b) {
+__device__ __forceinline__ void k_dt_b(const Blk &b, const Wg &wg) {
   int i, j;
-  bool active = flat_ij(b.imx - 1, b.jmx - 1, i, j, (int)blockIdx.x);
+  bool active = flat_ij(b.imx - 1, b.jmx - 1, i, j, (int)wg.x);
   i += 1;
   j += 1;
-  const int k = blockIdx.y + 1;
+  const int k = wg.y + 1;
   active = active && k <= b.kmx - 1;
   double dtv = 1e300;
   if (active) {
@@ -97,10 +100,13 @@ __global__ void __launch_bounds__(256) k_dt(Blk b) {
       __syncthreads();
     }
     if (tid == 0) {
-      long blk = blockIdx.x + (long)gridDim.x * blockIdx.y;
+      long blk = wg.x + (long)wg.nx * wg.y;
       b.red[blk] = smin[0];
     }
   }
+}
+__global__ void __launch_bounds__(256) k_dt(Blk b) {
+  k_dt_b(b, FEST_WG_HW);
 }
 
 // global step: minimum of the block minima, broadcast into delta_t
@@ -147,12 +153,12 @@ __device__ __forceinline__ double sa_implicit(const Blk &b, long c, long gi, dou
 }
 
 // residue + SST/SA source, then (UPD_APPLY) the conservative update of the cell
-__global__ void __launch_bounds__(256) k_update(Blk b, double TF, double SF, int flags) {
+__device__ __forceinline__ void k_update_b(const Blk &b, const Wg &wg, double TF, double SF, int flags) {
   int i, j;
-  if (!flat_ij(b.imx - 1, b.jmx - 1, i, j, plane_block(b.xcd_remap))) return;
+  if (!flat_ij(b.imx - 1, b.jmx - 1, i, j, plane_block(wg, b.xcd_remap))) return;
   i += 1;
   j += 1;
-  const int k = blockIdx.y + 1;
+  const int k = wg.y + 1;
   if (k > b.kmx - 1) return;
   const int nv = b.nv;
   const long nc = b.nc;
@@ -227,6 +233,9 @@ __global__ void __launch_bounds__(256) k_update(Blk b, double TF, double SF, int
     if (u2[6] >= 0.) b.qp[6 * nc + c] = u2[6];
   }
   if (sa) b.qp[5 * nc + c] = fmx(u2[5], 1.e-12);
+}
+__global__ void __launch_bounds__(256) k_update(Blk b, double TF, double SF, int flags) {
+  k_update_b(b, FEST_WG_HW, TF, SF, flags);
 }
 
 // per-block sums of residue^2 and of the boundary mass fluxes: blocks take

@@ -335,4 +335,104 @@ def sst_bump(ni=256, nj=128, nk=128, seed=2, flux="slau", recon="weno", first=1e
     return c, nodes, q
 
 
-WORKLOADS = {"laminar_box": laminar_box, "sst_bump": sst_bump}
+def flat_plate(ni=512, nj=256, nk=128, seed=5, flux="ldfss0", recon="ppm", turb="sst", first=5.5e-6,
+               reynolds=5.0e6, length=2.0, height=0.5, span=0.25):
+    """Turbulent flat plate: PPM + LDFSS(0), SST, M=0.2, Re=5e6 per L, plate
+    length 2L along the whole lower wall (adiabatic, -5), subsonic inlet
+    (-3) and outlet (-4), far field at jmax (-8), periodic in k.  Wall-normal
+    tanh clustering with first spacing 5.5e-6 L (y+ about 1 at Re_x = 5e6).
+    Initial state: a 1/7-power-law boundary layer u = U (y/delta)^(1/7) with
+    delta(x) = 0.37 x Re_x^(-1/5) (x measured from 0.05L ahead of the inlet),
+    plus 1% seeded noise; the inlet itself fixes uniform free-stream values
+    (the reference's inlet imposes fixed values only)."""
+    c = Case(ni + 1, nj + 1, nk + 1, flux=flux, recon=recon, turb=turb, time_local=1,
+             accur=1, lim=(1, 1, 1), tlim=(1, 1, 1), mu_sutherland=1,
+             bc_id=(-3, -4, -5, -8, -9, -9), CFL=0.5, name=f"flat_plate_{ni}x{nj}x{nk}")
+    freestream(c, mach=0.2, reynolds=reynolds, tu_percent=0.1, mu_ratio=0.01)
+    c.Pr = 0.72
+    c.tPr = 0.9
+    inf = (c.rho_inf, c.u_inf, c.v_inf, c.w_inf, c.p_inf, c.tk_inf, c.tw_inf)
+    c.fix_rho, c.fix_u, c.fix_v, c.fix_w = (inf[0],) * 6, (inf[1],) * 6, (inf[2],) * 6, (inf[3],) * 6
+    c.fix_p, c.fix_tk, c.fix_tw = (inf[4],) * 6, (inf[5],) * 6, (inf[6],) * 6
+    c.fix_twall = (0.0,) * 6
+    c.fix_tv = (c.tv_inf,) * 6
+    c.fix_tkl = (c.tkl_inf,) * 6
+    x = np.zeros(c.node_shape())
+    xs = np.linspace(0.0, length, c.imx)
+    ys = tanh_stretch(c.jmx, first / height, 1.0) * height
+    zs = np.linspace(0.0, span, c.kmx)
+    x[0, 3:-3, 3:-3, 3:-3] = xs[None, None, :]
+    x[1, 3:-3, 3:-3, 3:-3] = ys[None, :, None]
+    x[2, 3:-3, 3:-3, 3:-3] = zs[:, None, None]
+    q = freestream_state(c)
+    # cell centres of the interior cells
+    xc = 0.5 * (xs[1:] + xs[:-1]) + 0.05 * length
+    yc = 0.5 * (ys[1:] + ys[:-1])
+    rex = c.rho_inf * c.vel_mag * xc / c.mu_ref
+    delta = 0.37 * xc * rex ** -0.2
+    prof = np.minimum(1.0, (yc[:, None] / delta[None, :]) ** (1.0 / 7.0))  # (j, i)
+    q[1, 3:-3, 3:-3, 3:-3] = c.u_inf * prof[None, :, :]
+    q = uniform_noise(c, q, seed, 0.01)
+    return c, x, q
+
+
+def chain_blocks(nblocks=64, lo=16, hi=96, seed=3, flux="ausmUP", recon="muscl", h=1.0 / 64, warp=0.02):
+    """A seeded chain of `nblocks` laminar blocks (MUSCL + AUSM-UP, M=0.8):
+    extents drawn from U{lo..hi}; block b+1 sits against block b across the
+    faces of direction (b mod 3) and keeps block b's two extents along that
+    face (the reference's interfaces join faces of equal extent), drawing
+    the third anew.  Cells are cubes of side h, and every node is moved by
+    the smooth warp x += warp sin(2 pi y) sin(2 pi z) of the global
+    coordinates, so the nodes of joined faces coincide.  Faces not on the
+    chain: subsonic inlet (-3) on i-min, outlet (-4) on i-max, adiabatic
+    walls (-5) on j and k.  State: free stream plus 1% seeded noise.
+
+    Returns ([(case, nodes, q)], links)."""
+    from .multiblock import Link, aligned_link
+    u = splitmix_uniform(seed, 3 + nblocks)
+    ext = [int(lo + min(hi - lo, int(u[a] * (hi - lo + 1)))) for a in range(3)]
+    sizes, origins = [list(ext)], [[0, 0, 0]]
+    for b in range(1, nblocks):
+        d = (b - 1) % 3
+        e = list(sizes[-1])
+        o = list(origins[-1])
+        o[d] += e[d]
+        e[d] = int(lo + min(hi - lo, int(u[3 + b] * (hi - lo + 1))))
+        sizes.append(e)
+        origins.append(o)
+    blocks = []
+    for b in range(nblocks):
+        ni, nj, nk = sizes[b]
+        ids = [-3, -4, -5, -5, -5, -5]
+        if b > 0:
+            d = (b - 1) % 3
+            ids[2 * d] = b - 1
+        if b < nblocks - 1:
+            d = b % 3
+            ids[2 * d + 1] = b + 1
+        c = Case(ni + 1, nj + 1, nk + 1, flux=flux, recon=recon, turb="none", time_local=1, accur=0,
+                 lim=(1, 1, 1), tlim=(0, 0, 0), bc_id=tuple(ids), CFL=0.5, name=f"chain{nblocks}_b{b}")
+        freestream(c, mach=0.8, reynolds=1.0e4)
+        c.Pr = 0.72
+        c.fix_rho, c.fix_u, c.fix_v = (c.rho_inf,) * 6, (c.u_inf,) * 6, (c.v_inf,) * 6
+        c.fix_w, c.fix_p, c.fix_twall = (c.w_inf,) * 6, (c.p_inf,) * 6, (0.0,) * 6
+        nodes = np.zeros(c.node_shape())
+        ii = origins[b][0] + np.arange(ni + 1)
+        jj = origins[b][1] + np.arange(nj + 1)
+        kk = origins[b][2] + np.arange(nk + 1)
+        K, J, I = np.meshgrid(kk * h, jj * h, ii * h, indexing="ij")
+        nodes[0, 3:-3, 3:-3, 3:-3] = I + warp * np.sin(2 * np.pi * J) * np.sin(2 * np.pi * K)
+        nodes[1, 3:-3, 3:-3, 3:-3] = J
+        nodes[2, 3:-3, 3:-3, 3:-3] = K
+        q = uniform_noise(c, freestream_state(c), seed * 1000 + b, 0.01)
+        blocks.append((c, nodes, q))
+    cases = [c for c, _, _ in blocks]
+    links = []
+    for b in range(nblocks - 1):
+        d = b % 3
+        links.append(aligned_link(cases, b, 2 * d + 2, b + 1, 2 * d + 1))
+        links.append(aligned_link(cases, b + 1, 2 * d + 1, b, 2 * d + 2))
+    return blocks, links
+
+
+WORKLOADS = {"laminar_box": laminar_box, "sst_bump": sst_bump, "flat_plate": flat_plate}

@@ -96,6 +96,8 @@ def lib():
         L.fest_kernel_time.argtypes = [vp, ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_long)]
         L.fest_set_option.argtypes = [vp, ctypes.c_char_p, ctypes.c_long]
         L.fest_group_set_option.argtypes = [vp, ctypes.c_char_p, ctypes.c_long]
+        L.fest_group_autotune.argtypes = [vp, ip, dp, ip, ctypes.POINTER(ip), ctypes.POINTER(ip)]
+        L.fest_group_autotune.restype = ctypes.c_int
         for f in ("fest_set_option", "fest_group_set_option", "fest_create", "fest_upload_geometry",
                   "fest_set_grid", "fest_upload_state",
                   "fest_get_next_solution", "fest_residual", "fest_time_step", "fest_download", "fest_resnorm",
@@ -320,8 +322,22 @@ class GpuGroup:
         return out[: self.blocks[0].case.n_var + 1]
 
     def set_option(self, name: str, value: int):
-        """fest_group_set_option: "graphs" (0/1), "check_interval" (n >= 1)."""
+        """fest_group_set_option: "graphs" (0/1), "check_interval" (n >= 1),
+        the schedule "batched" (0/1), "grid_cap" (work-groups per batched
+        launch, 0 one per tile), "order" (0/1)."""
         self._check(lib().fest_group_set_option(self.g, name.encode(), int(value)))
+
+    AUTOTUNE_CANDIDATES = ["per-block"] + [f"batched cap={c} order={o}" for o in (0, 1)
+                                           for c in ("tiles", "16/CU", "8/CU", "4/CU")]
+
+    def autotune(self, steps: int = 4):
+        """fest_group_autotune: times every schedule on a snapshot of the
+        state (restored afterwards) and keeps the fastest.  Returns
+        (chosen index, [ms per step of each candidate])."""
+        ms = np.zeros(16)
+        nc, ch = ctypes.c_int(), ctypes.c_int()
+        self._check(lib().fest_group_autotune(self.g, int(steps), _ptr(ms), 16, ctypes.byref(nc), ctypes.byref(ch)))
+        return ch.value, list(ms[: nc.value])
 
     def get_next_solution(self, current_iter: int, cfl: float = None):
         c = self.blocks[0].case.CFL if cfl is None else cfl

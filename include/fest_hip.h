@@ -267,9 +267,29 @@ int fest_group_resnorm(fest_group *g, double *res_abs);
  * the interface exchange opening every residual evaluation
  * (src/update.f90:534). */
 int fest_group_get_next_solution(fest_group *g, int current_iter, double cfl);
-/* fest_set_option's "graphs" and "check_interval" for a group's step: the
- * exchange and every block's kernels of one step form one graph. */
+/* fest_set_option's "graphs" and "check_interval" for a group's step (the
+ * exchange and every block's kernels of one step form one graph), and its
+ * schedule (no counterpart in the reference, whose blocks are MPI ranks):
+ *   "batched"   0 (default): kernel by kernel per block; 1: every phase of a
+ *               stage (ghost cells, gradients, each face sweep, the time
+ *               step, the update) is one launch over the tile work list of
+ *               all blocks.  Needs blocks of one kernel variant (same
+ *               variables, reconstruction, model and precision; 3-D; local
+ *               time steps; RK4, RK2 or none), else FEST_E_CONFIG.
+ *   "grid_cap"  work-groups per batched launch, each walking the work list
+ *               (a persistent launch); 0 (default): one per tile.
+ *   "order"     0: each block's tiles together; 1: the blocks' tiles
+ *               interleaved.
+ * The results do not depend on the schedule. */
 int fest_group_set_option(fest_group *g, const char *name, long value);
+/* Autotuner of the group step's schedule (no counterpart in the reference):
+ * times `steps` (>= 3; the first two capture the graph) steps of every
+ * candidate -- per-block launches, and batched with one work-group per
+ * tile or a persistent 4, 8 or 16 work-groups per compute unit, tiles in
+ * either order -- on a snapshot of the state, restores the state and keeps
+ * the fastest.  ms[c] receives candidate c's time per step (up to ms_len),
+ * *ncand the number of candidates and *chosen the pick (nullable). */
+int fest_group_autotune(fest_group *g, int steps, double *ms, int ms_len, int *ncand, int *chosen);
 
 #ifdef __cplusplus
 }

@@ -276,3 +276,27 @@ def test_split_along_j_and_k_equals_single_block_inviscid(axis, cuts):
     parts = [np.take(p, np.arange(bounds[b + 1] - bounds[b]), axis=3 - axis) for b, p in enumerate(parts)]
     got = np.concatenate(parts, axis=3 - axis)
     assert np.array_equal(got, single.arr("qp")[:, 3:-3, 3:-3, 3:-3])
+
+
+def test_chain_blocks_join_matching_faces():
+    """The seeded chain of uneven blocks: every link joins faces of equal
+    extent whose nodes coincide (after the global warp), the links pass the
+    oracle's map check, and an oracle group step runs."""
+    blocks, links = C.chain_blocks(nblocks=7, lo=6, hi=12, seed=3)
+    for L in links:
+        cr, nr, _ = blocks[L.block]
+        cs, ns, _ = blocks[L.from_block]
+        d = (L.face - 1) // 2
+        ax = 3 - d  # node array axis of the face normal
+        idx_r = 3 if L.face % 2 == 1 else nr.shape[ax] - 4
+        idx_s = 3 if L.from_face % 2 == 1 else ns.shape[ax] - 4
+        fr = np.take(nr[:, 3:-3, 3:-3, 3:-3], idx_r - 3, axis=ax)
+        fs = np.take(ns[:, 3:-3, 3:-3, 3:-3], idx_s - 3, axis=ax)
+        assert fr.shape == fs.shape
+        np.testing.assert_allclose(fr, fs, rtol=0, atol=1e-12)
+    os_ = oracles(blocks)
+    assert group_exchange(os_, links) == 0
+    for it in (1, 2):
+        assert group_step(os_, links, it) == 0
+    full = C.chain_blocks.__defaults__
+    assert full[0] == 64  # the benchmark's chain has 64 blocks
