@@ -9,12 +9,16 @@
 !<
 !< and is compiled into the reference build in place of src/update.f90.  The
 !< first call creates the device block from the solver's derived types and
-!< uploads the geometry (cells, faces, wall distance); every call uploads
-!< qp, runs one whole time step on the device (Temp refresh, all RK stages
-!< or the LU-SGS update, the stage-1 time step) and downloads qp, Temp,
-!< residue, delta_t and the face fluxes F, G, H that find_resnorm and the
-!< writers read next.  The parallel LU-SGS variant 'plusgs' stays on the
-!< host path.
+!< uploads the geometry (cells, faces, wall distance).  By default every call
+!< uploads qp, runs one whole time step on the device (Temp refresh, all RK
+!< stages or the LU-SGS update, the stage-1 time step) and downloads qp,
+!< Temp, residue, delta_t and the face fluxes F, G, H that find_resnorm and
+!< the writers read next.  With gpu_resident = .true. the state stays on the
+!< device: qp is uploaded by the first call only, and qp and Temp come back
+!< only at the solver's checkpoint iterations (src/read_write/write/
+!< dump_solution.f90:42-44) or through gpu_state_to_host; the residual norms
+!< then come from the device (resnorm_gpu.f90, the drop-in resnorm module).
+!< The parallel LU-SGS variant 'plusgs' stays on the host path.
 module update
   use iso_c_binding
   use vartypes
@@ -23,11 +27,15 @@ module update
   implicit none
   private
 
-  type(c_ptr), save :: engine = c_null_ptr
+  !< the device block; read by the drop-in resnorm module
+  type(c_ptr), public, protected, save :: engine = c_null_ptr
+  !< .true.: the state stays on the device between calls (see above)
+  logical, public, save :: gpu_resident = .false.
 
   public :: setup_update
   public :: get_next_solution
   public :: destroy_update
+  public :: gpu_state_to_host
 
 contains
 
@@ -224,11 +232,24 @@ contains
     integer(c_int) :: rc
     integer(c_int32_t) :: what
 
-    if (.not. c_associated(engine)) call create_engine(cells, Ifaces, Jfaces, Kfaces, scheme, flow, bc, dims)
-    rc = fest_upload_state(engine, qp, c_null_ptr)
-    call fest_check(rc, engine, 'fest_upload_state')
+    logical :: first
+
+    first = .not. c_associated(engine)
+    if (first) call create_engine(cells, Ifaces, Jfaces, Kfaces, scheme, flow, bc, dims)
+    if (first .or. .not. gpu_resident) then
+      rc = fest_upload_state(engine, qp, c_null_ptr)
+      call fest_check(rc, engine, 'fest_upload_state')
+    end if
     rc = fest_get_next_solution(engine, int(control%current_iter, c_int), real(control%CFL, c_double))
     call fest_check(rc, engine, 'fest_get_next_solution')
+    if (gpu_resident) then
+      ! the state the checkpoint is about to dump (dump_solution.f90:42-44)
+      if (control%checkpoint_iter /= 0) then
+        if (mod(control%current_iter, control%checkpoint_iter) == 0 .or. &
+            control%current_iter == control%max_iters) call gpu_state_to_host(qp, Temp, dims)
+      end if
+      return
+    end if
     what = ior(ior(ior(FEST_DL_QP, FEST_DL_TEMP), ior(FEST_DL_RESIDUE, FEST_DL_DT)), &
                ior(ior(FEST_DL_F, FEST_DL_G), FEST_DL_H))
     rc = fest_download(engine, what, c_loc(qp), c_loc(Temp), c_loc(residue), c_loc(F), c_loc(G), &
@@ -236,5 +257,18 @@ contains
                        c_null_ptr, c_null_ptr, c_null_ptr)
     call fest_check(rc, engine, 'fest_download')
   end subroutine get_next_solution
+
+  !< qp and Temp of the device block into the host arrays
+  subroutine gpu_state_to_host(qp, Temp, dims)
+    type(extent), intent(in) :: dims
+    real(wp), dimension(-2:dims%imx+2,-2:dims%jmx+2,-2:dims%kmx+2,1:dims%n_var), intent(inout), target :: qp
+    real(wp), dimension(-2:dims%imx+2,-2:dims%jmx+2,-2:dims%kmx+2), intent(inout), target :: Temp
+    integer(c_int) :: rc
+    if (.not. c_associated(engine)) return
+    rc = fest_download(engine, ior(FEST_DL_QP, FEST_DL_TEMP), c_loc(qp), c_loc(Temp), c_null_ptr, c_null_ptr, &
+                       c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, &
+                       c_null_ptr, c_null_ptr, c_null_ptr)
+    call fest_check(rc, engine, 'fest_download')
+  end subroutine gpu_state_to_host
 
 end module update

@@ -47,14 +47,18 @@ def test_input_stream_layout(tmp_path):
 @pytest.mark.skipif(not (ref_available() and os.path.exists(FLANG)), reason="reference build or amdflang absent")
 def test_update_module_compiles_against_reference_types(tmp_path):
     ref = os.path.join(ROOT, "oracle", "_ref")
-    for src, obj in (("fest_gpu.f90", "fest_gpu.o"), ("update_gpu.f90", "update.o")):
+    for src, obj in (("fest_gpu.f90", "fest_gpu.o"), ("update_gpu.f90", "update.o"),
+                     ("resnorm_gpu.f90", "resnorm.o")):
         p = subprocess.run([FLANG, "-cpp", "-fdefault-real-8", "-O2", "-fPIC", "-I", ref, "-I", str(tmp_path),
                             "-module-dir", str(tmp_path), "-c", os.path.join(PKG, "fortran", src),
                             "-o", str(tmp_path / obj)], capture_output=True, text=True, timeout=300)
         assert p.returncode == 0, p.stderr
     syms = subprocess.run(["nm", str(tmp_path / "update.o")], capture_output=True, text=True).stdout
-    for name in ("setup_update", "get_next_solution"):
+    for name in ("setup_update", "get_next_solution", "gpu_state_to_host"):
         assert f"_QMupdateP{name}" in syms, name
+    syms = subprocess.run(["nm", str(tmp_path / "resnorm.o")], capture_output=True, text=True).stdout
+    for name in ("setup_resnorm", "find_resnorm"):
+        assert f"_QMresnormP{name}" in syms, name
 
 
 def _compare(case, nodes, q, nsteps, wall):
