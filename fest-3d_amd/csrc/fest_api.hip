@@ -29,10 +29,10 @@ using namespace fest;
 
 // kernel groups timed by fest_timing (one or a few launches each)
 enum { KG_STEP_BEGIN = 0, KG_BC_FACE, KG_BC_EDGES, KG_GRAD, KG_GRAD_BC, KG_FLUX_I, KG_FLUX_J, KG_FLUX_K,
-       KG_TIME_STEP, KG_UPDATE, KG_LU_DIAG, KG_LU_SWEEP, KG_LU_APPLY, KG_COUNT };
+       KG_TIME_STEP, KG_UPDATE, KG_LU_DIAG, KG_LU_SWEEP, KG_LU_APPLY, KG_KKL_SOURCE, KG_COUNT };
 static const char *const kg_names[KG_COUNT] = {"step_begin", "bc_face", "bc_edges", "grad", "grad_bc",
                                                "flux_i", "flux_j", "flux_k", "time_step", "update",
-                                               "lusgs_diag", "lusgs_sweep", "lusgs_apply"};
+                                               "lusgs_diag", "lusgs_sweep", "lusgs_apply", "kkl_source"};
 
 struct KgTimer {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
@@ -358,7 +358,7 @@ int launch_residual_front(fest_handle *h, int current_iter, int store_csnd) {
     }
     if (b.kkl) {
       box(b.imx - 1, b.jmx - 1, b.kmx - 1, g, t, b.xcd_remap);
-      KgScope ts(h, KG_GRAD_BC);
+      KgScope ts(h, KG_KKL_SOURCE);
       hipLaunchKernelGGL(k_kkl_source, g, t, 0, st, b);
     }
     LAUNCHCHK(h);
@@ -1243,7 +1243,7 @@ bool phase_grid(const Blk &b, int ph, dim3 &g) {
 }
 
 // why one batched kernel variant cannot serve every block ("" when it can)
-std::string batch_reason(const fest_group *g) {
+extern "C++" std::string batch_reason(const fest_group *g) {
   const Blk &a = g->blocks[0]->b;
   for (fest_handle *h : g->blocks) {
     const Blk &b = h->b;
@@ -1348,10 +1348,13 @@ int run_group_op_batched(fest_group *g, const StepOp &op, int current_iter) {
   auto grid = [g](int ph) -> unsigned {
     return (unsigned)(g->grid_cap > 0 ? std::min(g->ph_n[ph], g->grid_cap) : g->ph_n[ph]);
   };
-#define KM(ph, kern, bd, ...)                                                                              \
-  if (g->ph_n[ph])                                                                                        \
-  hipLaunchKernelGGL(kern, dim3(grid(ph)), bd, 0, st, (const Blk *)g->d_blks, (const WorkItem *)(g->d_items + g->ph_off[ph]), \
-                     g->ph_n[ph], ##__VA_ARGS__)
+  // one statement, so that it nests under if / else like a function call
+#define KM(ph, kern, bd, ...)                                                                        \
+  do {                                                                                               \
+    if (g->ph_n[ph])                                                                                 \
+      hipLaunchKernelGGL(kern, dim3(grid(ph)), bd, 0, st, (const Blk *)g->d_blks,                    \
+                         (const WorkItem *)(g->d_items + g->ph_off[ph]), g->ph_n[ph], ##__VA_ARGS__); \
+  } while (0)
   int rc;
   switch (op.kind) {
     case OP_BEGIN: KM(PH_BEGIN, km_step_begin, dim3(256), op.flags); break;

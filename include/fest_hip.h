@@ -194,7 +194,9 @@ int fest_device_ptr(fest_handle *h, const char *name, void **ptr, size_t *count)
  * groups whose launches are bracketed with HIP events on the handle's
  * stream -- NULL or "" none, "*" all, else a comma-separated list of
  * "step_begin", "bc_face", "bc_edges", "grad", "grad_bc", "flux_i",
- * "flux_j", "flux_k", "time_step", "update".  fest_kernel_time waits for
+ * "flux_j", "flux_k", "time_step", "update", "lusgs_diag", "lusgs_sweep",
+ * "lusgs_apply", "kkl_source" (the k-kL source pass, k-kL blocks only).
+ * fest_kernel_time waits for
  * the stream and returns one group's summed event time and the number of
  * bracketed launches since the last reset. */
 int fest_timing(fest_handle *h, const char *groups);
