@@ -258,6 +258,8 @@ def chain_run(args):
         g.upload_state(q)
         gs.append(g)
     grp = GpuGroup(gs, links)
+    if args.no_graphs:
+        grp.set_option("graphs", 0)
     it = 1
     for _ in range(args.warmup):
         grp.get_next_solution(it)
@@ -304,6 +306,8 @@ def main():
                     help="skip the run with FP32 face reconstruction and flux arithmetic")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"],
                     help="face arithmetic of the chain64 workload")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch kernel by kernel instead of replaying captured step graphs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -326,6 +330,8 @@ def main():
         import dataclasses
         case = dataclasses.replace(case, time_accuracy="implicit")
     g = GpuBlock(case)
+    if args.no_graphs:
+        g.set_option("graphs", 0)
     g.set_grid(nodes, wall_dist=case.sst_family)
     g.upload_state(q)
     it = 1

@@ -116,6 +116,7 @@ struct fest_group {
   int order = 0;          // 0: tiles block by block; 1: the blocks' tiles interleaved
   int lists_order = -1;   // order the device lists were built in
   Blk *d_blks = nullptr;
+  double *d_norms = nullptr;  // 16 doubles per block: norms, then the block's share from [8]
   double blk_cfl = -1.0;  // CFL of the uploaded records
   WorkItem *d_items = nullptr;
   int ph_off[PH_COUNT] = {}, ph_n[PH_COUNT] = {};
@@ -1552,6 +1553,7 @@ void fest_group_destroy(fest_group *g) {
   g->graphs.reset();
   if (g->d_blks) (void)hipFree(g->d_blks);
   if (g->d_items) (void)hipFree(g->d_items);
+  if (g->d_norms) (void)hipFree(g->d_norms);
   for (fest_handle *h : g->blocks) {
     h->stream = h->own_stream;
     h->group = nullptr;
@@ -1590,20 +1592,27 @@ int fest_group_resnorm(fest_group *g, double *res_abs) {
   if (!g || !res_abs) return FEST_E_ARG;
   if (g->broken) return FEST_E_ARG;
   const int nv = g->blocks[0]->b.nv;
-  std::vector<double> tot(nv + 1, 0.0), part(nv + 1);
-  for (fest_handle *h : g->blocks) {
+  const size_t nb = g->blocks.size();
+  // every block's share into one device array, then one copy and one wait
+  if (!g->d_norms && hipMalloc(&g->d_norms, sizeof(double) * 16 * nb) != hipSuccess)
+    return gfail(g, FEST_E_HIP, "hipMalloc of the group norms failed");
+  for (size_t n = 0; n < nb; ++n) {
+    fest_handle *h = g->blocks[n];
     Blk &b = h->b;
-    hipLaunchKernelGGL(k_norm_partial, dim3(h->norm_blocks), dim3(256), 0, h->stream, b, b.red);
-    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, h->stream, b, h->norm_blocks, (const double *)b.red,
-                       b.norms, 1);
-    if (hipGetLastError() != hipSuccess) return gfail(g, FEST_E_HIP, "norm launch failed");
-    if (hipMemcpyAsync(part.data(), b.norms + 8, sizeof(double) * (nv + 1), hipMemcpyDeviceToHost, h->stream) !=
-            hipSuccess ||
-        hipStreamSynchronize(h->stream) != hipSuccess)
-      return gfail(g, FEST_E_HIP, "norm download failed");
-    // MPI_ALLGATHER then the sum in process order (src/resnorm.f90:193-213)
-    for (int l = 0; l <= nv; ++l) tot[l] += part[l];
+    hipLaunchKernelGGL(k_norm_partial, dim3(h->norm_blocks), dim3(256), 0, g->stream, b, b.red);
+    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, g->stream, b, h->norm_blocks, (const double *)b.red,
+                       g->d_norms + 16 * n, 1);
   }
+  if (hipGetLastError() != hipSuccess) return gfail(g, FEST_E_HIP, "norm launch failed");
+  std::vector<double> part(16 * nb);
+  if (hipMemcpyAsync(part.data(), g->d_norms, sizeof(double) * 16 * nb, hipMemcpyDeviceToHost, g->stream) !=
+          hipSuccess ||
+      hipStreamSynchronize(g->stream) != hipSuccess)
+    return gfail(g, FEST_E_HIP, "norm download failed");
+  // MPI_ALLGATHER then the sum in process order (src/resnorm.f90:193-213)
+  std::vector<double> tot(nv + 1, 0.0);
+  for (size_t n = 0; n < nb; ++n)
+    for (int l = 0; l <= nv; ++l) tot[l] += part[16 * n + 8 + l];
   res_abs[0] = fabs(tot[0]);
   for (int l = 1; l <= nv; ++l) res_abs[l] = sqrt(tot[l]);
   return 0;

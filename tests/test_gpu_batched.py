@@ -76,6 +76,10 @@ def test_batched_chain_equals_per_block(opts):
         ref = o.arr("qp")[:, 3:-3, 3:-3, 3:-3]
         err = np.abs(x[:, 3:-3, 3:-3, 3:-3] - ref).max() / np.abs(ref).max()
         assert err <= TOL, err
+    # the group norms assemble the blocks' shares (src/resnorm.f90:193-213):
+    # for l >= 1, sqrt of the sum of the blocks' squared norms
+    want = np.sqrt(sum(o.resnorm()[1:] ** 2 for o in os_))
+    np.testing.assert_allclose(a.resnorm()[1:], want, rtol=1e-11)
     close(a, ga)
     close(b, gb)
 
