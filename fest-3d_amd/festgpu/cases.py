@@ -94,12 +94,15 @@ class Case:
     fix_tp: tuple = (0.0,) * 6      # total pressure of a -11 face
     tv_inf: float = 0.0             # Spalart-Allmaras free-stream nu-tilde
     fix_tv: tuple = (0.0,) * 6      # fixed nu-tilde of each face
-    transition: str = "none"       # 'bc': the intermittency of the SA and SST sources
+    transition: str = "none"       # 'bc': the intermittency of the SA and SST sources;
+                                    # 'lctm2015': the intermittency equation (variable 8, SST only)
     tu_inf: float = 0.0             # free-stream turbulence intensity, percent
     tkl_inf: float = 0.0            # k-kL free-stream kL
     fix_tkl: tuple = (0.0,) * 6     # fixed kL of each face (k-kL)
     pbs: tuple = (0, 0, 0)          # i/j/kPB_switch: the pressure-based switch of MUSCL and PPM
     limiter: str = "koren"          # MUSCL limiter: the reference's Koren-alpha, or "van_albada"
+    tgm_inf: float = 1.0            # free-stream intermittency (lctm2015; vartypes.f90:258)
+    fix_tgm: tuple = (0.0,) * 6     # fixed intermittency of each face (lctm2015)
     name: str = "case"
 
     @property
@@ -115,13 +118,17 @@ class Case:
 
     @property
     def n_var(self) -> int:
-        return 7 if self.two_equation else 6 if self.turb == "sa" else 5
+        """src/state.f90:281-320: lctm2015 adds the intermittency."""
+        n = 7 if self.two_equation else 6 if self.turb == "sa" else 5
+        return n + 1 if self.transition == "lctm2015" else n
 
     @property
     def n_grad(self) -> int:
         """Gradient slots u, v, w, T, then k, omega (SST), k, kL (k-kL) or
-        nu-tilde (SA) (src/gradients.f90:236-260)."""
-        return 6 if self.two_equation else 5 if self.turb == "sa" else 4
+        nu-tilde (SA), then the intermittency under lctm2015
+        (src/gradients.f90:236-275)."""
+        n = 6 if self.two_equation else 5 if self.turb == "sa" else 4
+        return n + 1 if self.transition == "lctm2015" else n
 
     @property
     def ncells(self) -> int:
@@ -191,6 +198,8 @@ def freestream_state(case: Case) -> np.ndarray:
     vals = [case.rho_inf, case.u_inf, case.v_inf, case.w_inf, case.p_inf]
     vals += ([case.tv_inf] if case.turb == "sa" else [case.tk_inf, case.tkl_inf] if case.turb == "kkl"
              else [case.tk_inf, case.tw_inf])
+    if case.transition == "lctm2015":
+        vals = vals[:7] + [case.tgm_inf]  # src/state.f90:255-268
     for l in range(case.n_var):
         q[l] = vals[l]
     return q
@@ -306,16 +315,21 @@ def laminar_box(ni=64, nj=64, nk=4, seed=1, flux="ausmP", recon="muscl", time_lo
     return c, nodes, q
 
 
-def sst_bump(ni=256, nj=128, nk=128, seed=2, flux="slau", recon="weno", first=1e-5, turb="sst"):
+def sst_bump(ni=256, nj=128, nk=128, seed=2, flux="slau", recon="weno", first=1e-5, turb="sst",
+             transition="none"):
     """RANS SST channel over a 0.05L sinusoidal bump: WENO5 + SLAU, M=0.2,
     Re=1e6 per L, Tu=0.1%, mu_t/mu=0.01, 0.5% seeded noise.  Boundaries:
     subsonic inlet (-3), subsonic outlet (-4), adiabatic wall at jmin (-5),
     far field at jmax (-8), periodic in k (-9).  turb="sa" gives the same
     channel with Spalart-Allmaras (nu-tilde = mu_ratio mu / rho), turb="kkl"
-    with k-kL (whose fixed omega values, never read, stay 0)."""
+    with k-kL (whose fixed omega values, never read, stay 0).
+    transition="lctm2015" adds the intermittency (free stream and inlet 1,
+    interior cells seeded uniformly in [0.05, 1] so that its production and
+    destruction terms are both active)."""
     c = Case(ni + 1, nj + 1, nk + 1, flux=flux, recon=recon, turb=turb, time_local=1,
              accur=1, lim=(1, 1, 1), tlim=(1, 1, 1), mu_sutherland=1,
-             bc_id=(-3, -4, -5, -8, -9, -9), CFL=0.5, name=f"sst_bump_{ni}x{nj}x{nk}")
+             bc_id=(-3, -4, -5, -8, -9, -9), CFL=0.5, name=f"sst_bump_{ni}x{nj}x{nk}",
+             transition=transition)
     freestream(c, mach=0.2, reynolds=1.0e6, tu_percent=0.1, mu_ratio=0.01)
     c.Pr = 0.72
     c.tPr = 0.9
@@ -330,8 +344,12 @@ def sst_bump(ni=256, nj=128, nk=128, seed=2, flux="slau", recon="weno", first=1e
     c.fix_twall = (0.0,) * 6
     c.fix_tv = (c.tv_inf,) * 6
     c.fix_tkl = (c.tkl_inf,) * 6
+    c.fix_tgm = (c.tgm_inf,) * 6
     nodes = bump_nodes(c, first=first)
     q = uniform_noise(c, freestream_state(c), seed, 0.005)
+    if transition == "lctm2015":
+        r = splitmix_uniform(seed + 101, c.ncells).reshape(c.kmx - 1, c.jmx - 1, c.imx - 1)
+        q[7, 3:-3, 3:-3, 3:-3] = 0.05 + 0.95 * r
     return c, nodes, q
 
 

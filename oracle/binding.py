@@ -45,7 +45,8 @@ class CCase(ctypes.Structure):
                 ("fix_tp", ctypes.c_double * 6), ("tv_inf", ctypes.c_double),
                 ("fix_tv", ctypes.c_double * 6), ("transition", ctypes.c_int32), ("tu_inf", ctypes.c_double),
                 ("tkl_inf", ctypes.c_double), ("fix_tkl", ctypes.c_double * 6),
-                ("pbs", ctypes.c_int32 * 3), ("muscl_limiter", ctypes.c_int32)]
+                ("pbs", ctypes.c_int32 * 3), ("muscl_limiter", ctypes.c_int32),
+                ("tgm_inf", ctypes.c_double), ("fix_tgm", ctypes.c_double * 6)]
 
 
 def to_ccase(case) -> CCase:
@@ -62,12 +63,14 @@ def to_ccase(case) -> CCase:
               "tw_inf", "vel_mag", "MInf", "gm", "R_gas", "mu_ref", "T_ref", "S_ref", "Pr", "tPr"):
         setattr(c, f, float(getattr(case, f)))
     c.tv_inf = float(case.tv_inf)
-    c.transition = {"none": 0, "bc": 1}[case.transition]
+    c.transition = {"none": 0, "bc": 1, "lctm2015": 2}[case.transition]
+    c.tgm_inf = float(case.tgm_inf)
     c.tu_inf = float(case.tu_inf)
     c.tkl_inf = float(case.tkl_inf)
     c.pbs[:] = [int(v) for v in case.pbs]
     c.muscl_limiter = {"koren": 0, "van_albada": 1}[case.limiter]
-    for f in ("fix_rho", "fix_u", "fix_v", "fix_w", "fix_p", "fix_tk", "fix_tw", "fix_twall", "fix_tp", "fix_tv", "fix_tkl"):
+    for f in ("fix_rho", "fix_u", "fix_v", "fix_w", "fix_p", "fix_tk", "fix_tw", "fix_twall", "fix_tp", "fix_tv", "fix_tkl",
+              "fix_tgm"):
         getattr(c, f)[:] = [float(v) for v in getattr(case, f)]
     return c
 

@@ -114,6 +114,8 @@ static inline double ipow(double x, int n) {
 #define SST(s) ((s)->c.turb == ORC_TURB_SST || (s)->c.turb == ORC_TURB_SST2003)
 #define SA(s) ((s)->c.turb == ORC_TURB_SA)
 #define KKL(s) ((s)->c.turb == ORC_TURB_KKL)
+/* the lctm2015 transition model: intermittency as variable 8 (SST only) */
+#define LCTM(s) ((s)->c.transition == 2)
 /* src/global/global_kkl.f90:5-14; sigma_k = sigma_phi = 1 */
 static const double zeta1_kkl = 1.2, zeta2_kkl = 0.97, zeta3_kkl = 0.13, cmu_kkl = 0.09, kappa_kkl = 0.41,
                     c11_kkl = 10.0, c12_kkl = 1.3, cd1_kkl = 4.7;
@@ -125,6 +127,7 @@ orc *orc_create(const orc_case *c) {
   s->c = *c;
   s->imx = c->imx; s->jmx = c->jmx; s->kmx = c->kmx; s->nv = c->n_var;
   s->ngrad = (c->turb == ORC_TURB_SA) ? 5 : (c->turb != ORC_TURB_NONE) ? 6 : 4;
+  if (c->transition == 2) s->ngrad += 1; /* the intermittency slot (src/gradients.f90:262-273) */
   s->pi = s->imx + 5; s->pj = s->jmx + 5; s->pk = s->kmx + 5;
   s->ncell = (size_t)s->pi * s->pj * s->pk;
   s->nif = (size_t)(s->imx + 6) * (s->jmx + 5) * (s->kmx + 5);
@@ -591,9 +594,16 @@ static void set_omega_at_wall(orc *s, int face) {
     }
 }
 
+/* lctm2015's intermittency takes a flat copy at every face that copies,
+ * whatever the turbulence variables take (src/boundary/bc_primitive.f90:
+ * 302, 385, 437, 475, 555) */
+static void tgm_copy(orc *s, int face) {
+  if (LCTM(s)) copy3(s, QPL(7), CP_FLAT, face);
+}
 static void turb_copy(orc *s, int type, int face) {
   if (SST(s) || KKL(s)) { copy3(s, QPL(5), type, face); copy3(s, QPL(6), type, face); }
   if (SA(s)) copy3(s, QPL(5), type, face);
+  tgm_copy(s, face);
 }
 /* kl_tw: the subsonic inlet fixes k-kL's kL from the omega values
  * (src/boundary/bc_primitive.f90:333-336), which k-kL never reads from
@@ -608,6 +618,7 @@ static void turb_fix_kl(orc *s, int face, int kl_tw) {
     fixv(s, QPL(6), kl_tw ? s->c.fix_tw[face] : s->c.fix_tkl[face], face);
   }
   if (SA(s)) fixv(s, QPL(5), s->c.fix_tv[face], face);
+  if (LCTM(s)) fixv(s, QPL(7), s->c.fix_tgm[face], face); /* :262, :343 */
 }
 static void turb_fix(orc *s, int face) { turb_fix_kl(s, face, 0); }
 
@@ -787,6 +798,7 @@ static int populate_ghost_primitive(orc *s, int current_iter) {
         if (SST(s)) { copy3(s, QPL(5), CP_ANTI, f); set_omega_at_wall(s, f); }
         if (SA(s)) copy3(s, QPL(5), CP_ANTI, f); /* no_slip, bc_primitive.f90:540-541 */
         if (KKL(s)) { copy3(s, QPL(5), CP_ANTI, f); copy3(s, QPL(6), CP_ANTI, f); } /* :548-550 */
+        tgm_copy(s, f); /* :553-558 */
         break;
       case -6: /* slip wall */
         copy3(s, QPL(0), CP_SYMM, f); copy3(s, QPL(4), CP_SYMM, f);
@@ -1405,7 +1417,7 @@ static void gradient_bc_face(orc *s, int face) {
 /* src/gradients.f90:276-402 */
 static int evaluate_all_gradients(orc *s) {
   double *gxs[3] = {s->a[ORC_GX], s->a[ORC_GY], s->a[ORC_GZ]};
-  const double *vars[6] = {QPL(1), QPL(2), QPL(3), s->a[ORC_TEMP], QPL(5), QPL(6)};
+  const double *vars[7] = {QPL(1), QPL(2), QPL(3), s->a[ORC_TEMP], QPL(5), QPL(6), LCTM(s) ? QPL(7) : NULL};
   int ng = s->ngrad;
   for (int dir = 0; dir < 3; ++dir) {
     if (dir == 2 && KMX <= 2) {
@@ -1479,6 +1491,19 @@ static int calculate_viscosity(orc *s) {
       double left = fmx(var1, var2);
       double arg1 = fmn(left, right);
       F1[ci] = tanh(arg1 * arg1 * arg1 * arg1);
+    }
+    if (LCTM(s)) {
+      /* src/viscosity.f90:265-279 (sst) and 390-404 (sst2003): Menter's
+       * 2015 blending, F1 = max(F1, exp(-(Ry/120)**8)).  Ry reads the
+       * scalars density and tk, which still hold the F1 loop's last cell
+       * (imx, jmx, kmx): the reference does not reload them per cell */
+      const double density = QP(0, IMX, JMX, KMX), tk = QP(5, IMX, JMX, KMX);
+      for (int k = 0; k <= KMX; ++k) for (int j = 0; j <= JMX; ++j) for (int i = 0; i <= IMX; ++i) {
+        size_t ci = CI(s, i, j, k);
+        double var1 = density * dist[ci] * sqrt(tk) / mu[ci];
+        double var2 = exp(-ipow(var1 / 120, 8));
+        F1[ci] = fmx(F1[ci], var2);
+      }
     }
   }
   if (KKL(s)) {
@@ -1579,6 +1604,19 @@ static void viscous_dir(orc *s, int d, int sst) {
           Fl[4 * nfl + fo] = Fl[4 * nfl + fo] - (area * (((Txx * uface + Txy * vface + Txz * wface + Qx) * nx) +
                                                         ((Txy * uface + Tyy * vface + Tyz * wface + Qy) * ny) +
                                                         ((Txz * uface + Tyz * vface + Tzz * wface + Qz) * nz)));
+        } else if (sst == 4) {
+          /* compute_viscous_fluxes_lctm2015, src/viscous.f90:659-746:
+           * (mu + mu_t) grad(gamma), no sigma */
+          double g0 = 0.5 * (GX(6, im, jm, km) + GX(6, i, j, k));
+          double g1 = 0.5 * (GY(6, im, jm, km) + GY(6, i, j, k));
+          double g2 = 0.5 * (GZ(6, im, jm, km) + GZ(6, i, j, k));
+          double deltgm = QP(7, i, j, k) - QP(7, im, jm, km);
+          double nc = (deltgm - (g0 * delx + g1 * dely + g2 * delz)) / d_LR;
+          g0 = g0 + (nc * delx / d_LR);
+          g1 = g1 + (nc * dely / d_LR);
+          g2 = g2 + (nc * delz / d_LR);
+          double mut_f = 0.5 * (mut[cL] + mut[cR]);
+          Fl[7 * nfl + fo] = Fl[7 * nfl + fo] - (area * ((mu_f + mut_f) * (g0 * nx + g1 * ny + g2 * nz)));
         } else if (sst == 2) {
           /* compute_viscous_fluxes_sa, src/viscous.f90:570-656 */
           double g0 = 0.5 * (GX(4, im, jm, km) + GX(4, i, j, k));
@@ -1642,6 +1680,13 @@ static int compute_viscous_fluxes(orc *s) {
   /* so does the k-kL flux (src/viscous.f90:106-109) */
   if (KKL(s))
     for (int d = 0; d < 3; ++d) viscous_dir(s, d, 3);
+  /* lctm2015: the intermittency flux, none on the k faces of a 2-D grid
+   * (src/viscous.f90:114-128) */
+  if (LCTM(s)) {
+    viscous_dir(s, 0, 4);
+    viscous_dir(s, 1, 4);
+    if (KMX != 2) viscous_dir(s, 2, 4);
+  }
   size_t nn[3] = {s->nF, s->nG, s->nH};
   double *arr[3] = {s->a[ORC_F], s->a[ORC_G], s->a[ORC_H]};
   for (int d = 0; d < 3; ++d)
@@ -1909,9 +1954,111 @@ static void add_kkl_source(orc *s) {
     res[6 * s->nR + ri] = res[6 * s->nR + ri] - S_kl;
   }
 }
+/* compute_gradient of src/CC.f90:130-195: the Green-Gauss gradient of a
+ * cell field over cells 0..imx x 0..jmx x 0..kmx into a padded cell array
+ * that is zero elsewhere */
+static void cc_gradient(orc *s, double *grad, const double *var, int dir) {
+  const double *iA = s->a[ORC_IA], *jA = s->a[ORC_JA], *kA = s->a[ORC_KA];
+  const double *in = s->a[ORC_INX + dir], *jn = s->a[ORC_JNX + dir], *kn = s->a[ORC_KNX + dir];
+  const double *vol = s->a[ORC_VOL];
+  for (size_t n = 0; n < s->ncell; ++n) grad[n] = 0.0;
+  for (int k = 0; k <= KMX; ++k)
+    for (int j = 0; j <= JMX; ++j)
+      for (int i = 0; i <= IMX; ++i) {
+        double vc = var[CI(s, i, j, k)];
+        grad[CI(s, i, j, k)] =
+            (-(var[CI(s, i - 1, j, k)] + vc) * in[IFI(s, i, j, k)] * iA[IFI(s, i, j, k)]
+             - (var[CI(s, i, j - 1, k)] + vc) * jn[JFI(s, i, j, k)] * jA[JFI(s, i, j, k)]
+             - (var[CI(s, i, j, k - 1)] + vc) * kn[KFI(s, i, j, k)] * kA[KFI(s, i, j, k)]
+             + (var[CI(s, i + 1, j, k)] + vc) * in[IFI(s, i + 1, j, k)] * iA[IFI(s, i + 1, j, k)]
+             + (var[CI(s, i, j + 1, k)] + vc) * jn[JFI(s, i, j + 1, k)] * jA[JFI(s, i, j + 1, k)]
+             + (var[CI(s, i, j, k + 1)] + vc) * kn[KFI(s, i, j, k + 1)] * kA[KFI(s, i, j, k + 1)]) /
+            (2 * vol[CI(s, i, j, k)]);
+      }
+}
+
+/* add_sst_source_lctm2015, src/source.f90:273-463, with the wall-normal
+ * velocity-gradient sensor of src/CC.f90.  find_DCCVn takes the gradient
+ * of the wall distance (CC.f90:117-120 pass `dist`, not CCVn), and the
+ * cell-centre normal is that same gradient normalised at set-up
+ * (CC.f90:83-98), so dvdy = |grad d|^2 / (|grad d| + 1e-12). */
+static void add_sst_source_lctm2015(orc *s) {
+  double *R = s->a[ORC_RES], *mut = s->a[ORC_MUT], *F1a = s->a[ORC_F1], *vol = s->a[ORC_VOL];
+  const double *mu = s->a[ORC_MU], *dist = s->a[ORC_DIST];
+  int s2003 = s->c.turb == ORC_TURB_SST2003;
+  double gama1 = s2003 ? 5.0 / 9.0 : (beta1 / bstar) - ((sigma_w1 * (kappa_sst * kappa_sst)) / sqrt(bstar));
+  double gama2 = s2003 ? 0.44 : (beta2 / bstar) - ((sigma_w2 * (kappa_sst * kappa_sst)) / sqrt(bstar));
+  int limiter = s2003 ? 10 : 20;
+  double cd_floor = s2003 ? 1e-10 : 1e-20;
+  size_t nc = s->ncell;
+  double *D = (double *)malloc(sizeof(double) * 3 * nc), *N = (double *)malloc(sizeof(double) * 3 * nc);
+  if (!D || !N) { free(D); free(N); return; }
+  for (int d = 0; d < 3; ++d) cc_gradient(s, N + d * nc, dist, d);
+  for (size_t n = 0; n < nc; ++n) {
+    double mag = sqrt(N[n] * N[n] + N[nc + n] * N[nc + n] + N[2 * nc + n] * N[2 * nc + n]);
+    N[n] = N[n] / (mag + 1e-12);
+    N[nc + n] = N[nc + n] / (mag + 1e-12);
+    N[2 * nc + n] = N[2 * nc + n] / (mag + 1e-12);
+  }
+  for (int d = 0; d < 3; ++d) cc_gradient(s, D + d * nc, dist, d);
+  for (int k = 1; k <= KMX - 1; ++k) for (int j = 1; j <= JMX - 1; ++j) for (int i = 1; i <= IMX - 1; ++i) {
+    size_t ci = CI(s, i, j, k), ri = RI(s, i, j, k);
+    double density = QP(0, i, j, k), tk = QP(5, i, j, k), tw = QP(6, i, j, k), gmi = QP(7, i, j, k);
+    double vort = sqrt(sq(GY(2, i, j, k) - GZ(1, i, j, k)) + sq(GZ(0, i, j, k) - GX(2, i, j, k)) +
+                       sq(GX(1, i, j, k) - GY(0, i, j, k)));
+    double strain = sqrt(sq(GY(2, i, j, k) + GZ(1, i, j, k)) + sq(GZ(0, i, j, k) + GX(2, i, j, k)) +
+                         sq(GX(1, i, j, k) + GY(0, i, j, k)) + 2 * sq(GX(0, i, j, k)) + 2 * sq(GY(1, i, j, k)) +
+                         2 * sq(GZ(2, i, j, k)));
+    double CD = 2 * density * sigma_w2 * (GX(4, i, j, k) * GX(5, i, j, k) + GY(4, i, j, k) * GY(5, i, j, k) +
+                                          GZ(4, i, j, k) * GZ(5, i, j, k)) / tw;
+    CD = fmx(CD, cd_floor);
+    double F1 = F1a[ci];
+    double gama = gama1 * F1 + gama2 * (1. - F1);
+    double beta = beta1 * F1 + beta2 * (1. - F1);
+    double D_k = bstar * density * tw * tk;
+    double D_w = beta * density * (tw * tw);
+    double divergence = GX(0, i, j, k) + GY(1, i, j, k) + GZ(2, i, j, k);
+    double P_k = mut[ci] * (vort * strain) - ((2.0 / 3.0) * density * tk * divergence);
+    P_k = fmn(P_k, limiter * D_k);
+    double P_w = (density * gama / mut[ci]) * P_k;
+    double lamda = (1. - F1) * CD;
+    double dvdy = D[ci] * N[ci] + D[nc + ci] * N[nc + ci] + D[2 * nc + ci] * N[2 * nc + ci];
+    double dd = dist[ci];
+    double lamd = (-7.57e-3) * (dvdy * dd * dd * density / mu[ci]) + 0.0128;
+    lamd = fmn(fmx(lamd, -1.0), 1.0);
+    double Fpg = (lamd >= 0.0) ? fmn(1.0 + 14.68 * lamd, 1.5) : fmn(1.0 - 7.34 * lamd, 3.0);
+    Fpg = fmx(Fpg, 0.0);
+    double TuL = fmn(100.0 * sqrt(2.0 * tk / 3.0) / (tw * dd), 100.0);
+    double Re_theta = 100.0 + 1000.0 * exp(-TuL * Fpg);
+    double Rev = density * dd * dd * strain / mu[ci];
+    double RT = density * tk / (mu[ci] * tw);
+    double Fturb = exp(-ipow(0.5 * RT, 4));
+    double Fonset1 = Rev / (2.2 * Re_theta);
+    double Fonset2 = fmn(Fonset1, 2.0);
+    double Fonset3 = fmx(1.0 - ipow(RT / 3.5, 3), 0.0);
+    double Fonset = fmx(Fonset2 - Fonset3, 0.0);
+    double P_gm = 100 * density * strain * gmi * (1.0 - gmi) * Fonset;
+    double D_gm = 0.06 * density * vort * gmi * Fturb * ((50.0 * gmi) - 1.0);
+    double Fon_lim = fmn(fmx((Rev / (2.2 * 1100.0)) - 1.0, 0.0), 3.0);
+    double Pk_lim = 5 * fmx(gmi - 0.2, 0.0) * (1.0 - gmi) * Fon_lim * fmx(3 * mu[ci] - mut[ci], 0.0) * strain * vort;
+    double S_k = gmi * P_k - fmx(gmi, 0.1) * D_k + Pk_lim;
+    double S_w = P_w - D_w + lamda;
+    double S_gm = P_gm - D_gm;
+    S_k = S_k * vol[ci];
+    S_w = S_w * vol[ci];
+    S_gm = S_gm * vol[ci];
+    R[5 * s->nR + ri] = R[5 * s->nR + ri] - S_k;
+    R[6 * s->nR + ri] = R[6 * s->nR + ri] - S_w;
+    R[7 * s->nR + ri] = R[7 * s->nR + ri] - S_gm;
+  }
+  free(D);
+  free(N);
+}
+
 /* add_source_term_residue, src/source.f90:94-155 */
 static void add_source_term_residue(orc *s) {
   int bc = s->c.transition == 1;
+  if (SST(s) && LCTM(s)) { add_sst_source_lctm2015(s); return; }
   if (SST(s)) { if (bc) add_sst_bc_source(s); else add_sst_source(s); }
   if (SA(s)) { if (bc) add_sa_bc_source(s); else add_sa_source(s); }
   if (KKL(s)) add_kkl_source(s); /* no transition model (src/source.f90:147-150) */
@@ -1958,8 +2105,13 @@ void orc_temp_refresh(orc *s) {
   for (size_t n = 0; n < s->ncell; ++n) T[n] = QPL(4)[n] / (s->c.R_gas * QPL(0)[n]);
 }
 
+/* lctm2015 runs with SST and SST-2003 only: src/source.f90:124-131 stops
+ * on it under SA, and k-kL's source never reads it (:147-150) */
+static int transition_ok(const orc *s) { return !LCTM(s) || SST(s); }
+
 int orc_stage(orc *s, int current_iter, int mask) {
   int st;
+  if (!transition_ok(s)) return ORC_E_CONFIG;
   if ((mask & 1) && (st = populate_ghost_primitive(s, current_iter))) return st;
   if ((mask & 2) && (st = compute_face_interpolant(s))) return st;
   if ((mask & 4) && (st = compute_fluxes(s))) return st;
@@ -1982,6 +2134,7 @@ void orc_face_flux(orc *s, const double *qL, const double *qR, double nx, double
 
 int orc_residual(orc *s, int current_iter) {
   int st;
+  if (!transition_ok(s)) return ORC_E_CONFIG;
   if ((st = populate_ghost_primitive(s, current_iter))) return st;
   if ((st = compute_face_interpolant(s))) return st;
   if ((st = compute_fluxes(s))) return st;
@@ -2335,6 +2488,8 @@ static double sa_diagonal(const orc *s, int i, int j, int k, double D) {
  * conservative update of the interior cells. */
 static int lusgs(orc *s) {
   const orc_case *c = &s->c;
+  /* update_lctm2015 (src/lusgs.f90:2262-2669) is not restated */
+  if (LCTM(s)) return ORC_E_CONFIG;
   int nv = s->nv, sst = SST(s), sa = SA(s), kkl = KKL(s);
   double gm = c->gm;
   size_t ndq = (size_t)(IMX + 1) * (JMX + 1) * (KMX + 1);
@@ -2500,7 +2655,7 @@ int orc_step(orc *s, int current_iter) {
 /* src/resnorm.f90:128-225 */
 void orc_resnorm(orc *s, double *out) {
   const orc_case *c = &s->c;
-  double scale[8];
+  double scale[9];
   scale[0] = 1.;
   scale[1] = c->rho_inf * c->vel_mag;
   scale[2] = scale[3] = scale[4] = c->rho_inf * c->vel_mag * c->vel_mag;
@@ -2508,6 +2663,10 @@ void orc_resnorm(orc *s, double *out) {
   /* nu-tilde for SA, k for SST (src/resnorm.f90:150-158) */
   scale[6] = c->rho_inf * c->vel_mag * (c->turb == ORC_TURB_SA ? c->tv_inf : c->tk_inf);
   scale[7] = c->rho_inf * c->vel_mag * (c->turb == ORC_TURB_KKL ? c->tkl_inf : c->tw_inf);
+  /* lctm2015's intermittency: setup_scale (resnorm.f90:128-168) leaves
+   * Res_scale(8) unset, so that norm has no defined value there; it is
+   * scaled here like the other turbulence variables, by rho V gamma_inf */
+  scale[8] = c->rho_inf * c->vel_mag * c->tgm_inf;
   double *R = s->a[ORC_RES];
   for (int l = 0; l < s->nv; ++l) {
     double sum = 0.;

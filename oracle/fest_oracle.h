@@ -13,7 +13,7 @@
  *   F (1:imx,1:jmx-1,1:kmx-1,n_var), G (1:imx-1,1:jmx,..), H (..,1:kmx)
  *   residue, R_store, delta_t (1:imx-1,1:jmx-1,1:kmx-1[,n_var])
  *   face states x (0:imx+1,1:jmx-1,1:kmx-1,n_var), y, z analogues
- *   gradients (0:imx,0:jmx,0:kmx,n_grad), slots [u,v,w,T(,k,w)]
+ *   gradients (0:imx,0:jmx,0:kmx,n_grad), slots [u,v,w,T(,k,w)(,gamma)]
  */
 #ifndef FEST_ORACLE_H
 #define FEST_ORACLE_H
@@ -57,12 +57,14 @@ typedef struct {
   double fix_tp[6]; /* total pressure of a -11 face */
   double tv_inf;     /* Spalart-Allmaras free-stream nu-tilde */
   double fix_tv[6];  /* fixed nu-tilde of each face */
-  int32_t transition; /* 0 none, 1 'bc' */
+  int32_t transition; /* 0 none, 1 'bc', 2 'lctm2015' (SST and SST-2003 only) */
   double tu_inf;      /* free-stream turbulence intensity, percent */
   double tkl_inf;     /* free-stream kL (k-kL) */
   double fix_tkl[6];  /* fixed kL of each face (k-kL) */
   int32_t pbs[3];     /* i/j/kPB_switch: pressure-based switch of MUSCL and PPM */
   int32_t muscl_limiter; /* ORC_LIM_* */
+  double tgm_inf;     /* free-stream intermittency (lctm2015, variable 8) */
+  double fix_tgm[6];  /* fixed intermittency of each face (lctm2015) */
 } orc_case;
 
 typedef struct orc orc;

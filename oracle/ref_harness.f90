@@ -27,6 +27,7 @@ module ref_harness
   use global_sst,                    only: sst_F1
   use wall_dist,                     only: setup_wall_dist, find_wall_dist, dist
   use lusgs,                         only: setup_lusgs, update_with_lusgs
+  use CC,                            only: setupCC, CCnormalX
   implicit none
   private
 
@@ -48,6 +49,7 @@ module ref_harness
     real(c_double) :: tu_inf
     real(c_double) :: tkl_inf, fix_tkl(6)
     integer(c_int32_t) :: pbs(3), muscl_limiter
+    real(c_double) :: tgm_inf, fix_tgm(6)
   end type hcase
 
   type(extent) :: dims
@@ -94,6 +96,7 @@ contains
     end if
     schm%transition = 'none'
     if (hc%transition == 1) schm%transition = 'bc'
+    if (hc%transition == 2) schm%transition = 'lctm2015'
     schm%accur = hc%accur
     schm%ilimiter_switch = hc%lim(1); schm%jlimiter_switch = hc%lim(2); schm%klimiter_switch = hc%lim(3)
     schm%itlimiter_switch = hc%tlim(1); schm%jtlimiter_switch = hc%tlim(2); schm%ktlimiter_switch = hc%tlim(3)
@@ -111,6 +114,7 @@ contains
     flow%tv_inf = hc%tv_inf
     flow%tkl_inf = hc%tkl_inf
     flow%tu_inf = hc%tu_inf
+    flow%tgm_inf = hc%tgm_inf
     flow%vel_mag = hc%vel_mag; flow%MInf = hc%MInf
     flow%gm = hc%gm; flow%R_gas = hc%R_gas; flow%mu_ref = hc%mu_ref
     ! set as state.f90:89 sets it
@@ -135,6 +139,7 @@ contains
     bc%fixed_Tpressure = hc%fix_tp
     bc%fixed_tv = hc%fix_tv
     bc%fixed_tkl = hc%fix_tkl
+    bc%fixed_tgm = hc%fix_tgm
     if (allocated(bc%make_F_flux_zero)) deallocate(bc%make_F_flux_zero, bc%make_G_flux_zero, bc%make_H_flux_zero)
     allocate(bc%make_F_flux_zero(1:imx), bc%make_G_flux_zero(1:jmx), bc%make_H_flux_zero(1:kmx))
     bc%make_F_flux_zero = 1; bc%make_G_flux_zero = 1; bc%make_H_flux_zero = 1
@@ -210,7 +215,15 @@ contains
     if (allocated(dist)) deallocate(dist)
     call setup_wall_dist(files, dims)
     call find_wall_dist(nodes, dims)
+    call cc_setup()
   end subroutine ref_wall_dist
+
+  ! the cell-centre normals of the lctm2015 sensor, from the wall distance,
+  ! as solver.f90:131-135 sets them up after find_wall_dist
+  subroutine cc_setup()
+    if (allocated(CCnormalX)) return
+    call setupCC(schm, cells, Ifaces, Jfaces, Kfaces, dims)
+  end subroutine cc_setup
 
   subroutine face_box(f, lo, hi)
     integer, intent(in) :: f
@@ -236,6 +249,7 @@ contains
   subroutine ref_set_dist(d) bind(c, name='ref_set_dist')
     real(c_double), intent(in) :: d(-2:imx+2, -2:jmx+2, -2:kmx+2)
     dist = d
+    call cc_setup()
   end subroutine ref_set_dist
 
   ! stage selects how far to go: 1 ghosts, 2 +face states, 3 +fluxes,

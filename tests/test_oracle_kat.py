@@ -702,3 +702,52 @@ def test_pressure_switch_limits(recon):
     a0 = _pb_states(case, nodes, q, (0, 0, 0))
     assert np.array_equal(a0[2], _pb_states(case, nodes, q, (1, 0, 0))[2])
     assert not np.array_equal(a0[0], L)
+
+
+def test_lctm2015_rk_update_leaves_the_intermittency():
+    """The conservative RK update writes variables 1-5 and the two
+    turbulence variables only (src/update.f90:456-484): under lctm2015 the
+    intermittency keeps its value through RK steps, while its residue is
+    computed (and enters the non-physical-state check)."""
+    case, nodes, q = C.sst_bump(12, 8, 6, flux="ausm", recon="muscl", transition="lctm2015")
+    o = Oracle(case)
+    o.set_nodes(nodes)
+    o.wall_dist()
+    o.set_state(q)
+    g0 = o.arr("qp")[7, 3:-3, 3:-3, 3:-3].copy()
+    k0 = o.arr("qp")[5, 3:-3, 3:-3, 3:-3].copy()
+    for it in (1, 2):
+        assert o.step(it) == 0
+    assert np.array_equal(o.arr("qp")[7, 3:-3, 3:-3, 3:-3], g0)
+    assert not np.array_equal(o.arr("qp")[5, 3:-3, 3:-3, 3:-3], k0)
+    assert np.abs(o.arr("res")[7]).max() > 0
+
+
+def test_lctm2015_intermittency_source_closed_form():
+    """With the intermittency at 1 everywhere its production vanishes
+    (P_gm carries gamma (1 - gamma), src/source.f90:430) and so does the
+    production limiter term of k; with k tiny and omega large the turbulent
+    Reynolds number RT = rho k/(mu omega) is ~1e-20, so Fturb =
+    exp(-(RT/2)**4) = 1.  The intermittency residue is then the face-flux
+    divergence plus D_gm V = 0.06 rho Omega (50 - 1) V, Omega the vorticity
+    magnitude of the cell's gradients."""
+    case, nodes, q = C.sst_bump(10, 8, 5, flux="ausm", recon="none", transition="lctm2015")
+    o = Oracle(case)
+    o.set_nodes(nodes)
+    o.wall_dist()
+    qq = q.copy()
+    qq[5] = 1e-12
+    qq[6] = 1e8
+    qq[7] = 1.0
+    o.set_state(qq)
+    o.temp_refresh()
+    assert o.residual(3) == 0
+    gx, gy, gz = o.arr("gx"), o.arr("gy"), o.arr("gz")
+    sl = (slice(1, -1),) * 3
+    vort = np.sqrt((gy[2][sl] - gz[1][sl]) ** 2 + (gz[0][sl] - gx[2][sl]) ** 2 + (gx[1][sl] - gy[0][sl]) ** 2)
+    rho = qq[0, 3:-3, 3:-3, 3:-3]
+    vol = o.arr("vol")[3:-3, 3:-3, 3:-3]
+    F, G, H = o.arr("F")[7], o.arr("G")[7], o.arr("H")[7]
+    div = (F[:, :, 1:] - F[:, :, :-1]) + (G[:, 1:, :] - G[:, :-1, :]) + (H[1:] - H[:-1])
+    want = div + 0.06 * rho * vort * 49.0 * vol
+    np.testing.assert_allclose(o.arr("res")[7], want, rtol=1e-10, atol=1e-12 * np.abs(want).max())

@@ -378,3 +378,45 @@ def test_lusgs_weno_nm_pressure_switch():
     case, nodes, q = C.sst_bump(12, 8, 5)
     compare_lusgs(dataclasses.replace(case, recon="weno_NM"), nodes, q, it=2, wall_dist=True)
     compare_lusgs(dataclasses.replace(case, recon="muscl", pbs=(1, 1, 1)), nodes, q, it=2, wall_dist=True)
+
+
+@pytest.mark.parametrize("turb", ["sst", "sst2003"])
+@pytest.mark.parametrize("flux,recon", [("ausm", "muscl"), ("slau", "weno"), ("ldfss0", "ppm")])
+def test_transition_lctm2015(turb, flux, recon):
+    """lctm2015: the intermittency as variable 8 -- its ghost cells, face
+    states and convective flux, its gradient slot, Menter's 2015 F1
+    blending (src/viscosity.f90:265-279, 390-404), its viscous flux
+    (src/viscous.f90:659-746) and the transition source with the CC.f90
+    pressure-gradient sensor (src/source.f90:273-463) -- bit for bit."""
+    case, nodes, q = C.sst_bump(14, 10, 6, flux=flux, recon=recon, turb=turb, transition="lctm2015")
+    compare(case, nodes, q, it=3, wall_dist=True, extra=("mut", "f1"))
+
+
+@pytest.mark.parametrize("bcs", [(-1, -2, -5, -6, -9, -9), (-3, -4, -7, -5, -8, -8), (-11, -4, -5, -8, -9, -9)])
+def test_transition_lctm2015_boundaries(bcs):
+    """The intermittency's ghost rules: fixed at inlets, a flat copy at
+    outlets, walls, slip walls and the pole, the turbulence variables'
+    inflow/outflow choice at far-field and total-pressure faces."""
+    case, nodes, q = C.sst_bump(12, 9, 6, flux="ausmUP", recon="muscl", transition="lctm2015")
+    case = dataclasses.replace(case, bc_id=bcs, fix_tgm=(0.7, 0.6, 0.5, 0.4, 0.3, 0.2),
+                               fix_tp=(1.02 * case.p_inf,) * 6)
+    compare(case, nodes, q, it=1, wall_dist=True, extra=("mut", "f1"))
+
+
+def test_transition_lctm2015_changes_the_residue():
+    """Guard against a silent no-op: against plain SST on the same state,
+    the transition changes the k and omega residues (F1 blending, the
+    intermittency-weighted production) and gives variable 8 a residue."""
+    res = {}
+    for tr in ("none", "lctm2015"):
+        case, nodes, q = C.sst_bump(14, 10, 6, flux="ausm", recon="muscl", transition=tr)
+        o = Oracle(case)
+        o.set_nodes(nodes)
+        o.wall_dist()
+        o.set_state(q)
+        o.temp_refresh()
+        assert o.residual(3) == 0
+        res[tr] = o.arr("res").copy()
+    assert (res["lctm2015"][5] != res["none"][5]).all()
+    assert (res["lctm2015"][6] != res["none"][6]).mean() > 0.5
+    assert np.abs(res["lctm2015"][7]).max() > 0
