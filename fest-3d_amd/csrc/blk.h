@@ -19,8 +19,12 @@
 namespace fest {
 
 // the fixed values of each face: rho, u, v, w, p, tk, tw, wall
-// temperature, total pressure, nu-tilde, kL
-enum { FX_RHO = 0, FX_U, FX_V, FX_W, FX_P, FX_TK, FX_TW, FX_TWALL, FX_TP, FX_TV, FX_TKL, FX_N };
+// temperature, total pressure, nu-tilde, kL, intermittency
+enum { FX_RHO = 0, FX_U, FX_V, FX_W, FX_P, FX_TK, FX_TW, FX_TWALL, FX_TP, FX_TV, FX_TKL, FX_TGM, FX_N };
+
+// residual norms per block: n_var + 1 absolute norms, then from
+// NORM_SHARE the block's unassembled share (n_var + 1 values)
+constexpr int NORM_SHARE = 9, NORM_SLOTS = 18;
 
 struct Blk {
   int imx, jmx, kmx, nv, ng;
@@ -38,7 +42,8 @@ struct Blk {
   double *flux[3];
   double *csnd[3];            // stage-1 face sound speed (time step)
   double *res, *dt, *ust, *rst;
-  double *src;                // SST source S_k, S_w of the interior cells (x volume)
+  double *src;                // SST source S_k, S_w (and lctm2015's S_gamma) of the interior cells (x volume)
+  double *dvdy;               // lctm2015: the wall-normal sensor of each cell (k_lctm_sensor)
   // LU-SGS inputs and sweep values in hyperplane-major storage (k_lusgs.h)
   int sk_j;                   // jmx + 1: positions per k row of a hyperplane
   long sk_p, sk_n;            // positions per hyperplane; hyperplanes x sk_p
@@ -68,6 +73,8 @@ struct Blk {
   double iPr, itPr, cpR;  // 1/Pr, 1/Pr_t, gamma R/(gamma-1): the heat-flux factors
   double rho_inf, u_inf, v_inf, w_inf, p_inf, tk_inf, tw_inf, tv_inf;
   int trans;                  // 1: the 'bc' intermittency in the turbulence source
+  int lctm;                   // 1: lctm2015, the intermittency as variable 8
+  double tgm_inf;             // its free-stream value
   int kkl;                    // k-kL: variables 6, 7 are k and kL
   double tkl_inf;
   double sigma_w2;            // SST's 0.856; 1 under k-kL, whose F1 stays 0

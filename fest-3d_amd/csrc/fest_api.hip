@@ -116,22 +116,22 @@ struct fest_group {
   int order = 0;          // 0: tiles block by block; 1: the blocks' tiles interleaved
   int lists_order = -1;   // order the device lists were built in
   Blk *d_blks = nullptr;
-  double *d_norms = nullptr;  // 16 doubles per block: norms, then the block's share from [8]
+  double *d_norms = nullptr;  // NORM_SLOTS doubles per block: norms, then the block's share
   double blk_cfl = -1.0;  // CFL of the uploaded records
   WorkItem *d_items = nullptr;
   int ph_off[PH_COUNT] = {}, ph_n[PH_COUNT] = {};
 };
 
 // Scratch of fest_resnorm_device, the allocation-free norm entry point: the
-// per-block partial sums of the norm reduction, 8 doubles for each of at most
-// NORM_BLOCKS_MAX blocks (64 KiB).  Each device holds its own copy until the
+// per-block partial sums of the norm reduction, NORM_ACC (9) doubles for
+// each of at most NORM_BLOCKS_MAX blocks (72 KiB).  Each device holds its own copy until the
 // process ends; host code reaches it only through the address that
 // hipGetSymbolAddress returns for the current device (scratch_dev).  Every
 // call on a device shares that device's copy: calls that overlap on one
 // device race.
 constexpr int NORM_BLOCKS_MAX = 1024;
 constexpr int SCRATCH_DEVICES = 64;
-static __device__ __align__(256) double scratch[8 * NORM_BLOCKS_MAX];
+static __device__ __align__(256) double scratch[NORM_ACC * NORM_BLOCKS_MAX];
 static double *scratch_dev[SCRATCH_DEVICES];
 
 namespace {
@@ -263,8 +263,12 @@ void launch_flux_t(const Blk &b, dim3 g, dim3 t, hipStream_t st, int store_csnd)
 // FEST_PREC_FP32_FLUX
 template <int D, int NV, bool FLAT>
 void launch_flux_nv(const Blk &b, dim3 g, dim3 t, hipStream_t st, int store_csnd) {
-  if (b.fp32) launch_flux_t<D, NV, FLAT, float>(b, g, t, st, store_csnd);
-  else launch_flux_t<D, NV, FLAT, double>(b, g, t, st, store_csnd);
+  if constexpr (NV == 8) {
+    launch_flux_t<D, NV, FLAT, double>(b, g, t, st, store_csnd);  // lctm2015 runs in FP64 only
+  } else {
+    if (b.fp32) launch_flux_t<D, NV, FLAT, float>(b, g, t, st, store_csnd);
+    else launch_flux_t<D, NV, FLAT, double>(b, g, t, st, store_csnd);
+  }
 }
 // the k faces of a 2-D grid (one cell in k) carry no convective flux
 template <int D, int NV>
@@ -279,7 +283,8 @@ void launch_flux_v(const Blk &b, dim3 g, dim3 t, hipStream_t st, int store_csnd)
 }
 template <int D>
 void launch_flux(const Blk &b, dim3 g, dim3 t, hipStream_t st, int store_csnd) {
-  if (b.nv == 7) launch_flux_v<D, 7>(b, g, t, st, store_csnd);
+  if (b.nv == 8) launch_flux_v<D, 8>(b, g, t, st, store_csnd);
+  else if (b.nv == 7) launch_flux_v<D, 7>(b, g, t, st, store_csnd);
   else if (b.nv == 6) launch_flux_v<D, 6>(b, g, t, st, store_csnd);
   else launch_flux_v<D, 5>(b, g, t, st, store_csnd);
 }
@@ -330,7 +335,10 @@ int launch_residual_front(fest_handle *h, int current_iter, int store_csnd) {
     box(b.imx + 1, b.jmx + 1, b.kmx + 1, g, t, b.xcd_remap);
     {
       KgScope ts(h, KG_GRAD);
-      if (b.trans) {
+      if (b.lctm) {
+        if (b.turb == FEST_TURB_SST2003) hipLaunchKernelGGL((k_grad<7, true>), g, t, 0, st, b);
+        else hipLaunchKernelGGL((k_grad<7, false>), g, t, 0, st, b);
+      } else if (b.trans) {
         if (b.turb == FEST_TURB_SST2003) hipLaunchKernelGGL((k_grad<6, true, true>), g, t, 0, st, b);
         else if (b.ng == 6) hipLaunchKernelGGL((k_grad<6, false, true>), g, t, 0, st, b);
         else hipLaunchKernelGGL((k_grad<5, false, true>), g, t, 0, st, b);
@@ -355,7 +363,7 @@ int launch_residual_front(fest_handle *h, int current_iter, int store_csnd) {
       for (int f = 0; f < 6; ++f) tp |= b.bc[f] == -11;
       // k-kL's mu_t needs no gradients: the ghosts keep the value from the
       // ghost state
-      if (tp && b.ng == 6 && !b.kkl) hipLaunchKernelGGL(k_mut_ghost_tp, gb, dim3(64, 4, 1), 0, st, b);
+      if (tp && b.ng >= 6 && !b.kkl) hipLaunchKernelGGL(k_mut_ghost_tp, gb, dim3(64, 4, 1), 0, st, b);
     }
     if (b.kkl) {
       box(b.imx - 1, b.jmx - 1, b.kmx - 1, g, t, b.xcd_remap);
@@ -641,6 +649,17 @@ int wall_nodes(fest_handle *h, std::vector<double> &wall) {
   return 0;
 }
 
+// lctm2015's wall-normal sensor from the current wall distance and metrics
+// (src/CC.f90 setupCC after find_wall_dist, solver.f90:131-135)
+int lctm_sensor(fest_handle *h) {
+  Blk &b = h->b;
+  if (!b.lctm) return 0;
+  dim3 g(cdiv(b.imx - 1, 64), cdiv(b.jmx - 1, 4), (unsigned)(b.kmx - 1));
+  hipLaunchKernelGGL(k_lctm_sensor, g, dim3(64, 4, 1), 0, h->stream, b);
+  LAUNCHCHK(h);
+  return 0;
+}
+
 // cell distances of one block to the surface nodes `wall`
 // (src/wall/wall_dist.f90:84-131)
 int dist_from_walls(fest_handle *h, const std::vector<double> &wall) {
@@ -656,6 +675,10 @@ int dist_from_walls(fest_handle *h, const std::vector<double> &wall) {
     hipLaunchKernelGGL(k_node_dist, dim3(cdiv(nn, 256)), dim3(256), 0, h->stream, b, (const double *)dwall, nw, nd);
     dim3 g(cdiv(b.imx + 5, 64), cdiv(b.jmx + 5, 4), (unsigned)(b.kmx + 5));
     hipLaunchKernelGGL(k_cell_dist, g, dim3(64, 4, 1), 0, h->stream, b, (const double *)nd);
+    if (b.lctm) {
+      dim3 gs(cdiv(b.imx - 1, 64), cdiv(b.jmx - 1, 4), (unsigned)(b.kmx - 1));
+      hipLaunchKernelGGL(k_lctm_sensor, gs, dim3(64, 4, 1), 0, h->stream, b);
+    }
     le = hipGetLastError();
   }
   hipError_t se = hipStreamSynchronize(h->stream);
@@ -688,10 +711,20 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
   if (scheme->turb == FEST_TURB_KKL && scheme->transition != FEST_TRANS_NONE)
     return bad("k-kL has no transition model (src/source.f90:147-150)");
   int nv_expect = scheme->turb == FEST_TURB_SA ? 6 : scheme->turb != FEST_TURB_NONE ? 7 : 5;
-  if (scheme->transition != FEST_TRANS_NONE && scheme->transition != FEST_TRANS_BC)
-    return bad("transition model not supported (none, bc)");
+  if (scheme->transition != FEST_TRANS_NONE && scheme->transition != FEST_TRANS_BC &&
+      scheme->transition != FEST_TRANS_LCTM2015)
+    return bad("transition model not supported (none, bc, lctm2015)");
   if (scheme->transition == FEST_TRANS_BC && (scheme->turb == FEST_TURB_NONE || flow->mu_ref == 0.0))
     return bad("the bc transition needs a turbulence model and mu_ref != 0");
+  if (scheme->transition == FEST_TRANS_LCTM2015) {
+    // src/source.f90:124-139: the lctm2015 source exists for SST only
+    if (scheme->turb != FEST_TURB_SST && scheme->turb != FEST_TURB_SST2003)
+      return bad("lctm2015 needs the SST or SST-2003 model (src/source.f90:124-139)");
+    if (scheme->time_accuracy == FEST_TA_IMPLICIT)
+      return bad("lctm2015 with LU-SGS (update_lctm2015, src/lusgs.f90:2262) is not built");
+    if (precision != FEST_PREC_FP64) return bad("lctm2015 runs in FP64 only");
+    nv_expect = 8;  // src/state.f90:309-317
+  }
   if (dims->n_var != nv_expect) return bad("n_var does not match the turbulence model");
   if (scheme->flux < 0 || scheme->flux > 5) return bad("unknown flux scheme");
   if (scheme->recon < 0 || scheme->recon > 4) return bad("unknown reconstruction");
@@ -744,6 +777,8 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
   memset(&b, 0, sizeof b);
   b.imx = dims->imx; b.jmx = dims->jmx; b.kmx = dims->kmx; b.nv = dims->n_var;
   b.ng = scheme->turb == FEST_TURB_SA ? 5 : scheme->turb ? 6 : 4;
+  b.lctm = scheme->transition == FEST_TRANS_LCTM2015;
+  if (b.lctm) b.ng = 7;  // the intermittency slot (src/gradients.f90:262-273)
   b.pi = b.imx + 5; b.pj = b.jmx + 5; b.pk = b.kmx + 5;
   b.nc = (long)b.pi * b.pj * b.pk;
   b.fpi[0] = b.imx + 6; b.fpj[0] = b.jmx + 5; b.fpk[0] = b.kmx + 5;
@@ -793,7 +828,7 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
   b.kkl = scheme->turb == FEST_TURB_KKL;
   b.sigma_w2 = b.kkl ? 1.0 : 0.856;
   b.tkl_inf = flow->tkl_inf;
-  b.trans = scheme->transition;
+  b.trans = scheme->transition == FEST_TRANS_BC;
   b.fp32 = precision == FEST_PREC_FP32_FLUX;
   // FP32 mode keeps only FP32 gradients where nothing but the ghost rule
   // and the face sweeps read them (no SA/k-kL source or LU-SGS diagonal)
@@ -801,6 +836,7 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
               (scheme->turb == FEST_TURB_NONE || scheme->turb == FEST_TURB_SST ||
                scheme->turb == FEST_TURB_SST2003);
   b.tu_inf = flow->tu_inf;
+  b.tgm_inf = flow->tgm_inf;
   b.Re = flow->density_inf * flow->vel_mag * 1.0 / flow->mu_ref;  // state.f90:89
   b.c2 = 1 + scheme->accur;
   b.c3 = 0.5 * scheme->accur;
@@ -813,6 +849,7 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
     b.fix[FX_TP][f] = bc->fixed_Tpressure[f];
     b.fix[FX_TV][f] = bc->fixed_tv[f];
     b.fix[FX_TKL][f] = bc->fixed_tkl[f];
+    b.fix[FX_TGM][f] = bc->fixed_tgm[f];
   }
   b.global_dt = scheme->global_time_step;
   b.CFLv = 1.0;
@@ -837,14 +874,15 @@ int fest_create(fest_handle **out, const fest_extent *dims, const fest_scheme *s
     b.q32 = (float *)q32;
     for (int d = 0; d < 3; ++d) b.g32[d] = (float *)g32 + (long)d * b.ngr * b.ng;
   }
-  AL(b.res, b.nres * b.nv); AL(b.dt, b.nres); AL(b.src, 2 * b.nres); AL(b.ust, b.nc * b.nv); AL(b.rst, b.nres * b.nv);
+  AL(b.res, b.nres * b.nv); AL(b.dt, b.nres); AL(b.src, (b.lctm ? 3 : 2) * b.nres);
+  if (b.lctm) AL(b.dvdy, b.nc); AL(b.ust, b.nc * b.nv); AL(b.rst, b.nres * b.nv);
   AL(b.nodes, 3L * (b.imx + 6) * (b.jmx + 6) * (b.kmx + 6));
   // reduction partials: the larger of the time-step block count and 8 per norm block
   long nblk_dt = (long)cdiv((long)(b.imx - 1) * (b.jmx - 1), FLAT_BLOCK) * (b.kmx - 1);
   h->norm_blocks = (int)std::min<long>(1024, std::max<long>(1, cdiv(b.nres, 256)));
-  h->red_cap = std::max<long>(nblk_dt, 8L * h->norm_blocks);
+  h->red_cap = std::max<long>(nblk_dt, (long)NORM_ACC * h->norm_blocks);
   AL(b.red, h->red_cap);
-  AL(b.norms, 16); AL(b.dtmin, 1);
+  AL(b.norms, NORM_SLOTS); AL(b.dtmin, 1);
   if (b.time_acc == FEST_TA_IMPLICIT) {
     // LU-SGS storage over the hyperplanes of (0:imx, 0:jmx, 0:kmx); zero
     // where nothing is written (the sweep values outside the interior)
@@ -909,6 +947,7 @@ int fest_upload_geometry(fest_handle *h, const double *cells4, const double *if4
       HIPCHK(h, hipMemcpyAsync(fd[a], src[d] + a * b.nfc[d], n, hipMemcpyHostToDevice, h->stream));
   }
   if (dist) HIPCHK(h, hipMemcpyAsync(b.dist, dist, nc, hipMemcpyHostToDevice, h->stream));
+  if (int rc = lctm_sensor(h)) return rc;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->have_grid = true;
   h->sk_geometry = false;
@@ -941,6 +980,7 @@ int fest_set_grid(fest_handle *h, const double *nodes, int with_wall_dist) {
     hipLaunchKernelGGL(k_cells, g, dim3(64, 4, 1), 0, h->stream, b);
   }
   LAUNCHCHK(h);
+  if (int rc = lctm_sensor(h)) return rc;  // with the distance as it stands; again below with the walls'
   if (with_wall_dist) {
     std::vector<double> wall;
     int rc = wall_nodes(h, wall);
@@ -1253,7 +1293,7 @@ extern "C++" std::string batch_reason(const fest_group *g) {
       return "blocks differ in variables, reconstruction, model, precision or integrator";
     if (b.turb != FEST_TURB_NONE && b.turb != FEST_TURB_SST && b.turb != FEST_TURB_SST2003)
       return "batched steps cover laminar and SST blocks";
-    if (b.trans) return "batched steps carry no transition model";
+    if (b.trans || b.lctm) return "batched steps carry no transition model";
     if (b.kmx == 2) return "batched steps need three-dimensional blocks";
     if (!b.time_local) return "batched steps use local time steps";
     if (b.time_acc != FEST_TA_RK4 && b.time_acc != FEST_TA_RK2 && b.time_acc != FEST_TA_NONE)
@@ -1594,25 +1634,25 @@ int fest_group_resnorm(fest_group *g, double *res_abs) {
   const int nv = g->blocks[0]->b.nv;
   const size_t nb = g->blocks.size();
   // every block's share into one device array, then one copy and one wait
-  if (!g->d_norms && hipMalloc(&g->d_norms, sizeof(double) * 16 * nb) != hipSuccess)
+  if (!g->d_norms && hipMalloc(&g->d_norms, sizeof(double) * NORM_SLOTS * nb) != hipSuccess)
     return gfail(g, FEST_E_HIP, "hipMalloc of the group norms failed");
   for (size_t n = 0; n < nb; ++n) {
     fest_handle *h = g->blocks[n];
     Blk &b = h->b;
     hipLaunchKernelGGL(k_norm_partial, dim3(h->norm_blocks), dim3(256), 0, g->stream, b, b.red);
     hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, g->stream, b, h->norm_blocks, (const double *)b.red,
-                       g->d_norms + 16 * n, 1);
+                       g->d_norms + NORM_SLOTS * n, 1);
   }
   if (hipGetLastError() != hipSuccess) return gfail(g, FEST_E_HIP, "norm launch failed");
-  std::vector<double> part(16 * nb);
-  if (hipMemcpyAsync(part.data(), g->d_norms, sizeof(double) * 16 * nb, hipMemcpyDeviceToHost, g->stream) !=
+  std::vector<double> part(NORM_SLOTS * nb);
+  if (hipMemcpyAsync(part.data(), g->d_norms, sizeof(double) * NORM_SLOTS * nb, hipMemcpyDeviceToHost, g->stream) !=
           hipSuccess ||
       hipStreamSynchronize(g->stream) != hipSuccess)
     return gfail(g, FEST_E_HIP, "norm download failed");
   // MPI_ALLGATHER then the sum in process order (src/resnorm.f90:193-213)
   std::vector<double> tot(nv + 1, 0.0);
   for (size_t n = 0; n < nb; ++n)
-    for (int l = 0; l <= nv; ++l) tot[l] += part[16 * n + 8 + l];
+    for (int l = 0; l <= nv; ++l) tot[l] += part[NORM_SLOTS * n + NORM_SHARE + l];
   res_abs[0] = fabs(tot[0]);
   for (int l = 1; l <= nv; ++l) res_abs[l] = sqrt(tot[l]);
   return 0;

@@ -73,19 +73,24 @@ __device__ void fixcol(const Blk &b, double *v, double val, int face, int a, int
 }
 
 // the turbulence variables: k and omega (SST), k and kL (k-kL), nu-tilde
-// (SA).  kl_tw: the subsonic inlet fixes k-kL's kL from the omega values
-// (src/boundary/bc_primitive.f90:333-336), which k-kL never reads from the
-// boundary file (read_bc.f90:98-108)
+// (SA); lctm2015's intermittency takes a flat copy wherever they copy
+// (bc_primitive.f90:302, 385, 437, 475, 555) and its own fixed value
+// wherever they are fixed (:262, :343).  kl_tw: the subsonic inlet fixes
+// k-kL's kL from the omega values (src/boundary/bc_primitive.f90:333-336),
+// which k-kL never reads from the boundary file (read_bc.f90:98-108)
+__device__ __forceinline__ bool two_eq(const Blk &b) { return b.ng >= 6; }
 __device__ void turb_copycol(const Blk &b, int type, int face, int a, int bb) {
-  if (b.nv >= 6) copy3col(b, b.qp + 5 * b.nc, type, face, a, bb);
-  if (b.nv == 7) copy3col(b, b.qp + 6 * b.nc, type, face, a, bb);
+  if (b.ng >= 5) copy3col(b, b.qp + 5 * b.nc, type, face, a, bb);
+  if (two_eq(b)) copy3col(b, b.qp + 6 * b.nc, type, face, a, bb);
+  if (b.lctm) copy3col(b, b.qp + 7 * b.nc, CP_FLAT, face, a, bb);
 }
 __device__ void turb_fixcol(const Blk &b, const FaceBC &p, int face, int a, int bb, bool kl_tw = false) {
-  if (b.nv == 7) {
+  if (two_eq(b)) {
     fixcol(b, b.qp + 5 * b.nc, p.fix[FX_TK], face, a, bb);
     fixcol(b, b.qp + 6 * b.nc, (b.kkl && !kl_tw) ? p.fix[FX_TKL] : p.fix[FX_TW], face, a, bb);
   }
-  if (b.nv == 6) fixcol(b, b.qp + 5 * b.nc, p.fix[FX_TV], face, a, bb);
+  if (b.ng == 5) fixcol(b, b.qp + 5 * b.nc, p.fix[FX_TV], face, a, bb);
+  if (b.lctm) fixcol(b, b.qp + 7 * b.nc, p.fix[FX_TGM], face, a, bb);
 }
 
 // unit normal of the boundary face at (a, bb)
@@ -266,9 +271,10 @@ __device__ __forceinline__ void k_bc_face_b(const Blk &b, const Wg &wg, int face
         copy3col(b, q + 1 * nc, CP_ANTI, face, a, bb);
         copy3col(b, q + 2 * nc, CP_ANTI, face, a, bb);
         copy3col(b, q + 3 * nc, CP_ANTI, face, a, bb);
-        if (b.nv >= 6) copy3col(b, q + 5 * nc, CP_ANTI, face, a, bb);  // k or nu-tilde
-        if (b.nv == 7 && b.kkl) copy3col(b, q + 6 * nc, CP_ANTI, face, a, bb);  // kL, :548-550
-        else if (b.nv == 7) omega_wall_col(b, face, a, bb);
+        if (b.ng >= 5) copy3col(b, q + 5 * nc, CP_ANTI, face, a, bb);  // k or nu-tilde
+        if (two_eq(b) && b.kkl) copy3col(b, q + 6 * nc, CP_ANTI, face, a, bb);  // kL, :548-550
+        else if (two_eq(b)) omega_wall_col(b, face, a, bb);
+        if (b.lctm) copy3col(b, q + 7 * nc, CP_FLAT, face, a, bb);  // :553-558
       }
       break;
     case -6:  // slip wall

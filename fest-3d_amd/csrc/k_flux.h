@@ -471,10 +471,11 @@ struct FaceMath {
     v[3] = (Txz * nx + Tyz * ny + Tzz * nz) * A;
     v[4] = A * (((Txx * uf + Txy * vf + Txz * wf + Qx) * nx) + ((Txy * uf + Tyy * vf + Tyz * wf + Qy) * ny) +
                 ((Txz * uf + Tyz * vf + Tzz * wf + Qz) * nz));
-    if constexpr (NV == 7) {
+    if constexpr (NV >= 7) {
       v[5] = T(0.0);
       v[6] = T(0.0);
     }
+    if constexpr (NV == 8) v[NV - 1] = T(0.0);
     if constexpr (NV == 6) {
       // nu-tilde diffusion with (mu + rho_f nu~_f) / sigma, sigma = 2/3; the
       // quotient taken as a product with 1.5 (within an ulp of it)
@@ -489,7 +490,7 @@ struct FaceMath {
       T mtv = T(0.5) * (T(qp[5 * nc + cm]) + T(qp[5 * nc + c])) * rhof;
       v[NV == 6 ? 5 : 0] = (A * ((mu_f + mtv) * (g0 * nx + g1 * ny + g2 * nz))) * T(1.5);
     }
-    if (NV == 7 && (!laminar_only || b.kkl)) {
+    if (NV >= 7 && (!laminar_only || b.kkl)) {
       // SST k / omega diffusion and the -2/3 rho k stress, src/viscous.f90:328-447;
       // k-kL's has sigma_k = sigma_phi = 1 and runs on the k faces of a 2-D
       // grid too (:450-567, :106-109)
@@ -520,15 +521,30 @@ struct FaceMath {
       v[3] = v[3] + Tkk * nz * A;
       T kfl = A * ((mu_f + sigma_kf * mut_f) * (gt[0][0] * nx + gt[0][1] * ny + gt[0][2] * nz));
       v[4] = v[4] + kfl;
-      v[NV == 7 ? 5 : 0] = kfl;
-      v[NV == 7 ? 6 : 0] = A * ((mu_f + sigma_wf * mut_f) * (gt[1][0] * nx + gt[1][1] * ny + gt[1][2] * nz));
+      v[NV >= 7 ? 5 : 0] = kfl;
+      v[NV >= 7 ? 6 : 0] = A * ((mu_f + sigma_wf * mut_f) * (gt[1][0] * nx + gt[1][1] * ny + gt[1][2] * nz));
+    }
+    if constexpr (NV == 8) {
+      // lctm2015's intermittency: (mu + mu_t) grad(gamma), no sigma, none on
+      // the k faces of a 2-D grid (src/viscous.f90:114-128, 659-746)
+      if (!laminar_only) {
+        T g0 = T(0.5) * (T(gx[6 * ngr + gL]) + T(gx[6 * ngr + gR]));
+        T g1 = T(0.5) * (T(gy[6 * ngr + gL]) + T(gy[6 * ngr + gR]));
+        T g2 = T(0.5) * (T(gz[6 * ngr + gL]) + T(gz[6 * ngr + gR]));
+        T ncmp = (T(qp[7 * nc + c] - qp[7 * nc + cm]) - (g0 * delx + g1 * dely + g2 * delz)) * id2;
+        g0 = g0 + ncmp * delx;
+        g1 = g1 + ncmp * dely;
+        g2 = g2 + ncmp * delz;
+        v[NV - 1] = A * ((mu_f + mut_f) * (g0 * nx + g1 * ny + g2 * nz));
+      }
     }
   }
 };
 
 // ---- the face kernel
 // D: sweep direction.  Faces: n = 1..N along D, 1..N-1 across.  NV: 5
-// (laminar), 6 (Spalart-Allmaras) or 7 (SST) variables; RECON: the
+// (laminar), 6 (Spalart-Allmaras), 7 (SST) or 8 (SST with lctm2015)
+// variables; RECON: the
 // reconstruction; T: the face arithmetic, double or float.  Compile-time
 // counts keep the per-variable arrays in registers.
 // Registers are fitted to 4 waves per SIMD: 5 spills the SST/WENO variants

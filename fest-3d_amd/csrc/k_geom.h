@@ -167,6 +167,41 @@ __global__ void k_cell_dist(Blk b, const double *nd) {
                                     nd[ndi(b, i + 1, j + 1, k)] + nd[ndi(b, i + 1, j + 1, k + 1)]);
 }
 
+// lctm2015's wall-normal sensor of each interior cell (src/CC.f90 and
+// src/source.f90:404-407): D = the Green-Gauss gradient of the wall
+// distance (CC.f90 compute_gradient; find_DCCVn passes the distance, not
+// the velocity component, CC.f90:117-120), n = D / (|D| + 1e-12) as the
+// set-up normalises it (CC.f90:89-97), dvdy = D . n.  The distance is fixed
+// once the grid is, so this runs at set-up, not per residual.
+__global__ void k_lctm_sensor(Blk b) {
+  int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1;
+  int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 1;
+  int k = (int)blockIdx.z + 1;
+  if (i > b.imx - 1 || j > b.jmx - 1 || k > b.kmx - 1) return;
+  const double *var = b.dist;
+  const long c = ci(b, i, j, k);
+  const double vc = var[c];
+  double D[3];
+  for (int d = 0; d < 3; ++d) {
+    const double *in = b.fx[0], *jn = b.fx[1], *kn = b.fx[2];
+    if (d == 1) { in = b.fy[0]; jn = b.fy[1]; kn = b.fy[2]; }
+    if (d == 2) { in = b.fz[0]; jn = b.fz[1]; kn = b.fz[2]; }
+    const long f0 = fi(b, 0, i, j, k), f1 = fi(b, 0, i + 1, j, k);
+    const long g0 = fi(b, 1, i, j, k), g1 = fi(b, 1, i, j + 1, k);
+    const long h0 = fi(b, 2, i, j, k), h1 = fi(b, 2, i, j, k + 1);
+    D[d] = (-(var[ci(b, i - 1, j, k)] + vc) * in[f0] * b.fA[0][f0]
+            - (var[ci(b, i, j - 1, k)] + vc) * jn[g0] * b.fA[1][g0]
+            - (var[ci(b, i, j, k - 1)] + vc) * kn[h0] * b.fA[2][h0]
+            + (var[ci(b, i + 1, j, k)] + vc) * in[f1] * b.fA[0][f1]
+            + (var[ci(b, i, j + 1, k)] + vc) * jn[g1] * b.fA[1][g1]
+            + (var[ci(b, i, j, k + 1)] + vc) * kn[h1] * b.fA[2][h1]) /
+           (2 * b.vol[c]);
+  }
+  const double mag = sqrt(D[0] * D[0] + D[1] * D[1] + D[2] * D[2]);
+  const double nx = D[0] / (mag + 1e-12), ny = D[1] / (mag + 1e-12), nz = D[2] / (mag + 1e-12);
+  b.dvdy[c] = D[0] * nx + D[1] * ny + D[2] * nz;
+}
+
 __global__ void k_fill(double *p, long n, double v) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) p[t] = v;

@@ -16,7 +16,8 @@
 
 namespace fest {
 
-// gradient slots: u, v, w, T, then k, omega (SST) or nu-tilde (SA)
+// gradient slots: u, v, w, T, then k, omega (SST) or nu-tilde (SA), then
+// lctm2015's intermittency
 __device__ __forceinline__ const double *grad_var(const Blk &b, int l) {
   return l < 3 ? b.qp + (long)(l + 1) * b.nc : l == 3 ? b.temp : b.qp + (long)(l + 1) * b.nc;
 }
@@ -24,9 +25,9 @@ __device__ __forceinline__ const double *grad_var(const Blk &b, int l) {
 // SST eddy viscosity and F1 of one cell from its gradients
 // (src/viscosity.f90:343-388); S2003: the 2003 form, with the strain rate in
 // place of the vorticity and 1e-10 floors (src/viscosity.f90:215-275)
-template <bool S2003>
+template <bool S2003, int G>
 __device__ __forceinline__ void sst_mut_f1(double rho, double tk, double tw, double dist, double mu,
-                                           const double (&g)[6][3], double &mut, double &f1) {
+                                           const double (&g)[G][3], double &mut, double &f1) {
   const double bstar = 0.09, a1 = 0.31, sigma_w2 = 0.856;
   double var1 = sqrt(tk) / (bstar * tw * dist);
   double var2 = 500 * (mu / rho) / ((dist * dist) * tw);
@@ -133,9 +134,10 @@ __device__ __forceinline__ double sa_bc_source(const Blk &b, double rho, double 
 // SST source with the 'bc' transition (src/source.f90:467-604): production
 // mu_t Omega^2 without the divergence term, limited at 20 D_k, the original
 // gama1/gama2 for either SST form, P_k scaled by gamma_BC after P_omega
+template <int G>
 __device__ __forceinline__ void sst_bc_source(const Blk &b, double rho, double vmag, double tk, double tw,
                                               double mut, double F1, double mu, double dist, double vol,
-                                              const double (&g)[6][3], double &S_k, double &S_w) {
+                                              const double (&g)[G][3], double &S_k, double &S_w) {
   const double sigma_w1 = 0.5, sigma_w2 = 0.856, beta1 = 0.075, beta2 = 0.0828, bstar = 0.09, kap = 0.41;
   const double gama1 = (beta1 / bstar) - ((sigma_w1 * (kap * kap)) / sqrt(bstar));
   const double gama2 = (beta2 / bstar) - ((sigma_w2 * (kap * kap)) / sqrt(bstar));
@@ -156,9 +158,9 @@ __device__ __forceinline__ void sst_bc_source(const Blk &b, double rho, double v
 // SST source of an interior cell, times its volume (src/source.f90:158-270);
 // S2003: production limited to 10 D_k, CD floored at 1e-10, gama1 = 5/9,
 // gama2 = 0.44 (src/source.f90:205-211)
-template <bool S2003>
+template <bool S2003, int G>
 __device__ __forceinline__ void sst_source(double rho, double tk, double tw, double mut, double F1, double vol,
-                                           const double (&g)[6][3], double &S_k, double &S_w) {
+                                           const double (&g)[G][3], double &S_k, double &S_w) {
   const double sigma_w1 = 0.5, sigma_w2 = 0.856, beta1 = 0.075, beta2 = 0.0828, bstar = 0.09, kap = 0.41;
   const double gama1 = S2003 ? 5.0 / 9.0 : (beta1 / bstar) - ((sigma_w1 * (kap * kap)) / sqrt(bstar));
   const double gama2 = S2003 ? 0.44 : (beta2 / bstar) - ((sigma_w2 * (kap * kap)) / sqrt(bstar));
@@ -176,6 +178,66 @@ __device__ __forceinline__ void sst_source(double rho, double tk, double tw, dou
   double lamda = (1. - F1) * CD;
   S_k = (P_k - D_k) * vol;
   S_w = (P_w - D_w + lamda) * vol;
+}
+
+// lctm2015's blending of F1 (src/viscosity.f90:265-279, 390-404):
+// F1 = max(F1, exp(-(Ry/120)**8)).  Ry = rho d sqrt(k)/mu reads the
+// reference's scalars density and tk, which still hold the last cell of its
+// F1 loop, (imx, jmx, kmx), not the current cell's: they are passed in.
+__device__ __forceinline__ double lctm_f1(double F1, double rho_last, double tk_last, double dist, double mu) {
+  const double var1 = rho_last * dist * sqrt(tk_last) / mu;
+  const double x = var1 / 120;
+  const double x8 = x * x * x * x * x * x * x * x;
+  return fmx(F1, exp(-x8));
+}
+
+// lctm2015 source of an interior cell, times its volume (src/source.f90:
+// 273-463): the SST terms with Kato-Launder production rho mu_t |Omega| S,
+// weighted by the intermittency, plus the onset limiter term Pk_lim; the
+// intermittency's production and destruction.  dvdy is the wall-normal
+// sensor of the cell (k_lctm_sensor); g holds slots u, v, w, T, k, omega.
+template <bool S2003, int G>
+__device__ __forceinline__ void lctm_source(double rho, double tk, double tw, double gmi, double mut, double F1,
+                                            double mu, double dist, double dvdy, double vol,
+                                            const double (&g)[G][3], double &S_k, double &S_w, double &S_gm) {
+  const double sigma_w1 = 0.5, sigma_w2 = 0.856, beta1 = 0.075, beta2 = 0.0828, bstar = 0.09, kap = 0.41;
+  const double gama1 = S2003 ? 5.0 / 9.0 : (beta1 / bstar) - ((sigma_w1 * (kap * kap)) / sqrt(bstar));
+  const double gama2 = S2003 ? 0.44 : (beta2 / bstar) - ((sigma_w2 * (kap * kap)) / sqrt(bstar));
+  const int limiter = S2003 ? 10 : 20;
+  double vort = sqrt(sqd(g[2][1] - g[1][2]) + sqd(g[0][2] - g[2][0]) + sqd(g[1][0] - g[0][1]));
+  double strain = sqrt(sqd(g[2][1] + g[1][2]) + sqd(g[0][2] + g[2][0]) + sqd(g[1][0] + g[0][1]) +
+                       2 * sqd(g[0][0]) + 2 * sqd(g[1][1]) + 2 * sqd(g[2][2]));
+  double CD = 2 * rho * sigma_w2 * (g[4][0] * g[5][0] + g[4][1] * g[5][1] + g[4][2] * g[5][2]) / tw;
+  CD = fmx(CD, S2003 ? 1e-10 : 1e-20);
+  double gama = gama1 * F1 + gama2 * (1. - F1);
+  double beta = beta1 * F1 + beta2 * (1. - F1);
+  double D_k = bstar * rho * tw * tk;
+  double D_w = beta * rho * (tw * tw);
+  double divergence = g[0][0] + g[1][1] + g[2][2];
+  double P_k = mut * (vort * strain) - ((2.0 / 3.0) * rho * tk * divergence);
+  P_k = fmn(P_k, limiter * D_k);
+  double P_w = (rho * gama / mut) * P_k;
+  double lamda = (1. - F1) * CD;
+  double lamd = (-7.57e-3) * (dvdy * dist * dist * rho / mu) + 0.0128;
+  lamd = fmn(fmx(lamd, -1.0), 1.0);
+  double Fpg = (lamd >= 0.0) ? fmn(1.0 + 14.68 * lamd, 1.5) : fmn(1.0 - 7.34 * lamd, 3.0);
+  Fpg = fmx(Fpg, 0.0);
+  double TuL = fmn(100.0 * sqrt(2.0 * tk / 3.0) / (tw * dist), 100.0);
+  double Re_theta = 100.0 + 1000.0 * exp(-TuL * Fpg);
+  double Rev = rho * dist * dist * strain / mu;
+  double RT = rho * tk / (mu * tw);
+  const double rt2 = 0.5 * RT, rt35 = RT / 3.5;
+  double Fturb = exp(-(rt2 * rt2 * rt2 * rt2));
+  double Fonset2 = fmn(Rev / (2.2 * Re_theta), 2.0);
+  double Fonset3 = fmx(1.0 - rt35 * rt35 * rt35, 0.0);
+  double Fonset = fmx(Fonset2 - Fonset3, 0.0);
+  double P_gm = 100 * rho * strain * gmi * (1.0 - gmi) * Fonset;
+  double D_gm = 0.06 * rho * vort * gmi * Fturb * ((50.0 * gmi) - 1.0);
+  double Fon_lim = fmn(fmx((Rev / (2.2 * 1100.0)) - 1.0, 0.0), 3.0);
+  double Pk_lim = 5 * fmx(gmi - 0.2, 0.0) * (1.0 - gmi) * Fon_lim * fmx(3 * mu - mut, 0.0) * strain * vort;
+  S_k = (gmi * P_k - fmx(gmi, 0.1) * D_k + Pk_lim) * vol;
+  S_w = (P_w - D_w + lamda) * vol;
+  S_gm = (P_gm - D_gm) * vol;
 }
 
 // One thread per cell 0..imx x 0..jmx x 0..kmx: all Green-Gauss gradients,
@@ -259,15 +321,24 @@ __device__ __forceinline__ void k_grad_b(const Blk &b, const Wg &wg) {
     b.mut[c] = mut;
     return;
   }
-  if constexpr (NG == 6) {
+  if constexpr (NG == 6 || NG == 7) {
   double tk = b.qp[5 * b.nc + c], tw = b.qp[6 * b.nc + c];
   double mut, F1;
   sst_mut_f1<S2003>(rho, tk, tw, b.dist[c], mu, gg, mut, F1);
+  if constexpr (NG == 7) {
+    const long cl = ci(b, b.imx, b.jmx, b.kmx);
+    F1 = lctm_f1(F1, b.qp[cl], b.qp[5 * b.nc + cl], b.dist[c], mu);
+  }
   b.mut[c] = mut;
   b.f1[c] = F1;
   if (i >= 1 && i <= b.imx - 1 && j >= 1 && j <= b.jmx - 1 && k >= 1 && k <= b.kmx - 1) {
     double S_k, S_w;
-    if constexpr (BC) {
+    if constexpr (NG == 7) {
+      double S_gm;
+      lctm_source<S2003>(rho, tk, tw, b.qp[7 * b.nc + c], mut, F1, mu, b.dist[c], b.dvdy[c], vol, gg, S_k, S_w,
+                         S_gm);
+      b.src[2 * b.nres + ri(b, i, j, k)] = S_gm;
+    } else if constexpr (BC) {
       const double u = b.qp[b.nc + c], v = b.qp[2 * b.nc + c], w = b.qp[3 * b.nc + c];
       sst_bc_source(b, rho, sqrt(u * u + v * v + w * w), tk, tw, mut, F1, mu, b.dist[c], vol, gg, S_k, S_w);
     } else {
@@ -320,7 +391,7 @@ __device__ __forceinline__ void visc_ghost(const Blk &b, int face, int a, int bb
   if (b.kkl && id == -7) return;  // k-kL: -4..-1, -6, -8, -9 and -5 (src/viscosity.f90:485-531)
   long cg = fcell(b, face, 0, a, bb), cn = fcell(b, face, 1, a, bb);
   b.mut[cg] = (id == -5) ? -b.mut[cn] : b.mut[cn];
-  if (b.ng == 6) b.f1[cg] = b.f1[cn];
+  if (b.ng >= 6) b.f1[cg] = b.f1[cn];
 }
 
 
@@ -415,7 +486,7 @@ __device__ __forceinline__ void k_mut_ghost_tp_b(const Blk &b, const Wg &wg) {
   int bb = wg.y * blockDim.y + threadIdx.y + 1;
   int na = face_na(b, face), nb = face_nb(b, face);
   if (a > na - 1 || bb > nb - 1) return;
-  if (b.bc[face] != -11 || b.ng != 6) return;
+  if (b.bc[face] != -11 || b.ng < 6) return;
   int d = face >> 1, hi = face & 1;
   int N = d == 0 ? b.imx : d == 1 ? b.jmx : b.kmx;
   int g = hi ? N : 0;  // the ghost layer next to the face
@@ -433,6 +504,10 @@ __device__ __forceinline__ void k_mut_ghost_tp_b(const Blk &b, const Wg &wg) {
   double rho = b.qp[cG], tk = b.qp[5 * b.nc + cG], tw = b.qp[6 * b.nc + cG];
   if (b.turb == FEST_TURB_SST2003) sst_mut_f1<true>(rho, tk, tw, b.dist[cG], b.mu[cG], gg, mut, F1);
   else sst_mut_f1<false>(rho, tk, tw, b.dist[cG], b.mu[cG], gg, mut, F1);
+  if (b.lctm) {
+    const long cl = ci(b, b.imx, b.jmx, b.kmx);
+    F1 = lctm_f1(F1, b.qp[cl], b.qp[5 * b.nc + cl], b.dist[cG], b.mu[cG]);
+  }
   b.mut[cG] = mut;
   b.f1[cG] = F1;
 }

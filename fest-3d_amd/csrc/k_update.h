@@ -166,7 +166,7 @@ __device__ __forceinline__ void k_update_b(const Blk &b, const Wg &wg, double TF
   long r = ri(b, i, j, k);
   long F0 = fli(b, 0, i, j, k), G0 = fli(b, 1, i, j, k), H0 = fli(b, 2, i, j, k);
   long sG = b.nfi[1], sH = (long)b.nfi[2] * b.nfj[2];
-  double R[7];
+  double R[8];
   for (int l = 0; l < nv; ++l) {
     const double *F = b.flux[0] + (long)l * b.nfl[0];
     const double *G = b.flux[1] + (long)l * b.nfl[1];
@@ -174,12 +174,13 @@ __device__ __forceinline__ void k_update_b(const Blk &b, const Wg &wg, double TF
     R[l] = (F[F0 + 1] - F[F0]) + (G[G0 + sG] - G[G0]) + (H[H0 + sH] - H[H0]);
   }
   double rho = b.qp[c];
-  const bool sst = b.ng == 6, sa = b.ng == 5;
+  const bool sst = b.ng >= 6, sa = b.ng == 5;
   if (sst) {
     // SST source (src/source.f90:158-270), computed with the gradients
     R[5] = R[5] - b.src[r];
     R[6] = R[6] - b.src[b.nres + r];
   }
+  if (b.lctm) R[7] = R[7] - b.src[2 * b.nres + r];  // src/source.f90:454-456
   if (sa) R[5] = R[5] - b.src[r];  // SA source (src/source.f90:835-983)
   if (flags & UPD_WRITE_RES)
     for (int l = 0; l < nv; ++l) b.res[(long)l * b.nres + r] = R[l];
@@ -187,7 +188,7 @@ __device__ __forceinline__ void k_update_b(const Blk &b, const Wg &wg, double TF
   // src/update.f90:367-485
   const double gm = b.gm;
   const double *Qu = (flags & UPD_STORE) ? b.ust : b.qp;
-  double u1[7], u2[7];
+  double u1[8], u2[8];
   u1[0] = Qu[c];
   for (int l = 1; l < nv; ++l) u1[l] = Qu[(long)l * nc + c] * u1[0];
   u1[4] = (u1[4] / (gm - 1.) + 0.5 * (u1[1] * u1[1] + u1[2] * u1[2] + u1[3] * u1[3])) / u1[0] + 0.0;
@@ -227,6 +228,8 @@ __device__ __forceinline__ void k_update_b(const Blk &b, const Wg &wg, double TF
     atomicOr(b.err, ERR_NONPHYS);
     return;
   }
+  // variables 1-5 and the two turbulence variables; lctm2015's
+  // intermittency is not written back (src/update.f90:456-484)
   for (int l = 0; l < 5; ++l) b.qp[(long)l * nc + c] = u2[l];
   if (sst) {
     if (u2[5] >= 0.) b.qp[5 * nc + c] = u2[5];
@@ -239,11 +242,13 @@ __global__ void __launch_bounds__(256) k_update(Blk b, double TF, double SF, int
 }
 
 // per-block sums of residue^2 and of the boundary mass fluxes: blocks take
-// whole i rows (j, k) of the residue in turn, threads the cells of a row
+// whole i rows (j, k) of the residue in turn, threads the cells of a row.
+// NORM_ACC sums per block: the mass imbalance and up to 8 variables.
+constexpr int NORM_ACC = 9;
 __global__ void __launch_bounds__(256) k_norm_partial(Blk b, double *red) {
-  __shared__ double s[8][256];
+  __shared__ double s[NORM_ACC][256];
   const int nv = b.nv;
-  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double acc[NORM_ACC] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   const int ni = b.imx - 1, nj = b.jmx - 1;
   const long rows = (long)nj * (b.kmx - 1);
   for (long r = blockIdx.x; r < rows; r += gridDim.x) {
@@ -252,7 +257,7 @@ __global__ void __launch_bounds__(256) k_norm_partial(Blk b, double *red) {
     for (int i = 1 + (int)threadIdx.x; i <= ni; i += blockDim.x) {
       const long t = row + (i - 1);
 #pragma unroll
-      for (int l = 0; l < 7; ++l)
+      for (int l = 0; l < NORM_ACC - 1; ++l)
         if (l < nv) {
           double v = b.res[(long)l * b.nres + t];
           acc[1 + l] += v * v;
@@ -266,54 +271,58 @@ __global__ void __launch_bounds__(256) k_norm_partial(Blk b, double *red) {
     }
   }
 #pragma unroll
-  for (int l = 0; l < 8; ++l) s[l][threadIdx.x] = acc[l];
+  for (int l = 0; l < NORM_ACC; ++l) s[l][threadIdx.x] = acc[l];
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
     if (threadIdx.x < st)
 #pragma unroll
-      for (int l = 0; l < 8; ++l) s[l][threadIdx.x] += s[l][threadIdx.x + st];
+      for (int l = 0; l < NORM_ACC; ++l) s[l][threadIdx.x] += s[l][threadIdx.x + st];
     __syncthreads();
   }
-  if (threadIdx.x < 8) red[(long)blockIdx.x * 8 + threadIdx.x] = s[threadIdx.x][0];
+  if (threadIdx.x < NORM_ACC) red[(long)blockIdx.x * NORM_ACC + threadIdx.x] = s[threadIdx.x][0];
 }
 
 // final norms, src/resnorm.f90:128-225: 256 threads sum the block
 // partials in a fixed order (strided, then a tree).  out receives the n_var+1
-// norms and, with shares, from out[8] the block's unassembled share.
+// norms and, with shares, from out[NORM_SHARE] the block's unassembled share.
 __global__ void __launch_bounds__(256) k_norm_final(Blk b, int nblk, const double *red, double *out, int shares) {
-  __shared__ double s[8][256];
-  double part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  __shared__ double s[NORM_ACC][256];
+  double part[NORM_ACC] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int t = threadIdx.x; t < nblk; t += blockDim.x)
 #pragma unroll
-    for (int l = 0; l < 8; ++l) part[l] += red[(long)t * 8 + l];
+    for (int l = 0; l < NORM_ACC; ++l) part[l] += red[(long)t * NORM_ACC + l];
 #pragma unroll
-  for (int l = 0; l < 8; ++l) s[l][threadIdx.x] = part[l];
+  for (int l = 0; l < NORM_ACC; ++l) s[l][threadIdx.x] = part[l];
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
     if (threadIdx.x < st)
 #pragma unroll
-      for (int l = 0; l < 8; ++l) s[l][threadIdx.x] += s[l][threadIdx.x + st];
+      for (int l = 0; l < NORM_ACC; ++l) s[l][threadIdx.x] += s[l][threadIdx.x + st];
     __syncthreads();
   }
   if (threadIdx.x != 0) return;
-  double acc[8];
+  double acc[NORM_ACC];
 #pragma unroll
-  for (int l = 0; l < 8; ++l) acc[l] = s[l][0];
+  for (int l = 0; l < NORM_ACC; ++l) acc[l] = s[l][0];
   double v = b.vel_mag;
-  double scale[8];
+  double scale[NORM_ACC];
   scale[0] = 1.;
   scale[1] = b.rho_inf * v;
   scale[2] = scale[3] = scale[4] = b.rho_inf * v * v;
   scale[5] = (0.5 * b.rho_inf * (v * v * v) + ((b.gm / (b.gm - 1.)) * b.p_inf));
   scale[6] = b.rho_inf * v * (b.ng == 5 ? b.tv_inf : b.tk_inf);  // nu-tilde for SA (resnorm.f90:158)
   scale[7] = b.rho_inf * v * (b.kkl ? b.tkl_inf : b.tw_inf);  // kL for k-kL (resnorm.f90:153)
+  // lctm2015's intermittency: resnorm.f90:128-168 leaves Res_scale(8)
+  // unset, so the reference's value is undefined; scaled here like the
+  // other turbulence variables
+  scale[8] = b.rho_inf * v * b.tgm_inf;
   out[0] = fabs(acc[0] / scale[0]);
   for (int l = 1; l <= b.nv; ++l) out[l] = sqrt(acc[l] / (scale[l] * scale[l]));
   if (!shares) return;
   // the block's share before the cross-block assembly (src/resnorm.f90:
   // 171-225): the signed mass imbalance and sum(R_l^2)/scale_l^2
-  out[8] = acc[0] / scale[0];
-  for (int l = 1; l <= b.nv; ++l) out[8 + l] = acc[l] / (scale[l] * scale[l]);
+  out[NORM_SHARE] = acc[0] / scale[0];
+  for (int l = 1; l <= b.nv; ++l) out[NORM_SHARE + l] = acc[l] / (scale[l] * scale[l]);
 }
 
 }  // namespace fest

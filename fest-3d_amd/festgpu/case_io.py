@@ -93,12 +93,14 @@ def read_mapping(path: str) -> List[Link]:
 def read_bc(path: str, case: Case) -> Case:
     """Fixed values of the six faces (read_bc.f90:16-190).  Under k-kL the
     omega values are neither filled nor read (read_bc.f90:98-108, 198-202):
-    they stay 0."""
+    they stay 0.  lctm2015's intermittency: the free stream, or FIX_tgm
+    (read_bc.f90:123-131, 213-218)."""
     vals = {k: list(getattr(case, k)) for k in ("fix_rho", "fix_u", "fix_v", "fix_w", "fix_p", "fix_tk",
-                                                  "fix_tw", "fix_twall", "fix_tp", "fix_tv", "fix_tkl")}
+                                                  "fix_tw", "fix_twall", "fix_tp", "fix_tv", "fix_tkl", "fix_tgm")}
     inf = dict(fix_rho=case.rho_inf, fix_u=case.u_inf, fix_v=case.v_inf, fix_w=case.w_inf, fix_p=case.p_inf,
                fix_tk=case.tk_inf, fix_tw=0.0 if case.turb == "kkl" else case.tw_inf, fix_twall=0.0, fix_tp=0.0,
-               fix_tv=case.tv_inf, fix_tkl=case.tkl_inf)
+               fix_tv=case.tv_inf, fix_tkl=case.tkl_inf,
+               fix_tgm=case.tgm_inf if case.transition == "lctm2015" else 0.0)
     for k in vals:  # fill_fixed_values: the free stream everywhere
         vals[k] = [inf[k]] * 6
     names = {"FIX_DENSITY": "fix_rho", "FIX_X_SPEED": "fix_u", "FIX_Y_SPEED": "fix_v", "FIX_Z_SPEED": "fix_w",
@@ -106,6 +108,8 @@ def read_bc(path: str, case: Case) -> Case:
              "TOTAL_PRESSURE": "fix_tp", "FIX_tv": "fix_tv", "FIX_tkl": "fix_tkl"}
     if case.turb == "kkl":
         del names["FIX_tw"]
+    if case.transition == "lctm2015":
+        names["FIX_tgm"] = "fix_tgm"
     face = -1
     with open(path) as f:
         for line in f:
@@ -134,10 +138,12 @@ def read_case(root: str):
     tlim = [int(x) for x in " ".join(scheme["Turbulence Limiter Switch"]).split()]
     turb = _first(scheme, "Turbulence model")
     transition = _first(scheme, "Transition model")
-    if transition not in ("none", "bc"):
-        # src/read_write/read/read.f90:219-222; lctm2015 adds an
-        # intermittency equation this engine does not carry
-        raise ValueError(f"transition model {transition!r} is not supported (none, bc)")
+    if transition not in ("none", "bc", "lctm2015"):
+        # src/read_write/read/read.f90:219-222
+        raise ValueError(f"transition model {transition!r} is not supported (none, bc, lctm2015)")
+    if transition == "lctm2015" and turb not in ("sst", "sst2003"):
+        # src/source.f90:124-139 stops on lctm2015 outside SST
+        raise ValueError("lctm2015 needs the sst or sst2003 turbulence model")
     if len(lim) < 6:
         raise ValueError("the limiter line needs three limiter and three pressure-based switches")
     pbs = tuple(lim[3:6])
@@ -163,7 +169,10 @@ def read_case(root: str):
                 S_ref=float(_first(flow, "Sutherland Temperature")), Pr=float(prs[0]), tPr=float(prs[1]),
                 transition=transition, pbs=pbs,
                 # percent, as flow%tu_inf (src/read_write/read/read.f90)
-                tu_inf=float(_first(flow, "Free Stream Turbulence Intensity")))
+                tu_inf=float(_first(flow, "Free Stream Turbulence Intensity")),
+                # read.f90:306-310; vartypes.f90:258 defaults it to 1
+                tgm_inf=float(_first(flow, "Free Stream Intermittency")) if "Free Stream Intermittency" in flow
+                else 1.0)
     if turb in ("sst", "sst2003"):
         tu = float(_first(flow, "Free Stream Turbulence Intensity")) / 100
         ratio = float(_first(flow, "Free Stream Viscosity Ratio"))
@@ -252,7 +261,8 @@ def restart_state(case: Case, path: str, restart_list: List[str]) -> np.ndarray:
     # of k-kL, nu-tilde of SA
     take = {"Velocity": [(1, "u"), (2, "v"), (3, "w")], "Density": [(0, "Density")], "Pressure": [(4, "Pressure")],
             "TKE": [(5, "TKE")], "Omega": [(6, "Omega")] if case.sst_family else [],
-            "Kl": [(6, "Kl")] if case.turb == "kkl" else [], "tv": [(5, "tv")] if case.turb == "sa" else []}
+            "Kl": [(6, "Kl")] if case.turb == "kkl" else [], "tv": [(5, "tv")] if case.turb == "sa" else [],
+            "tgm": [(7, "tgm")] if case.transition == "lctm2015" else []}  # read_output_tec.f90:130-131
     for var in restart_list:
         for l, name in take.get(var, []):
             if l < case.n_var:

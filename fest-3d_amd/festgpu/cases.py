@@ -29,7 +29,7 @@ FLUX = {"ausm": 0, "ausmP": 1, "ausmUP": 2, "slau": 3, "ldfss0": 4, "van_leer": 
 RECON = {"none": 0, "muscl": 1, "ppm": 2, "weno": 3, "weno_NM": 4}
 LIMITER = {"koren": 0, "van_albada": 1}
 TURB = {"none": 0, "sst": 1, "sst2003": 2, "sa": 3, "kkl": 4}
-TRANSITION = {"none": 0, "bc": 1}
+TRANSITION = {"none": 0, "bc": 1, "lctm2015": 2}
 TIME_ACC = {"none": 0, "RK2": 2, "RK4": 4, "TVDRK2": 12, "TVDRK3": 13, "implicit": 20}
 
 _GOLDEN = np.uint64(0x9E3779B97F4A7C15)

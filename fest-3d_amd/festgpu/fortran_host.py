@@ -23,9 +23,9 @@ def write_input(path, case: Case, nodes: np.ndarray, qp: np.ndarray, nsteps: int
     assert len(hdr) == 30
     flow = [f.density_inf, f.x_speed_inf, f.y_speed_inf, f.z_speed_inf, f.pressure_inf, f.tk_inf, f.tw_inf,
             f.vel_mag, f.MInf, f.gm, f.R_gas, f.mu_ref, f.T_ref, f.Sutherland_temp, f.Pr, f.tPr, f.tv_inf,
-            f.tu_inf, f.tkl_inf]
+            f.tu_inf, f.tkl_inf, f.tgm_inf]
     fixed = [b.fixed_density, b.fixed_x_speed, b.fixed_y_speed, b.fixed_z_speed, b.fixed_pressure, b.fixed_tk,
-             b.fixed_tw, b.fixed_wall_temperature, b.fixed_Tpressure, b.fixed_tv, b.fixed_tkl]
+             b.fixed_tw, b.fixed_wall_temperature, b.fixed_Tpressure, b.fixed_tv, b.fixed_tkl, b.fixed_tgm]
     with open(path, "wb") as fh:
         fh.write(np.asarray(hdr, dtype=np.int32).tobytes())
         fh.write(np.asarray([s.global_time_step, case.CFL] + flow, dtype=np.float64).tobytes())

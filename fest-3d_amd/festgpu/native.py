@@ -49,13 +49,14 @@ class Flow(ctypes.Structure):
                 ("density_inf", "x_speed_inf", "y_speed_inf", "z_speed_inf", "pressure_inf", "tk_inf", "tw_inf",
                  "vel_mag", "MInf", "gm", "R_gas", "mu_ref", "T_ref", "Sutherland_temp", "Pr", "tPr")] + \
                [("mu_sutherland", ctypes.c_int32), ("tv_inf", ctypes.c_double), ("tu_inf", ctypes.c_double),
-                ("tkl_inf", ctypes.c_double)]
+                ("tkl_inf", ctypes.c_double), ("tgm_inf", ctypes.c_double)]
 
 
 class BC(ctypes.Structure):
     _fields_ = [("id", ctypes.c_int32 * 6)] + [(n, ctypes.c_double * 6) for n in
                 ("fixed_density", "fixed_x_speed", "fixed_y_speed", "fixed_z_speed", "fixed_pressure",
-                 "fixed_tk", "fixed_tw", "fixed_wall_temperature", "fixed_Tpressure", "fixed_tv", "fixed_tkl")]
+                 "fixed_tk", "fixed_tw", "fixed_wall_temperature", "fixed_Tpressure", "fixed_tv", "fixed_tkl",
+                 "fixed_tgm")]
 
 
 _lib = None
@@ -134,6 +135,7 @@ def c_structs(case: Case):
     f.tv_inf = case.tv_inf
     f.tu_inf = case.tu_inf
     f.tkl_inf = case.tkl_inf
+    f.tgm_inf = case.tgm_inf
     b = BC()
     b.id[:] = list(case.bc_id)
     b.fixed_density[:] = list(case.fix_rho)
@@ -147,6 +149,7 @@ def c_structs(case: Case):
     b.fixed_Tpressure[:] = list(case.fix_tp)
     b.fixed_tv[:] = list(case.fix_tv)
     b.fixed_tkl[:] = list(case.fix_tkl)
+    b.fixed_tgm[:] = list(case.fix_tgm)
     return e, s, f, b
 
 
@@ -212,7 +215,7 @@ class GpuBlock:
         self._check(lib().fest_time_step(self.h, float(cfl)))
 
     def resnorm(self) -> np.ndarray:
-        out = np.zeros(8)
+        out = np.zeros(16)
         self._check(lib().fest_resnorm(self.h, _ptr(out)))
         return out[: self.case.n_var + 1]
 
@@ -317,7 +320,7 @@ class GpuGroup:
 
     def resnorm(self) -> np.ndarray:
         """Res_abs(0:n_var) of the whole group (find_resnorm)."""
-        out = np.zeros(8)
+        out = np.zeros(16)
         self._check(lib().fest_group_resnorm(self.g, _ptr(out)))
         return out[: self.blocks[0].case.n_var + 1]
 

@@ -114,6 +114,8 @@ def block_fields(case: Case, qp: np.ndarray, mu=None, mut=None, dist=None, f1=No
         out["TKE"], out["Kl"] = qp[5][inner], qp[6][inner]
     if case.turb == "sa":  # nu-tilde, write_output_tec.f90:163-164
         out["tv"] = qp[5][inner]
+    if case.transition == "lctm2015":  # write_output_tec.f90:166-167
+        out["tgm"] = qp[7][inner]
     for name, a in (("Mu", mu), ("Mu_t", mut), ("Wall_distance", dist), ("F1", f1)):
         if a is not None:
             out[name] = a[inner]

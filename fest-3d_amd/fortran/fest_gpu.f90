@@ -23,7 +23,7 @@ module fest_gpu
   integer(c_int32_t), parameter, public :: FEST_RECON_NONE = 0, FEST_RECON_MUSCL = 1, &
       FEST_RECON_PPM = 2, FEST_RECON_WENO = 3, FEST_RECON_WENO_NM = 4
   integer(c_int32_t), parameter, public :: FEST_LIM_KOREN = 0, FEST_LIM_VAN_ALBADA = 1
-  integer(c_int32_t), parameter, public :: FEST_TRANS_NONE = 0, FEST_TRANS_BC = 1
+  integer(c_int32_t), parameter, public :: FEST_TRANS_NONE = 0, FEST_TRANS_BC = 1, FEST_TRANS_LCTM2015 = 2
   integer(c_int32_t), parameter, public :: FEST_PREC_FP64 = 0, FEST_PREC_FP32_FLUX = 1
   integer(c_int32_t), parameter, public :: FEST_TURB_NONE = 0, FEST_TURB_SST = 1, FEST_TURB_SST2003 = 2, &
                                            FEST_TURB_SA = 3, FEST_TURB_KKL = 4
@@ -58,6 +58,7 @@ module fest_gpu
     integer(c_int32_t) :: mu_sutherland
     real(c_double) :: tv_inf, tu_inf
     real(c_double) :: tkl_inf
+    real(c_double) :: tgm_inf
   end type fest_flow
 
   !< hot-path fields of type(boundarytype), src/vartypes.f90:293-365
@@ -68,6 +69,7 @@ module fest_gpu
     real(c_double) :: fixed_Tpressure(6)
     real(c_double) :: fixed_tv(6)
     real(c_double) :: fixed_tkl(6)
+    real(c_double) :: fixed_tgm(6)
   end type fest_bc
 
   !< one interface link of a multiblock grid (the reference's map line,

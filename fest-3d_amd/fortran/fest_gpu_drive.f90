@@ -13,9 +13,9 @@
 !<   real64 global_time_step, CFL, density_inf, x_speed_inf, y_speed_inf,
 !<          z_speed_inf, pressure_inf, tk_inf, tw_inf, vel_mag, MInf, gm,
 !<          R_gas, mu_ref, T_ref, Sutherland_temp, Pr, tPr, tv_inf, tu_inf,
-!<          tkl_inf,
+!<          tkl_inf, tgm_inf,
 !<          fixed_{density, x_speed, y_speed, z_speed, pressure, tk, tw,
-!<          wall_temperature, Tpressure, tv, tkl}(6)
+!<          wall_temperature, Tpressure, tv, tkl, tgm}(6)
 !<   real64 nodes(-2:imx+3,-2:jmx+3,-2:kmx+3,3), interior 1..imx filled
 !<   real64 qp(-2:imx+2,-2:jmx+2,-2:kmx+2,n_var)
 !< OUT: real64 qp (same extent), residue(1:imx-1,1:jmx-1,1:kmx-1,n_var),
@@ -60,10 +60,10 @@ program fest_gpu_drive
   read(u) s%global_time_step, cfl
   read(u) f%density_inf, f%x_speed_inf, f%y_speed_inf, f%z_speed_inf, f%pressure_inf, &
           f%tk_inf, f%tw_inf, f%vel_mag, f%MInf, f%gm, f%R_gas, f%mu_ref, f%T_ref, &
-          f%Sutherland_temp, f%Pr, f%tPr, f%tv_inf, f%tu_inf, f%tkl_inf
+          f%Sutherland_temp, f%Pr, f%tPr, f%tv_inf, f%tu_inf, f%tkl_inf, f%tgm_inf
   read(u) b%fixed_density, b%fixed_x_speed, b%fixed_y_speed, b%fixed_z_speed, &
           b%fixed_pressure, b%fixed_tk, b%fixed_tw, b%fixed_wall_temperature, b%fixed_Tpressure, &
-          b%fixed_tv, b%fixed_tkl
+          b%fixed_tv, b%fixed_tkl, b%fixed_tgm
   allocate(nodes(-2:e%imx+3, -2:e%jmx+3, -2:e%kmx+3, 3))
   allocate(qp(-2:e%imx+2, -2:e%jmx+2, -2:e%kmx+2, e%n_var))
   allocate(res(1:e%imx-1, 1:e%jmx-1, 1:e%kmx-1, e%n_var))

@@ -55,6 +55,10 @@ contains
       case default
         continue
     end select
+    ! lctm2015: the reference leaves Res_scale(8) unset; the device scales
+    ! the intermittency's norm by rho V gamma_inf, and so does this table
+    if (trim(scheme%transition) == 'lctm2015' .and. n >= 8) &
+      Res_scale(8) = flow%density_inf*flow%vel_mag*flow%tgm_inf
     if (process_id == 0) then
       if (control%start_from == 0) then
         open(files%RESNORM_FILE_UNIT, file=files%resnorm_file)

@@ -124,6 +124,7 @@ contains
     select case (trim(scheme%transition))
       case ("none"); s%transition = FEST_TRANS_NONE
       case ("bc"); s%transition = FEST_TRANS_BC
+      case ("lctm2015"); s%transition = FEST_TRANS_LCTM2015
       case default
         print *, "update: transition model '", trim(scheme%transition), "' is not on the device path"
         stop 1
@@ -149,6 +150,7 @@ contains
     f%tv_inf = flow%tv_inf
     f%tu_inf = flow%tu_inf
     f%tkl_inf = flow%tkl_inf
+    f%tgm_inf = flow%tgm_inf
 
     b%id = bc%id
     b%fixed_density = bc%fixed_density
@@ -162,6 +164,7 @@ contains
     b%fixed_Tpressure = bc%fixed_Tpressure
     b%fixed_tv = bc%fixed_tv
     b%fixed_tkl = bc%fixed_tkl
+    b%fixed_tgm = bc%fixed_tgm
   end subroutine describe
 
   !< first call: the device block and its geometry, copied out of the

@@ -54,7 +54,10 @@ enum { FEST_LIM_KOREN = 0, FEST_LIM_VAN_ALBADA = 1 };
 enum { FEST_TURB_NONE = 0, FEST_TURB_SST = 1, FEST_TURB_SST2003 = 2 /* src/viscosity.f90:215, source.f90:205 */,
        FEST_TURB_SA = 3 /* 'sa', src/viscosity.f90:149, source.f90:835, lusgs.f90:1680 */,
        FEST_TURB_KKL = 4 /* 'kkl', src/viscosity.f90:469, source.f90:607, viscous.f90:450, lusgs.f90:1198 */ };
-enum { FEST_TRANS_NONE = 0, FEST_TRANS_BC = 1 /* the 'bc' intermittency of the SA and SST sources */ };
+enum { FEST_TRANS_NONE = 0, FEST_TRANS_BC = 1 /* the 'bc' intermittency of the SA and SST sources */,
+       FEST_TRANS_LCTM2015 = 2 /* 'lctm2015': the intermittency as variable 8 (n_var 8), SST and
+                                  SST-2003, FP64, RK (src/source.f90:273-463, viscous.f90:659-746,
+                                  viscosity.f90:265-279, CC.f90) */ };
 enum { FEST_TA_NONE = 0, FEST_TA_RK2 = 2, FEST_TA_RK4 = 4, FEST_TA_TVDRK2 = 12, FEST_TA_TVDRK3 = 13,
        FEST_TA_IMPLICIT = 20 /* LU-SGS, src/lusgs.f90 */ };
 
@@ -87,6 +90,7 @@ typedef struct {
   double tv_inf;           /* Spalart-Allmaras free-stream nu-tilde (state.f90:106) */
   double tu_inf;           /* free-stream turbulence intensity, percent (vartypes.f90) */
   double tkl_inf;          /* k-kL free-stream kL (state.f90:103) */
+  double tgm_inf;          /* lctm2015 free-stream intermittency (vartypes.f90:258, read.f90:306-310) */
 } fest_flow;
 
 /* the hot-path fields of type(boundarytype), src/vartypes.f90:293-365 */
@@ -97,6 +101,7 @@ typedef struct {
   double fixed_Tpressure[6];  /* total pressure of a -11 face (read_bc.f90:79-80) */
   double fixed_tv[6];         /* fixed nu-tilde of each face (read_bc.f90:106-113) */
   double fixed_tkl[6];        /* fixed kL of each face (read_bc.f90:103-106, 198-202) */
+  double fixed_tgm[6];        /* fixed intermittency of each face (read_bc.f90:123-131, 213-218) */
 } fest_bc;
 
 typedef struct fest_handle fest_handle;
@@ -179,7 +184,7 @@ int fest_resnorm(fest_handle *h, double *res_abs);
  * owns, in stream order (nothing waits for them).  No counterpart in the
  * reference: it is for callers that cannot allocate -- code under HIP graph
  * capture, serving loops, harnesses that forbid hipMalloc.  Its partial
- * sums live in a static per-device scratch of 64 KiB (at most 1024 partial
+ * sums live in a static per-device scratch of 72 KiB (at most 1024 partial
  * blocks; larger reductions are rejected with FEST_E_ARG), set up when a
  * handle is created on the device; calls that overlap on one device race
  * for it.  The handle's device must be current. */

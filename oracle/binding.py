@@ -240,7 +240,7 @@ class Oracle:
         return self.lib().orc_step(self.h, it)
 
     def resnorm(self):
-        out = (ctypes.c_double * 8)()
+        out = (ctypes.c_double * 16)()
         self.lib().orc_resnorm(self.h, out)
         return np.array(out[: self.case.n_var + 1])
 

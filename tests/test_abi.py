@@ -35,9 +35,9 @@ def test_struct_sizes_match_header():
     # transition, pbs(3) and muscl_limiter int32 (padded to 8)
     assert ctypes.sizeof(native.Extent) == 16
     assert ctypes.sizeof(native.Scheme) == 80
-    assert ctypes.sizeof(native.BC) == 24 + 8 * 66
-    # 16 doubles, mu_sutherland (int32, then 4 bytes of padding), tv_inf, tu_inf, tkl_inf
-    assert ctypes.sizeof(native.Flow) == 8 * 16 + 8 + 8 + 8 + 8
+    assert ctypes.sizeof(native.BC) == 24 + 8 * 72
+    # 16 doubles, mu_sutherland (int32, then 4 bytes of padding), tv_inf, tu_inf, tkl_inf, tgm_inf
+    assert ctypes.sizeof(native.Flow) == 8 * 16 + 8 + 8 + 8 + 8 + 8
 
 
 @pytest.mark.parametrize("change,code", [
@@ -47,6 +47,10 @@ def test_struct_sizes_match_header():
     (dict(time_accuracy="RK2", bc_id=(-9, -9, -5, -5, -9, -9), imx=3), -3),
     (dict(kmx=3), -3),  # two cells in k: neither 3-D nor the reference's 2-D
     (dict(turb="kkl", transition="bc", bc_id=(-9, -9, -5, -5, -9, -9)), -3),
+    # lctm2015 exists for SST and SST-2003 only, with RK
+    (dict(turb="sa", transition="lctm2015", mu_ref=1e-5), -3),
+    (dict(turb="kkl", transition="lctm2015", mu_ref=1e-5), -3),
+    (dict(turb="sst", transition="lctm2015", mu_ref=1e-5, time_accuracy="implicit"), -3),
 ])
 def test_config_errors(change, code):
     case, _, _ = C.laminar_box(8, 6, 5)
@@ -64,6 +68,22 @@ def test_precision_modes():
     with pytest.raises(native.FestError) as e:
         native.GpuBlock(case, precision=7)
     assert e.value.code == -3
+    # lctm2015 runs in FP64 only
+    lc = dataclasses.replace(case, turb="sst", transition="lctm2015", mu_ref=1e-5)
+    with pytest.raises(native.FestError) as e:
+        native.GpuBlock(lc, precision=1)
+    assert e.value.code == -3
+
+
+def test_lctm2015_passes_the_configuration_check():
+    """SST + lctm2015 with n_var 8 is accepted; without a GPU the handle
+    fails on the device, after the check."""
+    case, _, _ = C.sst_bump(8, 6, 5, transition="lctm2015")
+    assert case.n_var == 8 and case.n_grad == 7
+    try:
+        native.GpuBlock(case).close()
+    except native.FestError as e:
+        assert e.code == -4, e
 
 
 @pytest.mark.parametrize("ids", [(-7, -9, -5, -5, -9, -9), (-11, -4, -5, -5, -9, -9)])

@@ -50,10 +50,29 @@ def test_reads_transition_model_and_intensity(tmp_path):
     assert c.transition == "bc"
     flow = case_io.read_md(os.path.join(root, "system", "flow.md"))
     assert c.tu_inf == float(case_io._first(flow, "Free Stream Turbulence Intensity")) and c.tu_inf > 0
+    # lctm2015 (SST): n_var 8, the free-stream intermittency of flow.md
+    # (read.f90:306-310) and, on every face, that value (read_bc.f90:213-218)
     root2 = _case_copy(tmp_path / "b", "Tfp",
                        lambda s: s.replace("## Transition model\nnone", "## Transition model\nlctm2015"))
+    blocks, _, _ = case_io.read_case(root2)
+    c = blocks[0][0]
+    assert c.transition == "lctm2015" and c.n_var == 8 and c.n_grad == 7
+    assert c.tgm_inf == float(case_io._first(flow, "Free Stream Intermittency"))
+    assert c.fix_tgm == (c.tgm_inf,) * 6
+    # and stops outside SST, as src/source.f90:124-139 does
+    root3 = _case_copy(tmp_path / "c", "Tfp",
+                       lambda s: s.replace("## Transition model\nnone", "## Transition model\nlctm2015")
+                       .replace("## Turbulence model\nsst", "## Turbulence model\nsa"))
     with pytest.raises(ValueError, match="lctm2015"):
-        case_io.read_case(root2)
+        case_io.read_case(root3)
+
+
+def test_lctm2015_reads_fixed_intermittency(tmp_path):
+    case, _, _ = C.sst_bump(8, 6, 4, transition="lctm2015")
+    p = tmp_path / "bc.md"
+    p.write_text("#imin\n - FIX_tgm 0.25\n#imax\n#jmin\n#jmax\n#kmin\n#kmax\n")
+    r = case_io.read_bc(str(p), case)
+    assert r.fix_tgm == (0.25,) + (case.tgm_inf,) * 5
 
 
 def test_reads_pressure_based_switch(tmp_path):

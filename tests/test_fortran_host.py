@@ -38,7 +38,7 @@ def test_input_stream_layout(tmp_path):
     fin = tmp_path / "in.bin"
     FH.write_input(str(fin), case, nodes, q, 3, False)
     n = os.path.getsize(fin)
-    assert n == 30 * 4 + (21 + 66) * 8 + 8 * (nodes.size + q.size)
+    assert n == 30 * 4 + (22 + 72) * 8 + 8 * (nodes.size + q.size)
     hdr = np.fromfile(fin, dtype=np.int32, count=30)
     assert list(hdr[:4]) == [case.imx, case.jmx, case.kmx, 5] and hdr[23] == 3
 
