@@ -239,6 +239,55 @@ def fp32_flux_run(case, nodes, q, g64, warmup, steps, implicit, breakdown64):
                     "to each variable's largest magnitude, the velocity components' to the largest speed"}
 
 
+def fp32_long_error(case, nodes, q, stages):
+    """The FP32 face mode against the FP64 path after `stages` RK stages
+    (stages/4 RK4 steps) from the same initial state, both replayed from
+    captured step graphs.  The FP64 path is the parity path (held to the
+    oracle within 1e-12 per step by the GPU tests), so it stands in for the
+    FP64 oracle at full size.  Reported: the state per variable (relative to
+    its largest magnitude, velocities to the largest speed), the residue,
+    and on the j-min wall (when it is one) the pressure of the first cell
+    row relative to the dynamic pressure (a Cp error) and the skin friction
+    mu |u| / d there (relative to its largest value)."""
+    import numpy as np
+    from festgpu.native import GpuBlock
+    steps = max(1, stages // 4)
+    runs = {}
+    t0 = time.perf_counter()
+    for prec in (0, 1):
+        g = GpuBlock(case, precision=prec)
+        g.set_grid(nodes, wall_dist=case.sst_family)
+        g.upload_state(q)
+        for it in range(1, steps + 1):
+            g.get_next_solution(it)
+        runs[prec] = g.download("qp", "residue", "mu", "dist")
+        g.close()
+    a, b = runs[1], runs[0]
+    inner = (slice(None), slice(3, -3), slice(3, -3), slice(3, -3))
+    qa, qb = a["qp"][inner], b["qp"][inner]
+    speed = np.sqrt(qb[1] ** 2 + qb[2] ** 2 + qb[3] ** 2).max()
+    out = {"stages": 4 * steps,
+           "max_rel_err_state_per_var": [float(np.abs(qa[l] - qb[l]).max() /
+                                               (speed if l in (1, 2, 3) else np.abs(qb[l]).max()))
+                                         for l in range(qb.shape[0])],
+           "max_rel_err_residue": float(np.abs(a["residue"] - b["residue"]).max() / np.abs(b["residue"]).max())}
+    if case.bc_id[2] == -5:
+        qdyn = 0.5 * case.rho_inf * case.vel_mag ** 2
+        row = (slice(3, -3), 3, slice(3, -3))  # cells (i, j=1, k)
+
+        def wall(r):
+            qq = r["qp"]
+            p = qq[4][row]
+            um = np.sqrt(qq[1][row] ** 2 + qq[2][row] ** 2 + qq[3][row] ** 2)
+            return p, r["mu"][row] * um / r["dist"][row] / qdyn
+        pa, cfa = wall(a)
+        pb, cfb = wall(b)
+        out["wall_cp_err"] = float(np.abs(pa - pb).max() / qdyn)
+        out["wall_cf_rel_err"] = float(np.abs(cfa - cfb).max() / np.abs(cfb).max())
+    out["seconds"] = time.perf_counter() - t0
+    return out
+
+
 def chain_run(args):
     """The batched multiblock workload: 64 seeded uneven blocks (about 11.5M
     cells) joined in a chain (festgpu.cases.chain_blocks), MUSCL + AUSM-UP,
@@ -306,6 +355,8 @@ def main():
                     help="skip the run with FP32 face reconstruction and flux arithmetic")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"],
                     help="face arithmetic of the chain64 workload")
+    ap.add_argument("--fp32-error-stages", type=int, default=1000,
+                    help="RK stages after which the FP32 face mode is compared with FP64 (0: skip)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch kernel by kernel instead of replaying captured step graphs")
     args = ap.parse_args()
@@ -400,6 +451,8 @@ def main():
     if not args.no_fp32_flux:
         out["fp32_flux"] = fp32_flux_run(case, nodes, q, g, args.warmup, args.steps, implicit, breakdown)
         out["fp32_flux"]["speedup_vs_fp64"] = out["fp32_flux"]["value"] / value
+        if args.fp32_error_stages > 0 and not implicit:
+            out["fp32_flux"]["long_run"] = fp32_long_error(case, nodes, q, args.fp32_error_stages)
     if not args.no_cpu_baseline:
         dist = g.download("dist") if case.sst_family else None
         out["cpu_baseline"] = cpu_baseline(case_fn, ni, nj, nk, dist, implicit=implicit)

@@ -140,15 +140,18 @@ __device__ __forceinline__ bool flat_ij(int ni, int nj, int &i, int &j, int blk)
   return j < nj;
 }
 
-// Tiled variant: each 256-thread block is a 64 (i) x 4 (j) tile, tiles
+// Tiled variant: each 256-thread block is a TI (i) x 256/TI (j) tile --
+// 64 x 4 by default, 32 x 8 for narrow blocks of a batched group -- tiles
 // numbered i-fastest in blockIdx.x, blockIdx.y = k.  Sweeps whose stencil
-// runs along j or k keep four rows of one i-range in one block.
+// runs along j or k keep whole rows of one i-range in one block.
 constexpr int TILE_I = 64, TILE_J = 4;
+template <int TI = TILE_I>
 __device__ __forceinline__ bool tile_ij(int ni, int nj, int &i, int &j, int blk) {
-  const int ntx = (ni + TILE_I - 1) / TILE_I;
+  constexpr int TJ = 256 / TI;
+  const int ntx = (ni + TI - 1) / TI;
   const int tx = blk % ntx, ty = blk / ntx;
-  i = tx * TILE_I + ((int)threadIdx.x & (TILE_I - 1));
-  j = ty * TILE_J + ((int)threadIdx.x / TILE_I);
+  i = tx * TI + ((int)threadIdx.x & (TI - 1));
+  j = ty * TJ + ((int)threadIdx.x / TI);
   return i < ni && j < nj;
 }
 

@@ -115,6 +115,8 @@ struct fest_group {
   int grid_cap = 0;       // work-groups per batched launch; 0: one per tile
   int order = 0;          // 0: tiles block by block; 1: the blocks' tiles interleaved
   int lists_order = -1;   // order the device lists were built in
+  int tile_i = TILE_I;    // i width of the j/k sweeps' face tiles: 64 (x 4) or 32 (x 8)
+  int lists_tile = -1;    // tile width the device lists were built with
   Blk *d_blks = nullptr;
   double *d_norms = nullptr;  // NORM_SLOTS doubles per block: norms, then the block's share
   double blk_cfl = -1.0;  // CFL of the uploaded records
@@ -342,6 +344,11 @@ int launch_residual_front(fest_handle *h, int current_iter, int store_csnd) {
         if (b.turb == FEST_TURB_SST2003) hipLaunchKernelGGL((k_grad<6, true, true>), g, t, 0, st, b);
         else if (b.ng == 6) hipLaunchKernelGGL((k_grad<6, false, true>), g, t, 0, st, b);
         else hipLaunchKernelGGL((k_grad<5, false, true>), g, t, 0, st, b);
+      } else if (b.g32only) {
+        // FP32 mode: the gradient sums in FP32 (k_grad_b's F32 variant)
+        if (b.turb == FEST_TURB_SST2003) hipLaunchKernelGGL((k_grad<6, true, false, false, true>), g, t, 0, st, b);
+        else if (b.ng == 6) hipLaunchKernelGGL((k_grad<6, false, false, false, true>), g, t, 0, st, b);
+        else hipLaunchKernelGGL((k_grad<4, false, false, false, true>), g, t, 0, st, b);
       } else if (b.turb == FEST_TURB_SST2003) {
         hipLaunchKernelGGL((k_grad<6, true>), g, t, 0, st, b);
       } else if (b.kkl) {
@@ -1077,9 +1084,24 @@ int fest_download(fest_handle *h, uint32_t what, double *qp, double *temp, doubl
   DL(FEST_DL_QP, qp, b.qp, b.nc * b.nv);
   DL(FEST_DL_TEMP, temp, b.temp, b.nc);
   DL(FEST_DL_RESIDUE, residue, b.res, b.nres * b.nv);
-  DL(FEST_DL_F, F, b.flux[0], b.nfl[0] * b.nv);
-  DL(FEST_DL_G, G, b.flux[1], b.nfl[1] * b.nv);
-  DL(FEST_DL_H, H, b.flux[2], b.nfl[2] * b.nv);
+  if (b.fp32) {
+    // FP32 mode: the face fluxes are held in FP32; widened on the host
+    double *outs[3] = {F, G, H};
+    const uint32_t bits[3] = {FEST_DL_F, FEST_DL_G, FEST_DL_H};
+    for (int d = 0; d < 3; ++d) {
+      if (!(what & bits[d])) continue;
+      if (!outs[d]) return FEST_E_ARG;
+      const size_t n = (size_t)(b.nfl[d] * b.nv);
+      std::vector<float> tmp(n);
+      HIPCHK(h, hipMemcpyAsync(tmp.data(), b.flux[d], n * sizeof(float), hipMemcpyDeviceToHost, st));
+      HIPCHK(h, hipStreamSynchronize(st));
+      for (size_t m = 0; m < n; ++m) outs[d][m] = (double)tmp[m];
+    }
+  } else {
+    DL(FEST_DL_F, F, b.flux[0], b.nfl[0] * b.nv);
+    DL(FEST_DL_G, G, b.flux[1], b.nfl[1] * b.nv);
+    DL(FEST_DL_H, H, b.flux[2], b.nfl[2] * b.nv);
+  }
   DL(FEST_DL_DT, delta_t, b.dt, b.nres);
   DL(FEST_DL_MU, mu, b.mu, b.nc);
   DL(FEST_DL_MUT, mu_t, b.mut, b.nc);
@@ -1156,7 +1178,7 @@ int fest_device_ptr(fest_handle *h, const char *name, void **ptr, size_t *count)
   for (auto &e : t)
     if (!strcmp(e.n, name)) {
       *ptr = e.p;
-      if (count) *count = (size_t)e.c;
+      if (count) *count = (size_t)e.c;  // floats for F/G/H in FP32 mode
       return 0;
     }
   return fail(h, FEST_E_ARG, "unknown array '%s'", name);
@@ -1249,7 +1271,8 @@ int group_exchange(fest_group *g) {
 
 // the per-block launch grid of a phase (as launch_residual_front and the
 // step operations lay it out); false when the block has no tiles in it
-bool phase_grid(const Blk &b, int ph, dim3 &g) {
+bool phase_grid(const Blk &b, int ph, dim3 &g, int ti = TILE_I) {
+  const int tj = 256 / ti;
   auto N = [&](int d) { return d == 0 ? b.imx : d == 1 ? b.jmx : b.kmx; };
   switch (ph) {
     case PH_BEGIN: g = dim3(cdiv(b.nc, 256), 1, 1); return true;
@@ -1274,8 +1297,8 @@ bool phase_grid(const Blk &b, int ph, dim3 &g) {
       return b.mu_ref != 0.0;
     }
     case PH_FLUX_I: g = dim3(cdiv(flux_i_span(b), FLUX_I_SPAN), b.kmx - 1, 1); return true;
-    case PH_FLUX_J: g = dim3(cdiv(b.imx - 1, TILE_I) * cdiv(b.jmx, TILE_J), b.kmx - 1, 1); return true;
-    case PH_FLUX_K: g = dim3(cdiv(b.imx - 1, TILE_I) * cdiv(b.jmx - 1, TILE_J), b.kmx, 1); return true;
+    case PH_FLUX_J: g = dim3(cdiv(b.imx - 1, ti) * cdiv(b.jmx, tj), b.kmx - 1, 1); return true;
+    case PH_FLUX_K: g = dim3(cdiv(b.imx - 1, ti) * cdiv(b.jmx - 1, tj), b.kmx, 1); return true;
     case PH_DT: case PH_UPDATE:
       g = dim3(cdiv((long)(b.imx - 1) * (b.jmx - 1), FLAT_BLOCK), b.kmx - 1, 1);
       return true;
@@ -1310,8 +1333,22 @@ int build_lists(fest_group *g, int order) {
   for (int ph = 0; ph < PH_COUNT; ++ph) {
     std::vector<std::vector<WorkItem>> per(g->blocks.size());
     for (size_t n = 0; n < g->blocks.size(); ++n) {
+      const Blk &b = g->blocks[n]->b;
       dim3 gr;
-      if (!phase_grid(g->blocks[n]->b, ph, gr)) continue;
+      if (!phase_grid(b, ph, gr, g->tile_i)) continue;
+      if (ph == PH_GRAD_BC) {
+        // the ghost-gradient rule per face, exact tiles of the face
+        // interior, none on interface faces (k_grad_bc_b leaves those):
+        // the per-block launch's square grid over the largest extent
+        // is mostly idle tiles on uneven blocks
+        for (int z = 0; z < 6; ++z) {
+          if (b.bc[z] >= 0) continue;
+          const unsigned nx = cdiv(face_na(b, z) - 1, 64), ny = cdiv(face_nb(b, z) - 1, 4);
+          for (unsigned y = 0; y < ny; ++y)
+            for (unsigned x = 0; x < nx; ++x) per[n].push_back(WorkItem{(int)n, (int)x, (int)y, z});
+        }
+        continue;
+      }
       for (unsigned z = 0; z < gr.z; ++z)
         for (unsigned y = 0; y < gr.y; ++y)
           for (unsigned x = 0; x < gr.x; ++x) per[n].push_back(WorkItem{(int)n, (int)x, (int)y, (int)z});
@@ -1337,6 +1374,7 @@ int build_lists(fest_group *g, int order) {
                           hipSuccess)
     return gfail(g, FEST_E_HIP, "work list upload failed");
   g->lists_order = order;
+  g->lists_tile = g->tile_i;
   return 0;
 }
 
@@ -1355,8 +1393,13 @@ int upload_blks(fest_group *g) {
 extern "C++" {
 template <int D, int NV, int RECON, typename T>
 void km_flux_launch(fest_group *g, int ph, unsigned grid, int store_csnd) {
-  hipLaunchKernelGGL((km_flux<D, NV, RECON, false, T>), dim3(grid), dim3(256), 0, g->stream, g->d_blks,
-                     g->d_items + g->ph_off[ph], g->ph_n[ph], store_csnd);
+  // the j and k sweeps in 64 x 4 or 32 x 8 face tiles (the group's tile_i)
+  if (D != 0 && g->tile_i == 32)
+    hipLaunchKernelGGL((km_flux<D, NV, RECON, false, T, 32>), dim3(grid), dim3(256), 0, g->stream, g->d_blks,
+                       g->d_items + g->ph_off[ph], g->ph_n[ph], store_csnd);
+  else
+    hipLaunchKernelGGL((km_flux<D, NV, RECON, false, T>), dim3(grid), dim3(256), 0, g->stream, g->d_blks,
+                       g->d_items + g->ph_off[ph], g->ph_n[ph], store_csnd);
 }
 template <int D, int NV, typename T>
 void km_flux_recon(fest_group *g, int ph, unsigned grid, int store_csnd) {
@@ -1410,9 +1453,17 @@ int run_group_op_batched(fest_group *g, const StepOp &op, int current_iter) {
       KM(PH_EDGE2, km_bc_edges, dim3(256), 2);
       if (b0.fp32) KM(PH_F32, km_to_f32, dim3(256));
       if (b0.mu_ref != 0.0) {
-        if (b0.turb == FEST_TURB_SST2003) KM(PH_GRAD, (km_grad<6, true>), dim3(256));
-        else if (b0.ng == 6) KM(PH_GRAD, (km_grad<6, false>), dim3(256));
-        else KM(PH_GRAD, (km_grad<4, false>), dim3(256));
+        if (b0.g32only) {
+          if (b0.turb == FEST_TURB_SST2003) KM(PH_GRAD, (km_grad<6, true, true>), dim3(256));
+          else if (b0.ng == 6) KM(PH_GRAD, (km_grad<6, false, true>), dim3(256));
+          else KM(PH_GRAD, (km_grad<4, false, true>), dim3(256));
+        } else if (b0.turb == FEST_TURB_SST2003) {
+          KM(PH_GRAD, (km_grad<6, true>), dim3(256));
+        } else if (b0.ng == 6) {
+          KM(PH_GRAD, (km_grad<6, false>), dim3(256));
+        } else {
+          KM(PH_GRAD, (km_grad<4, false>), dim3(256));
+        }
         KM(PH_GRAD_BC, km_grad_bc, dim3(64, 4, 1));
       }
       if (g->ph_n[PH_FLUX_I]) km_flux_any<0>(g, PH_FLUX_I, grid(PH_FLUX_I), op.flags);
@@ -1518,11 +1569,12 @@ int fest_group_autotune(fest_group *g, int steps, double *ms, int ms_len, int *n
   if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
     return gfail(g, FEST_E_HIP, "device properties unavailable");
   const int cu = prop.multiProcessorCount;
-  struct Cand { int batched, cap, order; };
-  std::vector<Cand> cands = {{0, 0, 0}};
+  struct Cand { int batched, cap, order, tile; };
+  std::vector<Cand> cands = {{0, 0, 0, TILE_I}};
   if (g->batch_why.empty())
-    for (int order = 0; order < 2; ++order)
-      for (int cap : {0, 16 * cu, 8 * cu, 4 * cu}) cands.push_back({1, cap, order});
+    for (int tile : {64, 32})
+      for (int order = 0; order < 2; ++order)
+        for (int cap : {0, 16 * cu, 8 * cu, 4 * cu}) cands.push_back({1, cap, order, tile});
   // snapshot of the state: a step reads only qp (Temp is refreshed from it)
   std::vector<double *> saved;
   for (fest_handle *h : g->blocks) {
@@ -1540,7 +1592,7 @@ int fest_group_autotune(fest_group *g, int steps, double *ms, int ms_len, int *n
                            hipMemcpyDeviceToDevice, g->stream);
   };
   const double cfl = g->blocks[0]->b.CFLv;
-  const int keep_b = g->batched, keep_c = g->grid_cap, keep_o = g->order;
+  const int keep_b = g->batched, keep_c = g->grid_cap, keep_o = g->order, keep_t = g->tile_i;
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
@@ -1551,6 +1603,7 @@ int fest_group_autotune(fest_group *g, int steps, double *ms, int ms_len, int *n
     g->batched = cands[c].batched;
     g->grid_cap = cands[c].cap;
     g->order = cands[c].order;
+    g->tile_i = cands[c].tile;
     g->blk_cfl = -1.0;
     restore();
     // the first two steps capture the graph; the rest are timed
@@ -1576,11 +1629,13 @@ int fest_group_autotune(fest_group *g, int steps, double *ms, int ms_len, int *n
     g->batched = keep_b;
     g->grid_cap = keep_c;
     g->order = keep_o;
+    g->tile_i = keep_t;
     return rc;
   }
   g->batched = cands[best].batched;
   g->grid_cap = cands[best].cap;
   g->order = cands[best].order;
+  g->tile_i = cands[best].tile;
   if (ncand) *ncand = (int)cands.size();
   if (chosen) *chosen = best;
   for (int c = 0; ms && c < (int)cands.size() && c < ms_len; ++c) ms[c] = t[c];
@@ -1666,7 +1721,7 @@ int fest_group_get_next_solution(fest_group *g, int current_iter, double cfl) {
   for (fest_handle *h : g->blocks) h->b.CFLv = cfl;
   int rc0;
   if (g->batched) {
-    if (g->lists_order != g->order && (rc0 = build_lists(g, g->order))) return rc0;
+    if ((g->lists_order != g->order || g->lists_tile != g->tile_i) && (rc0 = build_lists(g, g->order))) return rc0;
     if (g->blk_cfl != cfl) {
       // the records carry the CFL number; nothing may still read them
       if (hipStreamSynchronize(g->stream) != hipSuccess) return gfail(g, FEST_E_HIP, "stream wait failed");
@@ -1702,6 +1757,12 @@ int fest_group_set_option(fest_group *g, const char *name, long value) {
     if (value < 0) return gfail(g, FEST_E_ARG, "grid_cap must be >= 0");
     g->graphs.reset();
     g->grid_cap = (int)value;
+    return 0;
+  }
+  if (!strcmp(name, "tile_i")) {
+    if (value != 64 && value != 32) return gfail(g, FEST_E_ARG, "tile_i must be 64 or 32");
+    g->graphs.reset();
+    g->tile_i = (int)value;
     return 0;
   }
   if (!strcmp(name, "order")) {

@@ -50,25 +50,23 @@ __device__ __forceinline__ FaceBC face_params(const Blk &b, int face) {
   return p;
 }
 
-// face-metric planes of direction d (0 i, 1 j, 2 k) without indexing the
-// pointer arrays at run time
-__device__ __forceinline__ const double *dsel(const double *const (&arr)[3], int d) {
-  return d == 0 ? arr[0] : (d == 1 ? arr[1] : arr[2]);
-}
-
 // src/boundary/copy_bc.f90:58-131 on one column
-__device__ void copy3col(const Blk &b, double *v, int type, int face, int a, int bb) {
+__device__ __forceinline__ void copy3col(const Blk &b, double *v, int type, int face, int a, int bb) {
   double a1, a2, a3;
-  int src[3];
-  if (type == CP_ANTI) { a1 = 1.; a2 = -1.; a3 = 0.; src[0] = 1; src[1] = 2; src[2] = 3; }
-  else if (type == CP_FLAT) { a1 = 1.; a2 = 1.; a3 = 0.; src[0] = 1; src[1] = 1; src[2] = 1; }
-  else { a1 = b.c1; a2 = b.c2; a3 = b.c3; src[0] = 1; src[1] = 2; src[2] = 3; }
-  for (int g = 0; g < 3; ++g)
-    v[fcell(b, face, -g, a, bb)] = (a2 * v[fcell(b, face, src[g], a, bb)] - a3 * v[fcell(b, face, src[g] + 1, a, bb)]) / a1;
+  if (type == CP_ANTI) { a1 = 1.; a2 = -1.; a3 = 0.; }
+  else if (type == CP_FLAT) { a1 = 1.; a2 = 1.; a3 = 0.; }
+  else { a1 = b.c1; a2 = b.c2; a3 = b.c3; }
+  // ghost layer g takes interior layer g+1 (anti, symm) or layer 1 (flat)
+#pragma unroll
+  for (int g = 0; g < 3; ++g) {
+    const int src = type == CP_FLAT ? 1 : g + 1;
+    v[fcell(b, face, -g, a, bb)] = (a2 * v[fcell(b, face, src, a, bb)] - a3 * v[fcell(b, face, src + 1, a, bb)]) / a1;
+  }
 }
 
 // src/boundary/bc_primitive.f90:484-525 on one column
-__device__ void fixcol(const Blk &b, double *v, double val, int face, int a, int bb) {
+__device__ __forceinline__ void fixcol(const Blk &b, double *v, double val, int face, int a, int bb) {
+#pragma unroll
   for (int g = 0; g < 3; ++g) v[fcell(b, face, -g, a, bb)] = val;
 }
 
@@ -79,12 +77,12 @@ __device__ void fixcol(const Blk &b, double *v, double val, int face, int a, int
 // k-kL's kL from the omega values (src/boundary/bc_primitive.f90:333-336),
 // which k-kL never reads from the boundary file (read_bc.f90:98-108)
 __device__ __forceinline__ bool two_eq(const Blk &b) { return b.ng >= 6; }
-__device__ void turb_copycol(const Blk &b, int type, int face, int a, int bb) {
+__device__ __forceinline__ void turb_copycol(const Blk &b, int type, int face, int a, int bb) {
   if (b.ng >= 5) copy3col(b, b.qp + 5 * b.nc, type, face, a, bb);
   if (two_eq(b)) copy3col(b, b.qp + 6 * b.nc, type, face, a, bb);
   if (b.lctm) copy3col(b, b.qp + 7 * b.nc, CP_FLAT, face, a, bb);
 }
-__device__ void turb_fixcol(const Blk &b, const FaceBC &p, int face, int a, int bb, bool kl_tw = false) {
+__device__ __forceinline__ void turb_fixcol(const Blk &b, const FaceBC &p, int face, int a, int bb, bool kl_tw = false) {
   if (two_eq(b)) {
     fixcol(b, b.qp + 5 * b.nc, p.fix[FX_TK], face, a, bb);
     fixcol(b, b.qp + 6 * b.nc, (b.kkl && !kl_tw) ? p.fix[FX_TKL] : p.fix[FX_TW], face, a, bb);
@@ -93,23 +91,25 @@ __device__ void turb_fixcol(const Blk &b, const FaceBC &p, int face, int a, int 
   if (b.lctm) fixcol(b, b.qp + 7 * b.nc, p.fix[FX_TGM], face, a, bb);
 }
 
-// unit normal of the boundary face at (a, bb)
+// unit normal of the boundary face at (a, bb); the metric planes are picked
+// per case with constant member indices (a run-time index into the
+// by-value argument's pointer arrays would put them in private memory)
 __device__ __forceinline__ void bnormal(const Blk &b, int face, int a, int bb, double &nx, double &ny, double &nz) {
-  int d = face >> 1;
   long f;
+  const double *px, *py, *pz;
   switch (face) {
-    case 0: f = fi(b, 0, 1, a, bb); break;
-    case 1: f = fi(b, 0, b.imx, a, bb); break;
-    case 2: f = fi(b, 1, a, 1, bb); break;
-    case 3: f = fi(b, 1, a, b.jmx, bb); break;
-    case 4: f = fi(b, 2, a, bb, 1); break;
-    default: f = fi(b, 2, a, bb, b.kmx); break;
+    case 0: f = fi(b, 0, 1, a, bb); px = b.fx[0]; py = b.fy[0]; pz = b.fz[0]; break;
+    case 1: f = fi(b, 0, b.imx, a, bb); px = b.fx[0]; py = b.fy[0]; pz = b.fz[0]; break;
+    case 2: f = fi(b, 1, a, 1, bb); px = b.fx[1]; py = b.fy[1]; pz = b.fz[1]; break;
+    case 3: f = fi(b, 1, a, b.jmx, bb); px = b.fx[1]; py = b.fy[1]; pz = b.fz[1]; break;
+    case 4: f = fi(b, 2, a, bb, 1); px = b.fx[2]; py = b.fy[2]; pz = b.fz[2]; break;
+    default: f = fi(b, 2, a, bb, b.kmx); px = b.fx[2]; py = b.fy[2]; pz = b.fz[2]; break;
   }
-  nx = dsel(b.fx, d)[f]; ny = dsel(b.fy, d)[f]; nz = dsel(b.fz, d)[f];
+  nx = px[f]; ny = py[f]; nz = pz[f];
 }
 
 // src/boundary/bc_primitive.f90:1778-1945
-__device__ void temp_based_density_col(const Blk &b, const FaceBC &p_, int face, int a, int bb) {
+__device__ __forceinline__ void temp_based_density_col(const Blk &b, const FaceBC &p_, int face, int a, int bb) {
   double Tw = p_.fix[FX_TWALL], R = b.R, gm = b.gm;
   double *rho = b.qp, *p = b.qp + 4 * b.nc;
   if (Tw < 0.0) {
@@ -130,7 +130,7 @@ __device__ void temp_based_density_col(const Blk &b, const FaceBC &p_, int face,
 }
 
 // src/boundary/bc_primitive.f90:564-643 (Sutherland mu whatever mu_variation)
-__device__ void omega_wall_col(const Blk &b, int face, int a, int bb) {
+__device__ __forceinline__ void omega_wall_col(const Blk &b, int face, int a, int bb) {
   double *rho = b.qp, *p = b.qp + 4 * b.nc, *tw = b.qp + 6 * b.nc;
   long c0 = fcell(b, face, 0, a, bb), c1 = fcell(b, face, 1, a, bb);
   for (int l = 1; l <= 3; ++l) {
@@ -148,7 +148,7 @@ __device__ void omega_wall_col(const Blk &b, int face, int a, int bb) {
 // pressure tp and the boundary Mach number and the density from the
 // boundary speed of sound.  Returns Unb and, when write != 0, stores layer 0
 // of rho,u,v,w,p
-__device__ double riemann_col(const Blk &b, int face, int a, int bb, int write, bool total, double tp) {
+__device__ __forceinline__ double riemann_col(const Blk &b, int face, int a, int bb, int write, bool total, double tp) {
   double gm = b.gm;
   double *rho = b.qp, *u = b.qp + b.nc, *v = b.qp + 2 * b.nc, *w = b.qp + 3 * b.nc, *p = b.qp + 4 * b.nc;
   double nx, ny, nz;
@@ -195,7 +195,7 @@ __device__ double riemann_col(const Blk &b, int face, int a, int bb, int write, 
 
 // src/boundary/FT_bc.f90:36-106; j and k faces reflect with the I-face
 // normal stored at the same index, as the reference does
-__device__ void flow_tangency_col(const Blk &b, int face, int a, int bb) {
+__device__ __forceinline__ void flow_tangency_col(const Blk &b, int face, int a, int bb) {
   double *u = b.qp + b.nc, *v = b.qp + 2 * b.nc, *w = b.qp + 3 * b.nc;
   double nx, ny, nz;
   bnormal(b, face, a, bb, nx, ny, nz);
@@ -315,24 +315,23 @@ __device__ __forceinline__ void k_bc_face_b(const Blk &b, const Wg &wg, int face
     case -9: {  // single-block periodic: whole plane, three layers
       // the reference's array assignment reads its source layers whole
       // before writing; with one or two cells in the direction they overlap
-      // the destination layers
+      // the destination layers.  Layers g = 0..2 sit at d0 + g*st (ghosts)
+      // and s0 + g*st (their sources) along the face normal's stride st.
+      long d0, s0, st;
+      switch (face) {
+        case 0: d0 = ci(b, -2, a, bb); s0 = ci(b, b.imx - 3, a, bb); st = 1; break;
+        case 1: d0 = ci(b, b.imx, a, bb); s0 = ci(b, 1, a, bb); st = 1; break;
+        case 2: d0 = ci(b, a, -2, bb); s0 = ci(b, a, b.jmx - 3, bb); st = b.pi; break;
+        case 3: d0 = ci(b, a, b.jmx, bb); s0 = ci(b, a, 1, bb); st = b.pi; break;
+        case 4: d0 = ci(b, a, bb, -2); s0 = ci(b, a, bb, b.kmx - 3); st = (long)b.pi * b.pj; break;
+        default: d0 = ci(b, a, bb, b.kmx); s0 = ci(b, a, bb, 1); st = (long)b.pi * b.pj; break;
+      }
       for (int l = 0; l < b.nv; ++l) {
         double *v = q + l * nc;
-        long dst[3];
-        double val[3];
-        for (int g = 0; g < 3; ++g) {
-          long src;
-          switch (face) {
-            case 0: dst[g] = ci(b, -2 + g, a, bb); src = ci(b, b.imx - 3 + g, a, bb); break;
-            case 1: dst[g] = ci(b, b.imx + g, a, bb); src = ci(b, 1 + g, a, bb); break;
-            case 2: dst[g] = ci(b, a, -2 + g, bb); src = ci(b, a, b.jmx - 3 + g, bb); break;
-            case 3: dst[g] = ci(b, a, b.jmx + g, bb); src = ci(b, a, 1 + g, bb); break;
-            case 4: dst[g] = ci(b, a, bb, -2 + g); src = ci(b, a, bb, b.kmx - 3 + g); break;
-            default: dst[g] = ci(b, a, bb, b.kmx + g); src = ci(b, a, bb, 1 + g); break;
-          }
-          val[g] = v[src];
-        }
-        for (int g = 0; g < 3; ++g) v[dst[g]] = val[g];
+        const double v0 = v[s0], v1 = v[s0 + st], v2 = v[s0 + 2 * st];
+        v[d0] = v0;
+        v[d0 + st] = v1;
+        v[d0 + 2 * st] = v2;
       }
       break;
     }

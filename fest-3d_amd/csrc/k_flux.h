@@ -562,7 +562,7 @@ __host__ __device__ inline long flux_i_span(const Blk &b) {
   return (b.imx + 2) + (long)(b.jmx + 1) * b.pi - flux_i_first(b) + 1;
 }
 
-template <int D, int NV, int RECON, bool FLAT, typename T>
+template <int D, int NV, int RECON, bool FLAT, typename T, int TI = TILE_I>
 __device__ __forceinline__ void k_flux_b(const Blk &b, const Wg &wg, int store_csnd) {
   using M = FaceMath<T>;
   const int ii = D == 0, jj = D == 1, kk = D == 2;
@@ -599,7 +599,7 @@ __device__ __forceinline__ void k_flux_b(const Blk &b, const Wg &wg, int store_c
     lds_row = &sq[0][t + 3];
   } else {
     // j, k sweeps: 64 x 4 tiles
-    if (!tile_ij(b.imx - 1 + ii, b.jmx - 1 + jj, i, j, blk)) return;
+    if (!tile_ij<TI>(b.imx - 1 + ii, b.jmx - 1 + jj, i, j, blk)) return;
     i += 1;
     j += 1;
     if (k > b.kmx - 1 + kk) return;
@@ -657,7 +657,9 @@ __device__ __forceinline__ void k_flux_b(const Blk &b, const Wg &wg, int store_c
 #pragma unroll
   for (int l = 0; l < nv; ++l) {
     bad |= isnan(out[l]);
-    b.flux[D][(long)l * b.nfl[D] + o] = (double)out[l];
+    // FP32 mode keeps the face fluxes in FP32 (the update widens them)
+    if constexpr (sizeof(T) == sizeof(float)) ((float *)b.flux[D])[(long)l * b.nfl[D] + o] = out[l];
+    else b.flux[D][(long)l * b.nfl[D] + o] = (double)out[l];
   }
   if (bad) atomicOr(b.err, ERR_NAN_FLUX);
   if (store_csnd) b.csnd[D][o] = (double)csf;

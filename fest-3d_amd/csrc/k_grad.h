@@ -246,7 +246,10 @@ __device__ __forceinline__ void lctm_source(double rho, double tk, double tw, do
 // source.  Interior gradients are final here: the ghost-gradient rule only
 // writes ghost cells.  NG is the number of gradient slots (4 laminar, 6
 // SST); a compile-time count keeps all 3*NG gradients in registers.
-template <int NG, bool S2003, bool BC, bool KKL>
+// F32: the FP32 mode when only FP32 gradients are kept (g32only) -- the
+// Green-Gauss sums run in FP32 over the FP32 state copy (Temp from its FP64
+// array), and mu, mu_t, F1 and the source take the FP32 gradients widened.
+template <int NG, bool S2003, bool BC, bool KKL, bool F32 = false>
 __device__ __forceinline__ void k_grad_b(const Blk &b, const Wg &wg) {
   int i, j;
   if (!flat_ij(b.imx + 1, b.jmx + 1, i, j, plane_block(wg, b.xcd_remap))) return;
@@ -274,6 +277,47 @@ __device__ __forceinline__ void k_grad_b(const Blk &b, const Wg &wg) {
   bool bad = false;
   int ndir = (b.kmx > 2) ? 3 : 2;
   double gg[NG][3];
+  if constexpr (F32) {
+    float nf[3][6], Af[6] = {(float)Ai0, (float)Aj0, (float)Ak0, (float)Ai1, (float)Aj1, (float)Ak1};
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int f = 0; f < 6; ++f) nf[d][f] = (float)n[d][f];
+    const float iv2 = (float)ivol2;
+#pragma unroll
+    for (int l = 0; l < NG; ++l) {
+      float vc, vmi, vmj, vmk, vpi, vpj, vpk;
+      if (l == 3) {
+        const double *v = b.temp;
+        vc = (float)v[c]; vmi = (float)v[c - si]; vmj = (float)v[c - sj]; vmk = (float)v[c - sk];
+        vpi = (float)v[c + si]; vpj = (float)v[c + sj]; vpk = (float)v[c + sk];
+      } else {
+        const float *v = b.q32 + (long)(l + 1) * b.nc;
+        vc = v[c]; vmi = v[c - si]; vmj = v[c - sj]; vmk = v[c - sk];
+        vpi = v[c + si]; vpj = v[c + sj]; vpk = v[c + sk];
+      }
+      const float sm_i = vmi + vc, sm_j = vmj + vc, sm_k = vmk + vc;
+      const float sp_i = vpi + vc, sp_j = vpj + vc, sp_k = vpk + vc;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        float g;
+        if (d < ndir) {
+          g = -(sm_i * nf[d][0] * Af[0]);
+          g = g - sm_j * nf[d][1] * Af[1];
+          g = g - sm_k * nf[d][2] * Af[2];
+          g = g + sp_i * nf[d][3] * Af[3];
+          g = g + sp_j * nf[d][4] * Af[4];
+          g = g + sp_k * nf[d][5] * Af[5];
+          g = g * iv2;
+        } else {
+          g = 0.0f;
+        }
+        bad |= isnan(g);
+        gg[l][d] = (double)g;
+        b.g32[d][(long)l * b.ngr + gidx] = g;
+      }
+    }
+  } else {
 #pragma unroll
   for (int l = 0; l < NG; ++l) {
     const double *v = grad_var(b, l);
@@ -299,6 +343,7 @@ __device__ __forceinline__ void k_grad_b(const Blk &b, const Wg &wg) {
       if (!b.g32only) b.g[d][(long)l * b.ngr + gidx] = g;
       if (b.fp32) b.g32[d][(long)l * b.ngr + gidx] = (float)g;
     }
+  }
   }
   if (bad) atomicOr(b.err, ERR_NAN_GRAD);
   double rho = b.qp[c];
@@ -379,9 +424,9 @@ __device__ __forceinline__ void k_grad_b(const Blk &b, const Wg &wg) {
   }
   }
 }
-template <int NG, bool S2003, bool BC = false, bool KKL = false>
+template <int NG, bool S2003, bool BC = false, bool KKL = false, bool F32 = false>
 __global__ void __launch_bounds__(256) k_grad(Blk b) {
-  k_grad_b<NG, S2003, BC, KKL>(b, FEST_WG_HW);
+  k_grad_b<NG, S2003, BC, KKL, F32>(b, FEST_WG_HW);
 }
 
 // mu_t / F1 ghost rules (src/viscosity.f90:407-465), face interiors

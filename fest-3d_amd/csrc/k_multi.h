@@ -42,19 +42,19 @@ __global__ void __launch_bounds__(256) km_bc_edges(const Blk *__restrict__ blks,
                                                    int n, int phase) {
   FEST_WALK(k_bc_edges_b(b, wg, phase))
 }
-template <int NG, bool S2003>
+template <int NG, bool S2003, bool F32 = false>
 __global__ void __launch_bounds__(256) km_grad(const Blk *__restrict__ blks, const WorkItem *__restrict__ items,
                                                int n) {
-  FEST_WALK((k_grad_b<NG, S2003, false, false>(b, wg)))
+  FEST_WALK((k_grad_b<NG, S2003, false, false, F32>(b, wg)))
 }
 __global__ void __launch_bounds__(256) km_grad_bc(const Blk *__restrict__ blks, const WorkItem *__restrict__ items,
                                                   int n) {
   FEST_WALK(k_grad_bc_b(b, wg))
 }
-template <int D, int NV, int RECON, bool FLAT, typename T>
+template <int D, int NV, int RECON, bool FLAT, typename T, int TI = TILE_I>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 km_flux(const Blk *__restrict__ blks, const WorkItem *__restrict__ items, int n, int store_csnd) {
-  FEST_WALK((k_flux_b<D, NV, RECON, FLAT, T>(b, wg, store_csnd)))
+  FEST_WALK((k_flux_b<D, NV, RECON, FLAT, T, TI>(b, wg, store_csnd)))
 }
 __global__ void __launch_bounds__(256) km_dt(const Blk *__restrict__ blks, const WorkItem *__restrict__ items,
                                              int n) {

@@ -167,11 +167,22 @@ __device__ __forceinline__ void k_update_b(const Blk &b, const Wg &wg, double TF
   long F0 = fli(b, 0, i, j, k), G0 = fli(b, 1, i, j, k), H0 = fli(b, 2, i, j, k);
   long sG = b.nfi[1], sH = (long)b.nfi[2] * b.nfj[2];
   double R[8];
-  for (int l = 0; l < nv; ++l) {
-    const double *F = b.flux[0] + (long)l * b.nfl[0];
-    const double *G = b.flux[1] + (long)l * b.nfl[1];
-    const double *H = b.flux[2] + (long)l * b.nfl[2];
-    R[l] = (F[F0 + 1] - F[F0]) + (G[G0 + sG] - G[G0]) + (H[H0 + sH] - H[H0]);
+  if (b.fp32) {
+    // FP32 mode: the face fluxes are stored in FP32, the sums taken in FP64
+    for (int l = 0; l < nv; ++l) {
+      const float *F = (const float *)b.flux[0] + (long)l * b.nfl[0];
+      const float *G = (const float *)b.flux[1] + (long)l * b.nfl[1];
+      const float *H = (const float *)b.flux[2] + (long)l * b.nfl[2];
+      R[l] = ((double)F[F0 + 1] - (double)F[F0]) + ((double)G[G0 + sG] - (double)G[G0]) +
+             ((double)H[H0 + sH] - (double)H[H0]);
+    }
+  } else {
+    for (int l = 0; l < nv; ++l) {
+      const double *F = b.flux[0] + (long)l * b.nfl[0];
+      const double *G = b.flux[1] + (long)l * b.nfl[1];
+      const double *H = b.flux[2] + (long)l * b.nfl[2];
+      R[l] = (F[F0 + 1] - F[F0]) + (G[G0 + sG] - G[G0]) + (H[H0 + sH] - H[H0]);
+    }
   }
   double rho = b.qp[c];
   const bool sst = b.ng >= 6, sa = b.ng == 5;
@@ -262,12 +273,16 @@ __global__ void __launch_bounds__(256) k_norm_partial(Blk b, double *red) {
           double v = b.res[(long)l * b.nres + t];
           acc[1 + l] += v * v;
         }
-      if (i == 1) acc[0] += b.flux[0][fli(b, 0, 1, j, k)];
-      if (i == ni) acc[0] -= b.flux[0][fli(b, 0, b.imx, j, k)];
-      if (j == 1) acc[0] += b.flux[1][fli(b, 1, i, 1, k)];
-      if (j == nj) acc[0] -= b.flux[1][fli(b, 1, i, b.jmx, k)];
-      if (k == 1) acc[0] += b.flux[2][fli(b, 2, i, j, 1)];
-      if (k == b.kmx - 1) acc[0] -= b.flux[2][fli(b, 2, i, j, b.kmx)];
+      // the mass fluxes (FP32 in the FP32 mode)
+      auto fl = [&](int d, long o) -> double {
+        return b.fp32 ? (double)((const float *)b.flux[d])[o] : b.flux[d][o];
+      };
+      if (i == 1) acc[0] += fl(0, fli(b, 0, 1, j, k));
+      if (i == ni) acc[0] -= fl(0, fli(b, 0, b.imx, j, k));
+      if (j == 1) acc[0] += fl(1, fli(b, 1, i, 1, k));
+      if (j == nj) acc[0] -= fl(1, fli(b, 1, i, b.jmx, k));
+      if (k == 1) acc[0] += fl(2, fli(b, 2, i, j, 1));
+      if (k == b.kmx - 1) acc[0] -= fl(2, fli(b, 2, i, j, b.kmx));
     }
   }
 #pragma unroll

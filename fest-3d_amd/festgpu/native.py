@@ -330,16 +330,17 @@ class GpuGroup:
         launch, 0 one per tile), "order" (0/1)."""
         self._check(lib().fest_group_set_option(self.g, name.encode(), int(value)))
 
-    AUTOTUNE_CANDIDATES = ["per-block"] + [f"batched cap={c} order={o}" for o in (0, 1)
+    AUTOTUNE_CANDIDATES = ["per-block"] + [f"batched cap={c} order={o} tile={t}x{256 // t}"
+                                           for t in (64, 32) for o in (0, 1)
                                            for c in ("tiles", "16/CU", "8/CU", "4/CU")]
 
     def autotune(self, steps: int = 4):
         """fest_group_autotune: times every schedule on a snapshot of the
         state (restored afterwards) and keeps the fastest.  Returns
         (chosen index, [ms per step of each candidate])."""
-        ms = np.zeros(16)
+        ms = np.zeros(32)
         nc, ch = ctypes.c_int(), ctypes.c_int()
-        self._check(lib().fest_group_autotune(self.g, int(steps), _ptr(ms), 16, ctypes.byref(nc), ctypes.byref(ch)))
+        self._check(lib().fest_group_autotune(self.g, int(steps), _ptr(ms), 32, ctypes.byref(nc), ctypes.byref(ch)))
         return ch.value, list(ms[: nc.value])
 
     def get_next_solution(self, current_iter: int, cfl: float = None):

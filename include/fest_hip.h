@@ -107,9 +107,10 @@ typedef struct {
 typedef struct fest_handle fest_handle;
 
 /* precision of fest_create: the plain FP64 path, or face reconstruction and
- * flux arithmetic in FP32 with metrics, gradients, the residue, the RK
- * update, the sources and the norms in FP64 (no counterpart in the
- * reference, which is FP64 throughout) */
+ * flux arithmetic in FP32 -- reading FP32 copies of the state and the
+ * gradients and storing the face fluxes in FP32 -- with metrics, the
+ * residue sums, the RK update, the sources and the norms in FP64 (no
+ * counterpart in the reference, which is FP64 throughout) */
 enum { FEST_PREC_FP64 = 0, FEST_PREC_FP32_FLUX = 1 };
 
 /* Allocates the device-resident block.  precision: FEST_PREC_*.
@@ -191,7 +192,9 @@ int fest_resnorm(fest_handle *h, double *res_abs);
 int fest_resnorm_device(fest_handle *h, double *res_abs_dev);
 
 /* Device pointers of the resident arrays for callers that keep the state
- * on the device.  name: "qp","temp","residue","F","G","H","dt". */
+ * on the device.  name: "qp","temp","residue","F","G","H","dt"; count is
+ * in doubles.  A handle created with FEST_PREC_FP32_FLUX holds F, G and H
+ * as floats: their count is then in floats. */
 int fest_device_ptr(fest_handle *h, const char *name, void **ptr, size_t *count);
 
 /* Kernel timing (no counterpart in the reference; it is for benchmarks
@@ -287,13 +290,17 @@ int fest_group_get_next_solution(fest_group *g, int current_iter, double cfl);
  *               (a persistent launch); 0 (default): one per tile.
  *   "order"     0: each block's tiles together; 1: the blocks' tiles
  *               interleaved.
+ *   "tile_i"    64 (default) or 32: the j and k face sweeps of a batched
+ *               step in 64 x 4 or 32 x 8 face tiles (narrow blocks leave
+ *               fewer lanes idle in the narrower tile).
  * The results do not depend on the schedule. */
 int fest_group_set_option(fest_group *g, const char *name, long value);
 /* Autotuner of the group step's schedule (no counterpart in the reference):
  * times `steps` (>= 3; the first two capture the graph) steps of every
  * candidate -- per-block launches, and batched with one work-group per
  * tile or a persistent 4, 8 or 16 work-groups per compute unit, tiles in
- * either order -- on a snapshot of the state, restores the state and keeps
+ * either order, 64 x 4 or 32 x 8 face tiles (17 in all) -- on a snapshot
+ * of the state, restores the state and keeps
  * the fastest.  ms[c] receives candidate c's time per step (up to ms_len),
  * *ncand the number of candidates and *chosen the pick (nullable). */
 int fest_group_autotune(fest_group *g, int steps, double *ms, int ms_len, int *ncand, int *chosen);

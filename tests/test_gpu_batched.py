@@ -2,7 +2,8 @@
 stage is one launch over the tile work list of all blocks (k_multi.h).  The
 tiles run the per-block kernels' own bodies, so a batched step equals the
 per-block step bit for bit, whatever the work-group count (a persistent
-launch walks the list) or the tile order; both follow the oracle's group
+launch walks the list), the tile order or the j/k face-tile shape (64 x 4
+or 32 x 8); both follow the oracle's group
 step (the reference's interface exchange, src/interface1.f90) to 1e-12.
 The autotuner leaves the state as it found it."""
 import dataclasses
@@ -51,7 +52,8 @@ def states(gs):
 
 
 @pytest.mark.parametrize("opts", [dict(batched=1), dict(batched=1, grid_cap=7), dict(batched=1, order=1),
-                                  dict(batched=1, grid_cap=3, order=1)])
+                                  dict(batched=1, grid_cap=3, order=1), dict(batched=1, tile_i=32),
+                                  dict(batched=1, tile_i=32, grid_cap=5, order=1)])
 def test_batched_chain_equals_per_block(opts):
     blocks, links = C.chain_blocks(nblocks=6, lo=8, hi=16, seed=3)
     say("batched chain", opts)
@@ -91,7 +93,7 @@ def test_batched_sst_split_equals_per_block(precision):
     case, nodes, q = C.sst_bump(24, 12, 12)
     blocks, links = M.split(case, nodes, q, 2, [5, 9])
     say("batched sst split, precision", precision)
-    a, ga = gpu_group(blocks, links, precision=precision, wall=True, batched=1)
+    a, ga = gpu_group(blocks, links, precision=precision, wall=True, batched=1, tile_i=32 if precision else 64)
     b, gb = gpu_group(blocks, links, precision=precision, wall=True)
     for it in range(1, 4):
         a.get_next_solution(it)
