@@ -17,6 +17,7 @@
 #include "fest_hip.h"
 #include "blk.h"
 #include "k_bc.h"
+#include "k_bc_kernels.h"
 #include "k_flux.h"
 #include "k_geom.h"
 #include "k_grad.h"
@@ -24,6 +25,8 @@
 #include "k_lusgs.h"
 #include "k_multi.h"
 #include "k_update.h"
+#include "flux_launch.h"
+#include "vjp_launch.h"
 
 using namespace fest;
 
@@ -87,6 +90,9 @@ struct fest_handle {
   StepGraphs graphs;
   int check_every = 1;  // read the device error word after every n-th step
   int unchecked = 0;
+  // fest_vjp's buffers, allocated at its first call: the residue adjoint,
+  // the per-face contributions of each direction and the cell adjoint
+  double *vjp_barR = nullptr, *vjp_c[3] = {}, *vjp_bar = nullptr;
 };
 
 // a group's view of its blocks; a block destroyed while in a group detaches
@@ -249,56 +255,15 @@ int check_err(fest_handle *h) {
   return fail(h, FEST_E_CONFIG, "device error bits 0x%x", e);
 }
 
-// the flux kernel of sweep D, specialised on the variable count and the
-// reconstruction
-template <int D, int NV, bool FLAT, typename T>
-void launch_flux_t(const Blk &b, dim3 g, dim3 t, hipStream_t st, int store_csnd) {
-  switch (b.recon) {
-    case 0: hipLaunchKernelGGL((k_flux<D, NV, 0, FLAT, T>), g, t, 0, st, b, store_csnd); break;
-    case 1: hipLaunchKernelGGL((k_flux<D, NV, 1, FLAT, T>), g, t, 0, st, b, store_csnd); break;
-    case 2: hipLaunchKernelGGL((k_flux<D, NV, 2, FLAT, T>), g, t, 0, st, b, store_csnd); break;
-    case 3: hipLaunchKernelGGL((k_flux<D, NV, 3, FLAT, T>), g, t, 0, st, b, store_csnd); break;
-    default: hipLaunchKernelGGL((k_flux<D, NV, 4, FLAT, T>), g, t, 0, st, b, store_csnd); break;
-  }
-}
-// FP64 face arithmetic, or FP32 when the handle was created with
-// FEST_PREC_FP32_FLUX
-template <int D, int NV, bool FLAT>
-void launch_flux_nv(const Blk &b, dim3 g, dim3 t, hipStream_t st, int store_csnd) {
-  if constexpr (NV == 8) {
-    launch_flux_t<D, NV, FLAT, double>(b, g, t, st, store_csnd);  // lctm2015 runs in FP64 only
-  } else {
-    if (b.fp32) launch_flux_t<D, NV, FLAT, float>(b, g, t, st, store_csnd);
-    else launch_flux_t<D, NV, FLAT, double>(b, g, t, st, store_csnd);
-  }
-}
-// the k faces of a 2-D grid (one cell in k) carry no convective flux
-template <int D, int NV>
-void launch_flux_v(const Blk &b, dim3 g, dim3 t, hipStream_t st, int store_csnd) {
-  if constexpr (D == 2) {
-    if (b.kmx == 2) {
-      launch_flux_nv<D, NV, true>(b, g, t, st, store_csnd);
-      return;
-    }
-  }
-  launch_flux_nv<D, NV, false>(b, g, t, st, store_csnd);
-}
-template <int D>
-void launch_flux(const Blk &b, dim3 g, dim3 t, hipStream_t st, int store_csnd) {
-  if (b.nv == 8) launch_flux_v<D, 8>(b, g, t, st, store_csnd);
-  else if (b.nv == 7) launch_flux_v<D, 7>(b, g, t, st, store_csnd);
-  else if (b.nv == 6) launch_flux_v<D, 6>(b, g, t, st, store_csnd);
-  else launch_flux_v<D, 5>(b, g, t, st, store_csnd);
-}
-
 // grid of 64 x 4 tiles over an (ni x nj x nk) box (the kernels use tile_ij)
 inline void tiles(long ni, long nj, long nk, dim3 &g, dim3 &t, int pad8) {
   t = dim3(TILE_I * TILE_J, 1, 1);
   g = dim3(plane_slots((long)cdiv(ni, TILE_I) * cdiv(nj, TILE_J), pad8), (unsigned)nk, 1);
 }
 
-// C2..C9(+C10 in k_update): ghosts, gradients/viscosity, face fluxes
-int launch_residual_front(fest_handle *h, int current_iter, int store_csnd) {
+// C2: the ghost cells of the block's own boundary rules, face by face, then
+// the edges and corners
+int launch_ghosts(fest_handle *h, int current_iter) {
   Blk &b = h->b;
   hipStream_t st = h->stream;
   // faces in the reference's order imin, imax, jmin, ...; the two faces of
@@ -327,6 +292,15 @@ int launch_residual_front(fest_handle *h, int current_iter, int store_csnd) {
     hipLaunchKernelGGL(k_bc_edges, dim3(cdiv(4L * (b.kmx + 5) * b.nv, 256)), dim3(256), 0, st, b, 1);
     hipLaunchKernelGGL(k_bc_edges, dim3(1), dim3(64), 0, st, b, 2);
   }
+  LAUNCHCHK(h);
+  return 0;
+}
+
+// C2..C9(+C10 in k_update): ghosts, gradients/viscosity, face fluxes
+int launch_residual_front(fest_handle *h, int current_iter, int store_csnd) {
+  Blk &b = h->b;
+  hipStream_t st = h->stream;
+  if (int rc = launch_ghosts(h, current_iter)) return rc;
   if (b.fp32) {
     KgScope ts(h, KG_BC_EDGES);
     hipLaunchKernelGGL(k_to_f32, dim3(cdiv(b.nc * b.nv, 256)), dim3(256), 0, st, b);
@@ -1024,6 +998,46 @@ int fest_residual(fest_handle *h, int current_iter) {
   return check_err(h);
 }
 
+// why fest_vjp cannot differentiate the block's residual (empty: it can)
+extern "C++" std::string vjp_reason(const Blk &b) {
+  if (b.mu_ref != 0.0) return "only the inviscid residual (mu_ref 0) is differentiated";
+  if (b.nv != 5) return "only the five flow variables are differentiated (no turbulence model)";
+  if (b.fp32) return "the adjoint is of the FP64 residual (FEST_PREC_FP64 handles)";
+  for (int d = 0; d < 3; ++d)
+    if (b.pbs[d]) return "the pressure-based switch is not differentiated";
+  // the ghost rules read interior layers 1..4 only with 4 interior cells
+  if (b.imx < 5 || b.jmx < 5 || (b.kmx != 2 && b.kmx < 5))
+    return "every 3-D direction needs at least 4 cells";
+  for (int f = 0; f < 6; ++f)
+    if (b.bc[f] == -5 && !(b.fix[FX_TWALL][f] >= 0.0 && b.fix[FX_TWALL][f] <= 1.0))
+      return "no-slip walls with an adiabatic or isothermal density rule are not differentiated";
+  return "";
+}
+
+int fest_vjp(fest_handle *h, const double *bar_residue, double *bar_qp, double *bar_nodes) {
+  if (!h || !bar_residue || !bar_qp) return FEST_E_ARG;
+  if (!h->have_grid || !h->have_state) return fail(h, FEST_E_ARG, "grid and state must be set first");
+  Blk &b = h->b;
+  const std::string why = bar_nodes ? std::string("the node-coordinate adjoint is not built (bar_nodes must be NULL)")
+                                    : vjp_reason(b);
+  if (!why.empty()) return fail(h, FEST_E_CONFIG, "fest_vjp: %s", why.c_str());
+  hipStream_t st = h->stream;
+  int rc;
+  if (!h->vjp_bar) {
+    if ((rc = dalloc(h, &h->vjp_barR, 5 * b.nres))) return rc;
+    for (int d = 0; d < 3; ++d)
+      if ((rc = dalloc(h, &h->vjp_c[d], vjp_contrib_len(b, d)))) return rc;
+    if ((rc = dalloc(h, &h->vjp_bar, 5 * b.nc))) return rc;
+  }
+  HIPCHK(h, hipMemcpyAsync(h->vjp_barR, bar_residue, sizeof(double) * 5 * b.nres, hipMemcpyHostToDevice, st));
+  // the ghost cells as fest_residual at its first iteration (fixed faces refixed)
+  if ((rc = launch_ghosts(h, 1))) return rc;
+  launch_vjp(b, st, h->vjp_barR, h->vjp_c, h->vjp_bar);
+  LAUNCHCHK(h);
+  HIPCHK(h, hipMemcpyAsync(bar_qp, h->vjp_bar, sizeof(double) * 5 * b.nc, hipMemcpyDeviceToHost, st));
+  return check_err(h);
+}
+
 int fest_time_step(fest_handle *h, double cfl) {
   if (!h) return FEST_E_ARG;
   h->b.CFLv = cfl;
@@ -1390,40 +1404,7 @@ int upload_blks(fest_group *g) {
   return 0;
 }
 
-extern "C++" {
-template <int D, int NV, int RECON, typename T>
-void km_flux_launch(fest_group *g, int ph, unsigned grid, int store_csnd) {
-  // the j and k sweeps in 64 x 4 or 32 x 8 face tiles (the group's tile_i)
-  if (D != 0 && g->tile_i == 32)
-    hipLaunchKernelGGL((km_flux<D, NV, RECON, false, T, 32>), dim3(grid), dim3(256), 0, g->stream, g->d_blks,
-                       g->d_items + g->ph_off[ph], g->ph_n[ph], store_csnd);
-  else
-    hipLaunchKernelGGL((km_flux<D, NV, RECON, false, T>), dim3(grid), dim3(256), 0, g->stream, g->d_blks,
-                       g->d_items + g->ph_off[ph], g->ph_n[ph], store_csnd);
-}
-template <int D, int NV, typename T>
-void km_flux_recon(fest_group *g, int ph, unsigned grid, int store_csnd) {
-  switch (g->blocks[0]->b.recon) {
-    case 0: km_flux_launch<D, NV, 0, T>(g, ph, grid, store_csnd); break;
-    case 1: km_flux_launch<D, NV, 1, T>(g, ph, grid, store_csnd); break;
-    case 2: km_flux_launch<D, NV, 2, T>(g, ph, grid, store_csnd); break;
-    case 3: km_flux_launch<D, NV, 3, T>(g, ph, grid, store_csnd); break;
-    default: km_flux_launch<D, NV, 4, T>(g, ph, grid, store_csnd); break;
-  }
-}
-template <int D>
-void km_flux_any(fest_group *g, int ph, unsigned grid, int store_csnd) {
-  const Blk &b = g->blocks[0]->b;
-  if (b.nv == 7) {
-    if (b.fp32) km_flux_recon<D, 7, float>(g, ph, grid, store_csnd);
-    else km_flux_recon<D, 7, double>(g, ph, grid, store_csnd);
-  } else {
-    if (b.fp32) km_flux_recon<D, 5, float>(g, ph, grid, store_csnd);
-    else km_flux_recon<D, 5, double>(g, ph, grid, store_csnd);
-  }
-}
 
-}  // extern "C++"
 
 // one operation on all blocks through the phases' work lists
 int run_group_op_batched(fest_group *g, const StepOp &op, int current_iter) {
@@ -1466,9 +1447,14 @@ int run_group_op_batched(fest_group *g, const StepOp &op, int current_iter) {
         }
         KM(PH_GRAD_BC, km_grad_bc, dim3(64, 4, 1));
       }
-      if (g->ph_n[PH_FLUX_I]) km_flux_any<0>(g, PH_FLUX_I, grid(PH_FLUX_I), op.flags);
-      if (g->ph_n[PH_FLUX_J]) km_flux_any<1>(g, PH_FLUX_J, grid(PH_FLUX_J), op.flags);
-      if (g->ph_n[PH_FLUX_K]) km_flux_any<2>(g, PH_FLUX_K, grid(PH_FLUX_K), op.flags);
+      for (int d = 0; d < 3; ++d) {
+        const int ph = PH_FLUX_I + d;
+        if (!g->ph_n[ph]) continue;
+        const WorkItem *items = g->d_items + g->ph_off[ph];
+        if (d == 0) launch_flux_batched<0>(b0, st, g->d_blks, items, g->ph_n[ph], grid(ph), g->tile_i, op.flags);
+        else if (d == 1) launch_flux_batched<1>(b0, st, g->d_blks, items, g->ph_n[ph], grid(ph), g->tile_i, op.flags);
+        else launch_flux_batched<2>(b0, st, g->d_blks, items, g->ph_n[ph], grid(ph), g->tile_i, op.flags);
+      }
       break;
     case OP_DT: KM(PH_DT, km_dt, dim3(256)); break;
     case OP_UPDATE: KM(PH_UPDATE, km_update, dim3(256), op.a, op.c, op.flags); break;

@@ -3,7 +3,8 @@
 // edge/corner averages.  src/boundary/bc_primitive.f90:55-227 and the
 // routines it calls; every face kernel gives one thread a column of three
 // ghost cells, which only reads its own column, so the per-face passes keep
-// the reference's sequential semantics.
+// the reference's sequential semantics.  The bodies and helpers are here,
+// their per-block kernels in k_bc_kernels.h (one translation unit).
 #pragma once
 // ghost cells and geometry keep the reference's unfused operation order
 #pragma clang fp contract(off)
@@ -146,49 +147,64 @@ __device__ __forceinline__ void omega_wall_col(const Blk &b, int face, int a, in
 // or, when total, the total-pressure inlet (bc_primitive.f90:1237-1776):
 // the same Riemann-invariant velocity, then the pressure from the total
 // pressure tp and the boundary Mach number and the density from the
-// boundary speed of sound.  Returns Unb and, when write != 0, stores layer 0
-// of rho,u,v,w,p
-__device__ __forceinline__ double riemann_col(const Blk &b, int face, int a, int bb, int write, bool total, double tp) {
-  double gm = b.gm;
-  double *rho = b.qp, *u = b.qp + b.nc, *v = b.qp + 2 * b.nc, *w = b.qp + 3 * b.nc, *p = b.qp + 4 * b.nc;
+// boundary speed of sound.  c is the first interior cell's rho, u, v, w, p;
+// returns Unb and, when full, the ghost layer 0 state g.  Written for a type
+// T: double, or the dual numbers of the adjoint (k_dual.h)
+template <typename T>
+__device__ __forceinline__ T riemann_state(const Blk &b, int face, int a, int bb, const T (&c)[5], bool full,
+                                           bool total, double tp, T (&g)[5]) {
+  const double gm = b.gm;
   double nx, ny, nz;
   bnormal(b, face, a, bb, nx, ny, nz);
   if ((face & 1) == 0) { nx = -nx; ny = -ny; nz = -nz; }
-  long c1 = fcell(b, face, 1, a, bb), cg = fcell(b, face, 0, a, bb);
-  double cexp = sqrt(gm * p[c1] / rho[c1]);
+  T cexp = sqrt(gm * c[4] / c[0]);
   double cinf = sqrt(gm * b.p_inf / b.rho_inf);
-  double Unexp = u[c1] * nx + v[c1] * ny + w[c1] * nz;
+  T Unexp = c[1] * nx + c[2] * ny + c[3] * nz;
   double Uninf = b.u_inf * nx + b.v_inf * ny + b.w_inf * nz;
   double Rinf = Uninf - 2 * cinf / (gm - 1.);
-  double Rexp = Unexp + 2 * cexp / (gm - 1.);
-  double Unb = 0.5 * (Rexp + Rinf);
-  double Cb = 0.25 * (gm - 1.) * (Rexp - Rinf);
-  if (!write) return Unb;
+  T Rexp = Unexp + 2 * cexp / (gm - 1.);
+  T Unb = 0.5 * (Rexp + Rinf);
+  T Cb = 0.25 * (gm - 1.) * (Rexp - Rinf);
+  if (!full) return Unb;
   if (Unb > 0.) {
-    double vd = Unb - Unexp;
-    u[cg] = u[c1] + vd * nx;
-    v[cg] = v[c1] + vd * ny;
-    w[cg] = w[c1] + vd * nz;
+    T vd = Unb - Unexp;
+    g[1] = c[1] + vd * nx;
+    g[2] = c[2] + vd * ny;
+    g[3] = c[3] + vd * nz;
     if (!total) {
-      double sen = p[c1] / pow(rho[c1], gm);
-      rho[cg] = pow(Cb * Cb / (gm * sen), 1. / (gm - 1.));
-      p[cg] = (rho[cg] * Cb * Cb / gm);
+      T sen = c[4] / pow(c[0], gm);
+      g[0] = pow(Cb * Cb / (gm * sen), 1. / (gm - 1.));
+      g[4] = (g[0] * Cb * Cb / gm);
     }
   } else {
-    double vd = Unb - Uninf;
-    u[cg] = b.u_inf + vd * nx;
-    v[cg] = b.v_inf + vd * ny;
-    w[cg] = b.w_inf + vd * nz;
+    T vd = Unb - Uninf;
+    g[1] = b.u_inf + vd * nx;
+    g[2] = b.v_inf + vd * ny;
+    g[3] = b.w_inf + vd * nz;
     if (!total) {
       double sen = b.p_inf / pow(b.rho_inf, gm);
-      rho[cg] = pow(Cb * Cb / (gm * sen), 1. / (gm - 1.));
-      p[cg] = (rho[cg] * Cb * Cb / gm);
+      g[0] = pow(Cb * Cb / (gm * sen), 1. / (gm - 1.));
+      g[4] = (g[0] * Cb * Cb / gm);
     }
   }
   if (total) {
-    double Mb = sqrt(u[cg] * u[cg] + v[cg] * v[cg] + w[cg] * w[cg]) / Cb;
-    p[cg] = tp / pow((1 + 0.5 * (gm - 1.) * Mb * Mb), gm / (gm - 1.));
-    rho[cg] = gm * p[cg] / (Cb * Cb);
+    T Mb = sqrt(g[1] * g[1] + g[2] * g[2] + g[3] * g[3]) / Cb;
+    g[4] = tp / pow((1 + 0.5 * (gm - 1.) * Mb * Mb), gm / (gm - 1.));
+    g[0] = gm * g[4] / (Cb * Cb);
+  }
+  return Unb;
+}
+// riemann_state on the resident state; when write != 0 it stores layer 0
+// of rho, u, v, w, p
+__device__ __forceinline__ double riemann_col(const Blk &b, int face, int a, int bb, int write, bool total, double tp) {
+  const long c1 = fcell(b, face, 1, a, bb), cg = fcell(b, face, 0, a, bb);
+  double c[5], g[5];
+#pragma unroll
+  for (int l = 0; l < 5; ++l) c[l] = b.qp[(long)l * b.nc + c1];
+  const double Unb = riemann_state(b, face, a, bb, c, write != 0, total, tp, g);
+  if (write) {
+#pragma unroll
+    for (int l = 0; l < 5; ++l) b.qp[(long)l * b.nc + cg] = g[l];
   }
   return Unb;
 }
@@ -339,9 +355,6 @@ __device__ __forceinline__ void k_bc_face_b(const Blk &b, const Wg &wg, int face
       break;  // interfaces (>=0) and -10 are filled by the interface exchange
   }
 }
-__global__ void k_bc_face(Blk b, int face0, int current_iter) {
-  k_bc_face_b(b, FEST_WG_HW, face0, current_iter);
-}
 
 // edges and corners, src/boundary/bc_primitive.f90:209-224: phase 0 the
 // four lines along i, phase 1 the four lines along k, phase 2 the corners
@@ -383,9 +396,6 @@ __device__ __forceinline__ void k_bc_edges_b(const Blk &b, const Wg &wg, int pha
     }
   }
 #undef Q
-}
-__global__ void k_bc_edges(Blk b, int phase) {
-  k_bc_edges_b(b, FEST_WG_HW, phase);
 }
 
 }  // namespace fest

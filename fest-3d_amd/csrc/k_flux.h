@@ -84,7 +84,7 @@ struct FaceMath {
   // the weights are the quotients g_i/D_i themselves.
   static __device__ __forceinline__ T weno_combine(T D1, T D2, T D3, T Q1, T Q2, T Q3) {
     const T g1 = T(1.0) / T(10.0), g2 = T(6.0) / T(10.0), g3 = T(3.0) / T(10.0);
-    if constexpr (sizeof(T) == sizeof(double)) {
+    if constexpr (!std::is_same<T, float>::value) {
       T e1 = g1 * (D2 * D3), e2 = g2 * (D1 * D3), e3 = g3 * (D1 * D2);
       return tdiv(e1 * Q1 + e2 * Q2 + e3 * Q3, T(6.0) * (e1 + e2 + e3));
     } else {

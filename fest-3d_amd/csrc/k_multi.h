@@ -5,30 +5,17 @@
 // launch walks the list; with fewer work-groups than tiles each work-group
 // takes tiles t, t + gridDim.x, ... in turn (a persistent launch).  The
 // tile bodies are the per-block kernels' own (k_*_b), so a tile computes
-// exactly what the per-block launch computes for it.
+// exactly what the per-block launch computes for it.  The face sweeps'
+// batched kernel, km_flux, is in k_flux_walk.h.
 #pragma once
 #include "blk.h"
 #include "k_bc.h"
 #include "k_flux.h"
 #include "k_grad.h"
 #include "k_update.h"
+#include "k_walk.h"
 
 namespace fest {
-
-struct WorkItem {
-  int blk, x, y, z;
-};
-
-// the tile walk; the barrier after each tile keeps a body's LDS tiles
-// (the i sweep's stencil rows) from being overwritten while read
-#define FEST_WALK(BODY_CALL)                                          \
-  for (int t = (int)blockIdx.x; t < n; t += (int)gridDim.x) {          \
-    const WorkItem it = items[t];                                      \
-    const Blk &b = blks[it.blk];                                       \
-    const Wg wg{it.x, it.y, it.z, 0};                                  \
-    BODY_CALL;                                                         \
-    __syncthreads();                                                   \
-  }
 
 __global__ void __launch_bounds__(256) km_step_begin(const Blk *__restrict__ blks,
                                                      const WorkItem *__restrict__ items, int n, int store_u) {
@@ -51,11 +38,6 @@ __global__ void __launch_bounds__(256) km_grad_bc(const Blk *__restrict__ blks, 
                                                   int n) {
   FEST_WALK(k_grad_bc_b(b, wg))
 }
-template <int D, int NV, int RECON, bool FLAT, typename T, int TI = TILE_I>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
-km_flux(const Blk *__restrict__ blks, const WorkItem *__restrict__ items, int n, int store_csnd) {
-  FEST_WALK((k_flux_b<D, NV, RECON, FLAT, T, TI>(b, wg, store_csnd)))
-}
 __global__ void __launch_bounds__(256) km_dt(const Blk *__restrict__ blks, const WorkItem *__restrict__ items,
                                              int n) {
   FEST_WALK(k_dt_b(b, wg))
@@ -68,7 +50,5 @@ __global__ void __launch_bounds__(256) km_to_f32(const Blk *__restrict__ blks, c
                                                  int n) {
   FEST_WALK(k_to_f32_b(b, wg))
 }
-
-#undef FEST_WALK
 
 }  // namespace fest

@@ -394,6 +394,33 @@ def flat_plate(ni=512, nj=256, nk=128, seed=5, flux="ldfss0", recon="ppm", turb=
     return c, x, q
 
 
+def euler_bump(ni=16, nj=12, nk=6, seed=7, flux="ausmP", recon="muscl", bc_id=(-3, -4, -6, -8, -9, -9),
+               mach=0.5, amp=0.05, noise=0.01):
+    """Inviscid (mu_ref 0) channel over the 0.05L bump in nondimensional
+    units: free stream rho 1, p 1/gamma (sound speed 1), |V| = mach along
+    x; every face's fixed values are the free stream's, the no-slip wall
+    keeps the symmetric density (Tw 0) and a -11 face gets a total pressure
+    0.2% above the free stream's.  Moderate wall clustering (first spacing
+    0.02), 8 Gaussian bumps of amplitude `amp` and seeded noise of `noise`,
+    so the limiters and the flux switches take varied branches."""
+    c = Case(ni + 1, nj + 1, nk + 1, flux=flux, recon=recon, turb="none", time_local=1, accur=0,
+             lim=(1, 1, 1), tlim=(0, 0, 0), bc_id=tuple(bc_id), CFL=0.5, name=f"euler_bump_{ni}x{nj}x{nk}")
+    c.rho_inf, c.p_inf = 1.0, 1.0 / c.gm
+    c.u_inf, c.v_inf, c.w_inf = float(mach), 0.0, 0.0
+    c.vel_mag = c.MInf = float(mach)
+    c.R_gas = 1.0 / c.gm  # T = p / (R rho) = 1 in the free stream
+    c.T_ref = 1.0
+    c.mu_ref = 0.0
+    c.fix_rho, c.fix_u, c.fix_v = (c.rho_inf,) * 6, (c.u_inf,) * 6, (0.0,) * 6
+    c.fix_w, c.fix_p, c.fix_twall = (0.0,) * 6, (c.p_inf,) * 6, (0.0,) * 6
+    pt = 1.002 * c.p_inf * (1 + 0.5 * (c.gm - 1) * mach * mach) ** (c.gm / (c.gm - 1))
+    c.fix_tp = (pt,) * 6
+    nodes = bump_nodes(c, first=0.02, span=0.25 if nk > 1 else 0.05)
+    q = gaussian_bumps(c, freestream_state(c), seed, amp=amp)
+    q = uniform_noise(c, q, seed + 1, noise)
+    return c, nodes, q
+
+
 def chain_blocks(nblocks=64, lo=16, hi=96, seed=3, flux="ausmUP", recon="muscl", h=1.0 / 64, warp=0.02):
     """A seeded chain of `nblocks` laminar blocks (MUSCL + AUSM-UP, M=0.8):
     extents drawn from U{lo..hi}; block b+1 sits against block b across the

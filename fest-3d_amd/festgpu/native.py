@@ -80,6 +80,7 @@ def lib():
         L.fest_upload_state.argtypes = [vp, dp, dp]
         L.fest_get_next_solution.argtypes = [vp, ip, ctypes.c_double]
         L.fest_residual.argtypes = [vp, ip]
+        L.fest_vjp.argtypes = [vp, dp, dp, dp]
         L.fest_time_step.argtypes = [vp, ctypes.c_double]
         L.fest_download.argtypes = [vp, ctypes.c_uint32] + [dp] * 14
         L.fest_resnorm.argtypes = [vp, dp]
@@ -101,7 +102,7 @@ def lib():
         L.fest_group_autotune.restype = ctypes.c_int
         for f in ("fest_set_option", "fest_group_set_option", "fest_create", "fest_upload_geometry",
                   "fest_set_grid", "fest_upload_state",
-                  "fest_get_next_solution", "fest_residual", "fest_time_step", "fest_download", "fest_resnorm",
+                  "fest_get_next_solution", "fest_residual", "fest_vjp", "fest_time_step", "fest_download", "fest_resnorm",
                   "fest_device_ptr", "fest_sync", "fest_timing", "fest_kernel_time", "fest_group_create",
                   "fest_group_exchange", "fest_group_wall_dist", "fest_group_resnorm",
                   "fest_group_get_next_solution"):
@@ -209,6 +210,18 @@ class GpuBlock:
 
     def residual(self, current_iter: int = 1):
         self._check(lib().fest_residual(self.h, int(current_iter)))
+
+    def vjp(self, bar_residue: np.ndarray) -> np.ndarray:
+        """fest_vjp: (dR/dqp)^T bar_residue for the inviscid residual R of
+        the resident state.  bar_residue has the residue's shape
+        (5, kmx-1, jmx-1, imx-1); the result the state's (5, padded cells),
+        zero in ghost cells but those of interface faces."""
+        c = self.case
+        bar = np.ascontiguousarray(bar_residue, dtype=np.float64)
+        assert bar.shape == self.shape_of("residue")
+        out = np.zeros((5,) + c.cell_shape())
+        self._check(lib().fest_vjp(self.h, _ptr(bar), _ptr(out), None))
+        return out
 
     def time_step(self, cfl: float = None):
         cfl = self.case.CFL if cfl is None else cfl

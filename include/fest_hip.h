@@ -152,6 +152,27 @@ int fest_get_next_solution(fest_handle *h, int current_iter, double cfl);
  * (src/update.f90:495-547), on the resident state and Temp. */
 int fest_residual(fest_handle *h, int current_iter);
 
+/* The discrete adjoint of the inviscid residual: bar_qp = (dR/dqp)^T
+ * bar_residue at the resident state, R being what fest_residual computes
+ * (get_total_conservative_Residue, src/update.f90:495-547: the fluxes of
+ * src/face/flux/convective/ on the face states of src/face/state/,
+ * differenced per cell as in src/face/flux/convective/scheme.f90:111-141),
+ * with the ghost cells of the block's own boundary rules
+ * (src/boundary/bc_primitive.f90) taken as functions of the interior cells.
+ * No counterpart in the reference, which has no adjoint: it is for
+ * optimisation and sensitivity callers.
+ * bar_residue has the residue's layout (1:imx-1,1:jmx-1,1:kmx-1,1:5),
+ * bar_qp the state's (-2:imx+2,-2:jmx+2,-2:kmx+2,1:5): interior cells hold
+ * the adjoint, the ghost cells of interface faces (ids >= 0 and -10) the
+ * share owed to the neighbouring block, every other ghost cell zero.  The
+ * ghost cells are refilled first, as fest_residual does at current_iter 1.
+ * Inviscid FP64 handles with five variables, no pressure-based switch and
+ * at least 4 cells in every 3-D direction, else FEST_E_CONFIG.  The result
+ * is the exact derivative of the branches (limiters, switches) the
+ * residual takes at this state.  bar_nodes, the adjoint with respect to
+ * the grid nodes, is not built: it must be NULL (else FEST_E_CONFIG). */
+int fest_vjp(fest_handle *h, const double *bar_residue, double *bar_qp, double *bar_nodes);
+
 /* compute_time_step (src/time.f90:291-334) from the face states of the
  * last residual evaluation. */
 int fest_time_step(fest_handle *h, double cfl);

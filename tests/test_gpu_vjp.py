@@ -1,0 +1,324 @@
+"""The discrete adjoint of the inviscid residual (fest_vjp), through the C ABI.
+
+fest_vjp returns bar = (dR/dq)^T w for the residual R that fest_residual
+computes, the ghost cells following from the boundary rules.  The checks:
+
+- Jacobian columns: bar[v, c] = d<w, R>/dq[v, c] by central differences
+  (h = 1e-7 of the variable's free-stream scale) of the GPU's own residual,
+  for every interior input of grids with each supported boundary rule, and
+  for seeded samples of inputs under every flux scheme and reconstruction
+  (2-D grids included);
+- the same columns from the oracle's residual: the adjoint of the residual
+  the oracle restates (and which is pinned to the reference);
+- the transposition identity <bar, dq> = d/dh <w, R(q + h dq)> over 32
+  seeded directions dq of all interior cells at once, for a scheme whose
+  residual is smooth at this state (WENO5 + van Leer, subsonic: no limiter
+  or switch changes branch).  With MUSCL, PPM or the AUSM family a
+  direction that moves every cell at once crosses some limiter kink or
+  switch inside [-h, h] (there are ~10^4 of them), and the difference
+  quotient then carries that kink's jump (1e-4..1e-2 relative), while
+  single-input columns stay at ~1e-8: that is why those schemes are
+  checked column by column;
+- torch.autograd.gradcheck of the residual as a differentiable function
+  (festgpu.autograd) on a 2-D 6 x 5 grid: the full Jacobian, column by
+  column from finite differences, row by row from fest_vjp;
+- the unsteady adjoint of 12 RK4 steps with checkpoints (festgpu.adjoint)
+  against central differences of the whole trajectory;
+- ghost cells hold no adjoint without interface faces, the adjoint is
+  linear in w and deterministic;
+- configurations it does not differentiate fail with FEST_E_CONFIG.
+
+Tolerance: 1e-6 relative (per column: of the largest |bar| of that
+variable; per direction: of the larger side).  The differences leave ~1e-8
+of rounding at h = 1e-7 (measured), far below the 1e-3..1 that a wrong
+sign, a dropped term or a missed ghost rule gives.
+"""
+import dataclasses
+import sys
+
+import numpy as np
+import pytest
+
+from binding import Oracle
+from festgpu import cases as C
+from festgpu import native
+from festgpu.native import FestError, GpuBlock
+
+pytestmark = pytest.mark.gpu
+H = 1e-7
+TOL = 1e-6
+INNER = (slice(None),) + (slice(3, -3),) * 3
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def scales(case):
+    return np.array([case.rho_inf, case.vel_mag, case.vel_mag, case.vel_mag, case.p_inf])
+
+
+def directions(case, seed, n):
+    """n seeded directions on the interior cells, scaled per variable."""
+    rng = np.random.default_rng(seed)
+    shape = (5, case.kmx - 1, case.jmx - 1, case.imx - 1)
+    s = scales(case)[:, None, None, None]
+    return [s * rng.uniform(-1.0, 1.0, shape) for _ in range(n)]
+
+
+def weights(case, seed):
+    rng = np.random.default_rng(seed + 99)
+    return rng.uniform(-1.0, 1.0, (5, case.kmx - 1, case.jmx - 1, case.imx - 1))
+
+
+def gpu_residual(g, q):
+    g.upload_state(q)
+    g.residual(1)
+    return g.download("residue")
+
+
+def fd_dot(resid, q, dq, w):
+    qp, qm = q.copy(), q.copy()
+    qp[INNER] += H * dq
+    qm[INNER] -= H * dq
+    return float(np.sum(w * (resid(qp) - resid(qm)))) / (2 * H)
+
+
+def check_identity(g, case, q, resid, seed, ndir):
+    w = weights(case, seed)
+    g.upload_state(q)
+    bar = g.vjp(w)
+    worst = 0.0
+    for dq in directions(case, seed, ndir):
+        a = float(np.sum(bar[INNER] * dq))
+        b = fd_dot(resid, q, dq, w)
+        err = abs(a - b) / max(abs(a), abs(b), 1e-300)
+        worst = max(worst, err)
+    return bar, worst
+
+
+def check_columns(g, case, q, resid, seed, sample=None):
+    """max over the checked inputs (v, k, j, i) of |bar - FD| / max|bar_v|;
+    sample: how many seeded inputs, None for every interior input."""
+    w = weights(case, seed)
+    g.upload_state(q)
+    bar = g.vjp(w)
+    inner = bar[INNER]
+    shape = inner.shape
+    n = int(np.prod(shape))
+    idx = range(n) if sample is None else np.random.default_rng(seed + 7).choice(n, sample, replace=False)
+    sc = scales(case)
+    top = np.abs(inner).reshape(5, -1).max(axis=1)
+    worst = 0.0
+    for f in idx:
+        v, k, j, i = np.unravel_index(int(f), shape)
+        d = H * sc[v]
+        qp, qm = q.copy(), q.copy()
+        qp[v, 3 + k, 3 + j, 3 + i] += d
+        qm[v, 3 + k, 3 + j, 3 + i] -= d
+        fd = float(np.sum(w * (resid(qp) - resid(qm)))) / (2 * d)
+        worst = max(worst, abs(inner[v, k, j, i] - fd) / top[v])
+    return bar, worst
+
+
+BC_MIXES = [
+    (-3, -4, -6, -6, -9, -9),   # subsonic inlet/outlet, slip walls, periodic k
+    (-8, -8, -6, -8, -9, -9),   # far field around a slip wall
+    (-1, -2, -5, -7, -9, -9),   # supersonic inlet/outlet, no-slip wall, pole
+    (-11, -4, -6, -8, -6, -6),  # total-pressure inlet, slip walls in k
+    (-9, -9, -5, -6, -9, -9),   # periodic i, no-slip and slip walls
+]
+
+
+@pytest.mark.parametrize("bcs", BC_MIXES)
+def test_jacobian_columns_boundary_rules(bcs):
+    """Every interior input of a 10 x 7 x 5 grid, MUSCL + AUSM+."""
+    say("vjp columns bcs", bcs)
+    case, nodes, q = C.euler_bump(10, 7, 5, bc_id=bcs)
+    g = GpuBlock(case)
+    g.set_grid(nodes)
+    bar, worst = check_columns(g, case, q, lambda x: gpu_residual(g, x), 11)
+    say("  worst", worst)
+    assert worst <= TOL
+    # no interface faces: every ghost cell ends without adjoint
+    ghost = np.ones(case.cell_shape(), bool)
+    ghost[3:-3, 3:-3, 3:-3] = False
+    assert np.all(bar[:, ghost] == 0.0)
+
+
+@pytest.mark.parametrize("flux", ["ausm", "ausmP", "ausmUP", "slau", "ldfss0", "van_leer"])
+@pytest.mark.parametrize("recon", ["none", "muscl", "ppm", "weno", "weno_NM"])
+def test_jacobian_columns_schemes(flux, recon):
+    """120 seeded inputs per scheme pair on a 12 x 9 x 5 grid."""
+    say("vjp columns", flux, recon)
+    case, nodes, q = C.euler_bump(12, 9, 5, flux=flux, recon=recon, seed=23)
+    g = GpuBlock(case)
+    g.set_grid(nodes)
+    _, worst = check_columns(g, case, q, lambda x: gpu_residual(g, x), 5, sample=120)
+    say("  worst", worst)
+    assert worst <= TOL
+
+
+@pytest.mark.parametrize("limiter", ["koren", "van_albada"])
+def test_jacobian_columns_two_dimensional(limiter):
+    say("vjp 2-D", limiter)
+    case, nodes, q = C.euler_bump(12, 8, 1, bc_id=(-3, -4, -6, -8, -9, -9))
+    case = dataclasses.replace(case, limiter=limiter)
+    g = GpuBlock(case)
+    g.set_grid(nodes)
+    _, worst = check_columns(g, case, q, lambda x: gpu_residual(g, x), 3)
+    say("  worst", worst)
+    assert worst <= TOL
+
+
+@pytest.mark.parametrize("bcs", [(-3, -4, -6, -6, -9, -9), (-8, -8, -5, -6, -6, -6)])
+def test_transposition_identity_32_directions(bcs):
+    """WENO5 + van Leer at M 0.5: a smooth residual, 32 seeded directions.
+    A far field runs through it only where the flow crosses it (imin, imax
+    of the second mix): where the normal velocity is near zero (a jmax far
+    field above a channel) its inflow/outflow switch is a jump that
+    directions moving every cell cross; the columns cover that face."""
+    say("vjp directions", bcs)
+    case, nodes, q = C.euler_bump(14, 10, 6, flux="van_leer", recon="weno", bc_id=bcs, seed=4)
+    g = GpuBlock(case)
+    g.set_grid(nodes)
+    _, worst = check_identity(g, case, q, lambda x: gpu_residual(g, x), 4, 32)
+    say("  worst", worst)
+    assert worst <= TOL
+
+
+@pytest.mark.parametrize("flux,recon,bcs", [
+    ("ausmP", "muscl", (-3, -4, -6, -8, -9, -9)),
+    ("slau", "weno", (-8, -8, -6, -8, -9, -9)),
+    ("ldfss0", "ppm", (-1, -2, -5, -7, -9, -9)),
+])
+def test_adjoint_of_the_oracle_residual(flux, recon, bcs):
+    """Columns of the oracle's residual: 150 seeded inputs."""
+    say("vjp vs oracle", flux, recon, bcs)
+    case, nodes, q = C.euler_bump(12, 9, 5, flux=flux, recon=recon, bc_id=bcs, seed=31)
+    g = GpuBlock(case)
+    g.set_grid(nodes)
+    o = Oracle(case)
+    o.set_nodes(nodes)
+
+    def oracle_residual(x):
+        o.set_state(x)
+        assert o.residual(1) == 0
+        return o.arr("res").copy()
+
+    _, worst = check_columns(g, case, q, oracle_residual, 17, sample=150)
+    say("  worst", worst)
+    assert worst <= TOL
+
+
+def test_adjoint_is_linear_in_the_weights():
+    case, nodes, q = C.euler_bump(12, 9, 5)
+    g = GpuBlock(case)
+    g.set_grid(nodes)
+    g.upload_state(q)
+    w1, w2 = weights(case, 1), weights(case, 2)
+    b1, b2 = g.vjp(w1), g.vjp(w2)
+    b12 = g.vjp(2.0 * w1 - 3.0 * w2)
+    scale = np.abs(b12).max()
+    assert np.abs(b12 - (2.0 * b1 - 3.0 * b2)).max() <= 1e-13 * scale
+    assert np.array_equal(g.vjp(w1), b1)  # deterministic: no atomics
+
+
+def test_gradcheck_through_torch():
+    torch = pytest.importorskip("torch")
+    from festgpu.autograd import inviscid_residual
+    case, nodes, q = C.euler_bump(6, 5, 1, bc_id=(-3, -4, -6, -8, -9, -9), seed=41)
+    g = GpuBlock(case)
+    g.set_grid(nodes)
+    qi = torch.from_numpy(np.ascontiguousarray(q[INNER])).requires_grad_(True)
+    say("gradcheck 6x5x1")
+    assert torch.autograd.gradcheck(lambda x: inviscid_residual(x, g), (qi,), eps=1e-6, atol=1e-7, rtol=1e-5)
+
+
+@pytest.mark.parametrize("change", [
+    dict(mu_ref=1e-3),                                   # viscous
+    dict(pbs=(1, 0, 0)),                                 # pressure-based switch
+    dict(bc_id=(-3, -4, -5, -8, -9, -9), fix_twall=(0, 0, 2.0, 0, 0, 0)),  # isothermal density rule
+])
+def test_undifferentiated_configurations_fail(change):
+    case, nodes, q = C.euler_bump(12, 9, 5)
+    case = dataclasses.replace(case, **change)
+    g = GpuBlock(case)
+    g.set_grid(nodes)
+    g.upload_state(q)
+    with pytest.raises(FestError) as e:
+        g.vjp(weights(case, 0))
+    assert e.value.code == -3
+
+
+def test_turbulent_and_fp32_handles_fail():
+    """The SST bump has seven variables: the raw entry point refuses it."""
+    case, nodes, q = C.sst_bump(12, 9, 5)
+    g = GpuBlock(case)
+    g.set_grid(nodes, wall_dist=True)
+    g.upload_state(q)
+    w = np.zeros(case.n_var * case.ncells)
+    out = np.zeros(5 * int(np.prod(case.cell_shape())))
+    with pytest.raises(FestError) as e:
+        g._check(native.lib().fest_vjp(g.h, native._ptr(w), native._ptr(out), None))
+    assert e.value.code == -3
+    ec, en, eq = C.euler_bump(12, 9, 5)
+    g32 = GpuBlock(ec, precision=1)
+    g32.set_grid(en)
+    g32.upload_state(eq)
+    with pytest.raises(FestError) as e:
+        g32.vjp(weights(ec, 0))
+    assert e.value.code == -3
+
+
+def test_node_adjoint_is_refused():
+    case, nodes, q = C.euler_bump(12, 9, 5)
+    g = GpuBlock(case)
+    g.set_grid(nodes)
+    g.upload_state(q)
+    w = weights(case, 0)
+    out = np.zeros((5,) + case.cell_shape())
+    bn = np.zeros(case.node_shape())
+    with pytest.raises(FestError) as e:
+        g._check(native.lib().fest_vjp(g.h, native._ptr(w), native._ptr(out), native._ptr(bn)))
+    assert e.value.code == -3
+
+
+def test_unsteady_adjoint_with_checkpoints():
+    """d J(q_N) / d q_0 through 12 RK4 steps (global time step, WENO5 + van
+    Leer), J the pressure x-force on the jmin slip wall: the host-driven step equals the
+    device's RK4 step, the reverse sweep with checkpoints every 4 steps
+    matches central differences of the whole trajectory, and equals the
+    sweep without intermediate checkpoints bit for bit."""
+    from festgpu.adjoint import INNER as IN, RK4Adjoint, wall_force_x
+    dt = 0.004
+    case, nodes, q = C.euler_bump(12, 9, 5, flux="van_leer", recon="weno", bc_id=(-3, -4, -6, -8, -9, -9),
+                                  seed=51)
+    case = dataclasses.replace(case, time_local=0, global_time_step=dt, time_accuracy="RK4")
+    g = GpuBlock(case)
+    g.set_grid(nodes)
+    adj = RK4Adjoint(g, dt)
+    qi = np.ascontiguousarray(q[IN])
+    g.upload_state(q)
+    g.get_next_solution(1)
+    qdev = g.download("qp")[IN]
+    qhost = adj.step(qi)
+    d = np.abs(qhost - qdev).max() / np.abs(qdev).max()
+    say("host step vs device step", d)
+    assert d <= 1e-13
+    J, dJ = wall_force_x(g, 2)
+    N, K = 12, 4
+    qN, grad = adj.gradient(qi, N, K, dJ)
+    assert adj.vjp_calls == 4 * N
+    worst = 0.0
+    for dq in directions(case, 61, 4):
+        a = float(np.sum(grad * dq))
+        jp = J(adj.forward(qi + H * dq, N, N)[0])
+        jm = J(adj.forward(qi - H * dq, N, N)[0])
+        b = (jp - jm) / (2 * H)
+        worst = max(worst, abs(a - b) / max(abs(a), abs(b)))
+    say("  unsteady worst", worst)
+    assert worst <= TOL
+    _, grad1 = adj.gradient(qi, N, N, dJ)
+    assert np.array_equal(grad, grad1)
