@@ -16,7 +16,10 @@ bytes of one launch (festgpu.roofline).  `traffic` is the measured HBM bytes
 per launch of that kernel (FETCH_SIZE + WRITE_SIZE, scaled by a copy of
 known size) from the rocprofv3 counter summary committed for this workload,
 or null when there is none.  `stage_roofline` prices a whole RK stage at the
-fused minimum of 406 (SST) / 326 (laminar) bytes per cell.
+fused minimum of 406 (SST) / 326 (laminar) bytes per cell.  `adjoint` is
+the kernel time of fest_vjp against that of the residual it differentiates,
+on the inviscid version of the workload (same extents, scheme and boundary
+ids).
 
 --time-accuracy implicit times the reference's LU-SGS integrator on the same
 workload instead: a step is one residual evaluation, the time step and the
@@ -288,6 +291,39 @@ def fp32_long_error(case, nodes, q, stages):
     return out
 
 
+def adjoint_run(case, reps=3):
+    """Cost of the adjoint against the residual it differentiates, on the
+    inviscid version of the workload (same extents, scheme and boundary
+    ids; fest_vjp covers the inviscid residual): kernel time per call from
+    HIP events, the host copies of fest_vjp's arguments excluded."""
+    import numpy as np
+    from festgpu import cases as C
+    from festgpu.native import GpuBlock
+    ec, en, eq = C.euler_bump(case.imx - 1, case.jmx - 1, case.kmx - 1, flux=case.flux, recon=case.recon,
+                              bc_id=case.bc_id)
+    g = GpuBlock(ec)
+    g.set_grid(en)
+    g.upload_state(eq)
+    w = np.random.default_rng(0).uniform(-1.0, 1.0, g.shape_of("residue"))
+    g.residual(1)
+    g.vjp(w)  # allocates the adjoint's buffers
+    g.timing("*")
+    for _ in range(reps):
+        g.residual(1)
+    g.sync()
+    res_ms = sum(g.kernel_time(grp)[0] for grp in g.KERNEL_GROUPS) / reps
+    g.timing("*")
+    for _ in range(reps):
+        g.vjp(w)
+    g.sync()
+    vjp_ms = sum(g.kernel_time(grp)[0] for grp in g.KERNEL_GROUPS) / reps
+    faces_ms = g.kernel_time("vjp")[0] / reps
+    g.timing(None)
+    g.close()
+    return {"workload": ec.name, "scheme": f"{ec.recon}+{ec.flux}", "bc": list(ec.bc_id),
+            "residual_ms": res_ms, "vjp_ms": vjp_ms, "vjp_kernels_ms": faces_ms, "ratio": vjp_ms / res_ms}
+
+
 def chain_run(args):
     """The batched multiblock workload: 64 seeded uneven blocks (about 11.5M
     cells) joined in a chain (festgpu.cases.chain_blocks), MUSCL + AUSM-UP,
@@ -359,6 +395,8 @@ def main():
                     help="RK stages after which the FP32 face mode is compared with FP64 (0: skip)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch kernel by kernel instead of replaying captured step graphs")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the cost measurement of fest_vjp on the inviscid version of the workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -453,6 +491,8 @@ def main():
         out["fp32_flux"]["speedup_vs_fp64"] = out["fp32_flux"]["value"] / value
         if args.fp32_error_stages > 0 and not implicit:
             out["fp32_flux"]["long_run"] = fp32_long_error(case, nodes, q, args.fp32_error_stages)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_run(case)
     if not args.no_cpu_baseline:
         dist = g.download("dist") if case.sst_family else None
         out["cpu_baseline"] = cpu_baseline(case_fn, ni, nj, nk, dist, implicit=implicit)

@@ -32,10 +32,10 @@ using namespace fest;
 
 // kernel groups timed by fest_timing (one or a few launches each)
 enum { KG_STEP_BEGIN = 0, KG_BC_FACE, KG_BC_EDGES, KG_GRAD, KG_GRAD_BC, KG_FLUX_I, KG_FLUX_J, KG_FLUX_K,
-       KG_TIME_STEP, KG_UPDATE, KG_LU_DIAG, KG_LU_SWEEP, KG_LU_APPLY, KG_KKL_SOURCE, KG_COUNT };
+       KG_TIME_STEP, KG_UPDATE, KG_LU_DIAG, KG_LU_SWEEP, KG_LU_APPLY, KG_KKL_SOURCE, KG_VJP, KG_COUNT };
 static const char *const kg_names[KG_COUNT] = {"step_begin", "bc_face", "bc_edges", "grad", "grad_bc",
                                                "flux_i", "flux_j", "flux_k", "time_step", "update",
-                                               "lusgs_diag", "lusgs_sweep", "lusgs_apply", "kkl_source"};
+                                               "lusgs_diag", "lusgs_sweep", "lusgs_apply", "kkl_source", "vjp"};
 
 struct KgTimer {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
@@ -1032,7 +1032,10 @@ int fest_vjp(fest_handle *h, const double *bar_residue, double *bar_qp, double *
   HIPCHK(h, hipMemcpyAsync(h->vjp_barR, bar_residue, sizeof(double) * 5 * b.nres, hipMemcpyHostToDevice, st));
   // the ghost cells as fest_residual at its first iteration (fixed faces refixed)
   if ((rc = launch_ghosts(h, 1))) return rc;
-  launch_vjp(b, st, h->vjp_barR, h->vjp_c, h->vjp_bar);
+  {
+    KgScope ts(h, KG_VJP);
+    launch_vjp(b, st, h->vjp_barR, h->vjp_c, h->vjp_bar);
+  }
   LAUNCHCHK(h);
   HIPCHK(h, hipMemcpyAsync(bar_qp, h->vjp_bar, sizeof(double) * 5 * b.nc, hipMemcpyDeviceToHost, st));
   return check_err(h);

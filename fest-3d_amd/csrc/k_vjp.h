@@ -4,11 +4,14 @@
 // on the face states of src/face/state/*.f90), with respect to the cell
 // state, through the ghost-cell rules of src/boundary/bc_primitive.f90.
 //
-// Each face's flux depends on the stencil cells n-3..n+2 of its direction.
-// k_vjp_faces evaluates the face's own arithmetic (FaceMath) in dual numbers,
-// once per stencil cell with the tangents seeded on that cell's five
-// variables: the exact local Jacobian, one 5 x 5 block per stencil cell,
-// contracted at once with the face's adjoint flux barF = barR(n-1) - barR(n).
+// Each face's flux depends on the stencil cells n-3..n+2 of its direction
+// through the face states qL, qR.  The two factors of the chain rule are
+// taken apart, both in dual numbers running the face's own arithmetic
+// (FaceMath): k_vjp_flux differentiates the flux with tangents on qL, then
+// on qR (the 5 x 10 flux Jacobian, contracted at once with the face's
+// adjoint flux barF = barR(n-1) - barR(n) into bar qL, bar qR);
+// k_vjp_recon, one thread per face and variable, differentiates that
+// variable's reconstruction with tangents on its six stencil values.
 // k_vjp_gather sums, for every cell, the contributions of the faces whose
 // stencils hold it (in a fixed order: deterministic), and k_vjp_bc moves the
 // adjoints of ghost cells onto the interior cells the ghost rules read.
@@ -24,71 +27,129 @@ namespace fest {
 constexpr int VJP_STENCIL = 6;  // stencil cells n-3..n+2 of a face
 constexpr int VJP_PER_FACE = VJP_STENCIL * 5;
 
-// one thread per face of direction D (faces 1..N along D, 1..N-1 across);
-// contrib[face][m][v]: d(barF . F)/dq(cell n-3+m, variable v)
-template <int D, int RECON>
-__global__ void __launch_bounds__(256) k_vjp_faces(Blk b, const double *__restrict__ barR, double *contrib) {
-  using DT = Dual<5>;
-  using M = FaceMath<DT>;
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= b.nfl[D]) return;
+// face t of direction D (faces 1..N along D, 1..N-1 across, in the F/G/H
+// order): its indices and the stencil's cell and stride
+struct VjpFace {
+  int i, j, k, n, N;
+  long c, s;
+};
+template <int D>
+__device__ __forceinline__ VjpFace vjp_face(const Blk &b, long t) {
+  VjpFace f;
   const int ni = b.nfi[D], nj = b.nfj[D];
-  const int i = (int)(t % ni) + 1, j = (int)((t / ni) % nj) + 1, k = (int)(t / ((long)ni * nj)) + 1;
+  f.i = (int)(t % ni) + 1;
+  f.j = (int)((t / ni) % nj) + 1;
+  f.k = (int)(t / ((long)ni * nj)) + 1;
+  f.N = D == 0 ? b.imx : D == 1 ? b.jmx : b.kmx;
+  f.n = D == 0 ? f.i : D == 1 ? f.j : f.k;
+  f.c = ci(b, f.i, f.j, f.k);
+  f.s = cstride(b, D);
+  return f;
+}
+
+// one thread per face of direction D: fbar[l][face] = bar qL_l and
+// fbar[5 + l][face] = bar qR_l, the flux's adjoint on the face states.  The
+// ten tangents (qL then qR) go through the flux KT at a time: fewer
+// tangents per pass keep fewer values live (the FP64 flux arithmetic is
+// register-bound) at the price of more passes
+constexpr int VJP_FLUX_TANGENTS = 5;
+template <int D, int RECON>
+__global__ void __launch_bounds__(256) k_vjp_flux(Blk b, const double *__restrict__ barR, double *fbar) {
+  constexpr int KT = VJP_FLUX_TANGENTS;
+  using DK = Dual<KT>;
+  const long nf = b.nfl[D];
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nf) return;
+  const VjpFace f = vjp_face<D>(b, t);
   const int ii = D == 0, jj = D == 1, kk = D == 2;
-  const int N = D == 0 ? b.imx : D == 1 ? b.jmx : b.kmx;
-  const int n = D == 0 ? i : D == 1 ? j : k;
-  double *out = contrib + t * VJP_PER_FACE;
-#pragma unroll
-  for (int q = 0; q < VJP_PER_FACE; ++q) out[q] = 0.0;
-  // the k faces of a 2-D grid carry no convective flux
-  if (D == 2 && b.kmx == 2) return;
   // barF = barR of the cell below (n-1) minus that of the cell above (n)
   double barF[5];
-  const bool below = n >= 2, above = n <= N - 1;
-  const long rb = below ? ri(b, i - ii, j - jj, k - kk) : 0, ra = above ? ri(b, i, j, k) : 0;
+  const bool below = f.n >= 2, above = f.n <= f.N - 1;
+  const long rb = below ? ri(b, f.i - ii, f.j - jj, f.k - kk) : 0, ra = above ? ri(b, f.i, f.j, f.k) : 0;
   bool any = false;
 #pragma unroll
   for (int l = 0; l < 5; ++l) {
     barF[l] = (below ? barR[(long)l * b.nres + rb] : 0.0) - (above ? barR[(long)l * b.nres + ra] : 0.0);
     any |= barF[l] != 0.0;
   }
-  if (!any) return;
-  const long s = cstride(b, D);
-  const long c = ci(b, i, j, k);
-  const long fo = fi(b, D, i, j, k);
-  const DT nx = b.fx[D][fo], ny = b.fy[D][fo], nz = b.fz[D][fo], A = b.fA[D][fo];
-  const DT zero = (n == 1) ? (double)b.mz[D][0] : (n == N) ? (double)b.mz[D][1] : 1.0;
-  typename M::Aux ax;
-  ax.pb = false;
-  if constexpr (RECON == 4) M::nm_blends(b.vol + c, s, ax);
-  // the stencil cells each reconstruction reads (k_flux.h face_states)
-  constexpr int m0 = RECON == 0 ? 2 : RECON == 1 ? 1 : 0;
-  constexpr int m1 = RECON == 0 ? 3 : RECON == 1 ? 4 : 5;
-  for (int m = m0; m <= m1; ++m) {
-    DT qL[5], qR[5], fl[5];
+  // the k faces of a 2-D grid carry no convective flux
+  if (D == 2 && b.kmx == 2) any = false;
+  double bq[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // bar qL, then bar qR
+  if (any) {
+    double q[10];  // qL, then qR
+    typename FaceMath<double>::Aux ax;
+    ax.pb = false;
+    if constexpr (RECON == 4) FaceMath<double>::nm_blends(b.vol + f.c, f.s, ax);
 #pragma unroll
-    for (int l = 0; l < 5; ++l) {
-      DT u[VJP_STENCIL];
+    for (int l = 0; l < 5; ++l)
+      FaceMath<double>::template face_states<RECON>(b, D, f.n, f.N, l, b.qp + (long)l * b.nc + f.c, f.s, ax, q[l],
+                                                    q[5 + l]);
+    const long fo = fi(b, D, f.i, f.j, f.k);
+    const DK nx = b.fx[D][fo], ny = b.fy[D][fo], nz = b.fz[D][fo], A = b.fA[D][fo];
+    const DK zero = (f.n == 1) ? (double)b.mz[D][0] : (f.n == f.N) ? (double)b.mz[D][1] : 1.0;
 #pragma unroll
-      for (int mm = 0; mm < VJP_STENCIL; ++mm) {
-        u[mm] = DT(b.qp[(long)l * b.nc + c + (long)(mm - 3) * s]);
-        if (mm == m) u[mm].d[l] = 1.0;
+    for (int p0 = 0; p0 < 10; p0 += KT) {
+      DK L[5], R[5], fl[5];
+#pragma unroll
+      for (int l = 0; l < 5; ++l) {
+        L[l] = DK(q[l]);
+        R[l] = DK(q[5 + l]);
       }
-      M::template face_states<RECON>(b, D, n, N, l, &u[3], 1, ax, qL[l], qR[l]);
-    }
-    M::template inviscid_flux<5>(b, qL, qR, nx, ny, nz, A, zero, fl);
 #pragma unroll
-    for (int v = 0; v < 5; ++v) {
-      double acc = 0.0;
+      for (int e = 0; e < KT; ++e) {
+        const int x = p0 + e;  // tangent e of this pass: input x
+        if (x < 5) L[x].d[e] = 1.0;
+        else if (x < 10) R[x - 5].d[e] = 1.0;
+      }
+      FaceMath<DK>::template inviscid_flux<5>(b, L, R, nx, ny, nz, A, zero, fl);
 #pragma unroll
-      for (int l = 0; l < 5; ++l) acc += fl[l].d[v] * barF[l];
-      out[m * 5 + v] = acc;
+      for (int e = 0; e < KT; ++e) {
+        double acc = 0.0;
+#pragma unroll
+        for (int l = 0; l < 5; ++l) acc += fl[l].d[e] * barF[l];
+        if (p0 + e < 10) bq[p0 + e] = acc;
+      }
     }
   }
+#pragma unroll
+  for (int l = 0; l < 10; ++l) fbar[(long)l * nf + t] = bq[l];
+}
+
+// one thread per face of direction D and variable l (l-major):
+// contrib[m * 5 + l][face] = bar qL_l dqL_l/du_m + bar qR_l dqR_l/du_m, the
+// face's contribution to the adjoint of variable l of stencil cell n-3+m
+template <int D, int RECON>
+__global__ void __launch_bounds__(256) k_vjp_recon(Blk b, const double *__restrict__ fbar, double *contrib) {
+  using D6 = Dual<VJP_STENCIL>;
+  const long nf = b.nfl[D];
+  const long tt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tt >= 5 * nf) return;
+  const int l = (int)(tt / nf);
+  const long t = tt - (long)l * nf;
+  const double bl = fbar[(long)l * nf + t], br = fbar[(long)(5 + l) * nf + t];
+  double o[VJP_STENCIL] = {0, 0, 0, 0, 0, 0};
+  if (bl != 0.0 || br != 0.0) {
+    const VjpFace f = vjp_face<D>(b, t);
+    typename FaceMath<D6>::Aux ax;
+    ax.pb = false;
+    if constexpr (RECON == 4) FaceMath<D6>::nm_blends(b.vol + f.c, f.s, ax);
+    D6 u[VJP_STENCIL], L, R;
+#pragma unroll
+    for (int m = 0; m < VJP_STENCIL; ++m) {
+      u[m] = D6(b.qp[(long)l * b.nc + f.c + (long)(m - 3) * f.s]);
+      u[m].d[m] = 1.0;
+    }
+    FaceMath<D6>::template face_states<RECON>(b, D, f.n, f.N, l, &u[3], 1, ax, L, R);
+#pragma unroll
+    for (int m = 0; m < VJP_STENCIL; ++m) o[m] = bl * L.d[m] + br * R.d[m];
+  }
+#pragma unroll
+  for (int m = 0; m < VJP_STENCIL; ++m) contrib[(long)(m * 5 + l) * nf + t] = o[m];
 }
 
 // one thread per padded cell: the sum over directions and over the faces
-// whose stencils hold the cell (face n = c + 3 - m of stencil position m)
+// whose stencils hold the cell (face n = c + 3 - m of stencil position m);
+// c0, c1, c2: the contributions of the three directions (k_vjp_recon)
 __global__ void __launch_bounds__(256) k_vjp_gather(Blk b, const double *__restrict__ c0,
                                                      const double *__restrict__ c1, const double *__restrict__ c2,
                                                      double *bar) {
@@ -97,6 +158,7 @@ __global__ void __launch_bounds__(256) k_vjp_gather(Blk b, const double *__restr
   const int i = (int)(t % b.pi) - 2, j = (int)((t / b.pi) % b.pj) - 2, k = (int)(t / ((long)b.pi * b.pj)) - 2;
   double acc[5] = {0, 0, 0, 0, 0};
   const double *cs[3] = {c0, c1, c2};
+  const long nf[3] = {b.nfl[0], b.nfl[1], b.nfl[2]};
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const int N = d == 0 ? b.imx : d == 1 ? b.jmx : b.kmx;
@@ -110,7 +172,7 @@ __global__ void __launch_bounds__(256) k_vjp_gather(Blk b, const double *__restr
       if (n < 1 || n > N) continue;
       const long f = d == 0 ? fli(b, 0, n, j, k) : d == 1 ? fli(b, 1, i, n, k) : fli(b, 2, i, j, n);
 #pragma unroll
-      for (int v = 0; v < 5; ++v) acc[v] += cs[d][f * VJP_PER_FACE + m * 5 + v];
+      for (int v = 0; v < 5; ++v) acc[v] += cs[d][(long)(m * 5 + v) * nf[d] + f];
     }
   }
 #pragma unroll

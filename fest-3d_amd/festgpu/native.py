@@ -240,7 +240,7 @@ class GpuBlock:
         self._check(lib().fest_set_option(self.h, name.encode(), int(value)))
 
     KERNEL_GROUPS = ("step_begin", "bc_face", "bc_edges", "grad", "grad_bc", "flux_i", "flux_j", "flux_k",
-                     "time_step", "update", "lusgs_diag", "lusgs_sweep", "lusgs_apply", "kkl_source")
+                     "time_step", "update", "lusgs_diag", "lusgs_sweep", "lusgs_apply", "kkl_source", "vjp")
 
     def timing(self, groups=None):
         """Resets the event totals and brackets the launches of the named

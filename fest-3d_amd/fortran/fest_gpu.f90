@@ -83,7 +83,7 @@ module fest_gpu
   end type fest_link
 
   public :: fest_create, fest_destroy, fest_last_error, fest_upload_geometry, fest_set_grid, &
-            fest_upload_state, fest_get_next_solution, fest_residual, fest_time_step, &
+            fest_upload_state, fest_get_next_solution, fest_residual, fest_vjp, fest_time_step, &
             fest_download, fest_resnorm, fest_sync, fest_check, &
             fest_group_create, fest_group_destroy, fest_group_exchange, fest_group_wall_dist, &
             fest_group_resnorm, fest_group_get_next_solution
@@ -143,6 +143,17 @@ module fest_gpu
       type(c_ptr), value :: h
       integer(c_int), value :: current_iter
     end function fest_residual
+
+    ! the adjoint of the inviscid residual (no counterpart in the reference):
+    ! bar_qp(-2:imx+2,-2:jmx+2,-2:kmx+2,1:5) = (dR/dqp)^T bar_residue; pass
+    ! c_null_ptr for bar_nodes
+    integer(c_int) function fest_vjp(h, bar_residue, bar_qp, bar_nodes) bind(c, name='fest_vjp')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: bar_residue(*)
+      real(c_double), intent(out) :: bar_qp(*)
+      type(c_ptr), value :: bar_nodes
+    end function fest_vjp
 
     integer(c_int) function fest_time_step(h, cfl) bind(c, name='fest_time_step')
       import :: c_int, c_ptr, c_double

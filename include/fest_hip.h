@@ -224,7 +224,8 @@ int fest_device_ptr(fest_handle *h, const char *name, void **ptr, size_t *count)
  * stream -- NULL or "" none, "*" all, else a comma-separated list of
  * "step_begin", "bc_face", "bc_edges", "grad", "grad_bc", "flux_i",
  * "flux_j", "flux_k", "time_step", "update", "lusgs_diag", "lusgs_sweep",
- * "lusgs_apply", "kkl_source" (the k-kL source pass, k-kL blocks only).
+ * "lusgs_apply", "kkl_source" (the k-kL source pass, k-kL blocks only),
+ * "vjp" (the adjoint kernels of fest_vjp, after its ghost-cell pass).
  * fest_kernel_time waits for
  * the stream and returns one group's summed event time and the number of
  * bracketed launches since the last reset. */
