@@ -93,6 +93,9 @@ struct fest_handle {
   // fest_vjp's buffers, allocated at its first call: the residue adjoint,
   // the per-face contributions of each direction and the cell adjoint
   double *vjp_barR = nullptr, *vjp_c[3] = {}, *vjp_bar = nullptr;
+  // and, for the node-coordinate adjoint: the metric adjoints, the slip
+  // walls' ghost-row faces on the j and k max faces, the node adjoint
+  double *vjp_m[3] = {}, *vjp_g3 = nullptr, *vjp_g5 = nullptr, *vjp_nodes = nullptr;
 };
 
 // a group's view of its blocks; a block destroyed while in a group detaches
@@ -1018,8 +1021,13 @@ int fest_vjp(fest_handle *h, const double *bar_residue, double *bar_qp, double *
   if (!h || !bar_residue || !bar_qp) return FEST_E_ARG;
   if (!h->have_grid || !h->have_state) return fail(h, FEST_E_ARG, "grid and state must be set first");
   Blk &b = h->b;
-  const std::string why = bar_nodes ? std::string("the node-coordinate adjoint is not built (bar_nodes must be NULL)")
-                                    : vjp_reason(b);
+  std::string why = vjp_reason(b);
+  if (why.empty() && bar_nodes) {
+    // the pole rewrites its faces' metrics, WENO-NM reads cell volumes
+    for (int f = 0; f < 6; ++f)
+      if (b.bc[f] == -7) why = "the node-coordinate adjoint does not cover the pole (-7)";
+    if (b.recon == FEST_RECON_WENO_NM) why = "the node-coordinate adjoint does not cover WENO-NM";
+  }
   if (!why.empty()) return fail(h, FEST_E_CONFIG, "fest_vjp: %s", why.c_str());
   hipStream_t st = h->stream;
   int rc;
@@ -1029,15 +1037,33 @@ int fest_vjp(fest_handle *h, const double *bar_residue, double *bar_qp, double *
       if ((rc = dalloc(h, &h->vjp_c[d], vjp_contrib_len(b, d)))) return rc;
     if ((rc = dalloc(h, &h->vjp_bar, 5 * b.nc))) return rc;
   }
+  const bool slip3 = b.bc[3] == -6, slip5 = b.bc[5] == -6 && b.kmx != 2;
+  const long nn = (long)(b.imx + 6) * (b.jmx + 6) * (b.kmx + 6);
+  VjpNodes nodes;
+  if (bar_nodes) {
+    if (!h->vjp_nodes) {
+      for (int d = 0; d < 3; ++d)
+        if ((rc = dalloc(h, &h->vjp_m[d], 4 * b.nfl[d]))) return rc;
+      if ((rc = dalloc(h, &h->vjp_g3, 6L * (b.imx - 1) * (b.kmx - 1)))) return rc;
+      if ((rc = dalloc(h, &h->vjp_g5, 6L * (b.imx - 1) * (b.jmx - 1)))) return rc;
+      if ((rc = dalloc(h, &h->vjp_nodes, 3 * nn))) return rc;
+    }
+    for (int d = 0; d < 3; ++d) nodes.mbar[d] = h->vjp_m[d];
+    nodes.g3 = slip3 ? h->vjp_g3 : nullptr;
+    nodes.g5 = slip5 ? h->vjp_g5 : nullptr;
+    nodes.bar_nodes = h->vjp_nodes;
+  }
   HIPCHK(h, hipMemcpyAsync(h->vjp_barR, bar_residue, sizeof(double) * 5 * b.nres, hipMemcpyHostToDevice, st));
   // the ghost cells as fest_residual at its first iteration (fixed faces refixed)
   if ((rc = launch_ghosts(h, 1))) return rc;
   {
     KgScope ts(h, KG_VJP);
-    launch_vjp(b, st, h->vjp_barR, h->vjp_c, h->vjp_bar);
+    launch_vjp(b, st, h->vjp_barR, h->vjp_c, h->vjp_bar, nodes);
   }
   LAUNCHCHK(h);
   HIPCHK(h, hipMemcpyAsync(bar_qp, h->vjp_bar, sizeof(double) * 5 * b.nc, hipMemcpyDeviceToHost, st));
+  if (bar_nodes)
+    HIPCHK(h, hipMemcpyAsync(bar_nodes, h->vjp_nodes, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, st));
   return check_err(h);
 }
 
@@ -1191,9 +1217,13 @@ int fest_device_ptr(fest_handle *h, const char *name, void **ptr, size_t *count)
   struct { const char *n; void *p; long c; } t[] = {
       {"qp", b.qp, b.nc * b.nv}, {"temp", b.temp, b.nc}, {"residue", b.res, b.nres * b.nv},
       {"F", b.flux[0], b.nfl[0] * b.nv}, {"G", b.flux[1], b.nfl[1] * b.nv}, {"H", b.flux[2], b.nfl[2] * b.nv},
-      {"dt", b.dt, b.nres}};
+      {"dt", b.dt, b.nres},
+      // the metric adjoints of the last fest_vjp with bar_nodes: bar (nx, ny, nz, A) per face, 4 planes
+      {"vjp_metrics_i", h->vjp_m[0], 4 * b.nfl[0]}, {"vjp_metrics_j", h->vjp_m[1], 4 * b.nfl[1]},
+      {"vjp_metrics_k", h->vjp_m[2], 4 * b.nfl[2]}};
   for (auto &e : t)
     if (!strcmp(e.n, name)) {
+      if (!e.p) return fail(h, FEST_E_ARG, "array '%s' is not allocated", name);
       *ptr = e.p;
       if (count) *count = (size_t)e.c;  // floats for F/G/H in FP32 mode
       return 0;

@@ -147,21 +147,21 @@ __device__ __forceinline__ void omega_wall_col(const Blk &b, int face, int a, in
 // or, when total, the total-pressure inlet (bc_primitive.f90:1237-1776):
 // the same Riemann-invariant velocity, then the pressure from the total
 // pressure tp and the boundary Mach number and the density from the
-// boundary speed of sound.  c is the first interior cell's rho, u, v, w, p;
-// returns Unb and, when full, the ghost layer 0 state g.  Written for a type
-// T: double, or the dual numbers of the adjoint (k_dual.h)
+// boundary speed of sound.  c is the first interior cell's rho, u, v, w, p
+// and nrm the boundary face's stored unit normal (bnormal); returns Unb and,
+// when full, the ghost layer 0 state g.  Written for a type T: double, or
+// the dual numbers of the adjoint (k_dual.h)
 template <typename T>
-__device__ __forceinline__ T riemann_state(const Blk &b, int face, int a, int bb, const T (&c)[5], bool full,
+__device__ __forceinline__ T riemann_state(const Blk &b, int face, const T (&c)[5], const T (&nrm)[3], bool full,
                                            bool total, double tp, T (&g)[5]) {
   const double gm = b.gm;
-  double nx, ny, nz;
-  bnormal(b, face, a, bb, nx, ny, nz);
+  T nx = nrm[0], ny = nrm[1], nz = nrm[2];
   if ((face & 1) == 0) { nx = -nx; ny = -ny; nz = -nz; }
   T cexp = sqrt(gm * c[4] / c[0]);
   double cinf = sqrt(gm * b.p_inf / b.rho_inf);
   T Unexp = c[1] * nx + c[2] * ny + c[3] * nz;
-  double Uninf = b.u_inf * nx + b.v_inf * ny + b.w_inf * nz;
-  double Rinf = Uninf - 2 * cinf / (gm - 1.);
+  T Uninf = b.u_inf * nx + b.v_inf * ny + b.w_inf * nz;
+  T Rinf = Uninf - 2 * cinf / (gm - 1.);
   T Rexp = Unexp + 2 * cexp / (gm - 1.);
   T Unb = 0.5 * (Rexp + Rinf);
   T Cb = 0.25 * (gm - 1.) * (Rexp - Rinf);
@@ -198,10 +198,11 @@ __device__ __forceinline__ T riemann_state(const Blk &b, int face, int a, int bb
 // of rho, u, v, w, p
 __device__ __forceinline__ double riemann_col(const Blk &b, int face, int a, int bb, int write, bool total, double tp) {
   const long c1 = fcell(b, face, 1, a, bb), cg = fcell(b, face, 0, a, bb);
-  double c[5], g[5];
+  double c[5], g[5], nrm[3];
 #pragma unroll
   for (int l = 0; l < 5; ++l) c[l] = b.qp[(long)l * b.nc + c1];
-  const double Unb = riemann_state(b, face, a, bb, c, write != 0, total, tp, g);
+  bnormal(b, face, a, bb, nrm[0], nrm[1], nrm[2]);
+  const double Unb = riemann_state(b, face, c, nrm, write != 0, total, tp, g);
   if (write) {
 #pragma unroll
     for (int l = 0; l < 5; ++l) b.qp[(long)l * b.nc + cg] = g[l];

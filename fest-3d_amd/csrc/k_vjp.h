@@ -48,13 +48,16 @@ __device__ __forceinline__ VjpFace vjp_face(const Blk &b, long t) {
 }
 
 // one thread per face of direction D: fbar[l][face] = bar qL_l and
-// fbar[5 + l][face] = bar qR_l, the flux's adjoint on the face states.  The
-// ten tangents (qL then qR) go through the flux KT at a time: fewer
-// tangents per pass keep fewer values live (the FP64 flux arithmetic is
-// register-bound) at the price of more passes
+// fbar[5 + l][face] = bar qR_l, the flux's adjoint on the face states, and,
+// when mbar is given, mbar[c][face] = the adjoint of the face's stored unit
+// normal (c = 0..2) and area (c = 3).  The tangents (qL, qR, then the
+// metrics) go through the flux KT at a time: fewer tangents per pass keep
+// fewer values live (the FP64 flux arithmetic is register-bound) at the
+// price of more passes
 constexpr int VJP_FLUX_TANGENTS = 5;
 template <int D, int RECON>
-__global__ void __launch_bounds__(256) k_vjp_flux(Blk b, const double *__restrict__ barR, double *fbar) {
+__global__ void __launch_bounds__(256) k_vjp_flux(Blk b, const double *__restrict__ barR, double *fbar,
+                                                  double *mbar) {
   constexpr int KT = VJP_FLUX_TANGENTS;
   using DK = Dual<KT>;
   const long nf = b.nfl[D];
@@ -74,7 +77,8 @@ __global__ void __launch_bounds__(256) k_vjp_flux(Blk b, const double *__restric
   }
   // the k faces of a 2-D grid carry no convective flux
   if (D == 2 && b.kmx == 2) any = false;
-  double bq[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // bar qL, then bar qR
+  const int ntan = mbar ? 14 : 10;
+  double bq[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // bar qL, bar qR, bar (n, A)
   if (any) {
     double q[10];  // qL, then qR
     typename FaceMath<double>::Aux ax;
@@ -85,34 +89,43 @@ __global__ void __launch_bounds__(256) k_vjp_flux(Blk b, const double *__restric
       FaceMath<double>::template face_states<RECON>(b, D, f.n, f.N, l, b.qp + (long)l * b.nc + f.c, f.s, ax, q[l],
                                                     q[5 + l]);
     const long fo = fi(b, D, f.i, f.j, f.k);
-    const DK nx = b.fx[D][fo], ny = b.fy[D][fo], nz = b.fz[D][fo], A = b.fA[D][fo];
+    const double m0[4] = {b.fx[D][fo], b.fy[D][fo], b.fz[D][fo], b.fA[D][fo]};
     const DK zero = (f.n == 1) ? (double)b.mz[D][0] : (f.n == f.N) ? (double)b.mz[D][1] : 1.0;
 #pragma unroll
-    for (int p0 = 0; p0 < 10; p0 += KT) {
-      DK L[5], R[5], fl[5];
+    for (int p0 = 0; p0 < 14; p0 += KT) {
+      if (p0 < ntan) {
+        DK L[5], R[5], fl[5], m[4];
 #pragma unroll
-      for (int l = 0; l < 5; ++l) {
-        L[l] = DK(q[l]);
-        R[l] = DK(q[5 + l]);
-      }
+        for (int l = 0; l < 5; ++l) {
+          L[l] = DK(q[l]);
+          R[l] = DK(q[5 + l]);
+        }
 #pragma unroll
-      for (int e = 0; e < KT; ++e) {
-        const int x = p0 + e;  // tangent e of this pass: input x
-        if (x < 5) L[x].d[e] = 1.0;
-        else if (x < 10) R[x - 5].d[e] = 1.0;
-      }
-      FaceMath<DK>::template inviscid_flux<5>(b, L, R, nx, ny, nz, A, zero, fl);
+        for (int c = 0; c < 4; ++c) m[c] = DK(m0[c]);
 #pragma unroll
-      for (int e = 0; e < KT; ++e) {
-        double acc = 0.0;
+        for (int e = 0; e < KT; ++e) {
+          const int x = p0 + e;  // tangent e of this pass: input x
+          if (x < 5) L[x].d[e] = 1.0;
+          else if (x < 10) R[x - 5].d[e] = 1.0;
+          else if (x < 14) m[x - 10].d[e] = 1.0;
+        }
+        FaceMath<DK>::template inviscid_flux<5>(b, L, R, m[0], m[1], m[2], m[3], zero, fl);
 #pragma unroll
-        for (int l = 0; l < 5; ++l) acc += fl[l].d[e] * barF[l];
-        if (p0 + e < 10) bq[p0 + e] = acc;
+        for (int e = 0; e < KT; ++e) {
+          double acc = 0.0;
+#pragma unroll
+          for (int l = 0; l < 5; ++l) acc += fl[l].d[e] * barF[l];
+          if (p0 + e < 14) bq[p0 + e] = acc;
+        }
       }
     }
   }
 #pragma unroll
   for (int l = 0; l < 10; ++l) fbar[(long)l * nf + t] = bq[l];
+  if (mbar) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) mbar[(long)c * nf + t] = bq[10 + c];
+  }
 }
 
 // one thread per face of direction D and variable l (l-major):
@@ -179,6 +192,30 @@ __global__ void __launch_bounds__(256) k_vjp_gather(Blk b, const double *__restr
   for (int v = 0; v < 5; ++v) bar[(long)v * b.nc + t] = acc[v];
 }
 
+// the boundary face of `face` at (a, bb) (bnormal's face) in the compact
+// face order of its direction, and the I face whose normal the slip wall
+// reflects with (flow_tangency_col): compact I-face index, or -1 for the j
+// and k max faces, whose I face lies in the first ghost row
+__device__ __forceinline__ long vjp_bface(const Blk &b, int face, int a, int bb) {
+  switch (face) {
+    case 0: return fli(b, 0, 1, a, bb);
+    case 1: return fli(b, 0, b.imx, a, bb);
+    case 2: return fli(b, 1, a, 1, bb);
+    case 3: return fli(b, 1, a, b.jmx, bb);
+    case 4: return fli(b, 2, a, bb, 1);
+    default: return fli(b, 2, a, bb, b.kmx);
+  }
+}
+__device__ __forceinline__ long vjp_rface(const Blk &b, int face, int a, int bb) {
+  switch (face) {
+    case 0: return fli(b, 0, 1, a, bb);
+    case 1: return fli(b, 0, b.imx, a, bb);
+    case 2: return fli(b, 0, a, 1, bb);
+    case 4: return fli(b, 0, a, bb, 1);
+    default: return -1;
+  }
+}
+
 // the transpose of one face's ghost-cell rule on the adjoint: the ghost
 // cells of the face interior hand their adjoints to the cells their values
 // were read from, then hold none.  Supported rules (fest_vjp checks): -9
@@ -186,8 +223,12 @@ __global__ void __launch_bounds__(256) k_vjp_gather(Blk b, const double *__restr
 // in [0, 1]), -6 slip wall, -1..-4 inlets and outlets, -7 pole, -8 far
 // field and -11 total-pressure inlet (their Riemann state differentiated in
 // dual numbers); interfaces (>= 0, -10) keep their ghost adjoints, which
-// belong to the neighbouring block.
-__global__ void __launch_bounds__(256) k_vjp_bc(Blk b, int face, double *bar) {
+// belong to the neighbouring block.  With mb0..mb2 (the metric adjoints of
+// the three directions, k_vjp_flux), the rules that read face normals (-6,
+// -8, -11) add their normals' adjoints there; the slip wall's ghost-row I
+// face of the j and k max faces goes to gface[c][(a-1) + (na-1)(bb-1)].
+__global__ void __launch_bounds__(256) k_vjp_bc(Blk b, int face, double *bar, double *mb0, double *mb1, double *mb2,
+                                                double *gface) {
   const int a = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1;
   const int bb = (int)blockIdx.y + 1;
   const int na = face_na(b, face), nb = face_nb(b, face);
@@ -196,6 +237,9 @@ __global__ void __launch_bounds__(256) k_vjp_bc(Blk b, int face, double *bar) {
   const int id = p.id;
   if (id >= 0 || id == -10) return;
   const long nc = b.nc;
+  const int d = face >> 1;
+  double *mb = d == 0 ? mb0 : d == 1 ? mb1 : mb2;  // the boundary face's direction
+  const long nfd = d == 0 ? b.nfl[0] : d == 1 ? b.nfl[1] : b.nfl[2];
   auto G = [&](int v, int L) -> double & { return bar[(long)v * nc + fcell(b, face, L, a, bb)]; };
   double gb[3][5];
 #pragma unroll
@@ -207,7 +251,7 @@ __global__ void __launch_bounds__(256) k_vjp_bc(Blk b, int face, double *bar) {
     }
   switch (id) {
     case -9: {  // ghost -g copies the cell across the block (k_bc_face_b)
-      const int d = face >> 1, hi = face & 1;
+      const int hi = face & 1;
       const int Nd = d == 0 ? b.imx : d == 1 ? b.jmx : b.kmx;
       for (int g = 0; g < 3; ++g) {
         const int L = hi ? 1 + g : Nd - 1 - g;  // along d
@@ -222,6 +266,7 @@ __global__ void __launch_bounds__(256) k_vjp_bc(Blk b, int face, double *bar) {
       // ghost -g = (c2 q(g+1) - c3 q(g+2)) / c1 (copy_bc.f90's symmetric copy)
       const double cc2 = b.c2 / b.c1, cc3 = b.c3 / b.c1;
       double nx = 0, ny = 0, nz = 0, rx = 0, ry = 0, rz = 0;
+      long rf = -1;
       if (id == -6) {
         // u_g = u_c - 2 (n . u_c) r, n the boundary normal and r the
         // I-face normal at the same index (flow_tangency_col)
@@ -236,7 +281,9 @@ __global__ void __launch_bounds__(256) k_vjp_bc(Blk b, int face, double *bar) {
           default: f = fi(b, 0, a, bb, b.kmx); break;
         }
         rx = b.fx[0][f]; ry = b.fy[0][f]; rz = b.fz[0][f];
+        rf = vjp_rface(b, face, a, bb);
       }
+      double bn[3] = {0, 0, 0}, br[3] = {0, 0, 0};
       for (int g = 0; g < 3; ++g) {
         const double *w = gb[g];
         G(0, g + 1) += cc2 * w[0];
@@ -248,10 +295,28 @@ __global__ void __launch_bounds__(256) k_vjp_bc(Blk b, int face, double *bar) {
           G(2, g + 1) -= w[2];
           G(3, g + 1) -= w[3];
         } else {
+          const long cc = fcell(b, face, g + 1, a, bb);
+          const double ux = b.qp[nc + cc], uy = b.qp[2 * nc + cc], uz = b.qp[3 * nc + cc];
           const double rg = rx * w[1] + ry * w[2] + rz * w[3];
+          const double nu = nx * ux + ny * uy + nz * uz;
           G(1, g + 1) += w[1] - 2.0 * nx * rg;
           G(2, g + 1) += w[2] - 2.0 * ny * rg;
           G(3, g + 1) += w[3] - 2.0 * nz * rg;
+          bn[0] -= 2.0 * rg * ux; bn[1] -= 2.0 * rg * uy; bn[2] -= 2.0 * rg * uz;
+          br[0] -= 2.0 * nu * w[1]; br[1] -= 2.0 * nu * w[2]; br[2] -= 2.0 * nu * w[3];
+        }
+      }
+      if (id == -6 && mb) {
+        const long bf = vjp_bface(b, face, a, bb);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) mb[(long)c * nfd + bf] += bn[c];
+        if (rf >= 0) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) mb0[(long)c * b.nfl[0] + rf] += br[c];
+        } else if (gface) {
+          const long ng = (long)(na - 1) * (nb - 1), gi = (a - 1) + (long)(na - 1) * (bb - 1);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) gface[(long)c * ng + gi] = br[c];
         }
       }
       break;
@@ -259,21 +324,32 @@ __global__ void __launch_bounds__(256) k_vjp_bc(Blk b, int face, double *bar) {
     case -8:      // far field and
     case -11: {   // total-pressure inlet: layer 0 from the Riemann state of
                   // the first interior cell, layers -1 and -2 its copies
-      using DT = Dual<5>;
-      DT c[5], g[5];
+      using DT = Dual<8>;
+      DT c[5], nrm[3], g[5];
       const long c1 = fcell(b, face, 1, a, bb);
 #pragma unroll
       for (int l = 0; l < 5; ++l) {
         c[l] = DT(b.qp[(long)l * nc + c1]);
         c[l].d[l] = 1.0;
       }
-      riemann_state(b, face, a, bb, c, true, id == -11, p.fix[FX_TP], g);
+      double n3[3];
+      bnormal(b, face, a, bb, n3[0], n3[1], n3[2]);
 #pragma unroll
-      for (int v = 0; v < 5; ++v) {
+      for (int x = 0; x < 3; ++x) {
+        nrm[x] = DT(n3[x]);
+        nrm[x].d[5 + x] = 1.0;
+      }
+      riemann_state(b, face, c, nrm, true, id == -11, p.fix[FX_TP], g);
+      double gs[5];
+#pragma unroll
+      for (int l = 0; l < 5; ++l) gs[l] = gb[0][l] + gb[1][l] + gb[2][l];
+#pragma unroll
+      for (int v = 0; v < 8; ++v) {
         double acc = 0.0;
 #pragma unroll
-        for (int l = 0; l < 5; ++l) acc += g[l].d[v] * (gb[0][l] + gb[1][l] + gb[2][l]);
-        G(v, 1) += acc;
+        for (int l = 0; l < 5; ++l) acc += g[l].d[v] * gs[l];
+        if (v < 5) G(v, 1) += acc;
+        else if (mb) mb[(long)(v - 5) * nfd + vjp_bface(b, face, a, bb)] += acc;
       }
       break;
     }
@@ -294,6 +370,159 @@ __global__ void __launch_bounds__(256) k_vjp_bc(Blk b, int face, double *bar) {
     default:  // -1: fixed values
       break;
   }
+}
+
+// ---- the node coordinates: the face metrics come from the nodes as
+// S = (d1 x d2)/2, A = |S|, n = S/A (k_face_metrics), with d1 = P1 - P0 and
+// d2 = P2 - P3 of the face's four nodes.  From the adjoints of the stored
+// n and A, bar S = bar_A n + (bar_n - n (n . bar_n))/A, then
+// bar d1 = (d2 x bar S)/2 and bar d2 = (bar S x d1)/2.
+
+// the face's four nodes (k_face_metrics): p[0..3] = P0, P1, P2, P3
+__device__ __forceinline__ void vjp_face_nodes(const Blk &b, int d, int i, int j, int k, long (&p)[4]) {
+  p[0] = ndi(b, i, j, k);
+  if (d == 0) { p[1] = ndi(b, i, j + 1, k + 1); p[2] = ndi(b, i, j, k + 1); p[3] = ndi(b, i, j + 1, k); }
+  else if (d == 1) { p[1] = ndi(b, i + 1, j, k + 1); p[2] = ndi(b, i + 1, j, k); p[3] = ndi(b, i, j, k + 1); }
+  else { p[1] = ndi(b, i + 1, j + 1, k); p[2] = ndi(b, i, j + 1, k); p[3] = ndi(b, i + 1, j, k); }
+}
+// bar d1, bar d2 of one face from the adjoints of its unit normal bn and area bA
+__device__ __forceinline__ void vjp_face_diagonals(const Blk &b, int d, int i, int j, int k, const double (&bn)[3],
+                                                   double bA, double (&out)[6]) {
+  const long nn = (long)(b.imx + 6) * (b.jmx + 6) * (b.kmx + 6);
+  const double *X = b.nodes, *Y = X + nn, *Z = Y + nn;
+  long p[4];
+  vjp_face_nodes(b, d, i, j, k, p);
+  const double d1[3] = {X[p[1]] - X[p[0]], Y[p[1]] - Y[p[0]], Z[p[1]] - Z[p[0]]};
+  const double d2[3] = {X[p[2]] - X[p[3]], Y[p[2]] - Y[p[3]], Z[p[2]] - Z[p[3]]};
+  const double S[3] = {0.5 * (d1[1] * d2[2] - d1[2] * d2[1]), 0.5 * (d1[2] * d2[0] - d1[0] * d2[2]),
+                       0.5 * (d1[0] * d2[1] - d1[1] * d2[0])};
+  const double A = sqrt(S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
+  double bS[3] = {0, 0, 0};
+  if (A != 0.0) {
+    const double n[3] = {S[0] / A, S[1] / A, S[2] / A};
+    const double nb = n[0] * bn[0] + n[1] * bn[1] + n[2] * bn[2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) bS[c] = bA * n[c] + (bn[c] - n[c] * nb) / A;
+  }
+  out[0] = 0.5 * (d2[1] * bS[2] - d2[2] * bS[1]);
+  out[1] = 0.5 * (d2[2] * bS[0] - d2[0] * bS[2]);
+  out[2] = 0.5 * (d2[0] * bS[1] - d2[1] * bS[0]);
+  out[3] = 0.5 * (bS[1] * d1[2] - bS[2] * d1[1]);
+  out[4] = 0.5 * (bS[2] * d1[0] - bS[0] * d1[2]);
+  out[5] = 0.5 * (bS[0] * d1[1] - bS[1] * d1[0]);
+}
+
+// one thread per face of direction D: dbar[c][face] = bar d1 (c = 0..2),
+// bar d2 (c = 3..5) from mbar
+template <int D>
+__global__ void __launch_bounds__(256) k_vjp_metrics(Blk b, const double *__restrict__ mbar, double *dbar) {
+  const long nf = b.nfl[D];
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nf) return;
+  const VjpFace f = vjp_face<D>(b, t);
+  const double bn[3] = {mbar[t], mbar[nf + t], mbar[2 * nf + t]};
+  double out[6];
+  vjp_face_diagonals(b, D, f.i, f.j, f.k, bn, mbar[3 * nf + t], out);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) dbar[(long)c * nf + t] = out[c];
+}
+
+// the ghost-row I faces of a slip wall on the j max (face 3) or k max
+// (face 5) face: gface holds bar n (3 planes) on entry, bar d1 and bar d2
+// (6 planes) on exit; one thread per face-interior column
+__global__ void __launch_bounds__(256) k_vjp_metrics_ghost(Blk b, int face, double *gface) {
+  const int a = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1;
+  const int bb = (int)blockIdx.y + 1;
+  const int na = face_na(b, face), nb = face_nb(b, face);
+  if (a > na - 1 || bb > nb - 1) return;
+  const long ng = (long)(na - 1) * (nb - 1), gi = (a - 1) + (long)(na - 1) * (bb - 1);
+  const double bn[3] = {gface[gi], gface[ng + gi], gface[2 * ng + gi]};
+  double out[6];
+  if (face == 3) vjp_face_diagonals(b, 0, a, b.jmx, bb, bn, 0.0, out);
+  else vjp_face_diagonals(b, 0, a, bb, b.kmx, bn, 0.0, out);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) gface[(long)c * ng + gi] = out[c];
+}
+
+// one thread per padded node: bar of the real nodes (1..imx, 1..jmx,
+// 1..kmx) as the sum over the faces that hold the node (P0: -bar d1, P1:
+// +bar d1, P2: +bar d2, P3: -bar d2), in a fixed order; ghost nodes get 0.
+// g3, g5 (or null): the ghost-row I faces of slip walls on faces 3 and 5,
+// whose ghost nodes are the linear extrapolations 2 X(N) - X(N-1)
+// (k_ghost_nodes)
+__global__ void __launch_bounds__(256) k_vjp_nodes(Blk b, const double *__restrict__ d0, const double *__restrict__ d1,
+                                                   const double *__restrict__ d2, const double *__restrict__ g3,
+                                                   const double *__restrict__ g5, double *bar_nodes) {
+  const long nn = (long)(b.imx + 6) * (b.jmx + 6) * (b.kmx + 6);
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nn) return;
+  const int I = (int)(t % (b.imx + 6)) - 2, J = (int)((t / (b.imx + 6)) % (b.jmx + 6)) - 2,
+            K = (int)(t / ((long)(b.imx + 6) * (b.jmx + 6))) - 2;
+  double acc[3] = {0, 0, 0};
+  if (I >= 1 && I <= b.imx && J >= 1 && J <= b.jmx && K >= 1 && K <= b.kmx) {
+    const double *db[3] = {d0, d1, d2};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const long nf = b.nfl[d];
+      const int ie = b.imx - 1 + (d == 0), je = b.jmx - 1 + (d == 1), ke = b.kmx - 1 + (d == 2);
+      // (face offset, node role): the faces of direction d holding the node
+      int oi[4], oj[4], ok[4];
+      oi[0] = 0; oj[0] = 0; ok[0] = 0;  // P0
+      if (d == 0) { oi[1] = 0; oj[1] = -1; ok[1] = -1; oi[2] = 0; oj[2] = 0; ok[2] = -1; oi[3] = 0; oj[3] = -1; ok[3] = 0; }
+      else if (d == 1) { oi[1] = -1; oj[1] = 0; ok[1] = -1; oi[2] = -1; oj[2] = 0; ok[2] = 0; oi[3] = 0; oj[3] = 0; ok[3] = -1; }
+      else { oi[1] = -1; oj[1] = -1; ok[1] = 0; oi[2] = 0; oj[2] = -1; ok[2] = 0; oi[3] = -1; oj[3] = 0; ok[3] = 0; }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int fi_ = I + oi[r], fj = J + oj[r], fk = K + ok[r];
+        if (fi_ < 1 || fi_ > ie || fj < 1 || fj > je || fk < 1 || fk > ke) continue;
+        const long f = fli(b, d, fi_, fj, fk);
+        const int c0 = (r == 0 || r == 1) ? 0 : 3;
+        const double sg = (r == 1 || r == 2) ? 1.0 : -1.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[c] += sg * db[d][(long)(c0 + c) * nf + f];
+      }
+    }
+    // slip wall on j max: I faces (a, jmx, bb), a = 1..imx-1, bb = 1..kmx-1,
+    // nodes P0 (a,jmx,bb), P1 (a,jmx+1,bb+1), P2 (a,jmx,bb+1), P3 (a,jmx+1,bb)
+    if (g3 && (J == b.jmx || J == b.jmx - 1)) {
+      const int na = b.imx, nb = b.kmx;
+      const long ng = (long)(na - 1) * (nb - 1);
+      const double w = J == b.jmx ? 2.0 : -1.0;  // weight of a ghost node's row
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {  // faces bb = K (s 0) and bb = K - 1 (s 1)
+        const int a = I, bb = K - s;
+        if (a < 1 || a > na - 1 || bb < 1 || bb > nb - 1) continue;
+        const long gi = (a - 1) + (long)(na - 1) * (bb - 1);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const double bd1 = g3[(long)c * ng + gi], bd2 = g3[(long)(3 + c) * ng + gi];
+          if (s == 0) acc[c] += (J == b.jmx ? -bd1 : 0.0) + w * (-bd2);  // P0, P3 (ghost)
+          else acc[c] += (J == b.jmx ? bd2 : 0.0) + w * bd1;              // P2, P1 (ghost)
+        }
+      }
+    }
+    // slip wall on k max: I faces (a, bb, kmx), a = 1..imx-1, bb = 1..jmx-1,
+    // nodes P0 (a,bb,kmx), P1 (a,bb+1,kmx+1), P2 (a,bb,kmx+1), P3 (a,bb+1,kmx)
+    if (g5 && (K == b.kmx || K == b.kmx - 1)) {
+      const int na = b.imx, nb = b.jmx;
+      const long ng = (long)(na - 1) * (nb - 1);
+      const double w = K == b.kmx ? 2.0 : -1.0;
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {  // faces bb = J (s 0) and bb = J - 1 (s 1)
+        const int a = I, bb = J - s;
+        if (a < 1 || a > na - 1 || bb < 1 || bb > nb - 1) continue;
+        const long gi = (a - 1) + (long)(na - 1) * (bb - 1);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const double bd1 = g5[(long)c * ng + gi], bd2 = g5[(long)(3 + c) * ng + gi];
+          if (s == 0) acc[c] += (K == b.kmx ? -bd1 : 0.0) + w * bd2;  // P0, P2 (ghost)
+          else acc[c] += (K == b.kmx ? -bd2 : 0.0) + w * bd1;          // P3, P1 (ghost)
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) bar_nodes[(long)c * nn + t] = acc[c];
 }
 
 }  // namespace fest

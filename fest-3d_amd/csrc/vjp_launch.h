@@ -11,12 +11,24 @@ namespace fest {
 // each face, then its 10 face-state adjoints
 long vjp_contrib_len(const Blk &b, int d);
 
+// the node-coordinate part of the adjoint (all null when it is not wanted):
+// the metric adjoints of each direction (4 planes of nfl[d]), the ghost-row
+// I faces of slip walls on the j and k max faces (6 planes of the face
+// interior each, null when that face is no slip wall) and the result
+// (3 planes of the padded node array)
+struct VjpNodes {
+  double *mbar[3] = {nullptr, nullptr, nullptr};
+  double *g3 = nullptr, *g5 = nullptr;
+  double *bar_nodes = nullptr;
+};
+
 // bar (5 planes of the padded cell array) = d(barR . R)/dq on the resident
 // state, whose ghost cells the block's own boundary rules filled: the face
 // Jacobians contracted with barR (two launches per direction), the per-cell
-// gather, then
-// the transposes of the ghost rules, face by face in reverse order.
+// gather, then the transposes of the ghost rules, face by face in reverse
+// order; with nodes.bar_nodes, also d(barR . R)/d(node coordinates).
 // contrib[d] holds vjp_contrib_len(b, d) doubles.
-void launch_vjp(const Blk &b, hipStream_t st, const double *barR, double *const contrib[3], double *bar);
+void launch_vjp(const Blk &b, hipStream_t st, const double *barR, double *const contrib[3], double *bar,
+                const VjpNodes &nodes);
 
 }  // namespace fest

@@ -211,17 +211,20 @@ class GpuBlock:
     def residual(self, current_iter: int = 1):
         self._check(lib().fest_residual(self.h, int(current_iter)))
 
-    def vjp(self, bar_residue: np.ndarray) -> np.ndarray:
+    def vjp(self, bar_residue: np.ndarray, nodes: bool = False):
         """fest_vjp: (dR/dqp)^T bar_residue for the inviscid residual R of
         the resident state.  bar_residue has the residue's shape
         (5, kmx-1, jmx-1, imx-1); the result the state's (5, padded cells),
-        zero in ghost cells but those of interface faces."""
+        zero in ghost cells but those of interface faces.  With nodes, also
+        (dR/dnodes)^T bar_residue in the node array's shape (3, padded
+        nodes), zero at ghost nodes: returns (bar_qp, bar_nodes)."""
         c = self.case
         bar = np.ascontiguousarray(bar_residue, dtype=np.float64)
         assert bar.shape == self.shape_of("residue")
         out = np.zeros((5,) + c.cell_shape())
-        self._check(lib().fest_vjp(self.h, _ptr(bar), _ptr(out), None))
-        return out
+        bn = np.zeros(c.node_shape()) if nodes else None
+        self._check(lib().fest_vjp(self.h, _ptr(bar), _ptr(out), _ptr(bn)))
+        return (out, bn) if nodes else out
 
     def time_step(self, cfl: float = None):
         cfl = self.case.CFL if cfl is None else cfl

@@ -169,8 +169,12 @@ int fest_residual(fest_handle *h, int current_iter);
  * Inviscid FP64 handles with five variables, no pressure-based switch and
  * at least 4 cells in every 3-D direction, else FEST_E_CONFIG.  The result
  * is the exact derivative of the branches (limiters, switches) the
- * residual takes at this state.  bar_nodes, the adjoint with respect to
- * the grid nodes, is not built: it must be NULL (else FEST_E_CONFIG). */
+ * residual takes at this state.  bar_nodes (nullable): also the adjoint
+ * with respect to the grid nodes, (dR/dnodes)^T bar_residue in the layout
+ * of fest_set_grid's nodes (-2:imx+3,-2:jmx+3,-2:kmx+3,1:3), through the
+ * face metrics (src/geometry.f90:236-314) and the ghost-node extrapolation
+ * (src/grid.f90:137-236) to the real nodes; ghost nodes hold zero.  Not
+ * for the pole (-7) or WENO-NM (FEST_E_CONFIG). */
 int fest_vjp(fest_handle *h, const double *bar_residue, double *bar_qp, double *bar_nodes);
 
 /* compute_time_step (src/time.f90:291-334) from the face states of the
