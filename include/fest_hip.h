@@ -219,7 +219,10 @@ int fest_resnorm_device(fest_handle *h, double *res_abs_dev);
 /* Device pointers of the resident arrays for callers that keep the state
  * on the device.  name: "qp","temp","residue","F","G","H","dt"; count is
  * in doubles.  A handle created with FEST_PREC_FP32_FLUX holds F, G and H
- * as floats: their count is then in floats. */
+ * as floats: their count is then in floats.  After a fest_vjp call with
+ * bar_nodes, "vjp_metrics_i", "_j", "_k" are the adjoints of the stored
+ * face normals and areas (4 planes: nx, ny, nz, A, in the F/G/H face
+ * order); FEST_E_ARG before. */
 int fest_device_ptr(fest_handle *h, const char *name, void **ptr, size_t *count);
 
 /* Kernel timing (no counterpart in the reference; it is for benchmarks

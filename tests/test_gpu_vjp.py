@@ -33,6 +33,7 @@ variable; per direction: of the larger side).  The differences leave ~1e-8
 of rounding at h = 1e-7 (measured), far below the 1e-3..1 that a wrong
 sign, a dropped term or a missed ghost rule gives.
 """
+import ctypes
 import dataclasses
 import sys
 
@@ -272,6 +273,18 @@ def test_turbulent_and_fp32_handles_fail():
     assert e.value.code == -3
 
 
+def warped(nodes):
+    """The grid moved by a smooth warp of all three coordinates, so that no
+    face normal has an exact zero component (see node_columns)."""
+    nodes = nodes.copy()
+    inn = (slice(3, -3),) * 3
+    X, Y, Z = (nodes[c][inn].copy() for c in range(3))
+    nodes[0][inn] = X + 0.01 * np.sin(2 * np.pi * (Y + Z))
+    nodes[1][inn] = Y + 0.005 * np.sin(2 * np.pi * (X + Z)) * (1.0 - Y)
+    nodes[2][inn] = Z + 0.01 * np.sin(2 * np.pi * (X + Y))
+    return nodes
+
+
 def node_columns(case, nodes, q, seed, sample=None):
     """max over real node coordinates of |bar_nodes - FD| / max|bar_nodes|,
     FD by central differences of <w, R> under node displacements of 1e-7,
@@ -287,12 +300,7 @@ def node_columns(case, nodes, q, seed, sample=None):
     the kink, and a central difference there averages the two one-sided
     slopes while the adjoint takes the branch the residual took (measured:
     the difference is the same at h = 1e-7, 1e-8, 1e-9)."""
-    nodes = nodes.copy()
-    inn = (slice(3, -3),) * 3
-    X, Y, Z = (nodes[c][inn].copy() for c in range(3))
-    nodes[0][inn] = X + 0.01 * np.sin(2 * np.pi * (Y + Z))
-    nodes[1][inn] = Y + 0.005 * np.sin(2 * np.pi * (X + Z)) * (1.0 - Y)
-    nodes[2][inn] = Z + 0.01 * np.sin(2 * np.pi * (X + Y))
+    nodes = warped(nodes)
     g = GpuBlock(case)
     g.set_grid(nodes)
     g.upload_state(q)
@@ -342,6 +350,61 @@ def test_node_adjoint_columns(flux, recon, bcs, nk):
     case, nodes, q = C.euler_bump(8, 6, nk, flux=flux, recon=recon, bc_id=bcs, seed=13)
     worst = node_columns(case, nodes, q, 19)
     say("  worst", worst)
+    assert worst <= TOL
+
+
+def test_metric_adjoints_against_uploaded_geometry():
+    """The adjoints of the stored face normals and areas (fest_device_ptr
+    "vjp_metrics_*" after fest_vjp with bar_nodes) against central
+    differences of <w, R> with one metric entry moved (fest_upload_geometry):
+    every face of the first two layers next to each j face, 40 seeded faces
+    of each direction elsewhere, all four entries."""
+    case, nodes, q = C.euler_bump(8, 6, 5, flux="slau", recon="weno", bc_id=(-8, -8, -6, -6, -6, -6), seed=29)
+    nodes = warped(nodes)
+    g = GpuBlock(case)
+    g.set_grid(nodes)
+    g.upload_state(q)
+    w = weights(case, 29)
+    g.vjp(w, nodes=True)
+    L = native.lib()
+    L.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    L.hipMemcpy.restype = ctypes.c_int
+    mb = []
+    for name in ("vjp_metrics_i", "vjp_metrics_j", "vjp_metrics_k"):
+        p, n = g.device_ptr(name)
+        out = np.empty(n)
+        assert L.hipMemcpy(out.ctypes.data, p, out.nbytes, 2) == 0
+        mb.append(out.reshape(4, -1))
+    cells = g.download("cells")
+    faces = g.download("faces")
+
+    def R(fc):
+        g.upload_geometry(cells, *fc)
+        g.upload_state(q)
+        g.residual(1)
+        return g.download("residue")
+
+    ext = [(case.imx, case.jmx - 1, case.kmx - 1), (case.imx - 1, case.jmx, case.kmx - 1),
+           (case.imx - 1, case.jmx - 1, case.kmx)]
+    rng = np.random.default_rng(31)
+    worst = 0.0
+    for d in range(3):
+        ni, nj, nk = ext[d]
+        allf = [(i, j, k) for k in range(1, nk + 1) for j in range(1, nj + 1) for i in range(1, ni + 1)]
+        near = [f for f in allf if f[1] <= 2 or f[1] >= nj - 1]
+        far = [f for f in allf if f not in near]
+        pick = near + [far[x] for x in rng.choice(len(far), min(40, len(far)), replace=False)]
+        top = np.abs(mb[d]).max()
+        for i, j, k in pick:
+            f = (i - 1) + ni * ((j - 1) + nj * (k - 1))
+            for c, comp in enumerate((1, 2, 3, 0)):  # (nx, ny, nz, A) in the download's (A, nx, ny, nz)
+                fp = [x.copy() for x in faces.values()]
+                fm = [x.copy() for x in faces.values()]
+                fp[d][comp, k + 2, j + 2, i + 2] += H
+                fm[d][comp, k + 2, j + 2, i + 2] -= H
+                fd = float(np.sum(w * (R(fp) - R(fm)))) / (2 * H)
+                worst = max(worst, abs(mb[d][c, f] - fd) / top)
+    say("metric adjoints worst", worst)
     assert worst <= TOL
 
 

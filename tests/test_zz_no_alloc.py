@@ -3,7 +3,10 @@ ABI, its norms (written to a device buffer the caller owns, partial sums in
 the library's static per-device scratch) equal fest_resnorm's bit for bit,
 on two calls in a row, the second on a smaller grid (fewer partial sums than
 the first call left in the shared scratch).  The caller's buffer comes from
-the HIP runtime the library links (hipMalloc through ctypes), so the test
+the HIP runtime the library links: hipMalloc is looked up through the
+library's handle, whose dependency tree holds that runtime (a process that
+has imported torch also maps torch's bundled copy of the runtime under
+another file name, and a lookup by file name may find that one).  The test
 needs nothing but the library."""
 import ctypes
 import sys
@@ -20,7 +23,7 @@ H2D, D2H = 1, 2  # hipMemcpyHostToDevice, hipMemcpyDeviceToHost
 
 
 def hip():
-    rt = ctypes.CDLL("libamdhip64.so")
+    rt = lib()  # dlsym through libfest_hip.so reaches the runtime it links
     rt.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
     rt.hipFree.argtypes = [ctypes.c_void_p]
     rt.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
