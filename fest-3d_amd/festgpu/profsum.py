@@ -38,7 +38,7 @@ GROUPS = [  # (substring of the demangled kernel name, kernel group)
     ("k_lusgs_sweep", "lusgs_sweep"), ("k_lusgs_prep", "lusgs_diag"),
     ("k_to_skew", "lusgs_diag"), ("k_lusgs_apply", "lusgs_apply"), ("k_exchange", "exchange"),
     ("k_kkl_source", "kkl_source"), ("k_to_f32", "f32_copy"), ("k_g32_widen", "f32_copy"),
-    ("k_vjp_faces", "vjp"), ("k_vjp_gather", "vjp"), ("k_vjp_bc", "vjp"),
+    ("k_vjp_", "vjp"),
     # batched group steps: one launch per phase over all blocks
     ("km_flux<0", "flux_i"), ("km_flux<1", "flux_j"), ("km_flux<2", "flux_k"), ("km_grad<", "grad"),
     ("km_grad_bc", "grad_bc"), ("km_update", "update"), ("km_bc_face", "bc_face"), ("km_bc_edges", "bc_edges"),
