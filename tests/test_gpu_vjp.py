@@ -457,3 +457,36 @@ def test_unsteady_adjoint_with_checkpoints():
     assert worst <= TOL
     _, grad1 = adj.gradient(qi, N, N, dJ)
     assert np.array_equal(grad, grad1)
+
+
+@pytest.mark.parametrize("recon", ["muscl", "weno"])
+def test_adjoint_of_split_blocks_composes(recon):
+    """A block cut in two along i (interface faces between the halves,
+    festgpu.multiblock.split with consistent ghost layers): each half's
+    fest_vjp leaves on its interface ghost cells the adjoint owed to the
+    other half; adding every half's adjoint at the global cell indices (the
+    transpose of the interface copy) gives the uncut block's adjoint."""
+    from festgpu.multiblock import split
+    case, nodes, q = C.euler_bump(16, 8, 5, recon=recon, bc_id=(-3, -4, -6, -6, -9, -9), seed=37)
+    w = weights(case, 37)
+    g = GpuBlock(case)
+    g.set_grid(nodes)
+    g.upload_state(q)
+    whole = g.vjp(w)
+    blocks, _ = split(case, nodes, q, 0, [9])
+    total = np.zeros_like(whole)
+    s = 1
+    for cb, nb, qb in blocks:
+        gb = GpuBlock(cb)
+        gb.set_grid(nb)
+        gb.upload_state(qb)
+        n = cb.imx  # nodes s..s+n-1: cells s..s+n-2, padded local -2..n+2
+        wb = np.ascontiguousarray(w[..., s - 1: s - 1 + n - 1])
+        bar = gb.vjp(wb)
+        # local padded i index a <-> global padded index a + s - 1
+        total[..., s - 1: s - 1 + n + 5] += bar
+        s += n - 1
+    inner = (slice(None),) + (slice(3, -3),) * 3
+    d = np.abs(total[inner] - whole[inner]).max() / np.abs(whole[inner]).max()
+    say("split blocks", recon, d)
+    assert d <= 1e-13
