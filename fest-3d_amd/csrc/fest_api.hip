@@ -109,6 +109,7 @@ struct fest_group {
   std::vector<fest_handle *> blocks;
   std::vector<fest_link> links;
   LinkDev *d_links = nullptr;
+  LinkDev *d_vjp_links = nullptr;  // the link table on the blocks' cell adjoints (fest_group_vjp)
   int max_layer = 0;  // largest cells-per-layer of any link
   hipStream_t stream = nullptr;
   bool broken = false;
@@ -1017,6 +1018,17 @@ extern "C++" std::string vjp_reason(const Blk &b) {
   return "";
 }
 
+// fest_vjp's state buffers, allocated at the first call
+int vjp_buffers(fest_handle *h) {
+  const Blk &b = h->b;
+  int rc;
+  if (h->vjp_bar) return 0;
+  if ((rc = dalloc(h, &h->vjp_barR, 5 * b.nres))) return rc;
+  for (int d = 0; d < 3; ++d)
+    if ((rc = dalloc(h, &h->vjp_c[d], vjp_contrib_len(b, d)))) return rc;
+  return dalloc(h, &h->vjp_bar, 5 * b.nc);
+}
+
 int fest_vjp(fest_handle *h, const double *bar_residue, double *bar_qp, double *bar_nodes) {
   if (!h || !bar_residue || !bar_qp) return FEST_E_ARG;
   if (!h->have_grid || !h->have_state) return fail(h, FEST_E_ARG, "grid and state must be set first");
@@ -1031,12 +1043,7 @@ int fest_vjp(fest_handle *h, const double *bar_residue, double *bar_qp, double *
   if (!why.empty()) return fail(h, FEST_E_CONFIG, "fest_vjp: %s", why.c_str());
   hipStream_t st = h->stream;
   int rc;
-  if (!h->vjp_bar) {
-    if ((rc = dalloc(h, &h->vjp_barR, 5 * b.nres))) return rc;
-    for (int d = 0; d < 3; ++d)
-      if ((rc = dalloc(h, &h->vjp_c[d], vjp_contrib_len(b, d)))) return rc;
-    if ((rc = dalloc(h, &h->vjp_bar, 5 * b.nc))) return rc;
-  }
+  if ((rc = vjp_buffers(h))) return rc;
   const bool slip3 = b.bc[3] == -6, slip5 = b.bc[5] == -6 && b.kmx != 2;
   const long nn = (long)(b.imx + 6) * (b.jmx + 6) * (b.kmx + 6);
   VjpNodes nodes;
@@ -1673,6 +1680,7 @@ void fest_group_destroy(fest_group *g) {
     h->group = nullptr;
   }
   if (g->d_links) (void)hipFree(g->d_links);
+  if (g->d_vjp_links) (void)hipFree(g->d_vjp_links);
   if (g->stream) (void)hipStreamDestroy(g->stream);
   delete g;
 }
@@ -1685,6 +1693,63 @@ int fest_group_exchange(fest_group *g) {
   int rc = group_exchange(g);
   if (rc) return rc;
   if (hipStreamSynchronize(g->stream) != hipSuccess) return gfail(g, FEST_E_HIP, "exchange failed");
+  return 0;
+}
+
+int fest_group_vjp(fest_group *g, const double *const *bar_residues, double *const *bar_qps) {
+  if (!g || !bar_residues || !bar_qps) return FEST_E_ARG;
+  if (g->broken) return FEST_E_ARG;
+  const int nb = (int)g->blocks.size();
+  for (int n = 0; n < nb; ++n) {
+    fest_handle *h = g->blocks[n];
+    if (!bar_residues[n] || !bar_qps[n]) return gfail(g, FEST_E_ARG, "block %d: null adjoint array", n);
+    if (!h->have_grid || !h->have_state) return gfail(g, FEST_E_ARG, "block %d: grid and state must be set first", n);
+    const std::string why = vjp_reason(h->b);
+    if (!why.empty()) return gfail(g, FEST_E_CONFIG, "block %d: fest_vjp: %s", n, why.c_str());
+  }
+  int rc;
+  // the interface ghost cells as the group's residual sees them
+  if ((rc = group_exchange(g))) return rc;
+  for (int n = 0; n < nb; ++n) {
+    fest_handle *h = g->blocks[n];
+    if ((rc = vjp_buffers(h))) return gfail(g, rc, "block %d: %s", n, h->err.c_str());
+    if (hipMemcpyAsync(h->vjp_barR, bar_residues[n], sizeof(double) * 5 * h->b.nres, hipMemcpyHostToDevice,
+                       g->stream) != hipSuccess)
+      return gfail(g, FEST_E_HIP, "block %d: adjoint upload failed", n);
+    if ((rc = launch_ghosts(h, 1))) return gfail(g, rc, "block %d: %s", n, h->err.c_str());
+    KgScope ts(h, KG_VJP);
+    launch_vjp(h->b, g->stream, h->vjp_barR, h->vjp_c, h->vjp_bar, VjpNodes());
+  }
+  // the interface ghost cells' adjoints go back to the cells they copied
+  if (!g->links.empty()) {
+    if (!g->d_vjp_links) {
+      std::vector<LinkDev> dl(g->links.size());
+      if (hipMemcpy(dl.data(), g->d_links, sizeof(LinkDev) * dl.size(), hipMemcpyDeviceToHost) != hipSuccess)
+        return gfail(g, FEST_E_HIP, "link table download failed");
+      for (LinkDev &L : dl)
+        for (fest_handle *h : g->blocks) {
+          if (L.src == h->b.qp) L.src = h->vjp_bar;
+          if (L.dst == h->b.qp) L.dst = h->vjp_bar;
+        }
+      if (hipMalloc(&g->d_vjp_links, sizeof(LinkDev) * dl.size()) != hipSuccess)
+        return gfail(g, FEST_E_HIP, "hipMalloc of the adjoint link table failed");
+      if (hipMemcpy(g->d_vjp_links, dl.data(), sizeof(LinkDev) * dl.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return gfail(g, FEST_E_HIP, "adjoint link table upload failed");
+    }
+    for (int e = (int)g->links.size() - 1; e >= 0; --e)
+      hipLaunchKernelGGL(k_exchange_adjoint, dim3(cdiv(3L * g->max_layer, 256)), dim3(256), 0, g->stream,
+                         (const LinkDev *)g->d_vjp_links, e);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return gfail(g, FEST_E_HIP, "adjoint exchange launch: %s", hipGetErrorString(e));
+  }
+  for (int n = 0; n < nb; ++n) {
+    fest_handle *h = g->blocks[n];
+    if (hipMemcpyAsync(bar_qps[n], h->vjp_bar, sizeof(double) * 5 * h->b.nc, hipMemcpyDeviceToHost, g->stream) !=
+        hipSuccess)
+      return gfail(g, FEST_E_HIP, "block %d: adjoint download failed", n);
+  }
+  for (int n = 0; n < nb; ++n)
+    if ((rc = check_err(g->blocks[n]))) return gfail(g, rc, "block %d: %s", n, g->blocks[n]->err.c_str());
   return 0;
 }
 

@@ -80,4 +80,34 @@ __global__ void __launch_bounds__(256) k_exchange(const LinkDev *links) {
   for (int n = 0; n < L.nv; ++n) L.dst[(long)n * L.dst_nc + cd] = L.src[(long)n * L.src_nc + cs];
 }
 
+// the transpose of one link's copy on the adjoint arrays (links[li] built
+// like the exchange's, with src and dst at the sender's and the receiver's
+// cell adjoints): each receiving ghost cell's adjoint is added to the
+// sender's interior cell it was copied from, then cleared.  One link per
+// launch: two links may read the same sender cell.
+__global__ void __launch_bounds__(256) k_exchange_adjoint(const LinkDev *links, int li) {
+  const LinkDev L = links[li];
+  const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (e >= 3 * L.per_layer) return;
+  const int l = e / L.per_layer + 1;
+  const int t = e - (l - 1) * L.per_layer;
+  const int as = t % L.s_n1 + 1, bs = t / L.s_n1 + 1;
+  const int inner = t % L.r_inner, outer = t / L.r_inner;
+  int ar, br;
+  if (L.dir_switch == 0) {
+    ar = L.lo1 + inner * L.d1;
+    br = L.lo2 + outer * L.d2;
+  } else {
+    ar = L.lo1 + outer * L.d1;
+    br = L.lo2 + inner * L.d2;
+  }
+  const long cs = layer_cell(L, l, as, bs), cd = ghost_cell(L, l, ar, br);
+  double *src = const_cast<double *>(L.src);
+  for (int n = 0; n < L.nv; ++n) {
+    const double v = L.dst[(long)n * L.dst_nc + cd];
+    L.dst[(long)n * L.dst_nc + cd] = 0.0;
+    src[(long)n * L.src_nc + cs] += v;
+  }
+}
+
 }  // namespace fest

@@ -91,6 +91,7 @@ def lib():
         L.fest_group_last_error.argtypes = [vp]
         L.fest_group_last_error.restype = ctypes.c_char_p
         L.fest_group_exchange.argtypes = [vp]
+        L.fest_group_vjp.argtypes = [vp, ctypes.POINTER(dp), ctypes.POINTER(dp)]
         L.fest_group_wall_dist.argtypes = [vp]
         L.fest_group_resnorm.argtypes = [vp, dp]
         L.fest_group_get_next_solution.argtypes = [vp, ip, ctypes.c_double]
@@ -104,7 +105,7 @@ def lib():
                   "fest_set_grid", "fest_upload_state",
                   "fest_get_next_solution", "fest_residual", "fest_vjp", "fest_time_step", "fest_download", "fest_resnorm",
                   "fest_device_ptr", "fest_sync", "fest_timing", "fest_kernel_time", "fest_group_create",
-                  "fest_group_exchange", "fest_group_wall_dist", "fest_group_resnorm",
+                  "fest_group_exchange", "fest_group_vjp", "fest_group_wall_dist", "fest_group_resnorm",
                   "fest_group_get_next_solution"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
@@ -326,6 +327,19 @@ class GpuGroup:
     def _check(self, rc):
         if rc != 0:
             raise FestError(rc, lib().fest_group_last_error(self.g).decode())
+
+    def vjp(self, bar_residues):
+        """fest_group_vjp: the blocks' cell adjoints (each (5, padded
+        cells)) for the per-block residue adjoints bar_residues, the
+        interface ghost shares added to the cells they came from."""
+        ws = [np.ascontiguousarray(w, dtype=np.float64) for w in bar_residues]
+        assert len(ws) == len(self.blocks)
+        outs = [np.zeros((5,) + blk.case.cell_shape()) for blk in self.blocks]
+        dp = ctypes.POINTER(ctypes.c_double)
+        win = (dp * len(ws))(*[_ptr(w) for w in ws])
+        wout = (dp * len(outs))(*[_ptr(o) for o in outs])
+        self._check(lib().fest_group_vjp(self.g, win, wout))
+        return outs
 
     def wall_dist(self):
         """Wall distance of every block to the walls of all blocks."""

@@ -85,7 +85,7 @@ module fest_gpu
   public :: fest_create, fest_destroy, fest_last_error, fest_upload_geometry, fest_set_grid, &
             fest_upload_state, fest_get_next_solution, fest_residual, fest_vjp, fest_time_step, &
             fest_download, fest_resnorm, fest_sync, fest_check, &
-            fest_group_create, fest_group_destroy, fest_group_exchange, fest_group_wall_dist, &
+            fest_group_create, fest_group_destroy, fest_group_exchange, fest_group_vjp, fest_group_wall_dist, &
             fest_group_resnorm, fest_group_get_next_solution
 
   interface
@@ -203,6 +203,15 @@ module fest_gpu
       import :: c_int, c_ptr
       type(c_ptr), value :: g
     end function fest_group_exchange
+
+    ! fest_vjp over the group's blocks (no counterpart in the reference):
+    ! arrays of c_loc pointers, one residue adjoint and one cell adjoint per block
+    integer(c_int) function fest_group_vjp(g, bar_residues, bar_qps) bind(c, name='fest_group_vjp')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: g
+      type(c_ptr), intent(in) :: bar_residues(*)
+      type(c_ptr), intent(in) :: bar_qps(*)
+    end function fest_group_vjp
 
     ! wall distance of every block to the no-slip walls of all blocks
     integer(c_int) function fest_group_wall_dist(g) bind(c, name='fest_group_wall_dist')

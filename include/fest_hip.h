@@ -297,6 +297,16 @@ int fest_group_exchange(fest_group *g);
  * (src/wall/wall.f90:55-111, src/wall/wall_dist.f90:84-131).  Call after
  * fest_set_grid on every block; replaces their single-block distances. */
 int fest_group_wall_dist(fest_group *g);
+
+/* fest_vjp over a group: bar_qps[n] = the adjoint of block n's cells for
+ * the residual of every block, bar_residues[n] having block n's residue
+ * layout.  The interfaces are exchanged first (as the group's residual
+ * opens), each block's adjoint is taken as by fest_vjp, and the adjoints
+ * the interface ghost cells hold are added to the cells they were copied
+ * from (the transpose of apply_interface, src/interface1.f90:96-705), so
+ * every ghost cell ends at zero.  No counterpart in the reference.  The
+ * conditions of fest_vjp hold for every block; no node adjoint. */
+int fest_group_vjp(fest_group *g, const double *const *bar_residues, double *const *bar_qps);
 /* Absolute residual norms Res_abs(0:n_var) of the whole group, assembled
  * from every block's last residue as find_resnorm does across processes
  * (src/resnorm.f90:62-91,171-225): the blocks' signed mass imbalances and

@@ -490,3 +490,62 @@ def test_adjoint_of_split_blocks_composes(recon):
     d = np.abs(total[inner] - whole[inner]).max() / np.abs(whole[inner]).max()
     say("split blocks", recon, d)
     assert d <= 1e-13
+
+
+@pytest.mark.parametrize("axis,bcs", [
+    (0, (-3, -4, -6, -6, -9, -9)),   # cut along i: two interface links
+    (2, (-3, -4, -6, -6, -9, -9)),   # cut along k of a periodic k: interfaces and a cross-block periodic pair
+])
+def test_group_adjoint_equals_the_uncut_block(axis, bcs):
+    """fest_group_vjp on a block cut in two: the interfaces exchanged, each
+    half's adjoint taken, the interface ghost adjoints sent back to the
+    cells they copied; every ghost cell ends at zero and the halves'
+    interiors equal the uncut block's adjoint.  The uncut block is a group
+    too (split with no cut): a periodic direction then becomes the
+    cross-block periodic pair (-10) on the block itself, whose faces take
+    the full stencil where the single-block periodic rule (-9) takes the
+    boundary face states, so the two residuals differ."""
+    from festgpu.multiblock import split
+    from festgpu.native import GpuGroup
+    case, nodes, q = C.euler_bump(12, 8, 8, recon="weno", bc_id=bcs, seed=43)
+    w = weights(case, 43)
+    (uc, un, uq), = split(case, nodes, q, axis, [])[0]
+    g = GpuBlock(uc)
+    g.set_grid(un)
+    g.upload_state(uq)
+    g1 = GpuGroup([g], split(case, nodes, q, axis, [])[1])
+    whole = g1.vjp([w])[0]
+    g1.close()
+    cut = (case.imx + 1) // 2 if axis == 0 else (case.kmx + 1) // 2
+    blocks, links = split(case, nodes, q, axis, [cut])
+    gs = []
+    for cb, nb, qb in blocks:
+        gb = GpuBlock(cb)
+        gb.set_grid(nb)
+        gb.upload_state(qb)
+        gs.append(gb)
+    grp = GpuGroup(gs, links)
+    ax = 3 - axis  # array axis of the cut direction in (k, j, i) + the variable axis
+    ws, s = [], 1
+    for cb, _, _ in blocks:
+        n = (cb.imx, cb.jmx, cb.kmx)[axis]
+        sl = [slice(None)] * 4
+        sl[ax] = slice(s - 1, s - 1 + n - 1)
+        ws.append(np.ascontiguousarray(w[tuple(sl)]))
+        s += n - 1
+    bars = grp.vjp(ws)
+    worst, s = 0.0, 1
+    scale = np.abs(whole).max()
+    for (cb, _, _), bar in zip(blocks, bars):
+        n = (cb.imx, cb.jmx, cb.kmx)[axis]
+        ghost = np.ones(cb.cell_shape(), bool)
+        ghost[3:-3, 3:-3, 3:-3] = False
+        assert np.all(bar[:, ghost] == 0.0)
+        inner = (slice(None),) + (slice(3, -3),) * 3
+        sl = [slice(None), slice(3, -3), slice(3, -3), slice(3, -3)]
+        sl[ax] = slice(s + 2, s + 2 + n - 1)
+        worst = max(worst, np.abs(bar[inner] - whole[tuple(sl)]).max() / scale)
+        s += n - 1
+    grp.close()
+    say("group adjoint", axis, worst)
+    assert worst <= 1e-13
