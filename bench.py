@@ -340,6 +340,40 @@ def adjoint_run(case, reps=3):
             "residual_ms": res_ms, "vjp_ms": vjp_ms, "vjp_kernels_ms": faces_ms, "ratio": vjp_ms / res_ms}
 
 
+def unsteady_adjoint_run(size, nsteps, every):
+    """The unsteady adjoint of `nsteps` RK4 steps (global time step) of the
+    inviscid bump with checkpoints every `every` steps (festgpu.adjoint),
+    for the pressure x-force on the j-min wall: wall time of the forward
+    trajectory against the gradient (forward with checkpoints, then the
+    reverse sweep recomputing each interval).  The stage arithmetic runs on
+    the host, the residuals and their transposes on the device."""
+    import dataclasses
+    import numpy as np
+    from festgpu import cases as C
+    from festgpu.adjoint import INNER, RK4Adjoint, wall_force_x
+    from festgpu.native import GpuBlock
+    ni, nj, nk = size
+    case, nodes, q = C.euler_bump(ni, nj, nk, flux="van_leer", recon="weno", bc_id=(-3, -4, -6, -8, -9, -9))
+    dt = 0.2 * 0.02 / 1.5  # CFL 0.2 on the first wall spacing
+    case = dataclasses.replace(case, time_local=0, global_time_step=dt)
+    g = GpuBlock(case)
+    g.set_grid(nodes)
+    adj = RK4Adjoint(g, dt)
+    J, dJ = wall_force_x(g, 2)
+    q0 = np.ascontiguousarray(q[INNER])
+    t0 = time.perf_counter()
+    adj.forward(q0, nsteps, nsteps)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, grad = adj.gradient(q0, nsteps, every, dJ)
+    t_grad = time.perf_counter() - t0
+    g.close()
+    return {"workload": f"{case.name} inviscid, weno+van_leer", "steps": nsteps, "checkpoint_every": every,
+            "forward_s": t_fwd, "gradient_s": t_grad, "ratio": t_grad / t_fwd,
+            "checkpoints": -(-nsteps // every), "stage_states_kept": 4 * every,
+            "grad_norm": float(np.sqrt(np.sum(grad * grad)))}
+
+
 def chain_run(args):
     """The batched multiblock workload: 64 seeded uneven blocks (about 11.5M
     cells) joined in a chain (festgpu.cases.chain_blocks), MUSCL + AUSM-UP,
@@ -413,6 +447,10 @@ def main():
                     help="launch kernel by kernel instead of replaying captured step graphs")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the cost measurement of fest_vjp on the inviscid version of the workload")
+    ap.add_argument("--unsteady-adjoint", type=int, default=0, metavar="STEPS",
+                    help="also time the unsteady adjoint of STEPS RK4 steps (checkpoints every STEPS/10) "
+                         "on the inviscid bump of --unsteady-size (0: skip)")
+    ap.add_argument("--unsteady-size", default="64x32x16")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -502,6 +540,10 @@ def main():
             out["fp32_flux"]["long_run"] = fp32_long_error(case, nodes, q, args.fp32_error_stages)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_run(case)
+    if args.unsteady_adjoint > 0:
+        n = args.unsteady_adjoint
+        size = tuple(int(v) for v in args.unsteady_size.split("x"))
+        out["unsteady_adjoint"] = unsteady_adjoint_run(size, n, max(1, n // 10))
     if not args.no_cpu_baseline:
         dist = g.download("dist") if case.sst_family else None
         out["cpu_baseline"] = cpu_baseline(case_fn, ni, nj, nk, dist, implicit=implicit)
