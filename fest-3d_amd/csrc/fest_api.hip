@@ -1029,37 +1029,48 @@ int vjp_buffers(fest_handle *h) {
   return dalloc(h, &h->vjp_bar, 5 * b.nc);
 }
 
+// why the node adjoint is not available on top of the state adjoint
+extern "C++" std::string vjp_nodes_reason(const Blk &b) {
+  // the pole rewrites its faces' metrics, WENO-NM reads cell volumes
+  for (int f = 0; f < 6; ++f)
+    if (b.bc[f] == -7) return "the node-coordinate adjoint does not cover the pole (-7)";
+  if (b.recon == FEST_RECON_WENO_NM) return "the node-coordinate adjoint does not cover WENO-NM";
+  return "";
+}
+
+// the node adjoint's buffers (allocated at the first use) in `nodes`
+int vjp_node_buffers(fest_handle *h, VjpNodes &nodes) {
+  const Blk &b = h->b;
+  const long nn = (long)(b.imx + 6) * (b.jmx + 6) * (b.kmx + 6);
+  int rc;
+  if (!h->vjp_nodes) {
+    for (int d = 0; d < 3; ++d)
+      if ((rc = dalloc(h, &h->vjp_m[d], 4 * b.nfl[d]))) return rc;
+    if ((rc = dalloc(h, &h->vjp_g3, 6L * (b.imx - 1) * (b.kmx - 1)))) return rc;
+    if ((rc = dalloc(h, &h->vjp_g5, 6L * (b.imx - 1) * (b.jmx - 1)))) return rc;
+    if ((rc = dalloc(h, &h->vjp_nodes, 3 * nn))) return rc;
+  }
+  for (int d = 0; d < 3; ++d) nodes.mbar[d] = h->vjp_m[d];
+  // slip walls on the j and k max faces reflect with ghost-row I faces
+  nodes.g3 = b.bc[3] == -6 ? h->vjp_g3 : nullptr;
+  nodes.g5 = b.bc[5] == -6 && b.kmx != 2 ? h->vjp_g5 : nullptr;
+  nodes.bar_nodes = h->vjp_nodes;
+  return 0;
+}
+
 int fest_vjp(fest_handle *h, const double *bar_residue, double *bar_qp, double *bar_nodes) {
   if (!h || !bar_residue || !bar_qp) return FEST_E_ARG;
   if (!h->have_grid || !h->have_state) return fail(h, FEST_E_ARG, "grid and state must be set first");
   Blk &b = h->b;
   std::string why = vjp_reason(b);
-  if (why.empty() && bar_nodes) {
-    // the pole rewrites its faces' metrics, WENO-NM reads cell volumes
-    for (int f = 0; f < 6; ++f)
-      if (b.bc[f] == -7) why = "the node-coordinate adjoint does not cover the pole (-7)";
-    if (b.recon == FEST_RECON_WENO_NM) why = "the node-coordinate adjoint does not cover WENO-NM";
-  }
+  if (why.empty() && bar_nodes) why = vjp_nodes_reason(b);
   if (!why.empty()) return fail(h, FEST_E_CONFIG, "fest_vjp: %s", why.c_str());
   hipStream_t st = h->stream;
   int rc;
   if ((rc = vjp_buffers(h))) return rc;
-  const bool slip3 = b.bc[3] == -6, slip5 = b.bc[5] == -6 && b.kmx != 2;
   const long nn = (long)(b.imx + 6) * (b.jmx + 6) * (b.kmx + 6);
   VjpNodes nodes;
-  if (bar_nodes) {
-    if (!h->vjp_nodes) {
-      for (int d = 0; d < 3; ++d)
-        if ((rc = dalloc(h, &h->vjp_m[d], 4 * b.nfl[d]))) return rc;
-      if ((rc = dalloc(h, &h->vjp_g3, 6L * (b.imx - 1) * (b.kmx - 1)))) return rc;
-      if ((rc = dalloc(h, &h->vjp_g5, 6L * (b.imx - 1) * (b.jmx - 1)))) return rc;
-      if ((rc = dalloc(h, &h->vjp_nodes, 3 * nn))) return rc;
-    }
-    for (int d = 0; d < 3; ++d) nodes.mbar[d] = h->vjp_m[d];
-    nodes.g3 = slip3 ? h->vjp_g3 : nullptr;
-    nodes.g5 = slip5 ? h->vjp_g5 : nullptr;
-    nodes.bar_nodes = h->vjp_nodes;
-  }
+  if (bar_nodes && (rc = vjp_node_buffers(h, nodes))) return rc;
   HIPCHK(h, hipMemcpyAsync(h->vjp_barR, bar_residue, sizeof(double) * 5 * b.nres, hipMemcpyHostToDevice, st));
   // the ghost cells as fest_residual at its first iteration (fixed faces refixed)
   if ((rc = launch_ghosts(h, 1))) return rc;
@@ -1696,7 +1707,8 @@ int fest_group_exchange(fest_group *g) {
   return 0;
 }
 
-int fest_group_vjp(fest_group *g, const double *const *bar_residues, double *const *bar_qps) {
+int fest_group_vjp(fest_group *g, const double *const *bar_residues, double *const *bar_qps,
+                   double *const *bar_nodes) {
   if (!g || !bar_residues || !bar_qps) return FEST_E_ARG;
   if (g->broken) return FEST_E_ARG;
   const int nb = (int)g->blocks.size();
@@ -1704,7 +1716,8 @@ int fest_group_vjp(fest_group *g, const double *const *bar_residues, double *con
     fest_handle *h = g->blocks[n];
     if (!bar_residues[n] || !bar_qps[n]) return gfail(g, FEST_E_ARG, "block %d: null adjoint array", n);
     if (!h->have_grid || !h->have_state) return gfail(g, FEST_E_ARG, "block %d: grid and state must be set first", n);
-    const std::string why = vjp_reason(h->b);
+    std::string why = vjp_reason(h->b);
+    if (why.empty() && bar_nodes && bar_nodes[n]) why = vjp_nodes_reason(h->b);
     if (!why.empty()) return gfail(g, FEST_E_CONFIG, "block %d: fest_vjp: %s", n, why.c_str());
   }
   int rc;
@@ -1716,9 +1729,12 @@ int fest_group_vjp(fest_group *g, const double *const *bar_residues, double *con
     if (hipMemcpyAsync(h->vjp_barR, bar_residues[n], sizeof(double) * 5 * h->b.nres, hipMemcpyHostToDevice,
                        g->stream) != hipSuccess)
       return gfail(g, FEST_E_HIP, "block %d: adjoint upload failed", n);
+    VjpNodes nodes;
+    if (bar_nodes && bar_nodes[n] && (rc = vjp_node_buffers(h, nodes)))
+      return gfail(g, rc, "block %d: %s", n, h->err.c_str());
     if ((rc = launch_ghosts(h, 1))) return gfail(g, rc, "block %d: %s", n, h->err.c_str());
     KgScope ts(h, KG_VJP);
-    launch_vjp(h->b, g->stream, h->vjp_barR, h->vjp_c, h->vjp_bar, VjpNodes());
+    launch_vjp(h->b, g->stream, h->vjp_barR, h->vjp_c, h->vjp_bar, nodes);
   }
   // the interface ghost cells' adjoints go back to the cells they copied
   if (!g->links.empty()) {
@@ -1747,6 +1763,11 @@ int fest_group_vjp(fest_group *g, const double *const *bar_residues, double *con
     if (hipMemcpyAsync(bar_qps[n], h->vjp_bar, sizeof(double) * 5 * h->b.nc, hipMemcpyDeviceToHost, g->stream) !=
         hipSuccess)
       return gfail(g, FEST_E_HIP, "block %d: adjoint download failed", n);
+    const long nn = (long)(h->b.imx + 6) * (h->b.jmx + 6) * (h->b.kmx + 6);
+    if (bar_nodes && bar_nodes[n] &&
+        hipMemcpyAsync(bar_nodes[n], h->vjp_nodes, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, g->stream) !=
+            hipSuccess)
+      return gfail(g, FEST_E_HIP, "block %d: node adjoint download failed", n);
   }
   for (int n = 0; n < nb; ++n)
     if ((rc = check_err(g->blocks[n]))) return gfail(g, rc, "block %d: %s", n, g->blocks[n]->err.c_str());

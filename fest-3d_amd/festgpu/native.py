@@ -91,7 +91,7 @@ def lib():
         L.fest_group_last_error.argtypes = [vp]
         L.fest_group_last_error.restype = ctypes.c_char_p
         L.fest_group_exchange.argtypes = [vp]
-        L.fest_group_vjp.argtypes = [vp, ctypes.POINTER(dp), ctypes.POINTER(dp)]
+        L.fest_group_vjp.argtypes = [vp, ctypes.POINTER(dp), ctypes.POINTER(dp), ctypes.POINTER(dp)]
         L.fest_group_wall_dist.argtypes = [vp]
         L.fest_group_resnorm.argtypes = [vp, dp]
         L.fest_group_get_next_solution.argtypes = [vp, ip, ctypes.c_double]
@@ -328,18 +328,21 @@ class GpuGroup:
         if rc != 0:
             raise FestError(rc, lib().fest_group_last_error(self.g).decode())
 
-    def vjp(self, bar_residues):
+    def vjp(self, bar_residues, nodes: bool = False):
         """fest_group_vjp: the blocks' cell adjoints (each (5, padded
         cells)) for the per-block residue adjoints bar_residues, the
-        interface ghost shares added to the cells they came from."""
+        interface ghost shares added to the cells they came from; with
+        nodes, also each block's node adjoint: returns (cells, nodes)."""
         ws = [np.ascontiguousarray(w, dtype=np.float64) for w in bar_residues]
         assert len(ws) == len(self.blocks)
         outs = [np.zeros((5,) + blk.case.cell_shape()) for blk in self.blocks]
         dp = ctypes.POINTER(ctypes.c_double)
         win = (dp * len(ws))(*[_ptr(w) for w in ws])
         wout = (dp * len(outs))(*[_ptr(o) for o in outs])
-        self._check(lib().fest_group_vjp(self.g, win, wout))
-        return outs
+        bns = [np.zeros(blk.case.node_shape()) for blk in self.blocks] if nodes else None
+        wn = (dp * len(bns))(*[_ptr(o) for o in bns]) if nodes else None
+        self._check(lib().fest_group_vjp(self.g, win, wout, wn))
+        return (outs, bns) if nodes else outs
 
     def wall_dist(self):
         """Wall distance of every block to the walls of all blocks."""

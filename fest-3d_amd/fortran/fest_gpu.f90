@@ -205,12 +205,14 @@ module fest_gpu
     end function fest_group_exchange
 
     ! fest_vjp over the group's blocks (no counterpart in the reference):
-    ! arrays of c_loc pointers, one residue adjoint and one cell adjoint per block
-    integer(c_int) function fest_group_vjp(g, bar_residues, bar_qps) bind(c, name='fest_group_vjp')
+    ! arrays of c_loc pointers, one residue adjoint and one cell adjoint per
+    ! block, and optionally one node adjoint per block
+    integer(c_int) function fest_group_vjp(g, bar_residues, bar_qps, bar_nodes) bind(c, name='fest_group_vjp')
       import :: c_int, c_ptr
       type(c_ptr), value :: g
       type(c_ptr), intent(in) :: bar_residues(*)
       type(c_ptr), intent(in) :: bar_qps(*)
+      type(c_ptr), value :: bar_nodes  ! c_null_ptr, or c_loc of an array of per-block pointers
     end function fest_group_vjp
 
     ! wall distance of every block to the no-slip walls of all blocks

@@ -305,8 +305,12 @@ int fest_group_wall_dist(fest_group *g);
  * the interface ghost cells hold are added to the cells they were copied
  * from (the transpose of apply_interface, src/interface1.f90:96-705), so
  * every ghost cell ends at zero.  No counterpart in the reference.  The
- * conditions of fest_vjp hold for every block; no node adjoint. */
-int fest_group_vjp(fest_group *g, const double *const *bar_residues, double *const *bar_qps);
+ * conditions of fest_vjp hold for every block.  bar_nodes (nullable, or
+ * per block nullable): each block's node adjoint as fest_vjp's -- a
+ * block's residual reads only its own nodes, so nothing crosses the
+ * interfaces; a node two blocks share gets a share in each. */
+int fest_group_vjp(fest_group *g, const double *const *bar_residues, double *const *bar_qps,
+                   double *const *bar_nodes);
 /* Absolute residual norms Res_abs(0:n_var) of the whole group, assembled
  * from every block's last residue as find_resnorm does across processes
  * (src/resnorm.f90:62-91,171-225): the blocks' signed mass imbalances and

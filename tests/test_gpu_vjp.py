@@ -500,7 +500,8 @@ def test_group_adjoint_equals_the_uncut_block(axis, bcs):
     """fest_group_vjp on a block cut in two: the interfaces exchanged, each
     half's adjoint taken, the interface ghost adjoints sent back to the
     cells they copied; every ghost cell ends at zero and the halves'
-    interiors equal the uncut block's adjoint.  The uncut block is a group
+    interiors equal the uncut block's adjoint; the halves' node adjoints,
+    summed on the nodes they share, equal the uncut block's.  The uncut block is a group
     too (split with no cut): a periodic direction then becomes the
     cross-block periodic pair (-10) on the block itself, whose faces take
     the full stencil where the single-block periodic rule (-9) takes the
@@ -514,7 +515,7 @@ def test_group_adjoint_equals_the_uncut_block(axis, bcs):
     g.set_grid(un)
     g.upload_state(uq)
     g1 = GpuGroup([g], split(case, nodes, q, axis, [])[1])
-    whole = g1.vjp([w])[0]
+    (whole,), (whole_nodes,) = g1.vjp([w], nodes=True)
     g1.close()
     cut = (case.imx + 1) // 2 if axis == 0 else (case.kmx + 1) // 2
     blocks, links = split(case, nodes, q, axis, [cut])
@@ -533,7 +534,21 @@ def test_group_adjoint_equals_the_uncut_block(axis, bcs):
         sl[ax] = slice(s - 1, s - 1 + n - 1)
         ws.append(np.ascontiguousarray(w[tuple(sl)]))
         s += n - 1
-    bars = grp.vjp(ws)
+    bars, bns = grp.vjp(ws, nodes=True)
+    # node adjoints: each block's real nodes at their global node index,
+    # a node on the cut shared by the two blocks
+    total_nodes = np.zeros_like(whole_nodes)
+    s = 1
+    for (cb, _, _), bn in zip(blocks, bns):
+        n = (cb.imx, cb.jmx, cb.kmx)[axis]
+        src = [slice(None), slice(3, cb.kmx + 3), slice(3, cb.jmx + 3), slice(3, cb.imx + 3)]
+        dst = list(src)
+        dst[ax] = slice(s + 2, s + 2 + n)
+        total_nodes[tuple(dst)] += bn[tuple(src)]
+        s += n - 1
+    dn = np.abs(total_nodes - whole_nodes).max() / np.abs(whole_nodes).max()
+    say("group node adjoint", axis, dn)
+    assert dn <= 1e-13
     worst, s = 0.0, 1
     scale = np.abs(whole).max()
     for (cb, _, _), bar in zip(blocks, bars):
