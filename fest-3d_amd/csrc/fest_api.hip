@@ -1006,8 +1006,6 @@ int fest_residual(fest_handle *h, int current_iter) {
 extern "C++" std::string vjp_reason(const Blk &b) {
   if (b.mu_ref != 0.0) return "only the inviscid residual (mu_ref 0) is differentiated";
   if (b.nv != 5) return "only the five flow variables are differentiated (no turbulence model)";
-  for (int d = 0; d < 3; ++d)
-    if (b.pbs[d]) return "the pressure-based switch is not differentiated";
   // the ghost rules read interior layers 1..4 only with 4 interior cells
   if (b.imx < 5 || b.jmx < 5 || (b.kmx != 2 && b.kmx < 5))
     return "every 3-D direction needs at least 4 cells";

@@ -82,7 +82,8 @@ __global__ void __launch_bounds__(256) k_vjp_flux(Blk b, const double *__restric
   if (any) {
     double q[10];  // qL, then qR
     typename FaceMath<double>::Aux ax;
-    ax.pb = false;
+    ax.pb = (RECON == 1 || RECON == 2) && b.pbs[D];
+    if (ax.pb) FaceMath<double>::pb_factors(b, f.n, f.N, b.qp + 4 * b.nc + f.c, f.s, ax);
     if constexpr (RECON == 4) FaceMath<double>::nm_blends(b.vol + f.c, f.s, ax);
 #pragma unroll
     for (int l = 0; l < 5; ++l)
@@ -130,31 +131,68 @@ __global__ void __launch_bounds__(256) k_vjp_flux(Blk b, const double *__restric
 
 // one thread per face of direction D and variable l (l-major):
 // contrib[m * 5 + l][face] = bar qL_l dqL_l/du_m + bar qR_l dqR_l/du_m, the
-// face's contribution to the adjoint of variable l of stencil cell n-3+m
+// face's contribution to the adjoint of variable l of stencil cell n-3+m.
+// Under the pressure-based switch (MUSCL, PPM) the face states of every
+// variable also read the pressures of the stencil: the tangents 6..11 sit
+// on them, and a variable l < 4 leaves its pressure contributions in
+// pbc[m * 4 + l][face] (the pressure's own thread adds them to its slot).
 template <int D, int RECON>
-__global__ void __launch_bounds__(256) k_vjp_recon(Blk b, const double *__restrict__ fbar, double *contrib) {
+__global__ void __launch_bounds__(256) k_vjp_recon(Blk b, const double *__restrict__ fbar, double *contrib,
+                                                   double *pbc) {
   using D6 = Dual<VJP_STENCIL>;
+  using D12 = Dual<2 * VJP_STENCIL>;
   const long nf = b.nfl[D];
   const long tt = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (tt >= 5 * nf) return;
   const int l = (int)(tt / nf);
   const long t = tt - (long)l * nf;
   const double bl = fbar[(long)l * nf + t], br = fbar[(long)(5 + l) * nf + t];
-  double o[VJP_STENCIL] = {0, 0, 0, 0, 0, 0};
+  const bool pb = (RECON == 1 || RECON == 2) && b.pbs[D];
+  double o[VJP_STENCIL] = {0, 0, 0, 0, 0, 0}, op[VJP_STENCIL] = {0, 0, 0, 0, 0, 0};
   if (bl != 0.0 || br != 0.0) {
     const VjpFace f = vjp_face<D>(b, t);
-    typename FaceMath<D6>::Aux ax;
-    ax.pb = false;
-    if constexpr (RECON == 4) FaceMath<D6>::nm_blends(b.vol + f.c, f.s, ax);
-    D6 u[VJP_STENCIL], L, R;
+    if constexpr (RECON == 1 || RECON == 2) {
+      if (pb) {
+        typename FaceMath<D12>::Aux ax;
+        ax.pb = true;
+        D12 u[VJP_STENCIL], pu[VJP_STENCIL], L, R;
 #pragma unroll
-    for (int m = 0; m < VJP_STENCIL; ++m) {
-      u[m] = D6(b.qp[(long)l * b.nc + f.c + (long)(m - 3) * f.s]);
-      u[m].d[m] = 1.0;
+        for (int m = 0; m < VJP_STENCIL; ++m) {
+          u[m] = D12(b.qp[(long)l * b.nc + f.c + (long)(m - 3) * f.s]);
+          u[m].d[m] = 1.0;
+          pu[m] = D12(b.qp[4L * b.nc + f.c + (long)(m - 3) * f.s]);
+          pu[m].d[VJP_STENCIL + m] = 1.0;
+        }
+        FaceMath<D12>::pb_factors(b, f.n, f.N, &pu[3], 1, ax);
+        FaceMath<D12>::template face_states<RECON>(b, D, f.n, f.N, l, &u[3], 1, ax, L, R);
+#pragma unroll
+        for (int m = 0; m < VJP_STENCIL; ++m) {
+          o[m] = bl * L.d[m] + br * R.d[m];
+          op[m] = bl * L.d[VJP_STENCIL + m] + br * R.d[VJP_STENCIL + m];
+        }
+      }
     }
-    FaceMath<D6>::template face_states<RECON>(b, D, f.n, f.N, l, &u[3], 1, ax, L, R);
+    if (!pb) {
+      typename FaceMath<D6>::Aux ax;
+      ax.pb = false;
+      if constexpr (RECON == 4) FaceMath<D6>::nm_blends(b.vol + f.c, f.s, ax);
+      D6 u[VJP_STENCIL], L, R;
 #pragma unroll
-    for (int m = 0; m < VJP_STENCIL; ++m) o[m] = bl * L.d[m] + br * R.d[m];
+      for (int m = 0; m < VJP_STENCIL; ++m) {
+        u[m] = D6(b.qp[(long)l * b.nc + f.c + (long)(m - 3) * f.s]);
+        u[m].d[m] = 1.0;
+      }
+      FaceMath<D6>::template face_states<RECON>(b, D, f.n, f.N, l, &u[3], 1, ax, L, R);
+#pragma unroll
+      for (int m = 0; m < VJP_STENCIL; ++m) o[m] = bl * L.d[m] + br * R.d[m];
+    }
+  }
+  if (l == 4) {
+#pragma unroll
+    for (int m = 0; m < VJP_STENCIL; ++m) o[m] += op[m];
+  } else if (pb) {
+#pragma unroll
+    for (int m = 0; m < VJP_STENCIL; ++m) pbc[(long)(m * 4 + l) * nf + t] = op[m];
   }
 #pragma unroll
   for (int m = 0; m < VJP_STENCIL; ++m) contrib[(long)(m * 5 + l) * nf + t] = o[m];
@@ -162,7 +200,8 @@ __global__ void __launch_bounds__(256) k_vjp_recon(Blk b, const double *__restri
 
 // one thread per padded cell: the sum over directions and over the faces
 // whose stencils hold the cell (face n = c + 3 - m of stencil position m);
-// c0, c1, c2: the contributions of the three directions (k_vjp_recon)
+// c0, c1, c2: the contributions of the three directions (k_vjp_recon), with
+// the pressure-based switch's share after the face-state adjoints
 __global__ void __launch_bounds__(256) k_vjp_gather(Blk b, const double *__restrict__ c0,
                                                      const double *__restrict__ c1, const double *__restrict__ c2,
                                                      double *bar) {
@@ -186,6 +225,12 @@ __global__ void __launch_bounds__(256) k_vjp_gather(Blk b, const double *__restr
       const long f = d == 0 ? fli(b, 0, n, j, k) : d == 1 ? fli(b, 1, i, n, k) : fli(b, 2, i, j, n);
 #pragma unroll
       for (int v = 0; v < 5; ++v) acc[v] += cs[d][(long)(m * 5 + v) * nf[d] + f];
+      // the pressure-based switch's share of the pressure, from variables 0..3
+      if ((b.recon == 1 || b.recon == 2) && b.pbs[d]) {
+        const double *pbc = cs[d] + (long)(VJP_PER_FACE + 10) * nf[d];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[4] += pbc[(long)(m * 4 + v) * nf[d] + f];
+      }
     }
   }
 #pragma unroll

@@ -15,7 +15,8 @@ void vjp_dir_t(const Blk &b, hipStream_t st, const double *barR, double *c, doub
   const long nf = b.nfl[D];
   double *fbar = c + (long)VJP_PER_FACE * nf;  // bar qL, bar qR after the contributions
   hipLaunchKernelGGL((k_vjp_flux<D, RECON>), dim3(blocks(nf)), dim3(256), 0, st, b, barR, fbar, mbar);
-  hipLaunchKernelGGL((k_vjp_recon<D, RECON>), dim3(blocks(5 * nf)), dim3(256), 0, st, b, fbar, c);
+  double *pbc = fbar + 10 * nf;  // the pressure-based switch's pressure contributions, when it is on
+  hipLaunchKernelGGL((k_vjp_recon<D, RECON>), dim3(blocks(5 * nf)), dim3(256), 0, st, b, fbar, c, pbc);
 }
 template <int D>
 void vjp_dir(const Blk &b, hipStream_t st, const double *barR, double *c, double *mbar) {
@@ -30,7 +31,10 @@ void vjp_dir(const Blk &b, hipStream_t st, const double *barR, double *c, double
 
 }  // namespace
 
-long vjp_contrib_len(const Blk &b, int d) { return (long)b.nfl[d] * (VJP_PER_FACE + 10); }
+long vjp_contrib_len(const Blk &b, int d) {
+  const bool pb = (b.recon == 1 || b.recon == 2) && b.pbs[d];
+  return (long)b.nfl[d] * (VJP_PER_FACE + 10 + (pb ? 4 * VJP_STENCIL : 0));
+}
 
 void launch_vjp(const Blk &b, hipStream_t st, const double *barR, double *const contrib[3], double *bar,
                 const VjpNodes &nodes) {

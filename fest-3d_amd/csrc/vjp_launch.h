@@ -8,7 +8,8 @@
 namespace fest {
 
 // doubles of the face buffer of sweep direction d: the 30 contributions of
-// each face, then its 10 face-state adjoints
+// each face, its 10 face-state adjoints, and under the pressure-based
+// switch its 24 pressure contributions from the other variables
 long vjp_contrib_len(const Blk &b, int d);
 
 // the node-coordinate part of the adjoint (all null when it is not wanted):

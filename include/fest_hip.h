@@ -166,8 +166,8 @@ int fest_residual(fest_handle *h, int current_iter);
  * the adjoint, the ghost cells of interface faces (ids >= 0 and -10) the
  * share owed to the neighbouring block, every other ghost cell zero.  The
  * ghost cells are refilled first, as fest_residual does at current_iter 1.
- * Inviscid handles with five variables, no pressure-based switch and at
- * least 4 cells in every 3-D direction, else FEST_E_CONFIG; on a
+ * Inviscid handles with five variables and at least 4 cells in every 3-D
+ * direction, else FEST_E_CONFIG; on a
  * FEST_PREC_FP32_FLUX handle it is the adjoint of the FP64 residual (the
  * face arithmetic in FP64) at the resident FP64 state.  The result
  * is the exact derivative of the branches (limiters, switches) the
