@@ -96,6 +96,8 @@ struct fest_handle {
   // and, for the node-coordinate adjoint: the metric adjoints, the slip
   // walls' ghost-row faces on the j and k max faces, the node adjoint
   double *vjp_m[3] = {}, *vjp_g3 = nullptr, *vjp_g5 = nullptr, *vjp_nodes = nullptr;
+  // and, for a viscous block: the temperature and gradient adjoints
+  double *vjp_bt = nullptr, *vjp_gb = nullptr;
 };
 
 // a group's view of its blocks; a block destroyed while in a group detaches
@@ -300,16 +302,11 @@ int launch_ghosts(fest_handle *h, int current_iter) {
   return 0;
 }
 
-// C2..C9(+C10 in k_update): ghosts, gradients/viscosity, face fluxes
-int launch_residual_front(fest_handle *h, int current_iter, int store_csnd) {
+// C4..C6: gradients, viscosities (and turbulence sources) and the ghost
+// cells' gradients of a viscous block; nothing for an inviscid one
+int launch_grads(fest_handle *h) {
   Blk &b = h->b;
   hipStream_t st = h->stream;
-  if (int rc = launch_ghosts(h, current_iter)) return rc;
-  if (b.fp32) {
-    KgScope ts(h, KG_BC_EDGES);
-    hipLaunchKernelGGL(k_to_f32, dim3(cdiv(b.nc * b.nv, 256)), dim3(256), 0, st, b);
-  }
-  LAUNCHCHK(h);
   dim3 g, t;
   if (b.mu_ref != 0.0) {
     box(b.imx + 1, b.jmx + 1, b.kmx + 1, g, t, b.xcd_remap);
@@ -357,6 +354,21 @@ int launch_residual_front(fest_handle *h, int current_iter, int store_csnd) {
     }
     LAUNCHCHK(h);
   }
+  return 0;
+}
+
+// C2..C9(+C10 in k_update): ghosts, gradients/viscosity, face fluxes
+int launch_residual_front(fest_handle *h, int current_iter, int store_csnd) {
+  Blk &b = h->b;
+  hipStream_t st = h->stream;
+  if (int rc = launch_ghosts(h, current_iter)) return rc;
+  if (b.fp32) {
+    KgScope ts(h, KG_BC_EDGES);
+    hipLaunchKernelGGL(k_to_f32, dim3(cdiv(b.nc * b.nv, 256)), dim3(256), 0, st, b);
+  }
+  LAUNCHCHK(h);
+  if (int rc = launch_grads(h)) return rc;
+  dim3 g, t;
   // the i sweep: blocks of FLUX_I_SPAN consecutive padded-plane offsets
   g = dim3(plane_slots(cdiv(flux_i_span(b), FLUX_I_SPAN), b.xcd_remap), (unsigned)(b.kmx - 1), 1);
   t = dim3(FLUX_I_SPAN, 1, 1);
@@ -1004,8 +1016,13 @@ int fest_residual(fest_handle *h, int current_iter) {
 
 // why fest_vjp cannot differentiate the block's residual (empty: it can)
 extern "C++" std::string vjp_reason(const Blk &b) {
-  if (b.mu_ref != 0.0) return "only the inviscid residual (mu_ref 0) is differentiated";
   if (b.nv != 5) return "only the five flow variables are differentiated (no turbulence model)";
+  if (b.mu_ref != 0.0) {
+    if (b.kmx == 2) return "the viscous terms are differentiated on 3-D grids only";
+    if (b.fp32) return "the viscous terms are differentiated on FP64 handles only";
+    for (int f = 0; f < 6; ++f)
+      if (b.bc[f] >= 0 || b.bc[f] == -10) return "the viscous terms are differentiated without interface faces";
+  }
   // the ghost rules read interior layers 1..4 only with 4 interior cells
   if (b.imx < 5 || b.jmx < 5 || (b.kmx != 2 && b.kmx < 5))
     return "every 3-D direction needs at least 4 cells";
@@ -1023,6 +1040,10 @@ int vjp_buffers(fest_handle *h) {
   if ((rc = dalloc(h, &h->vjp_barR, 5 * b.nres))) return rc;
   for (int d = 0; d < 3; ++d)
     if ((rc = dalloc(h, &h->vjp_c[d], vjp_contrib_len(b, d)))) return rc;
+  if (b.mu_ref != 0.0) {
+    if ((rc = dalloc(h, &h->vjp_bt, b.nc))) return rc;
+    if ((rc = dalloc(h, &h->vjp_gb, 12 * b.ngr))) return rc;
+  }
   return dalloc(h, &h->vjp_bar, 5 * b.nc);
 }
 
@@ -1032,6 +1053,7 @@ extern "C++" std::string vjp_nodes_reason(const Blk &b) {
   for (int f = 0; f < 6; ++f)
     if (b.bc[f] == -7) return "the node-coordinate adjoint does not cover the pole (-7)";
   if (b.recon == FEST_RECON_WENO_NM) return "the node-coordinate adjoint does not cover WENO-NM";
+  if (b.mu_ref != 0.0) return "the node-coordinate adjoint covers the inviscid residual";
   return "";
 }
 
@@ -1071,9 +1093,16 @@ int fest_vjp(fest_handle *h, const double *bar_residue, double *bar_qp, double *
   HIPCHK(h, hipMemcpyAsync(h->vjp_barR, bar_residue, sizeof(double) * 5 * b.nres, hipMemcpyHostToDevice, st));
   // the ghost cells as fest_residual at its first iteration (fixed faces refixed)
   if ((rc = launch_ghosts(h, 1))) return rc;
+  // the viscous faces' primal inputs: gradients and viscosities
+  if ((rc = launch_grads(h))) return rc;
   {
     KgScope ts(h, KG_VJP);
-    launch_vjp(b, st, h->vjp_barR, h->vjp_c, h->vjp_bar, nodes);
+    VjpVisc visc;
+    if (b.mu_ref != 0.0) {
+      visc.bt = h->vjp_bt;
+      visc.gb = h->vjp_gb;
+    }
+    launch_vjp(b, st, h->vjp_barR, h->vjp_c, h->vjp_bar, nodes, visc);
   }
   LAUNCHCHK(h);
   HIPCHK(h, hipMemcpyAsync(bar_qp, h->vjp_bar, sizeof(double) * 5 * b.nc, hipMemcpyDeviceToHost, st));
@@ -1714,6 +1743,7 @@ int fest_group_vjp(fest_group *g, const double *const *bar_residues, double *con
     if (!bar_residues[n] || !bar_qps[n]) return gfail(g, FEST_E_ARG, "block %d: null adjoint array", n);
     if (!h->have_grid || !h->have_state) return gfail(g, FEST_E_ARG, "block %d: grid and state must be set first", n);
     std::string why = vjp_reason(h->b);
+    if (why.empty() && h->b.mu_ref != 0.0) why = "the group adjoint covers inviscid blocks";
     if (why.empty() && bar_nodes && bar_nodes[n]) why = vjp_nodes_reason(h->b);
     if (!why.empty()) return gfail(g, FEST_E_CONFIG, "block %d: fest_vjp: %s", n, why.c_str());
   }
@@ -1731,7 +1761,7 @@ int fest_group_vjp(fest_group *g, const double *const *bar_residues, double *con
       return gfail(g, rc, "block %d: %s", n, h->err.c_str());
     if ((rc = launch_ghosts(h, 1))) return gfail(g, rc, "block %d: %s", n, h->err.c_str());
     KgScope ts(h, KG_VJP);
-    launch_vjp(h->b, g->stream, h->vjp_barR, h->vjp_c, h->vjp_bar, nodes);
+    launch_vjp(h->b, g->stream, h->vjp_barR, h->vjp_c, h->vjp_bar, nodes, VjpVisc());
   }
   // the interface ghost cells' adjoints go back to the cells they copied
   if (!g->links.empty()) {

@@ -570,4 +570,290 @@ __global__ void __launch_bounds__(256) k_vjp_nodes(Blk b, const double *__restri
   for (int c = 0; c < 3; ++c) bar_nodes[(long)c * nn + t] = acc[c];
 }
 
+// ---- the laminar viscous terms (mu_ref != 0, five variables, 3-D)
+//
+// The viscous flux of a face (FaceMath<T>::visc_face) is a function of 22
+// values: u, v, w, T of the cell below (cm) and above (c), the two cells'
+// viscosities, and the average of their gradients of u, v, w, T.  T is
+// p/(R rho) of the state; the gradients are the Green-Gauss sums of
+// k_grad_b (of u, v, w and the temperature plane) and, in the ghost cells
+// of boundary faces, the rule of k_grad_bc_b; the viscosities come from
+// Sutherland's law of T when the case uses it (src/viscosity.f90), else they
+// are constants.  The temperature plane is the state's p/(R rho) at upload
+// (src/update.f90:170, taken before the ghost refill): its interior cells
+// are functions of the state, its ghost cells constants of the call.
+constexpr int VJP_VISC_IN = 22;
+
+// visc_face's arithmetic on the 22 inputs x = (u, v, w, T of cm, u, v, w, T
+// of c, mu of cm, mu of c, the averaged gradients [var][x, y, z]); dl the
+// centre offset c - cm, id2 = 1/|dl|^2.  v[0..3]: the momentum and energy
+// components of the viscous flux.
+template <typename T>
+__device__ __forceinline__ void vjp_visc_core(const Blk &b, const T (&x)[VJP_VISC_IN], const double (&dl)[3],
+                                              double id2, double nx, double ny, double nz, double A, T (&v)[4]) {
+  T gr[4][3];
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    const T del = x[4 + l] - x[l];
+    const T ncmp = (del - (x[10 + 3 * l] * T(dl[0]) + x[11 + 3 * l] * T(dl[1]) + x[12 + 3 * l] * T(dl[2]))) * T(id2);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) gr[l][c] = x[10 + 3 * l + c] + ncmp * T(dl[c]);
+  }
+  const T tm = T(0.5) * (x[8] + x[9]);
+  const T div3 = (gr[0][0] + gr[1][1] + gr[2][2]) * T(1.0 / 3.0);
+  const T Txx = T(2.0) * tm * (gr[0][0] - div3), Tyy = T(2.0) * tm * (gr[1][1] - div3);
+  const T Tzz = T(2.0) * tm * (gr[2][2] - div3);
+  const T Txy = tm * (gr[1][0] + gr[0][1]), Txz = tm * (gr[2][0] + gr[0][2]), Tyz = tm * (gr[2][1] + gr[1][2]);
+  const T K = tm * T(b.iPr * b.cpR);
+  const T uf = T(0.5) * (x[0] + x[4]), vf = T(0.5) * (x[1] + x[5]), wf = T(0.5) * (x[2] + x[6]);
+  const T Nx = T(nx), Ny = T(ny), Nz = T(nz), AA = T(A);
+  v[0] = (Txx * Nx + Txy * Ny + Txz * Nz) * AA;
+  v[1] = (Txy * Nx + Tyy * Ny + Tyz * Nz) * AA;
+  v[2] = (Txz * Nx + Tyz * Ny + Tzz * Nz) * AA;
+  v[3] = AA * ((Txx * uf + Txy * vf + Txz * wf + K * gr[3][0]) * Nx + (Txy * uf + Tyy * vf + Tyz * wf + K * gr[3][1]) * Ny +
+               (Txz * uf + Tyz * vf + Tzz * wf + K * gr[3][2]) * Nz);
+}
+
+// one thread per face of direction D: vf[e][face] = d<barF, -v>/dx_e, the
+// adjoint of the face's 22 inputs (the residue takes the flux minus v)
+template <int D>
+__global__ void __launch_bounds__(256) k_vjp_visc(Blk b, const double *__restrict__ barR, double *vf) {
+  constexpr int KT = 6;
+  using DK = Dual<KT>;
+  const long nf = b.nfl[D];
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nf) return;
+  const VjpFace f = vjp_face<D>(b, t);
+  const int ii = D == 0, jj = D == 1, kk = D == 2;
+  const bool below = f.n >= 2, above = f.n <= f.N - 1;
+  const long rb = below ? ri(b, f.i - ii, f.j - jj, f.k - kk) : 0, ra = above ? ri(b, f.i, f.j, f.k) : 0;
+  double bv[4];
+  bool any = false;
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    const double bF = (below ? barR[(long)(l + 1) * b.nres + rb] : 0.0) -
+                      (above ? barR[(long)(l + 1) * b.nres + ra] : 0.0);
+    bv[l] = -bF;
+    any |= bF != 0.0;
+  }
+  double out[VJP_VISC_IN];
+#pragma unroll
+  for (int e = 0; e < VJP_VISC_IN; ++e) out[e] = 0.0;
+  if (any) {
+    const long c = f.c, cm = f.c - f.s, nc = b.nc;
+    const long gL = gri(b, f.i - ii, f.j - jj, f.k - kk), gR = gri(b, f.i, f.j, f.k);
+    const long fo = fi(b, D, f.i, f.j, f.k);
+    const double nx = b.fx[D][fo], ny = b.fy[D][fo], nz = b.fz[D][fo], A = b.fA[D][fo];
+    const double dl[3] = {b.cx[c] - b.cx[cm], b.cy[c] - b.cy[cm], b.cz[c] - b.cz[cm]};
+    const double id2 = 1.0 / (dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2]);
+    double x0[VJP_VISC_IN];
+    const long cc[2] = {cm, c};
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      x0[4 * s + 0] = b.qp[nc + cc[s]];
+      x0[4 * s + 1] = b.qp[2 * nc + cc[s]];
+      x0[4 * s + 2] = b.qp[3 * nc + cc[s]];
+      x0[4 * s + 3] = b.qp[4 * nc + cc[s]] / (b.qp[cc[s]] * b.R);
+      x0[8 + s] = b.mu[cc[s]];
+    }
+#pragma unroll
+    for (int l = 0; l < 4; ++l)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) x0[10 + 3 * l + q] = 0.5 * (b.g[q][l * b.ngr + gL] + b.g[q][l * b.ngr + gR]);
+#pragma unroll
+    for (int p0 = 0; p0 < VJP_VISC_IN; p0 += KT) {
+      DK x[VJP_VISC_IN], v[4];
+#pragma unroll
+      for (int e = 0; e < VJP_VISC_IN; ++e) {
+        x[e] = DK(x0[e]);
+        if (e >= p0 && e < p0 + KT) x[e].d[e - p0] = 1.0;
+      }
+      vjp_visc_core<DK>(b, x, dl, id2, nx, ny, nz, A, v);
+#pragma unroll
+      for (int e = 0; e < KT; ++e)
+        if (p0 + e < VJP_VISC_IN) out[p0 + e] = bv[0] * v[0].d[e] + bv[1] * v[1].d[e] + bv[2] * v[2].d[e] + bv[3] * v[3].d[e];
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < VJP_VISC_IN; ++e) vf[(long)e * nf + t] = out[e];
+}
+
+// one thread per padded cell: the sum of the viscous faces' adjoints of the
+// cell (as c of the face below it, as cm of the face above it, in each
+// direction, in a fixed order).  Adds bar u, v, w to bar; writes bt, the
+// adjoint of T = p/(R rho) of the state (with Sutherland's law folded in),
+// and gb[(3 l + comp) ngr + g], the adjoint of the cell's gradient (half
+// of each face's averaged-gradient adjoint), for the cells holding one
+// (0..imx, 0..jmx, 0..kmx); zero elsewhere.
+__global__ void __launch_bounds__(256) k_vjp_visc_gather(Blk b, const double *__restrict__ v0,
+                                                         const double *__restrict__ v1,
+                                                         const double *__restrict__ v2, double *bar, double *bt,
+                                                         double *gb) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= b.nc) return;
+  const int i = (int)(t % b.pi) - 2, j = (int)((t / b.pi) % b.pj) - 2, k = (int)(t / ((long)b.pi * b.pj)) - 2;
+  if (i < 0 || i > b.imx || j < 0 || j > b.jmx || k < 0 || k > b.kmx) {
+    bt[t] = 0.0;
+    return;
+  }
+  const double *vs[3] = {v0, v1, v2};
+  double acc[17] = {};  // u, v, w, T, mu, then the 12 gradient adjoints (twice the cell's share)
+  const int p[3] = {i, j, k};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int N = d == 0 ? b.imx : d == 1 ? b.jmx : b.kmx;
+    const int o1 = d == 0 ? 1 : 0, o2 = d == 2 ? 1 : 2;  // the two other directions
+    const int a = p[o1], bb = p[o2];
+    const int Na = o1 == 0 ? b.imx : b.jmx, Nb = o2 == 1 ? b.jmx : b.kmx;
+    if (a < 1 || a > Na - 1 || bb < 1 || bb > Nb - 1) continue;
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {  // 0: the face below (the cell is its c), 1: above (its cm)
+      const int n = p[d] + side;
+      if (n < 1 || n > N) continue;
+      int q[3] = {i, j, k};
+      q[d] = n;
+      const long nf = b.nfl[d];
+      const long fo = (long)(q[0] - 1) + (long)b.nfi[d] * ((long)(q[1] - 1) + (long)b.nfj[d] * (q[2] - 1));
+      const double *v = vs[d];
+      const int base = side == 0 ? 4 : 0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) acc[e] += v[(long)(base + e) * nf + fo];
+      acc[4] += v[(long)(side == 0 ? 9 : 8) * nf + fo];
+#pragma unroll
+      for (int e = 0; e < 12; ++e) acc[5 + e] += v[(long)(10 + e) * nf + fo];
+    }
+  }
+  bar[b.nc + t] += acc[0];
+  bar[2 * b.nc + t] += acc[1];
+  bar[3 * b.nc + t] += acc[2];
+  double bT = acc[3];
+  if (b.mu_suth && acc[4] != 0.0) {
+    // mu = mu_ref (T/T_ref)^1.5 (T_ref + S)/(T + S): dmu/dT = mu (1.5/T - 1/(T + S))
+    const double T = b.qp[4 * b.nc + t] / (b.qp[t] * b.R);
+    const double x = T / b.T_ref;
+    const double mu = b.mu_ref * (x * sqrt(x)) * ((b.T_ref + b.S_ref) / (T + b.S_ref));
+    bT += acc[4] * mu * (1.5 / T - 1.0 / (T + b.S_ref));
+  }
+  bt[t] = bT;
+  const long g = gri(b, i, j, k);
+#pragma unroll
+  for (int e = 0; e < 12; ++e) gb[(long)e * b.ngr + g] = 0.5 * acc[5 + e];
+}
+
+// the transpose of k_grad_bc_b on one face (boundary ids < 0), one thread
+// per face-interior column: the ghost cell's gradient adjoint goes to the
+// interior cell's gradient and to the differences of the two cells' values
+// it reads, then is cleared (the Green-Gauss sum the rule overwrites is
+// unused).  u, v, w of either cell into bar, T of the interior cell into bt
+// (the ghost cell's temperature is a constant of the call)
+__global__ void __launch_bounds__(256) k_vjp_grad_bc(Blk b, int face, double *bar, double *bt, double *gb) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x + 1, bb = blockIdx.y + 1;
+  const int na = face_na(b, face), nb = face_nb(b, face);
+  if (a > na - 1 || bb > nb - 1) return;
+  const int id = b.bc[face];
+  if (id >= 0) return;
+  const int d = face >> 1, hi = face & 1;
+  const double Tw = b.fix[FX_TWALL][face];
+  const int N = d == 0 ? b.imx : d == 1 ? b.jmx : b.kmx;
+  const int fp = hi ? N : 1;
+  int fi_, fj_, fk_;
+  if (d == 0) { fi_ = fp; fj_ = a; fk_ = bb; } else if (d == 1) { fi_ = a; fj_ = fp; fk_ = bb; } else { fi_ = a; fj_ = bb; fk_ = fp; }
+  int Ii = fi_, Ij = fj_, Ik = fk_, Gi = fi_, Gj = fj_, Gk = fk_;
+  if (hi) { if (d == 0) Ii--; else if (d == 1) Ij--; else Ik--; }
+  else { if (d == 0) Gi--; else if (d == 1) Gj--; else Gk--; }
+  const long fo = fi(b, 0, fi_, fj_, fk_);  // the rule reads every direction's metrics at the I-face offset
+  const double nx = b.fx[d][fo], ny = b.fy[d][fo], nz = b.fz[d][fo], A = b.fA[d][fo];
+  const long cI = ci(b, Ii, Ij, Ik), cG = ci(b, Gi, Gj, Gk);
+  const double vI = b.vol[cI];
+  const double cx = A * nx / vI, cy = A * ny / vI, cz = A * nz / vI;
+  const double sig = hi ? -1.0 : 1.0;
+  const long gI = gri(b, Ii, Ij, Ik), gG = gri(b, Gi, Gj, Gk);
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    double *gx = gb + (long)(3 * l) * b.ngr, *gy = gx + b.ngr, *gz = gy + b.ngr;
+    const double rx = gx[gG], ry = gy[gG], rz = gz[gG];
+    gx[gG] = 0.0;
+    gy[gG] = 0.0;
+    gz[gG] = 0.0;
+    const double dot = rx * nx + ry * ny + rz * nz;
+    double ix = rx - dot * nx, iy = ry - dot * ny, iz = rz - dot * nz;
+    if (l == 3 && id == -5 && (Tw < 1. && Tw >= 0.)) {
+      ix -= rx;
+      iy -= ry;
+      iz -= rz;
+    } else {
+      const double bdq = sig * (cx * rx + cy * ry + cz * rz);
+      if (l < 3) {
+        bar[(long)(l + 1) * b.nc + cI] += bdq;
+        bar[(long)(l + 1) * b.nc + cG] -= bdq;
+      } else {
+        bt[cI] += bdq;
+      }
+    }
+    gx[gI] += ix;
+    gy[gI] += iy;
+    gz[gI] += iz;
+  }
+}
+
+// one thread per padded cell x: the transpose of the Green-Gauss gradients
+// (k_grad_b) onto u, v, w and the temperature of x, from the gradient
+// adjoints of x and its six neighbours; then the temperature adjoint of x
+// to its density and pressure (T = p/(R rho)).  The temperature plane's
+// ghost cells are constants of the call: their share is dropped.
+__global__ void __launch_bounds__(256) k_vjp_grad(Blk b, const double *__restrict__ gb, const double *__restrict__ bt,
+                                                  double *bar) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= b.nc) return;
+  const int i = (int)(t % b.pi) - 2, j = (int)((t / b.pi) % b.pj) - 2, k = (int)(t / ((long)b.pi * b.pj)) - 2;
+  double acc[4] = {0, 0, 0, 0};
+  const int p[3] = {i, j, k};
+  // (the cell c holding a gradient, sign, the face of c between c and x);
+  // c = x: all six faces of x
+#pragma unroll
+  for (int m = 0; m < 7; ++m) {
+    int q[3] = {i, j, k};
+    const int d = m == 0 ? 0 : (m - 1) >> 1;
+    const int up = m == 0 ? 0 : ((m - 1) & 1) ? 1 : -1;  // c = x + up e_d
+    q[d] += up;
+    if (q[0] < 0 || q[0] > b.imx || q[1] < 0 || q[1] > b.jmx || q[2] < 0 || q[2] > b.kmx) continue;
+    const long c = ci(b, q[0], q[1], q[2]);
+    const double ivol2 = 1.0 / (2 * b.vol[c]);
+    double co[3] = {0, 0, 0};
+    if (m == 0) {
+#pragma unroll
+      for (int e = 0; e < 3; ++e) {
+        const int e0 = e == 0, e1 = e == 1, e2 = e == 2;
+        const long lo = fi(b, e, q[0], q[1], q[2]), hi = fi(b, e, q[0] + e0, q[1] + e1, q[2] + e2);
+        co[0] += b.fx[e][hi] * b.fA[e][hi] - b.fx[e][lo] * b.fA[e][lo];
+        co[1] += b.fy[e][hi] * b.fA[e][hi] - b.fy[e][lo] * b.fA[e][lo];
+        co[2] += b.fz[e][hi] * b.fA[e][hi] - b.fz[e][lo] * b.fA[e][lo];
+      }
+    } else {
+      // x = c - e_d: x is c's lower neighbour, through c's lower face (up = +1);
+      // x = c + e_d: through c's upper face (up = -1)
+      const long f = up > 0 ? fi(b, d, q[0], q[1], q[2]) : fi(b, d, p[0], p[1], p[2]);
+      const double sg = up > 0 ? -1.0 : 1.0;
+      co[0] = sg * b.fx[d][f] * b.fA[d][f];
+      co[1] = sg * b.fy[d][f] * b.fA[d][f];
+      co[2] = sg * b.fz[d][f] * b.fA[d][f];
+    }
+    const long g = gri(b, q[0], q[1], q[2]);
+#pragma unroll
+    for (int l = 0; l < 4; ++l)
+      acc[l] += ivol2 * (co[0] * gb[(long)(3 * l) * b.ngr + g] + co[1] * gb[(long)(3 * l + 1) * b.ngr + g] +
+                         co[2] * gb[(long)(3 * l + 2) * b.ngr + g]);
+  }
+  bar[b.nc + t] += acc[0];
+  bar[2 * b.nc + t] += acc[1];
+  bar[3 * b.nc + t] += acc[2];
+  const bool inner = i >= 1 && i <= b.imx - 1 && j >= 1 && j <= b.jmx - 1 && k >= 1 && k <= b.kmx - 1;
+  const double bT = bt[t] + (inner ? acc[3] : 0.0);
+  if (bT != 0.0) {
+    const double rho = b.qp[t], T = b.qp[4 * b.nc + t] / (rho * b.R);
+    bar[t] -= bT * T / rho;
+    bar[4 * b.nc + t] += bT / (rho * b.R);
+  }
+}
+
 }  // namespace fest

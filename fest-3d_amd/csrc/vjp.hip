@@ -37,11 +37,24 @@ long vjp_contrib_len(const Blk &b, int d) {
 }
 
 void launch_vjp(const Blk &b, hipStream_t st, const double *barR, double *const contrib[3], double *bar,
-                const VjpNodes &nodes) {
+                const VjpNodes &nodes, const VjpVisc &visc) {
   vjp_dir<0>(b, st, barR, contrib[0], nodes.mbar[0]);
   vjp_dir<1>(b, st, barR, contrib[1], nodes.mbar[1]);
   vjp_dir<2>(b, st, barR, contrib[2], nodes.mbar[2]);
   hipLaunchKernelGGL(k_vjp_gather, dim3(blocks(b.nc)), dim3(256), 0, st, b, contrib[0], contrib[1], contrib[2], bar);
+  if (visc.bt) {
+    hipLaunchKernelGGL(k_vjp_visc<0>, dim3(blocks(b.nfl[0])), dim3(256), 0, st, b, barR, contrib[0]);
+    hipLaunchKernelGGL(k_vjp_visc<1>, dim3(blocks(b.nfl[1])), dim3(256), 0, st, b, barR, contrib[1]);
+    hipLaunchKernelGGL(k_vjp_visc<2>, dim3(blocks(b.nfl[2])), dim3(256), 0, st, b, barR, contrib[2]);
+    hipLaunchKernelGGL(k_vjp_visc_gather, dim3(blocks(b.nc)), dim3(256), 0, st, b, contrib[0], contrib[1], contrib[2],
+                       bar, visc.bt, visc.gb);
+    for (int face = 5; face >= 0; --face) {
+      const int na = face_na(b, face), nb = face_nb(b, face);
+      hipLaunchKernelGGL(k_vjp_grad_bc, dim3(blocks(na - 1), (unsigned)(nb - 1)), dim3(256), 0, st, b, face, bar,
+                         visc.bt, visc.gb);
+    }
+    hipLaunchKernelGGL(k_vjp_grad, dim3(blocks(b.nc)), dim3(256), 0, st, b, visc.gb, visc.bt, bar);
+  }
   // the k faces of a 2-D grid (one cell in k) feed no flux
   for (int face = b.kmx == 2 ? 3 : 5; face >= 0; --face) {
     const int na = face_na(b, face), nb = face_nb(b, face);

@@ -23,13 +23,22 @@ struct VjpNodes {
   double *bar_nodes = nullptr;
 };
 
+// the laminar viscous part (null when the block is inviscid): bt, the
+// temperature adjoint (one plane of the padded cell array), gb, the gradient
+// adjoints (12 planes of the gradient array, ngr each)
+struct VjpVisc {
+  double *bt = nullptr, *gb = nullptr;
+};
+
 // bar (5 planes of the padded cell array) = d(barR . R)/dq on the resident
 // state, whose ghost cells the block's own boundary rules filled: the face
 // Jacobians contracted with barR (two launches per direction), the per-cell
 // gather, then the transposes of the ghost rules, face by face in reverse
-// order; with nodes.bar_nodes, also d(barR . R)/d(node coordinates).
+// order; with nodes.bar_nodes, also d(barR . R)/d(node coordinates).  With
+// visc.bt, the viscous faces' adjoints are added before the ghost rules'
+// transposes (the viscous faces reuse contrib after the gather: 22 planes).
 // contrib[d] holds vjp_contrib_len(b, d) doubles.
 void launch_vjp(const Blk &b, hipStream_t st, const double *barR, double *const contrib[3], double *bar,
-                const VjpNodes &nodes);
+                const VjpNodes &nodes, const VjpVisc &visc);
 
 }  // namespace fest

@@ -29,6 +29,10 @@ class RK4Adjoint:
     """Forward RK4 steps and their adjoint on one GpuBlock (grid set)."""
 
     def __init__(self, block, dt: float):
+        if block.case.mu_ref != 0.0:
+            # the stage states are uploaded with zero ghost cells, whose
+            # temperatures the viscous gradients would read
+            raise ValueError("RK4Adjoint drives inviscid blocks (mu_ref 0)")
         self.g = block
         self.case = block.case
         self.gm = self.case.gm

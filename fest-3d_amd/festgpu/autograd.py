@@ -25,6 +25,10 @@ def _function():
     class InviscidResidual(torch.autograd.Function):
         @staticmethod
         def forward(ctx, q_interior, block):
+            if block.case.mu_ref != 0.0:
+                # the ghost cells are uploaded as zeros: the viscous gradients
+                # would read their temperatures
+                raise ValueError("inviscid_residual takes inviscid blocks (mu_ref 0)")
             q = np.zeros((5,) + block.case.cell_shape())
             q[INNER] = q_interior.detach().cpu().numpy()
             block.upload_state(q)

@@ -166,8 +166,12 @@ int fest_residual(fest_handle *h, int current_iter);
  * the adjoint, the ghost cells of interface faces (ids >= 0 and -10) the
  * share owed to the neighbouring block, every other ghost cell zero.  The
  * ghost cells are refilled first, as fest_residual does at current_iter 1.
- * Inviscid handles with five variables and at least 4 cells in every 3-D
- * direction, else FEST_E_CONFIG; on a
+ * Handles with five variables and at least 4 cells in every 3-D direction,
+ * inviscid, or laminar (the viscous fluxes, the Green-Gauss gradients and
+ * their ghost-cell rule, Sutherland's law; src/viscous.f90,
+ * src/gradients.f90) on 3-D FP64 handles without interface faces, where
+ * the temperature plane's ghost cells (the state's p/(R rho) at upload,
+ * src/update.f90:170) are constants; else FEST_E_CONFIG; on a
  * FEST_PREC_FP32_FLUX handle it is the adjoint of the FP64 residual (the
  * face arithmetic in FP64) at the resident FP64 state.  The result
  * is the exact derivative of the branches (limiters, switches) the
@@ -176,7 +180,7 @@ int fest_residual(fest_handle *h, int current_iter);
  * of fest_set_grid's nodes (-2:imx+3,-2:jmx+3,-2:kmx+3,1:3), through the
  * face metrics (src/geometry.f90:236-314) and the ghost-node extrapolation
  * (src/grid.f90:137-236) to the real nodes; ghost nodes hold zero.  Not
- * for the pole (-7) or WENO-NM (FEST_E_CONFIG). */
+ * for the pole (-7), WENO-NM or viscous handles (FEST_E_CONFIG). */
 int fest_vjp(fest_handle *h, const double *bar_residue, double *bar_qp, double *bar_nodes);
 
 /* compute_time_step (src/time.f90:291-334) from the face states of the

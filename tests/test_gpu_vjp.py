@@ -26,6 +26,8 @@ computes, the ghost cells following from the boundary rules.  The checks:
   against central differences of the whole trajectory;
 - ghost cells hold no adjoint without interface faces, the adjoint is
   linear in w and deterministic;
+- the laminar viscous terms (viscous fluxes, Green-Gauss gradients and
+  their ghost-cell rule, Sutherland's law) by the same columns;
 - configurations it does not differentiate fail with FEST_E_CONFIG.
 
 Tolerance: 1e-6 relative (per column: of the largest |bar| of that
@@ -111,7 +113,7 @@ def check_columns(g, case, q, resid, seed, sample=None):
     idx = range(n) if sample is None else np.random.default_rng(seed + 7).choice(n, sample, replace=False)
     sc = scales(case)
     top = np.abs(inner).reshape(5, -1).max(axis=1)
-    worst = 0.0
+    worst, where = 0.0, None
     for f in idx:
         v, k, j, i = np.unravel_index(int(f), shape)
         d = H * sc[v]
@@ -119,7 +121,11 @@ def check_columns(g, case, q, resid, seed, sample=None):
         qp[v, 3 + k, 3 + j, 3 + i] += d
         qm[v, 3 + k, 3 + j, 3 + i] -= d
         fd = float(np.sum(w * (resid(qp) - resid(qm)))) / (2 * d)
-        worst = max(worst, abs(inner[v, k, j, i] - fd) / top[v])
+        err = abs(inner[v, k, j, i] - fd) / top[v]
+        if err > worst:
+            worst, where = err, (v, k, j, i, inner[v, k, j, i], fd)
+    if worst > TOL:
+        say("  worst column (v, k, j, i, adjoint, difference):", where)
     return bar, worst
 
 
@@ -173,6 +179,57 @@ def test_jacobian_columns_pressure_based_switch(recon, pbs):
     _, worst = check_columns(g, case, q, lambda x: gpu_residual(g, x), 17)
     say("  worst", worst)
     assert worst <= TOL
+
+
+VISC_MIXES = [
+    ((-3, -4, -5, -6, -9, -9), 1, "muscl"),  # adiabatic no-slip wall, slip wall, periodic k; Sutherland
+    ((-8, -8, -5, -8, -6, -6), 1, "weno"),   # far fields, slip walls in k
+    ((-11, -4, -5, -5, -9, -9), 0, "muscl"),  # total-pressure inlet, two walls; constant viscosity
+    ((-9, -9, -5, -6, -9, -9), 1, "none"),   # periodic i
+]
+
+
+@pytest.mark.parametrize("bcs,suth,recon", VISC_MIXES)
+def test_jacobian_columns_viscous(bcs, suth, recon):
+    """Every interior input of a 10 x 7 x 5 laminar grid: the viscous
+    fluxes, the Green-Gauss gradients and their ghost-cell rule, and the
+    viscosity's temperature dependence on top of the inviscid residual.
+    (The state of euler_bump's seed 41 puts a MUSCL limiter kink on one
+    pressure column: the inviscid adjoint shows the same 2e-3 there.)"""
+    say("vjp columns viscous", bcs, suth, recon)
+    case, nodes, q = C.euler_bump(10, 7, 5, bc_id=bcs, recon=recon)
+    case = dataclasses.replace(case, mu_ref=0.02, mu_sutherland=suth, S_ref=0.4)
+    g = GpuBlock(case)
+    g.set_grid(nodes)
+    bar, worst = check_columns(g, case, q, lambda x: gpu_residual(g, x), 29)
+    say("  worst", worst)
+    assert worst <= TOL
+    # the viscous terms are there: the inviscid adjoint differs
+    inv = GpuBlock(dataclasses.replace(case, mu_ref=0.0, mu_sutherland=0))
+    inv.set_grid(nodes)
+    inv.upload_state(q)
+    assert np.max(np.abs(inv.vjp(weights(case, 29)) - bar)) > 1e-3 * np.max(np.abs(bar))
+
+
+def test_viscous_adjoint_refusals():
+    """2-D viscous grids and the node adjoint of a viscous block are not
+    differentiated."""
+    case, nodes, q = C.euler_bump(12, 8, 1, bc_id=(-3, -4, -5, -8, -9, -9))
+    case = dataclasses.replace(case, mu_ref=0.02, mu_sutherland=1)
+    g = GpuBlock(case)
+    g.set_grid(nodes)
+    g.upload_state(q)
+    with pytest.raises(FestError) as e:
+        g.vjp(weights(case, 0))
+    assert e.value.code == -3
+    case, nodes, q = C.euler_bump(10, 7, 5, bc_id=(-3, -4, -5, -6, -9, -9))
+    case = dataclasses.replace(case, mu_ref=0.02, mu_sutherland=1)
+    g = GpuBlock(case)
+    g.set_grid(nodes)
+    g.upload_state(q)
+    with pytest.raises(FestError) as e:
+        g.vjp(weights(case, 0), nodes=True)
+    assert e.value.code == -3
 
 
 @pytest.mark.parametrize("limiter", ["koren", "van_albada"])
@@ -252,7 +309,7 @@ def test_gradcheck_through_torch():
 
 
 @pytest.mark.parametrize("change", [
-    dict(mu_ref=1e-3),                                   # viscous
+    dict(mu_ref=1e-3, bc_id=(-3, -4, 0, -8, -9, -9)),  # viscous with an interface face
     dict(bc_id=(-3, -4, -5, -8, -9, -9), fix_twall=(0, 0, 2.0, 0, 0, 0)),  # isothermal density rule
 ])
 def test_undifferentiated_configurations_fail(change):
